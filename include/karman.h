@@ -1,0 +1,148 @@
+/*
+ * karman.h -- C-ABI of libkarman_hip.so, the MI355X (gfx950) SIMPLE hot path.
+ *
+ * Plain C: pointers and sizes only, no torch types.  Every entry point
+ * replaces one reference interface (cited as file:line of the reference
+ * repository houkensjtu/karman_taichi).
+ *
+ * Layout contract
+ *   Fields are FP64 device arrays in the reference's padded shapes, i-major
+ *   with j contiguous (what ti.field.to_numpy() returns):
+ *     u, u0, u_mid : (nx+3) x (ny+2)      simple.py:27,30,33
+ *     v, v0, v_mid : (nx+2) x (ny+3)      simple.py:28,31,34
+ *     p, pcor, mdiv, mask : (nx+2) x (ny+2)   simple.py:36-40
+ *   A coefficient array is SoA: 5 planes of the field's shape, plane k is
+ *   0 aP, 1 aW, 2 aE, 3 aN, 4 aS (simple.py:42-47, comments :81-92).
+ *   The obstacle mask is uint8 (1 = solid cell), karman_taichi.py:49.
+ *
+ * Conventions
+ *   Return value: KM_OK (0) or a solver status (> 0) or an error (< 0).
+ *   A context is bound to one device and one HIP stream; calls enqueue work
+ *   on that stream and return without synchronising, except those that
+ *   return host values (km_dot, km_mdiv_stats, the solvers' km_solve_info).
+ *   The caller owns every field; the context owns its solver workspace,
+ *   captured graphs and tile choices.  A context is not thread-safe.
+ */
+#ifndef KARMAN_H
+#define KARMAN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status and error codes */
+#define KM_OK 0
+#define KM_BREAKDOWN 1   /* rho == 0 or <r~,v> == 0 or omega == 0 (bicgsolver.py:96-98, karman_taichi.py:451-454) */
+#define KM_MAXITER 2     /* iteration cap reached (cgsolver.py:14 caps at NX*NY) */
+#define KM_NONFINITE 3   /* NaN/Inf residual */
+#define KM_STALL 4       /* preconditioned solve stalled (the plain fallback then ran) */
+#define KM_EINVAL (-1)
+#define KM_EHIP (-2)
+#define KM_ENOMEM (-3)
+
+/* vector/coefficient precision of a Krylov solve */
+#define KM_F64 0
+#define KM_F32 1
+
+/* km_bicgstab / km_cg flags */
+#define KM_FLAG_X0 1          /* use x as the initial guess (default: x0 = 0, bicgsolver.py:21-35) */
+#define KM_FLAG_MIXED 2       /* FP32 matvec/axpy inner solves, FP64 scalars and FP64 true-residual refinement */
+#define KM_FLAG_NO_GRAPH 4    /* launch kernels directly instead of replaying captured graphs */
+
+/* east boundary kinds */
+#define KM_EAST_WALL 0        /* simple.py:184-186, :214-216 (with the bicgsimp.py:196 fix) */
+#define KM_EAST_OUTFLOW 1     /* zero gradient + outlet flux fix, karman_taichi.py:180-183, :586-614 */
+
+typedef struct km_ctx km_ctx;
+
+/* Grid, physics and boundary data of a SIMPLESolver (simple.py:11-39). */
+typedef struct km_grid {
+    int32_t nx, ny;
+    int32_t east;        /* KM_EAST_WALL or KM_EAST_OUTFLOW */
+    int32_t pin_pcor;    /* 1: p' = 0 at cell (1,1), bicgsimp.py:146-151 */
+    double dx, dy, rho, mu, dt;
+    double alpha_u, alpha_p, alpha_m;
+    double bc[4][2];     /* sides w, e, n, s; [normal, tangential] as simple.py:39 */
+} km_grid;
+
+typedef struct km_solve_info {
+    int32_t iterations;     /* Krylov iterations (inner iterations summed in mixed mode) */
+    int32_t status;         /* KM_OK, KM_BREAKDOWN, KM_MAXITER, KM_NONFINITE, KM_STALL */
+    int32_t refinements;    /* mixed mode: FP64 refinement sweeps */
+    int32_t fallback;       /* learned mode: 1 if the plain solver finished the solve */
+    double rel_resid;       /* recursive sqrt(r.r / r0.r0) (bicgsolver.py:130) */
+    double rel_resid_true;  /* ||b - A x|| / ||b|| recomputed in FP64 */
+    double bnorm;           /* ||b|| */
+} km_solve_info;
+
+/* ---- context (replaces ti.init, simple.py:7) ---- */
+int km_create(km_ctx** out, int device, void* hip_stream);
+int km_destroy(km_ctx* ctx);
+int km_set_stream(km_ctx* ctx, void* hip_stream);
+const char* km_last_error(const km_ctx* ctx);
+int km_synchronize(km_ctx* ctx);
+/* Tile shape (rows x columns of one workgroup) of the stencil kernels for an
+ * NX x NY system; ti = tj = 0 clears the override and re-enables autotuning. */
+int km_set_tile(km_ctx* ctx, int NX, int NY, int prec, int ti, int tj);
+int km_get_tile(km_ctx* ctx, int NX, int NY, int prec, int* ti, int* tj);
+
+/* ---- discretisation (SIMPLESolver kernels) ---- */
+/* compute_coef_u + the u half of set_bc (simple.py:77-92, :177-207) + obstacle rows */
+int km_assemble_u(km_ctx* ctx, const km_grid* g, const double* u, const double* v, const double* p,
+                  const double* u0, const uint8_t* mask, double* coef_u, double* b_u);
+/* compute_coef_v + the v half of set_bc (simple.py:95-110, :188-216) + obstacle rows */
+int km_assemble_v(km_ctx* ctx, const km_grid* g, const double* u, const double* v, const double* p,
+                  const double* v0, const uint8_t* mask, double* coef_v, double* b_v);
+/* ghost/boundary velocities written by set_bc (simple.py:181, :186, :192, :196) */
+int km_apply_ghosts(km_ctx* ctx, const km_grid* g, double* u, double* v);
+/* update_velocity: u = a_m u* + (1 - a_m) u (simple.py:232-238) */
+int km_blend(km_ctx* ctx, const km_grid* g, double* u, double* v, const double* u_mid, const double* v_mid);
+/* zero-gradient outlet + flux fix (karman_taichi.py:180-183, correct_conserv :586-614) */
+int km_outflow_fix(km_ctx* ctx, const km_grid* g, double* u);
+/* compute_coef_p (simple.py:143-170): writes coef_p, b_p and mdiv */
+int km_assemble_p(km_ctx* ctx, const km_grid* g, const double* u, const double* v, const double* coef_u,
+                  const double* coef_v, const uint8_t* mask, double* coef_p, double* b_p, double* mdiv);
+/* correct_pressure (parts bit 1) and/or correct_velocity (parts bit 2) in one
+ * launch (simple.py:218-230, karman_taichi.py:507-542) */
+int km_correct(km_ctx* ctx, const km_grid* g, double* p, double* u, double* v, const double* pcor,
+               const double* coef_u, const double* coef_v, const uint8_t* mask, int parts);
+/* compute_mdiv (simple.py:137-141); stats (may be NULL) receives {max|mdiv|, sum mdiv^2}
+ * over interior cells (bicgsimp.py:114-122 returns the max) and makes the call synchronous. */
+int km_mdiv(km_ctx* ctx, const km_grid* g, const double* u, const double* v, double* mdiv, double* stats);
+/* time_forward: u0 <- u, v0 <- v (karman_taichi.py:617-622) */
+int km_time_forward(km_ctx* ctx, const km_grid* g, const double* u, const double* v, double* u0, double* v0);
+
+/* ---- Krylov (CGSolver / BICGSolver) ---- */
+/* y = A x over the whole padded NX x NY field (bicgsolver.py:65-76, cgsolver.py:56-66) */
+int km_stencil_apply(km_ctx* ctx, int NX, int NY, int prec, const void* coef, const void* x, void* y);
+/* <a, b> in FP64 over n elements (CGSolver.reduce, cgsolver.py:49-54); synchronous */
+int km_dot(km_ctx* ctx, int64_t n, int prec, const void* a, const void* b, double* out);
+/* r = b - A x and ||r||^2 (cgsolver.py:121-131 intent); rnorm2 may be NULL (then asynchronous) */
+int km_residual(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, const double* x,
+                double* r, double* rnorm2);
+/* BICGSolver.solve (bicgsolver.py:88-136): FP64 coefficients, right-hand side and solution. */
+int km_bicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x,
+                double eps, int max_iter, int flags, km_solve_info* info);
+/* CGSolver.solve (cgsolver.py:83-114). */
+int km_cg(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x,
+          double eps, int max_iter, int flags, km_solve_info* info);
+
+/* ---- measurement hook (no counterpart in the reference) ----
+ * Average device time in microseconds of one launch of a solver kernel on an
+ * NX x NY system, from HIP events around `reps` launches on the context's
+ * stream.  Used by the benchmark to report achieved bandwidth of the kernel
+ * that dominates the solve; rocprofv3 --kernel-trace reports the same kernels. */
+#define KM_KERNEL_BICG_A 0   /* p update + stencil + <r~, v>  */
+#define KM_KERNEL_BICG_B 1   /* s update + stencil + <t, s>, <t, t> */
+#define KM_KERNEL_BICG_C 2   /* x, r update + <r, r>, <r, r~> */
+#define KM_KERNEL_STENCIL 3  /* plain y = A x */
+int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps,
+                   double* avg_us);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KARMAN_H */

@@ -1,0 +1,143 @@
+// Context, error reporting and workspace arena of libkarman_hip.so.
+#include "km_common.h"
+
+#include <cstdlib>
+#include <cstring>
+
+int km_fail(km_ctx* ctx, int code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+void km_drop_graphs(km_ctx* ctx) {
+    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+}
+
+// Fixed header of the arena: KState slots, partials, small results.  Vector
+// storage follows it.  Growing the arena frees the old one, so every captured
+// graph (which holds raw pointers) is dropped.
+static size_t header_bytes() {
+    size_t h = 8 * sizeof(KState);
+    h = (h + 255) & ~size_t(255);
+    h += KM_MAX_PARTIALS * sizeof(double);
+    h += 64 * sizeof(double);
+    return (h + 255) & ~size_t(255);
+}
+
+int km_ensure_workspace(km_ctx* ctx, size_t vec_bytes) {
+    const size_t need = header_bytes() + ((vec_bytes + 255) & ~size_t(255));
+    if (ctx->ws && ctx->ws_bytes >= need) return KM_OK;
+    KM_HIP(ctx, hipSetDevice(ctx->device));
+    if (ctx->ws) {
+        KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        km_drop_graphs(ctx);
+        KM_HIP(ctx, hipFree(ctx->ws));
+        ctx->ws = nullptr;
+        ctx->ws_bytes = 0;
+    }
+    size_t bytes = need + need / 4;   // headroom so small growth does not thrash
+    hipError_t e = hipMalloc(&ctx->ws, bytes);
+    if (e != hipSuccess) {
+        ctx->ws = nullptr;
+        return km_fail(ctx, KM_ENOMEM, "workspace hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    ctx->ws_bytes = bytes;
+    ctx->ws_gen++;
+    // zero once, synchronously, outside any captured graph
+    KM_HIP(ctx, hipMemset(ctx->ws, 0, header_bytes()));
+    char* base = static_cast<char*>(ctx->ws);
+    size_t off = 0;
+    ctx->st = reinterpret_cast<KState*>(base + off);
+    off += (8 * sizeof(KState) + 255) & ~size_t(255);
+    ctx->partials = reinterpret_cast<double*>(base + off);
+    off += KM_MAX_PARTIALS * sizeof(double);
+    ctx->dres = reinterpret_cast<double*>(base + off);
+    return KM_OK;
+}
+
+void* km_ws_vectors(km_ctx* ctx) { return static_cast<char*>(ctx->ws) + header_bytes(); }
+
+extern "C" {
+
+int km_create(km_ctx** out, int device, void* hip_stream) {
+    if (!out) return KM_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return KM_EHIP;
+    if (device < 0 || device >= n) return KM_EINVAL;
+    km_ctx* ctx = new km_ctx();
+    ctx->device = device;
+    ctx->stream = static_cast<hipStream_t>(hip_stream);
+    if (hipSetDevice(device) != hipSuccess) { delete ctx; return KM_EHIP; }
+    if (hipHostMalloc(reinterpret_cast<void**>(&ctx->h_st), 4 * sizeof(KState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&ctx->h_res), 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        delete ctx;
+        return KM_ENOMEM;
+    }
+    if (hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev[1], hipEventDisableTiming) != hipSuccess) {
+        delete ctx;
+        return KM_EHIP;
+    }
+    *out = ctx;
+    return KM_OK;
+}
+
+int km_destroy(km_ctx* ctx) {
+    if (!ctx) return KM_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    km_drop_graphs(ctx);
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->h_st) (void)hipHostFree(ctx->h_st);
+    if (ctx->h_res) (void)hipHostFree(ctx->h_res);
+    if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete ctx;
+    return KM_OK;
+}
+
+int km_set_stream(km_ctx* ctx, void* hip_stream) {
+    if (!ctx) return KM_EINVAL;
+    ctx->stream = static_cast<hipStream_t>(hip_stream);
+    return KM_OK;
+}
+
+const char* km_last_error(const km_ctx* ctx) { return ctx ? ctx->err.c_str() : "NULL context"; }
+
+int km_synchronize(km_ctx* ctx) {
+    if (!ctx) return KM_EINVAL;
+    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KM_OK;
+}
+
+int km_set_tile(km_ctx* ctx, int NX, int NY, int prec, int ti, int tj) {
+    if (!ctx) return KM_EINVAL;
+    auto key = std::make_tuple(NX, NY, prec);
+    if (ti == 0 && tj == 0) {
+        ctx->tile_override.erase(key);
+        ctx->tiles.erase(key);
+        return KM_OK;
+    }
+    KM_REQUIRE(ctx, ti >= 1 && tj >= 1 && (int64_t)(ti + 2) * (tj + 2) * 8 <= 64 * 1024,
+               "tile %dx%d is invalid or needs more than 64 KiB of LDS", ti, tj);
+    ctx->tile_override[key] = {ti, tj};
+    return KM_OK;
+}
+
+int km_get_tile(km_ctx* ctx, int NX, int NY, int prec, int* ti, int* tj) {
+    if (!ctx || !ti || !tj) return KM_EINVAL;
+    return km_choose_tile(ctx, NX, NY, prec, ti, tj);
+}
+
+}  // extern "C"

@@ -1,0 +1,279 @@
+"""Drop-in ``SIMPLESolver`` (simple.py:9-285) on hand-written gfx950 kernels.
+
+Fields are FP64 torch tensors on a ROCm device in the reference's padded
+shapes (simple.py:27-47); ``coef_u``/``coef_v``/``coef_p`` are (NX, NY, 5)
+views of SoA storage, so ``coef_u[i, j, 0]`` is aP exactly as in the
+reference while the kernels stream five contiguous planes.  All arithmetic
+runs in libkarman_hip.so; torch only allocates and moves memory.
+
+Fused kernels change the intermediate state, not the result of the
+reference's call sequence:
+
+* ``compute_coef_u`` / ``compute_coef_v`` already apply the boundary folds of
+  ``set_bc`` (simple.py:177-216) and the obstacle rows; ``set_bc`` then only
+  writes the ghost velocities.  Calling them in the reference's order
+  (compute_coef_u, compute_coef_v, set_bc) gives the reference's final state.
+* ``compute_coef_p`` writes ``mdiv`` and ``b_p`` as simple.py:147-148 does.
+
+The channel extensions (obstacle mask ``ct``, zero-gradient outlet with flux
+fix, p' = 0 on the outlet column, time stepping) follow karman_taichi.py and
+are selected by ``east`` and ``ct``.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib, cases
+from ._lib import KmGrid, ptr
+from .solvers import BICGSolver, CGSolver, SolveInfo
+
+
+class SIMPLESolver:
+    def __init__(self, lx, ly, nx, ny, device=None, case: Optional[cases.Case] = None):
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.device = torch.device(device)
+        self.lx, self.ly, self.nx, self.ny = float(lx), float(ly), int(nx), int(ny)
+        self.dx = self.lx / self.nx
+        self.dy = self.ly / self.ny
+        # reference defaults (simple.py:18-25); a Case overrides them
+        self.rho = 1.00
+        self.mu = 0.01
+        self.dt = 10000
+        self.alpha_p = 0.001
+        self.alpha_u = 0.001
+        self.alpha_m = 0.001
+        self.bc = {'w': [0.0, 0.0], 'e': [0.0, 0.0], 'n': [0.0, 0.0], 's': [0.0, 0.0]}
+        self.east = cases.EAST_WALL
+        self.pin_pcor = False
+        self.case = case
+        if case is not None:
+            self.rho, self.mu, self.dt = case.rho, case.mu, case.dt
+            self.alpha_p, self.alpha_u, self.alpha_m = case.alpha_p, case.alpha_u, case.alpha_m
+            self.bc = {k: list(v) for k, v in case.bc.items()}
+            self.east, self.pin_pcor = case.east, case.pin_pcor
+
+        f64 = dict(dtype=torch.float64, device=self.device)
+        su, sv, sp = (nx + 3, ny + 2), (nx + 2, ny + 3), (nx + 2, ny + 2)
+        self.u = torch.zeros(su, **f64)
+        self.v = torch.zeros(sv, **f64)
+        self.u_mid = torch.zeros(su, **f64)
+        self.v_mid = torch.zeros(sv, **f64)
+        self.u0 = torch.zeros(su, **f64)
+        self.v0 = torch.zeros(sv, **f64)
+        self.p = torch.zeros(sp, **f64)
+        self.pcor = torch.zeros(sp, **f64)
+        self.mdiv = torch.zeros(sp, **f64)
+        # cell type: 1 = solid obstacle (karman_taichi.py:46-49); unused at simple.py:40
+        self.ct = torch.zeros(sp, dtype=torch.uint8, device=self.device)
+        self._coef_u = torch.zeros((5,) + su, **f64)
+        self._coef_v = torch.zeros((5,) + sv, **f64)
+        self._coef_p = torch.zeros((5,) + sp, **f64)
+        self.coef_u = self._coef_u.permute(1, 2, 0)
+        self.coef_v = self._coef_v.permute(1, 2, 0)
+        self.coef_p = self._coef_p.permute(1, 2, 0)
+        self.b_u = torch.zeros(su, **f64)
+        self.b_v = torch.zeros(sv, **f64)
+        self.b_p = torch.zeros(sp, **f64)
+        # solvers are built once and rebound (bicgsimp.py:260-262) instead of
+        # re-instantiated every outer step (simple.py:246-247, :254)
+        self.u_momentum_solver = BICGSolver(self._coef_u, self.b_u, self.u_mid)
+        self.v_momentum_solver = BICGSolver(self._coef_v, self.b_v, self.v_mid)
+        self.p_correction_solver = BICGSolver(self._coef_p, self.b_p, self.pcor)
+        self.p_correction_cg = CGSolver(self._coef_p, self.b_p, self.pcor)
+        self.ctx = self.u_momentum_solver.ctx
+        self._mdiv_stats = torch.zeros(2, dtype=torch.float64)
+        self.eps_mom = 1e-8
+        self.eps_p = 1e-8
+        self.p_solver = "bicgstab"    # or "cg" (simple.py:254)
+        self.mixed = False            # FP32 inner solves + FP64 refinement
+        self.warm_start_p = False
+        self.last = {}
+
+    @classmethod
+    def from_case(cls, case: cases.Case, device=None) -> "SIMPLESolver":
+        s = cls(case.lx, case.ly, case.nx, case.ny, device=device, case=case)
+        s.eps_mom, s.eps_p = case.eps_mom, case.eps_p
+        s.ct.copy_(torch.from_numpy(cases.make_mask(case)))
+        return s
+
+    # ------------------------------------------------------------ plumbing
+    def _grid(self) -> KmGrid:
+        g = KmGrid()
+        g.nx, g.ny = self.nx, self.ny
+        g.east, g.pin_pcor = int(self.east), int(bool(self.pin_pcor))
+        g.dx, g.dy, g.rho, g.mu, g.dt = self.dx, self.dy, self.rho, self.mu, float(self.dt)
+        g.alpha_u, g.alpha_p, g.alpha_m = self.alpha_u, self.alpha_p, self.alpha_m
+        for s, side in enumerate("wens"):
+            g.bc[s][0] = float(self.bc[side][0])
+            g.bc[s][1] = float(self.bc[side][1])
+        return g
+
+    def _call(self, name, *args):
+        self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+        g = self._grid()
+        return self.ctx.call(name, ctypes.byref(g), *args)
+
+    # ------------------------------------------------------------ kernels
+    def compute_coef_u(self):
+        """simple.py:77-92 + the u half of set_bc (:177-207) + obstacle rows."""
+        self._call("km_assemble_u", ptr(self.u), ptr(self.v), ptr(self.p), ptr(self.u0), ptr(self.ct),
+                   ptr(self._coef_u), ptr(self.b_u))
+
+    def compute_coef_v(self):
+        """simple.py:95-110 + the v half of set_bc (:188-216) + obstacle rows."""
+        self._call("km_assemble_v", ptr(self.u), ptr(self.v), ptr(self.p), ptr(self.v0), ptr(self.ct),
+                   ptr(self._coef_v), ptr(self.b_v))
+
+    def set_bc(self):
+        """Ghost velocities of set_bc (simple.py:181, :186, :192, :196); the
+        coefficient folds were applied by compute_coef_u/v."""
+        self._call("km_apply_ghosts", ptr(self.u), ptr(self.v))
+
+    def compute_mdiv(self):
+        """simple.py:137-141."""
+        self._call("km_mdiv", ptr(self.u), ptr(self.v), ptr(self.mdiv), None)
+
+    def mdiv_stats(self):
+        """(max|mdiv|, ||mdiv||_2) over interior cells; recomputes mdiv (bicgsimp.py:114-122)."""
+        out = (ctypes.c_double * 2)()
+        self._call("km_mdiv", ptr(self.u), ptr(self.v), ptr(self.mdiv), out)
+        return out[0], math.sqrt(max(out[1], 0.0))
+
+    def compute_coef_p(self):
+        """simple.py:143-170 with the channel's p' Dirichlet cells."""
+        self._call("km_assemble_p", ptr(self.u), ptr(self.v), ptr(self._coef_u), ptr(self._coef_v),
+                   ptr(self.ct), ptr(self._coef_p), ptr(self.b_p), ptr(self.mdiv))
+
+    def correct_pressure(self):
+        """simple.py:218-222."""
+        self._correct(1)
+
+    def correct_velocity(self):
+        """simple.py:224-230."""
+        self._correct(2)
+
+    def _correct(self, parts=3):
+        """Both corrections in one launch when parts == 3."""
+        self._call("km_correct", ptr(self.p), ptr(self.u), ptr(self.v), ptr(self.pcor), ptr(self._coef_u),
+                   ptr(self._coef_v), ptr(self.ct), int(parts))
+
+    def update_velocity(self):
+        """simple.py:232-238."""
+        self._call("km_blend", ptr(self.u), ptr(self.v), ptr(self.u_mid), ptr(self.v_mid))
+
+    def correct_conserv(self):
+        """Zero-gradient outlet + flux fix (karman_taichi.py:180-183, :586-614)."""
+        if self.east == cases.EAST_OUTFLOW:
+            self._call("km_outflow_fix", ptr(self.u))
+
+    def time_forward(self):
+        """karman_taichi.py:617-622."""
+        self._call("km_time_forward", ptr(self.u), ptr(self.v), ptr(self.u0), ptr(self.v0))
+
+    # ------------------------------------------------------------ solves
+    def solve_momentum_eqn(self, n_iter=1):
+        """simple.py:241-250."""
+        iu = iv = None
+        for _ in range(n_iter):
+            self.compute_coef_u()
+            self.compute_coef_v()
+            self.set_bc()
+            iu = self.u_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed)
+            iv = self.v_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed)
+            self.update_velocity()
+        return iu, iv
+
+    def solve_pcorrection_eqn(self):
+        """simple.py:252-255 (BiCGSTAB by default, CG on request)."""
+        self.compute_coef_p()
+        if self.p_solver == "cg":
+            return self.p_correction_cg.solve(eps=self.eps_p)
+        return self.p_correction_solver.solve(eps=self.eps_p, mixed=self.mixed, x0=self.warm_start_p)
+
+    def init(self, seed_fields=True):
+        """simple.py:257-264; with a Case, the seeded channel start state."""
+        if self.case is not None and seed_fields:
+            u, v, p, u0, v0 = cases.initial_fields(self.case, self.ct.cpu().numpy())
+            for dst, src in ((self.u, u), (self.v, v), (self.p, p), (self.u0, u0), (self.v0, v0)):
+                dst.copy_(torch.from_numpy(src))
+        else:
+            for t in (self.u, self.v, self.p, self.u0, self.v0):
+                t.zero_()
+        self.pcor.zero_()
+
+    def step(self):
+        """One SIMPLE outer iteration (simple.py:270-281 order; the outlet fix
+        sits between the momentum blend and the p' assembly as in
+        karman_taichi.py:645-646)."""
+        iu, iv = self.solve_momentum_eqn(1)
+        self.correct_conserv()
+        ip = self.solve_pcorrection_eqn()
+        self._correct()
+        self.compute_mdiv()
+        self.last = {"u": iu, "v": iv, "p": ip}
+        return self.last
+
+    def solve(self, n_steps=None, n_outer=None, callback=None):
+        """Time loop: n_outer SIMPLE iterations per time step, then u0 <- u."""
+        c = self.case
+        n_steps = n_steps if n_steps is not None else (c.n_steps if c else 2000)
+        n_outer = n_outer if n_outer is not None else (c.n_outer if c else 1)
+        for ts in range(n_steps):
+            for k in range(n_outer):
+                self.step()
+                if callback is not None:
+                    callback(self, ts, k)
+            self.time_forward()
+
+    # ------------------------------------------------------------ outputs
+    def fields_numpy(self):
+        return {name: getattr(self, name).detach().cpu().numpy()
+                for name in ("u", "v", "p", "pcor", "mdiv")}
+
+    def dump_matrix(self, step, msg, folder="log"):
+        """CSV dumps named as simple.py:51-53."""
+        for name, val in self.fields_numpy().items():
+            np.savetxt(f"{folder}/{step:06}-{name}-{msg}.csv", val, delimiter=",")
+
+    def forces(self):
+        """Pressure + viscous force on the obstacle (drag, lift) per unit depth.
+
+        Pressure acts on the obstacle's outer faces (cell-centre p of the fluid
+        neighbour); wall shear is mu * (u_t - 0) / (d/2) with the tangential
+        velocity of the adjacent fluid face (the same 2 mu / d wall model the
+        momentum equations use).
+        """
+        m = self.ct.cpu().numpy().astype(bool)
+        if not m.any():
+            return 0.0, 0.0
+        p = self.p.cpu().numpy()
+        u = self.u.cpu().numpy()
+        v = self.v.cpu().numpy()
+        ii, jj = np.nonzero(m)
+        i0, i1, j0, j1 = ii.min(), ii.max(), jj.min(), jj.max()
+        dx, dy, mu = self.dx, self.dy, self.mu
+        J = slice(j0, j1 + 1)
+        I = slice(i0, i1 + 1)
+        fx = (p[i0 - 1, J] - p[i1 + 1, J]).sum() * dy
+        fy = (p[I, j0 - 1] - p[I, j1 + 1]).sum() * dx
+        # shear on top/bottom faces from u at faces i0+1..i1 (interior of the span)
+        ut = 0.5 * (u[i0:i1 + 1, j1 + 1] + u[i0 + 1:i1 + 2, j1 + 1])
+        ub = 0.5 * (u[i0:i1 + 1, j0 - 1] + u[i0 + 1:i1 + 2, j0 - 1])
+        fx += mu * (ut.sum() + ub.sum()) * dx / (0.5 * dy)
+        vl = 0.5 * (v[i0 - 1, j0:j1 + 1] + v[i0 - 1, j0 + 1:j1 + 2])
+        vr = 0.5 * (v[i1 + 1, j0:j1 + 1] + v[i1 + 1, j0 + 1:j1 + 2])
+        fy += mu * (vl.sum() + vr.sum()) * dy / (0.5 * dx)
+        return float(fx), float(fy)
+
+    def drag_lift_coefficients(self, u_ref=1.0):
+        d = self.case.obstacle_diameter() if self.case else 1.0
+        fx, fy = self.forces()
+        q = 0.5 * self.rho * u_ref * u_ref * d
+        return fx / q, fy / q

@@ -1,0 +1,298 @@
+"""NumPy FP64 restatement of the implicit 5-point stencil solvers (oracle, test-only).
+
+Vectors are padded 2-D arrays (NX, NY); the matrix is the SoA coefficient
+array (5, NX, NY), k = 0 aP, 1 aW, 2 aE, 3 aN, 4 aS.  The stencil runs over
+the whole padded range (cgsolver.py:13 ``N_ext = 0``); neighbours outside the
+array read as zero (the reference reads out of bounds there and relies on
+zero coefficients, cgsolver.py:59-60 -- parity unpinned for non-zero halos).
+
+Status codes match the C-ABI (include/karman.h).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+OK = 0
+BREAKDOWN = 1
+MAXITER = 2
+NONFINITE = 3
+STALL = 4
+
+
+def stencil(coef, x):
+    """y = c0 x + c1 x_W + c2 x_E + c3 x_N + c4 x_S (bicgsolver.py:65-76,
+    cgsolver.py:56-66), summed left to right as the reference writes it."""
+    xp = np.zeros((x.shape[0] + 2, x.shape[1] + 2), dtype=x.dtype)
+    xp[1:-1, 1:-1] = x
+    return (coef[0] * x + coef[1] * xp[:-2, 1:-1] + coef[2] * xp[2:, 1:-1]
+            + coef[3] * xp[1:-1, 2:] + coef[4] * xp[1:-1, :-2])
+
+
+def dot(a, b):
+    """<a, b> over the whole padded field (cgsolver.py:49-54)."""
+    return float(np.dot(a.ravel(), b.ravel()))
+
+
+def to_csr(coef):
+    """The same operator as a scipy CSR matrix over the flattened padded field."""
+    import scipy.sparse as sp
+    _, NX, NY = coef.shape
+    n = NX * NY
+    idx = np.arange(n).reshape(NX, NY)
+    rows, cols, vals = [idx.ravel()], [idx.ravel()], [coef[0].ravel()]
+    for k, (di, dj) in zip((1, 2, 3, 4), ((-1, 0), (1, 0), (0, 1), (0, -1))):
+        ii, jj = np.meshgrid(np.arange(NX), np.arange(NY), indexing='ij')
+        ni, nj = ii + di, jj + dj
+        ok = (ni >= 0) & (ni < NX) & (nj >= 0) & (nj < NY)
+        rows.append(idx[ok])
+        cols.append(idx[ni[ok], nj[ok]])
+        vals.append(coef[k][ok])
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    return A
+
+
+def direct_solve(coef, b):
+    """Exact solution by sparse LU over the rows that are not identically zero
+    (the padded halo rows are decoupled 0 = 0 equations, left at zero)."""
+    import scipy.sparse.linalg as spla
+    A = to_csr(coef)
+    live = np.flatnonzero(np.abs(A).sum(axis=1).A.ravel() != 0)
+    x = np.zeros(b.size)
+    if live.size:
+        sub = A[live][:, live].tocsc()
+        x[live] = spla.spsolve(sub, b.ravel()[live])
+    return x.reshape(b.shape)
+
+
+class Info(dict):
+    pass
+
+
+def bicgstab(coef, b, eps=1e-8, x0=None, max_iter=None, precond=None, history=None):
+    """BiCGSTAB of bicgsolver.py:88-136 (van der Vorst, right-preconditioned hooks).
+
+    x0 = 0 and r~ = r = b (bicgsolver.py:21-35) unless ``x0`` is given (then
+    r = b - A x0).  Stop when sqrt(r.r / b.b) < eps with the recursive residual
+    (bicgsolver.py:130, initial_rTr is r0.r0 which equals b.b for x0 = 0).
+    ``precond`` is the M^-1 hook of update_phat/update_shat (bicgsolver.py:48-58;
+    identity when None).  Guards: rho == 0 -> BREAKDOWN (bicgsolver.py:96-98),
+    <r~, v> == 0 -> BREAKDOWN, <t, t> == 0 -> omega = 0 and, unless that
+    converged, BREAKDOWN (the omega == 0 check of karman_taichi.py:451-454);
+    non-finite residual -> NONFINITE.  max_iter defaults to NX*NY
+    (cgsolver.py:14).
+    """
+    NX, NY = b.shape
+    steps = NX * NY if max_iter is None else max_iter
+    M = (lambda z: z) if precond is None else precond
+    if x0 is None:
+        x = np.zeros_like(b)
+        r = b.copy()
+    else:
+        x = x0.copy()
+        r = b - stencil(coef, x)
+    r_tld = r.copy()
+    p = np.zeros_like(b)
+    v = np.zeros_like(b)
+    omega = alpha = rho_1 = 1.0
+    initial_rTr = dot(r, r)
+    info = Info(iterations=0, status=OK, rel_resid=0.0, rho=[], alpha=[], omega=[])
+    if initial_rTr == 0.0:
+        info["rel_resid_true"] = 0.0
+        return x, info
+    rTr = initial_rTr
+    status = MAXITER
+    it = 0
+    for i in range(steps):
+        rho = dot(r, r_tld)
+        if rho == 0.0:
+            status = BREAKDOWN
+            break
+        if i == 0:
+            p = r.copy()
+        else:
+            beta = (rho / rho_1) * (alpha / omega)
+            p = r + beta * (p - omega * v)
+        p_hat = M(p)
+        v = stencil(coef, p_hat)
+        rv = dot(r_tld, v)
+        if rv == 0.0:
+            status = BREAKDOWN
+            break
+        alpha = rho / rv
+        s = r - alpha * v
+        s_hat = M(s)
+        t = stencil(coef, s_hat)
+        tt = dot(t, t)
+        omega = dot(t, s) / tt if tt != 0.0 else 0.0
+        x = x + (alpha * p_hat + omega * s_hat)
+        r = s - omega * t
+        rTr = dot(r, r)
+        it = i + 1
+        info["rho"].append(rho)
+        info["alpha"].append(alpha)
+        info["omega"].append(omega)
+        if history is not None:
+            history.append(math.sqrt(max(rTr, 0.0)))
+        if not math.isfinite(rTr):
+            status = NONFINITE
+            break
+        if math.sqrt(rTr / initial_rTr) < eps:
+            status = OK
+            break
+        if omega == 0.0:
+            status = BREAKDOWN
+            break
+        rho_1 = rho
+    info["iterations"] = it
+    info["status"] = status
+    info["rel_resid"] = math.sqrt(rTr / initial_rTr) if math.isfinite(rTr) else float("nan")
+    bb = dot(b, b)
+    rt = b - stencil(coef, x)
+    info["rel_resid_true"] = math.sqrt(dot(rt, rt) / bb) if bb > 0 else 0.0
+    return x, info
+
+
+def cg(coef, b, eps=1e-8, max_iter=None):
+    """Conjugate gradient of cgsolver.py:83-114 (x0 = 0, p0 = r0 = b,
+    stop when |r| < eps |r0|)."""
+    NX, NY = b.shape
+    steps = NX * NY if max_iter is None else max_iter
+    x = np.zeros_like(b)
+    r = b.copy()
+    p = r.copy()
+    initial_rTr = dot(r, r)
+    old = initial_rTr
+    info = Info(iterations=0, status=OK)
+    if initial_rTr == 0.0:
+        info["rel_resid"] = 0.0
+        info["rel_resid_true"] = 0.0
+        return x, info
+    status = MAXITER
+    it = 0
+    new = old
+    for i in range(steps):
+        Ap = stencil(coef, p)
+        pAp = dot(p, Ap)
+        if pAp == 0.0:
+            status = BREAKDOWN
+            break
+        alpha = old / pAp
+        x = x + alpha * p
+        r = r - alpha * Ap
+        new = dot(r, r)
+        it = i + 1
+        if not math.isfinite(new):
+            status = NONFINITE
+            break
+        if math.sqrt(new) < math.sqrt(initial_rTr) * eps:
+            status = OK
+            break
+        beta = new / old
+        p = r + beta * p
+        old = new
+    info["iterations"] = it
+    info["status"] = status
+    info["rel_resid"] = math.sqrt(new / initial_rTr) if math.isfinite(new) else float("nan")
+    rt = b - stencil(coef, x)
+    info["rel_resid_true"] = math.sqrt(dot(rt, rt) / initial_rTr)
+    return x, info
+
+
+def jacobi_precond(coef):
+    """M^-1 = diag(A)^-1, the hook commented out at bicgsolver.py:51,57 and
+    active at karman_taichi.py:399,419; zero diagonal rows map to zero."""
+    d = coef[0]
+    inv = np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 0.0)
+    return lambda z: inv * z
+
+
+def fbicgstab(coef, b, eps=1e-8, x0=None, max_iter=None, precond=None, stall_window=200,
+              stall_ratio=0.9, fallback=True):
+    """Flexible right-preconditioned BiCGSTAB judged only by the true residual.
+
+    The preconditioner may change between applications (a neural network).
+    Convergence: ||b - A x|| / ||b|| < eps with the residual recomputed from x.
+    Stall: when the best true residual has not dropped by ``stall_ratio`` over
+    ``stall_window`` iterations (or the iteration broke down), restart with
+    plain BiCGSTAB from the current x.  Returns (x, info) with info['fallback'].
+    """
+    NX, NY = b.shape
+    steps = NX * NY if max_iter is None else max_iter
+    bb = dot(b, b)
+    info = Info(iterations=0, status=OK, fallback=False)
+    if bb == 0.0:
+        info["rel_resid_true"] = 0.0
+        return np.zeros_like(b), info
+    bn = math.sqrt(bb)
+    x = np.zeros_like(b) if x0 is None else x0.copy()
+    r = b - stencil(coef, x)
+    rel = math.sqrt(dot(r, r)) / bn
+    if rel < eps:
+        info["rel_resid_true"] = rel
+        return x, info
+    r_tld = r.copy()
+    p = np.zeros_like(b)
+    v = np.zeros_like(b)
+    omega = alpha = rho_1 = 1.0
+    best, best_it = rel, 0
+    it = 0
+    status = MAXITER
+    M = (lambda z: z) if precond is None else precond
+    for i in range(steps):
+        rho = dot(r, r_tld)
+        if rho == 0.0 or not math.isfinite(rho):
+            status = BREAKDOWN
+            break
+        if i == 0:
+            p = r.copy()
+        else:
+            beta = (rho / rho_1) * (alpha / omega)
+            p = r + beta * (p - omega * v)
+        p_hat = M(p)
+        v = stencil(coef, p_hat)
+        rv = dot(r_tld, v)
+        if rv == 0.0 or not math.isfinite(rv):
+            status = BREAKDOWN
+            break
+        alpha = rho / rv
+        s = r - alpha * v
+        s_hat = M(s)
+        t = stencil(coef, s_hat)
+        tt = dot(t, t)
+        omega = dot(t, s) / tt if tt != 0.0 else 0.0
+        x_new = x + (alpha * p_hat + omega * s_hat)
+        it = i + 1
+        r_true = b - stencil(coef, x_new)
+        rel = math.sqrt(dot(r_true, r_true)) / bn
+        if not math.isfinite(rel):
+            status = NONFINITE
+            break
+        x = x_new
+        r = s - omega * t
+        if rel < eps:
+            status = OK
+            break
+        if rel < stall_ratio * best:
+            best, best_it = rel, it
+        elif it - best_it >= stall_window:
+            status = STALL
+            break
+        if omega == 0.0:
+            status = BREAKDOWN
+            break
+        rho_1 = rho
+    info["iterations"] = it
+    info["status"] = status
+    if status != OK and fallback:
+        info["fallback"] = True
+        r0 = b - stencil(coef, x)
+        d, sub = bicgstab(coef, r0, eps=eps * bn / max(math.sqrt(dot(r0, r0)), 1e-300),
+                          max_iter=max_iter)
+        x = x + d
+        info["fallback_iterations"] = sub["iterations"]
+        info["status"] = sub["status"]
+    rt = b - stencil(coef, x)
+    info["rel_resid_true"] = math.sqrt(dot(rt, rt)) / bn
+    return x, info

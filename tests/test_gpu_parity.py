@@ -1,0 +1,382 @@
+"""GPU parity: HIP kernels (through the C-ABI) against the FP64 oracle.
+
+Bars
+* discretisation kernels (assembly, BCs, mask, blend, outlet fix, corrections,
+  mdiv, time advance): bit-exact (the library is built without FMA
+  contraction and mirrors each expression's evaluation order);
+* stencil: bit-exact in FP64 and FP32 (same products summed in the same order);
+* reductions: relative 1e-13 (different, but deterministic, summation order);
+* solvers: same status, iteration counts within a band, solution against the
+  exact (LU) solution within the bound the stopping tolerance implies;
+* SIMPLE step: relative L2 error of u, v, p <= 1e-9 after one outer step from
+  the seeded C1 state with inner eps = 1e-13 (oracle: exact LU solves).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from karman_taichi_amd import _lib, cases
+from karman_taichi_amd._lib import KmSolveInfo, ptr
+from karman_taichi_amd.simple import SIMPLESolver
+from karman_taichi_amd.solvers import BICGSolver, CGSolver
+from oracle import fvm, krylov
+
+pytestmark = pytest.mark.gpu
+
+
+def T(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+def random_state(c, seed):
+    rng = np.random.default_rng(seed)
+    m = cases.make_mask(c)
+    u = rng.random(c.shape_u) + 0.5
+    v = 0.2 * rng.standard_normal(c.shape_v)
+    p = rng.standard_normal(c.shape_p)
+    u0 = rng.random(c.shape_u)
+    v0 = 0.2 * rng.standard_normal(c.shape_v)
+    return u, v, p, u0, v0, m
+
+
+def solver_with(c, dev, u, v, p, u0, v0, m):
+    s = SIMPLESolver.from_case(c, device=dev)
+    for dst, src in ((s.u, u), (s.v, v), (s.p, p), (s.u0, u0), (s.v0, v0)):
+        dst.copy_(T(src, dev))
+    s.ct.copy_(T(m, dev))
+    return s
+
+
+def wall_case():
+    return cases.Case(name="walls", nx=13, ny=9, lx=1.3, ly=1.0, mu=0.02, dt=0.05, east=cases.EAST_WALL,
+                      bc={'w': [0.7, 0.2], 'e': [0.4, -0.3], 'n': [0.1, 1.0], 's': [-0.2, 0.5]},
+                      pin_pcor=True)
+
+
+CASES = [cases.c1, wall_case, lambda: cases.Case(name="odd", nx=37, ny=23, lx=2.0, obstacle_cells=3,
+                                                 obstacle_xc=0.6, mu=0.01, dt=0.02)]
+
+
+@pytest.mark.parametrize("mk", CASES)
+def test_assembly_bit_exact(mk, dev):
+    c = mk()
+    u, v, p, u0, v0, m = random_state(c, 1)
+    s = solver_with(c, dev, u, v, p, u0, v0, m)
+    s.compute_coef_u()
+    s.compute_coef_v()
+    cu, bu = fvm.assemble_u(c, u, v, p, u0, m)
+    cv, bv = fvm.assemble_v(c, u, v, p, v0, m)
+    np.testing.assert_array_equal(N(s._coef_u), cu)
+    np.testing.assert_array_equal(N(s.b_u), bu)
+    np.testing.assert_array_equal(N(s._coef_v), cv)
+    np.testing.assert_array_equal(N(s.b_v), bv)
+    s.compute_coef_p()
+    cp, bp, md = fvm.assemble_p(c, u, v, cu[0], cv[0], m)
+    np.testing.assert_array_equal(N(s._coef_p), cp)
+    np.testing.assert_array_equal(N(s.b_p), bp)
+    np.testing.assert_array_equal(N(s.mdiv), md)
+
+
+@pytest.mark.parametrize("mk", CASES)
+def test_pointwise_kernels_bit_exact(mk, dev):
+    c = mk()
+    u, v, p, u0, v0, m = random_state(c, 2)
+    rng = np.random.default_rng(3)
+    um, vm = rng.random(c.shape_u), rng.standard_normal(c.shape_v)
+    pc = rng.standard_normal(c.shape_p)
+    s = solver_with(c, dev, u, v, p, u0, v0, m)
+    s.u_mid.copy_(T(um, dev)); s.v_mid.copy_(T(vm, dev)); s.pcor.copy_(T(pc, dev))
+    s.compute_coef_u(); s.compute_coef_v()
+    cu, _ = fvm.assemble_u(c, u, v, p, u0, m)
+    cv, _ = fvm.assemble_v(c, u, v, p, v0, m)
+    # ghosts (set_bc)
+    s.set_bc()
+    ur, vr = u.copy(), v.copy()
+    fvm.apply_ghosts(c, ur, vr)
+    np.testing.assert_array_equal(N(s.u), ur)
+    np.testing.assert_array_equal(N(s.v), vr)
+    # blend (update_velocity)
+    s.update_velocity()
+    fvm.blend(c, ur, vr, um, vm)
+    np.testing.assert_array_equal(N(s.u), ur)
+    np.testing.assert_array_equal(N(s.v), vr)
+    # outlet fix
+    if c.east == cases.EAST_OUTFLOW:
+        s.correct_conserv()
+        fvm.outflow_fix(c, ur)
+        np.testing.assert_array_equal(N(s.u), ur)
+    # corrections, separately and fused
+    pr = p.copy()
+    s.correct_pressure()
+    fvm.correct(c, pr, ur.copy(), vr.copy(), pc, cu[0], cv[0], m)
+    np.testing.assert_array_equal(N(s.p), pr)
+    s.correct_velocity()
+    fvm.correct(c, pr.copy(), ur, vr, pc, cu[0], cv[0], m)
+    np.testing.assert_array_equal(N(s.u), ur)
+    np.testing.assert_array_equal(N(s.v), vr)
+    # mdiv with statistics
+    mx, l2 = s.mdiv_stats()
+    md = fvm.mdiv(c, ur, vr)
+    np.testing.assert_array_equal(N(s.mdiv), md)
+    assert mx == np.abs(md).max()
+    assert l2 == pytest.approx(np.linalg.norm(md), rel=1e-13)
+    # time advance
+    s.time_forward()
+    np.testing.assert_array_equal(N(s.u0), ur)
+    np.testing.assert_array_equal(N(s.v0), vr)
+
+
+@pytest.mark.parametrize("shape", [(5, 3), (37, 29), (322, 66), (1282, 258), (300, 700)])
+@pytest.mark.parametrize("prec", [_lib.KM_F64, _lib.KM_F32])
+def test_stencil_bit_exact(shape, prec, dev):
+    rng = np.random.default_rng(shape[0])
+    npdt = np.float64 if prec == _lib.KM_F64 else np.float32
+    coef = rng.standard_normal((5,) + shape).astype(npdt)
+    x = rng.standard_normal(shape).astype(npdt)
+    ctx = _lib.context_for(dev)
+    ct, xt = T(coef, dev), T(x, dev)
+    yt = torch.empty_like(xt)
+    want = krylov.stencil(coef, x)
+    tiles = [(0, 0), (2, 3), (1, shape[1]), (3, 17), (8, 64)]
+    for ti, tj in tiles:
+        if ti:
+            ctx.call("km_set_tile", shape[0], shape[1], prec, ti, tj)
+        ctx.call("km_stencil_apply", shape[0], shape[1], prec, ptr(ct), ptr(xt), ptr(yt))
+        np.testing.assert_array_equal(N(yt), want, err_msg=f"tile {ti}x{tj}")
+        ctx.call("km_set_tile", shape[0], shape[1], prec, 0, 0)
+
+
+def test_dot_and_residual(dev):
+    rng = np.random.default_rng(4)
+    a, b = rng.standard_normal(100003), rng.standard_normal(100003)
+    ctx = _lib.context_for(dev)
+    out = ctypes.c_double()
+    at, bt = T(a, dev), T(b, dev)     # keep both alive: the C call only sees raw pointers
+    ctx.call("km_dot", ctypes.c_int64(a.size), _lib.KM_F64, ptr(at), ptr(bt), ctypes.byref(out))
+    assert out.value == pytest.approx(float(np.dot(a, b)), rel=1e-12)
+    coef = rng.standard_normal((5, 60, 40)); x = rng.standard_normal((60, 40)); bb = rng.standard_normal((60, 40))
+    r = torch.empty((60, 40), dtype=torch.float64, device=dev)
+    n2 = ctypes.c_double()
+    ct, bt, xt = T(coef, dev), T(bb, dev), T(x, dev)
+    ctx.call("km_residual", 60, 40, ptr(ct), ptr(bt), ptr(xt), ptr(r), ctypes.byref(n2))
+    want = bb - krylov.stencil(coef, x)
+    np.testing.assert_array_equal(N(r), want)
+    assert n2.value == pytest.approx(float((want ** 2).sum()), rel=1e-12)
+
+
+def pcor_system(c, steps=1):
+    m = cases.make_mask(c)
+    st = fvm.State(c, *cases.initial_fields(c, m), m)
+    for _ in range(steps):
+        fvm.simple_step(st, solver="direct")
+    coef, b, _ = fvm.assemble_p(c, st.u, st.v, st.coef_u[0], st.coef_v[0], m)
+    return coef, b
+
+
+@pytest.fixture(scope="module")
+def c1_pcor():
+    return pcor_system(cases.c1(), 1)
+
+
+def bicg(dev, coef, b, eps, max_iter=0, flags=0, x_init=None):
+    ct, bt = T(coef, dev), T(b, dev)
+    xt = T(x_init, dev) if x_init is not None else torch.full_like(bt, 7.0)
+    s = BICGSolver(ct, bt, xt)
+    info = s.solve(eps=eps, max_iter=max_iter, mixed=bool(flags & _lib.KM_FLAG_MIXED),
+                   x0=bool(flags & _lib.KM_FLAG_X0), graph=not (flags & _lib.KM_FLAG_NO_GRAPH))
+    return N(xt), info
+
+
+def test_bicgstab_first_iterations_track_oracle(dev, c1_pcor):
+    coef, b = c1_pcor
+    for k in (1, 2, 3, 5, 10):
+        x, info = bicg(dev, coef, b, eps=0.0, max_iter=k)
+        xo, io = krylov.bicgstab(coef, b, eps=0.0, max_iter=k)
+        assert info.iterations == io["iterations"] == k
+        assert info.status == _lib.KM_MAXITER
+        err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        print(f"k={k} rel err vs oracle {err:.3e}")
+        assert err < 1e-11
+
+
+def test_bicgstab_converges_like_oracle(dev, c1_pcor):
+    coef, b = c1_pcor
+    xd = krylov.direct_solve(coef, b)
+    for eps in (1e-8, 1e-12):
+        x, info = bicg(dev, coef, b, eps=eps)
+        _, io = krylov.bicgstab(coef, b, eps=eps)
+        print(f"eps={eps}: gpu {info.iterations} it, oracle {io['iterations']} it, "
+              f"true resid {info.rel_resid_true:.3e}, err vs LU {np.linalg.norm(x - xd) / np.linalg.norm(xd):.3e}")
+        assert info.status == _lib.KM_OK and io["status"] == krylov.OK
+        assert abs(info.iterations - io["iterations"]) <= 0.1 * io["iterations"] + 5
+        assert info.rel_resid < eps
+        assert info.rel_resid_true < 10 * eps
+        assert np.linalg.norm(x - xd) <= 1e4 * eps * np.linalg.norm(xd)
+
+
+def test_bicgstab_edge_cases(dev, c1_pcor):
+    coef, b = c1_pcor
+    x, info = bicg(dev, coef, np.zeros_like(b), eps=1e-8)
+    assert info.status == _lib.KM_OK and info.iterations == 0 and not x.any()
+    bn = b.copy(); bn[10, 10] = np.nan
+    _, info = bicg(dev, coef, bn, eps=1e-8)
+    assert info.status == _lib.KM_NONFINITE
+    _, info = bicg(dev, coef, b, eps=1e-30, max_iter=1)
+    assert info.status == _lib.KM_MAXITER and info.iterations == 1
+    # <r~, v> = 0 breakdown of the oracle's hand-built case
+    c = np.zeros((5, 3, 4)); c[2, 1, 1] = 1.0; c[1, 2, 1] = -1.0
+    bb = np.zeros((3, 4)); bb[1, 1] = 1.0; bb[2, 1] = 1.0
+    _, info = bicg(dev, c, bb, eps=1e-8)
+    assert info.status == _lib.KM_BREAKDOWN
+
+
+def test_bicgstab_deterministic_graph_and_direct(dev, c1_pcor):
+    coef, b = c1_pcor
+    x1, i1 = bicg(dev, coef, b, eps=1e-10)
+    x2, i2 = bicg(dev, coef, b, eps=1e-10)
+    x3, i3 = bicg(dev, coef, b, eps=1e-10, flags=_lib.KM_FLAG_NO_GRAPH)
+    assert i1.iterations == i2.iterations == i3.iterations
+    np.testing.assert_array_equal(x1, x2)
+    np.testing.assert_array_equal(x1, x3)
+
+
+def test_bicgstab_warm_start(dev, c1_pcor):
+    coef, b = c1_pcor
+    xd = krylov.direct_solve(coef, b)
+    x0 = xd * (1 + 1e-3 * np.random.default_rng(0).standard_normal(xd.shape))
+    x, info = bicg(dev, coef, b, eps=1e-8, flags=_lib.KM_FLAG_X0, x_init=x0)
+    _, cold = bicg(dev, coef, b, eps=1e-8)
+    assert info.status == _lib.KM_OK and info.iterations < cold.iterations
+    assert info.rel_resid_true < 1e-8 * 10
+
+
+def test_cg_matches_oracle(dev, c1_pcor):
+    coef, b = c1_pcor
+    xd = krylov.direct_solve(coef, b)
+    ct, bt = T(coef, dev), T(b, dev)
+    xt = torch.zeros_like(bt)
+    info = CGSolver(ct, bt, xt).solve(eps=1e-10)
+    _, io = krylov.cg(coef, b, eps=1e-10)
+    print(f"CG gpu {info.iterations} it, oracle {io['iterations']} it")
+    assert info.status == _lib.KM_OK
+    assert abs(info.iterations - io["iterations"]) <= 0.05 * io["iterations"] + 3
+    assert np.linalg.norm(N(xt) - xd) <= 1e-5 * np.linalg.norm(xd)
+
+
+def test_mixed_precision_refinement(dev, c1_pcor):
+    """FP32 inner solves + FP64 refinement.  A well-conditioned momentum system
+    converges in FP32 sweeps; the ill-conditioned p' system (kappa ~ 3e5), on
+    which FP32 BiCGSTAB diverges, must still end converged through the FP64
+    fallback, never with a non-finite x."""
+    c = cases.c1()
+    m = cases.make_mask(c)
+    st = fvm.State(c, *cases.initial_fields(c, m), m)
+    cu, bu = fvm.assemble_u(c, st.u, st.v, st.p, st.u0, m)
+    for coef, b, name in ((cu, bu, "u"), c1_pcor + ("p",)):
+        xd = krylov.direct_solve(coef, b)
+        x, info = bicg(dev, coef, b, eps=1e-8, flags=_lib.KM_FLAG_MIXED)
+        print(f"mixed {name}: {info.iterations} it, {info.refinements} sweeps, fallback {info.fallback}, "
+              f"true {info.rel_resid_true:.3e}")
+        assert info.status == _lib.KM_OK and info.rel_resid_true < 1e-8
+        assert np.isfinite(x).all()
+        assert np.linalg.norm(x - xd) <= 1e-4 * np.linalg.norm(xd)
+        if name == "u":
+            assert not info.fallback
+
+
+def test_aos_coefficients_drop_in(dev, c1_pcor):
+    coef, b = c1_pcor
+    aos = np.ascontiguousarray(np.moveaxis(coef, 0, -1))      # reference layout (NX, NY, 5)
+    x1, _ = bicg(dev, coef, b, eps=1e-10)
+    ct, bt = T(aos, dev), T(b, dev)
+    xt = torch.zeros_like(bt)
+    BICGSolver(ct, bt, xt).solve(eps=1e-10)
+    np.testing.assert_array_equal(N(xt), x1)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_simple_step_parity_c1(dev):
+    c = cases.c1()
+    m = cases.make_mask(c)
+    init = cases.initial_fields(c, m)
+    st = fvm.State(c, *init, m)
+    fvm.simple_step(st, solver="direct")
+    s = SIMPLESolver.from_case(c, device=dev)
+    s.init()
+    s.eps_mom = s.eps_p = 1e-13
+    s.step()
+    errs = {k: rel(N(getattr(s, k)), getattr(st, k)) for k in ("u", "v", "p", "pcor")}
+    md_err = np.linalg.norm(N(s.mdiv) - st.mdiv) / np.linalg.norm(st.b_p)
+    print("one-step parity:", errs, "mdiv (rel to ||b_p||)", md_err,
+          {k: (v.iterations, v.status) for k, v in s.last.items()})
+    for k in ("u", "v", "p"):
+        assert errs[k] <= 1e-9, (k, errs[k])
+    assert md_err <= 1e-9
+
+
+def test_simple_multistep_parity_c1(dev):
+    """Two time steps x 4 outer iterations, same seeded start, same inner eps."""
+    c = cases.c1(n_outer=4)
+    m = cases.make_mask(c)
+    st = fvm.State(c, *cases.initial_fields(c, m), m)
+    s = SIMPLESolver.from_case(c, device=dev)
+    s.init()
+    s.eps_mom = s.eps_p = 1e-13
+    for ts in range(2):
+        for k in range(c.n_outer):
+            fvm.simple_step(st, solver="direct")
+            s.step()
+        fvm.time_forward(st)
+        s.time_forward()
+    errs = {k: rel(N(getattr(s, k)), getattr(st, k)) for k in ("u", "v", "p")}
+    print("8-step parity:", errs)
+    for k in ("u", "v", "p"):
+        assert errs[k] <= 1e-8, (k, errs[k])
+
+
+def test_large_grid_properties(dev):
+    """C5-size operator: linearity and symmetry of the assembled p' matrix,
+    bitwise run-to-run repeatability of the fused reductions."""
+    c = cases.c5()
+    s = SIMPLESolver.from_case(c, device=dev)
+    s.init()
+    s.compute_coef_u(); s.compute_coef_v(); s.set_bc()
+    s.compute_coef_p()
+    NX, NY = c.shape_p
+    g = torch.Generator(device="cpu").manual_seed(0)
+    x = torch.randn((NX, NY), generator=g, dtype=torch.float64).to(dev)
+    y = torch.randn((NX, NY), generator=g, dtype=torch.float64).to(dev)
+    live = (s._coef_p[0] != 0).to(torch.float64)
+    x, y = x * live, y * live
+    sol = BICGSolver(s._coef_p, s.b_p, s.pcor)
+    sol.x = x
+    Ax = sol.compute_Ax()
+    sol.x = y
+    Ay = sol.compute_Ax()
+    sol.x = 2.0 * x - 3.0 * y
+    Az = sol.compute_Ax()
+    lin = (Az - (2.0 * Ax - 3.0 * Ay)).abs().max().item() / Az.abs().max().item()
+    sym = abs(sol.reduce(Ax, y) - sol.reduce(x, Ay)) / abs(sol.reduce(Ax, y))
+    print(f"C5 linearity {lin:.2e}, symmetry {sym:.2e}")
+    assert lin < 1e-14 and sym < 1e-12
+    assert sol.reduce(Ax, y) == sol.reduce(Ax, y)
+
+
+def test_c2_pcor_solve_true_residual(dev):
+    c = cases.c2()
+    s = SIMPLESolver.from_case(c, device=dev)
+    s.init()
+    s.solve_momentum_eqn(1)
+    s.correct_conserv()
+    info = s.solve_pcorrection_eqn()
+    print(f"C2 p' solve: {info.iterations} it, recursive {info.rel_resid:.2e}, true {info.rel_resid_true:.2e}")
+    assert info.status == _lib.KM_OK and info.rel_resid_true < 1e-7

@@ -1,0 +1,181 @@
+"""Pins of the Krylov oracle (oracle/krylov.py) without Taichi.
+
+* stencil == scipy CSR matvec built from the same coefficients;
+* BiCGSTAB / CG solutions == scipy sparse LU (cg_to_sparse/spsolver.py:40-46 idea);
+* BiCGSTAB scalar sequence == an independent dense-matrix textbook loop;
+* iteration counts in a band around scipy.sparse.linalg.bicgstab (validation.py:26 idea);
+* the failure paths the reference has (rho == 0 break, bicgsolver.py:96-98) and
+  the ones it lacks (NaN, iteration cap).
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from karman_taichi_amd import cases
+from oracle import fvm, krylov
+
+
+def random_5pt(NX, NY, seed=0, shift=0.1, halo=True):
+    """Random 5-diagonal system with diag = -sum(links) + shift (debug/ti_bicg_test.py:24-41)."""
+    rng = np.random.default_rng(seed)
+    c = np.zeros((5, NX, NY))
+    c[1:] = -rng.random((4, NX, NY))
+    if halo:   # decoupled zero halo as in the reference's padded fields
+        c[:, 0, :] = c[:, -1, :] = 0
+        c[:, :, 0] = c[:, :, -1] = 0
+    c[1, 1, :] = 0; c[2, -2, :] = 0; c[4, :, 1] = 0; c[3, :, -2] = 0
+    c[0] = -(c[1] + c[2] + c[3] + c[4]) + shift
+    if halo:
+        c[0, 0, :] = c[0, -1, :] = 0
+        c[0, :, 0] = c[0, :, -1] = 0
+    b = np.zeros((NX, NY))
+    b[1:-1, 1:-1] = rng.standard_normal((NX - 2, NY - 2))
+    return c, b
+
+
+def p_system(case, steps=1):
+    """A p' system from the channel after `steps` SIMPLE steps of the oracle."""
+    m = cases.make_mask(case)
+    st = fvm.State(case, *cases.initial_fields(case, m), m)
+    for _ in range(steps):
+        fvm.simple_step(st, solver="direct")
+    coef, b, _ = fvm.assemble_p(case, st.u, st.v, st.coef_u[0], st.coef_v[0], m)
+    return coef, b
+
+
+@pytest.mark.parametrize("shape", [(3, 4), (9, 7), (40, 17)])
+def test_stencil_equals_csr(shape):
+    rng = np.random.default_rng(1)
+    coef = rng.standard_normal((5,) + shape)
+    x = rng.standard_normal(shape)
+    y = krylov.stencil(coef, x)
+    y_csr = (krylov.to_csr(coef) @ x.ravel()).reshape(shape)
+    np.testing.assert_allclose(y, y_csr, rtol=1e-13, atol=1e-13)
+
+
+def test_stencil_neighbour_directions():
+    # a single unit link per plane must pick the neighbour named in cgsolver.py:59-60
+    x = np.arange(20.0).reshape(4, 5)
+    for k, (di, dj) in zip(range(1, 5), ((-1, 0), (1, 0), (0, 1), (0, -1))):
+        c = np.zeros((5, 4, 5))
+        c[k, 2, 2] = 1.0
+        assert krylov.stencil(c, x)[2, 2] == x[2 + di, 2 + dj]
+
+
+def test_bicgstab_matches_lu_random():
+    c, b = random_5pt(24, 18, seed=3)
+    x, info = krylov.bicgstab(c, b, eps=1e-12)
+    assert info["status"] == krylov.OK
+    xd = krylov.direct_solve(c, b)
+    np.testing.assert_allclose(x, xd, rtol=0, atol=1e-9 * np.abs(xd).max())
+
+
+def test_bicgstab_matches_lu_channel_pcor():
+    coef, b = p_system(cases.c1(n_outer=1), steps=1)
+    x, info = krylov.bicgstab(coef, b, eps=1e-12)
+    xd = krylov.direct_solve(coef, b)
+    assert info["status"] == krylov.OK and info["rel_resid_true"] < 1e-10
+    assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
+
+
+def _textbook_bicgstab(A, b, n_iter):
+    """Dense textbook BiCGSTAB (van der Vorst 1992), x0 = 0, r~ = b."""
+    x = np.zeros_like(b); r = b.copy(); rt = b.copy()
+    p = np.zeros_like(b); v = np.zeros_like(b)
+    rho_old = alpha = omega = 1.0
+    seq = []
+    for _ in range(n_iter):
+        rho = rt @ r
+        beta = (rho / rho_old) * (alpha / omega)
+        p = r + beta * (p - omega * v)
+        v = A @ p
+        alpha = rho / (rt @ v)
+        s = r - alpha * v
+        t = A @ s
+        omega = (t @ s) / (t @ t)
+        x = x + alpha * p + omega * s
+        r = s - omega * t
+        rho_old = rho
+        seq.append((rho, alpha, omega))
+    return x, seq
+
+
+def test_bicgstab_scalar_sequence_vs_textbook():
+    c, b = random_5pt(8, 8, seed=5)
+    A = krylov.to_csr(c).toarray()
+    x_ref, seq = _textbook_bicgstab(A, b.ravel(), 6)
+    x, info = krylov.bicgstab(c, b, eps=0.0, max_iter=6)
+    assert info["iterations"] == 6
+    for (rho, al, om), r2, a2, o2 in zip(seq, info["rho"], info["alpha"], info["omega"]):
+        assert math.isclose(rho, r2, rel_tol=1e-12)
+        assert math.isclose(al, a2, rel_tol=1e-12)
+        assert math.isclose(om, o2, rel_tol=1e-12)
+    np.testing.assert_allclose(x.ravel(), x_ref, rtol=1e-11, atol=1e-14)
+
+
+def test_bicgstab_iterations_near_scipy():
+    c, b = random_5pt(32, 32, seed=7, shift=0.01)
+    A = krylov.to_csr(c)
+    live = np.flatnonzero(np.abs(A).sum(axis=1).A.ravel() != 0)
+    sub = A[live][:, live]
+    it_sp = [0]
+
+    def cb(_):
+        it_sp[0] += 1
+    spla.bicgstab(sub, b.ravel()[live], rtol=1e-8, atol=0.0, callback=cb, maxiter=5000)
+    _, info = krylov.bicgstab(c, b, eps=1e-8)
+    assert info["status"] == krylov.OK
+    assert 0.5 * it_sp[0] <= info["iterations"] <= 2.0 * it_sp[0] + 5
+
+
+def test_bicgstab_zero_rhs_and_failures():
+    c, b = random_5pt(6, 6, seed=2)
+    x, info = krylov.bicgstab(c, np.zeros_like(b))
+    assert info["status"] == krylov.OK and info["iterations"] == 0 and not x.any()
+    bn = b.copy(); bn[2, 2] = np.nan
+    _, info = krylov.bicgstab(c, bn)
+    assert info["status"] == krylov.NONFINITE
+    _, info = krylov.bicgstab(c, b, eps=1e-30, max_iter=1)
+    assert info["status"] == krylov.MAXITER and info["iterations"] == 1
+
+
+def test_bicgstab_rho_breakdown():
+    # r~ = b and A b orthogonal to b -> <r~, v> = 0 at the first step
+    c = np.zeros((5, 3, 4))
+    c[2, 1, 1] = 1.0     # row (1,1) picks x(2,1)
+    c[1, 2, 1] = -1.0    # row (2,1) picks -x(1,1)
+    b = np.zeros((3, 4)); b[1, 1] = 1.0; b[2, 1] = 1.0
+    _, info = krylov.bicgstab(c, b)
+    assert info["status"] == krylov.BREAKDOWN
+
+
+def test_cg_matches_lu_on_symmetric_pcor():
+    coef, b = p_system(cases.c1(n_outer=1), steps=1)
+    x, info = krylov.cg(coef, b, eps=1e-12)
+    xd = krylov.direct_solve(coef, b)
+    assert info["status"] == krylov.OK
+    assert np.linalg.norm(x - xd) <= 1e-7 * np.linalg.norm(xd)
+
+
+def test_flexible_identity_equals_plain_and_fallback():
+    c, b = random_5pt(16, 12, seed=4)
+    x1, i1 = krylov.bicgstab(c, b, eps=1e-10)
+    x2, i2 = krylov.fbicgstab(c, b, eps=1e-10)
+    assert i2["status"] == krylov.OK and not i2["fallback"]
+    np.testing.assert_allclose(x2, krylov.direct_solve(c, b), atol=1e-8 * np.abs(x1).max())
+    # a useless preconditioner (returns zero) stalls -> plain fallback still converges
+    x3, i3 = krylov.fbicgstab(c, b, eps=1e-10, precond=lambda z: np.zeros_like(z), stall_window=8)
+    assert i3["fallback"] and i3["rel_resid_true"] < 1e-10
+    # a NaN-producing preconditioner is caught as well
+    x4, i4 = krylov.fbicgstab(c, b, eps=1e-10, precond=lambda z: z * np.nan, stall_window=8)
+    assert i4["fallback"] and i4["rel_resid_true"] < 1e-10
+
+
+def test_jacobi_preconditioner_cuts_iterations():
+    coef, b = p_system(cases.c1(n_outer=1), steps=1)
+    _, plain = krylov.bicgstab(coef, b, eps=1e-8)
+    _, jac = krylov.fbicgstab(coef, b, eps=1e-8, precond=krylov.jacobi_precond(coef), fallback=False)
+    assert jac["status"] == krylov.OK
+    assert jac["iterations"] <= plain["iterations"]
