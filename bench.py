@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark: SIMPLE outer steps/s and p' solve time on one MI355X.
+
+Workload: the 1280x256 square-cylinder channel (Re = 150, dt = 0.0025,
+32x32-cell obstacle, seeded +-1e-3 v perturbation).  A "step" is one SIMPLE
+outer iteration: u/v assembly with boundary folds, two BiCGSTAB momentum
+solves to 1e-8, blend, outlet fix, p' assembly, BiCGSTAB p' solve to 1e-8,
+corrections and the continuity residual.  Every ``n_outer`` steps the time
+level advances (u0 <- u), as in a time-accurate run.
+
+Prints ONE JSON line (rank 0).  Extra keys:
+  roofline      achieved HBM GB/s of the dominant solver kernel vs 8 TB/s,
+                algorithmic bytes / kernel time from HIP events on the
+                solver's stream; traffic from rocprofv3 counters (profiles/)
+  cpu_baseline  the C/OpenMP FP64 restatement with the reference's kernel
+                decomposition (oracle/cpu_simple.c) on this host's cores,
+                timed on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell of the fused BiCGSTAB kernels (FP64)
+BYTES_PER_CELL = {"bicg_a": 11 * 8, "bicg_b": 9 * 8, "bicg_c": 7 * 8}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--case", default="c2")
+    ap.add_argument("--mode", default="plain", choices=["plain", "mixed", "cg"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="outer steps in the CPU baseline sample")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from karman_taichi_amd import _lib, cases
+    from karman_taichi_amd.simple import SIMPLESolver
+
+    case = cases.CASES[args.case]()
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    s = SIMPLESolver.from_case(case, device=dev)
+    s.init()
+    if args.mode == "mixed":
+        s.mixed = True
+    elif args.mode == "cg":
+        s.p_solver = "cg"
+    n_outer = case.n_outer
+    cells_p = case.shape_p[0] * case.shape_p[1]
+
+    step_count = [0]
+
+    def one_step(rec=None):
+        t0 = time.perf_counter()
+        iu, iv = s.solve_momentum_eqn(1)
+        s.correct_conserv()
+        t1 = time.perf_counter()
+        ip = s.solve_pcorrection_eqn()      # synchronous: returns after the solve
+        t2 = time.perf_counter()
+        s._correct()
+        s.compute_mdiv()
+        step_count[0] += 1
+        if step_count[0] % n_outer == 0:
+            s.time_forward()
+        if rec is not None:
+            rec.append({"p_ms": (t2 - t1) * 1e3, "p_it": ip.iterations, "u_it": iu.iterations,
+                        "v_it": iv.iterations, "p_true": ip.rel_resid_true, "p_status": ip.status})
+
+    log(f"bench: case {case.name} mode {args.mode}, warmup {args.warmup}, steps {args.steps}")
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    rec = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        one_step(rec)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    steps_per_s = args.steps / wall
+    p_ms = [r["p_ms"] for r in rec]
+    p_it = [r["p_it"] for r in rec]
+    log(f"bench: {steps_per_s:.3f} steps/s, p' median {statistics.median(p_ms):.2f} ms, "
+        f"p' iters median {statistics.median(p_it)}")
+
+    # ---- roofline: dominant solver kernel, timed with HIP events on the solver stream
+    import ctypes
+    ctx = s.ctx
+    ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    NX, NY = case.shape_p
+    kern = {}
+    for name, kid in (("bicg_a", _lib.KM_KERNEL_BICG_A), ("bicg_b", _lib.KM_KERNEL_BICG_B),
+                      ("bicg_c", _lib.KM_KERNEL_BICG_C)):
+        us = ctypes.c_double()
+        ctx.call("km_time_kernel", kid, NX, NY, _lib.ptr(s._coef_p), _lib.ptr(s.b_p), 200, ctypes.byref(us))
+        kern[name] = us.value
+    dom = max(kern, key=kern.get)      # one launch of each per iteration: the longest dominates
+    alg_bytes = BYTES_PER_CELL[dom] * cells_p
+    achieved = alg_bytes / (kern[dom] * 1e-6) / 1e9
+    traffic = None
+    try:
+        with open(args.traffic_json) as f:
+            tj = json.load(f)
+        traffic = tj.get(args.case, {}).get(dom)
+    except (OSError, ValueError):
+        pass
+    iter_us = sum(kern.values())
+    log(f"bench: kernel us {kern}, dominant {dom}: {achieved:.0f} GB/s")
+
+    # ---- CPU baseline: C/OpenMP restatement with the reference decomposition
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(case, args.cpu_steps)
+
+    total_p_it = sum(p_it)
+    total_p_s = sum(p_ms) / 1e3
+    out = {
+        "metric": "SIMPLE steps/s + p' solve time to 1e-8 rel resid: learned+FP32 vs plain, 1 GPU",
+        "value": round(steps_per_s, 4),
+        "unit": "SIMPLE steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.mode != "mixed" else "f32+f64",
+        "data": "synthetic (seeded channel start state)",
+        "config": {"workload": f"{case.name}: {case.nx}x{case.ny} channel, Re={case.re:g}, dt={case.dt}, "
+                               f"{case.obstacle_cells}x{case.obstacle_cells} cylinder, "
+                               f"{n_outer} outer steps per time step, inner eps 1e-8",
+                   "mode": args.mode, "p_solver": "cg" if args.mode == "cg" else "bicgstab"},
+        "p_solve_ms_median": round(statistics.median(p_ms), 3),
+        "p_iters_median": statistics.median(p_it),
+        "p_rel_resid_true_max": max(r["p_true"] for r in rec),
+        "p_cells_iters_per_s": round(cells_p * total_p_it / total_p_s, 1),
+        "us_per_p_iter": round(total_p_s / total_p_it * 1e6, 3),
+        "kernel_us": {k: round(v, 3) for k, v in kern.items()},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "alg_bytes_per_launch": alg_bytes,
+                     "note": "working set of one p' system is ~34 MB and fits the 256 MB MALL"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(case, n_steps):
+    """Time the C/OpenMP oracle on the same seeded start state: the first
+    ``n_steps`` outer steps (all cores of this process's affinity mask)."""
+    import numpy as np
+    from karman_taichi_amd import cases as kc
+    cores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    from oracle import cpu as ocpu
+    if not ocpu.LIB.exists():
+        ocpu.build()
+    m = kc.make_mask(case)
+    st = ocpu.State(case, *kc.initial_fields(case, m), m)
+    threads = ocpu.threads()
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(n_steps):
+        _, it = st.step()
+        its.append(it)
+    dt = time.perf_counter() - t0
+    cells = case.shape_p[0] * case.shape_p[1]
+    p_it = sum(i[2] for i in its)
+    log(f"cpu baseline: {n_steps} steps in {dt:.1f} s on {threads} threads, iterations {its}")
+    return {"value": round(n_steps / dt, 5), "unit": "SIMPLE steps/s", "cores": threads, "kind": "port",
+            "sample": f"first {n_steps} outer steps of {case.name} from the seeded start "
+                      f"(u/v/p' BiCGSTAB iterations {its}), C/OpenMP FP64, reference kernel decomposition",
+            "p_cells_iters_per_s": round(cells * p_it / dt, 1)}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,6 +1,6 @@
 // Shared definitions of the gfx950 SIMPLE kernels: context, error handling,
-// wave64 reductions, the deterministic last-workgroup reduction and the
-// XCD-aware tile mapping.
+// wave64 reductions, deterministic partial-sum reductions and the XCD-aware
+// tile mapping.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,31 +19,76 @@
 #define KM_NXCD 8           // MI355X: 8 XCDs, workgroups dispatched round-robin
 #define KM_MAX_PARTIALS (1 << 18)
 
-// Device-resident Krylov scalars.  Written only by the last workgroup of a
-// reducing kernel; read by the following kernels and copied to the host when
-// the solver polls for completion.
+// Device-resident Krylov state.  The per-iteration scalars live in a ring of
+// KM_RING slots indexed by the iteration's phase (iteration mod KM_RING, a
+// kernel argument baked into each captured chunk); a kernel reads the slots
+// of earlier kernels and only workgroup 0 writes the slot of the current
+// iteration, so no slot is read and written by the same launch.
+#define KM_RING 4
+struct KScal {
+    double rho, alpha, omega, beta;
+    int32_t iter, pad;   // iter: iterations completed when this phase's last kernel finished
+};
 struct KState {
-    double rho, rho_1, alpha, omega, beta;
-    double r0;       // initial r.r
-    double rTr;      // latest r.r
-    double eps;      // relative tolerance on sqrt(r.r / r0)
-    double aux[8];   // dot results of helper kernels
-    int32_t iter, status, done, max_iter;
-    uint32_t counter[8];   // arrival counters of the last-workgroup reductions
+    double r0;        // stopping-test denominator (r0.r0, or ||b||^2 for a warm start)
+    double eps;       // relative tolerance on sqrt(r.r / r0)
+    double rTr;       // latest r.r
+    double ref;       // > 0: stopping denominator given by the caller
+    int32_t max_iter, done, status, iter;
+    KScal ring[KM_RING];
+    double aux[8];
 };
 
-// Geometry of a 2-D tiled stencil launch: TI rows x TJ columns per workgroup.
+// Reductions: a reducing launch has at most KM_MAX_RED workgroups, each
+// writing its partial sums with plain stores.  The following launch sums the
+// partials itself, in every workgroup and in one fixed order, so all
+// workgroups derive bitwise-identical scalars and only kernel boundaries
+// order memory.  (Detecting the last workgroup with a device-scope atomic
+// costs ~14 us per launch on gfx950, more than the whole stencil sweep of a
+// 1280x256 system.)
+#define KM_MAX_RED 1024
+#define KM_PA 0                          // [1][KM_MAX_RED]
+#define KM_PB (KM_MAX_RED)               // [2][KM_MAX_RED]
+#define KM_PC(par) (3 * KM_MAX_RED + (par) * 2 * KM_MAX_RED)   // [2][KM_MAX_RED], double-buffered
+#define KM_PX (8 * KM_MAX_RED)           // helpers (dot, residual, mdiv): [2][KM_MAX_RED]
+
+// Geometry of a 2-D tiled stencil launch: TI rows x TJ columns per tile,
+// tpb consecutive tiles per workgroup, G workgroups (<= KM_MAX_RED).
 struct Tile {
-    int NX, NY, TI, TJ, tiles_j, G;
+    int NX, NY, TI, TJ, tiles_j, T, tpb, G;
 };
 
 static inline Tile make_tile(int NX, int NY, int TI, int TJ) {
     Tile t;
     t.NX = NX; t.NY = NY; t.TI = TI; t.TJ = TJ;
     t.tiles_j = (NY + TJ - 1) / TJ;
-    int tiles_i = (NX + TI - 1) / TI;
-    t.G = tiles_i * t.tiles_j;
+    const int tiles_i = (NX + TI - 1) / TI;
+    t.T = tiles_i * t.tiles_j;
+    t.tpb = (t.T + KM_MAX_RED - 1) / KM_MAX_RED;
+    t.G = (t.T + t.tpb - 1) / t.tpb;
     return t;
+}
+
+// A flat (pointwise) launch over n elements: G workgroups, each owning a
+// contiguous chunk (a multiple of the workgroup size) of the index range.
+struct Flat {
+    int64_t n, chunk;
+    int G;
+};
+
+static inline Flat make_flat(int64_t n) {
+    Flat f;
+    f.n = n;
+    int64_t g = (n + 255) / 256;
+    if (g > KM_MAX_RED) g = KM_MAX_RED;
+    if (g < 1) g = 1;
+    int64_t chunk = (n + g - 1) / g;
+    chunk = (chunk + 255) / 256 * 256;
+    if (chunk < 256) chunk = 256;
+    f.chunk = chunk;
+    f.G = (int)((n + chunk - 1) / chunk);
+    if (f.G < 1) f.G = 1;
+    return f;
 }
 
 struct GraphKey {
@@ -141,48 +186,36 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
     return r;
 }
 
-__device__ __forceinline__ void store_agent(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_agent(const double* p) {
-    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
-// Publish NV per-workgroup partials and report whether this workgroup arrived
-// last.  partials is laid out [slot][G]; every thread of the workgroup gets
-// the answer.  The arrival counter is reset by the last workgroup.
-//
-// Only agent-scope atomics carry data between workgroups here: the partials
-// are agent-coherent stores and the last workgroup reads them back with
-// agent-coherent loads.  The store therefore only has to be complete before
-// the counter increment, which the explicit s_waitcnt guarantees.  An
-// acquire/release fence at agent scope would be correct too, but on gfx950 it
-// writes back and invalidates the XCD's whole L2 in every workgroup, which
-// cost ~30 us per reduction kernel.
-template <int NV>
-__device__ __forceinline__ bool publish_partials(const double (&vals)[NV], int slot_index, int G,
-                                                 double* partials, uint32_t* counter) {
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) store_agent(&partials[(size_t)q * G + slot_index], vals[q]);
-        __builtin_amdgcn_s_waitcnt(0);   // vmcnt(0) expcnt(0) lgkmcnt(0): the stores above are done
-        uint32_t prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == (uint32_t)(G - 1));
-        if (last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+// Sum of v over the workgroup, in a fixed order, returned to every thread.
+__device__ __forceinline__ double block_sum_all(double v, double* sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     __syncthreads();
-    return last;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    __syncthreads();
+    return r;
 }
 
-// In the last workgroup: deterministic sum of partials[q*G .. q*G+G) for each
-// q (fixed strided order, then the fixed block tree).  Result in thread 0.
+// Totals of the partials P[q * KM_MAX_RED + k], k < G, for q < NV, returned
+// to every thread of the workgroup.  Fixed order: eight strided accumulators
+// per thread (eight loads in flight), a fixed tree, then the workgroup sum.
 template <int NV>
-__device__ __forceinline__ void finish_partials(const double* partials, int G, double (&out)[NV], double* sh) {
+__device__ __forceinline__ void sum_partials(const double* __restrict__ P, int G, double (&out)[NV], double* sh) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-        double a = 0.0;
-        for (int k = threadIdx.x; k < G; k += KM_BLOCK) a += load_agent(&partials[(size_t)q * G + k]);
-        out[q] = block_sum(a, sh);
+        const double* base = P + (size_t)q * KM_MAX_RED;
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = threadIdx.x; k < G; k += 8 * KM_BLOCK) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = (k + u * KM_BLOCK < G) ? base[k + u * KM_BLOCK] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[u] += v[u];
+        }
+        const double a = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        out[q] = block_sum_all(a, sh);
     }
 }

@@ -286,37 +286,47 @@ k_correct(GridD g, double* __restrict__ p, double* __restrict__ u, double* __res
     }
 }
 
-// compute_mdiv (simple.py:137-141) plus max|mdiv| and sum mdiv^2 over the
-// interior (bicgsimp.py:114-122), reduced deterministically into res[0..1].
+// compute_mdiv (simple.py:137-141) with per-workgroup partials of max|mdiv|
+// and sum mdiv^2 over the interior (bicgsimp.py:114-122 returns the max).
 __global__ void __launch_bounds__(KM_BLOCK)
-k_mdiv(GridD g, const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ mdiv,
-       double* __restrict__ partials, uint32_t* __restrict__ counter, double* __restrict__ res) {
-    const int NX = g.nx + 2, NY = g.ny + 2, NYU = g.ny + 2, NYV = g.ny + 3;
-    const int64_t N = (int64_t)NX * NY;
-    const int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x;
-    double md = 0.0;
-    if (k < N) {
+k_mdiv(GridD g, Flat fl, const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ mdiv,
+       double* __restrict__ P) {
+    const int NY = g.ny + 2, NYU = g.ny + 2, NYV = g.ny + 3;
+    __shared__ double sh[4];
+    double mx = 0.0, sq = 0.0;
+    const int64_t k0 = (int64_t)blockIdx.x * fl.chunk;
+    const int64_t k1 = k0 + fl.chunk < fl.n ? k0 + fl.chunk : fl.n;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += KM_BLOCK) {
         const int i = (int)(k / NY), j = (int)(k - (int64_t)i * NY);
+        double md = 0.0;
         if (i >= 1 && i <= g.nx && j >= 1 && j <= g.ny) {
             const int64_t ku = (int64_t)i * NYU + j, kv = (int64_t)i * NYV + j;
             md = g.rho * (u[ku] - u[ku + NYU]) * g.dy + g.rho * (v[kv] - v[kv + 1]) * g.dx;
         }
         mdiv[k] = md;
+        mx = fmax(mx, fabs(md));
+        sq += md * md;
     }
+    const double bmx = block_max(mx, sh);
+    const double bsq = block_sum_all(sq, sh);
+    if (threadIdx.x == 0) {
+        P[blockIdx.x] = bmx;
+        P[KM_MAX_RED + blockIdx.x] = bsq;
+    }
+}
+
+// One workgroup: res[0] = max of the max partials, res[1] = sum of the sums.
+__global__ void __launch_bounds__(KM_BLOCK)
+k_mdiv_finish(const double* __restrict__ P, int G, double* __restrict__ res) {
     __shared__ double sh[4];
-    const int G = gridDim.x;
-    double mx = block_max(fabs(md), sh);
-    double sq = block_sum(md * md, sh);
-    double vals[2] = {mx, sq};
-    if (publish_partials<2>(vals, blockIdx.x, G, partials, counter)) {
-        double a = 0.0, s = 0.0;
-        for (int q = threadIdx.x; q < G; q += KM_BLOCK) {
-            a = fmax(a, load_agent(&partials[q]));
-            s += load_agent(&partials[G + q]);
-        }
-        a = block_max(a, sh);
-        s = block_sum(s, sh);
-        if (threadIdx.x == 0) { res[0] = a; res[1] = s; }
+    double mx = 0.0;
+    for (int k = threadIdx.x; k < G; k += KM_BLOCK) mx = fmax(mx, P[k]);
+    mx = block_max(mx, sh);
+    double s[1];
+    sum_partials<1>(P + KM_MAX_RED, G, s, sh);
+    if (threadIdx.x == 0) {
+        res[0] = mx;
+        res[1] = s[0];
     }
 }
 
@@ -424,14 +434,14 @@ int km_mdiv(km_ctx* ctx, const km_grid* g, const double* u, const double* v, dou
     if (!ctx) return KM_EINVAL;
     if (int rc = check_grid(ctx, g)) return rc;
     KM_REQUIRE(ctx, u && v && mdiv, "km_mdiv: NULL pointer");
-    const int64_t N = (int64_t)(g->nx + 2) * (g->ny + 2);
-    const int G = ceil_div_i64(N, KM_BLOCK);
-    KM_REQUIRE(ctx, 2 * (int64_t)G <= KM_MAX_PARTIALS, "km_mdiv: grid too large for the partial buffer");
+    const Flat fl = make_flat((int64_t)(g->nx + 2) * (g->ny + 2));
     if (int rc = km_ensure_workspace(ctx, 0)) return rc;
-    hipLaunchKernelGGL(k_mdiv, dim3(G), dim3(KM_BLOCK), 0, ctx->stream, to_dev(g), u, v, mdiv, ctx->partials,
-                       &ctx->st[1].counter[0], ctx->dres);
+    double* P = ctx->partials + KM_PX;
+    hipLaunchKernelGGL(k_mdiv, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, to_dev(g), fl, u, v, mdiv, P);
     KM_CHECK_LAUNCH(ctx);
     if (stats) {
+        hipLaunchKernelGGL(k_mdiv_finish, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, fl.G, ctx->dres);
+        KM_CHECK_LAUNCH(ctx);
         KM_HIP(ctx, hipMemcpyAsync(ctx->h_res, ctx->dres, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
         stats[0] = ctx->h_res[0];

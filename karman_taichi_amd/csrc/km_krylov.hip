@@ -7,15 +7,16 @@
 //   B  s  = r - alpha v'           on an LDS tile + halo, t  = A s,   <t, s>, <t, t>
 //   C  x += alpha p' + omega s,  r = s - omega t,                     <r, r>, <r, r~>
 //
-// Each reduction is finished by the last workgroup to arrive, which sums the
-// per-workgroup partials in a fixed order (bitwise reproducible) and updates
-// the FP64 scalars (rho, alpha, omega, beta), the convergence test and the
-// iteration counter in device memory.  Kernels return immediately once the
-// device-side ``done`` flag is set, so the host enqueues fixed-size chunks of
-// iterations (replayed from a captured hipGraph) and polls the flag between
-// chunks instead of after every reduction.  p and v are double-buffered by
-// iteration parity because kernel A reads the old values in its halo while
-// writing the new ones.
+// Each kernel leaves per-workgroup partial sums; the next kernel reduces them
+// in every workgroup (same order everywhere, so every workgroup derives the
+// same FP64 scalar) and workgroup 0 records the scalars, the stopping test and
+// the iteration count in device memory.  Kernel A of iteration i judges
+// iteration i-1 (the reference's test right after update_r).  Kernels return
+// at once when the device-side ``done`` flag is set, so the host enqueues
+// fixed-size chunks of iterations (replayed from a captured hipGraph) and
+// polls between chunks instead of after every reduction.  p and v are
+// double-buffered by iteration parity because kernel A reads old values in
+// its halo while writing new ones.
 //
 // T = double (plain FP64 path) or float (inner solves of the mixed-precision
 // path); dot products and scalars are always FP64.
@@ -23,17 +24,14 @@
 
 extern __shared__ __align__(16) unsigned char km_smem[];
 
-void* km_ws_vectors(km_ctx* ctx);
-
 namespace {
 
 struct TileOrigin {
     int i0, j0;
 };
 
-__device__ __forceinline__ TileOrigin tile_origin(const Tile& tl) {
-    const int L = xcd_remap(blockIdx.x, tl.G);
-    const int ti = L / tl.tiles_j, tj = L - ti * tl.tiles_j;
+__device__ __forceinline__ TileOrigin tile_origin(const Tile& tl, int tile) {
+    const int ti = tile / tl.tiles_j, tj = tile - ti * tl.tiles_j;
     return {ti * tl.TI, tj * tl.TJ};
 }
 
@@ -77,10 +75,44 @@ __device__ __forceinline__ void apply(const Tile& tl, const TileOrigin& o, const
     }
 }
 
+// Run stage + apply over this workgroup's tiles (XCD-aware: each XCD owns a
+// contiguous range of tiles, so neighbouring rows share its L2).
+template <typename T, typename F, typename G>
+__device__ __forceinline__ void tile_sweep(const Tile& tl, T* lds, F f, T* center, const T* __restrict__ coef, G g) {
+    const int L = xcd_remap(blockIdx.x, tl.G);
+    const int t0 = L * tl.tpb;
+    const int t1 = t0 + tl.tpb < tl.T ? t0 + tl.tpb : tl.T;
+    for (int tile = t0; tile < t1; ++tile) {
+        const TileOrigin o = tile_origin(tl, tile);
+        stage<T>(tl, o, lds, f, center);
+        __syncthreads();
+        apply<T>(tl, o, lds, coef, g);
+        __syncthreads();
+    }
+}
+
+template <typename F>
+__device__ __forceinline__ void flat_sweep(const Flat& fl, F f) {
+    const int64_t k0 = (int64_t)blockIdx.x * fl.chunk;
+    const int64_t k1 = k0 + fl.chunk < fl.n ? k0 + fl.chunk : fl.n;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += KM_BLOCK) f(k);
+}
+
+// Partials of the workgroup that runs this code: slot = xcd-remapped index for
+// tiled launches, blockIdx for flat ones.
+__device__ __forceinline__ void write_partial(double* P, int slot, double v, double* sh) {
+    const double s = block_sum_all(v, sh);
+    if (threadIdx.x == 0) P[slot] = s;
+}
+
 __device__ __forceinline__ bool finite(double x) { return x == x && x - x == 0.0; }
 
-__device__ __forceinline__ void set_done(KState* st, int status) {
+__device__ __forceinline__ bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+__device__ __forceinline__ void finish(KState* st, int status, int iter, double rTr) {
     st->status = status;
+    st->iter = iter;
+    st->rTr = rTr;
     st->done = 1;
 }
 
@@ -90,50 +122,42 @@ template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_stencil(Tile tl, const T* __restrict__ coef, const T* __restrict__ x, T* __restrict__ y) {
     T* lds = reinterpret_cast<T*>(km_smem);
-    const TileOrigin o = tile_origin(tl);
-    stage<T>(tl, o, lds, [&](int64_t k) { return x[k]; }, (T*)nullptr);
-    __syncthreads();
-    apply<T>(tl, o, lds, coef, [&](int64_t k, T v, T) { y[k] = v; });
+    tile_sweep<T>(tl, lds, [&](int64_t k) { return x[k]; }, (T*)nullptr, coef, [&](int64_t k, T v, T) { y[k] = v; });
 }
 
-// r = b - A x (FP64) and ||r||^2 into res[0]
+// r = b - A x (FP64) with per-workgroup partials of ||r||^2
 __global__ void __launch_bounds__(KM_BLOCK)
 k_residual(Tile tl, const double* __restrict__ coef, const double* __restrict__ b, const double* __restrict__ x,
-           double* __restrict__ r, double* __restrict__ partials, uint32_t* __restrict__ counter,
-           double* __restrict__ res) {
+           double* __restrict__ r, double* __restrict__ P) {
     double* lds = reinterpret_cast<double*>(km_smem);
     __shared__ double sh[4];
-    const TileOrigin o = tile_origin(tl);
-    stage<double>(tl, o, lds, [&](int64_t k) { return x[k]; }, (double*)nullptr);
-    __syncthreads();
     double acc = 0.0;
-    apply<double>(tl, o, lds, coef, [&](int64_t k, double y, double) {
-        const double rr = b[k] - y;
-        if (r) r[k] = rr;
-        acc += rr * rr;
-    });
-    double vals[1] = {block_sum(acc, sh)};
-    if (publish_partials<1>(vals, blockIdx.x, tl.G, partials, counter)) {
-        double out[1];
-        finish_partials<1>(partials, tl.G, out, sh);
-        if (threadIdx.x == 0) res[0] = out[0];
-    }
+    tile_sweep<double>(tl, lds, [&](int64_t k) { return x[k]; }, (double*)nullptr, coef,
+                       [&](int64_t k, double y, double) {
+                           const double rr = b[k] - y;
+                           if (r) r[k] = rr;
+                           acc += rr * rr;
+                       });
+    write_partial(P, xcd_remap(blockIdx.x, tl.G), acc, sh);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_dot(int64_t n, const T* __restrict__ a, const T* __restrict__ b, double* __restrict__ partials,
-      uint32_t* __restrict__ counter, double* __restrict__ res) {
+k_dot(Flat fl, const T* __restrict__ a, const T* __restrict__ b, double* __restrict__ P) {
     __shared__ double sh[4];
     double acc = 0.0;
-    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK)
-        acc += (double)a[k] * (double)b[k];
-    double vals[1] = {block_sum(acc, sh)};
-    if (publish_partials<1>(vals, blockIdx.x, gridDim.x, partials, counter)) {
-        double out[1];
-        finish_partials<1>(partials, gridDim.x, out, sh);
-        if (threadIdx.x == 0) res[0] = out[0];
-    }
+    flat_sweep(fl, [&](int64_t k) { acc += (double)a[k] * (double)b[k]; });
+    write_partial(P, blockIdx.x, acc, sh);
+}
+
+// One workgroup: out[q] = sum of P[q * KM_MAX_RED + k], k < G.
+template <int NV>
+__global__ void __launch_bounds__(KM_BLOCK) k_finish(const double* __restrict__ P, int G, double* __restrict__ out) {
+    __shared__ double sh[4];
+    double s[NV];
+    sum_partials<NV>(P, G, s, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < NV; ++q) out[q] = s[q];
 }
 
 __global__ void __launch_bounds__(KM_BLOCK)
@@ -151,15 +175,16 @@ k_add_scaled(int64_t n, double* __restrict__ x, const float* __restrict__ d, dou
 // ------------------------------------------------------------ BiCGSTAB
 
 // bicgsolver.py:21-35: x = 0, r = r~ = b (or r precomputed from x0), the
-// parity-0 p and v buffers zeroed, rho_1 = alpha = omega = 1; r0 = <r, r>.
+// parity-0 p and v buffers zeroed; partials of <r, r> (twice: it is also
+// rho of iteration 0, <r, r~> with r~ = r) for kernel A of iteration 0.
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_bicg_init(int64_t n, const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r, T* __restrict__ rt,
+k_bicg_init(Flat fl, const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r, T* __restrict__ rt,
             T* __restrict__ p0, T* __restrict__ v0, int from_r, double eps, int max_iter, double ref_norm2,
-            KState* st, double* __restrict__ partials) {
+            KState* st, double* __restrict__ P) {
     __shared__ double sh[4];
     double acc = 0.0;
-    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK) {
+    flat_sweep(fl, [&](int64_t k) {
         T rk;
         if (from_r) {
             rk = r[k];
@@ -172,255 +197,258 @@ k_bicg_init(int64_t n, const T* __restrict__ b, T* __restrict__ x, T* __restrict
         p0[k] = T(0);
         v0[k] = T(0);
         acc += (double)rk * (double)rk;
+    });
+    const double s = block_sum_all(acc, sh);
+    if (threadIdx.x == 0) {
+        P[KM_PC(1) + blockIdx.x] = s;
+        P[KM_PC(1) + KM_MAX_RED + blockIdx.x] = s;
     }
-    double vals[1] = {block_sum(acc, sh)};
-    if (publish_partials<1>(vals, blockIdx.x, gridDim.x, partials, &st->counter[0])) {
-        double out[1];
-        finish_partials<1>(partials, gridDim.x, out, sh);
-        if (threadIdx.x == 0) {
-            const double r0 = out[0];
-            // stopping test denominator: r0.r0 (bicgsolver.py:90), or ||b||^2
-            // for a warm start so that x0 is not credited through it
-            st->r0 = ref_norm2 > 0.0 ? ref_norm2 : r0;
-            st->rTr = r0;
-            st->rho = r0;          // rho of iteration 0 is <r, r~> = <r, r>
-            st->rho_1 = 1.0;
-            st->alpha = 1.0;
-            st->omega = 1.0;
-            st->beta = 0.0;        // p' = r + 0 (0 - omega 0) = r on iteration 0
-            st->eps = eps;
-            st->max_iter = max_iter;
-            st->iter = 0;
-            st->status = KM_MAXITER;
-            st->done = 0;
-            if (!finite(r0)) set_done(st, KM_NONFINITE);
-            else if (r0 == 0.0) set_done(st, KM_OK);
-            else if (max_iter <= 0) set_done(st, KM_MAXITER);
-        }
+    if (lead()) {
+        st->eps = eps;
+        st->max_iter = max_iter;
+        st->ref = ref_norm2;
+        st->done = 0;
+        st->status = KM_MAXITER;
+        st->iter = 0;
+        st->ring[KM_RING - 1].iter = 0;
     }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_bicg_a(Tile tl, const T* __restrict__ coef, const T* __restrict__ r, const T* __restrict__ rt, T* pbuf, T* vbuf,
-         KState* st, double* __restrict__ partials) {
+k_bicg_a(Tile tl, Flat fc, int phase, const T* __restrict__ coef, const T* __restrict__ r, const T* __restrict__ rt,
+         T* pbuf, T* vbuf, KState* st, double* __restrict__ P) {
     if (st->done) return;
     T* lds = reinterpret_cast<T*>(km_smem);
     __shared__ double sh[4];
+    // <r, r> and <r, r~> left by kernel C of the previous iteration (or init)
+    double prev[2];
+    sum_partials<2>(P + KM_PC((phase + 1) & 1), fc.G, prev, sh);
+    const double rTr = prev[0], rho = prev[1];
+    const KScal sp = st->ring[(phase + KM_RING - 1) % KM_RING];
+    const int i = sp.iter;
+    double r0 = 0.0;
+    int status = -1;
+    if (i == 0) {
+        r0 = st->ref > 0.0 ? st->ref : rTr;
+        if (!finite(rTr)) status = KM_NONFINITE;
+        else if (rTr == 0.0) status = KM_OK;
+        else if (st->max_iter <= 0) status = KM_MAXITER;
+    } else {
+        r0 = st->r0;
+        if (!finite(rTr)) status = KM_NONFINITE;
+        else if (sqrt(rTr / r0) < st->eps) status = KM_OK;            // bicgsolver.py:130
+        else if (sp.omega == 0.0) status = KM_BREAKDOWN;               // karman_taichi.py:451
+        else if (i >= st->max_iter) status = KM_MAXITER;
+        else if (!finite(rho)) status = KM_NONFINITE;
+        else if (rho == 0.0) status = KM_BREAKDOWN;                    // bicgsolver.py:96-98
+    }
+    if (status >= 0) {
+        if (lead()) {
+            if (i == 0) st->r0 = r0;
+            finish(st, status, i, rTr);
+        }
+        return;
+    }
+    // bicgsolver.py:102 (beta = 0 on iteration 0: p = r + 0 (0 - omega 0) = r)
+    const double beta = i == 0 ? 0.0 : (rho / sp.rho) * (sp.alpha / sp.omega);
+    const double omega = i == 0 ? 1.0 : sp.omega;
+    if (lead()) {
+        if (i == 0) st->r0 = r0;
+        st->ring[phase].rho = rho;
+        st->ring[phase].beta = beta;
+        st->rTr = rTr;
+        st->iter = i;
+    }
     const int64_t N = (int64_t)tl.NX * tl.NY;
-    const int par = st->iter & 1;
+    const int par = phase & 1;
     const T* __restrict__ p_old = pbuf + par * N;
     const T* __restrict__ v_old = vbuf + par * N;
     T* __restrict__ p_new = pbuf + (1 - par) * N;
     T* __restrict__ v_new = vbuf + (1 - par) * N;
-    const T beta = (T)st->beta, omega = (T)st->omega;
-    const TileOrigin o = tile_origin(tl);
-    // update_p (bicgsolver.py:43-46): p = r + beta (p - omega v)
-    stage<T>(tl, o, lds, [&](int64_t k) { return r[k] + beta * (p_old[k] - omega * v_old[k]); }, p_new);
-    __syncthreads();
+    const T bt = (T)beta, om = (T)omega;
     double acc = 0.0;
-    // compute_Ap (bicgsolver.py:72-76) and reduce(r~, v) (bicgsolver.py:107)
-    apply<T>(tl, o, lds, coef, [&](int64_t k, T y, T) {
-        v_new[k] = y;
-        acc += (double)rt[k] * (double)y;
-    });
-    double vals[1] = {block_sum(acc, sh)};
-    if (publish_partials<1>(vals, blockIdx.x, tl.G, partials, &st->counter[1])) {
-        double out[1];
-        finish_partials<1>(partials, tl.G, out, sh);
-        if (threadIdx.x == 0) {
-            const double rv = out[0];
-            if (!finite(rv)) set_done(st, KM_NONFINITE);
-            else if (rv == 0.0) set_done(st, KM_BREAKDOWN);
-            else st->alpha = st->rho / rv;   // bicgsolver.py:108
-        }
-    }
+    // update_p (bicgsolver.py:43-46), compute_Ap (:72-76), reduce(r~, v) (:107)
+    tile_sweep<T>(tl, lds, [&](int64_t k) { return r[k] + bt * (p_old[k] - om * v_old[k]); }, p_new, coef,
+                  [&](int64_t k, T y, T) {
+                      v_new[k] = y;
+                      acc += (double)rt[k] * (double)y;
+                  });
+    write_partial(P + KM_PA, xcd_remap(blockIdx.x, tl.G), acc, sh);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_bicg_b(Tile tl, const T* __restrict__ coef, const T* __restrict__ r, const T* vbuf, T* __restrict__ s,
-         T* __restrict__ t, KState* st, double* __restrict__ partials) {
+k_bicg_b(Tile tl, int phase, const T* __restrict__ coef, const T* __restrict__ r, const T* vbuf, T* __restrict__ s,
+         T* __restrict__ t, KState* st, double* __restrict__ P) {
     if (st->done) return;
     T* lds = reinterpret_cast<T*>(km_smem);
     __shared__ double sh[4];
+    double rv[1];
+    sum_partials<1>(P + KM_PA, tl.G, rv, sh);
+    const int i = st->ring[(phase + KM_RING - 1) % KM_RING].iter;
+    if (!finite(rv[0]) || rv[0] == 0.0) {
+        if (lead()) finish(st, rv[0] == 0.0 ? KM_BREAKDOWN : KM_NONFINITE, i, st->rTr);
+        return;
+    }
+    const double alpha = st->ring[phase].rho / rv[0];   // bicgsolver.py:108
+    if (lead()) st->ring[phase].alpha = alpha;
     const int64_t N = (int64_t)tl.NX * tl.NY;
-    const int par = st->iter & 1;
-    const T* __restrict__ v = vbuf + (1 - par) * N;
-    const T alpha = (T)st->alpha;
-    const TileOrigin o = tile_origin(tl);
-    // update_s (bicgsolver.py:60-63): s = r - alpha v
-    stage<T>(tl, o, lds, [&](int64_t k) { return r[k] - alpha * v[k]; }, s);
-    __syncthreads();
+    const T* __restrict__ v = vbuf + (1 - (phase & 1)) * N;
+    const T al = (T)alpha;
     double ts = 0.0, tt = 0.0;
-    // compute_As (bicgsolver.py:65-69), reduce(t, s), reduce(t, t) (:116-117)
-    apply<T>(tl, o, lds, coef, [&](int64_t k, T y, T sk) {
+    // update_s (bicgsolver.py:60-63), compute_As (:65-69), reduce(t, s), reduce(t, t) (:116-117)
+    tile_sweep<T>(tl, lds, [&](int64_t k) { return r[k] - al * v[k]; }, s, coef, [&](int64_t k, T y, T sk) {
         t[k] = y;
         ts += (double)y * (double)sk;
         tt += (double)y * (double)y;
     });
-    double vals[2] = {block_sum(ts, sh), block_sum(tt, sh)};
-    if (publish_partials<2>(vals, blockIdx.x, tl.G, partials, &st->counter[2])) {
-        double out[2];
-        finish_partials<2>(partials, tl.G, out, sh);
-        if (threadIdx.x == 0) {
-            if (!finite(out[0]) || !finite(out[1])) set_done(st, KM_NONFINITE);
-            else st->omega = out[1] != 0.0 ? out[0] / out[1] : 0.0;   // bicgsolver.py:118
-        }
-    }
+    const int slot = xcd_remap(blockIdx.x, tl.G);
+    write_partial(P + KM_PB, slot, ts, sh);
+    write_partial(P + KM_PB + KM_MAX_RED, slot, tt, sh);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_bicg_c(int64_t n, T* __restrict__ x, const T* pbuf, const T* __restrict__ s, const T* __restrict__ t,
-         T* __restrict__ r, const T* __restrict__ rt, KState* st, double* __restrict__ partials) {
+k_bicg_c(Flat fl, int Gb, int phase, T* __restrict__ x, const T* pbuf, const T* __restrict__ s,
+         const T* __restrict__ t, T* __restrict__ r, const T* __restrict__ rt, KState* st, double* __restrict__ P) {
     if (st->done) return;
     __shared__ double sh[4];
-    const int par = st->iter & 1;
-    const T* __restrict__ p = pbuf + (1 - par) * n;
-    const T alpha = (T)st->alpha, omega = (T)st->omega;
+    double tsv[2];
+    sum_partials<2>(P + KM_PB, Gb, tsv, sh);
+    const int i = st->ring[(phase + KM_RING - 1) % KM_RING].iter;
+    if (!finite(tsv[0]) || !finite(tsv[1])) {
+        if (lead()) finish(st, KM_NONFINITE, i, st->rTr);
+        return;
+    }
+    const double omega = tsv[1] != 0.0 ? tsv[0] / tsv[1] : 0.0;   // bicgsolver.py:118
+    const double alpha = st->ring[phase].alpha;
+    if (lead()) {
+        st->ring[phase].omega = omega;
+        st->ring[phase].iter = i + 1;
+    }
+    const T* __restrict__ p = pbuf + (1 - (phase & 1)) * fl.n;
+    const T al = (T)alpha, om = (T)omega;
     double rr = 0.0, rrt = 0.0;
-    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK) {
+    flat_sweep(fl, [&](int64_t k) {
         // update_x, update_r (bicgsolver.py:78-86)
-        x[k] = x[k] + (alpha * p[k] + omega * s[k]);
-        const T rk = s[k] - omega * t[k];
+        x[k] = x[k] + (al * p[k] + om * s[k]);
+        const T rk = s[k] - om * t[k];
         r[k] = rk;
         rr += (double)rk * (double)rk;
         rrt += (double)rk * (double)rt[k];
-    }
-    double vals[2] = {block_sum(rr, sh), block_sum(rrt, sh)};
-    if (publish_partials<2>(vals, blockIdx.x, gridDim.x, partials, &st->counter[3])) {
-        double out[2];
-        finish_partials<2>(partials, gridDim.x, out, sh);
-        if (threadIdx.x == 0) {
-            const double rTr = out[0], rho_new = out[1];
-            st->rTr = rTr;
-            st->iter = st->iter + 1;
-            if (!finite(rTr)) set_done(st, KM_NONFINITE);
-            else if (sqrt(rTr / st->r0) < st->eps) set_done(st, KM_OK);       // bicgsolver.py:130
-            else if (st->omega == 0.0) set_done(st, KM_BREAKDOWN);           // karman_taichi.py:451
-            else if (st->iter >= st->max_iter) set_done(st, KM_MAXITER);
-            else {
-                st->rho_1 = st->rho;                                           // bicgsolver.py:135
-                st->rho = rho_new;                                             // bicgsolver.py:95
-                if (!finite(rho_new)) set_done(st, KM_NONFINITE);
-                else if (rho_new == 0.0) set_done(st, KM_BREAKDOWN);           // bicgsolver.py:96-98
-                else st->beta = (rho_new / st->rho_1) * (st->alpha / st->omega);   // bicgsolver.py:102
-            }
-        }
-    }
+    });
+    write_partial(P + KM_PC(phase & 1), blockIdx.x, rr, sh);
+    write_partial(P + KM_PC(phase & 1) + KM_MAX_RED, blockIdx.x, rrt, sh);
 }
 
 // ------------------------------------------------------------ CG (cgsolver.py:83-114)
 
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_cg_init(int64_t n, const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r, T* __restrict__ p0,
-          double eps, int max_iter, KState* st, double* __restrict__ partials) {
+k_cg_init(Flat fl, const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r, T* __restrict__ p0, double eps,
+          int max_iter, KState* st, double* __restrict__ P) {
     __shared__ double sh[4];
     double acc = 0.0;
-    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK) {
+    flat_sweep(fl, [&](int64_t k) {
         const T rk = b[k];
         r[k] = rk;
         x[k] = T(0);
         p0[k] = T(0);
         acc += (double)rk * (double)rk;
-    }
-    double vals[1] = {block_sum(acc, sh)};
-    if (publish_partials<1>(vals, blockIdx.x, gridDim.x, partials, &st->counter[0])) {
-        double out[1];
-        finish_partials<1>(partials, gridDim.x, out, sh);
-        if (threadIdx.x == 0) {
-            const double r0 = out[0];
-            st->r0 = r0;
-            st->rTr = r0;
-            st->rho = r0;     // old_rTr
-            st->beta = 0.0;   // p = r + 0 p (cgsolver.py:228)
-            st->alpha = 0.0;
-            st->eps = eps;
-            st->max_iter = max_iter;
-            st->iter = 0;
-            st->status = KM_MAXITER;
-            st->done = 0;
-            if (!finite(r0)) set_done(st, KM_NONFINITE);
-            else if (r0 == 0.0) set_done(st, KM_OK);
-            else if (max_iter <= 0) set_done(st, KM_MAXITER);
-        }
+    });
+    write_partial(P + KM_PC(1), blockIdx.x, acc, sh);
+    if (lead()) {
+        st->eps = eps;
+        st->max_iter = max_iter;
+        st->ref = 0.0;
+        st->done = 0;
+        st->status = KM_MAXITER;
+        st->iter = 0;
+        st->ring[KM_RING - 1].iter = 0;
     }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_cg_a(Tile tl, const T* __restrict__ coef, const T* __restrict__ r, T* pbuf, T* __restrict__ Ap, KState* st,
-       double* __restrict__ partials) {
+k_cg_a(Tile tl, Flat fc, int phase, const T* __restrict__ coef, const T* __restrict__ r, T* pbuf, T* __restrict__ Ap,
+       KState* st, double* __restrict__ P) {
     if (st->done) return;
     T* lds = reinterpret_cast<T*>(km_smem);
     __shared__ double sh[4];
+    double nw[1];
+    sum_partials<1>(P + KM_PC((phase + 1) & 1), fc.G, nw, sh);
+    const double rTr = nw[0];
+    const KScal sp = st->ring[(phase + KM_RING - 1) % KM_RING];
+    const int i = sp.iter;
+    int status = -1;
+    const double r0 = i == 0 ? rTr : st->r0;
+    if (!finite(rTr)) status = KM_NONFINITE;
+    else if (i == 0 && rTr == 0.0) status = KM_OK;
+    else if (i > 0 && sqrt(rTr) < sqrt(r0) * st->eps) status = KM_OK;   // cgsolver.py:240
+    else if (i >= st->max_iter) status = KM_MAXITER;
+    if (status >= 0) {
+        if (lead()) {
+            if (i == 0) st->r0 = r0;
+            finish(st, status, i, rTr);
+        }
+        return;
+    }
+    const double beta = i == 0 ? 0.0 : rTr / sp.rho;   // cgsolver.py:245
+    if (lead()) {
+        if (i == 0) st->r0 = r0;
+        st->ring[phase].rho = rTr;   // old_rTr of this iteration
+        st->ring[phase].beta = beta;
+        st->rTr = rTr;
+        st->iter = i;
+    }
     const int64_t N = (int64_t)tl.NX * tl.NY;
-    const int par = st->iter & 1;
+    const int par = phase & 1;
     const T* __restrict__ p_old = pbuf + par * N;
     T* __restrict__ p_new = pbuf + (1 - par) * N;
-    const T beta = (T)st->beta;
-    const TileOrigin o = tile_origin(tl);
-    // update_p (cgsolver.py:216-219): p = r + beta p
-    stage<T>(tl, o, lds, [&](int64_t k) { return r[k] + beta * p_old[k]; }, p_new);
-    __syncthreads();
+    const T bt = (T)beta;
     double acc = 0.0;
-    apply<T>(tl, o, lds, coef, [&](int64_t k, T y, T pk) {
+    // update_p (cgsolver.py:216-219), compute_Ap (:194-198), reduce(p, Ap) (:233)
+    tile_sweep<T>(tl, lds, [&](int64_t k) { return r[k] + bt * p_old[k]; }, p_new, coef, [&](int64_t k, T y, T pk) {
         Ap[k] = y;
         acc += (double)pk * (double)y;
     });
-    double vals[1] = {block_sum(acc, sh)};
-    if (publish_partials<1>(vals, blockIdx.x, tl.G, partials, &st->counter[1])) {
-        double out[1];
-        finish_partials<1>(partials, tl.G, out, sh);
-        if (threadIdx.x == 0) {
-            const double pAp = out[0];
-            if (!finite(pAp)) set_done(st, KM_NONFINITE);
-            else if (pAp == 0.0) set_done(st, KM_BREAKDOWN);
-            else st->alpha = st->rho / pAp;   // cgsolver.py:233-234
-        }
-    }
+    write_partial(P + KM_PA, xcd_remap(blockIdx.x, tl.G), acc, sh);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_cg_b(int64_t n, T* __restrict__ x, T* __restrict__ r, const T* pbuf, const T* __restrict__ Ap, KState* st,
-       double* __restrict__ partials) {
+k_cg_b(Flat fl, int Ga, int phase, T* __restrict__ x, T* __restrict__ r, const T* pbuf, const T* __restrict__ Ap,
+       KState* st, double* __restrict__ P) {
     if (st->done) return;
     __shared__ double sh[4];
-    const int par = st->iter & 1;
-    const T* __restrict__ p = pbuf + (1 - par) * n;
-    const T alpha = (T)st->alpha;
+    double pap[1];
+    sum_partials<1>(P + KM_PA, Ga, pap, sh);
+    const int i = st->ring[(phase + KM_RING - 1) % KM_RING].iter;
+    if (!finite(pap[0]) || pap[0] == 0.0) {
+        if (lead()) finish(st, pap[0] == 0.0 ? KM_BREAKDOWN : KM_NONFINITE, i, st->rTr);
+        return;
+    }
+    const double alpha = st->ring[phase].rho / pap[0];   // cgsolver.py:234
+    if (lead()) {
+        st->ring[phase].alpha = alpha;
+        st->ring[phase].iter = i + 1;
+    }
+    const T* __restrict__ p = pbuf + (1 - (phase & 1)) * fl.n;
+    const T al = (T)alpha;
     double acc = 0.0;
-    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK) {
-        x[k] = x[k] + alpha * p[k];          // cgsolver.py:206-209
-        const T rk = r[k] - alpha * Ap[k];   // cgsolver.py:211-214
+    flat_sweep(fl, [&](int64_t k) {
+        x[k] = x[k] + al * p[k];          // cgsolver.py:206-209
+        const T rk = r[k] - al * Ap[k];   // cgsolver.py:211-214
         r[k] = rk;
         acc += (double)rk * (double)rk;
-    }
-    double vals[1] = {block_sum(acc, sh)};
-    if (publish_partials<1>(vals, blockIdx.x, gridDim.x, partials, &st->counter[2])) {
-        double out[1];
-        finish_partials<1>(partials, gridDim.x, out, sh);
-        if (threadIdx.x == 0) {
-            const double nw = out[0];
-            st->rTr = nw;
-            st->iter = st->iter + 1;
-            if (!finite(nw)) set_done(st, KM_NONFINITE);
-            else if (sqrt(nw) < sqrt(st->r0) * st->eps) set_done(st, KM_OK);   // cgsolver.py:240
-            else if (st->iter >= st->max_iter) set_done(st, KM_MAXITER);
-            else {
-                st->beta = nw / st->rho;   // cgsolver.py:245
-                st->rho = nw;
-            }
-        }
-    }
+    });
+    write_partial(P + KM_PC(phase & 1), blockIdx.x, acc, sh);
 }
 
 // ------------------------------------------------------------ host helpers
 
-int flat_grid(int64_t n) {
+int grid_stride_blocks(int64_t n) {
     int64_t g = (n + KM_BLOCK - 1) / KM_BLOCK;
     if (g > 2048) g = 2048;
     if (g < 1) g = 1;
@@ -440,12 +468,12 @@ int get_tile(km_ctx* ctx, int NX, int NY, int prec, Tile* out) {
     int TI = 0, TJ = 0;
     if (int rc = km_choose_tile(ctx, NX, NY, prec, &TI, &TJ)) return rc;
     *out = make_tile(NX, NY, TI, TJ);
-    KM_REQUIRE(ctx, 2 * (int64_t)out->G <= KM_MAX_PARTIALS, "tile grid of %d workgroups exceeds the partial buffer", out->G);
     return KM_OK;
 }
 
 // Poll loop: enqueue chunks of iterations (a captured graph, or direct
-// launches), two chunks in flight, until the device sets ``done``.
+// launches), two chunks in flight, until the device sets ``done``.  The chunk
+// length is a multiple of KM_RING so that every chunk starts at phase 0.
 template <typename F>
 int run_chunks(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int chunk, int max_iter, F enqueue,
                KState* final_state) {
@@ -459,7 +487,7 @@ int run_chunks(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int chunk,
         } else {
             hipGraph_t graph = nullptr;
             KM_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
-            for (int c = 0; c < chunk; ++c) enqueue(ctx->cap_stream);
+            for (int c = 0; c < chunk; ++c) enqueue(ctx->cap_stream, c % KM_RING);
             hipError_t le = hipGetLastError();
             hipError_t ce = hipStreamEndCapture(ctx->cap_stream, &graph);
             if (le != hipSuccess || ce != hipSuccess) {
@@ -477,7 +505,7 @@ int run_chunks(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int chunk,
         if (exec) {
             KM_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
         } else {
-            for (int c = 0; c < chunk; ++c) enqueue(ctx->stream);
+            for (int c = 0; c < chunk; ++c) enqueue(ctx->stream, c % KM_RING);
             KM_CHECK_LAUNCH(ctx);
         }
         return KM_OK;
@@ -502,7 +530,7 @@ int run_chunks(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int chunk,
     return KM_OK;
 }
 
-const int kChunk = 8;
+const int kChunk = 8;   // iterations per captured chunk (a multiple of KM_RING)
 
 // Plain BiCGSTAB on typed buffers.  ws holds 8n elements of T.
 template <typename T>
@@ -515,16 +543,18 @@ int bicgstab_typed(km_ctx* ctx, const Tile& tl, const T* coef, const T* b, T* x,
     T* vbuf = ws + 4 * n;   // 2n
     T* s = ws + 6 * n;
     T* t = ws + 7 * n;
-    const int fg = flat_grid(n);
-    hipLaunchKernelGGL(k_bicg_init<T>, dim3(fg), dim3(KM_BLOCK), 0, ctx->stream, n, b, x, r, rt, pbuf, vbuf,
-                       from_r ? 1 : 0, eps, max_iter, ref_norm2, st, ctx->partials);
+    const Flat fl = make_flat(n);
+    double* P = ctx->partials;
+    hipLaunchKernelGGL(k_bicg_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, rt, pbuf, vbuf,
+                       from_r ? 1 : 0, eps, max_iter, ref_norm2, st, P);
     KM_CHECK_LAUNCH(ctx);
     const size_t lds = tile_lds<T>(tl);
-    double* partials = ctx->partials;
-    auto enqueue = [=](hipStream_t s_) {
-        hipLaunchKernelGGL(k_bicg_a<T>, dim3(tl.G), dim3(KM_BLOCK), lds, s_, tl, coef, r, rt, pbuf, vbuf, st, partials);
-        hipLaunchKernelGGL(k_bicg_b<T>, dim3(tl.G), dim3(KM_BLOCK), lds, s_, tl, coef, r, vbuf, s, t, st, partials);
-        hipLaunchKernelGGL(k_bicg_c<T>, dim3(fg), dim3(KM_BLOCK), 0, s_, n, x, pbuf, s, t, r, rt, st, partials);
+    auto enqueue = [=](hipStream_t s_, int phase) {
+        hipLaunchKernelGGL(k_bicg_a<T>, dim3(tl.G), dim3(KM_BLOCK), lds, s_, tl, fl, phase, coef, r, rt, pbuf, vbuf,
+                           st, P);
+        hipLaunchKernelGGL(k_bicg_b<T>, dim3(tl.G), dim3(KM_BLOCK), lds, s_, tl, phase, coef, r, vbuf, s, t, st, P);
+        hipLaunchKernelGGL(k_bicg_c<T>, dim3(fl.G), dim3(KM_BLOCK), 0, s_, fl, tl.G, phase, x, pbuf, s, t, r, rt,
+                           st, P);
     };
     GraphKey key;
     key.v = {1, sizeof(T), (uintptr_t)tl.NX, (uintptr_t)tl.NY, (uintptr_t)tl.TI, (uintptr_t)tl.TJ,
@@ -539,15 +569,15 @@ int cg_typed(km_ctx* ctx, const Tile& tl, const T* coef, const T* b, T* x, T* ws
     T* r = ws;
     T* pbuf = ws + n;   // 2n
     T* Ap = ws + 3 * n;
-    const int fg = flat_grid(n);
-    hipLaunchKernelGGL(k_cg_init<T>, dim3(fg), dim3(KM_BLOCK), 0, ctx->stream, n, b, x, r, pbuf, eps, max_iter, st,
-                       ctx->partials);
+    const Flat fl = make_flat(n);
+    double* P = ctx->partials;
+    hipLaunchKernelGGL(k_cg_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, pbuf, eps, max_iter,
+                       st, P);
     KM_CHECK_LAUNCH(ctx);
     const size_t lds = tile_lds<T>(tl);
-    double* partials = ctx->partials;
-    auto enqueue = [=](hipStream_t s_) {
-        hipLaunchKernelGGL(k_cg_a<T>, dim3(tl.G), dim3(KM_BLOCK), lds, s_, tl, coef, r, pbuf, Ap, st, partials);
-        hipLaunchKernelGGL(k_cg_b<T>, dim3(fg), dim3(KM_BLOCK), 0, s_, n, x, r, pbuf, Ap, st, partials);
+    auto enqueue = [=](hipStream_t s_, int phase) {
+        hipLaunchKernelGGL(k_cg_a<T>, dim3(tl.G), dim3(KM_BLOCK), lds, s_, tl, fl, phase, coef, r, pbuf, Ap, st, P);
+        hipLaunchKernelGGL(k_cg_b<T>, dim3(fl.G), dim3(KM_BLOCK), 0, s_, fl, tl.G, phase, x, r, pbuf, Ap, st, P);
     };
     GraphKey key;
     key.v = {2, sizeof(T), (uintptr_t)tl.NX, (uintptr_t)tl.NY, (uintptr_t)tl.TI, (uintptr_t)tl.TJ,
@@ -557,8 +587,9 @@ int cg_typed(km_ctx* ctx, const Tile& tl, const T* coef, const T* b, T* x, T* ws
 
 int residual_norm2(km_ctx* ctx, const Tile& tl, const double* coef, const double* b, const double* x, double* r,
                    bool sync, double* out) {
-    hipLaunchKernelGGL(k_residual, dim3(tl.G), dim3(KM_BLOCK), tile_lds<double>(tl), ctx->stream, tl, coef, b, x, r,
-                       ctx->partials, &ctx->st[1].counter[0], ctx->dres);
+    double* P = ctx->partials + KM_PX;
+    hipLaunchKernelGGL(k_residual, dim3(tl.G), dim3(KM_BLOCK), tile_lds<double>(tl), ctx->stream, tl, coef, b, x, r, P);
+    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, tl.G, ctx->dres);
     KM_CHECK_LAUNCH(ctx);
     if (sync) {
         KM_HIP(ctx, hipMemcpyAsync(ctx->h_res, ctx->dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -569,13 +600,15 @@ int residual_norm2(km_ctx* ctx, const Tile& tl, const double* coef, const double
 }
 
 int dot_sync(km_ctx* ctx, int64_t n, int prec, const void* a, const void* b, double* out) {
-    const int g = flat_grid(n);
+    const Flat fl = make_flat(n);
+    double* P = ctx->partials + KM_PX;
     if (prec == KM_F64)
-        hipLaunchKernelGGL(k_dot<double>, dim3(g), dim3(KM_BLOCK), 0, ctx->stream, n, (const double*)a,
-                           (const double*)b, ctx->partials, &ctx->st[1].counter[1], ctx->dres + 1);
+        hipLaunchKernelGGL(k_dot<double>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, (const double*)a,
+                           (const double*)b, P);
     else
-        hipLaunchKernelGGL(k_dot<float>, dim3(g), dim3(KM_BLOCK), 0, ctx->stream, n, (const float*)a,
-                           (const float*)b, ctx->partials, &ctx->st[1].counter[1], ctx->dres + 1);
+        hipLaunchKernelGGL(k_dot<float>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, (const float*)a,
+                           (const float*)b, P);
+    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, fl.G, ctx->dres + 1);
     KM_CHECK_LAUNCH(ctx);
     KM_HIP(ctx, hipMemcpyAsync(ctx->h_res + 1, ctx->dres + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -608,8 +641,8 @@ int bicgstab_mixed(km_ctx* ctx, int NX, int NY, const double* coef, const double
     float* d = rhs + n;
     float* w32 = d + n;
     const bool use_graph = !(flags & KM_FLAG_NO_GRAPH);
-    const int fg = flat_grid(5 * n);
-    hipLaunchKernelGGL(k_to_float, dim3(fg), dim3(KM_BLOCK), 0, ctx->stream, 5 * n, coef, c32, 1.0);
+    hipLaunchKernelGGL(k_to_float, dim3(grid_stride_blocks(5 * n)), dim3(KM_BLOCK), 0, ctx->stream, 5 * n, coef, c32,
+                       1.0);
     KM_CHECK_LAUNCH(ctx);
     double bb = 0.0;
     if (int rc = dot_sync(ctx, n, KM_F64, b, b, &bb)) return rc;
@@ -630,8 +663,8 @@ int bicgstab_mixed(km_ctx* ctx, int NX, int NY, const double* coef, const double
         if (rel < eps) { status = KM_OK; break; }
         if (total >= max_iter || sweeps >= 64) { status = KM_MAXITER; break; }
         const double rn = sqrt(rr);
-        hipLaunchKernelGGL(k_to_float, dim3(flat_grid(n)), dim3(KM_BLOCK), 0, ctx->stream, n, (const double*)r, rhs,
-                           1.0 / rn);
+        hipLaunchKernelGGL(k_to_float, dim3(grid_stride_blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n,
+                           (const double*)r, rhs, 1.0 / rn);
         KM_CHECK_LAUNCH(ctx);
         double inner_eps = eps / rel;
         if (inner_eps < inner_floor) inner_eps = inner_floor;
@@ -642,7 +675,8 @@ int bicgstab_mixed(km_ctx* ctx, int NX, int NY, const double* coef, const double
             return rc;
         total += s.iter;
         sweeps++;
-        hipLaunchKernelGGL(k_add_scaled, dim3(flat_grid(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const float*)d, rn);
+        hipLaunchKernelGGL(k_add_scaled, dim3(grid_stride_blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x,
+                           (const float*)d, rn);
         KM_CHECK_LAUNCH(ctx);
         const double prev = rel;
         if (int rc = residual_norm2(ctx, t64, coef, b, x, r, true, &rr)) return rc;
@@ -651,7 +685,7 @@ int bicgstab_mixed(km_ctx* ctx, int NX, int NY, const double* coef, const double
             // FP32 recurrences lost accuracy: take the sweep back (or restart
             // from zero if it produced non-finite values)
             if (rel == rel && rel < 1e300) {
-                hipLaunchKernelGGL(k_add_scaled, dim3(flat_grid(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x,
+                hipLaunchKernelGGL(k_add_scaled, dim3(grid_stride_blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x,
                                    (const float*)d, -rn);
                 KM_CHECK_LAUNCH(ctx);
             } else {
@@ -688,7 +722,7 @@ int bicgstab_mixed(km_ctx* ctx, int NX, int NY, const double* coef, const double
 
 // Tile choice: an explicit override, else the cached choice, else a default
 // that splits each row into near-equal segments of at most 256 columns and
-// stacks rows until a workgroup covers about 256 cells.
+// stacks rows until a tile covers about 256 cells.
 int km_choose_tile(km_ctx* ctx, int NX, int NY, int prec, int* TI, int* TJ) {
     auto key = std::make_tuple(NX, NY, prec);
     auto ov = ctx->tile_override.find(key);
@@ -726,8 +760,8 @@ int km_stencil_apply(km_ctx* ctx, int NX, int NY, int prec, const void* coef, co
 
 // Average device duration of one launch of a solver kernel, timed with HIP
 // events around `reps` back-to-back launches on the context's stream.  The
-// workspace is set up as for a solve of (coef, b); the kernels run with
-// ``done`` clear so every launch does its full work.
+// workspace is set up as for a solve of (coef, b) at iteration 0, which every
+// launch repeats in full.
 int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps,
                    double* avg_us) {
     if (!ctx) return KM_EINVAL;
@@ -742,35 +776,40 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
     double *r = ws, *rt = ws + n, *pbuf = ws + 2 * n, *vbuf = ws + 4 * n, *sv = ws + 6 * n, *t = ws + 7 * n;
     double* x = ws + 8 * n;
     KState* st = &ctx->st[2];
-    const int fg = flat_grid(n);
-    hipLaunchKernelGGL(k_bicg_init<double>, dim3(fg), dim3(KM_BLOCK), 0, ctx->stream, n, b, x, r, rt, pbuf, vbuf, 0,
-                       0.0, 1 << 30, 0.0, st, ctx->partials);
-    KM_CHECK_LAUNCH(ctx);
+    const Flat fl = make_flat(n);
+    double* P = ctx->partials;
+    hipLaunchKernelGGL(k_bicg_init<double>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, rt, pbuf, vbuf,
+                       0, 0.0, 1 << 30, 0.0, st, P);
     const size_t lds = tile_lds<double>(tl);
-    auto launch = [&]() {
-        switch (which) {
+    auto launch = [&](int w) {
+        switch (w) {
             case KM_KERNEL_BICG_A:
-                hipLaunchKernelGGL(k_bicg_a<double>, dim3(tl.G), dim3(KM_BLOCK), lds, ctx->stream, tl, coef, r, rt,
-                                   pbuf, vbuf, st, ctx->partials);
+                hipLaunchKernelGGL(k_bicg_a<double>, dim3(tl.G), dim3(KM_BLOCK), lds, ctx->stream, tl, fl, 0, coef,
+                                   r, rt, pbuf, vbuf, st, P);
                 break;
             case KM_KERNEL_BICG_B:
-                hipLaunchKernelGGL(k_bicg_b<double>, dim3(tl.G), dim3(KM_BLOCK), lds, ctx->stream, tl, coef, r, vbuf,
-                                   sv, t, st, ctx->partials);
+                hipLaunchKernelGGL(k_bicg_b<double>, dim3(tl.G), dim3(KM_BLOCK), lds, ctx->stream, tl, 0, coef, r,
+                                   vbuf, sv, t, st, P);
                 break;
             case KM_KERNEL_BICG_C:
-                hipLaunchKernelGGL(k_bicg_c<double>, dim3(fg), dim3(KM_BLOCK), 0, ctx->stream, n, x, pbuf, sv, t, r,
-                                   rt, st, ctx->partials);
+                hipLaunchKernelGGL(k_bicg_c<double>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, tl.G, 0, x, pbuf,
+                                   sv, t, r, rt, st, P);
                 break;
             default:
-                hipLaunchKernelGGL(k_stencil<double>, dim3(tl.G), dim3(KM_BLOCK), lds, ctx->stream, tl, coef, r, t);
+                hipLaunchKernelGGL(k_stencil<double>, dim3(tl.G), dim3(KM_BLOCK), lds, ctx->stream, tl, coef,
+                                   (const double*)r, t);
         }
     };
-    launch();   // warm-up
+    // set up the partials each kernel consumes: A, then B
+    launch(KM_KERNEL_BICG_A);
+    launch(KM_KERNEL_BICG_B);
+    launch(which);   // warm-up
+    KM_CHECK_LAUNCH(ctx);
     hipEvent_t e0, e1;
     KM_HIP(ctx, hipEventCreate(&e0));
     KM_HIP(ctx, hipEventCreate(&e1));
     KM_HIP(ctx, hipEventRecord(e0, ctx->stream));
-    for (int i = 0; i < reps; ++i) launch();
+    for (int i = 0; i < reps; ++i) launch(which);
     KM_HIP(ctx, hipEventRecord(e1, ctx->stream));
     KM_HIP(ctx, hipEventSynchronize(e1));
     float ms = 0.f;
@@ -778,6 +817,8 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     KM_CHECK_LAUNCH(ctx);
+    KM_HIP(ctx, hipMemcpy(&ctx->h_st[3], st, sizeof(KState), hipMemcpyDeviceToHost));
+    KM_REQUIRE(ctx, !ctx->h_st[3].done, "km_time_kernel: the system finished during timing (b = 0?)");
     *avg_us = 1e3 * (double)ms / reps;
     return KM_OK;
 }
