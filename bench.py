@@ -30,8 +30,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per cell of the fused BiCGSTAB kernels (FP64)
-BYTES_PER_CELL = {"bicg_a": 11 * 8, "bicg_b": 9 * 8, "bicg_c": 7 * 8}
+# algorithmic bytes per cell of the two fused BiCGSTAB kernels (FP64):
+#   A reads s, t, p, v, x, r~ and 5 coefficient planes, writes r, p', v', x
+#   B reads r, v', r~ and 5 coefficient planes, writes s, t
+BYTES_PER_CELL = {"bicg_a": 15 * 8, "bicg_b": 10 * 8}
 
 
 def parse():
@@ -43,7 +45,9 @@ def parse():
     ap.add_argument("--mode", default="plain", choices=["plain", "mixed", "cg"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2, help="outer steps in the CPU baseline sample")
-    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic.json"))
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "k_bicg_traffic_c2_1280x256.json"))
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run a device-to-device copy of this many MiB (counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -113,12 +117,14 @@ def main():
     ctx = s.ctx
     ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
     NX, NY = case.shape_p
-    kern = {}
-    for name, kid in (("bicg_a", _lib.KM_KERNEL_BICG_A), ("bicg_b", _lib.KM_KERNEL_BICG_B),
-                      ("bicg_c", _lib.KM_KERNEL_BICG_C)):
+    kern, kern_ev = {}, {}
+    for name, kid in (("bicg_a", _lib.KM_KERNEL_BICG_A), ("bicg_b", _lib.KM_KERNEL_BICG_B)):
         us = ctypes.c_double()
-        ctx.call("km_time_kernel", kid, NX, NY, _lib.ptr(s._coef_p), _lib.ptr(s.b_p), 200, ctypes.byref(us))
+        ctx.call("km_time_kernel", kid | _lib.KM_KERNEL_BACK_TO_BACK, NX, NY, _lib.ptr(s._coef_p), _lib.ptr(s.b_p),
+                 200, ctypes.byref(us))
         kern[name] = us.value
+        ctx.call("km_time_kernel", kid, NX, NY, _lib.ptr(s._coef_p), _lib.ptr(s.b_p), 200, ctypes.byref(us))
+        kern_ev[name] = us.value
     dom = max(kern, key=kern.get)      # one launch of each per iteration: the longest dominates
     alg_bytes = BYTES_PER_CELL[dom] * cells_p
     achieved = alg_bytes / (kern[dom] * 1e-6) / 1e9
@@ -131,6 +137,17 @@ def main():
         pass
     iter_us = sum(kern.values())
     log(f"bench: kernel us {kern}, dominant {dom}: {achieved:.0f} GB/s")
+
+    if args.calib_copy_mb:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings
+        # calibrate the counters of the solver kernels in the same run
+        nb = args.calib_copy_mb << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: calibration copy of {nb} bytes x 4")
 
     # ---- CPU baseline: C/OpenMP restatement with the reference decomposition
     cpu = None
@@ -162,6 +179,7 @@ def main():
         "p_cells_iters_per_s": round(cells_p * total_p_it / total_p_s, 1),
         "us_per_p_iter": round(total_p_s / total_p_it * 1e6, 3),
         "kernel_us": {k: round(v, 3) for k, v in kern.items()},
+        "kernel_us_dispatch_events": {k: round(v, 3) for k, v in kern_ev.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes,

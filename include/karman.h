@@ -88,6 +88,11 @@ int km_synchronize(km_ctx* ctx);
  * NX x NY system; ti = tj = 0 clears the override and re-enables autotuning. */
 int km_set_tile(km_ctx* ctx, int NX, int NY, int prec, int ti, int tj);
 int km_get_tile(km_ctx* ctx, int NX, int NY, int prec, int* ti, int* tj);
+/* Tile autotuning (on by default): the first solve of a shape times each valid
+ * tile candidate on that system and keeps the fastest (no reference
+ * counterpart; the reference's only tuning knob is ti.block_dim,
+ * benchmark/para_vs_seri.py:27). */
+int km_set_autotune(km_ctx* ctx, int on);
 
 /* ---- discretisation (SIMPLESolver kernels) ---- */
 /* compute_coef_u + the u half of set_bc (simple.py:77-92, :177-207) + obstacle rows */
@@ -135,10 +140,10 @@ int km_cg(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, doub
  * NX x NY system, from HIP events around `reps` launches on the context's
  * stream.  Used by the benchmark to report achieved bandwidth of the kernel
  * that dominates the solve; rocprofv3 --kernel-trace reports the same kernels. */
-#define KM_KERNEL_BICG_A 0   /* p update + stencil + <r~, v>  */
-#define KM_KERNEL_BICG_B 1   /* s update + stencil + <t, s>, <t, t> */
-#define KM_KERNEL_BICG_C 2   /* x, r update + <r, r>, <r, r~> */
+#define KM_KERNEL_BICG_A 0   /* x, r, p updates + stencil + <r~, v>, <r, r> */
+#define KM_KERNEL_BICG_B 1   /* s update + stencil + <t, s>, <t, t>, <s, r~>, <t, r~> */
 #define KM_KERNEL_STENCIL 3  /* plain y = A x */
+#define KM_KERNEL_BACK_TO_BACK 0x100   /* or-ed in: time `reps` identical back-to-back launches between two events */
 int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps,
                    double* avg_us);
 

@@ -17,7 +17,8 @@ KM_EINVAL, KM_EHIP, KM_ENOMEM = -1, -2, -3
 KM_F64, KM_F32 = 0, 1
 KM_FLAG_X0, KM_FLAG_MIXED, KM_FLAG_NO_GRAPH = 1, 2, 4
 KM_EAST_WALL, KM_EAST_OUTFLOW = 0, 1
-KM_KERNEL_BICG_A, KM_KERNEL_BICG_B, KM_KERNEL_BICG_C, KM_KERNEL_STENCIL = 0, 1, 2, 3
+KM_KERNEL_BICG_A, KM_KERNEL_BICG_B, KM_KERNEL_STENCIL = 0, 1, 3
+KM_KERNEL_BACK_TO_BACK = 0x100
 
 STATUS_NAMES = {KM_OK: "ok", KM_BREAKDOWN: "breakdown", KM_MAXITER: "maxiter",
                 KM_NONFINITE: "nonfinite", KM_STALL: "stall"}
@@ -59,6 +60,7 @@ SIGNATURES = {
     "km_synchronize": [_CTX],
     "km_set_tile": [_CTX, _I, _I, _I, _I, _I],
     "km_get_tile": [_CTX, _I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)],
+    "km_set_autotune": [_CTX, _I],
     "km_assemble_u": [_CTX, _GRID, _P, _P, _P, _P, _P, _P, _P],
     "km_assemble_v": [_CTX, _GRID, _P, _P, _P, _P, _P, _P, _P],
     "km_apply_ghosts": [_CTX, _GRID, _P, _P],

@@ -4,11 +4,13 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -27,7 +29,7 @@
 #define KM_RING 4
 struct KScal {
     double rho, alpha, omega, beta;
-    int32_t iter, pad;   // iter: iterations completed when this phase's last kernel finished
+    int32_t iter, pad;   // iteration index of this phase
 };
 struct KState {
     double r0;        // stopping-test denominator (r0.r0, or ||b||^2 for a warm start)
@@ -46,11 +48,11 @@ struct KState {
 // order memory.  (Detecting the last workgroup with a device-scope atomic
 // costs ~14 us per launch on gfx950, more than the whole stencil sweep of a
 // 1280x256 system.)
-#define KM_MAX_RED 1024
-#define KM_PA 0                          // [1][KM_MAX_RED]
-#define KM_PB (KM_MAX_RED)               // [2][KM_MAX_RED]
-#define KM_PC(par) (3 * KM_MAX_RED + (par) * 2 * KM_MAX_RED)   // [2][KM_MAX_RED], double-buffered
-#define KM_PX (8 * KM_MAX_RED)           // helpers (dot, residual, mdiv): [2][KM_MAX_RED]
+#define KM_MAX_RED 2048
+#define KM_PA 0                          // [2][KM_MAX_RED]
+#define KM_PB (2 * KM_MAX_RED)           // [4][KM_MAX_RED]
+#define KM_PC(par) (6 * KM_MAX_RED + (par) * 2 * KM_MAX_RED)   // [2][KM_MAX_RED], double-buffered
+#define KM_PX (10 * KM_MAX_RED)          // helpers (dot, residual, mdiv): [2][KM_MAX_RED]
 
 // Geometry of a 2-D tiled stencil launch: TI rows x TJ columns per tile,
 // tpb consecutive tiles per workgroup, G workgroups (<= KM_MAX_RED).
@@ -67,6 +69,16 @@ static inline Tile make_tile(int NX, int NY, int TI, int TJ) {
     t.tpb = (t.T + KM_MAX_RED - 1) / KM_MAX_RED;
     t.G = (t.T + t.tpb - 1) / t.tpb;
     return t;
+}
+
+// Tiles the register-staged iteration kernels accept: at most 4 cells per
+// thread and a halo'd tile of at most (cells-per-thread + 2) positions per
+// thread (cells per thread rounded up to 1, 2 or 4).
+static inline bool tile_ok(int TI, int TJ) {
+    if (TI < 1 || TJ < 1 || TI * TJ > 4 * KM_BLOCK) return false;
+    const int c = (TI * TJ + KM_BLOCK - 1) / KM_BLOCK;
+    const int cpt = c <= 1 ? 1 : (c <= 2 ? 2 : 4);
+    return (TI + 2) * (TJ + 2) <= (cpt + 2) * KM_BLOCK;
 }
 
 // A flat (pointwise) launch over n elements: G workgroups, each owning a
@@ -114,6 +126,8 @@ struct km_ctx {
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tiles;   // (NX, NY, prec) -> (TI, TJ)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tile_override;
     std::map<GraphKey, hipGraphExec_t> graphs;
+    int autotune = 1;              // time tile candidates on the first solve of a shape
+    std::set<std::tuple<int, int, int>> tuned;
 };
 
 int km_fail(km_ctx* ctx, int code, const char* fmt, ...);

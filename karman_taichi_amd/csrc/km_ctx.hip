@@ -127,11 +127,19 @@ int km_set_tile(km_ctx* ctx, int NX, int NY, int prec, int ti, int tj) {
     if (ti == 0 && tj == 0) {
         ctx->tile_override.erase(key);
         ctx->tiles.erase(key);
+        ctx->tuned.erase(key);
         return KM_OK;
     }
-    KM_REQUIRE(ctx, ti >= 1 && tj >= 1 && (int64_t)(ti + 2) * (tj + 2) * 8 <= 64 * 1024,
-               "tile %dx%d is invalid or needs more than 64 KiB of LDS", ti, tj);
+    KM_REQUIRE(ctx, tile_ok(ti, tj),
+               "tile %dx%d is invalid: at most 1024 cells, and (ti+2)(tj+2) within (cells per thread + 2) x 256",
+               ti, tj);
     ctx->tile_override[key] = {ti, tj};
+    return KM_OK;
+}
+
+int km_set_autotune(km_ctx* ctx, int on) {
+    if (!ctx) return KM_EINVAL;
+    ctx->autotune = on ? 1 : 0;
     return KM_OK;
 }
 

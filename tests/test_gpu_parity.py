@@ -143,7 +143,7 @@ def test_stencil_bit_exact(shape, prec, dev):
     ct, xt = T(coef, dev), T(x, dev)
     yt = torch.empty_like(xt)
     want = krylov.stencil(coef, x)
-    tiles = [(0, 0), (2, 3), (1, shape[1]), (3, 17), (8, 64)]
+    tiles = [(0, 0), (2, 3), (1, min(shape[1], 250)), (3, 17), (8, 64)]
     for ti, tj in tiles:
         if ti:
             ctx.call("km_set_tile", shape[0], shape[1], prec, ti, tj)
