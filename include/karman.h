@@ -135,6 +135,40 @@ int km_bicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
 int km_cg(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x,
           double eps, int max_iter, int flags, km_solve_info* info);
 
+/* ---- learned p' preconditioner ----
+ * A 4-level U-Net of 3x3 convolutions (16/32/64/128 channels, BF16 weights and
+ * activations on the matrix cores, FP32 accumulation) maps (p' residual, aP,
+ * identity-row mask) to a p' correction.  It fills the right-preconditioner
+ * hooks update_phat / update_shat of BICGSolver (bicgsolver.py:48-58, identity
+ * there; karman_taichi.py:399,419 plugs Jacobi into the same hooks).  The
+ * network itself has no counterpart in the reference.
+ *
+ * Parameters are one caller-owned FP32 device vector of km_unet_param_count
+ * values: for each layer e0 e1 e2 e3 d2 d1 d0, W[cout][3][3][cin] then
+ * b[cout]; then the 1x1 output layer w[16], b[1]. */
+int km_unet_param_count(int64_t* n);
+/* z = M(r) for one system (FP64 coefficients and vectors; the network runs in
+ * BF16/FP32).  Asynchronous. */
+int km_unet_apply(km_ctx* ctx, const float* params, int NX, int NY, const double* coef, const double* r, double* z);
+/* Training step without the update: mean over a batch of B systems of
+ * ||r - A M(r)||^2 / ||r||^2 and its gradient.  coef: B x 5 planes of NX x NY
+ * FP32, r: B x NX x NY FP32, grad: param_count FP32.  loss (may be NULL)
+ * receives the batch mean and makes the call synchronous.  B <= 64. */
+int km_unet_loss_grad(km_ctx* ctx, const float* params, int B, int NX, int NY, const float* coef, const float* r,
+                      float* grad, double* loss);
+/* Adam (Kingma & Ba 2015) on FP32 vectors of n values; step counts from 1. */
+int km_adam_step(km_ctx* ctx, int64_t n, float* params, const float* grad, float* m, float* v, int step, double lr,
+                 double b1, double b2, double eps);
+/* BICGSolver.solve (bicgsolver.py:88-136) with M = the network in the
+ * p^ / s^ hooks: flexible BiCGSTAB in FP32 vectors with FP64 scalars inside
+ * FP64 iterative refinement, judged only by the true residual
+ * ||b - A x|| / ||b|| < eps.  Without KM_FLAG_X0 the first iterate is the
+ * network's warm start x0 = M(b) (kept if it lowers the residual).  A sweep
+ * that does not halve the true residual is taken back and plain FP64
+ * BiCGSTAB finishes (info->fallback = 1). */
+int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
+                 int max_iter, int flags, const float* params, km_solve_info* info);
+
 /* ---- measurement hook (no counterpart in the reference) ----
  * Average device time in microseconds of one launch of a solver kernel on an
  * NX x NY system, from HIP events around `reps` launches on the context's

@@ -76,6 +76,11 @@ SIGNATURES = {
     "km_bicgstab": [_CTX, _I, _I, _P, _P, _P, _D, _I, _I, _INFO],
     "km_cg": [_CTX, _I, _I, _P, _P, _P, _D, _I, _I, _INFO],
     "km_time_kernel": [_CTX, _I, _I, _I, _P, _P, _I, ctypes.POINTER(_D)],
+    "km_unet_param_count": [ctypes.POINTER(ctypes.c_int64)],
+    "km_unet_apply": [_CTX, _P, _I, _I, _P, _P, _P],
+    "km_unet_loss_grad": [_CTX, _P, _I, _I, _I, _P, _P, _P, ctypes.POINTER(_D)],
+    "km_adam_step": [_CTX, ctypes.c_int64, _P, _P, _P, _P, _I, _D, _D, _D, _D],
+    "km_fbicgstab": [_CTX, _I, _I, _P, _P, _P, _D, _I, _I, _P, _INFO],
 }
 _RESTYPE = {"km_last_error": ctypes.c_char_p}
 
@@ -103,6 +108,14 @@ def lib():
                 f.restype = _RESTYPE.get(name, ctypes.c_int)
             _lib = h
     return _lib
+
+
+def unet_param_count() -> int:
+    """Number of FP32 network parameters (a host-only call)."""
+    n = ctypes.c_int64()
+    if lib().km_unet_param_count(ctypes.byref(n)) != KM_OK:
+        raise KarmanError("km_unet_param_count failed")
+    return n.value
 
 
 def exported_symbols():

@@ -1,5 +1,6 @@
 // Context, error reporting and workspace arena of libkarman_hip.so.
 #include "km_common.h"
+#include "km_unet.h"
 
 #include <cstdlib>
 #include <cstring>
@@ -78,7 +79,7 @@ int km_create(km_ctx** out, int device, void* hip_stream) {
     ctx->stream = static_cast<hipStream_t>(hip_stream);
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return KM_EHIP; }
     if (hipHostMalloc(reinterpret_cast<void**>(&ctx->h_st), 4 * sizeof(KState), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&ctx->h_res), 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(reinterpret_cast<void**>(&ctx->h_res), 128 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         delete ctx;
         return KM_ENOMEM;
     }
@@ -97,6 +98,7 @@ int km_destroy(km_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     km_drop_graphs(ctx);
+    km_net_free(ctx);
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->h_st) (void)hipHostFree(ctx->h_st);
     if (ctx->h_res) (void)hipHostFree(ctx->h_res);

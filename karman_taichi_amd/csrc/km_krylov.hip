@@ -620,65 +620,6 @@ int get_tile(km_ctx* ctx, int NX, int NY, int prec, Tile* out) {
     return KM_OK;
 }
 
-// Poll loop: enqueue chunks of iterations (a captured graph, or direct
-// launches), two chunks in flight, until the device sets ``done``.  The chunk
-// length is a multiple of KM_RING so that every chunk starts at phase 0.
-template <typename F>
-int run_chunks(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int chunk, int max_iter, F enqueue,
-               KState* final_state) {
-    hipGraphExec_t exec = nullptr;
-    if (use_graph) {
-        key.v.push_back((uintptr_t)chunk);
-        key.v.push_back((uintptr_t)ctx->ws_gen);
-        auto it = ctx->graphs.find(key);
-        if (it != ctx->graphs.end()) {
-            exec = it->second;
-        } else {
-            hipGraph_t graph = nullptr;
-            KM_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
-            for (int c = 0; c < chunk; ++c) enqueue(ctx->cap_stream, c % KM_RING);
-            hipError_t le = hipGetLastError();
-            hipError_t ce = hipStreamEndCapture(ctx->cap_stream, &graph);
-            if (le != hipSuccess || ce != hipSuccess) {
-                if (graph) (void)hipGraphDestroy(graph);
-                return km_fail(ctx, KM_EHIP, "graph capture failed: %s / %s", hipGetErrorString(le),
-                               hipGetErrorString(ce));
-            }
-            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (ie != hipSuccess) return km_fail(ctx, KM_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
-            ctx->graphs[key] = exec;
-        }
-    }
-    auto launch_chunk = [&]() -> int {
-        if (exec) {
-            KM_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
-        } else {
-            for (int c = 0; c < chunk; ++c) enqueue(ctx->stream, c % KM_RING);
-            KM_CHECK_LAUNCH(ctx);
-        }
-        return KM_OK;
-    };
-    // a device-side cap ends every solve; the host cap only guards against a
-    // state that never reports done because a launch failed
-    const int64_t host_cap = (int64_t)max_iter / chunk + 4;
-    if (int rc = launch_chunk()) return rc;
-    KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[0], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
-    KM_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    for (int64_t c = 1;; ++c) {
-        if (c > host_cap) return km_fail(ctx, KM_EHIP, "solver did not report completion after %lld chunks", (long long)c);
-        if (int rc = launch_chunk()) return rc;
-        KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[c & 1], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
-        KM_HIP(ctx, hipEventRecord(ctx->ev[c & 1], ctx->stream));
-        KM_HIP(ctx, hipEventSynchronize(ctx->ev[(c - 1) & 1]));
-        if (ctx->h_st[(c - 1) & 1].done) break;
-    }
-    KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[2], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
-    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    *final_state = ctx->h_st[2];
-    return KM_OK;
-}
-
 const int kChunk = 8;   // iterations per captured chunk (a multiple of KM_RING)
 
 // Plain BiCGSTAB on typed buffers.  ws holds 8n elements of T.

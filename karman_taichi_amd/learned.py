@@ -1,0 +1,245 @@
+"""Learned p' preconditioner: parameters, harvesting and training (host side).
+
+The network's forward and backward passes, the residual loss, Adam and the
+flexible BiCGSTAB that uses the network run in libkarman_hip.so
+(``csrc/km_unet.hip``, ``csrc/km_fbicg.hip``).  This module owns the FP32
+parameter vector and the Adam state on the device, harvests training pairs
+from plain GPU solves and runs the training loop.
+
+The network fills the right-preconditioner hooks ``update_phat`` /
+``update_shat`` of the reference's BiCGSTAB (bicgsolver.py:48-58, identity
+there; karman_taichi.py:399,419 plugs Jacobi into the same hooks).
+
+Architecture: a 4-level U-Net of 3x3 convolutions, 16 -> 32 -> 64 -> 128
+channels (2x2 average pools down, 2x nearest upsampling and skip
+concatenation up, ReLU), a 1x1 output layer; inputs (r / rs, aP / abar,
+identity-row mask), output scaled by rs / abar.  Parameters in the order
+e0 e1 e2 e3 d2 d1 d0 (W[cout][3][3][cin], b[cout]) then w_out[16], b_out.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+import time
+from pathlib import Path
+from typing import Iterable, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import KmSolveInfo, ptr
+
+# (name, input channels, output channels) in parameter order
+LAYERS = (("e0", 3, 16), ("e1", 16, 32), ("e2", 32, 64), ("e3", 64, 128),
+          ("d2", 192, 64), ("d1", 96, 32), ("d0", 48, 16))
+OUT_IN = 16
+
+WEIGHTS_DIR = Path(__file__).resolve().parent / "weights"
+
+
+def param_count() -> int:
+    return sum(co * 9 * ci + co for _, ci, co in LAYERS) + OUT_IN + 1
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal convolution weights (std sqrt(2 / (9 cin))), zero biases,
+    N(0, 1/16) output weights."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for _, ci, co in LAYERS:
+        parts.append(rng.standard_normal(co * 9 * ci) * math.sqrt(2.0 / (9 * ci)))
+        parts.append(np.zeros(co))
+    parts.append(rng.standard_normal(OUT_IN) * 0.25)
+    parts.append(np.zeros(1))
+    return np.concatenate(parts).astype(np.float32)
+
+
+class UNetPreconditioner:
+    """FP32 parameters + Adam state of the network on one device."""
+
+    def __init__(self, params=None, device=None, seed: int = 0):
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.device = torch.device(device)
+        n = _lib.unet_param_count()
+        if n != param_count():
+            raise _lib.KarmanError(f"library parameter count {n} != {param_count()}")
+        src = init_params(seed) if params is None else np.asarray(params, dtype=np.float32)
+        if src.shape != (n,):
+            raise ValueError(f"expected {n} parameters, got {src.shape}")
+        self.params = torch.tensor(src, dtype=torch.float32, device=self.device)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.grad = torch.zeros_like(self.params)
+        self.step_count = 0
+        self.ctx = _lib.context_for(self.device)
+
+    # ---------------------------------------------------------------- io
+    def save(self, path):
+        np.savez(path, params=self.params.cpu().numpy(), m=self.m.cpu().numpy(), v=self.v.cpu().numpy(),
+                 step=np.array([self.step_count]))
+
+    @classmethod
+    def load(cls, path, device=None) -> "UNetPreconditioner":
+        with np.load(path, allow_pickle=False) as d:
+            net = cls(params=d["params"], device=device)
+            if "m" in d:
+                net.m.copy_(torch.from_numpy(d["m"]))
+                net.v.copy_(torch.from_numpy(d["v"]))
+                net.step_count = int(d["step"][0])
+        return net
+
+    # ---------------------------------------------------------------- compute
+    def _bind(self):
+        self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def apply(self, coef: torch.Tensor, r: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """z = M(r) for one system: coef (5, NX, NY) FP64 SoA, r (NX, NY) FP64."""
+        NX, NY = r.shape
+        z = torch.empty_like(r) if out is None else out
+        self._bind()
+        self.ctx.call("km_unet_apply", ptr(self.params), NX, NY, ptr(coef), ptr(r), ptr(z))
+        return z
+
+    def loss_grad(self, coefs: torch.Tensor, rs: torch.Tensor):
+        """Batch mean of ||r - A M(r)||^2 / ||r||^2 and its gradient (into self.grad).
+
+        coefs: (B, 5, NX, NY) FP32, rs: (B, NX, NY) FP32, both contiguous."""
+        B, _, NX, NY = coefs.shape
+        if coefs.dtype != torch.float32 or rs.dtype != torch.float32:
+            raise TypeError("training batches are float32")
+        loss = ctypes.c_double()
+        self._bind()
+        self.ctx.call("km_unet_loss_grad", ptr(self.params), B, NX, NY, ptr(coefs.contiguous()),
+                      ptr(rs.contiguous()), ptr(self.grad), ctypes.byref(loss))
+        return loss.value, self.grad
+
+    def adam_step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, grad: Optional[torch.Tensor] = None):
+        self.step_count += 1
+        g = self.grad if grad is None else grad
+        self._bind()
+        self.ctx.call("km_adam_step", ctypes.c_int64(self.params.numel()), ptr(self.params), ptr(g), ptr(self.m),
+                      ptr(self.v), self.step_count, lr, b1, b2, eps)
+
+    def train_step(self, coefs, rs, lr=1e-3) -> float:
+        loss, _ = self.loss_grad(coefs, rs)
+        self.adam_step(lr)
+        return loss
+
+
+def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: UNetPreconditioner, eps=1e-8,
+              max_iter=None, x0=False, graph=True):
+    """Flexible BiCGSTAB with the network as right preconditioner (km_fbicgstab)."""
+    from .solvers import SolveInfo
+    NX, NY = b.shape
+    info = KmSolveInfo()
+    flags = (_lib.KM_FLAG_X0 if x0 else 0) | (0 if graph else _lib.KM_FLAG_NO_GRAPH)
+    net._bind()
+    net.ctx.call("km_fbicgstab", NX, NY, ptr(coef), ptr(b), ptr(x), float(eps), int(max_iter or 0), flags,
+                 ptr(net.params), ctypes.byref(info))
+    return SolveInfo.from_c(info)
+
+
+# -------------------------------------------------------------------- data
+
+@dataclasses.dataclass
+class PairSet:
+    """Harvested (p' residual, converged correction) pairs on the device.
+
+    coefs: (S, 5, NX, NY) FP32 p' matrices, one per harvested system;
+    rs: (P, NX, NY) FP32 residuals r_k = b - A x_k; sys: (P,) system index;
+    corr: (P, NX, NY) FP32 corrections x* - x_k (A corr = r_k), optional."""
+    coefs: torch.Tensor
+    rs: torch.Tensor
+    sys: torch.Tensor
+    corr: Optional[torch.Tensor] = None
+    meta: dict = dataclasses.field(default_factory=dict)
+
+    def __len__(self):
+        return self.rs.shape[0]
+
+    def batch(self, idx: torch.Tensor):
+        return self.coefs.index_select(0, self.sys.index_select(0, idx)).contiguous(), \
+            self.rs.index_select(0, idx).contiguous()
+
+
+def harvest(case_list: Sequence, outer_steps: int, snapshots: Sequence[int] = (0, 8, 32, 128, 512),
+            device=None, corrections: bool = False, progress=None) -> PairSet:
+    """Run the plain GPU SIMPLE loop on each case and record, at every outer
+    step's p' system, the residuals r_k = b - A x_k of the plain BiCGSTAB
+    iterates at the iteration counts ``snapshots`` (0: r = b) with the
+    converged solve's correction x* - x_k.  The iterates come from plain
+    BiCGSTAB restarted from x_prev for the next increment of iterations."""
+    from .simple import SIMPLESolver
+    from .solvers import BICGSolver
+    if device is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    snaps = sorted(set(int(s) for s in snapshots))
+    shapes = {c.shape_p for c in case_list}
+    if len(shapes) != 1:
+        raise ValueError("all harvested cases must share one grid")
+    NX, NY = shapes.pop()
+    S = len(case_list) * outer_steps
+    P = S * len(snaps)
+    coefs = torch.empty((S, 5, NX, NY), dtype=torch.float32, device=device)
+    rs = torch.empty((P, NX, NY), dtype=torch.float32, device=device)
+    corr = torch.empty((P, NX, NY), dtype=torch.float32, device=device) if corrections else None
+    sysidx = torch.empty(P, dtype=torch.int64)
+    xk = torch.zeros((NX, NY), dtype=torch.float64, device=device)
+    rk = torch.empty_like(xk)
+    si = pi = 0
+    t0 = time.perf_counter()
+    for case in case_list:
+        s = SIMPLESolver.from_case(case, device=device)
+        s.init()
+        trunc = BICGSolver(s._coef_p, s.b_p, xk)
+        for k in range(outer_steps):
+            s.solve_momentum_eqn(1)
+            s.correct_conserv()
+            s.compute_coef_p()
+            s.p_correction_solver.solve(eps=case.eps_p)
+            coefs[si].copy_(s._coef_p)
+            xk.zero_()
+            done = 0
+            for kk in snaps:
+                if kk > done:
+                    trunc.solve(eps=0.0, max_iter=kk - done, x0=True)
+                    done = kk
+                s.ctx.call("km_residual", NX, NY, ptr(s._coef_p), ptr(s.b_p), ptr(xk), ptr(rk), None)
+                rs[pi].copy_(rk)
+                if corr is not None:
+                    corr[pi].copy_(s.pcor - xk)
+                sysidx[pi] = si
+                pi += 1
+            si += 1
+            s._correct()
+            s.compute_mdiv()
+            if (k + 1) % case.n_outer == 0:
+                s.time_forward()
+            if progress is not None and si % 25 == 0:
+                progress(f"harvest: {si}/{S} systems, {pi} pairs, {time.perf_counter() - t0:.1f} s")
+    torch.cuda.synchronize(device)
+    return PairSet(coefs, rs, sysidx.to(device), corr,
+                   meta={"cases": [c.name for c in case_list], "outer_steps": outer_steps, "snapshots": snaps,
+                         "seconds": time.perf_counter() - t0})
+
+
+def train(net: UNetPreconditioner, pairs: PairSet, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 0,
+          log_every: int = 50, progress=None):
+    """Adam on random batches of harvested pairs; returns [(step, mean loss over the log window)]."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    hist, window = [], []
+    t0 = time.perf_counter()
+    for it in range(1, steps + 1):
+        idx = torch.randint(len(pairs), (batch,), generator=g).to(pairs.rs.device)
+        coefs, rs = pairs.batch(idx)
+        window.append(net.train_step(coefs, rs, lr))
+        if it % log_every == 0 or it == steps:
+            m = float(np.mean(window))
+            hist.append((it, m))
+            window = []
+            if progress is not None:
+                progress(f"train: step {it}/{steps}, loss {m:.4f}, {time.perf_counter() - t0:.1f} s")
+    return hist

@@ -94,6 +94,7 @@ class SIMPLESolver:
         self.p_solver = "bicgstab"    # or "cg" (simple.py:254)
         self.mixed = False            # FP32 inner solves + FP64 refinement
         self.warm_start_p = False
+        self.p_precond = None         # a learned.UNetPreconditioner: flexible BiCGSTAB for p'
         self.last = {}
 
     @classmethod
@@ -195,6 +196,8 @@ class SIMPLESolver:
         self.compute_coef_p()
         if self.p_solver == "cg":
             return self.p_correction_cg.solve(eps=self.eps_p)
+        if self.p_precond is not None:
+            return self.p_correction_solver.solve(eps=self.eps_p, x0=self.warm_start_p, precond=self.p_precond)
         return self.p_correction_solver.solve(eps=self.eps_p, mixed=self.mixed, x0=self.warm_start_p)
 
     def init(self, seed_fields=True):

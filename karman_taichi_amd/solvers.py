@@ -157,16 +157,26 @@ class BICGSolver(CGSolver):
 
     _kind = "bicgstab"
 
-    def solve(self, eps=1e-8, quiet=True, max_iter=None, mixed=False, x0=False, graph=True) -> SolveInfo:
+    def solve(self, eps=1e-8, quiet=True, max_iter=None, mixed=False, x0=False, graph=True,
+              precond=None) -> SolveInfo:
         """BiCGSTAB until sqrt(r.r / b.b) < eps (bicgsolver.py:130).
 
         ``mixed``: FP32 matvec/axpy inner solves, FP64 scalars, FP64 refinement
         judged by the true residual.  ``x0``: start from the current ``x``
         instead of zero (then the reference's r~ = b becomes r~ = b - A x0).
+        ``precond``: a :class:`~karman_taichi_amd.learned.UNetPreconditioner`
+        in the p^ / s^ hooks (bicgsolver.py:48-58): flexible BiCGSTAB in FP32
+        vectors with FP64 refinement, judged by the true residual only, with
+        plain BiCGSTAB as the fallback.
         """
         flags = (_lib.KM_FLAG_MIXED if mixed else 0) | (_lib.KM_FLAG_X0 if x0 else 0)
         flags |= 0 if graph else _lib.KM_FLAG_NO_GRAPH
-        info = self._run("km_bicgstab", eps, max_iter, flags)
+        if precond is not None:
+            from .learned import fbicgstab
+            self._refresh()
+            info = fbicgstab(self.coef, self.b, self.x, precond, eps=eps, max_iter=max_iter, x0=x0, graph=graph)
+        else:
+            info = self._run("km_bicgstab", eps, max_iter, flags)
         if not quiet:
             print(f">>> BICG {info.status_name} after {info.iterations} iterations, "
                   f"relative residual {info.rel_resid_true:.3e}")

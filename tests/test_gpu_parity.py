@@ -206,6 +206,12 @@ def test_bicgstab_first_iterations_track_oracle(dev, c1_pcor):
 
 
 def test_bicgstab_converges_like_oracle(dev, c1_pcor):
+    """Iteration counts within 25%: on this system (kappa ~ 3e5) the count to
+    1e-8 / 1e-12 moves by up to 17% under rounding alone -- the oracle itself
+    gives 687 / 839 with numpy dots, 732 / 893 with exactly rounded dots
+    (math.fsum) and 768 / 981 with rho formed as <s, r~> - omega <t, r~>, the
+    identity the GPU kernels use.  The first iterations are pinned tightly by
+    test_bicgstab_first_iterations_track_oracle and the solution by LU."""
     coef, b = c1_pcor
     xd = krylov.direct_solve(coef, b)
     for eps in (1e-8, 1e-12):
@@ -214,7 +220,7 @@ def test_bicgstab_converges_like_oracle(dev, c1_pcor):
         print(f"eps={eps}: gpu {info.iterations} it, oracle {io['iterations']} it, "
               f"true resid {info.rel_resid_true:.3e}, err vs LU {np.linalg.norm(x - xd) / np.linalg.norm(xd):.3e}")
         assert info.status == _lib.KM_OK and io["status"] == krylov.OK
-        assert abs(info.iterations - io["iterations"]) <= 0.1 * io["iterations"] + 5
+        assert abs(info.iterations - io["iterations"]) <= 0.25 * io["iterations"] + 5
         assert info.rel_resid < eps
         assert info.rel_resid_true < 10 * eps
         assert np.linalg.norm(x - xd) <= 1e4 * eps * np.linalg.norm(xd)
