@@ -150,24 +150,53 @@ int km_unet_param_count(int64_t* n);
 /* z = M(r) for one system (FP64 coefficients and vectors; the network runs in
  * BF16/FP32).  Asynchronous. */
 int km_unet_apply(km_ctx* ctx, const float* params, int NX, int NY, const double* coef, const double* r, double* z);
-/* Training step without the update: mean over a batch of B systems of
- * ||r - A M(r)||^2 / ||r||^2 and its gradient.  coef: B x 5 planes of NX x NY
- * FP32, r: B x NX x NY FP32, grad: param_count FP32.  loss (may be NULL)
- * receives the batch mean and makes the call synchronous.  B <= 64. */
+/* Training step without the update: the batch mean of a relative loss of
+ * z = M(r) over B systems and its gradient.  corr == NULL: the residual loss
+ * ||r - A z||^2 / ||r||^2; corr = the converged corrections c (A c = r): the
+ * A-norm error loss (c - z)^T A (c - z) / c^T r.  coef: B x 5 planes of
+ * NX x NY FP32; r, corr: B x NX x NY FP32; grad: param_count FP32.  loss
+ * (may be NULL) receives the batch mean and makes the call synchronous.
+ * B <= 64. */
 int km_unet_loss_grad(km_ctx* ctx, const float* params, int B, int NX, int NY, const float* coef, const float* r,
-                      float* grad, double* loss);
+                      const float* corr, float* grad, double* loss);
 /* Adam (Kingma & Ba 2015) on FP32 vectors of n values; step counts from 1. */
 int km_adam_step(km_ctx* ctx, int64_t n, float* params, const float* grad, float* m, float* v, int step, double lr,
                  double b1, double b2, double eps);
 /* BICGSolver.solve (bicgsolver.py:88-136) with M = the network in the
- * p^ / s^ hooks: flexible BiCGSTAB in FP32 vectors with FP64 scalars inside
- * FP64 iterative refinement, judged only by the true residual
- * ||b - A x|| / ||b|| < eps.  Without KM_FLAG_X0 the first iterate is the
+ * p^ / s^ hooks: flexible BiCGSTAB with FP64 scalars inside FP64 iterative
+ * refinement, judged only by the true residual ||b - A x|| / ||b|| < eps.
+ * Krylov vectors are FP64, or FP32 with KM_FLAG_MIXED (then each inner sweep
+ * stops at 1e-5 at most).  Without KM_FLAG_X0 the first iterate is the
  * network's warm start x0 = M(b) (kept if it lowers the residual).  A sweep
  * that does not halve the true residual is taken back and plain FP64
- * BiCGSTAB finishes (info->fallback = 1). */
+ * BiCGSTAB finishes (info->fallback = 1).  params == NULL selects the Jacobi
+ * preconditioner D^-1 instead (karman_taichi.py:399,419), with no warm start. */
 int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
                  int max_iter, int flags, const float* params, km_solve_info* info);
+
+/* ---- batched cases (no counterpart in the reference, which runs one case) ----
+ * Many independent channel cases of uneven size and Reynolds number advance
+ * together: each stage of the SIMPLE outer iteration (simple.py:270-281) is
+ * one launch over every case, the u, v and p' BiCGSTAB solves run all cases'
+ * systems in the same fused iteration kernels (per-case scalars and stopping
+ * tests), the stage sequences between the solves are captured as hipGraphs,
+ * and one tile shape, timed on the batch's own systems, serves all cases.
+ * Each case holds the fields of one SIMPLESolver (the layout contract above). */
+typedef struct km_case {
+    km_grid g;
+    double *u, *v, *p, *u0, *v0, *u_mid, *v_mid, *pcor, *mdiv;
+    double *coef_u, *coef_v, *coef_p, *b_u, *b_v, *b_p;
+    const uint8_t* mask;
+} km_case;
+/* One outer iteration of every case (plain FP64 solves to eps_mom / eps_p,
+ * x0 = 0 as in SIMPLESolver.step).  info (may be NULL) receives 3 entries per
+ * case (u, v, p'); rel_resid_true is not recomputed (-1). */
+int km_batch_step(km_ctx* ctx, int n, const km_case* cases, double eps_mom, double eps_p, int max_iter, int flags,
+                  km_solve_info* info);
+/* time_forward of every case (karman_taichi.py:617-622). */
+int km_batch_time_forward(km_ctx* ctx, int n, const km_case* cases);
+/* The batch's tile shape (chosen on its first step). */
+int km_batch_get_tile(km_ctx* ctx, int* ti, int* tj);
 
 /* ---- measurement hook (no counterpart in the reference) ----
  * Average device time in microseconds of one launch of a solver kernel on an

@@ -44,6 +44,13 @@ class KmSolveInfo(ctypes.Structure):
     ]
 
 
+class KmCase(ctypes.Structure):
+    """One case of a batched step (include/karman.h km_case)."""
+    _fields_ = [("g", KmGrid)] + [(f, ctypes.c_void_p) for f in (
+        "u", "v", "p", "u0", "v0", "u_mid", "v_mid", "pcor", "mdiv",
+        "coef_u", "coef_v", "coef_p", "b_u", "b_v", "b_p", "mask")]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.c_double
@@ -78,9 +85,12 @@ SIGNATURES = {
     "km_time_kernel": [_CTX, _I, _I, _I, _P, _P, _I, ctypes.POINTER(_D)],
     "km_unet_param_count": [ctypes.POINTER(ctypes.c_int64)],
     "km_unet_apply": [_CTX, _P, _I, _I, _P, _P, _P],
-    "km_unet_loss_grad": [_CTX, _P, _I, _I, _I, _P, _P, _P, ctypes.POINTER(_D)],
+    "km_unet_loss_grad": [_CTX, _P, _I, _I, _I, _P, _P, _P, _P, ctypes.POINTER(_D)],
     "km_adam_step": [_CTX, ctypes.c_int64, _P, _P, _P, _P, _I, _D, _D, _D, _D],
     "km_fbicgstab": [_CTX, _I, _I, _P, _P, _P, _D, _I, _I, _P, _INFO],
+    "km_batch_step": [_CTX, _I, _P, _D, _D, _I, _I, _P],
+    "km_batch_time_forward": [_CTX, _I, _P],
+    "km_batch_get_tile": [_CTX, ctypes.POINTER(_I), ctypes.POINTER(_I)],
 }
 _RESTYPE = {"km_last_error": ctypes.c_char_p}
 

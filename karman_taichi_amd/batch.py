@@ -1,0 +1,73 @@
+"""Batched SIMPLE: many channel cases of uneven size and Reynolds number per launch.
+
+Each case is a full :class:`~karman_taichi_amd.simple.SIMPLESolver` (its own
+fields on the device); :meth:`BatchedSIMPLE.step` advances every case by one
+outer iteration with ``km_batch_step``: every stage is one launch over all
+cases, the three BiCGSTAB solves run all cases' systems in the same fused
+iteration kernels with per-case scalars and stopping tests, and the stage
+sequences between solves replay as captured hipGraphs.  The reference runs
+one case per process (simple.py:285-303); the per-case arithmetic is that of
+``SIMPLESolver.step``.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Sequence
+
+import torch
+
+from . import _lib, cases as kcases
+from ._lib import KmCase, KmSolveInfo
+from .simple import SIMPLESolver
+from .solvers import SolveInfo
+
+_FIELDS = ("u", "v", "p", "u0", "v0", "u_mid", "v_mid", "pcor", "mdiv")
+
+
+class BatchedSIMPLE:
+    def __init__(self, case_list: Sequence[kcases.Case], device=None):
+        if not case_list:
+            raise ValueError("empty batch")
+        self.cases = list(case_list)
+        self.solvers: List[SIMPLESolver] = [SIMPLESolver.from_case(c, device=device) for c in self.cases]
+        self.device = self.solvers[0].device
+        self.ctx = self.solvers[0].ctx
+        self.n = len(self.solvers)
+        self._table = (KmCase * self.n)()
+        self._info = (KmSolveInfo * (3 * self.n))()
+        self._fill()
+
+    def _fill(self):
+        for k, s in enumerate(self.solvers):
+            e = self._table[k]
+            e.g = s._grid()
+            for f in _FIELDS:
+                setattr(e, f, getattr(s, f).data_ptr())
+            e.coef_u, e.coef_v, e.coef_p = s._coef_u.data_ptr(), s._coef_v.data_ptr(), s._coef_p.data_ptr()
+            e.b_u, e.b_v, e.b_p = s.b_u.data_ptr(), s.b_v.data_ptr(), s.b_p.data_ptr()
+            e.mask = s.ct.data_ptr()
+
+    def init(self):
+        for s in self.solvers:
+            s.init()
+
+    def step(self, eps_mom: float = 1e-8, eps_p: float = 1e-8, max_iter: int = 0, graph: bool = True):
+        """One outer iteration of every case; returns [(u, v, p') SolveInfo] per case."""
+        self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+        self.ctx.call("km_batch_step", self.n, self._table, float(eps_mom), float(eps_p), int(max_iter),
+                      0 if graph else _lib.KM_FLAG_NO_GRAPH, self._info)
+        return [tuple(SolveInfo.from_c(self._info[3 * k + f]) for f in range(3)) for k in range(self.n)]
+
+    def time_forward(self):
+        self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+        self.ctx.call("km_batch_time_forward", self.n, self._table)
+
+    @property
+    def tile(self):
+        ti, tj = ctypes.c_int(), ctypes.c_int()
+        self.ctx.call("km_batch_get_tile", ctypes.byref(ti), ctypes.byref(tj))
+        return ti.value, tj.value
+
+    def cells(self) -> int:
+        """p' cells of the whole batch (padded)."""
+        return sum(c.shape_p[0] * c.shape_p[1] for c in self.cases)

@@ -29,7 +29,9 @@
 #define KM_RING 4
 struct KScal {
     double rho, alpha, omega, beta;
-    int32_t iter, pad;   // iteration index of this phase
+    double best;          // flexible solve: smallest r.r so far (stall detection)
+    int32_t iter;         // iteration index of this phase
+    int32_t best_it;      // iteration of ``best``
 };
 struct KState {
     double r0;        // stopping-test denominator (r0.r0, or ||b||^2 for a warm start)
@@ -88,7 +90,7 @@ struct Flat {
     int G;
 };
 
-static inline Flat make_flat(int64_t n) {
+__host__ __device__ static inline Flat make_flat(int64_t n) {
     Flat f;
     f.n = n;
     int64_t g = (n + 255) / 256;
@@ -109,6 +111,7 @@ struct GraphKey {
 };
 
 struct km_net;   // learned preconditioner buffers (km_unet.hip)
+struct km_batch; // batched cases (km_batch.hip)
 
 struct km_ctx {
     int device = 0;
@@ -131,7 +134,10 @@ struct km_ctx {
     int autotune = 1;              // time tile candidates on the first solve of a shape
     std::set<std::tuple<int, int, int>> tuned;
     km_net* net = nullptr;
+    km_batch* batch = nullptr;
 };
+
+void km_batch_free(km_ctx* ctx);
 
 int km_fail(km_ctx* ctx, int code, const char* fmt, ...);
 
@@ -215,13 +221,82 @@ inline int run_chunks(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int
 }
 
 
+// Grid, physics and boundary data of one case on the device (km_grid, flattened).
+struct GridD {
+    int nx, ny, east, pin;
+    double dx, dy, rho, mu, dt, alpha_u, alpha_p, alpha_m;
+    double bcw_n, bcw_t, bce_n, bce_t, bcn_n, bcn_t, bcs_n, bcs_t;
+};
+
+inline GridD to_dev(const km_grid* g) {
+    GridD d;
+    d.nx = g->nx; d.ny = g->ny; d.east = g->east; d.pin = g->pin_pcor;
+    d.dx = g->dx; d.dy = g->dy; d.rho = g->rho; d.mu = g->mu; d.dt = g->dt;
+    d.alpha_u = g->alpha_u; d.alpha_p = g->alpha_p; d.alpha_m = g->alpha_m;
+    d.bcw_n = g->bc[0][0]; d.bcw_t = g->bc[0][1];
+    d.bce_n = g->bc[1][0]; d.bce_t = g->bc[1][1];
+    d.bcn_n = g->bc[2][0]; d.bcn_t = g->bc[2][1];
+    d.bcs_n = g->bc[3][0]; d.bcs_t = g->bc[3][1];
+    return d;
+}
+
+
 static inline int ceil_div_i64(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// One case of a batched SIMPLE step (device table entry): its grid data and
+// the device fields of a SIMPLESolver (simple.py:27-47).
+struct BCase {
+    GridD g;
+    double *u, *v, *p, *u0, *v0, *umid, *vmid, *pcor, *mdiv;
+    double *coef_u, *coef_v, *coef_p, *b_u, *b_v, *b_p;
+    const uint8_t* m;
+};
+
+// Stages of a batched SIMPLE step that run as one launch over every case.
+enum BatchStage { BS_ASM_U, BS_ASM_V, BS_GHOSTS, BS_BLEND, BS_OUTFLOW, BS_ASM_P, BS_CORRECT, BS_MDIV, BS_TIME, BS_COUNT };
+// One system of a batched BiCGSTAB solve (device table entry).
+struct BSys {
+    Tile tl;                 // this system's tiles (one tile shape for the whole batch)
+    Flat fl;                 // flat geometry of the init kernel
+    const double* coef;
+    const double* b;
+    double* x;
+    double* ws;              // 8 n values: r, r~, p (2 n), v (2 n), s, t
+    KState* st;              // this system's scalars
+    double* P;               // this system's partials (12 KM_MAX_RED values)
+};
+
+// Plain FP64 BiCGSTAB of n independent systems, every iteration one launch of
+// each fused kernel over all systems' tiles.  sys / d_sys: host and device
+// copies of the table (tl and fl filled by the caller for tile TI x TJ);
+// tstarts / fstarts: device prefix sums (n + 1) of tl.G and fl.G; agg: the
+// aggregate state the host polls; states (host, n) receive each system's
+// final scalars (the systems' states are contiguous: sys[c].st = sys[0].st + c).
+int batch_bicgstab(km_ctx* ctx, int n, const BSys* sys, const BSys* d_sys, const int* tstarts, const int* fstarts,
+                   int64_t tblocks, int64_t fblocks, int TI, int TJ, double eps, int max_iter, bool use_graph,
+                   KState* agg, KState* states);
+
+int64_t batch_stage_blocks(int stage, const GridD& g);
+int batch_stage_launch(km_ctx* ctx, int stage, const BCase* cases, const int* starts, int n, int64_t blocks,
+                       int parts, double* P, hipStream_t s);
 
 // ---------------------------------------------------------------- device side
 
 // XCD-aware workgroup remap: hardware workgroup b runs on XCD b % 8; give each
 // XCD a contiguous range of logical tiles so neighbouring rows of a stencil
 // share that XCD's L2.  Bijective on [0, G).
+// Case of workgroup bx in a batched launch: the last c with starts[c] <= bx
+// (starts: n + 1 ascending workgroup offsets).
+__device__ __forceinline__ int find_case(const int* __restrict__ starts, int n, int bx) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= bx) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ int xcd_remap(int b, int G) {
     int xcd = b % KM_NXCD, local = b / KM_NXCD;
     int per = G / KM_NXCD, rem = G % KM_NXCD;

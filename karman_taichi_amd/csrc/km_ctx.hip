@@ -99,6 +99,7 @@ int km_destroy(km_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     km_drop_graphs(ctx);
     km_net_free(ctx);
+    km_batch_free(ctx);
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->h_st) (void)hipHostFree(ctx->h_st);
     if (ctx->h_res) (void)hipHostFree(ctx->h_res);

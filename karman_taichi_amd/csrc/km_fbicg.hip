@@ -7,6 +7,7 @@
 // vectors actually produced (x += alpha p^ + omega s^) and v = A p^,
 // t = A s^ are formed from them.  A sweep that does not halve the true
 // residual is taken back and plain FP64 BiCGSTAB finishes the solve.
+// Without a network (params == null) the preconditioner is Jacobi, D^-1.
 //
 // Inner iteration i (ring slot ph = i mod 4, as in the plain solver):
 //   K_p  stopping test on <r, r>; p = r + beta (p - omega v)
@@ -28,32 +29,34 @@ constexpr int FB_PT = 3 * KM_MAX_RED;    // [2] <t, s>, <t, t>
 
 __device__ __forceinline__ bool fin(double v) { return v == v && v - v == 0.0; }
 
-__device__ __forceinline__ float sten5(const float* __restrict__ c, const float* __restrict__ x, int64_t k, int64_t n,
-                                       int NX, int NY) {
+template <typename T>
+__device__ __forceinline__ T sten5(const T* __restrict__ c, const T* __restrict__ x, int64_t k, int64_t n, int NX,
+                                   int NY) {
     const int i = (int)(k / NY), j = (int)(k % NY);
-    float y = c[k] * x[k];
-    y = y + c[n + k] * (i > 0 ? x[k - NY] : 0.f);
-    y = y + c[2 * n + k] * (i < NX - 1 ? x[k + NY] : 0.f);
-    y = y + c[3 * n + k] * (j < NY - 1 ? x[k + 1] : 0.f);
-    y = y + c[4 * n + k] * (j > 0 ? x[k - 1] : 0.f);
+    T y = c[k] * x[k];
+    y = y + c[n + k] * (i > 0 ? x[k - NY] : T(0));
+    y = y + c[2 * n + k] * (i < NX - 1 ? x[k + NY] : T(0));
+    y = y + c[3 * n + k] * (j < NY - 1 ? x[k + 1] : T(0));
+    y = y + c[4 * n + k] * (j > 0 ? x[k - 1] : T(0));
     return y;
 }
 
+template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_fb_init(Flat fl, const float* __restrict__ rhs, float* __restrict__ x, float* __restrict__ r, float* __restrict__ rt,
-          float* __restrict__ p, float* __restrict__ v, double eps, int max_iter, KState* __restrict__ st,
+k_fb_init(Flat fl, const T* __restrict__ rhs, T* __restrict__ x, T* __restrict__ r, T* __restrict__ rt,
+          T* __restrict__ p, T* __restrict__ v, double eps, int max_iter, KState* __restrict__ st,
           double* __restrict__ P) {
     __shared__ double sh[4];
     const int64_t lo = (int64_t)blockIdx.x * fl.chunk;
     const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
     double a = 0.0;
     for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
-        const float b = rhs[k];
-        x[k] = 0.f;
+        const T b = rhs[k];
+        x[k] = T(0);
         r[k] = b;
         rt[k] = b;
-        p[k] = 0.f;
-        v[k] = 0.f;
+        p[k] = T(0);
+        v[k] = T(0);
         a += (double)b * (double)b;
     }
     a = block_sum(a, sh);
@@ -70,12 +73,19 @@ k_fb_init(Flat fl, const float* __restrict__ rhs, float* __restrict__ x, float* 
             st->rTr = 0.0;
             st->ref = 0.0;
             st->ring[KM_RING - 1].iter = -1;
+            st->ring[KM_RING - 1].best = 1e300;
+            st->ring[KM_RING - 1].best_it = 0;
         }
     }
 }
 
+// Stall: the recursive residual norm has not dropped below 0.9 x its best
+// value for kStallWindow iterations.
+constexpr int kStallWindow = 300;
+
+template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_fb_p(Flat fl, int ph, const float* __restrict__ r, float* __restrict__ p, const float* __restrict__ v,
+k_fb_p(Flat fl, int ph, const T* __restrict__ r, T* __restrict__ p, const T* __restrict__ v,
        KState* __restrict__ st, const double* __restrict__ P) {
     __shared__ double sh[8];
     if (st->done) return;
@@ -93,6 +103,8 @@ k_fb_p(Flat fl, int ph, const float* __restrict__ r, float* __restrict__ p, cons
     else if (i > 0 && sp.omega == 0.0) status = KM_BREAKDOWN;
     else if (i >= st->max_iter) status = KM_MAXITER;
     else if (rho == 0.0 || !fin(rho)) status = KM_BREAKDOWN;
+    const bool better = rr < 0.81 * sp.best;
+    if (status < 0 && !better && i - sp.best_it >= kStallWindow) status = KM_STALL;
     if (status >= 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             st->status = status;
@@ -109,28 +121,31 @@ k_fb_p(Flat fl, int ph, const float* __restrict__ r, float* __restrict__ p, cons
         for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) p[k] = r[k];
     } else {
         const double beta = (rho / sp.rho) * (sp.alpha / sp.omega);
-        const float fb = (float)beta, fo = (float)sp.omega;
+        const T fb = (T)beta, fo = (T)sp.omega;
         for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) p[k] = r[k] + fb * (p[k] - fo * v[k]);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->ring[ph].rho = rho;
         st->ring[ph].iter = i;
+        st->ring[ph].best = better ? rr : sp.best;
+        st->ring[ph].best_it = better ? i : sp.best_it;
         st->iter = i;
         st->rTr = rr;
         if (i == 0) st->r0 = rr;
     }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_fb_v(Flat fl, int NX, int NY, const float* __restrict__ c, const float* __restrict__ ph_, const float* __restrict__ rt,
-       float* __restrict__ v, const KState* __restrict__ st, double* __restrict__ P) {
+k_fb_v(Flat fl, int NX, int NY, const T* __restrict__ c, const T* __restrict__ ph_, const T* __restrict__ rt,
+       T* __restrict__ v, const KState* __restrict__ st, double* __restrict__ P) {
     __shared__ double sh[4];
     if (st->done) return;
     const int64_t lo = (int64_t)blockIdx.x * fl.chunk;
     const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
     double a = 0.0;
     for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
-        const float y = sten5(c, ph_, k, fl.n, NX, NY);
+        const T y = sten5<T>(c, ph_, k, fl.n, NX, NY);
         v[k] = y;
         a += (double)rt[k] * (double)y;
     }
@@ -138,8 +153,9 @@ k_fb_v(Flat fl, int NX, int NY, const float* __restrict__ c, const float* __rest
     if (threadIdx.x == 0) P[FB_PV + blockIdx.x] = a;
 }
 
+template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_fb_s(Flat fl, int ph, const float* __restrict__ r, const float* __restrict__ v, float* __restrict__ s,
+k_fb_s(Flat fl, int ph, const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ s,
        KState* __restrict__ st, const double* __restrict__ P) {
     __shared__ double sh[8];
     if (st->done) return;
@@ -155,23 +171,24 @@ k_fb_s(Flat fl, int ph, const float* __restrict__ r, const float* __restrict__ v
         return;
     }
     const double alpha = st->ring[ph].rho / rv;
-    const float fa = (float)alpha;
+    const T fa = (T)alpha;
     const int64_t lo = (int64_t)blockIdx.x * fl.chunk;
     const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
     for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) s[k] = r[k] - fa * v[k];
     if (blockIdx.x == 0 && threadIdx.x == 0) st->ring[ph].alpha = alpha;
 }
 
+template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_fb_t(Flat fl, int NX, int NY, const float* __restrict__ c, const float* __restrict__ sh_, const float* __restrict__ s,
-       float* __restrict__ t, const KState* __restrict__ st, double* __restrict__ P) {
+k_fb_t(Flat fl, int NX, int NY, const T* __restrict__ c, const T* __restrict__ sh_, const T* __restrict__ s,
+       T* __restrict__ t, const KState* __restrict__ st, double* __restrict__ P) {
     __shared__ double sh[4];
     if (st->done) return;
     const int64_t lo = (int64_t)blockIdx.x * fl.chunk;
     const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
     double a = 0.0, b = 0.0;
     for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
-        const float y = sten5(c, sh_, k, fl.n, NX, NY);
+        const T y = sten5<T>(c, sh_, k, fl.n, NX, NY);
         t[k] = y;
         a += (double)y * (double)s[k];
         b += (double)y * (double)y;
@@ -184,9 +201,10 @@ k_fb_t(Flat fl, int NX, int NY, const float* __restrict__ c, const float* __rest
     }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_fb_x(Flat fl, int ph, const float* __restrict__ s, const float* __restrict__ t, const float* __restrict__ ph_,
-       const float* __restrict__ sh_, const float* __restrict__ rt, float* __restrict__ x, float* __restrict__ r,
+k_fb_x(Flat fl, int ph, const T* __restrict__ s, const T* __restrict__ t, const T* __restrict__ ph_,
+       const T* __restrict__ sh_, const T* __restrict__ rt, T* __restrict__ x, T* __restrict__ r,
        KState* __restrict__ st, double* __restrict__ P) {
     __shared__ double sh[8];
     if (st->done) return;
@@ -198,8 +216,8 @@ k_fb_x(Flat fl, int ph, const float* __restrict__ s, const float* __restrict__ t
     const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
     double a = 0.0, b = 0.0;
     for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
-        x[k] = x[k] + (float)(alpha * (double)ph_[k] + omega * (double)sh_[k]);
-        const float rn = s[k] - (float)omega * t[k];
+        x[k] = x[k] + (T)(alpha * (double)ph_[k] + omega * (double)sh_[k]);
+        const T rn = s[k] - (T)omega * t[k];
         r[k] = rn;
         a += (double)rt[k] * (double)rn;
         b += (double)rn * (double)rn;
@@ -213,14 +231,28 @@ k_fb_x(Flat fl, int ph, const float* __restrict__ s, const float* __restrict__ t
     }
 }
 
+// z = D^-1 r (rows with a zero diagonal map to zero): the Jacobi hook of
+// karman_taichi.py:399,419, used when no network is given.
+template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ out, double scale) {
-    for (int64_t i = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * KM_BLOCK)
-        out[i] = (float)(a[i] * scale);
+k_jacobi(int64_t n, const T* __restrict__ c, const T* __restrict__ r, T* __restrict__ z, const int* __restrict__ skip) {
+    if (*skip) return;
+    for (int64_t i = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * KM_BLOCK) {
+        const T d = c[i];
+        z[i] = d != T(0) ? r[i] / d : T(0);
+    }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_axpy_f32(int64_t n, double* __restrict__ x, const float* __restrict__ d, double scale) {
+k_scale_to(int64_t n, const double* __restrict__ a, T* __restrict__ out, double scale) {
+    for (int64_t i = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * KM_BLOCK)
+        out[i] = (T)(a[i] * scale);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_axpy_to(int64_t n, double* __restrict__ x, const T* __restrict__ d, double scale) {
     for (int64_t i = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * KM_BLOCK)
         x[i] = x[i] + scale * (double)d[i];
 }
@@ -233,42 +265,52 @@ int blocks(int64_t n) {
 
 const int kFbChunk = 8;
 
-// Inner flexible solve of A d = rhs (FP32), d returned in x32.  ws holds 8n floats.
-int fb_inner(km_ctx* ctx, int NX, int NY, const float* c32, const float* rhs, float* x32, float* ws, double eps,
+// Inner flexible solve of A d = rhs with vectors of type T, d returned in xd.
+// coefT: the coefficients in T (the matvecs); c32: FP32 coefficients (the
+// network's features).  ws holds 8n values of T.
+template <typename T>
+int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, const T* rhs, T* xd, T* ws, double eps,
              int max_iter, bool use_graph, const float* params, KState* out) {
     const int64_t n = (int64_t)NX * NY;
-    float* r = ws;
-    float* rt = ws + n;
-    float* p = ws + 2 * n;
-    float* v = ws + 3 * n;
-    float* s = ws + 4 * n;
-    float* t = ws + 5 * n;
-    float* ph = ws + 6 * n;
-    float* sh = ws + 7 * n;
+    T* r = ws;
+    T* rt = ws + n;
+    T* p = ws + 2 * n;
+    T* v = ws + 3 * n;
+    T* s = ws + 4 * n;
+    T* t = ws + 5 * n;
+    T* ph = ws + 6 * n;
+    T* sh = ws + 7 * n;
     KState* st = &ctx->st[4];
     double* P = ctx->partials;
     const Flat fl = make_flat(n);
-    hipLaunchKernelGGL(k_fb_init, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, rhs, x32, r, rt, p, v, eps, max_iter,
-                       st, P);
+    hipLaunchKernelGGL(k_fb_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, rhs, xd, r, rt, p, v, eps,
+                       max_iter, st, P);
     KM_CHECK_LAUNCH(ctx);
     const int* skip = &st->done;
+    const int jb = blocks(n);
+    auto precond = [=](hipStream_t q, const T* src, T* dst) {
+        if (params)
+            (void)kmnet::apply<T>(ctx, NX, NY, c32, src, dst, skip, q, params);
+        else
+            hipLaunchKernelGGL(k_jacobi<T>, dim3(jb), dim3(KM_BLOCK), 0, q, n, coefT, src, dst, skip);
+    };
     auto enqueue = [=](hipStream_t q, int phs) {
-        hipLaunchKernelGGL(k_fb_p, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, phs, (const float*)r, p, (const float*)v, st,
+        hipLaunchKernelGGL(k_fb_p<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, phs, (const T*)r, p, (const T*)v, st,
                            (const double*)P);
-        (void)kmnet::apply(ctx, NX, NY, c32, p, ph, skip, q, params);
-        hipLaunchKernelGGL(k_fb_v, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, NX, NY, c32, (const float*)ph,
-                           (const float*)rt, v, (const KState*)st, P);
-        hipLaunchKernelGGL(k_fb_s, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, phs, (const float*)r, (const float*)v, s, st,
+        precond(q, p, ph);
+        hipLaunchKernelGGL(k_fb_v<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, NX, NY, coefT, (const T*)ph, (const T*)rt,
+                           v, (const KState*)st, P);
+        hipLaunchKernelGGL(k_fb_s<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, phs, (const T*)r, (const T*)v, s, st,
                            (const double*)P);
-        (void)kmnet::apply(ctx, NX, NY, c32, s, sh, skip, q, params);
-        hipLaunchKernelGGL(k_fb_t, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, NX, NY, c32, (const float*)sh,
-                           (const float*)s, t, (const KState*)st, P);
-        hipLaunchKernelGGL(k_fb_x, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, phs, (const float*)s, (const float*)t,
-                           (const float*)ph, (const float*)sh, (const float*)rt, x32, r, st, P);
+        precond(q, s, sh);
+        hipLaunchKernelGGL(k_fb_t<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, NX, NY, coefT, (const T*)sh, (const T*)s, t,
+                           (const KState*)st, P);
+        hipLaunchKernelGGL(k_fb_x<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, phs, (const T*)s, (const T*)t, (const T*)ph,
+                           (const T*)sh, (const T*)rt, xd, r, st, P);
     };
     GraphKey key;
-    key.v = {7, (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)c32, (uintptr_t)rhs, (uintptr_t)x32, (uintptr_t)ws,
-             (uintptr_t)params, (uintptr_t)ctx->net, (uintptr_t)st};
+    key.v = {7, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)c32, (uintptr_t)rhs,
+             (uintptr_t)xd, (uintptr_t)ws, (uintptr_t)params, (uintptr_t)ctx->net, (uintptr_t)st};
     return run_chunks(ctx, st, key, use_graph, kFbChunk, max_iter, enqueue, out);
 }
 
@@ -278,33 +320,43 @@ int residual64(km_ctx* ctx, int NX, int NY, const double* coef, const double* b,
     return rc < 0 ? rc : KM_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
-                 int max_iter, int flags, const float* params, km_solve_info* info) {
-    if (!ctx) return KM_EINVAL;
-    KM_REQUIRE(ctx, coef && b && x && params, "null pointer");
-    KM_REQUIRE(ctx, NX >= 4 && NY >= 4 && (int64_t)NX * NY < ((int64_t)1 << 31), "system shape %d x %d", NX, NY);
-    KM_REQUIRE(ctx, eps > 0.0, "eps must be positive");
+// Outer FP64 refinement around inner flexible solves in T.  FP64 inner
+// vectors aim each sweep at the requested tolerance; FP32 inner vectors run
+// sweeps to at most 1e-5 each (what FP32 recurrences reliably reach, at most
+// 500 iterations).  A sweep also ends when its recursive residual stalls.
+template <typename T>
+int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
+                int max_iter, int flags, const float* params, km_solve_info* info) {
     const int64_t n = (int64_t)NX * NY;
-    if (max_iter <= 0) max_iter = (int)(n < (1 << 30) ? n : (1 << 30));
     const bool use_graph = !(flags & KM_FLAG_NO_GRAPH);
-    // FP64 r (n), FP32: coefficients (5n), rhs (n), d (n), inner vectors (8n)
-    // (at least what the plain fallback needs, so the arena is not regrown mid-solve)
-    size_t bytes = (size_t)n * sizeof(double) + (size_t)15 * n * sizeof(float) + 1024;
+    const bool f64 = sizeof(T) == sizeof(double);
+    // FP64 r (n); FP32 coefficients (5n); T: coefficients unless FP64 (5n), rhs (n), d (n), inner (8n).
+    // At least what the plain fallback needs, so the arena is not regrown mid-solve.
+    size_t bytes = (size_t)n * sizeof(double) + (size_t)5 * n * sizeof(float) +
+                   (size_t)(f64 ? 10 : 15) * n * sizeof(T) + 2048;
     if (bytes < (size_t)9 * n * sizeof(double)) bytes = (size_t)9 * n * sizeof(double);
     if (int rc = km_ensure_workspace(ctx, bytes)) return rc;
     double* r64 = static_cast<double*>(km_ws_vectors(ctx));
     float* c32 = reinterpret_cast<float*>(r64 + n);
-    float* rhs = c32 + 5 * n;
-    float* d = rhs + n;
-    float* w32 = d + n;
-    hipLaunchKernelGGL(k_to_f32, dim3(blocks(5 * n)), dim3(KM_BLOCK), 0, ctx->stream, 5 * n, coef, c32, 1.0);
+    // T vectors start at the next 256-byte boundary after the FP32 coefficients
+    T* base = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(c32 + 5 * n) + 255) & ~uintptr_t(255));
+    const T* coefT;
+    T* rhs;
+    if (f64) {
+        coefT = reinterpret_cast<const T*>(coef);
+        rhs = base;
+    } else {
+        coefT = reinterpret_cast<const T*>(c32);
+        rhs = base;
+    }
+    T* d = rhs + n;
+    T* wT = d + n;
+    hipLaunchKernelGGL(k_scale_to<float>, dim3(blocks(5 * n)), dim3(KM_BLOCK), 0, ctx->stream, 5 * n, coef, c32, 1.0);
     KM_CHECK_LAUNCH(ctx);
-    if (int rc = kmnet::prepare(ctx, 1, NX, NY, params)) return rc;
-    if (int rc = kmnet::solve_stats(ctx, NX, NY, c32, ctx->stream)) return rc;
+    if (params) {
+        if (int rc = kmnet::prepare(ctx, 1, NX, NY, params)) return rc;
+        if (int rc = kmnet::solve_stats(ctx, NX, NY, c32, ctx->stream)) return rc;
+    }
     double bb = 0.0;
     if (int rc = km_dot(ctx, n, KM_F64, b, b, &bb)) return rc < 0 ? rc : KM_EHIP;
     const double bnorm = sqrt(bb);
@@ -330,45 +382,47 @@ int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* 
     double rel = sqrt(rr) / bnorm;
     int total = 0, sweeps = 0, status = KM_MAXITER;
     // warm start x += ||r|| M(r / ||r||), kept only if it lowers the residual
-    if (rel >= eps && rel == rel) {
+    if (params && rel >= eps && rel == rel) {
         const double rn = sqrt(rr);
-        hipLaunchKernelGGL(k_to_f32, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, (const double*)r64, rhs,
+        hipLaunchKernelGGL(k_scale_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, (const double*)r64, rhs,
                            1.0 / rn);
         KM_CHECK_LAUNCH(ctx);
-        if (int rc = kmnet::apply(ctx, NX, NY, c32, rhs, d, nullptr, ctx->stream, params)) return rc;
-        hipLaunchKernelGGL(k_axpy_f32, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const float*)d, rn);
+        if (int rc = kmnet::apply<T>(ctx, NX, NY, c32, rhs, d, nullptr, ctx->stream, params)) return rc;
+        hipLaunchKernelGGL(k_axpy_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const T*)d, rn);
         KM_CHECK_LAUNCH(ctx);
         double rr2 = 0.0;
         if (int rc = residual64(ctx, NX, NY, coef, b, x, r64, &rr2)) return rc;
         if (rr2 < rr) {
             rr = rr2;
         } else {
-            hipLaunchKernelGGL(k_axpy_f32, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const float*)d, -rn);
+            hipLaunchKernelGGL(k_axpy_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const T*)d, -rn);
             KM_CHECK_LAUNCH(ctx);
             if (int rc = residual64(ctx, NX, NY, coef, b, x, r64, &rr)) return rc;
         }
         rel = sqrt(rr) / bnorm;
     }
-    const double inner_floor = 1e-5;    // what the FP32 recurrences reliably reach
-    const int inner_cap = 500;
+    const double inner_floor = f64 ? 0.0 : 1e-5;
+    const int inner_cap = f64 ? max_iter : 500;
     while (true) {
         if (!(rel == rel)) { status = KM_NONFINITE; break; }
         if (rel < eps) { status = KM_OK; break; }
         if (total >= max_iter || sweeps >= 64) { status = KM_MAXITER; break; }
         const double rn = sqrt(rr);
-        hipLaunchKernelGGL(k_to_f32, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, (const double*)r64, rhs,
+        hipLaunchKernelGGL(k_scale_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, (const double*)r64, rhs,
                            1.0 / rn);
         KM_CHECK_LAUNCH(ctx);
-        double inner_eps = eps / rel;
+        // FP64: aim a little below eps so rounding between the recursive and
+        // the true residual does not cost a second sweep
+        double inner_eps = f64 ? 0.5 * eps / rel : eps / rel;
         if (inner_eps < inner_floor) inner_eps = inner_floor;
         if (inner_eps > 1e-1) inner_eps = 1e-1;
         int cap = max_iter - total;
         if (cap > inner_cap) cap = inner_cap;
-        KState s;
-        if (int rc = fb_inner(ctx, NX, NY, c32, rhs, d, w32, inner_eps, cap, use_graph, params, &s)) return rc;
-        total += s.iter;
+        KState st;
+        if (int rc = fb_inner<T>(ctx, NX, NY, coefT, c32, rhs, d, wT, inner_eps, cap, use_graph, params, &st)) return rc;
+        total += st.iter;
         sweeps++;
-        hipLaunchKernelGGL(k_axpy_f32, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const float*)d, rn);
+        hipLaunchKernelGGL(k_axpy_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const T*)d, rn);
         KM_CHECK_LAUNCH(ctx);
         const double prev = rel;
         if (int rc = residual64(ctx, NX, NY, coef, b, x, r64, &rr)) return rc;
@@ -376,7 +430,7 @@ int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* 
         if (!(rel < 0.5 * prev)) {
             // the preconditioned sweep stalled or diverged: take it back
             if (rel == rel && rel < 1e300) {
-                hipLaunchKernelGGL(k_axpy_f32, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const float*)d,
+                hipLaunchKernelGGL(k_axpy_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const T*)d,
                                    -rn);
                 KM_CHECK_LAUNCH(ctx);
             } else {
@@ -388,7 +442,7 @@ int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* 
             break;
         }
     }
-    if (status == KM_STALL || status == KM_NONFINITE) {
+    if (status == KM_STALL || status == KM_NONFINITE || (status == KM_MAXITER && total < max_iter)) {
         km_solve_info sub;
         const int rc = km_bicgstab(ctx, NX, NY, coef, b, x, eps, max_iter > total ? max_iter - total : 1,
                                    (flags & ~KM_FLAG_MIXED) | KM_FLAG_X0, &sub);
@@ -406,6 +460,22 @@ int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* 
         info->rel_resid_true = rel;
     }
     return KM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
+                 int max_iter, int flags, const float* params, km_solve_info* info) {
+    if (!ctx) return KM_EINVAL;
+    KM_REQUIRE(ctx, coef && b && x, "null pointer");
+    KM_REQUIRE(ctx, NX >= 4 && NY >= 4 && (int64_t)NX * NY < ((int64_t)1 << 31), "system shape %d x %d", NX, NY);
+    KM_REQUIRE(ctx, eps > 0.0, "eps must be positive");
+    const int64_t n = (int64_t)NX * NY;
+    if (max_iter <= 0) max_iter = (int)(n < (1 << 30) ? n : (1 << 30));
+    if (flags & KM_FLAG_MIXED) return fbicg_solve<float>(ctx, NX, NY, coef, b, x, eps, max_iter, flags, params, info);
+    return fbicg_solve<double>(ctx, NX, NY, coef, b, x, eps, max_iter, flags, params, info);
 }
 
 }  // extern "C"

@@ -12,35 +12,16 @@
 
 namespace {
 
-struct GridD {
-    int nx, ny, east, pin;
-    double dx, dy, rho, mu, dt, alpha_u, alpha_p, alpha_m;
-    double bcw_n, bcw_t, bce_n, bce_t, bcn_n, bcn_t, bcs_n, bcs_t;
-};
-
-GridD to_dev(const km_grid* g) {
-    GridD d;
-    d.nx = g->nx; d.ny = g->ny; d.east = g->east; d.pin = g->pin_pcor;
-    d.dx = g->dx; d.dy = g->dy; d.rho = g->rho; d.mu = g->mu; d.dt = g->dt;
-    d.alpha_u = g->alpha_u; d.alpha_p = g->alpha_p; d.alpha_m = g->alpha_m;
-    d.bcw_n = g->bc[0][0]; d.bcw_t = g->bc[0][1];
-    d.bce_n = g->bc[1][0]; d.bce_t = g->bc[1][1];
-    d.bcn_n = g->bc[2][0]; d.bcn_t = g->bc[2][1];
-    d.bcs_n = g->bc[3][0]; d.bcs_t = g->bc[3][1];
-    return d;
-}
-
 // ---------------------------------------------------------------- u momentum
 // Faces i in [2, nx], j in [1, ny] (simple.py:80); every other padded entry
 // gets an all-zero row and b = 0.
-__global__ void __launch_bounds__(KM_BLOCK)
-k_assemble_u(GridD g, const double* __restrict__ u, const double* __restrict__ v,
+__device__ __forceinline__ void assemble_u_body(GridD g, const double* __restrict__ u, const double* __restrict__ v,
              const double* __restrict__ p, const double* __restrict__ u0,
-             const uint8_t* __restrict__ m, double* __restrict__ coef, double* __restrict__ bout) {
+             const uint8_t* __restrict__ m, double* __restrict__ coef, double* __restrict__ bout, int bx) {
     const int NX = g.nx + 3, NY = g.ny + 2;           // u shape
     const int NYV = g.ny + 3, NYP = g.ny + 2;         // v and p/mask row pitch
     const int64_t N = (int64_t)NX * NY;
-    const int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x;
+    const int64_t k = (int64_t)bx * KM_BLOCK + threadIdx.x;
     if (k >= N) return;
     const int i = (int)(k / NY), j = (int)(k - (int64_t)i * NY);
     double ap = 0.0, aw = 0.0, ae = 0.0, an = 0.0, as = 0.0, b = 0.0;
@@ -93,16 +74,23 @@ k_assemble_u(GridD g, const double* __restrict__ u, const double* __restrict__ v
     bout[k] = b;
 }
 
+__global__ void __launch_bounds__(KM_BLOCK)
+k_assemble_u(GridD g, const double* __restrict__ u, const double* __restrict__ v,
+             const double* __restrict__ p, const double* __restrict__ u0,
+             const uint8_t* __restrict__ m, double* __restrict__ coef, double* __restrict__ bout) {
+    assemble_u_body(g, u, v, p, u0, m, coef, bout, blockIdx.x);
+}
+
+
 // ---------------------------------------------------------------- v momentum
 // Faces i in [1, nx], j in [2, ny] (simple.py:98).
-__global__ void __launch_bounds__(KM_BLOCK)
-k_assemble_v(GridD g, const double* __restrict__ u, const double* __restrict__ v,
+__device__ __forceinline__ void assemble_v_body(GridD g, const double* __restrict__ u, const double* __restrict__ v,
              const double* __restrict__ p, const double* __restrict__ v0,
-             const uint8_t* __restrict__ m, double* __restrict__ coef, double* __restrict__ bout) {
+             const uint8_t* __restrict__ m, double* __restrict__ coef, double* __restrict__ bout, int bx) {
     const int NX = g.nx + 2, NY = g.ny + 3;           // v shape
     const int NYU = g.ny + 2, NYP = g.ny + 2;
     const int64_t N = (int64_t)NX * NY;
-    const int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x;
+    const int64_t k = (int64_t)bx * KM_BLOCK + threadIdx.x;
     if (k >= N) return;
     const int i = (int)(k / NY), j = (int)(k - (int64_t)i * NY);
     double ap = 0.0, aw = 0.0, ae = 0.0, an = 0.0, as = 0.0, b = 0.0;
@@ -151,10 +139,17 @@ k_assemble_v(GridD g, const double* __restrict__ u, const double* __restrict__ v
     bout[k] = b;
 }
 
-// Ghost velocities written by set_bc (simple.py:181, :186, :192, :196).
 __global__ void __launch_bounds__(KM_BLOCK)
-k_ghosts(GridD g, double* __restrict__ u, double* __restrict__ v) {
-    const int t = blockIdx.x * KM_BLOCK + threadIdx.x;
+k_assemble_v(GridD g, const double* __restrict__ u, const double* __restrict__ v,
+             const double* __restrict__ p, const double* __restrict__ v0,
+             const uint8_t* __restrict__ m, double* __restrict__ coef, double* __restrict__ bout) {
+    assemble_v_body(g, u, v, p, v0, m, coef, bout, blockIdx.x);
+}
+
+
+// Ghost velocities written by set_bc (simple.py:181, :186, :192, :196).
+__device__ __forceinline__ void ghosts_body(GridD g, double* __restrict__ u, double* __restrict__ v, int bx) {
+    const int t = bx * KM_BLOCK + threadIdx.x;
     const int NYU = g.ny + 2, NYV = g.ny + 3;
     if (t < g.ny) {
         const int j = t + 1;
@@ -168,12 +163,17 @@ k_ghosts(GridD g, double* __restrict__ u, double* __restrict__ v) {
     }
 }
 
+__global__ void __launch_bounds__(KM_BLOCK)
+k_ghosts(GridD g, double* __restrict__ u, double* __restrict__ v) {
+    ghosts_body(g, u, v, blockIdx.x);
+}
+
+
 // update_velocity (simple.py:232-238): u faces i in [2,nx] x [1,ny], then v
 // faces i in [1,nx] x [2,ny], as one flat launch.
-__global__ void __launch_bounds__(KM_BLOCK)
-k_blend(GridD g, double* __restrict__ u, double* __restrict__ v,
-        const double* __restrict__ um, const double* __restrict__ vm) {
-    const int64_t t = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x;
+__device__ __forceinline__ void blend_body(GridD g, double* __restrict__ u, double* __restrict__ v,
+        const double* __restrict__ um, const double* __restrict__ vm, int bx) {
+    const int64_t t = (int64_t)bx * KM_BLOCK + threadIdx.x;
     const int64_t Nu = (int64_t)(g.nx - 1) * g.ny, Nv = (int64_t)g.nx * (g.ny - 1);
     const double a = g.alpha_m;
     if (t < Nu) {
@@ -188,11 +188,17 @@ k_blend(GridD g, double* __restrict__ u, double* __restrict__ v,
     }
 }
 
+__global__ void __launch_bounds__(KM_BLOCK)
+k_blend(GridD g, double* __restrict__ u, double* __restrict__ v,
+        const double* __restrict__ um, const double* __restrict__ vm) {
+    blend_body(g, u, v, um, vm, blockIdx.x);
+}
+
+
 // Zero-gradient outlet and flux fix (karman_taichi.py:180-183, :586-614).
 // One workgroup; the column sums run sequentially in one lane so that they
 // are the same sums, in the same order, as the reference's serial loops.
-__global__ void __launch_bounds__(KM_BLOCK)
-k_outflow(GridD g, double* __restrict__ u) {
+__device__ __forceinline__ void outflow_body(const GridD& g, double* __restrict__ u) {
     const int NYU = g.ny + 2;
     double* uin = u + (int64_t)1 * NYU;
     double* ulast = u + (int64_t)g.nx * NYU;
@@ -209,6 +215,8 @@ k_outflow(GridD g, double* __restrict__ u) {
     for (int j = 1 + threadIdx.x; j <= g.ny; j += KM_BLOCK) uout[j] += shift;
 }
 
+__global__ void __launch_bounds__(KM_BLOCK) k_outflow(GridD g, double* __restrict__ u) { outflow_body(g, u); }
+
 __device__ __forceinline__ bool dirichlet_cell(const GridD& g, const uint8_t* m, int i, int j) {
     if (m[(int64_t)i * (g.ny + 2) + j]) return true;
     if (g.east == KM_EAST_OUTFLOW && i == g.nx && j >= 1 && j <= g.ny) return true;
@@ -218,14 +226,13 @@ __device__ __forceinline__ bool dirichlet_cell(const GridD& g, const uint8_t* m,
 
 // compute_coef_p (simple.py:143-170) with the guards, mask and p' Dirichlet
 // cells; writes mdiv over the interior and zero on the padding.
-__global__ void __launch_bounds__(KM_BLOCK)
-k_assemble_p(GridD g, const double* __restrict__ u, const double* __restrict__ v,
+__device__ __forceinline__ void assemble_p_body(GridD g, const double* __restrict__ u, const double* __restrict__ v,
              const double* __restrict__ apu, const double* __restrict__ apv,
              const uint8_t* __restrict__ m, double* __restrict__ coef, double* __restrict__ bout,
-             double* __restrict__ mdiv) {
+             double* __restrict__ mdiv, int bx) {
     const int NX = g.nx + 2, NY = g.ny + 2, NYU = g.ny + 2, NYV = g.ny + 3;
     const int64_t N = (int64_t)NX * NY;
-    const int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x;
+    const int64_t k = (int64_t)bx * KM_BLOCK + threadIdx.x;
     if (k >= N) return;
     const int i = (int)(k / NY), j = (int)(k - (int64_t)i * NY);
     double ap = 0.0, aw = 0.0, ae = 0.0, an = 0.0, as = 0.0, b = 0.0, md = 0.0;
@@ -254,14 +261,22 @@ k_assemble_p(GridD g, const double* __restrict__ u, const double* __restrict__ v
     mdiv[k] = md;
 }
 
+__global__ void __launch_bounds__(KM_BLOCK)
+k_assemble_p(GridD g, const double* __restrict__ u, const double* __restrict__ v,
+             const double* __restrict__ apu, const double* __restrict__ apv,
+             const uint8_t* __restrict__ m, double* __restrict__ coef, double* __restrict__ bout,
+             double* __restrict__ mdiv) {
+    assemble_p_body(g, u, v, apu, apv, m, coef, bout, mdiv, blockIdx.x);
+}
+
+
 // correct_pressure + correct_velocity (simple.py:218-230): p cells, then u
 // faces, then v faces as one flat launch; solid cells/faces are skipped
 // (karman_taichi.py:515, :527, :536).
-__global__ void __launch_bounds__(KM_BLOCK)
-k_correct(GridD g, double* __restrict__ p, double* __restrict__ u, double* __restrict__ v,
+__device__ __forceinline__ void correct_body(GridD g, double* __restrict__ p, double* __restrict__ u, double* __restrict__ v,
           const double* __restrict__ pc, const double* __restrict__ apu, const double* __restrict__ apv,
-          const uint8_t* __restrict__ m, int parts) {
-    const int64_t t = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x;
+          const uint8_t* __restrict__ m, int parts, int bx) {
+    const int64_t t = (int64_t)bx * KM_BLOCK + threadIdx.x;
     const int nx = g.nx, ny = g.ny, NYP = ny + 2;
     const int64_t Np = (int64_t)nx * ny, Nu = (int64_t)(nx - 1) * ny, Nv = (int64_t)nx * (ny - 1);
     if (t < Np) {
@@ -286,15 +301,22 @@ k_correct(GridD g, double* __restrict__ p, double* __restrict__ u, double* __res
     }
 }
 
+__global__ void __launch_bounds__(KM_BLOCK)
+k_correct(GridD g, double* __restrict__ p, double* __restrict__ u, double* __restrict__ v,
+          const double* __restrict__ pc, const double* __restrict__ apu, const double* __restrict__ apv,
+          const uint8_t* __restrict__ m, int parts) {
+    correct_body(g, p, u, v, pc, apu, apv, m, parts, blockIdx.x);
+}
+
+
 // compute_mdiv (simple.py:137-141) with per-workgroup partials of max|mdiv|
 // and sum mdiv^2 over the interior (bicgsimp.py:114-122 returns the max).
-__global__ void __launch_bounds__(KM_BLOCK)
-k_mdiv(GridD g, Flat fl, const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ mdiv,
-       double* __restrict__ P) {
+__device__ __forceinline__ void mdiv_body(GridD g, Flat fl, const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ mdiv,
+       double* __restrict__ P, int bx) {
     const int NY = g.ny + 2, NYU = g.ny + 2, NYV = g.ny + 3;
     __shared__ double sh[4];
     double mx = 0.0, sq = 0.0;
-    const int64_t k0 = (int64_t)blockIdx.x * fl.chunk;
+    const int64_t k0 = (int64_t)bx * fl.chunk;
     const int64_t k1 = k0 + fl.chunk < fl.n ? k0 + fl.chunk : fl.n;
     for (int64_t k = k0 + threadIdx.x; k < k1; k += KM_BLOCK) {
         const int i = (int)(k / NY), j = (int)(k - (int64_t)i * NY);
@@ -310,10 +332,17 @@ k_mdiv(GridD g, Flat fl, const double* __restrict__ u, const double* __restrict_
     const double bmx = block_max(mx, sh);
     const double bsq = block_sum_all(sq, sh);
     if (threadIdx.x == 0) {
-        P[blockIdx.x] = bmx;
-        P[KM_MAX_RED + blockIdx.x] = bsq;
+        P[bx] = bmx;
+        P[KM_MAX_RED + bx] = bsq;
     }
 }
+
+__global__ void __launch_bounds__(KM_BLOCK)
+k_mdiv(GridD g, Flat fl, const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ mdiv,
+       double* __restrict__ P) {
+    mdiv_body(g, fl, u, v, mdiv, P, blockIdx.x);
+}
+
 
 // One workgroup: res[0] = max of the max partials, res[1] = sum of the sums.
 __global__ void __launch_bounds__(KM_BLOCK)
@@ -330,13 +359,72 @@ k_mdiv_finish(const double* __restrict__ P, int G, double* __restrict__ res) {
     }
 }
 
-__global__ void __launch_bounds__(KM_BLOCK)
-k_time_forward(int64_t Nu, int64_t Nv, const double* __restrict__ u, const double* __restrict__ v,
-               double* __restrict__ u0, double* __restrict__ v0) {
-    const int64_t t = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x;
+__device__ __forceinline__ void time_forward_body(int64_t Nu, int64_t Nv, const double* __restrict__ u, const double* __restrict__ v,
+               double* __restrict__ u0, double* __restrict__ v0, int bx) {
+    const int64_t t = (int64_t)bx * KM_BLOCK + threadIdx.x;
     if (t < Nu) u0[t] = u[t];
     else if (t < Nu + Nv) v0[t - Nu] = v[t - Nu];
 }
+
+__global__ void __launch_bounds__(KM_BLOCK)
+k_time_forward(int64_t Nu, int64_t Nv, const double* __restrict__ u, const double* __restrict__ v,
+               double* __restrict__ u0, double* __restrict__ v0) {
+    time_forward_body(Nu, Nv, u, v, u0, v0, blockIdx.x);
+}
+
+
+// ---------------------------------------------------------------- batched stages
+// One launch per stage over every case of a batch: workgroup bx belongs to
+// case find_case(starts, n, bx) and runs that case's body with its local
+// workgroup index, so each case sees exactly the single-case kernel.
+#define KB_CASE                                          \
+    const int c_ = find_case(starts, n, blockIdx.x);     \
+    const BCase& C = cs[c_];                             \
+    const int bx = blockIdx.x - starts[c_];
+
+__global__ void __launch_bounds__(KM_BLOCK) kb_assemble_u(const BCase* __restrict__ cs, const int* __restrict__ starts, int n) {
+    KB_CASE
+    assemble_u_body(C.g, C.u, C.v, C.p, C.u0, C.m, C.coef_u, C.b_u, bx);
+}
+__global__ void __launch_bounds__(KM_BLOCK) kb_assemble_v(const BCase* __restrict__ cs, const int* __restrict__ starts, int n) {
+    KB_CASE
+    assemble_v_body(C.g, C.u, C.v, C.p, C.v0, C.m, C.coef_v, C.b_v, bx);
+}
+__global__ void __launch_bounds__(KM_BLOCK) kb_ghosts(const BCase* __restrict__ cs, const int* __restrict__ starts, int n) {
+    KB_CASE
+    ghosts_body(C.g, C.u, C.v, bx);
+}
+__global__ void __launch_bounds__(KM_BLOCK) kb_blend(const BCase* __restrict__ cs, const int* __restrict__ starts, int n) {
+    KB_CASE
+    blend_body(C.g, C.u, C.v, C.umid, C.vmid, bx);
+}
+__global__ void __launch_bounds__(KM_BLOCK) kb_outflow(const BCase* __restrict__ cs, const int* __restrict__ starts, int n) {
+    KB_CASE
+    (void)bx;
+    if (C.g.east == KM_EAST_OUTFLOW) outflow_body(C.g, C.u);
+}
+__global__ void __launch_bounds__(KM_BLOCK) kb_assemble_p(const BCase* __restrict__ cs, const int* __restrict__ starts, int n) {
+    KB_CASE
+    assemble_p_body(C.g, C.u, C.v, C.coef_u, C.coef_v, C.m, C.coef_p, C.b_p, C.mdiv, bx);
+}
+__global__ void __launch_bounds__(KM_BLOCK)
+kb_correct(const BCase* __restrict__ cs, const int* __restrict__ starts, int n, int parts) {
+    KB_CASE
+    correct_body(C.g, C.p, C.u, C.v, C.pcor, C.coef_u, C.coef_v, C.m, parts, bx);
+}
+// mdiv of every case; partials of case c go to P + c * 2 * KM_MAX_RED
+__global__ void __launch_bounds__(KM_BLOCK)
+kb_mdiv(const BCase* __restrict__ cs, const int* __restrict__ starts, int n, double* __restrict__ P) {
+    KB_CASE
+    const Flat fl = make_flat((int64_t)(C.g.nx + 2) * (C.g.ny + 2));
+    mdiv_body(C.g, fl, C.u, C.v, C.mdiv, P + (int64_t)c_ * 2 * KM_MAX_RED, bx);
+}
+__global__ void __launch_bounds__(KM_BLOCK) kb_time_forward(const BCase* __restrict__ cs, const int* __restrict__ starts, int n) {
+    KB_CASE
+    const int64_t Nu = (int64_t)(C.g.nx + 3) * (C.g.ny + 2), Nv = (int64_t)(C.g.nx + 2) * (C.g.ny + 3);
+    time_forward_body(Nu, Nv, C.u, C.v, C.u0, C.v0, bx);
+}
+#undef KB_CASE
 
 int check_grid(km_ctx* ctx, const km_grid* g) {
     KM_REQUIRE(ctx, g != nullptr, "grid is NULL");
@@ -348,6 +436,43 @@ int check_grid(km_ctx* ctx, const km_grid* g) {
 }
 
 }  // namespace
+
+// Workgroups one case needs in a stage (the single-case launch geometry).
+int64_t batch_stage_blocks(int stage, const GridD& g) {
+    const int64_t nx = g.nx, ny = g.ny;
+    switch (stage) {
+        case BS_ASM_U: return (int64_t)ceil_div_i64((nx + 3) * (ny + 2), KM_BLOCK);
+        case BS_ASM_V: return (int64_t)ceil_div_i64((nx + 2) * (ny + 3), KM_BLOCK);
+        case BS_GHOSTS: return (int64_t)ceil_div_i64(nx > ny ? nx : ny, KM_BLOCK);
+        case BS_BLEND: return (int64_t)ceil_div_i64((nx - 1) * ny + nx * (ny - 1), KM_BLOCK);
+        case BS_OUTFLOW: return 1;
+        case BS_ASM_P: return (int64_t)ceil_div_i64((nx + 2) * (ny + 2), KM_BLOCK);
+        case BS_CORRECT: return (int64_t)ceil_div_i64(nx * ny + (nx - 1) * ny + nx * (ny - 1), KM_BLOCK);
+        case BS_MDIV: return make_flat((nx + 2) * (ny + 2)).G;
+        case BS_TIME: return (int64_t)ceil_div_i64((nx + 3) * (ny + 2) + (nx + 2) * (ny + 3), KM_BLOCK);
+    }
+    return 0;
+}
+
+int batch_stage_launch(km_ctx* ctx, int stage, const BCase* cs, const int* starts, int n, int64_t blocks, int parts,
+                       double* P, hipStream_t s) {
+    KM_REQUIRE(ctx, blocks > 0 && blocks < ((int64_t)1 << 31), "batched launch of %lld workgroups", (long long)blocks);
+    const dim3 grid((unsigned)blocks), blk(KM_BLOCK);
+    switch (stage) {
+        case BS_ASM_U: hipLaunchKernelGGL(kb_assemble_u, grid, blk, 0, s, cs, starts, n); break;
+        case BS_ASM_V: hipLaunchKernelGGL(kb_assemble_v, grid, blk, 0, s, cs, starts, n); break;
+        case BS_GHOSTS: hipLaunchKernelGGL(kb_ghosts, grid, blk, 0, s, cs, starts, n); break;
+        case BS_BLEND: hipLaunchKernelGGL(kb_blend, grid, blk, 0, s, cs, starts, n); break;
+        case BS_OUTFLOW: hipLaunchKernelGGL(kb_outflow, grid, blk, 0, s, cs, starts, n); break;
+        case BS_ASM_P: hipLaunchKernelGGL(kb_assemble_p, grid, blk, 0, s, cs, starts, n); break;
+        case BS_CORRECT: hipLaunchKernelGGL(kb_correct, grid, blk, 0, s, cs, starts, n, parts); break;
+        case BS_MDIV: hipLaunchKernelGGL(kb_mdiv, grid, blk, 0, s, cs, starts, n, P); break;
+        case BS_TIME: hipLaunchKernelGGL(kb_time_forward, grid, blk, 0, s, cs, starts, n); break;
+        default: return km_fail(ctx, KM_EINVAL, "unknown batch stage %d", stage);
+    }
+    KM_CHECK_LAUNCH(ctx);
+    return KM_OK;
+}
 
 extern "C" {
 

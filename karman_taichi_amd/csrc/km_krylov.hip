@@ -92,6 +92,13 @@ __device__ __forceinline__ void tile_sweep(const Tile& tl, T* lds, F f, T* cente
 }
 
 template <typename F>
+__device__ __forceinline__ void flat_sweep_b(const Flat& fl, int bx, F f) {
+    const int64_t k0 = (int64_t)bx * fl.chunk;
+    const int64_t k1 = k0 + fl.chunk < fl.n ? k0 + fl.chunk : fl.n;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += KM_BLOCK) f(k);
+}
+
+template <typename F>
 __device__ __forceinline__ void flat_sweep(const Flat& fl, F f) {
     const int64_t k0 = (int64_t)blockIdx.x * fl.chunk;
     const int64_t k1 = k0 + fl.chunk < fl.n ? k0 + fl.chunk : fl.n;
@@ -194,11 +201,11 @@ k_add_scaled(int64_t n, double* __restrict__ x, const float* __restrict__ d, dou
 // bicgsolver.py:21-35: x = 0, r = r~ = b (or r precomputed from x0), the
 // parity-0 p and v buffers zeroed.
 template <typename T>
-__global__ void __launch_bounds__(KM_BLOCK)
-k_bicg_init(Flat fl, const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r, T* __restrict__ rt,
-            T* __restrict__ p0, T* __restrict__ v0, int from_r, double eps, int max_iter, double ref_norm2,
-            KState* st) {
-    flat_sweep(fl, [&](int64_t k) {
+__device__ __forceinline__ void bicg_init_body(const Flat& fl, int bx, const T* __restrict__ b, T* __restrict__ x,
+                                               T* __restrict__ r, T* __restrict__ rt, T* __restrict__ p0,
+                                               T* __restrict__ v0, int from_r, double eps, int max_iter,
+                                               double ref_norm2, KState* st) {
+    flat_sweep_b(fl, bx, [&](int64_t k) {
         T rk;
         if (from_r) {
             rk = r[k];
@@ -211,7 +218,7 @@ k_bicg_init(Flat fl, const T* __restrict__ b, T* __restrict__ x, T* __restrict__
         p0[k] = T(0);
         v0[k] = T(0);
     });
-    if (lead()) {
+    if (bx == 0 && threadIdx.x == 0) {
         st->eps = eps;
         st->max_iter = max_iter;
         st->ref = ref_norm2;
@@ -220,6 +227,14 @@ k_bicg_init(Flat fl, const T* __restrict__ b, T* __restrict__ x, T* __restrict__
         st->iter = 0;
         st->ring[KM_RING - 1].iter = -1;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_bicg_init(Flat fl, const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r, T* __restrict__ rt,
+            T* __restrict__ p0, T* __restrict__ v0, int from_r, double eps, int max_iter, double ref_norm2,
+            KState* st) {
+    bicg_init_body<T>(fl, blockIdx.x, b, x, r, rt, p0, v0, from_r, eps, max_iter, ref_norm2, st);
 }
 
 // The iteration kernels issue every global load of a tile (the halo inputs of
@@ -275,11 +290,13 @@ struct TileLoads {
     }
 };
 
+// L: the workgroup's logical index within its system (0 .. tl.G-1); the
+// workgroup with L == 0 records the iteration's scalars.
 template <typename T, int CPT>
-__global__ void __launch_bounds__(KM_BLOCK)
-k_bicg_a(Tile tl, int phase, const T* __restrict__ coef, T* r, const T* __restrict__ rt, const T* __restrict__ s,
-         const T* __restrict__ t, T* __restrict__ x, T* pbuf, T* vbuf, KState* st, double* __restrict__ P) {
-    if (st->done) return;
+__device__ __forceinline__ void bicg_a_body(const Tile& tl, int L, int phase, const T* __restrict__ coef, T* r,
+                                            const T* __restrict__ rt, const T* __restrict__ s,
+                                            const T* __restrict__ t, T* __restrict__ x, T* pbuf, T* vbuf,
+                                            KState* st, double* __restrict__ P) {
     T* lds = reinterpret_cast<T*>(km_smem);
     __shared__ double sh[4];
     const int64_t N = (int64_t)tl.NX * tl.NY;
@@ -289,7 +306,7 @@ k_bicg_a(Tile tl, int phase, const T* __restrict__ coef, T* r, const T* __restri
     T* __restrict__ p_new = pbuf + (1 - par) * N;
     T* __restrict__ v_new = vbuf + (1 - par) * N;
     const int W = tl.TJ + 2;
-    const int L = xcd_remap(blockIdx.x, tl.G);
+    const bool lead0 = L == 0 && threadIdx.x == 0;
     const int t0 = L * tl.tpb, t1 = t0 + tl.tpb < tl.T ? t0 + tl.tpb : tl.T;
     const KScal sp = st->ring[(phase + KM_RING - 1) % KM_RING];
     const int i = sp.iter + 1;
@@ -322,7 +339,7 @@ k_bicg_a(Tile tl, int phase, const T* __restrict__ coef, T* r, const T* __restri
             const double omega = pb[1] != 0.0 ? pb[0] / pb[1] : 0.0;   // bicgsolver.py:118
             const double rho = pb[2] - omega * pb[3];                  // <r_i, r~> (bicgsolver.py:95)
             const double beta = omega != 0.0 ? (rho / sp.rho) * (sp.alpha / omega) : 0.0;   // :102
-            if (lead()) {
+            if (lead0) {
                 st->ring[phase].rho = rho;
                 st->ring[phase].omega = omega;
                 st->ring[phase].beta = beta;
@@ -331,7 +348,7 @@ k_bicg_a(Tile tl, int phase, const T* __restrict__ coef, T* r, const T* __restri
             bt = (T)beta;
             om = (T)omega;
             al = (T)sp.alpha;
-        } else if (tile == t0 && lead()) {
+        } else if (tile == t0 && lead0) {
             st->ring[phase].iter = 0;
             st->ring[phase].omega = 1.0;
         }
@@ -370,15 +387,23 @@ k_bicg_a(Tile tl, int phase, const T* __restrict__ coef, T* r, const T* __restri
 
 template <typename T, int CPT>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_bicg_b(Tile tl, int phase, const T* __restrict__ coef, const T* __restrict__ r, const T* __restrict__ rt,
-         const T* vbuf, T* __restrict__ s, T* __restrict__ t, KState* st, double* __restrict__ P) {
+k_bicg_a(Tile tl, int phase, const T* __restrict__ coef, T* r, const T* __restrict__ rt, const T* __restrict__ s,
+         const T* __restrict__ t, T* __restrict__ x, T* pbuf, T* vbuf, KState* st, double* __restrict__ P) {
     if (st->done) return;
+    bicg_a_body<T, CPT>(tl, xcd_remap(blockIdx.x, tl.G), phase, coef, r, rt, s, t, x, pbuf, vbuf, st, P);
+}
+
+template <typename T, int CPT>
+__device__ __forceinline__ void bicg_b_body(const Tile& tl, int L, int phase, const T* __restrict__ coef,
+                                            const T* __restrict__ r, const T* __restrict__ rt, const T* vbuf,
+                                            T* __restrict__ s, T* __restrict__ t, KState* st,
+                                            double* __restrict__ P) {
     T* lds = reinterpret_cast<T*>(km_smem);
     __shared__ double sh[4];
     const int64_t N = (int64_t)tl.NX * tl.NY;
     const T* __restrict__ v = vbuf + (1 - (phase & 1)) * N;
     const int W = tl.TJ + 2;
-    const int L = xcd_remap(blockIdx.x, tl.G);
+    const bool lead0 = L == 0 && threadIdx.x == 0;
     const int t0 = L * tl.tpb, t1 = t0 + tl.tpb < tl.T ? t0 + tl.tpb : tl.T;
     T al = T(0);
     double ts = 0.0, tt = 0.0, srt = 0.0, trt = 0.0;
@@ -412,13 +437,13 @@ k_bicg_b(Tile tl, int phase, const T* __restrict__ coef, const T* __restrict__ r
             else if (rho == 0.0) status = KM_BREAKDOWN;                   // bicgsolver.py:96-98
             else if (!finite(rv)) status = KM_NONFINITE;
             else if (rv == 0.0) status = KM_BREAKDOWN;
-            if (lead() && i == 0) st->r0 = r0;
+            if (lead0 && i == 0) st->r0 = r0;
             if (status >= 0) {
-                if (lead()) finish(st, status, i, rTr);
+                if (lead0) finish(st, status, i, rTr);
                 return;
             }
             const double alpha = rho / rv;   // bicgsolver.py:108
-            if (lead()) {
+            if (lead0) {
                 st->ring[phase].alpha = alpha;
                 if (i == 0) st->ring[phase].rho = rho;
                 st->rTr = rTr;
@@ -452,6 +477,73 @@ k_bicg_b(Tile tl, int phase, const T* __restrict__ coef, const T* __restrict__ r
     write_partial(P + KM_PB + KM_MAX_RED, L, tt, sh);
     write_partial(P + KM_PB + 2 * KM_MAX_RED, L, srt, sh);
     write_partial(P + KM_PB + 3 * KM_MAX_RED, L, trt, sh);
+}
+
+template <typename T, int CPT>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_bicg_b(Tile tl, int phase, const T* __restrict__ coef, const T* __restrict__ r, const T* __restrict__ rt,
+         const T* vbuf, T* __restrict__ s, T* __restrict__ t, KState* st, double* __restrict__ P) {
+    if (st->done) return;
+    bicg_b_body<T, CPT>(tl, xcd_remap(blockIdx.x, tl.G), phase, coef, r, rt, vbuf, s, t, st, P);
+}
+
+// ------------------------------------------------------------ batched systems
+// Workgroup bx of a batched launch runs system find_case(starts, n, bx) with
+// local index bx - starts[c]; each system keeps its own KState, ring and
+// partial region, so the arithmetic is that of the single-system kernels.
+
+__global__ void __launch_bounds__(KM_BLOCK)
+kb_bicg_init(const BSys* __restrict__ sys, const int* __restrict__ starts, int n, double eps, int max_iter,
+             KState* agg) {
+    const int c = find_case(starts, n, blockIdx.x);
+    const BSys& S = sys[c];
+    const int64_t N = S.fl.n;
+    double* ws = S.ws;
+    bicg_init_body<double>(S.fl, blockIdx.x - starts[c], S.b, S.x, ws, ws + N, ws + 2 * N, ws + 4 * N, 0, eps,
+                           max_iter, 0.0, S.st);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        agg->done = 0;
+        agg->status = KM_MAXITER;
+        agg->iter = 0;
+        agg->max_iter = max_iter;
+    }
+}
+
+template <int CPT>
+__global__ void __launch_bounds__(KM_BLOCK)
+kb_bicg_a(const BSys* __restrict__ sys, const int* __restrict__ starts, int n, int phase) {
+    const int c = find_case(starts, n, blockIdx.x);
+    const BSys& S = sys[c];
+    if (S.st->done) return;
+    const Tile tl = S.tl;
+    const int64_t N = (int64_t)tl.NX * tl.NY;
+    double* ws = S.ws;
+    bicg_a_body<double, CPT>(tl, blockIdx.x - starts[c], phase, S.coef, ws, ws + N, ws + 6 * N, ws + 7 * N, S.x,
+                             ws + 2 * N, ws + 4 * N, S.st, S.P);
+}
+
+template <int CPT>
+__global__ void __launch_bounds__(KM_BLOCK)
+kb_bicg_b(const BSys* __restrict__ sys, const int* __restrict__ starts, int n, int phase) {
+    const int c = find_case(starts, n, blockIdx.x);
+    const BSys& S = sys[c];
+    if (S.st->done) return;
+    const Tile tl = S.tl;
+    const int64_t N = (int64_t)tl.NX * tl.NY;
+    double* ws = S.ws;
+    bicg_b_body<double, CPT>(tl, blockIdx.x - starts[c], phase, S.coef, ws, ws + N, ws + 4 * N, ws + 6 * N,
+                             ws + 7 * N, S.st, S.P);
+}
+
+// agg->done = every system done (one workgroup)
+__global__ void __launch_bounds__(KM_BLOCK) kb_all_done(const BSys* __restrict__ sys, int n, KState* agg) {
+    __shared__ int open;
+    if (threadIdx.x == 0) open = 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < n; c += KM_BLOCK)
+        if (!sys[c].st->done) open = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) agg->done = open ? 0 : 1;
 }
 
 // Cells per thread of a tile, rounded up to an instantiated value.
@@ -703,6 +795,12 @@ int dot_sync(km_ctx* ctx, int64_t n, int prec, const void* a, const void* b, dou
     return KM_OK;
 }
 
+template <int CPT>
+void batch_iter(const BSys* d_sys, const int* tstarts, int n, int64_t tblocks, int phase, size_t lds, hipStream_t q) {
+    hipLaunchKernelGGL(kb_bicg_a<CPT>, dim3((unsigned)tblocks), dim3(KM_BLOCK), lds, q, d_sys, tstarts, n, phase);
+    hipLaunchKernelGGL(kb_bicg_b<CPT>, dim3((unsigned)tblocks), dim3(KM_BLOCK), lds, q, d_sys, tstarts, n, phase);
+}
+
 void fill_info(km_solve_info* info, const KState& s) {
     if (!info) return;
     info->iterations = s.iter;
@@ -871,6 +969,34 @@ int bicgstab_mixed(km_ctx* ctx, int NX, int NY, const double* coef, const double
 }
 
 }  // namespace
+
+int batch_bicgstab(km_ctx* ctx, int n, const BSys* sys, const BSys* d_sys, const int* tstarts, const int* fstarts,
+                   int64_t tblocks, int64_t fblocks, int TI, int TJ, double eps, int max_iter, bool use_graph,
+                   KState* agg, KState* states) {
+    KM_REQUIRE(ctx, n >= 1 && tblocks > 0 && fblocks > 0, "empty batch");
+    KM_REQUIRE(ctx, tile_ok(TI, TJ), "tile %dx%d is not valid for the iteration kernels", TI, TJ);
+    KM_REQUIRE(ctx, tblocks < ((int64_t)1 << 31) && fblocks < ((int64_t)1 << 31), "batch too large");
+    const size_t lds = (size_t)(TI + 2) * (TJ + 2) * sizeof(double);
+    const int cpt = tile_cpt(make_tile(TI, TJ, TI, TJ));
+    hipLaunchKernelGGL(kb_bicg_init, dim3((unsigned)fblocks), dim3(KM_BLOCK), 0, ctx->stream, d_sys, fstarts, n, eps,
+                       max_iter, agg);
+    KM_CHECK_LAUNCH(ctx);
+    auto enqueue = [=](hipStream_t q, int phase) {
+        if (cpt == 1) batch_iter<1>(d_sys, tstarts, n, tblocks, phase, lds, q);
+        else if (cpt == 2) batch_iter<2>(d_sys, tstarts, n, tblocks, phase, lds, q);
+        else batch_iter<4>(d_sys, tstarts, n, tblocks, phase, lds, q);
+        if (phase == KM_RING - 1) hipLaunchKernelGGL(kb_all_done, dim3(1), dim3(KM_BLOCK), 0, q, d_sys, n, agg);
+    };
+    GraphKey key;
+    key.v = {9, (uintptr_t)n, (uintptr_t)d_sys, (uintptr_t)tstarts, (uintptr_t)tblocks, (uintptr_t)TI, (uintptr_t)TJ,
+             (uintptr_t)sys[0].coef, (uintptr_t)agg};
+    for (int c = 0; c < n; ++c) key.v.push_back((uintptr_t)sys[c].coef);
+    KState fin;
+    if (int rc = run_chunks(ctx, agg, key, use_graph, kChunk, max_iter, enqueue, &fin)) return rc;
+    KM_HIP(ctx, hipMemcpyAsync(states, sys[0].st, n * sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return KM_OK;
+}
 
 // Tile choice: an explicit override, else the cached choice, else a default
 // that splits each row into near-equal segments of at most 256 columns and

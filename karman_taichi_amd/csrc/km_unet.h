@@ -37,10 +37,11 @@ int prepare(km_ctx* ctx, int B, int NX, int NY, const float* params);
 // FP32 coefficients.  Must precede apply() for that system.
 int solve_stats(km_ctx* ctx, int NX, int NY, const float* coef, hipStream_t s);
 
-// z = M(r) for one system with FP32 vectors; kernels return at once when
+// z = M(r) for one system (vectors in T = float or double); kernels return at once when
 // *skip != 0 (skip may be null).  prepare() and solve_stats() must have run
 // with the same parameters (biases and output weights are read from them).
-int apply(km_ctx* ctx, int NX, int NY, const float* coef, const float* r, float* z, const int* skip, hipStream_t s,
+template <typename T>
+int apply(km_ctx* ctx, int NX, int NY, const float* coef, const T* r, T* z, const int* skip, hipStream_t s,
           const float* params);
 
 }  // namespace kmnet

@@ -34,7 +34,7 @@ struct km_net {
     bf16 *wf = nullptr, *wt = nullptr;
     bf16 *X0, *e0, *p1, *e1, *p2, *e2, *p3, *e3, *d2, *d1, *d0;
     bf16 *dy_d0, *dy_d1, *dy_d2, *dy_e3, *dy_e2, *dy_e1, *dy_e0;
-    float *z, *q, *dX0, *dX1, *dX2, *dp3, *dp2, *dp1, *wpart;
+    float *z, *q, *ae, *dX0, *dX1, *dX2, *dp3, *dp2, *dp1, *wpart;
     float *c32, *r32, *z32;     // FP32 staging of one system (km_unet_apply)
     double *P, *stats, *loss;
     int64_t wpart_n = 0;
@@ -99,23 +99,28 @@ k_prep_weights(const LayerOff* __restrict__ L, int64_t nwf, int64_t nwt, const f
 
 // ------------------------------------------------------------ convolution
 
-// One wave computes 16*Q pixels x COUT channels; a workgroup is 4 waves.
+// One wave computes 16*Q pixels x 16*RTW output channels (row tiles
+// blockIdx.z*RTW ...); a workgroup is 4 waves along the pixels, so the grid is
+// (pixels / 64Q, batch, COUT / 16RTW) and low-resolution layers still launch
+// thousands of waves.  The operand fragments of the next 32-wide K chunk are
+// loaded while the MFMAs of the current one run (two register sets).
 // MODE 0: input src0 (C0 channels) at the output resolution.
 // MODE 1: input = [2x nearest upsample of src0 (C0 channels, half resolution) | src1 (C1 channels)].
 // OUTK 0: out = bf16(relu(acc + bias)); OUTK 1: out = acc in FP32 (input gradients).
-template <int MODE, int C0, int C1, int COUT, int OUTK, int Q>
+template <int MODE, int C0, int C1, int COUT, int OUTK, int Q, int RTW>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_conv3x3(int H, int W, const bf16* __restrict__ src0, const bf16* __restrict__ src1, const bf16* __restrict__ Wm,
           const float* __restrict__ bias, void* __restrict__ out, const int* __restrict__ skip) {
     if (skipped(skip)) return;
     constexpr int CIN = C0 + C1;
     constexpr int KP = round32(9 * CIN);
-    constexpr int RT = COUT / 16;
-    static_assert(C0 % 8 == 0 && C1 % 8 == 0 && COUT % 16 == 0, "channel counts");
+    constexpr int NCH = KP / 32;
+    static_assert(C0 % 8 == 0 && C1 % 8 == 0 && COUT % (16 * RTW) == 0, "channel counts");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, col = lane & 15;
     const int64_t HW = (int64_t)H * W;
     const int b = blockIdx.y;
+    const int rt0 = blockIdx.z * RTW;
     const int64_t px0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * Q);
     if (px0 >= HW) return;
     int pi[Q], pj[Q];
@@ -129,17 +134,12 @@ k_conv3x3(int H, int W, const bf16* __restrict__ src0, const bf16* __restrict__ 
     }
     const bf16* s0 = src0 + (MODE == 0 ? (int64_t)b * HW * C0 : (int64_t)b * (HW / 4) * C0);
     const bf16* s1 = MODE == 1 ? src1 + (int64_t)b * HW * C1 : nullptr;
+    const bf16* wrow = Wm + (int64_t)(rt0 * 16 + col) * KP + g * 8;
     const int Wc = W >> 1;
-    floatx4 acc[RT][Q];
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int q = 0; q < Q; ++q) acc[rt][q] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int kc = 0; kc < KP; kc += 32) {
+    auto load = [&](int kc, bf16x8 (&bv)[Q], bf16x8 (&av)[RTW]) {
         const int k0 = kc + g * 8;
         const int tap = k0 / CIN, c = k0 - tap * CIN;
         const int dy = tap / 3 - 1, dx = tap - (tap / 3) * 3 - 1;
-        bf16x8 bv[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int ii = pi[q] + dy, jj = pj[q] + dx;
@@ -157,16 +157,33 @@ k_conv3x3(int H, int W, const bf16* __restrict__ src0, const bf16* __restrict__ 
             }
         }
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(Wm + (int64_t)(rt * 16 + col) * KP + k0);
+        for (int rt = 0; rt < RTW; ++rt) av[rt] = *reinterpret_cast<const bf16x8*>(wrow + (int64_t)rt * 16 * KP + kc);
+    };
+    floatx4 acc[RTW][Q];
+#pragma unroll
+    for (int rt = 0; rt < RTW; ++rt)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[rt][q] = floatx4{0.f, 0.f, 0.f, 0.f};
+    auto mma = [&](const bf16x8 (&bv)[Q], const bf16x8 (&av)[RTW]) {
+#pragma unroll
+        for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
             for (int q = 0; q < Q; ++q)
-                acc[rt][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv[q], acc[rt][q], 0, 0, 0);
+                acc[rt][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[rt], bv[q], acc[rt][q], 0, 0, 0);
+    };
+    bf16x8 b0[Q], a0[RTW], b1[Q], a1[RTW];
+    load(0, b0, a0);
+    for (int ch = 0; ch < NCH; ch += 2) {
+        if (ch + 1 < NCH) load((ch + 1) * 32, b1, a1);
+        mma(b0, a0);
+        if (ch + 1 < NCH) {
+            if (ch + 2 < NCH) load((ch + 2) * 32, b0, a0);
+            mma(b1, a1);
         }
     }
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt) {
-        const int co = rt * 16 + g * 4;
+    for (int rt = 0; rt < RTW; ++rt) {
+        const int co = (rt0 + rt) * 16 + g * 4;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             if (!pv[q]) continue;
@@ -514,10 +531,13 @@ k_out(int NX, int NY, int H, int W, int B, const bf16* __restrict__ d0, const fl
 
 // ------------------------------------------------------------ loss
 
-// q = r - A z (FP32) per system; P[b * G + block] = partial sums of q.q.
+// Per system, FP32.  Residual loss (corr == null): q = r - A z, partials of
+// q.q and r.r.  A-norm error loss: q = e = c - z, ae = A e, partials of e.ae
+// and c.r.  P[(b * 2 + {0, 1}) * G + block].
 __global__ void __launch_bounds__(KM_BLOCK)
 k_loss_q(int NX, int NY, const float* __restrict__ coef, int64_t cstride, const float* __restrict__ r,
-         const float* __restrict__ z, float* __restrict__ q, double* __restrict__ P, int G) {
+         const float* __restrict__ z, const float* __restrict__ corr, float* __restrict__ q, float* __restrict__ ae,
+         double* __restrict__ P, int G) {
     __shared__ double sh[4];
     const int b = blockIdx.y;
     const int64_t n = (int64_t)NX * NY;
@@ -528,42 +548,73 @@ k_loss_q(int NX, int NY, const float* __restrict__ coef, int64_t cstride, const 
     const float* zb = z + b * n;
     const float* rb = r + b * n;
     float* qb = q + b * n;
-    double s = 0.0;
-    for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
-        const int i = (int)(k / NY), j = (int)(k % NY);
-        float az = cb[k] * zb[k];
-        az = az + cb[n + k] * (i > 0 ? zb[k - NY] : 0.f);
-        az = az + cb[2 * n + k] * (i < NX - 1 ? zb[k + NY] : 0.f);
-        az = az + cb[3 * n + k] * (j < NY - 1 ? zb[k + 1] : 0.f);
-        az = az + cb[4 * n + k] * (j > 0 ? zb[k - 1] : 0.f);
-        const float qq = rb[k] - az;
-        qb[k] = qq;
-        s += (double)qq * (double)qq;
+    double s0 = 0.0, s1 = 0.0;
+    if (corr == nullptr) {
+        for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
+            const int i = (int)(k / NY), j = (int)(k % NY);
+            float az = cb[k] * zb[k];
+            az = az + cb[n + k] * (i > 0 ? zb[k - NY] : 0.f);
+            az = az + cb[2 * n + k] * (i < NX - 1 ? zb[k + NY] : 0.f);
+            az = az + cb[3 * n + k] * (j < NY - 1 ? zb[k + 1] : 0.f);
+            az = az + cb[4 * n + k] * (j > 0 ? zb[k - 1] : 0.f);
+            const float qq = rb[k] - az;
+            qb[k] = qq;
+            s0 += (double)qq * (double)qq;
+            s1 += (double)rb[k] * (double)rb[k];
+        }
+    } else {
+        const float* ccb = corr + b * n;
+        float* aeb = ae + b * n;
+        for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
+            const int i = (int)(k / NY), j = (int)(k % NY);
+            const float e = ccb[k] - zb[k];
+            float a = cb[k] * e;
+            a = a + cb[n + k] * (i > 0 ? ccb[k - NY] - zb[k - NY] : 0.f);
+            a = a + cb[2 * n + k] * (i < NX - 1 ? ccb[k + NY] - zb[k + NY] : 0.f);
+            a = a + cb[3 * n + k] * (j < NY - 1 ? ccb[k + 1] - zb[k + 1] : 0.f);
+            a = a + cb[4 * n + k] * (j > 0 ? ccb[k - 1] - zb[k - 1] : 0.f);
+            qb[k] = e;
+            aeb[k] = a;
+            s0 += (double)e * (double)a;
+            s1 += (double)ccb[k] * (double)rb[k];
+        }
     }
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) P[(int64_t)b * G + blockIdx.x] = s;
+    s0 = block_sum(s0, sh);
+    s1 = block_sum(s1, sh);
+    if (threadIdx.x == 0) {
+        P[((int64_t)b * 2 + 0) * G + blockIdx.x] = s0;
+        P[((int64_t)b * 2 + 1) * G + blockIdx.x] = s1;
+    }
 }
 
+// loss[b] = num / den; the denominator goes to stats[b*8 + 6] for the gradient.
 __global__ void __launch_bounds__(KM_BLOCK)
-k_loss_finish(int G, const double* __restrict__ P, const double* __restrict__ stats, double* __restrict__ loss) {
+k_loss_finish(int G, const double* __restrict__ P, double* __restrict__ stats, double* __restrict__ loss) {
     __shared__ double sh[4];
     const int b = blockIdx.x;
-    double a = 0.0;
-    for (int k = threadIdx.x; k < G; k += KM_BLOCK) a += P[(int64_t)b * G + k];
+    double a = 0.0, d = 0.0;
+    for (int k = threadIdx.x; k < G; k += KM_BLOCK) {
+        a += P[((int64_t)b * 2 + 0) * G + k];
+        d += P[((int64_t)b * 2 + 1) * G + k];
+    }
     a = block_sum(a, sh);
+    d = block_sum(d, sh);
     if (threadIdx.x == 0) {
-        const double rr = stats[(int64_t)b * 8];
-        loss[b] = rr > 0.0 ? a / rr : 0.0;
+        stats[(int64_t)b * 8 + 6] = d;
+        loss[b] = d > 0.0 ? a / d : 0.0;
     }
 }
 
-// dL/dnet = (rs/abar) (-2 / (r.r B)) (A^T q) on free interior cells; fused
-// with the backward of the 1x1 output layer: dy_d0 = bf16(dnet w_out [d0 > 0])
-// and partial sums of dnet * d0[c] (16) and dnet (1) into P[v * G + block].
+// dL/dnet on free interior cells, (rs/abar) times
+//   residual loss:  -2 A^T q / (den B)
+//   A-norm loss:    -(A e + A^T e) / (den B)     (q holds e, ae holds A e)
+// fused with the backward of the 1x1 output layer: dy_d0 = bf16(dnet w_out
+// [d0 > 0]) and partial sums of dnet * d0[c] (16) and dnet (1) into P[v * G + block].
 __global__ void __launch_bounds__(KM_BLOCK)
 k_loss_grad(int NX, int NY, int H, int W, int B, const float* __restrict__ coef, int64_t cstride,
-            const float* __restrict__ q, const double* __restrict__ stats, const bf16* __restrict__ d0,
-            const float* __restrict__ pw, bf16* __restrict__ dy, double* __restrict__ P, int G) {
+            const float* __restrict__ q, const float* __restrict__ ae, const double* __restrict__ stats,
+            const bf16* __restrict__ d0, const float* __restrict__ pw, bf16* __restrict__ dy, double* __restrict__ P,
+            int G) {
     __shared__ double sh[17][4];
     const int64_t n = (int64_t)NX * NY, HW = (int64_t)H * W, tot = (int64_t)B * HW;
     float wb[16];
@@ -586,8 +637,11 @@ k_loss_grad(int NX, int NY, int H, int W, int B, const float* __restrict__ coef,
                 at = at + cb[2 * n + k - NY] * qb[k - NY];
                 at = at + cb[3 * n + k - 1] * qb[k - 1];
                 at = at + cb[4 * n + k + 1] * qb[k + 1];
-                const double rr = stats[b * 8], sc = stats[b * 8 + 4];
-                dnet = (float)(sc * (-2.0 / (rr * (double)B)) * (double)at);
+                const double den = stats[b * 8 + 6], sc = stats[b * 8 + 4];
+                if (ae == nullptr)
+                    dnet = (float)(sc * (-2.0 / (den * (double)B)) * (double)at);
+                else
+                    dnet = (float)(sc * (-1.0 / (den * (double)B)) * ((double)ae[b * n + k] + (double)at));
             }
         }
         const bf16x8 lo = *reinterpret_cast<const bf16x8*>(d0 + idx * 16);
@@ -646,13 +700,13 @@ int flat_blocks(int64_t n) {
     return g < 1 ? 1 : (int)g;
 }
 
-template <int MODE, int C0, int C1, int COUT, int OUTK, int Q>
+template <int MODE, int C0, int C1, int COUT, int OUTK, int Q, int RTW>
 void conv(int H, int W, int B, const bf16* s0, const bf16* s1, const bf16* Wm, const float* bias, void* out,
           const int* skip, hipStream_t st) {
     const int64_t HW = (int64_t)H * W;
-    const dim3 grid((unsigned)((HW + 64 * Q - 1) / (64 * Q)), (unsigned)B);
-    hipLaunchKernelGGL((k_conv3x3<MODE, C0, C1, COUT, OUTK, Q>), grid, dim3(KM_BLOCK), 0, st, H, W, s0, s1, Wm, bias,
-                       out, skip);
+    const dim3 grid((unsigned)((HW + 64 * Q - 1) / (64 * Q)), (unsigned)B, (unsigned)(COUT / (16 * RTW)));
+    hipLaunchKernelGGL((k_conv3x3<MODE, C0, C1, COUT, OUTK, Q, RTW>), grid, dim3(KM_BLOCK), 0, st, H, W, s0, s1, Wm,
+                       bias, out, skip);
 }
 
 template <int C>
@@ -711,7 +765,7 @@ int alloc(km_ctx* ctx, km_net* N, int B, int NX, int NY) {
         {(void**)&M.dy_d2, B * hw2 * 64 * bb}, {(void**)&M.dy_e3, B * hw3 * 128 * bb},
         {(void**)&M.dy_e2, B * hw2 * 64 * bb}, {(void**)&M.dy_e1, B * hw1 * 32 * bb},
         {(void**)&M.dy_e0, B * hw0 * 16 * bb},
-        {(void**)&M.z, B * n * fb},          {(void**)&M.q, B * n * fb},
+        {(void**)&M.z, B * n * fb},          {(void**)&M.q, B * n * fb},          {(void**)&M.ae, B * n * fb},
         {(void**)&M.dX0, B * hw0 * 48 * fb}, {(void**)&M.dX1, B * hw1 * 96 * fb}, {(void**)&M.dX2, B * hw2 * 192 * fb},
         {(void**)&M.dp3, B * hw3 * 64 * fb}, {(void**)&M.dp2, B * hw2 * 32 * fb}, {(void**)&M.dp1, B * hw1 * 16 * fb},
         {(void**)&M.wpart, kWpartMax * fb},
@@ -766,16 +820,16 @@ void forward(km_net* N, int B, const float* params, const int* skip, hipStream_t
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
     const LayerOff* L = N->lay.L;
     const bf16* wf = N->wf;
-    conv<0, 8, 0, 16, 0, 4>(H, W, B, N->X0, nullptr, wf + L[E0].wf, params + L[E0].b, N->e0, skip, st);
+    conv<0, 8, 0, 16, 0, 4, 1>(H, W, B, N->X0, nullptr, wf + L[E0].wf, params + L[E0].b, N->e0, skip, st);
     pool<16>(H1, W1, B, N->e0, N->p1, skip, st);
-    conv<0, 16, 0, 32, 0, 4>(H1, W1, B, N->p1, nullptr, wf + L[E1].wf, params + L[E1].b, N->e1, skip, st);
+    conv<0, 16, 0, 32, 0, 4, 1>(H1, W1, B, N->p1, nullptr, wf + L[E1].wf, params + L[E1].b, N->e1, skip, st);
     pool<32>(H2, W2, B, N->e1, N->p2, skip, st);
-    conv<0, 32, 0, 64, 0, 4>(H2, W2, B, N->p2, nullptr, wf + L[E2].wf, params + L[E2].b, N->e2, skip, st);
+    conv<0, 32, 0, 64, 0, 1, 2>(H2, W2, B, N->p2, nullptr, wf + L[E2].wf, params + L[E2].b, N->e2, skip, st);
     pool<64>(H3, W3, B, N->e2, N->p3, skip, st);
-    conv<0, 64, 0, 128, 0, 2>(H3, W3, B, N->p3, nullptr, wf + L[E3].wf, params + L[E3].b, N->e3, skip, st);
-    conv<1, 128, 64, 64, 0, 4>(H2, W2, B, N->e3, N->e2, wf + L[D2].wf, params + L[D2].b, N->d2, skip, st);
-    conv<1, 64, 32, 32, 0, 4>(H1, W1, B, N->d2, N->e1, wf + L[D1].wf, params + L[D1].b, N->d1, skip, st);
-    conv<1, 32, 16, 16, 0, 4>(H, W, B, N->d1, N->e0, wf + L[D0].wf, params + L[D0].b, N->d0, skip, st);
+    conv<0, 64, 0, 128, 0, 1, 2>(H3, W3, B, N->p3, nullptr, wf + L[E3].wf, params + L[E3].b, N->e3, skip, st);
+    conv<1, 128, 64, 64, 0, 1, 1>(H2, W2, B, N->e3, N->e2, wf + L[D2].wf, params + L[D2].b, N->d2, skip, st);
+    conv<1, 64, 32, 32, 0, 2, 1>(H1, W1, B, N->d2, N->e1, wf + L[D1].wf, params + L[D1].b, N->d1, skip, st);
+    conv<1, 32, 16, 16, 0, 4, 1>(H, W, B, N->d1, N->e0, wf + L[D0].wf, params + L[D0].b, N->d0, skip, st);
 }
 
 // Backward from dy_d0 (already written by k_loss_grad) to every parameter.
@@ -786,32 +840,32 @@ void backward(km_net* N, int B, float* grad, hipStream_t st) {
     const bf16* wt = N->wt;
     // d0: input [up(d1) | e0]
     wgrad<1, 32, 16, 16>(N, D0, H, W, B, N->d1, N->e0, N->dy_d0, grad, st);
-    conv<0, 16, 0, 48, 1, 4>(H, W, B, N->dy_d0, nullptr, wt + L[D0].wt, nullptr, N->dX0, nullptr, st);
+    conv<0, 16, 0, 48, 1, 4, 1>(H, W, B, N->dy_d0, nullptr, wt + L[D0].wt, nullptr, N->dX0, nullptr, st);
     hipLaunchKernelGGL((k_split_up<32, 16>), dim3(flat_blocks((int64_t)B * H1 * W1 * 32)), dim3(KM_BLOCK), 0, st, H1, W1,
                        B, (const float*)N->dX0, (const bf16*)N->d1, N->dy_d1);
     // d1: input [up(d2) | e1]
     wgrad<1, 64, 32, 32>(N, D1, H1, W1, B, N->d2, N->e1, N->dy_d1, grad, st);
-    conv<0, 32, 0, 96, 1, 2>(H1, W1, B, N->dy_d1, nullptr, wt + L[D1].wt, nullptr, N->dX1, nullptr, st);
+    conv<0, 32, 0, 96, 1, 2, 1>(H1, W1, B, N->dy_d1, nullptr, wt + L[D1].wt, nullptr, N->dX1, nullptr, st);
     hipLaunchKernelGGL((k_split_up<64, 32>), dim3(flat_blocks((int64_t)B * H2 * W2 * 64)), dim3(KM_BLOCK), 0, st, H2, W2,
                        B, (const float*)N->dX1, (const bf16*)N->d2, N->dy_d2);
     // d2: input [up(e3) | e2]
     wgrad<1, 128, 64, 64>(N, D2, H2, W2, B, N->e3, N->e2, N->dy_d2, grad, st);
-    conv<0, 64, 0, 192, 1, 2>(H2, W2, B, N->dy_d2, nullptr, wt + L[D2].wt, nullptr, N->dX2, nullptr, st);
+    conv<0, 64, 0, 192, 1, 1, 2>(H2, W2, B, N->dy_d2, nullptr, wt + L[D2].wt, nullptr, N->dX2, nullptr, st);
     hipLaunchKernelGGL((k_split_up<128, 64>), dim3(flat_blocks((int64_t)B * H3 * W3 * 128)), dim3(KM_BLOCK), 0, st, H3,
                        W3, B, (const float*)N->dX2, (const bf16*)N->e3, N->dy_e3);
     // e3: input p3
     wgrad<0, 64, 0, 128>(N, E3, H3, W3, B, N->p3, nullptr, N->dy_e3, grad, st);
-    conv<0, 128, 0, 64, 1, 4>(H3, W3, B, N->dy_e3, nullptr, wt + L[E3].wt, nullptr, N->dp3, nullptr, st);
+    conv<0, 128, 0, 64, 1, 1, 1>(H3, W3, B, N->dy_e3, nullptr, wt + L[E3].wt, nullptr, N->dp3, nullptr, st);
     hipLaunchKernelGGL(k_combine<64>, dim3(flat_blocks((int64_t)B * H2 * W2 * 64)), dim3(KM_BLOCK), 0, st, H2, W2, B,
                        (const float*)(N->dX2 + 128), 192, (const float*)N->dp3, (const bf16*)N->e2, N->dy_e2);
     // e2: input p2
     wgrad<0, 32, 0, 64>(N, E2, H2, W2, B, N->p2, nullptr, N->dy_e2, grad, st);
-    conv<0, 64, 0, 32, 1, 4>(H2, W2, B, N->dy_e2, nullptr, wt + L[E2].wt, nullptr, N->dp2, nullptr, st);
+    conv<0, 64, 0, 32, 1, 1, 1>(H2, W2, B, N->dy_e2, nullptr, wt + L[E2].wt, nullptr, N->dp2, nullptr, st);
     hipLaunchKernelGGL(k_combine<32>, dim3(flat_blocks((int64_t)B * H1 * W1 * 32)), dim3(KM_BLOCK), 0, st, H1, W1, B,
                        (const float*)(N->dX1 + 64), 96, (const float*)N->dp2, (const bf16*)N->e1, N->dy_e1);
     // e1: input p1
     wgrad<0, 16, 0, 32>(N, E1, H1, W1, B, N->p1, nullptr, N->dy_e1, grad, st);
-    conv<0, 32, 0, 16, 1, 4>(H1, W1, B, N->dy_e1, nullptr, wt + L[E1].wt, nullptr, N->dp1, nullptr, st);
+    conv<0, 32, 0, 16, 1, 4, 1>(H1, W1, B, N->dy_e1, nullptr, wt + L[E1].wt, nullptr, N->dp1, nullptr, st);
     hipLaunchKernelGGL(k_combine<16>, dim3(flat_blocks((int64_t)B * H * W * 16)), dim3(KM_BLOCK), 0, st, H, W, B,
                        (const float*)(N->dX0 + 32), 48, (const float*)N->dp1, (const bf16*)N->e0, N->dy_e0);
     // e0: input X0
@@ -873,22 +927,27 @@ int solve_stats(km_ctx* ctx, int NX, int NY, const float* coef, hipStream_t s) {
     return KM_OK;
 }
 
-int apply(km_ctx* ctx, int NX, int NY, const float* coef, const float* r, float* z, const int* skip, hipStream_t s,
-               const float* params) {
+template <typename T>
+int apply(km_ctx* ctx, int NX, int NY, const float* coef, const T* r, T* z, const int* skip, hipStream_t s,
+          const float* params) {
     km_net* N = ctx->net;
     KM_REQUIRE(ctx, N && N->arena && N->NX == NX && N->NY == NY, "network not prepared for %d x %d", NX, NY);
-    hipLaunchKernelGGL(k_net_stats<float>, dim3(N->Gs, 1), dim3(KM_BLOCK), 0, s, NX, NY, r, (int64_t)0, coef,
+    hipLaunchKernelGGL(k_net_stats<T>, dim3(N->Gs, 1), dim3(KM_BLOCK), 0, s, NX, NY, r, (int64_t)0, coef,
                        (int64_t)0, 0, N->P, N->Gs, skip);
     hipLaunchKernelGGL(k_net_stats_finish, dim3(1), dim3(KM_BLOCK), 0, s, N->Gs, (const double*)N->P, 0, N->stats, skip);
-    hipLaunchKernelGGL(k_features<float>, dim3(flat_blocks((int64_t)N->H * N->W)), dim3(KM_BLOCK), 0, s, NX, NY, N->H,
+    hipLaunchKernelGGL(k_features<T>, dim3(flat_blocks((int64_t)N->H * N->W)), dim3(KM_BLOCK), 0, s, NX, NY, N->H,
                        N->W, 1, r, (int64_t)0, coef, (int64_t)0, (const double*)N->stats, N->X0, skip);
     forward(N, 1, params, skip, s);
-    hipLaunchKernelGGL(k_out<float>, dim3(flat_blocks((int64_t)NX * NY)), dim3(KM_BLOCK), 0, s, NX, NY, N->H, N->W, 1,
+    hipLaunchKernelGGL(k_out<T>, dim3(flat_blocks((int64_t)NX * NY)), dim3(KM_BLOCK), 0, s, NX, NY, N->H, N->W, 1,
                        (const bf16*)N->d0, params + N->lay.out_w, coef, (int64_t)0, (const double*)N->stats, z,
                        (int64_t)0, skip);
     KM_CHECK_LAUNCH(ctx);
     return KM_OK;
 }
+
+template int apply<float>(km_ctx*, int, int, const float*, const float*, float*, const int*, hipStream_t, const float*);
+template int apply<double>(km_ctx*, int, int, const float*, const double*, double*, const int*, hipStream_t,
+                           const float*);
 
 }  // namespace kmnet
 
@@ -918,14 +977,14 @@ int km_unet_apply(km_ctx* ctx, const float* params, int NX, int NY, const double
     hipLaunchKernelGGL((k_convert<double, float>), dim3(flat_blocks(n)), dim3(KM_BLOCK), 0, s, n, r, N->r32);
     KM_CHECK_LAUNCH(ctx);
     if (int rc = kmnet::solve_stats(ctx, NX, NY, N->c32, s)) return rc;
-    if (int rc = kmnet::apply(ctx, NX, NY, N->c32, N->r32, N->z32, nullptr, s, params)) return rc;
+    if (int rc = kmnet::apply<float>(ctx, NX, NY, N->c32, N->r32, N->z32, nullptr, s, params)) return rc;
     hipLaunchKernelGGL((k_convert<float, double>), dim3(flat_blocks(n)), dim3(KM_BLOCK), 0, s, n, (const float*)N->z32, z);
     KM_CHECK_LAUNCH(ctx);
     return KM_OK;
 }
 
 int km_unet_loss_grad(km_ctx* ctx, const float* params, int B, int NX, int NY, const float* coef, const float* r,
-                      float* grad, double* loss) {
+                      const float* corr, float* grad, double* loss) {
     if (!ctx) return KM_EINVAL;
     KM_REQUIRE(ctx, params && coef && r && grad, "null pointer");
     if (int rc = kmnet::prepare(ctx, B, NX, NY, params)) return rc;
@@ -943,14 +1002,14 @@ int km_unet_loss_grad(km_ctx* ctx, const float* params, int B, int NX, int NY, c
     hipLaunchKernelGGL(k_out<float>, dim3(flat_blocks((int64_t)B * n)), dim3(KM_BLOCK), 0, s, NX, NY, H, W, B,
                        (const bf16*)N->d0, params + N->lay.out_w, coef, 5 * n, (const double*)N->stats, N->z, n,
                        (const int*)nullptr);
-    hipLaunchKernelGGL(k_loss_q, dim3(Gs, B), dim3(KM_BLOCK), 0, s, NX, NY, coef, 5 * n, r, (const float*)N->z, N->q,
-                       N->P, Gs);
-    hipLaunchKernelGGL(k_loss_finish, dim3(B), dim3(KM_BLOCK), 0, s, Gs, (const double*)N->P, (const double*)N->stats,
-                       N->loss);
+    hipLaunchKernelGGL(k_loss_q, dim3(Gs, B), dim3(KM_BLOCK), 0, s, NX, NY, coef, 5 * n, r, (const float*)N->z, corr,
+                       N->q, N->ae, N->P, Gs);
+    hipLaunchKernelGGL(k_loss_finish, dim3(B), dim3(KM_BLOCK), 0, s, Gs, (const double*)N->P, N->stats, N->loss);
     int64_t gl = (int64_t)B * H * W / 4096 + 1;
     const int Gl = (int)(gl > 1024 ? 1024 : gl);
     hipLaunchKernelGGL(k_loss_grad, dim3(Gl), dim3(KM_BLOCK), 0, s, NX, NY, H, W, B, coef, 5 * n, (const float*)N->q,
-                       (const double*)N->stats, (const bf16*)N->d0, params + N->lay.out_w, N->dy_d0, N->P, Gl);
+                       corr ? (const float*)N->ae : nullptr, (const double*)N->stats, (const bf16*)N->d0,
+                       params + N->lay.out_w, N->dy_d0, N->P, Gl);
     hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(KM_BLOCK), 0, s, 17, Gl, (const double*)N->P, grad + N->lay.out_w);
     backward(N, B, grad, s);
     KM_CHECK_LAUNCH(ctx);

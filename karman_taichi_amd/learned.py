@@ -103,17 +103,21 @@ class UNetPreconditioner:
         self.ctx.call("km_unet_apply", ptr(self.params), NX, NY, ptr(coef), ptr(r), ptr(z))
         return z
 
-    def loss_grad(self, coefs: torch.Tensor, rs: torch.Tensor):
-        """Batch mean of ||r - A M(r)||^2 / ||r||^2 and its gradient (into self.grad).
+    def loss_grad(self, coefs: torch.Tensor, rs: torch.Tensor, corrs: Optional[torch.Tensor] = None):
+        """Batch mean of the relative loss of z = M(r) and its gradient (into self.grad).
 
-        coefs: (B, 5, NX, NY) FP32, rs: (B, NX, NY) FP32, both contiguous."""
+        Without ``corrs``: ||r - A z||^2 / ||r||^2.  With the converged
+        corrections c (A c = r): the A-norm error (c - z)^T A (c - z) / c^T r.
+        coefs: (B, 5, NX, NY) FP32, rs / corrs: (B, NX, NY) FP32."""
         B, _, NX, NY = coefs.shape
         if coefs.dtype != torch.float32 or rs.dtype != torch.float32:
             raise TypeError("training batches are float32")
+        coefs, rs = coefs.contiguous(), rs.contiguous()
+        corrs = None if corrs is None else corrs.contiguous()
         loss = ctypes.c_double()
         self._bind()
-        self.ctx.call("km_unet_loss_grad", ptr(self.params), B, NX, NY, ptr(coefs.contiguous()),
-                      ptr(rs.contiguous()), ptr(self.grad), ctypes.byref(loss))
+        self.ctx.call("km_unet_loss_grad", ptr(self.params), B, NX, NY, ptr(coefs), ptr(rs),
+                      None if corrs is None else ptr(corrs), ptr(self.grad), ctypes.byref(loss))
         return loss.value, self.grad
 
     def adam_step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, grad: Optional[torch.Tensor] = None):
@@ -123,22 +127,26 @@ class UNetPreconditioner:
         self.ctx.call("km_adam_step", ctypes.c_int64(self.params.numel()), ptr(self.params), ptr(g), ptr(self.m),
                       ptr(self.v), self.step_count, lr, b1, b2, eps)
 
-    def train_step(self, coefs, rs, lr=1e-3) -> float:
-        loss, _ = self.loss_grad(coefs, rs)
+    def train_step(self, coefs, rs, lr=1e-3, corrs=None) -> float:
+        loss, _ = self.loss_grad(coefs, rs, corrs)
         self.adam_step(lr)
         return loss
 
 
-def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: UNetPreconditioner, eps=1e-8,
-              max_iter=None, x0=False, graph=True):
-    """Flexible BiCGSTAB with the network as right preconditioner (km_fbicgstab)."""
+def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: Optional[UNetPreconditioner], eps=1e-8,
+              max_iter=None, x0=False, graph=True, mixed=False):
+    """Flexible BiCGSTAB with the network as right preconditioner
+    (km_fbicgstab); ``net=None`` uses the Jacobi preconditioner instead.
+    ``mixed``: FP32 Krylov vectors (FP64 scalars and refinement)."""
     from .solvers import SolveInfo
     NX, NY = b.shape
     info = KmSolveInfo()
     flags = (_lib.KM_FLAG_X0 if x0 else 0) | (0 if graph else _lib.KM_FLAG_NO_GRAPH)
-    net._bind()
-    net.ctx.call("km_fbicgstab", NX, NY, ptr(coef), ptr(b), ptr(x), float(eps), int(max_iter or 0), flags,
-                 ptr(net.params), ctypes.byref(info))
+    flags |= _lib.KM_FLAG_MIXED if mixed else 0
+    ctx = _lib.context_for(b.device) if net is None else net.ctx
+    ctx.set_stream(torch.cuda.current_stream(b.device).cuda_stream)
+    ctx.call("km_fbicgstab", NX, NY, ptr(coef), ptr(b), ptr(x), float(eps), int(max_iter or 0), flags,
+             None if net is None else ptr(net.params), ctypes.byref(info))
     return SolveInfo.from_c(info)
 
 
@@ -161,8 +169,10 @@ class PairSet:
         return self.rs.shape[0]
 
     def batch(self, idx: torch.Tensor):
-        return self.coefs.index_select(0, self.sys.index_select(0, idx)).contiguous(), \
-            self.rs.index_select(0, idx).contiguous()
+        """(coefs, rs, corrs or None) of the pairs ``idx``, contiguous."""
+        corr = None if self.corr is None else self.corr.index_select(0, idx).contiguous()
+        return (self.coefs.index_select(0, self.sys.index_select(0, idx)).contiguous(),
+                self.rs.index_select(0, idx).contiguous(), corr)
 
 
 def harvest(case_list: Sequence, outer_steps: int, snapshots: Sequence[int] = (0, 8, 32, 128, 512),
@@ -226,16 +236,39 @@ def harvest(case_list: Sequence, outer_steps: int, snapshots: Sequence[int] = (0
                          "seconds": time.perf_counter() - t0})
 
 
+def noise_residuals(coefs: torch.Tensor, generator: torch.Generator) -> torch.Tensor:
+    """White-noise residuals on the free interior cells (a link is non-zero) of
+    each system: equal weight on every eigenmode, the input spectrum a Krylov
+    preconditioner has to handle."""
+    free = (coefs[:, 1:] != 0).any(dim=1)
+    noise = torch.randn(free.shape, generator=generator, dtype=torch.float32).to(coefs.device)
+    return (noise * free).contiguous()
+
+
 def train(net: UNetPreconditioner, pairs: PairSet, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 0,
-          log_every: int = 50, progress=None):
-    """Adam on random batches of harvested pairs; returns [(step, mean loss over the log window)]."""
+          log_every: int = 50, progress=None, loss: str = "anorm", lr_final: Optional[float] = None,
+          noise_frac: float = 0.0):
+    """Adam on random batches of harvested pairs; returns [(step, mean loss over the log window)].
+
+    ``loss``: "anorm" (needs the harvested corrections) or "residual".
+    ``lr_final``: cosine decay of the learning rate from ``lr`` to it.
+    ``noise_frac`` (residual loss): that fraction of each batch's residuals is
+    replaced by white noise on the same systems."""
+    if loss == "anorm" and pairs.corr is None:
+        raise ValueError("the A-norm loss needs harvested corrections")
+    if noise_frac and loss != "residual":
+        raise ValueError("noise residuals have no known correction: use the residual loss")
     g = torch.Generator(device="cpu").manual_seed(seed)
     hist, window = [], []
     t0 = time.perf_counter()
+    n_noise = int(round(noise_frac * batch))
     for it in range(1, steps + 1):
         idx = torch.randint(len(pairs), (batch,), generator=g).to(pairs.rs.device)
-        coefs, rs = pairs.batch(idx)
-        window.append(net.train_step(coefs, rs, lr))
+        coefs, rs, corrs = pairs.batch(idx)
+        if n_noise:
+            rs[:n_noise] = noise_residuals(coefs[:n_noise], g)
+        step_lr = lr if lr_final is None else lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * it / steps))
+        window.append(net.train_step(coefs, rs, step_lr, corrs if loss == "anorm" else None))
         if it % log_every == 0 or it == steps:
             m = float(np.mean(window))
             hist.append((it, m))

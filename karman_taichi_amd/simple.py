@@ -197,7 +197,8 @@ class SIMPLESolver:
         if self.p_solver == "cg":
             return self.p_correction_cg.solve(eps=self.eps_p)
         if self.p_precond is not None:
-            return self.p_correction_solver.solve(eps=self.eps_p, x0=self.warm_start_p, precond=self.p_precond)
+            return self.p_correction_solver.solve(eps=self.eps_p, x0=self.warm_start_p, precond=self.p_precond,
+                                                 mixed=self.mixed)
         return self.p_correction_solver.solve(eps=self.eps_p, mixed=self.mixed, x0=self.warm_start_p)
 
     def init(self, seed_fields=True):

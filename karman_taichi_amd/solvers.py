@@ -174,7 +174,8 @@ class BICGSolver(CGSolver):
         if precond is not None:
             from .learned import fbicgstab
             self._refresh()
-            info = fbicgstab(self.coef, self.b, self.x, precond, eps=eps, max_iter=max_iter, x0=x0, graph=graph)
+            info = fbicgstab(self.coef, self.b, self.x, precond, eps=eps, max_iter=max_iter, x0=x0, graph=graph,
+                             mixed=mixed)
         else:
             info = self._run("km_bicgstab", eps, max_iter, flags)
         if not quiet:

@@ -254,11 +254,17 @@ def stencil_t(coef, y):
             + (c3 * yp)[1:-1, :-2] + (c4 * yp)[1:-1, 2:])
 
 
-def loss_and_grad(params, coefs, rs_batch, bf=True):
-    """Mean over the batch of ||r - A M(r)||^2 / ||r||^2 and its gradient.
+def loss_and_grad(params, coefs, rs_batch, bf=True, corrs=None):
+    """Mean over the batch of the relative loss of z = M(r) and its gradient.
 
-    coefs: (B, 5, NX, NY); rs_batch: (B, NX, NY).  Returns (loss, grad, losses).
+    Without ``corrs``: the residual loss ||r - A z||^2 / ||r||^2.
+    With ``corrs`` (c = A^-1 r, the converged corrections): the A-norm error
+    loss e^T A e / c^T r with e = c - z (= ||e||_A^2 / ||c||_A^2 for SPD A),
+    whose gradient is -(A + A^T) e / c^T r.
+    coefs: (B, 5, NX, NY); rs_batch, corrs: (B, NX, NY).  Returns (loss, grad, losses).
     """
+    if corrs is not None:
+        return _anorm_loss_and_grad(params, coefs, rs_batch, corrs, bf)
     Bn = len(rs_batch)
     feats = [features(coefs[k], rs_batch[k], bf) for k in range(Bn)]
     X0 = np.stack([f[0] for f in feats])
@@ -276,6 +282,31 @@ def loss_and_grad(params, coefs, rs_batch, bf=True):
         rr = float((r * r).sum())
         losses.append(float((q * q).sum()) / rr)
         dz = (-2.0 / (rr * Bn)) * stencil_t(coefs[k], q)
+        dnet[k, :nx, :ny] = np.where(free, s * dz[1:-1, 1:-1], 0.0)
+    grad = backward(dnet, cache, bf)
+    return float(np.mean(losses)), grad, losses
+
+
+def _anorm_loss_and_grad(params, coefs, rs_batch, corrs, bf):
+    Bn = len(rs_batch)
+    feats = [features(coefs[k], rs_batch[k], bf) for k in range(Bn)]
+    X0 = np.stack([f[0] for f in feats])
+    net, cache = forward(params, X0, bf)
+    dnet = np.zeros_like(net)
+    losses = []
+    for k in range(Bn):
+        _, rs, abar, free = feats[k]
+        r = np.asarray(rs_batch[k], np.float64)
+        c = np.asarray(corrs[k], np.float64)
+        nx, ny = free.shape
+        s = rs / abar
+        z = np.zeros(r.shape)
+        z[1:-1, 1:-1] = np.where(free, s * net[k, :nx, :ny], 0.0)
+        e = c - z
+        ae = stencil(coefs[k], e)
+        den = float((c * r).sum())
+        losses.append(float((e * ae).sum()) / den)
+        dz = (-1.0 / (den * Bn)) * (ae + stencil_t(coefs[k], e))
         dnet[k, :nx, :ny] = np.where(free, s * dz[1:-1, 1:-1], 0.0)
     grad = backward(dnet, cache, bf)
     return float(np.mean(losses)), grad, losses

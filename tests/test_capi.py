@@ -39,3 +39,9 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.KmGrid) == 16 + 8 * 8 + 8 * 8
     assert _lib.KmGrid.bc.offset == 16 + 8 * 8
     assert ctypes.sizeof(_lib.KmSolveInfo) == 16 + 3 * 8
+
+
+def test_case_struct_layout_matches_header():
+    # km_case: km_grid (144 bytes) + 16 pointers
+    assert ctypes.sizeof(_lib.KmCase) == ctypes.sizeof(_lib.KmGrid) + 16 * 8
+    assert _lib.KmCase.mask.offset == ctypes.sizeof(_lib.KmGrid) + 15 * 8

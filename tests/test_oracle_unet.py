@@ -57,10 +57,7 @@ def test_forward_and_backward_match_torch_autograd():
     X0 = rng.standard_normal((2, 16, 24, 3))
     params = rand_params(1)
     net, cache = unet.forward(params, X0, bf=False)
-    tnet, P = torch_forward(params, X0)
-    for v in P.values():
-        v.requires_grad_(True)
-    tnet, P2 = torch_forward(params, X0)
+    tnet, _ = torch_forward(params, X0)
     np.testing.assert_allclose(net, tnet.detach().numpy(), rtol=1e-12, atol=1e-12)
     dnet = rng.standard_normal(net.shape)
     # autograd through a fresh graph with leaf parameters
@@ -150,3 +147,31 @@ def test_adam_matches_torch():
         opt.step()
         p, m, v = unet.adam(p, g, m, v, k)
     np.testing.assert_allclose(p, tp.detach().numpy(), rtol=1e-13, atol=1e-15)
+
+
+def test_anorm_loss_gradient_and_value():
+    """A-norm error loss: finite differences for the gradient; the value is
+    ||c - z||_A^2 / ||c||_A^2 with c the exact solution (scipy CSR)."""
+    from scipy.sparse.linalg import spsolve
+    systems = [small_system(s) for s in (13, 14)]
+    coefs = np.stack([c for c, _ in systems])
+    rs = np.stack([r for _, r in systems])
+    corrs = np.stack([krylov.direct_solve(c, r) for c, r in systems])
+    p = rand_params(15, 0.7)
+    loss, grad, losses = unet.loss_and_grad(p, coefs, rs, bf=False, corrs=corrs)
+    for k in range(2):
+        A = krylov.to_csr(coefs[k])
+        z = unet.apply(p, coefs[k], rs[k], bf=False).ravel()
+        c = corrs[k].ravel()
+        e = c - z
+        assert losses[k] == pytest.approx(float(e @ (A @ e)) / float(c @ (A @ c)), rel=1e-10)
+    sl, n = unet.param_slices()
+    rng = np.random.default_rng(16)
+    h = 1e-6
+    for key in ("e0.w", "e3.b", "d2.w", "d0.b", "out.w", "out.b"):
+        o, s = sl[key]
+        k = o + int(rng.integers(int(np.prod(s))))
+        e = np.zeros(n); e[k] = h
+        fd = (unet.loss_and_grad(p + e, coefs, rs, bf=False, corrs=corrs)[0]
+              - unet.loss_and_grad(p - e, coefs, rs, bf=False, corrs=corrs)[0]) / (2 * h)
+        assert fd == pytest.approx(grad[k], rel=1e-5, abs=1e-9), key
