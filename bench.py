@@ -65,6 +65,8 @@ def main():
     from karman_taichi_amd import _lib, cases
     from karman_taichi_amd.simple import SIMPLESolver
 
+    if args.case == "c3":
+        return bench_batch(args)
     case = cases.CASES[args.case]()
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -186,6 +188,40 @@ def main():
                      "note": "working set of one p' system is ~34 MB and fits the 256 MB MALL"},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args):
+    """The 96 uneven channel cases (nx 256..2048, Re 80..300) advanced together
+    with km_batch_step: one launch per SIMPLE stage over all cases, one batched
+    BiCGSTAB per solve.  A step = one outer iteration of every case."""
+    import torch
+    from karman_taichi_amd import cases
+    from karman_taichi_amd.batch import BatchedSIMPLE
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    case_list = cases.batched_cases(96, seed=0)
+    b = BatchedSIMPLE(case_list, device=dev)
+    b.init()
+    log(f"bench c3: {b.n} cases, {b.cells()} p' cells, warmup {args.warmup}, steps {args.steps}")
+    for _ in range(args.warmup):
+        b.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = [b.step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    p_it = [inf[2].iterations for rec in recs for inf in rec]
+    cell_its = sum(c.shape_p[0] * c.shape_p[1] * inf[2].iterations for rec in recs for c, inf in zip(case_list, rec))
+    out = {"metric": "batched SIMPLE outer steps/s (every case advanced one outer iteration)",
+           "value": round(args.steps / wall, 4), "unit": "batch steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded channel start states)",
+           "config": {"workload": f"96 uneven channels nx 256..2048, ny nx/5, D ny/8, Re 80..300, "
+                                  f"{b.cells()} p' cells in total", "tile": list(b.tile)},
+           "p_iters_min_median_max": [min(p_it), sorted(p_it)[len(p_it) // 2], max(p_it)],
+           "p_cells_iters_per_s": round(cell_its / wall, 1),
+           "case_steps_per_s": round(args.steps * b.n / wall, 3)}
     print(json.dumps(out), flush=True)
 
 

@@ -51,6 +51,7 @@ extern "C" {
 #define KM_FLAG_X0 1          /* use x as the initial guess (default: x0 = 0, bicgsolver.py:21-35) */
 #define KM_FLAG_MIXED 2       /* FP32 matvec/axpy inner solves, FP64 scalars and FP64 true-residual refinement */
 #define KM_FLAG_NO_GRAPH 4    /* launch kernels directly instead of replaying captured graphs */
+#define KM_FLAG_COARSE 8      /* km_fbicgstab: two-level preconditioner (aggregation coarse correction first) */
 
 /* east boundary kinds */
 #define KM_EAST_WALL 0        /* simple.py:184-186, :214-216 (with the bicgsimp.py:196 fix) */
@@ -170,7 +171,10 @@ int km_adam_step(km_ctx* ctx, int64_t n, float* params, const float* grad, float
  * network's warm start x0 = M(b) (kept if it lowers the residual).  A sweep
  * that does not halve the true residual is taken back and plain FP64
  * BiCGSTAB finishes (info->fallback = 1).  params == NULL selects the Jacobi
- * preconditioner D^-1 instead (karman_taichi.py:399,419), with no warm start. */
+ * preconditioner D^-1 instead (karman_taichi.py:399,419), with no warm start.
+ * KM_FLAG_COARSE makes the preconditioner two-level: z1 = P A_c^-1 P^T r on
+ * B x B aggregates of free cells (Galerkin A_c, inverted once per solve), then
+ * the network -- or 3 sweeps of damped Jacobi -- on r - A z1, added to z1. */
 int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
                  int max_iter, int flags, const float* params, km_solve_info* info);
 
@@ -197,6 +201,15 @@ int km_batch_step(km_ctx* ctx, int n, const km_case* cases, double eps_mom, doub
 int km_batch_time_forward(km_ctx* ctx, int n, const km_case* cases);
 /* The batch's tile shape (chosen on its first step). */
 int km_batch_get_tile(km_ctx* ctx, int* ti, int* tj);
+
+/* Aggregate size of the coarse level (0: automatic, the smallest of 8, 16, ..., 256
+ * giving at most 1024 coarse unknowns).  No counterpart in the reference. */
+int km_set_coarse(km_ctx* ctx, int block);
+/* z1 = P A_c^-1 P^T r and r1 = r - A z1 for one FP64 system (sets the coarse
+ * level up first); the training data of the two-level network and tests use it. */
+int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const double* r, double* z1, double* r1);
+/* Aggregate size and coarse unknowns of the last coarse level. */
+int km_coarse_info(km_ctx* ctx, int* block, int* unknowns);
 
 /* ---- measurement hook (no counterpart in the reference) ----
  * Average device time in microseconds of one launch of a solver kernel on an

@@ -15,7 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libkarman_hip.so"
 KM_OK, KM_BREAKDOWN, KM_MAXITER, KM_NONFINITE, KM_STALL = 0, 1, 2, 3, 4
 KM_EINVAL, KM_EHIP, KM_ENOMEM = -1, -2, -3
 KM_F64, KM_F32 = 0, 1
-KM_FLAG_X0, KM_FLAG_MIXED, KM_FLAG_NO_GRAPH = 1, 2, 4
+KM_FLAG_X0, KM_FLAG_MIXED, KM_FLAG_NO_GRAPH, KM_FLAG_COARSE = 1, 2, 4, 8
 KM_EAST_WALL, KM_EAST_OUTFLOW = 0, 1
 KM_KERNEL_BICG_A, KM_KERNEL_BICG_B, KM_KERNEL_STENCIL = 0, 1, 3
 KM_KERNEL_BACK_TO_BACK = 0x100
@@ -91,6 +91,9 @@ SIGNATURES = {
     "km_batch_step": [_CTX, _I, _P, _D, _D, _I, _I, _P],
     "km_batch_time_forward": [_CTX, _I, _P],
     "km_batch_get_tile": [_CTX, ctypes.POINTER(_I), ctypes.POINTER(_I)],
+    "km_set_coarse": [_CTX, _I],
+    "km_coarse_correct": [_CTX, _I, _I, _P, _P, _P, _P],
+    "km_coarse_info": [_CTX, ctypes.POINTER(_I), ctypes.POINTER(_I)],
 }
 _RESTYPE = {"km_last_error": ctypes.c_char_p}
 

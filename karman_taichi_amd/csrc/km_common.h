@@ -112,6 +112,7 @@ struct GraphKey {
 
 struct km_net;   // learned preconditioner buffers (km_unet.hip)
 struct km_batch; // batched cases (km_batch.hip)
+struct km_coarse; // coarse level of the two-level preconditioner (km_coarse.hip)
 
 struct km_ctx {
     int device = 0;
@@ -135,6 +136,7 @@ struct km_ctx {
     std::set<std::tuple<int, int, int>> tuned;
     km_net* net = nullptr;
     km_batch* batch = nullptr;
+    km_coarse* coarse = nullptr;
 };
 
 void km_batch_free(km_ctx* ctx);

@@ -1,6 +1,7 @@
 // Context, error reporting and workspace arena of libkarman_hip.so.
 #include "km_common.h"
 #include "km_unet.h"
+#include "km_coarse.h"
 
 #include <cstdlib>
 #include <cstring>
@@ -100,6 +101,7 @@ int km_destroy(km_ctx* ctx) {
     km_drop_graphs(ctx);
     km_net_free(ctx);
     km_batch_free(ctx);
+    km_coarse_free(ctx);
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->h_st) (void)hipHostFree(ctx->h_st);
     if (ctx->h_res) (void)hipHostFree(ctx->h_res);

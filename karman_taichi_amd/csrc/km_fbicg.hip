@@ -18,6 +18,7 @@
 //   K_t  t = A s^, partials <t, s>, <t, t>
 //   K_x  omega = <t, s> / <t, t>; x += alpha p^ + omega s^; r = s - omega t; partials <r~, r>, <r, r>
 #include "km_unet.h"
+#include "km_coarse.h"
 
 #include <cmath>
 
@@ -26,6 +27,11 @@ namespace {
 constexpr int FB_PC = 0;                 // [2][KM_MAX_RED] <r~, r>, <r, r>
 constexpr int FB_PV = 2 * KM_MAX_RED;    // [1] <r~, v>
 constexpr int FB_PT = 3 * KM_MAX_RED;    // [2] <t, s>, <t, t>
+constexpr int FB_PN = 5 * KM_MAX_RED;    // [1] p.p (the network's input scale)
+constexpr int FB_PS = 6 * KM_MAX_RED;    // [1] s.s
+constexpr int FB_PR = 7 * KM_MAX_RED;    // [1] r1.r1 after the coarse correction
+constexpr int kJacobiSweeps = 3;         // damped Jacobi sweeps of the two-level smoother
+constexpr double kJacobiOmega = 0.8;
 
 __device__ __forceinline__ bool fin(double v) { return v == v && v - v == 0.0; }
 
@@ -86,7 +92,7 @@ constexpr int kStallWindow = 300;
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_fb_p(Flat fl, int ph, const T* __restrict__ r, T* __restrict__ p, const T* __restrict__ v,
-       KState* __restrict__ st, const double* __restrict__ P) {
+       KState* __restrict__ st, const double* __restrict__ P, double* __restrict__ Pw) {
     __shared__ double sh[8];
     if (st->done) return;
     const int prev = (ph + KM_RING - 1) % KM_RING;
@@ -117,13 +123,24 @@ k_fb_p(Flat fl, int ph, const T* __restrict__ r, T* __restrict__ p, const T* __r
     }
     const int64_t lo = (int64_t)blockIdx.x * fl.chunk;
     const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
+    double pp = 0.0;
     if (i == 0) {
-        for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) p[k] = r[k];
+        for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
+            const T pk = r[k];
+            p[k] = pk;
+            pp += (double)pk * (double)pk;
+        }
     } else {
         const double beta = (rho / sp.rho) * (sp.alpha / sp.omega);
         const T fb = (T)beta, fo = (T)sp.omega;
-        for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) p[k] = r[k] + fb * (p[k] - fo * v[k]);
+        for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
+            const T pk = r[k] + fb * (p[k] - fo * v[k]);
+            p[k] = pk;
+            pp += (double)pk * (double)pk;
+        }
     }
+    pp = block_sum(pp, sh);
+    if (threadIdx.x == 0) Pw[FB_PN + blockIdx.x] = pp;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->ring[ph].rho = rho;
         st->ring[ph].iter = i;
@@ -156,7 +173,7 @@ k_fb_v(Flat fl, int NX, int NY, const T* __restrict__ c, const T* __restrict__ p
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_fb_s(Flat fl, int ph, const T* __restrict__ r, const T* __restrict__ v, T* __restrict__ s,
-       KState* __restrict__ st, const double* __restrict__ P) {
+       KState* __restrict__ st, const double* __restrict__ P, double* __restrict__ Pw) {
     __shared__ double sh[8];
     if (st->done) return;
     double tot[1];
@@ -174,8 +191,17 @@ k_fb_s(Flat fl, int ph, const T* __restrict__ r, const T* __restrict__ v, T* __r
     const T fa = (T)alpha;
     const int64_t lo = (int64_t)blockIdx.x * fl.chunk;
     const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
-    for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) s[k] = r[k] - fa * v[k];
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->ring[ph].alpha = alpha;
+    double ss = 0.0;
+    for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
+        const T sk = r[k] - fa * v[k];
+        s[k] = sk;
+        ss += (double)sk * (double)sk;
+    }
+    ss = block_sum(ss, sh);
+    if (threadIdx.x == 0) {
+        Pw[FB_PS + blockIdx.x] = ss;
+        if (blockIdx.x == 0) st->ring[ph].alpha = alpha;
+    }
 }
 
 template <typename T>
@@ -267,10 +293,12 @@ const int kFbChunk = 8;
 
 // Inner flexible solve of A d = rhs with vectors of type T, d returned in xd.
 // coefT: the coefficients in T (the matvecs); c32: FP32 coefficients (the
-// network's features).  ws holds 8n values of T.
+// network's features); coef64: the FP64 system (coarse level).  ws holds 10n
+// values of T.  The preconditioner: the network (params) or Jacobi, after
+// the coarse correction when coarse is set.
 template <typename T>
-int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, const T* rhs, T* xd, T* ws, double eps,
-             int max_iter, bool use_graph, const float* params, KState* out) {
+int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, const double* coef64, const T* rhs, T* xd,
+             T* ws, double eps, int max_iter, bool use_graph, const float* params, bool coarse, KState* out) {
     const int64_t n = (int64_t)NX * NY;
     T* r = ws;
     T* rt = ws + n;
@@ -280,6 +308,8 @@ int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, cons
     T* t = ws + 5 * n;
     T* ph = ws + 6 * n;
     T* sh = ws + 7 * n;
+    T* r1 = ws + 8 * n;      // residual after the coarse correction
+    T* tmp = ws + 9 * n;     // Jacobi ping-pong
     KState* st = &ctx->st[4];
     double* P = ctx->partials;
     const Flat fl = make_flat(n);
@@ -288,21 +318,39 @@ int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, cons
     KM_CHECK_LAUNCH(ctx);
     const int* skip = &st->done;
     const int jb = blocks(n);
-    auto precond = [=](hipStream_t q, const T* src, T* dst) {
-        if (params)
-            (void)kmnet::apply<T>(ctx, NX, NY, c32, src, dst, skip, q, params);
-        else
+    auto precond = [=](hipStream_t q, const T* src, T* dst, const double* norm_partials) {
+        if (coarse) {
+            if (params) {
+                (void)kmcoarse::apply<T>(ctx, coef64, coefT, src, dst, r1, P + FB_PR, skip, q);
+                (void)kmnet::apply_pre<T>(ctx, NX, NY, c32, r1, dst, skip, q, params, P + FB_PR, fl.G, true);
+            } else {
+                // z1 into tmp, then sweeps tmp -> dst -> tmp -> dst (the first one reuses r1)
+                (void)kmcoarse::apply<T>(ctx, coef64, coefT, src, tmp, r1, nullptr, skip, q);
+                T* zin = tmp;
+                T* zout = dst;
+                for (int k = 0; k < kJacobiSweeps; ++k) {
+                    (void)kmcoarse::jacobi<T>(ctx, NX, NY, coefT, src, k == 0 ? r1 : nullptr, zin, zout, kJacobiOmega,
+                                              skip, q);
+                    T* t_ = zin;
+                    zin = zout;
+                    zout = t_;
+                }
+            }
+        } else if (params) {
+            (void)kmnet::apply_pre<T>(ctx, NX, NY, c32, src, dst, skip, q, params, norm_partials, fl.G, false);
+        } else {
             hipLaunchKernelGGL(k_jacobi<T>, dim3(jb), dim3(KM_BLOCK), 0, q, n, coefT, src, dst, skip);
+        }
     };
     auto enqueue = [=](hipStream_t q, int phs) {
         hipLaunchKernelGGL(k_fb_p<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, phs, (const T*)r, p, (const T*)v, st,
-                           (const double*)P);
-        precond(q, p, ph);
+                           (const double*)P, P);
+        precond(q, p, ph, P + FB_PN);
         hipLaunchKernelGGL(k_fb_v<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, NX, NY, coefT, (const T*)ph, (const T*)rt,
                            v, (const KState*)st, P);
         hipLaunchKernelGGL(k_fb_s<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, phs, (const T*)r, (const T*)v, s, st,
-                           (const double*)P);
-        precond(q, s, sh);
+                           (const double*)P, P);
+        precond(q, s, sh, P + FB_PS);
         hipLaunchKernelGGL(k_fb_t<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, NX, NY, coefT, (const T*)sh, (const T*)s, t,
                            (const KState*)st, P);
         hipLaunchKernelGGL(k_fb_x<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, phs, (const T*)s, (const T*)t, (const T*)ph,
@@ -310,7 +358,8 @@ int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, cons
     };
     GraphKey key;
     key.v = {7, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)c32, (uintptr_t)rhs,
-             (uintptr_t)xd, (uintptr_t)ws, (uintptr_t)params, (uintptr_t)ctx->net, (uintptr_t)st};
+             (uintptr_t)xd, (uintptr_t)ws, (uintptr_t)params, (uintptr_t)ctx->net, (uintptr_t)st,
+             (uintptr_t)coarse, (uintptr_t)ctx->coarse};
     return run_chunks(ctx, st, key, use_graph, kFbChunk, max_iter, enqueue, out);
 }
 
@@ -333,7 +382,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     // FP64 r (n); FP32 coefficients (5n); T: coefficients unless FP64 (5n), rhs (n), d (n), inner (8n).
     // At least what the plain fallback needs, so the arena is not regrown mid-solve.
     size_t bytes = (size_t)n * sizeof(double) + (size_t)5 * n * sizeof(float) +
-                   (size_t)(f64 ? 10 : 15) * n * sizeof(T) + 2048;
+                   (size_t)(f64 ? 12 : 17) * n * sizeof(T) + 2048;
     if (bytes < (size_t)9 * n * sizeof(double)) bytes = (size_t)9 * n * sizeof(double);
     if (int rc = km_ensure_workspace(ctx, bytes)) return rc;
     double* r64 = static_cast<double*>(km_ws_vectors(ctx));
@@ -353,10 +402,13 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     T* wT = d + n;
     hipLaunchKernelGGL(k_scale_to<float>, dim3(blocks(5 * n)), dim3(KM_BLOCK), 0, ctx->stream, 5 * n, coef, c32, 1.0);
     KM_CHECK_LAUNCH(ctx);
+    const bool coarse = (flags & KM_FLAG_COARSE) != 0;
     if (params) {
         if (int rc = kmnet::prepare(ctx, 1, NX, NY, params)) return rc;
         if (int rc = kmnet::solve_stats(ctx, NX, NY, c32, ctx->stream)) return rc;
     }
+    if (coarse)
+        if (int rc = kmcoarse::setup(ctx, NX, NY, coef, ctx->stream)) return rc;
     double bb = 0.0;
     if (int rc = km_dot(ctx, n, KM_F64, b, b, &bb)) return rc < 0 ? rc : KM_EHIP;
     const double bnorm = sqrt(bb);
@@ -387,7 +439,16 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         hipLaunchKernelGGL(k_scale_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, (const double*)r64, rhs,
                            1.0 / rn);
         KM_CHECK_LAUNCH(ctx);
-        if (int rc = kmnet::apply<T>(ctx, NX, NY, c32, rhs, d, nullptr, ctx->stream, params)) return rc;
+        if (coarse) {
+            T* r1w = wT + 8 * n;
+            double* Pw = ctx->partials + FB_PR;
+            if (int rc = kmcoarse::apply<T>(ctx, coef, coefT, rhs, d, r1w, Pw, nullptr, ctx->stream)) return rc;
+            if (int rc = kmnet::apply_pre<T>(ctx, NX, NY, c32, r1w, d, nullptr, ctx->stream, params, Pw,
+                                             make_flat(n).G, true))
+                return rc;
+        } else if (int rc = kmnet::apply<T>(ctx, NX, NY, c32, rhs, d, nullptr, ctx->stream, params)) {
+            return rc;
+        }
         hipLaunchKernelGGL(k_axpy_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const T*)d, rn);
         KM_CHECK_LAUNCH(ctx);
         double rr2 = 0.0;
@@ -419,7 +480,9 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         int cap = max_iter - total;
         if (cap > inner_cap) cap = inner_cap;
         KState st;
-        if (int rc = fb_inner<T>(ctx, NX, NY, coefT, c32, rhs, d, wT, inner_eps, cap, use_graph, params, &st)) return rc;
+        if (int rc = fb_inner<T>(ctx, NX, NY, coefT, c32, coef, rhs, d, wT, inner_eps, cap, use_graph, params, coarse,
+                                 &st))
+            return rc;
         total += st.iter;
         sweeps++;
         hipLaunchKernelGGL(k_axpy_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const T*)d, rn);

@@ -43,6 +43,12 @@ int solve_stats(km_ctx* ctx, int NX, int NY, const float* coef, hipStream_t s);
 template <typename T>
 int apply(km_ctx* ctx, int NX, int NY, const float* coef, const T* r, T* z, const int* skip, hipStream_t s,
           const float* params);
+// apply() with r.r given as G per-workgroup partial sums (written by the
+// kernel that produced r), which saves the statistics pass over r.
+// accumulate: z += M(r) on free cells (z keeps its other values) instead of z = M(r).
+template <typename T>
+int apply_pre(km_ctx* ctx, int NX, int NY, const float* coef, const T* r, T* z, const int* skip, hipStream_t s,
+              const float* params, const double* rr_partials, int G, bool accumulate);
 
 }  // namespace kmnet
 

@@ -106,11 +106,24 @@ k_prep_weights(const LayerOff* __restrict__ L, int64_t nwf, int64_t nwt, const f
 // loaded while the MFMAs of the current one run (two register sets).
 // MODE 0: input src0 (C0 channels) at the output resolution.
 // MODE 1: input = [2x nearest upsample of src0 (C0 channels, half resolution) | src1 (C1 channels)].
-// OUTK 0: out = bf16(relu(acc + bias)); OUTK 1: out = acc in FP32 (input gradients).
-template <int MODE, int C0, int C1, int COUT, int OUTK, int Q, int RTW>
+// MODE 2: input = 2x2 average pool of src0 (C0 channels, double resolution), rounded to BF16 as k_pool does.
+// OUTK 0: out = bf16(relu(acc + bias)); OUTK 1: out = acc in FP32 (input gradients);
+// OUTK 2 (COUT 16, RTW 1): the network output fused in: the 1x1 layer over
+// bf16(relu(acc + bias)) scaled as k_out does, written to the padded FP32 or
+// FP64 field ``out`` (type T) on free interior cells (fo: output parameters,
+// coef / stats / NX / NY: the system).
+struct FusedOut {
+    const float* pw;
+    const float* coef;
+    const double* stats;
+    int NX, NY;
+    int accum;    // add to the output field instead of overwriting it
+};
+
+template <int MODE, int C0, int C1, int COUT, int OUTK, int Q, int RTW, typename T = float>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_conv3x3(int H, int W, const bf16* __restrict__ src0, const bf16* __restrict__ src1, const bf16* __restrict__ Wm,
-          const float* __restrict__ bias, void* __restrict__ out, const int* __restrict__ skip) {
+          const float* __restrict__ bias, void* __restrict__ out, const int* __restrict__ skip, FusedOut fo) {
     if (skipped(skip)) return;
     constexpr int CIN = C0 + C1;
     constexpr int KP = round32(9 * CIN);
@@ -132,7 +145,8 @@ k_conv3x3(int H, int W, const bf16* __restrict__ src0, const bf16* __restrict__ 
         pi[q] = pv[q] ? (int)(p / W) : 0;
         pj[q] = pv[q] ? (int)(p % W) : 0;
     }
-    const bf16* s0 = src0 + (MODE == 0 ? (int64_t)b * HW * C0 : (int64_t)b * (HW / 4) * C0);
+    const bf16* s0 = src0 + (MODE == 0 ? (int64_t)b * HW * C0 : MODE == 1 ? (int64_t)b * (HW / 4) * C0
+                                                                          : (int64_t)b * (4 * HW) * C0);
     const bf16* s1 = MODE == 1 ? src1 + (int64_t)b * HW * C1 : nullptr;
     const bf16* wrow = Wm + (int64_t)(rt0 * 16 + col) * KP + g * 8;
     const int Wc = W >> 1;
@@ -144,14 +158,27 @@ k_conv3x3(int H, int W, const bf16* __restrict__ src0, const bf16* __restrict__ 
         for (int q = 0; q < Q; ++q) {
             const int ii = pi[q] + dy, jj = pj[q] + dx;
             if (tap < 9 && pv[q] && ii >= 0 && ii < H && jj >= 0 && jj < W) {
-                const bf16* ptr;
-                if (MODE == 0)
-                    ptr = s0 + ((int64_t)ii * W + jj) * C0 + c;
-                else if (c < C0)
-                    ptr = s0 + ((int64_t)(ii >> 1) * Wc + (jj >> 1)) * C0 + c;
-                else
-                    ptr = s1 + ((int64_t)ii * W + jj) * C1 + (c - C0);
-                bv[q] = *reinterpret_cast<const bf16x8*>(ptr);
+                if (MODE == 2) {
+                    const bf16* f = s0 + ((int64_t)(2 * ii) * (2 * W) + 2 * jj) * C0 + c;
+                    const bf16x8 x00 = *reinterpret_cast<const bf16x8*>(f);
+                    const bf16x8 x01 = *reinterpret_cast<const bf16x8*>(f + C0);
+                    const bf16x8 x10 = *reinterpret_cast<const bf16x8*>(f + (int64_t)2 * W * C0);
+                    const bf16x8 x11 = *reinterpret_cast<const bf16x8*>(f + (int64_t)2 * W * C0 + C0);
+                    bf16x8 o;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        o[e] = (bf16)(0.25f * ((((float)x00[e] + (float)x01[e]) + (float)x10[e]) + (float)x11[e]));
+                    bv[q] = o;
+                } else {
+                    const bf16* ptr;
+                    if (MODE == 0)
+                        ptr = s0 + ((int64_t)ii * W + jj) * C0 + c;
+                    else if (c < C0)
+                        ptr = s0 + ((int64_t)(ii >> 1) * Wc + (jj >> 1)) * C0 + c;
+                    else
+                        ptr = s1 + ((int64_t)ii * W + jj) * C1 + (c - C0);
+                    bv[q] = *reinterpret_cast<const bf16x8*>(ptr);
+                }
             } else {
                 bv[q] = zero8();
             }
@@ -180,6 +207,46 @@ k_conv3x3(int H, int W, const bf16* __restrict__ src0, const bf16* __restrict__ 
             if (ch + 2 < NCH) load((ch + 2) * 32, b0, a0);
             mma(b1, a1);
         }
+    }
+    if constexpr (OUTK == 2) {
+        static_assert(COUT == 16 && RTW == 1, "fused output needs all 16 channels in one wave");
+        // lane (g, col) holds channels 4g .. 4g+3 of pixel col: gather the 16
+        // activations into lane g = 0 and sum them in channel order
+        const int64_t n = (int64_t)fo.NX * fo.NY;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            float y[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float a = acc[0][q][i] + bias[g * 4 + i];
+                y[i] = (float)(bf16)(a > 0.f ? a : 0.f);
+            }
+            float all[16];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                all[i] = y[i];
+                all[4 + i] = __shfl(y[i], col + 16, 64);
+                all[8 + i] = __shfl(y[i], col + 32, 64);
+                all[12 + i] = __shfl(y[i], col + 48, 64);
+            }
+            if (g == 0 && pv[q]) {
+                const int64_t p = px0 + q * 16 + col;
+                const int a = (int)(p / W), cc = (int)(p % W);
+                if (a < fo.NX - 2 && cc < fo.NY - 2) {
+                    const int64_t k = (int64_t)(a + 1) * fo.NY + (cc + 1);
+                    if (free_cell(fo.coef, n, k)) {
+                        float sum = 0.f;
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) sum += (float)(bf16)fo.pw[e] * all[e];
+                        sum += fo.pw[16];
+                        const T v = (T)(float)(fo.stats[(int64_t)b * 8 + 4] * (double)sum);
+                        T* o = static_cast<T*>(out) + (int64_t)b * n + k;
+                        *o = fo.accum ? *o + v : v;
+                    }
+                }
+            }
+        }
+        return;
     }
 #pragma unroll
     for (int rt = 0; rt < RTW; ++rt) {
@@ -453,8 +520,10 @@ k_net_stats_finish(int G, const double* __restrict__ P, int full, double* __rest
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int k = threadIdx.x; k < G; k += KM_BLOCK) {
         a0 += P[((int64_t)b * 3 + 0) * G + k];
-        a1 += P[((int64_t)b * 3 + 1) * G + k];
-        a2 += P[((int64_t)b * 3 + 2) * G + k];
+        if (full) {
+            a1 += P[((int64_t)b * 3 + 1) * G + k];
+            a2 += P[((int64_t)b * 3 + 2) * G + k];
+        }
     }
     a0 = block_sum(a0, sh);
     a1 = block_sum(a1, sh);
@@ -474,12 +543,18 @@ k_net_stats_finish(int G, const double* __restrict__ P, int full, double* __rest
 }
 
 // X0[b][a][c][0..7] = (r / rs, aP / abar on free cells, 1 on identity rows, 0 ...)
+// zc (may be null): an output field of B x NX x NY values cleared on the way
+// (the fused output layer then writes only free interior cells).
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_features(int NX, int NY, int H, int W, int B, const T* __restrict__ r, int64_t rstride, const float* __restrict__ coef,
-           int64_t cstride, const double* __restrict__ stats, bf16* __restrict__ X0, const int* __restrict__ skip) {
+           int64_t cstride, const double* __restrict__ stats, bf16* __restrict__ X0, const int* __restrict__ skip,
+           T* __restrict__ zc) {
     if (skipped(skip)) return;
     const int64_t HW = (int64_t)H * W, n = (int64_t)NX * NY, tot = (int64_t)B * HW;
+    if (zc)
+        for (int64_t i = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; i < (int64_t)B * n; i += (int64_t)gridDim.x * KM_BLOCK)
+            zc[i] = T(0);
     for (int64_t idx = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; idx < tot; idx += (int64_t)gridDim.x * KM_BLOCK) {
         const int64_t b = idx / HW, p = idx % HW;
         const int a = (int)(p / W), c = (int)(p % W);
@@ -700,13 +775,13 @@ int flat_blocks(int64_t n) {
     return g < 1 ? 1 : (int)g;
 }
 
-template <int MODE, int C0, int C1, int COUT, int OUTK, int Q, int RTW>
+template <int MODE, int C0, int C1, int COUT, int OUTK, int Q, int RTW, typename T = float>
 void conv(int H, int W, int B, const bf16* s0, const bf16* s1, const bf16* Wm, const float* bias, void* out,
-          const int* skip, hipStream_t st) {
+          const int* skip, hipStream_t st, FusedOut fo = FusedOut{}) {
     const int64_t HW = (int64_t)H * W;
     const dim3 grid((unsigned)((HW + 64 * Q - 1) / (64 * Q)), (unsigned)B, (unsigned)(COUT / (16 * RTW)));
-    hipLaunchKernelGGL((k_conv3x3<MODE, C0, C1, COUT, OUTK, Q, RTW>), grid, dim3(KM_BLOCK), 0, st, H, W, s0, s1, Wm,
-                       bias, out, skip);
+    hipLaunchKernelGGL((k_conv3x3<MODE, C0, C1, COUT, OUTK, Q, RTW, T>), grid, dim3(KM_BLOCK), 0, st, H, W, s0, s1,
+                       Wm, bias, out, skip, fo);
 }
 
 template <int C>
@@ -832,6 +907,25 @@ void forward(km_net* N, int B, const float* params, const int* skip, hipStream_t
     conv<1, 32, 16, 16, 0, 4, 1>(H, W, B, N->d1, N->e0, wf + L[D0].wf, params + L[D0].b, N->d0, skip, st);
 }
 
+// Inference: pools folded into the next convolution's gather and the output
+// layer into d0's epilogue (same arithmetic as forward() + k_out).
+template <typename T>
+void forward_infer(km_net* N, const float* params, const int* skip, hipStream_t st, T* z, const float* coef, int NX,
+                   int NY, int accum = 0) {
+    const int H = N->H, W = N->W;
+    const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
+    const LayerOff* L = N->lay.L;
+    const bf16* wf = N->wf;
+    conv<0, 8, 0, 16, 0, 4, 1>(H, W, 1, N->X0, nullptr, wf + L[E0].wf, params + L[E0].b, N->e0, skip, st);
+    conv<2, 16, 0, 32, 0, 4, 1>(H1, W1, 1, N->e0, nullptr, wf + L[E1].wf, params + L[E1].b, N->e1, skip, st);
+    conv<2, 32, 0, 64, 0, 1, 2>(H2, W2, 1, N->e1, nullptr, wf + L[E2].wf, params + L[E2].b, N->e2, skip, st);
+    conv<2, 64, 0, 128, 0, 1, 2>(H3, W3, 1, N->e2, nullptr, wf + L[E3].wf, params + L[E3].b, N->e3, skip, st);
+    conv<1, 128, 64, 64, 0, 1, 1>(H2, W2, 1, N->e3, N->e2, wf + L[D2].wf, params + L[D2].b, N->d2, skip, st);
+    conv<1, 64, 32, 32, 0, 2, 1>(H1, W1, 1, N->d2, N->e1, wf + L[D1].wf, params + L[D1].b, N->d1, skip, st);
+    const FusedOut fo{params + N->lay.out_w, coef, (const double*)N->stats, NX, NY, accum};
+    conv<1, 32, 16, 16, 2, 4, 1, T>(H, W, 1, N->d1, N->e0, wf + L[D0].wf, params + L[D0].b, z, skip, st, fo);
+}
+
 // Backward from dy_d0 (already written by k_loss_grad) to every parameter.
 void backward(km_net* N, int B, float* grad, hipStream_t st) {
     const int H = N->H, W = N->W;
@@ -936,15 +1030,31 @@ int apply(km_ctx* ctx, int NX, int NY, const float* coef, const T* r, T* z, cons
                        (int64_t)0, 0, N->P, N->Gs, skip);
     hipLaunchKernelGGL(k_net_stats_finish, dim3(1), dim3(KM_BLOCK), 0, s, N->Gs, (const double*)N->P, 0, N->stats, skip);
     hipLaunchKernelGGL(k_features<T>, dim3(flat_blocks((int64_t)N->H * N->W)), dim3(KM_BLOCK), 0, s, NX, NY, N->H,
-                       N->W, 1, r, (int64_t)0, coef, (int64_t)0, (const double*)N->stats, N->X0, skip);
-    forward(N, 1, params, skip, s);
-    hipLaunchKernelGGL(k_out<T>, dim3(flat_blocks((int64_t)NX * NY)), dim3(KM_BLOCK), 0, s, NX, NY, N->H, N->W, 1,
-                       (const bf16*)N->d0, params + N->lay.out_w, coef, (int64_t)0, (const double*)N->stats, z,
-                       (int64_t)0, skip);
+                       N->W, 1, r, (int64_t)0, coef, (int64_t)0, (const double*)N->stats, N->X0, skip, z);
+    forward_infer<T>(N, params, skip, s, z, coef, NX, NY);
     KM_CHECK_LAUNCH(ctx);
     return KM_OK;
 }
 
+// The same with r.r already reduced to G partials by the kernel that wrote r.
+template <typename T>
+int apply_pre(km_ctx* ctx, int NX, int NY, const float* coef, const T* r, T* z, const int* skip, hipStream_t s,
+              const float* params, const double* rr_partials, int G, bool accumulate) {
+    km_net* N = ctx->net;
+    KM_REQUIRE(ctx, N && N->arena && N->NX == NX && N->NY == NY, "network not prepared for %d x %d", NX, NY);
+    hipLaunchKernelGGL(k_net_stats_finish, dim3(1), dim3(KM_BLOCK), 0, s, G, rr_partials, 0, N->stats, skip);
+    hipLaunchKernelGGL(k_features<T>, dim3(flat_blocks((int64_t)N->H * N->W)), dim3(KM_BLOCK), 0, s, NX, NY, N->H,
+                       N->W, 1, r, (int64_t)0, coef, (int64_t)0, (const double*)N->stats, N->X0, skip,
+                       accumulate ? (T*)nullptr : z);
+    forward_infer<T>(N, params, skip, s, z, coef, NX, NY, accumulate ? 1 : 0);
+    KM_CHECK_LAUNCH(ctx);
+    return KM_OK;
+}
+
+template int apply_pre<float>(km_ctx*, int, int, const float*, const float*, float*, const int*, hipStream_t,
+                              const float*, const double*, int, bool);
+template int apply_pre<double>(km_ctx*, int, int, const float*, const double*, double*, const int*, hipStream_t,
+                               const float*, const double*, int, bool);
 template int apply<float>(km_ctx*, int, int, const float*, const float*, float*, const int*, hipStream_t, const float*);
 template int apply<double>(km_ctx*, int, int, const float*, const double*, double*, const int*, hipStream_t,
                            const float*);
@@ -997,7 +1107,7 @@ int km_unet_loss_grad(km_ctx* ctx, const float* params, int B, int NX, int NY, c
     hipLaunchKernelGGL(k_net_stats_finish, dim3(B), dim3(KM_BLOCK), 0, s, Gs, (const double*)N->P, 1, N->stats,
                        (const int*)nullptr);
     hipLaunchKernelGGL(k_features<float>, dim3(flat_blocks((int64_t)B * H * W)), dim3(KM_BLOCK), 0, s, NX, NY, H, W, B,
-                       r, n, coef, 5 * n, (const double*)N->stats, N->X0, (const int*)nullptr);
+                       r, n, coef, 5 * n, (const double*)N->stats, N->X0, (const int*)nullptr, (float*)nullptr);
     forward(N, B, params, nullptr, s);
     hipLaunchKernelGGL(k_out<float>, dim3(flat_blocks((int64_t)B * n)), dim3(KM_BLOCK), 0, s, NX, NY, H, W, B,
                        (const bf16*)N->d0, params + N->lay.out_w, coef, 5 * n, (const double*)N->stats, N->z, n,

@@ -134,15 +134,17 @@ class UNetPreconditioner:
 
 
 def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: Optional[UNetPreconditioner], eps=1e-8,
-              max_iter=None, x0=False, graph=True, mixed=False):
+              max_iter=None, x0=False, graph=True, mixed=False, coarse=False):
     """Flexible BiCGSTAB with the network as right preconditioner
     (km_fbicgstab); ``net=None`` uses the Jacobi preconditioner instead.
-    ``mixed``: FP32 Krylov vectors (FP64 scalars and refinement)."""
+    ``mixed``: FP32 Krylov vectors (FP64 scalars and refinement).
+    ``coarse``: two-level preconditioner (aggregation coarse correction, then
+    the network or damped Jacobi on the remaining residual)."""
     from .solvers import SolveInfo
     NX, NY = b.shape
     info = KmSolveInfo()
     flags = (_lib.KM_FLAG_X0 if x0 else 0) | (0 if graph else _lib.KM_FLAG_NO_GRAPH)
-    flags |= _lib.KM_FLAG_MIXED if mixed else 0
+    flags |= (_lib.KM_FLAG_MIXED if mixed else 0) | (_lib.KM_FLAG_COARSE if coarse else 0)
     ctx = _lib.context_for(b.device) if net is None else net.ctx
     ctx.set_stream(torch.cuda.current_stream(b.device).cuda_stream)
     ctx.call("km_fbicgstab", NX, NY, ptr(coef), ptr(b), ptr(x), float(eps), int(max_iter or 0), flags,
@@ -176,12 +178,19 @@ class PairSet:
 
 
 def harvest(case_list: Sequence, outer_steps: int, snapshots: Sequence[int] = (0, 8, 32, 128, 512),
-            device=None, corrections: bool = False, progress=None) -> PairSet:
+            device=None, corrections: bool = False, progress=None, coarse: bool = False,
+            noise_per_system: int = 0, seed: int = 0) -> PairSet:
     """Run the plain GPU SIMPLE loop on each case and record, at every outer
     step's p' system, the residuals r_k = b - A x_k of the plain BiCGSTAB
     iterates at the iteration counts ``snapshots`` (0: r = b) with the
     converged solve's correction x* - x_k.  The iterates come from plain
-    BiCGSTAB restarted from x_prev for the next increment of iterations."""
+    BiCGSTAB restarted from x_prev for the next increment of iterations.
+
+    ``noise_per_system`` adds that many white-noise residuals per system (free
+    cells; their correction A^-1 r is not known, so the residual loss only).
+    ``coarse``: store what the network sees inside the two-level
+    preconditioner -- r1 = r - A z1 and c - z1 with z1 the coarse correction
+    of r (km_coarse_correct)."""
     from .simple import SIMPLESolver
     from .solvers import BICGSolver
     if device is None:
@@ -192,13 +201,29 @@ def harvest(case_list: Sequence, outer_steps: int, snapshots: Sequence[int] = (0
         raise ValueError("all harvested cases must share one grid")
     NX, NY = shapes.pop()
     S = len(case_list) * outer_steps
-    P = S * len(snaps)
+    per = len(snaps) + noise_per_system
+    P = S * per
     coefs = torch.empty((S, 5, NX, NY), dtype=torch.float32, device=device)
     rs = torch.empty((P, NX, NY), dtype=torch.float32, device=device)
     corr = torch.empty((P, NX, NY), dtype=torch.float32, device=device) if corrections else None
     sysidx = torch.empty(P, dtype=torch.int64)
     xk = torch.zeros((NX, NY), dtype=torch.float64, device=device)
     rk = torch.empty_like(xk)
+    z1 = torch.empty_like(xk)
+    r1 = torch.empty_like(xk)
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+
+    def put(ctx, coef, r, corr_k, slot, sys_i):
+        if coarse:
+            ctx.call("km_coarse_correct", NX, NY, ptr(coef), ptr(r), ptr(z1), ptr(r1))
+            rs[slot].copy_(r1)
+            if corr is not None and corr_k is not None:
+                corr[slot].copy_(corr_k - z1)
+        else:
+            rs[slot].copy_(r)
+            if corr is not None and corr_k is not None:
+                corr[slot].copy_(corr_k)
+        sysidx[slot] = sys_i
     si = pi = 0
     t0 = time.perf_counter()
     for case in case_list:
@@ -218,11 +243,16 @@ def harvest(case_list: Sequence, outer_steps: int, snapshots: Sequence[int] = (0
                     trunc.solve(eps=0.0, max_iter=kk - done, x0=True)
                     done = kk
                 s.ctx.call("km_residual", NX, NY, ptr(s._coef_p), ptr(s.b_p), ptr(xk), ptr(rk), None)
-                rs[pi].copy_(rk)
-                if corr is not None:
-                    corr[pi].copy_(s.pcor - xk)
-                sysidx[pi] = si
+                put(s.ctx, s._coef_p, rk, s.pcor - xk if corr is not None else None, pi, si)
                 pi += 1
+            if noise_per_system:
+                free = (s._coef_p[1:] != 0).any(dim=0)
+                for _ in range(noise_per_system):
+                    noise = torch.randn((NX, NY), generator=gen, dtype=torch.float64).to(device) * free
+                    put(s.ctx, s._coef_p, noise.contiguous(), None, pi, si)
+                    if corr is not None:
+                        corr[pi].zero_()
+                    pi += 1
             si += 1
             s._correct()
             s.compute_mdiv()
@@ -233,6 +263,7 @@ def harvest(case_list: Sequence, outer_steps: int, snapshots: Sequence[int] = (0
     torch.cuda.synchronize(device)
     return PairSet(coefs, rs, sysidx.to(device), corr,
                    meta={"cases": [c.name for c in case_list], "outer_steps": outer_steps, "snapshots": snaps,
+                         "noise_per_system": noise_per_system, "coarse": coarse,
                          "seconds": time.perf_counter() - t0})
 
 

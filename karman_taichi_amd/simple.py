@@ -95,6 +95,7 @@ class SIMPLESolver:
         self.mixed = False            # FP32 inner solves + FP64 refinement
         self.warm_start_p = False
         self.p_precond = None         # a learned.UNetPreconditioner: flexible BiCGSTAB for p'
+        self.p_coarse = False         # two-level p' preconditioner (coarse correction + network or Jacobi)
         self.last = {}
 
     @classmethod
@@ -196,9 +197,9 @@ class SIMPLESolver:
         self.compute_coef_p()
         if self.p_solver == "cg":
             return self.p_correction_cg.solve(eps=self.eps_p)
-        if self.p_precond is not None:
+        if self.p_precond is not None or self.p_coarse:
             return self.p_correction_solver.solve(eps=self.eps_p, x0=self.warm_start_p, precond=self.p_precond,
-                                                 mixed=self.mixed)
+                                                 mixed=self.mixed, coarse=self.p_coarse)
         return self.p_correction_solver.solve(eps=self.eps_p, mixed=self.mixed, x0=self.warm_start_p)
 
     def init(self, seed_fields=True):

@@ -158,24 +158,26 @@ class BICGSolver(CGSolver):
     _kind = "bicgstab"
 
     def solve(self, eps=1e-8, quiet=True, max_iter=None, mixed=False, x0=False, graph=True,
-              precond=None) -> SolveInfo:
+              precond=None, coarse=False) -> SolveInfo:
         """BiCGSTAB until sqrt(r.r / b.b) < eps (bicgsolver.py:130).
 
         ``mixed``: FP32 matvec/axpy inner solves, FP64 scalars, FP64 refinement
         judged by the true residual.  ``x0``: start from the current ``x``
         instead of zero (then the reference's r~ = b becomes r~ = b - A x0).
         ``precond``: a :class:`~karman_taichi_amd.learned.UNetPreconditioner`
-        in the p^ / s^ hooks (bicgsolver.py:48-58): flexible BiCGSTAB in FP32
-        vectors with FP64 refinement, judged by the true residual only, with
-        plain BiCGSTAB as the fallback.
+        in the p^ / s^ hooks (bicgsolver.py:48-58): flexible BiCGSTAB with
+        FP64 refinement (FP32 vectors with ``mixed``), judged by the true
+        residual only, with plain BiCGSTAB as the fallback.  ``coarse``: the
+        two-level preconditioner (aggregation coarse correction, then the
+        network -- or damped Jacobi without one -- on the remaining residual).
         """
         flags = (_lib.KM_FLAG_MIXED if mixed else 0) | (_lib.KM_FLAG_X0 if x0 else 0)
         flags |= 0 if graph else _lib.KM_FLAG_NO_GRAPH
-        if precond is not None:
+        if precond is not None or coarse:
             from .learned import fbicgstab
             self._refresh()
             info = fbicgstab(self.coef, self.b, self.x, precond, eps=eps, max_iter=max_iter, x0=x0, graph=graph,
-                             mixed=mixed)
+                             mixed=mixed, coarse=coarse)
         else:
             info = self._run("km_bicgstab", eps, max_iter, flags)
         if not quiet:

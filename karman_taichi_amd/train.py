@@ -39,30 +39,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def evaluate(net, case, outer_steps: int, eps: float = 1e-8, device=None, nets=None):
-    """Per outer step of ``case``: plain FP64 BiCGSTAB vs each network's
+def evaluate(case, outer_steps: int, variants: dict, eps: float = 1e-8, device=None):
+    """Per outer step of ``case``: plain FP64 BiCGSTAB and every variant's
     flexible solve on the same p' system (the SIMPLE run continues with the
-    plain solution).  Labels ending in "_fp32" run with FP32 Krylov vectors.  Returns {label: {"it": [...], "ms": [...], "fallback": n, "true": max}}."""
-    device = device or net.device
-    nets = dict(nets or {})
-    nets.setdefault("learned", net)
-    nets.setdefault("learned_fp32", net)
+    plain solution).  variants: label -> (network or None for Jacobi, coarse,
+    mixed).  Returns {label: {"it": [...], "ms": [...], "fallback": n, "true": max}}."""
     s = SIMPLESolver.from_case(case, device=device)
     s.init()
-    res = {k: {"it": [], "ms": [], "fallback": 0, "true": 0.0} for k in ["plain", *nets]}
+    res = {k: {"it": [], "ms": [], "fallback": 0, "true": 0.0} for k in ["plain", *variants]}
     x = torch.zeros_like(s.b_p)
     for k in range(outer_steps):
         s.solve_momentum_eqn(1)
         s.correct_conserv()
         s.compute_coef_p()
-        for label, m in nets.items():
-            mixed = label.endswith("_fp32")
-            if isinstance(m, tuple):
-                m = m[0]
+        for label, (m, coarse, mixed) in variants.items():
             x.zero_()
             torch.cuda.synchronize(device)
             t0 = time.perf_counter()
-            info = learned.fbicgstab(s._coef_p, s.b_p, x, m, eps=eps, mixed=mixed)
+            info = learned.fbicgstab(s._coef_p, s.b_p, x, m, eps=eps, mixed=mixed, coarse=coarse)
             torch.cuda.synchronize(device)
             r = res[label]
             r["ms"].append((time.perf_counter() - t0) * 1e3)
@@ -82,6 +76,16 @@ def evaluate(net, case, outer_steps: int, eps: float = 1e-8, device=None, nets=N
         if (k + 1) % case.n_outer == 0:
             s.time_forward()
     return res
+
+
+def variants_for(net, untrained, coarse: bool) -> dict:
+    """The learned solve and its ablations: the same structure with an untrained
+    network and with damped Jacobi in the network's place."""
+    v = {"learned": (net, coarse, False), "learned_fp32": (net, coarse, True),
+         "untrained": (untrained, coarse, False), "jacobi": (None, coarse, False)}
+    if coarse:
+        v["jacobi_fp32"] = (None, True, True)
+    return v
 
 
 def summarize(res):
@@ -109,7 +113,13 @@ def main(argv=None):
     ap.add_argument("--eps", type=float, default=1e-8)
     ap.add_argument("--noise-frac", type=float, default=0.0, help="residual loss: fraction of white-noise residuals")
     ap.add_argument("--eval-only", action="store_true", help="evaluate --init (and the baselines) without training")
+    ap.add_argument("--coarse", action="store_true",
+                    help="train the network as the smoother of the two-level preconditioner (residuals after the "
+                         "coarse correction) and evaluate the two-level solve")
+    ap.add_argument("--noise-per-system", type=int, default=0, help="white-noise residuals harvested per system")
     args = ap.parse_args(argv)
+    if args.noise_per_system and args.loss == "anorm":
+        ap.error("noise residuals have no known correction: use --loss residual")
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     res = [float(v) for v in args.re.split(",") if v]
@@ -119,14 +129,15 @@ def main(argv=None):
         net = learned.UNetPreconditioner.load(args.init, device=dev) if args.init else \
             learned.UNetPreconditioner(device=dev, seed=args.seed)
         untrained = learned.UNetPreconditioner(device=dev, seed=args.seed)
-        ev = evaluate(net, cases.c2(), args.eval_steps, args.eps, dev, nets={"untrained": untrained, "jacobi": None})
-        out = {"eval": summarize(ev), "weights": args.init}
+        ev = evaluate(cases.c2(), args.eval_steps, variants_for(net, untrained, args.coarse), args.eps, dev)
+        out = {"eval": summarize(ev), "weights": args.init, "coarse": args.coarse}
         log(json.dumps({k: {kk: vv for kk, vv in v.items() if kk != "iters"} for k, v in out["eval"].items()}))
         print(json.dumps(out), flush=True)
         return
     log(f"harvest: Re {res}, {args.outer_steps} outer steps each, snapshots {snaps}")
     pairs = learned.harvest(case_list, args.outer_steps, snaps, device=dev, corrections=args.loss == "anorm",
-                            progress=log)
+                            progress=log, coarse=args.coarse, noise_per_system=args.noise_per_system,
+                            seed=args.seed)
     log(f"harvest: {len(pairs)} pairs from {pairs.coefs.shape[0]} systems in {pairs.meta['seconds']:.1f} s")
     net = learned.UNetPreconditioner.load(args.init, device=dev) if args.init else \
         learned.UNetPreconditioner(device=dev, seed=args.seed)
@@ -147,7 +158,7 @@ def main(argv=None):
         log(f"saved {args.out}")
     if args.eval_steps:
         untrained = learned.UNetPreconditioner(device=dev, seed=args.seed)
-        ev = evaluate(net, cases.c2(), args.eval_steps, args.eps, dev, nets={"untrained": untrained, "jacobi": None})
+        ev = evaluate(cases.c2(), args.eval_steps, variants_for(net, untrained, args.coarse), args.eps, dev)
         out["eval"] = summarize(ev)
         log(json.dumps({k: {kk: vv for kk, vv in v.items() if kk != "iters"} for k, v in out["eval"].items()}))
     print(json.dumps(out), flush=True)

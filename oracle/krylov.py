@@ -296,3 +296,62 @@ def fbicgstab(coef, b, eps=1e-8, x0=None, max_iter=None, precond=None, stall_win
     rt = b - stencil(coef, x)
     info["rel_resid_true"] = math.sqrt(dot(rt, rt)) / bn
     return x, info
+
+
+# ---------------------------------------------------------------- two-level
+
+def coarse_block(nx, ny, req=0):
+    """Aggregate edge length: req, else the smallest of 8, 16, ..., 256 giving
+    at most 1024 coarse unknowns."""
+    if req > 0:
+        return req
+    for B in (8, 16, 32, 64, 128, 256):
+        if (-(-nx // B)) * (-(-ny // B)) <= 1024:
+            return B
+    return 512
+
+
+def free_cells(coef):
+    """Cells with a non-zero link (identity rows and the halo are not free)."""
+    return (coef[1] != 0) | (coef[2] != 0) | (coef[3] != 0) | (coef[4] != 0)
+
+
+def aggregation(coef, B):
+    """P (n x m, indicator of the free cells of each B x B interior aggregate,
+    i-major aggregate order) and the Galerkin operator A_c = P^T A P (dense)."""
+    import scipy.sparse as sp
+    _, NX, NY = coef.shape
+    nx, ny = NX - 2, NY - 2
+    mI, mJ = -(-nx // B), -(-ny // B)
+    ii, jj = np.meshgrid(np.arange(NX), np.arange(NY), indexing="ij")
+    agg = (np.clip(ii - 1, 0, None) // B) * mJ + np.clip(jj - 1, 0, None) // B
+    rows = np.flatnonzero(free_cells(coef).ravel())
+    P = sp.csr_matrix((np.ones(rows.size), (rows, agg.ravel()[rows])), shape=(NX * NY, mI * mJ))
+    A = to_csr(coef)
+    Ac = (P.T @ A @ P).toarray()
+    empty = np.asarray(P.sum(axis=0)).ravel() == 0
+    Ac[empty, empty] = 1.0          # aggregates without free cells: identity rows
+    return P, Ac
+
+
+def coarse_correct(coef, r, B):
+    """z1 = P A_c^-1 P^T r and r1 = r - A z1."""
+    P, Ac = aggregation(coef, B)
+    z1 = (P @ np.linalg.solve(Ac, P.T @ r.ravel())).reshape(r.shape)
+    return z1, r - stencil(coef, z1)
+
+
+def twolevel_jacobi_precond(coef, B, sweeps=3, omega=0.8):
+    """M(r) = z1 + damped Jacobi sweeps on r - A z: the preconditioner of the
+    flexible solve with the coarse level and no network."""
+    P, Ac = aggregation(coef, B)
+    Aci = np.linalg.inv(Ac)
+    d = coef[0]
+    dinv = np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 0.0)
+
+    def M(r):
+        z = (P @ (Aci @ (P.T @ r.ravel()))).reshape(r.shape)
+        for _ in range(sweeps):
+            z = z + omega * dinv * (r - stencil(coef, z))
+        return z
+    return M
