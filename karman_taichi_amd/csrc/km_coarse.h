@@ -16,10 +16,11 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s);
 
 // z1 = P A_c^-1 P^T r and r1 = r - A z1 (A in T for the residual, the FP64
 // coefficients for the free-cell mask); Prr (may be null) receives the
-// per-workgroup partials of r1.r1 of a make_flat(NX * NY) launch.
+// per-workgroup partials of r1.r1 of a make_flat(NX * NY) launch.  omega > 0
+// fuses the first damped Jacobi sweep: z1 <- z1 + omega D^-1 r1, r1 unused.
 template <typename T>
 int apply(km_ctx* ctx, const double* coef, const T* coefT, const T* r, T* z1, T* r1, double* Prr, const int* skip,
-          hipStream_t s);
+          hipStream_t s, double omega = 0.0);
 
 // z_out = z_in + omega D^-1 res, with res = r - A z_in computed here when null.
 template <typename T>

@@ -183,12 +183,15 @@ __device__ __forceinline__ double coarse_value(int i, int j, int B, int mJ, int 
     return yc[((i - 1) / B) * mJ + (j - 1) / B];
 }
 
-// z1 = P yc on free cells; r1 = r - A z1; partials of r1.r1 (P_rr, may be null)
+// z1 = P yc on free cells; r1 = r - A z1; partials of r1.r1 (P_rr, may be null).
+// omega > 0: the first damped Jacobi sweep fused in -- z1 receives
+// z1 + omega D^-1 r1 and r1 is not written.
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_coarse_prolong(Flat fl, int NX, int NY, int B, int mJ, const double* __restrict__ coef,
                  const T* __restrict__ coefT, const T* __restrict__ r, const double* __restrict__ yc,
-                 T* __restrict__ z1, T* __restrict__ r1, double* __restrict__ Prr, const int* __restrict__ skip) {
+                 T* __restrict__ z1, T* __restrict__ r1, double* __restrict__ Prr, double omega,
+                 const int* __restrict__ skip) {
     if (skip && *skip) return;
     __shared__ double sh[4];
     const int nx = NX - 2, ny = NY - 2;
@@ -211,8 +214,13 @@ k_coarse_prolong(Flat fl, int NX, int NY, int B, int mJ, const double* __restric
         az = az + coefT[3 * n + k] * zc(i, j + 1);
         az = az + coefT[4 * n + k] * zc(i, j - 1);
         const T rv = r[k] - az;
-        z1[k] = zP;
-        r1[k] = rv;
+        if (omega > 0.0) {
+            const T d = coefT[k];
+            z1[k] = zP + (d != T(0) ? (T)omega * rv / d : T(0));
+        } else {
+            z1[k] = zP;
+            r1[k] = rv;
+        }
         a += (double)rv * (double)rv;
     }
     if (Prr) {
@@ -311,7 +319,7 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
 
 template <typename T>
 int apply(km_ctx* ctx, const double* coef, const T* coefT, const T* r, T* z1, T* r1, double* Prr, const int* skip,
-          hipStream_t s) {
+          hipStream_t s, double omega) {
     km_coarse* C = ctx->coarse;
     KM_REQUIRE(ctx, C && C->arena && C->m > 0, "coarse level not set up");
     const Flat fl = make_flat((int64_t)C->NX * C->NY);
@@ -320,7 +328,7 @@ int apply(km_ctx* ctx, const double* coef, const T* coefT, const T* r, T* z1, T*
     hipLaunchKernelGGL(k_coarse_matvec, dim3((C->m + 3) / 4), dim3(KM_BLOCK), 0, s, C->m, (const double*)C->inv,
                        (const double*)C->rc, C->yc, skip);
     hipLaunchKernelGGL(k_coarse_prolong<T>, dim3(fl.G), dim3(KM_BLOCK), 0, s, fl, C->NX, C->NY, C->B, C->mJ, coef,
-                       coefT, r, (const double*)C->yc, z1, r1, Prr, skip);
+                       coefT, r, (const double*)C->yc, z1, r1, Prr, omega, skip);
     KM_CHECK_LAUNCH(ctx);
     return KM_OK;
 }
@@ -336,9 +344,9 @@ int jacobi(km_ctx* ctx, int NX, int NY, const T* coefT, const T* r, const T* res
 }
 
 template int apply<float>(km_ctx*, const double*, const float*, const float*, float*, float*, double*, const int*,
-                          hipStream_t);
+                          hipStream_t, double);
 template int apply<double>(km_ctx*, const double*, const double*, const double*, double*, double*, double*,
-                           const int*, hipStream_t);
+                           const int*, hipStream_t, double);
 template int jacobi<float>(km_ctx*, int, int, const float*, const float*, const float*, const float*, float*, double,
                            const int*, hipStream_t);
 template int jacobi<double>(km_ctx*, int, int, const double*, const double*, const double*, const double*, double*,
@@ -367,7 +375,7 @@ int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const dou
     KM_REQUIRE(ctx, coef && r && z1 && r1, "null pointer");
     KM_REQUIRE(ctx, NX >= 3 && NY >= 3 && (int64_t)NX * NY < ((int64_t)1 << 31), "system shape %d x %d", NX, NY);
     if (int rc = kmcoarse::setup(ctx, NX, NY, coef, ctx->stream)) return rc;
-    return kmcoarse::apply<double>(ctx, coef, coef, r, z1, r1, nullptr, nullptr, ctx->stream);
+    return kmcoarse::apply<double>(ctx, coef, coef, r, z1, r1, nullptr, nullptr, ctx->stream, 0.0);
 }
 
 int km_coarse_info(km_ctx* ctx, int* block, int* unknowns) {

@@ -30,7 +30,7 @@ constexpr int FB_PT = 3 * KM_MAX_RED;    // [2] <t, s>, <t, t>
 constexpr int FB_PN = 5 * KM_MAX_RED;    // [1] p.p (the network's input scale)
 constexpr int FB_PS = 6 * KM_MAX_RED;    // [1] s.s
 constexpr int FB_PR = 7 * KM_MAX_RED;    // [1] r1.r1 after the coarse correction
-constexpr int kJacobiSweeps = 3;         // damped Jacobi sweeps of the two-level smoother
+constexpr int kJacobiSweeps = 3;         // damped Jacobi sweeps of the two-level smoother (odd: ends in dst)
 constexpr double kJacobiOmega = 0.8;
 
 __device__ __forceinline__ bool fin(double v) { return v == v && v - v == 0.0; }
@@ -324,13 +324,12 @@ int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, cons
                 (void)kmcoarse::apply<T>(ctx, coef64, coefT, src, dst, r1, P + FB_PR, skip, q);
                 (void)kmnet::apply_pre<T>(ctx, NX, NY, c32, r1, dst, skip, q, params, P + FB_PR, fl.G, true);
             } else {
-                // z1 into tmp, then sweeps tmp -> dst -> tmp -> dst (the first one reuses r1)
-                (void)kmcoarse::apply<T>(ctx, coef64, coefT, src, tmp, r1, nullptr, skip, q);
-                T* zin = tmp;
-                T* zout = dst;
-                for (int k = 0; k < kJacobiSweeps; ++k) {
-                    (void)kmcoarse::jacobi<T>(ctx, NX, NY, coefT, src, k == 0 ? r1 : nullptr, zin, zout, kJacobiOmega,
-                                              skip, q);
+                // z1 + the first sweep into dst, then sweeps dst -> tmp -> dst
+                (void)kmcoarse::apply<T>(ctx, coef64, coefT, src, dst, r1, nullptr, skip, q, kJacobiOmega);
+                T* zin = dst;
+                T* zout = tmp;
+                for (int k = 1; k < kJacobiSweeps; ++k) {
+                    (void)kmcoarse::jacobi<T>(ctx, NX, NY, coefT, src, nullptr, zin, zout, kJacobiOmega, skip, q);
                     T* t_ = zin;
                     zin = zout;
                     zout = t_;
