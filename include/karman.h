@@ -210,6 +210,12 @@ int km_set_coarse(km_ctx* ctx, int block);
 int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const double* r, double* z1, double* r1);
 /* Aggregate size and coarse unknowns of the last coarse level. */
 int km_coarse_info(km_ctx* ctx, int* block, int* unknowns);
+/* z = M(r) for one FP64 system: the two-level Jacobi preconditioner of the
+ * flexible solve with KM_FLAG_COARSE and no network, z = P A_c^-1 P^T r then
+ * three sweeps z += 0.8 D^-1 (r - A z).  It fills the p^ / s^ hooks of the
+ * reference's BiCGSTAB (bicgsolver.py:48-58, karman_taichi.py:399,419 pass
+ * a Jacobi scaling there); sets the coarse level up first. */
+int km_twolevel_precond(km_ctx* ctx, int NX, int NY, const double* coef, const double* r, double* z);
 
 /* ---- measurement hook (no counterpart in the reference) ----
  * Average device time in microseconds of one launch of a solver kernel on an

@@ -1,31 +1,60 @@
-// Internal interface of the aggregation coarse level (km_coarse.hip) used by
-// the flexible BiCGSTAB (km_fbicg.hip).  Not part of the C-ABI.
+// Internal interface of the aggregation coarse level (km_coarse.hip) and the
+// fused two-level flexible BiCGSTAB (km_twolevel.hip), used by the flexible
+// solver (km_fbicg.hip).  Not part of the C-ABI.
 #pragma once
 
 #include "km_common.h"
 
+// Coarse level of one p' system: B x B aggregates of the interior, i-major.
+struct km_coarse {
+    int NX = 0, NY = 0, B = 0, mI = 0, mJ = 0, m = 0, bw = 0;
+    int block_req = 0;        // km_set_coarse (0: automatic)
+    char* arena = nullptr;
+    size_t bytes = 0;
+    double* cc = nullptr;     // 5 planes of m coarse coefficients
+    double* band = nullptr;   // m x (2 bw + 1) band of A_c, then its LU factors
+    double* inv = nullptr;    // m x m A_c^-1, row-major
+    double* rc = nullptr;     // m restricted residual
+    double* yc = nullptr;     // m coarse solution
+    uint8_t* mask = nullptr;  // NX x NY: 1 on the free interior cells (the support of P)
+};
+
 namespace kmcoarse {
+
+// Damped Jacobi sweeps of the two-level smoother and their damping.
+constexpr int kSweeps = 3;
+constexpr double kOmega = 0.8;
 
 // Aggregate edge length for an nx x ny interior (req > 0: that size; else the
 // smallest of 8, 16, ..., 256 that keeps the coarse level at <= 1024 unknowns).
 int choose_block(int nx, int ny, int req);
 
-// Galerkin coarse operator P^T A P of the FP64 system, its band LU and its
-// explicit inverse (enqueued on s).
+// Galerkin coarse operator P^T A P of the FP64 system, its band LU, its
+// explicit inverse and the free-cell mask (enqueued on s).
 int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s);
+
+// rc = P^T r (one workgroup per aggregate) and yc = A_c^-1 rc.
+template <typename T>
+int restrict_solve(km_ctx* ctx, const T* r, const int* skip, hipStream_t s);
+int coarse_solve(km_ctx* ctx, const int* skip, hipStream_t s);
 
 // z1 = P A_c^-1 P^T r and r1 = r - A z1 (A in T for the residual, the FP64
 // coefficients for the free-cell mask); Prr (may be null) receives the
-// per-workgroup partials of r1.r1 of a make_flat(NX * NY) launch.  omega > 0
-// fuses the first damped Jacobi sweep: z1 <- z1 + omega D^-1 r1, r1 unused.
+// per-workgroup partials of r1.r1 of a make_flat(NX * NY) launch.
 template <typename T>
 int apply(km_ctx* ctx, const double* coef, const T* coefT, const T* r, T* z1, T* r1, double* Prr, const int* skip,
-          hipStream_t s, double omega = 0.0);
+          hipStream_t s);
 
-// z_out = z_in + omega D^-1 res, with res = r - A z_in computed here when null.
+// z = M(r): z1 = P yc from the current coarse solution yc, then kSweeps damped
+// Jacobi sweeps on r - A z (one tiled launch).
 template <typename T>
-int jacobi(km_ctx* ctx, int NX, int NY, const T* coefT, const T* r, const T* res, const T* zin, T* zout, double omega,
-           const int* skip, hipStream_t s);
+int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s);
+
+// Inner flexible BiCGSTAB for A d = rhs with the two-level Jacobi
+// preconditioner, six launches per iteration; ws holds 8n values of T.
+template <typename T>
+int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
+                bool use_graph, KState* out);
 
 }  // namespace kmcoarse
 

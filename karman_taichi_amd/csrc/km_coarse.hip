@@ -13,18 +13,6 @@
 
 #include <cmath>
 
-struct km_coarse {
-    int NX = 0, NY = 0, B = 0, mI = 0, mJ = 0, m = 0, bw = 0;
-    int block_req = 0;       // km_set_coarse (0: automatic)
-    char* arena = nullptr;
-    size_t bytes = 0;
-    double* cc = nullptr;     // 5 planes of m coarse coefficients
-    double* band = nullptr;   // m x (2 bw + 1) band of A_c, then its LU factors
-    double* inv = nullptr;    // m x m A_c^-1 (symmetric)
-    double* rc = nullptr;     // m restricted residual
-    double* yc = nullptr;     // m coarse solution
-};
-
 namespace {
 
 __device__ __forceinline__ bool free_link(const double* __restrict__ c, int64_t n, int64_t k) {
@@ -75,6 +63,17 @@ k_coarse_coef(int NX, int NY, int B, int mI, int mJ, const double* __restrict__ 
     }
 }
 
+// mask[k] = 1 on interior cells with a non-zero link
+__global__ void __launch_bounds__(KM_BLOCK)
+k_coarse_mask(int NX, int NY, const double* __restrict__ coef, uint8_t* __restrict__ mask) {
+    const int64_t n = (int64_t)NX * NY;
+    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK) {
+        const int i = (int)(k / NY), j = (int)(k % NY);
+        const bool interior = i > 0 && i < NX - 1 && j > 0 && j < NY - 1;
+        mask[k] = interior && free_link(coef, n, k) ? 1 : 0;
+    }
+}
+
 // Band storage: band[r * (2 bw + 1) + bw + (c - r)] = A_c[r][c], |c - r| <= bw.
 // Neighbours of aggregate r in i-major order: r -+ 1 (S / N) and r -+ mJ (W / E).
 __global__ void __launch_bounds__(1024)
@@ -97,55 +96,117 @@ k_coarse_band(int m, int mJ, int bw, const double* __restrict__ cc, double* __re
 
 // Banded LU without pivoting (A_c is SPD) in one workgroup: step k updates the
 // (bw x bw) window below and right of the pivot; L (unit) overwrites the
-// strict lower band, U the rest.
+// strict lower band, U the rest.  The bw + 1 rows that steps k .. k + bw touch
+// live in an LDS ring (row r in slot r mod (bw + 1)); row k goes back to the
+// band once step k is done and row k + bw + 1, untouched so far, takes its slot.
 __global__ void __launch_bounds__(1024)
 k_coarse_factor(int m, int bw, double* __restrict__ band) {
-    const int W = 2 * bw + 1;
+    extern __shared__ double ring[];
+    const int W = 2 * bw + 1, R = bw + 1;
+    for (int e = threadIdx.x; e < R * W; e += blockDim.x) {
+        const int row = e / W;
+        ring[e] = row < m ? band[(int64_t)row * W + (e - row * W)] : 0.0;
+    }
+    __syncthreads();
     for (int k = 0; k < m; ++k) {
         const int rows = (k + bw < m - 1 ? k + bw : m - 1) - k;   // rows k+1 .. k+rows
-        const double piv = band[(int64_t)k * W + bw];
+        const double* pk = ring + (k % R) * W + bw;                 // pk[o] = A[k][k + o]
+        const double piv = pk[0];
         for (int t = threadIdx.x; t < rows * rows; t += blockDim.x) {
             const int i = k + 1 + t / rows, j = k + 1 + t % rows;
-            const double l = band[(int64_t)i * W + bw + (k - i)] / piv;
+            double* ri = ring + (i % R) * W + bw;                   // ri[o] = A[i][i + o]
+            const double l = ri[k - i] / piv;
             const int o = j - i;
-            if (o >= -bw && o <= bw) band[(int64_t)i * W + bw + o] -= l * band[(int64_t)k * W + bw + (j - k)];
+            if (o >= -bw && o <= bw) ri[o] -= l * pk[j - k];
         }
         __syncthreads();
         for (int t = threadIdx.x; t < rows; t += blockDim.x) {
             const int i = k + 1 + t;
-            band[(int64_t)i * W + bw + (k - i)] /= piv;
+            ring[(i % R) * W + bw + (k - i)] /= piv;
+        }
+        __syncthreads();
+        double* slot = ring + (k % R) * W;
+        for (int e = threadIdx.x; e < W; e += blockDim.x) {
+            band[(int64_t)k * W + e] = slot[e];
+            slot[e] = k + R < m ? band[(int64_t)(k + R) * W + e] : 0.0;
         }
         __syncthreads();
     }
 }
 
-// Column c of A_c^-1 (= row c, A_c symmetric) by forward and back substitution
-// with the band factors; one thread per column.
+// A_c^-1, row-major, from the band factors: thread c owns column c and all
+// threads of a workgroup walk the rows together, so the factor entries of row
+// i are the same for every thread and the last BWT values of the column's
+// substitution stay in registers (BWT >= bw).  Forward: L y = e_c for rows
+// from the workgroup's first column on (y = 0 above it); back: U x = y, the
+// rows in reverse.  y waits in inv until the back substitution overwrites it.
+template <int BWT>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_coarse_inverse(int m, int bw, const double* __restrict__ band, double* __restrict__ inv) {
+    const int c0 = blockIdx.x * KM_BLOCK, c = c0 + threadIdx.x;
+    const bool own = c < m;
+    const int W = 2 * bw + 1;
+    double w[BWT];
+#pragma unroll
+    for (int d = 0; d < BWT; ++d) w[d] = 0.0;
+    for (int i = c0; i < m; ++i) {   // w[d] = y[i - 1 - d]
+        const double* row = band + (int64_t)i * W + bw;
+        double l[BWT];
+#pragma unroll
+        for (int d = 0; d < BWT; ++d) l[d] = row[d < bw ? -1 - d : 0];   // in-row, no branch
+        double v = i == c ? 1.0 : 0.0;
+#pragma unroll
+        for (int d = 0; d < BWT; ++d) v -= (d < bw ? l[d] : 0.0) * w[d];
+#pragma unroll
+        for (int d = BWT - 1; d > 0; --d) w[d] = w[d - 1];
+        w[0] = v;
+        if (own) inv[(int64_t)i * m + c] = v;
+    }
+#pragma unroll
+    for (int d = 0; d < BWT; ++d) w[d] = 0.0;
+    for (int i = m - 1; i >= 0; --i) {   // w[d] = x[i + 1 + d]
+        const double* row = band + (int64_t)i * W + bw;
+        double u[BWT];
+#pragma unroll
+        for (int d = 0; d < BWT; ++d) u[d] = row[d < bw ? 1 + d : 0];
+        double v = (own && i >= c0) ? inv[(int64_t)i * m + c] : 0.0;
+#pragma unroll
+        for (int d = 0; d < BWT; ++d) v -= (d < bw ? u[d] : 0.0) * w[d];
+        v = v / row[0];
+#pragma unroll
+        for (int d = BWT - 1; d > 0; --d) w[d] = w[d - 1];
+        w[0] = v;
+        if (own) inv[(int64_t)i * m + c] = v;
+    }
+}
+
+// The same for any bandwidth: one thread per column with the substitution in
+// the output row, each step reading its own factor entries.
+__global__ void __launch_bounds__(KM_BLOCK)
+k_coarse_inverse_wide(int m, int bw, const double* __restrict__ band, double* __restrict__ x) {
     const int c = blockIdx.x * KM_BLOCK + threadIdx.x;
     if (c >= m) return;
     const int W = 2 * bw + 1;
-    double* x = inv + (int64_t)c * m;
-    for (int i = 0; i < c; ++i) x[i] = 0.0;
+    auto X = [&](int i) -> double& { return x[(int64_t)i * m + c]; };
+    for (int i = 0; i < c; ++i) X(i) = 0.0;
     for (int i = c; i < m; ++i) {   // L y = e_c (unit lower)
         double v = i == c ? 1.0 : 0.0;
         const int k0 = i - bw > c ? i - bw : c;
-        for (int k = k0; k < i; ++k) v -= band[(int64_t)i * W + bw + (k - i)] * x[k];
-        x[i] = v;
+        for (int k = k0; k < i; ++k) v -= band[(int64_t)i * W + bw + (k - i)] * X(k);
+        X(i) = v;
     }
     for (int i = m - 1; i >= 0; --i) {   // U x = y
-        double v = x[i];
+        double v = X(i);
         const int k1 = i + bw < m - 1 ? i + bw : m - 1;
-        for (int k = i + 1; k <= k1; ++k) v -= band[(int64_t)i * W + bw + (k - i)] * x[k];
-        x[i] = v / band[(int64_t)i * W + bw];
+        for (int k = i + 1; k <= k1; ++k) v -= band[(int64_t)i * W + bw + (k - i)] * X(k);
+        X(i) = v / band[(int64_t)i * W + bw];
     }
 }
 
 // rc[agg] = sum of r over the aggregate's free cells (one workgroup per aggregate)
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_coarse_restrict(int NX, int NY, int B, int mJ, const double* __restrict__ coef, const T* __restrict__ r,
+k_coarse_restrict(int NX, int NY, int B, int mJ, const uint8_t* __restrict__ mask, const T* __restrict__ r,
                   double* __restrict__ rc, const int* __restrict__ skip) {
     if (skip && *skip) return;
     __shared__ double sh[4];
@@ -154,11 +215,10 @@ k_coarse_restrict(int NX, int NY, int B, int mJ, const double* __restrict__ coef
     const int nx = NX - 2, ny = NY - 2;
     const int i0 = 1 + I * B, j0 = 1 + J * B;
     const int bi = (i0 + B - 1 < nx ? i0 + B - 1 : nx) - i0 + 1, bj = (j0 + B - 1 < ny ? j0 + B - 1 : ny) - j0 + 1;
-    const int64_t n = (int64_t)NX * NY;
     double a = 0.0;
     for (int t = threadIdx.x; t < bi * bj; t += KM_BLOCK) {
         const int64_t k = (int64_t)(i0 + t / bj) * NY + (j0 + t % bj);
-        if (free_link(coef, n, k)) a += (double)r[k];
+        if (mask[k]) a += (double)r[k];
     }
     a = block_sum(a, sh);
     if (threadIdx.x == 0) rc[agg] = a;
@@ -183,15 +243,12 @@ __device__ __forceinline__ double coarse_value(int i, int j, int B, int mJ, int 
     return yc[((i - 1) / B) * mJ + (j - 1) / B];
 }
 
-// z1 = P yc on free cells; r1 = r - A z1; partials of r1.r1 (P_rr, may be null).
-// omega > 0: the first damped Jacobi sweep fused in -- z1 receives
-// z1 + omega D^-1 r1 and r1 is not written.
+// z1 = P yc on free cells; r1 = r - A z1; partials of r1.r1 (P_rr, may be null)
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_coarse_prolong(Flat fl, int NX, int NY, int B, int mJ, const double* __restrict__ coef,
                  const T* __restrict__ coefT, const T* __restrict__ r, const double* __restrict__ yc,
-                 T* __restrict__ z1, T* __restrict__ r1, double* __restrict__ Prr, double omega,
-                 const int* __restrict__ skip) {
+                 T* __restrict__ z1, T* __restrict__ r1, double* __restrict__ Prr, const int* __restrict__ skip) {
     if (skip && *skip) return;
     __shared__ double sh[4];
     const int nx = NX - 2, ny = NY - 2;
@@ -214,42 +271,13 @@ k_coarse_prolong(Flat fl, int NX, int NY, int B, int mJ, const double* __restric
         az = az + coefT[3 * n + k] * zc(i, j + 1);
         az = az + coefT[4 * n + k] * zc(i, j - 1);
         const T rv = r[k] - az;
-        if (omega > 0.0) {
-            const T d = coefT[k];
-            z1[k] = zP + (d != T(0) ? (T)omega * rv / d : T(0));
-        } else {
-            z1[k] = zP;
-            r1[k] = rv;
-        }
+        z1[k] = zP;
+        r1[k] = rv;
         a += (double)rv * (double)rv;
     }
     if (Prr) {
         a = block_sum(a, sh);
         if (threadIdx.x == 0) Prr[blockIdx.x] = a;
-    }
-}
-
-// z_out = z_in + omega D^-1 (r - A z_in); with res (r - A z_in already known) the stencil is skipped
-template <typename T>
-__global__ void __launch_bounds__(KM_BLOCK)
-k_jacobi_sweep(int64_t n, int NX, int NY, const T* __restrict__ c, const T* __restrict__ r, const T* __restrict__ res,
-               const T* __restrict__ zin, T* __restrict__ zout, double omega, const int* __restrict__ skip) {
-    if (skip && *skip) return;
-    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK) {
-        T rk;
-        if (res) {
-            rk = res[k];
-        } else {
-            const int i = (int)(k / NY), j = (int)(k % NY);
-            T az = c[k] * zin[k];
-            az = az + c[n + k] * (i > 0 ? zin[k - NY] : T(0));
-            az = az + c[2 * n + k] * (i < NX - 1 ? zin[k + NY] : T(0));
-            az = az + c[3 * n + k] * (j < NY - 1 ? zin[k + 1] : T(0));
-            az = az + c[4 * n + k] * (j > 0 ? zin[k - 1] : T(0));
-            rk = r[k] - az;
-        }
-        const T d = c[k];
-        zout[k] = zin[k] + (d != T(0) ? (T)omega * rk / d : T(0));
     }
 }
 
@@ -287,8 +315,12 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
     const int mI = (nx + B - 1) / B, mJ = (ny + B - 1) / B, m = mI * mJ;
     KM_REQUIRE(ctx, m <= 8192, "coarse level of %d unknowns is too large (aggregate %d)", m, B);
     const int bw = mJ;
+    // the factorisation's LDS ring: (bw + 1) rows of 2 bw + 1
+    KM_REQUIRE(ctx, (size_t)(bw + 1) * (2 * bw + 1) * sizeof(double) <= 64 * 1024,
+               "coarse band of width %d is too wide (aggregate %d)", bw, B);
+    const int64_t n = (int64_t)NX * NY;
     const size_t need = a256(5 * (size_t)m * 8) + a256((size_t)m * (2 * bw + 1) * 8) + a256((size_t)m * m * 8) +
-                        2 * a256((size_t)m * 8);
+                        2 * a256((size_t)m * 8) + a256((size_t)n);
     if (!C->arena || C->bytes < need) {
         KM_HIP(ctx, hipStreamSynchronize(s));
         km_drop_graphs(ctx);
@@ -305,52 +337,66 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
     C->band = reinterpret_cast<double*>(q); q += a256((size_t)m * (2 * bw + 1) * 8);
     C->inv = reinterpret_cast<double*>(q); q += a256((size_t)m * m * 8);
     C->rc = reinterpret_cast<double*>(q); q += a256((size_t)m * 8);
-    C->yc = reinterpret_cast<double*>(q);
+    C->yc = reinterpret_cast<double*>(q); q += a256((size_t)m * 8);
+    C->mask = reinterpret_cast<uint8_t*>(q);
     C->NX = NX; C->NY = NY; C->B = B; C->mI = mI; C->mJ = mJ; C->m = m; C->bw = bw;
+    hipLaunchKernelGGL(k_coarse_mask, dim3(flat_blocks2(n)), dim3(KM_BLOCK), 0, s, NX, NY, coef, C->mask);
     hipLaunchKernelGGL(k_coarse_coef, dim3(m), dim3(KM_BLOCK), 0, s, NX, NY, B, mI, mJ, coef, C->cc);
     hipLaunchKernelGGL(k_coarse_band, dim3(flat_blocks2((int64_t)m * (2 * bw + 1))), dim3(1024), 0, s, m, mJ, bw,
                        (const double*)C->cc, C->band);
-    hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(1024), 0, s, m, bw, C->band);
-    hipLaunchKernelGGL(k_coarse_inverse, dim3((m + KM_BLOCK - 1) / KM_BLOCK), dim3(KM_BLOCK), 0, s, m, bw,
-                       (const double*)C->band, C->inv);
+    hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(1024), (size_t)(bw + 1) * (2 * bw + 1) * sizeof(double), s, m,
+                       bw, C->band);
+    const dim3 gi((m + KM_BLOCK - 1) / KM_BLOCK);
+    const double* band = C->band;
+    if (bw <= 8)
+        hipLaunchKernelGGL(k_coarse_inverse<8>, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv);
+    else if (bw <= 16)
+        hipLaunchKernelGGL(k_coarse_inverse<16>, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv);
+    else if (bw <= 32)
+        hipLaunchKernelGGL(k_coarse_inverse<32>, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv);
+    else
+        hipLaunchKernelGGL(k_coarse_inverse_wide, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv);
     KM_CHECK_LAUNCH(ctx);
     return KM_OK;
+}
+
+int coarse_solve(km_ctx* ctx, const int* skip, hipStream_t s) {
+    km_coarse* C = ctx->coarse;
+    KM_REQUIRE(ctx, C && C->arena && C->m > 0, "coarse level not set up");
+    hipLaunchKernelGGL(k_coarse_matvec, dim3((C->m + 3) / 4), dim3(KM_BLOCK), 0, s, C->m, (const double*)C->inv,
+                       (const double*)C->rc, C->yc, skip);
+    KM_CHECK_LAUNCH(ctx);
+    return KM_OK;
+}
+
+template <typename T>
+int restrict_solve(km_ctx* ctx, const T* r, const int* skip, hipStream_t s) {
+    km_coarse* C = ctx->coarse;
+    KM_REQUIRE(ctx, C && C->arena && C->m > 0, "coarse level not set up");
+    hipLaunchKernelGGL(k_coarse_restrict<T>, dim3(C->m), dim3(KM_BLOCK), 0, s, C->NX, C->NY, C->B, C->mJ,
+                       (const uint8_t*)C->mask, r, C->rc, skip);
+    KM_CHECK_LAUNCH(ctx);
+    return coarse_solve(ctx, skip, s);
 }
 
 template <typename T>
 int apply(km_ctx* ctx, const double* coef, const T* coefT, const T* r, T* z1, T* r1, double* Prr, const int* skip,
-          hipStream_t s, double omega) {
+          hipStream_t s) {
+    if (int rc = restrict_solve<T>(ctx, r, skip, s)) return rc;
     km_coarse* C = ctx->coarse;
-    KM_REQUIRE(ctx, C && C->arena && C->m > 0, "coarse level not set up");
     const Flat fl = make_flat((int64_t)C->NX * C->NY);
-    hipLaunchKernelGGL(k_coarse_restrict<T>, dim3(C->m), dim3(KM_BLOCK), 0, s, C->NX, C->NY, C->B, C->mJ, coef, r,
-                       C->rc, skip);
-    hipLaunchKernelGGL(k_coarse_matvec, dim3((C->m + 3) / 4), dim3(KM_BLOCK), 0, s, C->m, (const double*)C->inv,
-                       (const double*)C->rc, C->yc, skip);
     hipLaunchKernelGGL(k_coarse_prolong<T>, dim3(fl.G), dim3(KM_BLOCK), 0, s, fl, C->NX, C->NY, C->B, C->mJ, coef,
-                       coefT, r, (const double*)C->yc, z1, r1, Prr, omega, skip);
+                       coefT, r, (const double*)C->yc, z1, r1, Prr, skip);
     KM_CHECK_LAUNCH(ctx);
     return KM_OK;
 }
 
-template <typename T>
-int jacobi(km_ctx* ctx, int NX, int NY, const T* coefT, const T* r, const T* res, const T* zin, T* zout, double omega,
-           const int* skip, hipStream_t s) {
-    const int64_t n = (int64_t)NX * NY;
-    hipLaunchKernelGGL(k_jacobi_sweep<T>, dim3(flat_blocks2(n)), dim3(KM_BLOCK), 0, s, n, NX, NY, coefT, r, res, zin,
-                       zout, omega, skip);
-    KM_CHECK_LAUNCH(ctx);
-    return KM_OK;
-}
-
+template int restrict_solve<float>(km_ctx*, const float*, const int*, hipStream_t);
+template int restrict_solve<double>(km_ctx*, const double*, const int*, hipStream_t);
 template int apply<float>(km_ctx*, const double*, const float*, const float*, float*, float*, double*, const int*,
-                          hipStream_t, double);
+                          hipStream_t);
 template int apply<double>(km_ctx*, const double*, const double*, const double*, double*, double*, double*,
-                           const int*, hipStream_t, double);
-template int jacobi<float>(km_ctx*, int, int, const float*, const float*, const float*, const float*, float*, double,
                            const int*, hipStream_t);
-template int jacobi<double>(km_ctx*, int, int, const double*, const double*, const double*, const double*, double*,
-                            double, const int*, hipStream_t);
 
 }  // namespace kmcoarse
 
@@ -375,7 +421,7 @@ int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const dou
     KM_REQUIRE(ctx, coef && r && z1 && r1, "null pointer");
     KM_REQUIRE(ctx, NX >= 3 && NY >= 3 && (int64_t)NX * NY < ((int64_t)1 << 31), "system shape %d x %d", NX, NY);
     if (int rc = kmcoarse::setup(ctx, NX, NY, coef, ctx->stream)) return rc;
-    return kmcoarse::apply<double>(ctx, coef, coef, r, z1, r1, nullptr, nullptr, ctx->stream, 0.0);
+    return kmcoarse::apply<double>(ctx, coef, coef, r, z1, r1, nullptr, nullptr, ctx->stream);
 }
 
 int km_coarse_info(km_ctx* ctx, int* block, int* unknowns) {

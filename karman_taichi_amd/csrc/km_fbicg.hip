@@ -7,7 +7,8 @@
 // vectors actually produced (x += alpha p^ + omega s^) and v = A p^,
 // t = A s^ are formed from them.  A sweep that does not halve the true
 // residual is taken back and plain FP64 BiCGSTAB finishes the solve.
-// Without a network (params == null) the preconditioner is Jacobi, D^-1.
+// Without a network (params == null) the preconditioner is Jacobi, D^-1, or
+// with the coarse level the two-level Jacobi preconditioner (km_twolevel.hip).
 //
 // Inner iteration i (ring slot ph = i mod 4, as in the plain solver):
 //   K_p  stopping test on <r, r>; p = r + beta (p - omega v)
@@ -30,8 +31,6 @@ constexpr int FB_PT = 3 * KM_MAX_RED;    // [2] <t, s>, <t, t>
 constexpr int FB_PN = 5 * KM_MAX_RED;    // [1] p.p (the network's input scale)
 constexpr int FB_PS = 6 * KM_MAX_RED;    // [1] s.s
 constexpr int FB_PR = 7 * KM_MAX_RED;    // [1] r1.r1 after the coarse correction
-constexpr int kJacobiSweeps = 3;         // damped Jacobi sweeps of the two-level smoother (odd: ends in dst)
-constexpr double kJacobiOmega = 0.8;
 
 __device__ __forceinline__ bool fin(double v) { return v == v && v - v == 0.0; }
 
@@ -295,7 +294,8 @@ const int kFbChunk = 8;
 // coefT: the coefficients in T (the matvecs); c32: FP32 coefficients (the
 // network's features); coef64: the FP64 system (coarse level).  ws holds 10n
 // values of T.  The preconditioner: the network (params) or Jacobi, after
-// the coarse correction when coarse is set.
+// the coarse correction when coarse is set.  The coarse level with Jacobi
+// smoothing runs as the fused iteration of km_twolevel.hip instead.
 template <typename T>
 int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, const double* coef64, const T* rhs, T* xd,
              T* ws, double eps, int max_iter, bool use_graph, const float* params, bool coarse, KState* out) {
@@ -309,7 +309,6 @@ int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, cons
     T* ph = ws + 6 * n;
     T* sh = ws + 7 * n;
     T* r1 = ws + 8 * n;      // residual after the coarse correction
-    T* tmp = ws + 9 * n;     // Jacobi ping-pong
     KState* st = &ctx->st[4];
     double* P = ctx->partials;
     const Flat fl = make_flat(n);
@@ -319,22 +318,9 @@ int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, cons
     const int* skip = &st->done;
     const int jb = blocks(n);
     auto precond = [=](hipStream_t q, const T* src, T* dst, const double* norm_partials) {
-        if (coarse) {
-            if (params) {
-                (void)kmcoarse::apply<T>(ctx, coef64, coefT, src, dst, r1, P + FB_PR, skip, q);
-                (void)kmnet::apply_pre<T>(ctx, NX, NY, c32, r1, dst, skip, q, params, P + FB_PR, fl.G, true);
-            } else {
-                // z1 + the first sweep into dst, then sweeps dst -> tmp -> dst
-                (void)kmcoarse::apply<T>(ctx, coef64, coefT, src, dst, r1, nullptr, skip, q, kJacobiOmega);
-                T* zin = dst;
-                T* zout = tmp;
-                for (int k = 1; k < kJacobiSweeps; ++k) {
-                    (void)kmcoarse::jacobi<T>(ctx, NX, NY, coefT, src, nullptr, zin, zout, kJacobiOmega, skip, q);
-                    T* t_ = zin;
-                    zin = zout;
-                    zout = t_;
-                }
-            }
+        if (coarse && params) {
+            (void)kmcoarse::apply<T>(ctx, coef64, coefT, src, dst, r1, P + FB_PR, skip, q);
+            (void)kmnet::apply_pre<T>(ctx, NX, NY, c32, r1, dst, skip, q, params, P + FB_PR, fl.G, true);
         } else if (params) {
             (void)kmnet::apply_pre<T>(ctx, NX, NY, c32, src, dst, skip, q, params, norm_partials, fl.G, false);
         } else {
@@ -479,9 +465,11 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         int cap = max_iter - total;
         if (cap > inner_cap) cap = inner_cap;
         KState st;
-        if (int rc = fb_inner<T>(ctx, NX, NY, coefT, c32, coef, rhs, d, wT, inner_eps, cap, use_graph, params, coarse,
-                                 &st))
-            return rc;
+        const int rc = coarse && !params
+                           ? kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st)
+                           : fb_inner<T>(ctx, NX, NY, coefT, c32, coef, rhs, d, wT, inner_eps, cap, use_graph, params,
+                                         coarse, &st);
+        if (rc) return rc;
         total += st.iter;
         sweeps++;
         hipLaunchKernelGGL(k_axpy_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const T*)d, rn);

@@ -1,0 +1,428 @@
+// Fused flexible BiCGSTAB with the two-level Jacobi preconditioner
+//   M(r) = z1 + kSweeps damped Jacobi sweeps on r - A z,  z1 = P A_c^-1 P^T r
+// (the aggregation coarse level of km_coarse.hip), the preconditioner of the
+// p^ / s^ hooks of bicgsolver.py:48-58.
+//
+// Six launches per inner iteration i (ring slot ph = i mod KM_RING):
+//   K_P  one workgroup per aggregate.  omega = <t,s> / <t,t>,
+//        rho = <s,r~> - omega <t,r~>, beta = (rho / rho') (alpha / omega);
+//        x += alpha p^ + omega s^, r = s - omega t (iteration i-1),
+//        p = r + beta (p - omega v); rc = P^T p; partial <r,r>
+//   K_M  yc = A_c^-1 rc
+//   K_V  2-D tiles with a 4-cell halo: p^ = M(p) (z1 from yc, three sweeps
+//        on shrinking halos in LDS), v = A p^; partial <r~,v>
+//   K_S  one workgroup per aggregate.  Stopping test on <r,r>;
+//        alpha = rho / <r~,v>; s = r - alpha v; rc = P^T s
+//   K_M  yc = A_c^-1 rc
+//   K_T  tiles: s^ = M(s), t = A s^; partials <t,s>, <t,t>, <s,r~>, <t,r~>
+// The unfused sequence (restrict, matvec, prolongation, three sweeps and the
+// Krylov updates as separate kernels) takes fifteen launches; every vector of
+// the fused form is written once per iteration.
+#include "km_coarse.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int TL_PR = 0;                 // [1] <r, r> (aggregate workgroups)
+constexpr int TL_PV = KM_MAX_RED;        // [1] <r~, v> (tile workgroups)
+constexpr int TL_PT = 2 * KM_MAX_RED;    // [4] <t, s>, <t, t>, <s, r~>, <t, r~>
+constexpr int kStallWindow = 300;        // as the unfused flexible solve
+constexpr int kChunk = 8;
+constexpr int kTI = 32, kTJ = 32;        // tile of the smoothing launches
+
+__device__ __forceinline__ bool fin(double v) { return v == v && v - v == 0.0; }
+
+__device__ __forceinline__ void finish_state(KState* st, int status, int iter, double rr) {
+    st->status = status;
+    st->iter = iter;
+    st->rTr = rr;
+    st->done = 1;
+}
+
+// Geometry of the aggregate launches: aggregate agg covers the interior block
+// [i0, i0 + bi) x [j0, j0 + bj); edge aggregates also own the adjacent halo
+// cells, so every cell of the NX x NY system belongs to exactly one aggregate.
+struct AggGeom {
+    int NX, NY, B, mI, mJ, m, G;
+};
+
+// f(k, free) over the cells of aggregate agg: the interior block first, in the
+// thread-strided order of the unfused restriction, then the halo cells.
+template <typename F>
+__device__ __forceinline__ void agg_cells(const AggGeom& g, int agg, const uint8_t* __restrict__ mask, F f) {
+    const int I = agg / g.mJ, J = agg - I * g.mJ;
+    const int nx = g.NX - 2, ny = g.NY - 2;
+    const int i0 = 1 + I * g.B, j0 = 1 + J * g.B;
+    const int bi = (i0 + g.B - 1 < nx ? i0 + g.B - 1 : nx) - i0 + 1;
+    const int bj = (j0 + g.B - 1 < ny ? j0 + g.B - 1 : ny) - j0 + 1;
+    for (int t = threadIdx.x; t < bi * bj; t += KM_BLOCK) {
+        const int64_t k = (int64_t)(i0 + t / bj) * g.NY + (j0 + t % bj);
+        f(k, mask[k] != 0);
+    }
+    if (I == 0 || I == g.mI - 1 || J == 0 || J == g.mJ - 1) {
+        const int ie0 = I == 0 ? 0 : i0, ie1 = I == g.mI - 1 ? g.NX - 1 : i0 + bi - 1;
+        const int je0 = J == 0 ? 0 : j0, je1 = J == g.mJ - 1 ? g.NY - 1 : j0 + bj - 1;
+        const int ei = ie1 - ie0 + 1, ej = je1 - je0 + 1;
+        for (int t = threadIdx.x; t < ei * ej; t += KM_BLOCK) {
+            const int i = ie0 + t / ej, j = je0 + t % ej;
+            if (i >= i0 && i < i0 + bi && j >= j0 && j < j0 + bj) continue;
+            f((int64_t)i * g.NY + j, false);
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_tl_init(Flat fl, const T* __restrict__ rhs, T* __restrict__ x, T* __restrict__ r, T* __restrict__ rt, double eps,
+          int max_iter, KState* __restrict__ st) {
+    const int64_t lo = (int64_t)blockIdx.x * fl.chunk;
+    const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
+    for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
+        const T b = rhs[k];
+        x[k] = T(0);
+        r[k] = b;
+        rt[k] = b;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->eps = eps;
+        st->max_iter = max_iter;
+        st->done = 0;
+        st->status = KM_MAXITER;
+        st->iter = 0;
+        st->r0 = 0.0;
+        st->rTr = 0.0;
+        st->ref = 0.0;
+        st->ring[KM_RING - 1].iter = -1;
+        st->ring[KM_RING - 1].best = 1e300;
+        st->ring[KM_RING - 1].best_it = 0;
+    }
+}
+
+// K_P: the x / r update of iteration i-1, the p update of iteration i, P^T p
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restrict__ x, T* __restrict__ r,
+       T* __restrict__ p, const T* __restrict__ v, const T* __restrict__ s, const T* __restrict__ t,
+       const T* __restrict__ ph_, const T* __restrict__ sh_, double* __restrict__ rc, KState* __restrict__ st,
+       double* __restrict__ P) {
+    __shared__ double sh[8];
+    if (st->done) return;
+    const KScal sp = st->ring[(ph + KM_RING - 1) % KM_RING];
+    const int i = sp.iter + 1;
+    double omega = 0.0, beta = 0.0;
+    if (i > 0) {
+        double pt[4];
+        sum_partials<4>(P + TL_PT, Gt, pt, sh);
+        omega = pt[1] != 0.0 ? pt[0] / pt[1] : 0.0;                  // bicgsolver.py:118
+        const double rho = pt[2] - omega * pt[3];                    // <r_i, r~> (bicgsolver.py:95)
+        beta = omega != 0.0 ? (rho / sp.rho) * (sp.alpha / omega) : 0.0;   // bicgsolver.py:102
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->ring[ph].rho = rho;
+            st->ring[ph].omega = omega;
+            st->ring[ph].beta = beta;
+            st->ring[ph].iter = i;
+        }
+    } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->ring[ph].iter = 0;
+        st->ring[ph].omega = 1.0;
+    }
+    const double alpha = sp.alpha;
+    const T fo = (T)omega, fb = (T)beta;
+    double rr = 0.0;
+    for (int agg = blockIdx.x; agg < g.m; agg += g.G) {
+        double a = 0.0;
+        agg_cells(g, agg, mask, [&](int64_t k, bool free) {
+            T rn;
+            T pk;
+            if (i == 0) {
+                rn = r[k];
+                pk = rn;
+            } else {
+                x[k] = x[k] + (T)(alpha * (double)ph_[k] + omega * (double)sh_[k]);
+                rn = s[k] - fo * t[k];
+                r[k] = rn;
+                pk = rn + fb * (p[k] - fo * v[k]);
+            }
+            p[k] = pk;
+            rr += (double)rn * (double)rn;
+            if (free) a += (double)pk;
+        });
+        a = block_sum(a, sh);
+        if (threadIdx.x == 0) rc[agg] = a;
+    }
+    rr = block_sum(rr, sh);
+    if (threadIdx.x == 0) P[TL_PR + blockIdx.x] = rr;
+}
+
+// K_S: stopping test, alpha, s = r - alpha v, P^T s
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_tl_s(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, const T* __restrict__ r,
+       const T* __restrict__ v, T* __restrict__ s, double* __restrict__ rc, KState* __restrict__ st,
+       double* __restrict__ P) {
+    __shared__ double sh[8];
+    if (st->done) return;
+    double tv[1], tr[1];
+    sum_partials<1>(P + TL_PV, Gt, tv, sh);
+    sum_partials<1>(P + TL_PR, g.G, tr, sh);
+    const double rv = tv[0], rr = tr[0];
+    const KScal sp = st->ring[ph];
+    const KScal spp = st->ring[(ph + KM_RING - 1) % KM_RING];
+    const int i = sp.iter;
+    const double rho = i == 0 ? rr : sp.rho;
+    const double r0 = i == 0 ? rr : st->r0;
+    const bool lead0 = blockIdx.x == 0 && threadIdx.x == 0;
+    int status = -1;
+    if (!fin(rr)) status = KM_NONFINITE;
+    else if (i == 0 && rr == 0.0) status = KM_OK;
+    else if (i > 0 && sqrt(rr / r0) < st->eps) status = KM_OK;   // bicgsolver.py:130
+    else if (i > 0 && sp.omega == 0.0) status = KM_BREAKDOWN;
+    else if (i >= st->max_iter) status = KM_MAXITER;
+    else if (!fin(rho)) status = KM_NONFINITE;
+    else if (rho == 0.0) status = KM_BREAKDOWN;
+    else if (!fin(rv)) status = KM_NONFINITE;
+    else if (rv == 0.0) status = KM_BREAKDOWN;
+    const bool better = rr < 0.81 * spp.best;
+    if (status < 0 && !better && i - spp.best_it >= kStallWindow) status = KM_STALL;
+    if (lead0 && i == 0) st->r0 = r0;
+    if (status >= 0) {
+        if (lead0) finish_state(st, status, i, rr);
+        return;
+    }
+    const double alpha = rho / rv;   // bicgsolver.py:108
+    if (lead0) {
+        st->ring[ph].alpha = alpha;
+        if (i == 0) st->ring[ph].rho = rho;
+        st->ring[ph].best = better ? rr : spp.best;
+        st->ring[ph].best_it = better ? i : spp.best_it;
+        st->rTr = rr;
+        st->iter = i;
+    }
+    const T fa = (T)alpha;
+    for (int agg = blockIdx.x; agg < g.m; agg += g.G) {
+        double a = 0.0;
+        agg_cells(g, agg, mask, [&](int64_t k, bool free) {
+            const T sk = r[k] - fa * v[k];
+            s[k] = sk;
+            if (free) a += (double)sk;
+        });
+        a = block_sum(a, sh);
+        if (threadIdx.x == 0) rc[agg] = a;
+    }
+}
+
+// K_V / K_T / the standalone smoother: z = M(src) on the tiles, then (MODE 1)
+// v = A z with <r~, v> or (MODE 2) t = A z with <t,s>, <t,t>, <s,r~>, <t,r~>.
+// The coefficients and src of the tile's 3-cell halo frame stay in registers
+// (Q cells per thread); z lives in LDS on the 4-cell halo frame, ping-ponged
+// between two buffers over the sweeps.  The arithmetic is that of one
+// damped Jacobi sweep per cell: z + omega (src - A z) / aP, neighbours
+// outside the system read as zero, rows with aP = 0 left unchanged.
+template <typename T, int TI, int TJ, int MODE>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_tl_smooth(Tile tl, int B, int mJ, const T* __restrict__ c, const T* __restrict__ src,
+            const uint8_t* __restrict__ mask, const double* __restrict__ yc, const T* __restrict__ rt,
+            T* __restrict__ z, T* __restrict__ y, const KState* __restrict__ st, double* __restrict__ P) {
+    constexpr int W4 = TJ + 8, H4 = TI + 8;
+    constexpr int W3 = TJ + 6, H3 = TI + 6;
+    constexpr int N3 = W3 * H3;
+    constexpr int Q = (N3 + KM_BLOCK - 1) / KM_BLOCK;
+    __shared__ T za[H4 * W4];
+    __shared__ T zb[H4 * W4];
+    __shared__ double sh[8];
+    if (st && st->done) return;
+    const int L = xcd_remap(blockIdx.x, tl.G);
+    const int NX = tl.NX, NY = tl.NY;
+    const int64_t n = (int64_t)NX * NY;
+    const T om = (T)kmcoarse::kOmega;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const int t0 = L * tl.tpb, t1 = t0 + tl.tpb < tl.T ? t0 + tl.tpb : tl.T;
+    for (int tile = t0; tile < t1; ++tile) {
+        const int ti0 = (tile / tl.tiles_j) * TI, tj0 = (tile % tl.tiles_j) * TJ;
+        T cr[Q][5], sr[Q];
+        int64_t kq[Q];
+        bool in[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int idx = threadIdx.x + q * KM_BLOCK;
+            const int a = idx / W3, b = idx - (idx / W3) * W3;
+            const int i = ti0 - 3 + a, j = tj0 - 3 + b;
+            in[q] = idx < N3 && i >= 0 && i < NX && j >= 0 && j < NY;
+            kq[q] = in[q] ? (int64_t)i * NY + j : 0;
+#pragma unroll
+            for (int e = 0; e < 5; ++e) cr[q][e] = in[q] ? c[e * n + kq[q]] : T(0);
+            sr[q] = in[q] ? src[kq[q]] : T(0);
+        }
+        // z1 = P yc on the 4-cell halo frame (zero off the free cells and outside the system)
+        for (int idx = threadIdx.x; idx < H4 * W4; idx += KM_BLOCK) {
+            const int a = idx / W4, b = idx - (idx / W4) * W4;
+            const int i = ti0 - 4 + a, j = tj0 - 4 + b;
+            T v = T(0);
+            if (i >= 0 && i < NX && j >= 0 && j < NY && mask[(int64_t)i * NY + j])
+                v = (T)yc[((i - 1) / B) * mJ + (j - 1) / B];
+            za[idx] = v;
+        }
+        __syncthreads();
+        // sweeps on the halo-3, -2 and -1 frames: za -> zb -> za -> zb
+#pragma unroll
+        for (int h = 3; h >= 1; --h) {
+            const T* __restrict__ zi = (h == 2) ? zb : za;
+            T* __restrict__ zo = (h == 2) ? za : zb;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int idx = threadIdx.x + q * KM_BLOCK;
+                const int a = idx / W3, b = idx - (idx / W3) * W3;
+                if (idx >= N3 || a < 3 - h || a > TI + 2 + h || b < 3 - h || b > TJ + 2 + h) continue;
+                const int p4 = (a + 1) * W4 + (b + 1);
+                T out = T(0);
+                if (in[q]) {
+                    const T zc = zi[p4];
+                    T az = cr[q][0] * zc;
+                    az = az + cr[q][1] * zi[p4 - W4];
+                    az = az + cr[q][2] * zi[p4 + W4];
+                    az = az + cr[q][3] * zi[p4 + 1];
+                    az = az + cr[q][4] * zi[p4 - 1];
+                    const T rk = sr[q] - az;
+                    const T d = cr[q][0];
+                    out = zc + (d != T(0) ? om * rk / d : T(0));
+                }
+                zo[p4] = out;
+            }
+            __syncthreads();
+        }
+        // the tile itself: z out, then y = A z
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int idx = threadIdx.x + q * KM_BLOCK;
+            const int a = idx / W3, b = idx - (idx / W3) * W3;
+            if (idx >= N3 || a < 3 || a > TI + 2 || b < 3 || b > TJ + 2 || !in[q]) continue;
+            const int p4 = (a + 1) * W4 + (b + 1);
+            const int64_t k = kq[q];
+            const T zc = zb[p4];
+            z[k] = zc;
+            if (MODE > 0) {
+                T yv = cr[q][0] * zc;
+                yv = yv + cr[q][1] * zb[p4 - W4];
+                yv = yv + cr[q][2] * zb[p4 + W4];
+                yv = yv + cr[q][3] * zb[p4 + 1];
+                yv = yv + cr[q][4] * zb[p4 - 1];
+                y[k] = yv;
+                const double rtk = (double)rt[k];
+                if (MODE == 1) {
+                    a0 += rtk * (double)yv;
+                } else {
+                    const double sk = (double)sr[q];
+                    a0 += (double)yv * sk;
+                    a1 += (double)yv * (double)yv;
+                    a2 += sk * rtk;
+                    a3 += (double)yv * rtk;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (MODE == 1) {
+        a0 = block_sum(a0, sh);
+        if (threadIdx.x == 0) P[TL_PV + L] = a0;
+    } else if (MODE == 2) {
+        a0 = block_sum(a0, sh);
+        a1 = block_sum(a1, sh);
+        a2 = block_sum(a2, sh);
+        a3 = block_sum(a3, sh);
+        if (threadIdx.x == 0) {
+            P[TL_PT + L] = a0;
+            P[TL_PT + KM_MAX_RED + L] = a1;
+            P[TL_PT + 2 * KM_MAX_RED + L] = a2;
+            P[TL_PT + 3 * KM_MAX_RED + L] = a3;
+        }
+    }
+}
+
+AggGeom agg_geom(const km_coarse* C) {
+    AggGeom g;
+    g.NX = C->NX; g.NY = C->NY; g.B = C->B; g.mI = C->mI; g.mJ = C->mJ; g.m = C->m;
+    g.G = C->m < KM_MAX_RED ? C->m : KM_MAX_RED;
+    return g;
+}
+
+}  // namespace
+
+namespace kmcoarse {
+
+template <typename T>
+int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s) {
+    const km_coarse* C = ctx->coarse;
+    KM_REQUIRE(ctx, C && C->arena && C->m > 0, "coarse level not set up");
+    const Tile tl = make_tile(C->NX, C->NY, kTI, kTJ);
+    hipLaunchKernelGGL((k_tl_smooth<T, kTI, kTJ, 0>), dim3(tl.G), dim3(KM_BLOCK), 0, s, tl, C->B, C->mJ, coefT, r,
+                       (const uint8_t*)C->mask, (const double*)C->yc, (const T*)nullptr, z, (T*)nullptr,
+                       (const KState*)nullptr, (double*)nullptr);
+    KM_CHECK_LAUNCH(ctx);
+    return KM_OK;
+}
+
+template <typename T>
+int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
+                bool use_graph, KState* out) {
+    km_coarse* C = ctx->coarse;
+    KM_REQUIRE(ctx, C && C->arena && C->NX == NX && C->NY == NY, "coarse level not set up for this system");
+    const int64_t n = (int64_t)NX * NY;
+    T* r = ws;
+    T* rt = ws + n;
+    T* p = ws + 2 * n;
+    T* v = ws + 3 * n;
+    T* s = ws + 4 * n;
+    T* t = ws + 5 * n;
+    T* ph = ws + 6 * n;
+    T* sh = ws + 7 * n;
+    KState* st = &ctx->st[4];
+    double* P = ctx->partials;
+    const Flat fl = make_flat(n);
+    const Tile tl = make_tile(NX, NY, kTI, kTJ);
+    const AggGeom g = agg_geom(C);
+    const uint8_t* mask = C->mask;
+    double* rc = C->rc;
+    const double* yc = C->yc;
+    hipLaunchKernelGGL(k_tl_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, rhs, xd, r, rt, eps, max_iter, st);
+    KM_CHECK_LAUNCH(ctx);
+    const int* skip = &st->done;
+    auto enqueue = [=](hipStream_t q, int phs) {
+        hipLaunchKernelGGL(k_tl_p<T>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, tl.G, phs, mask, xd, r, p, (const T*)v,
+                           (const T*)s, (const T*)t, (const T*)ph, (const T*)sh, rc, st, P);
+        (void)coarse_solve(ctx, skip, q);
+        hipLaunchKernelGGL((k_tl_smooth<T, kTI, kTJ, 1>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, g.B, g.mJ, coefT,
+                           (const T*)p, mask, yc, (const T*)rt, ph, v, (const KState*)st, P);
+        hipLaunchKernelGGL(k_tl_s<T>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, tl.G, phs, mask, (const T*)r, (const T*)v, s,
+                           rc, st, P);
+        (void)coarse_solve(ctx, skip, q);
+        hipLaunchKernelGGL((k_tl_smooth<T, kTI, kTJ, 2>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, g.B, g.mJ, coefT,
+                           (const T*)s, mask, yc, (const T*)rt, sh, t, (const KState*)st, P);
+    };
+    GraphKey key;
+    key.v = {8, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
+             (uintptr_t)ws, (uintptr_t)st, (uintptr_t)C, (uintptr_t)C->arena, (uintptr_t)C->m, (uintptr_t)C->B};
+    return run_chunks(ctx, st, key, use_graph, kChunk, max_iter, enqueue, out);
+}
+
+template int smooth<float>(km_ctx*, const float*, const float*, float*, hipStream_t);
+template int smooth<double>(km_ctx*, const double*, const double*, double*, hipStream_t);
+template int fused_inner<float>(km_ctx*, int, int, const float*, const float*, float*, float*, double, int, bool,
+                                KState*);
+template int fused_inner<double>(km_ctx*, int, int, const double*, const double*, double*, double*, double, int,
+                                 bool, KState*);
+
+}  // namespace kmcoarse
+
+extern "C" {
+
+int km_twolevel_precond(km_ctx* ctx, int NX, int NY, const double* coef, const double* r, double* z) {
+    if (!ctx) return KM_EINVAL;
+    KM_REQUIRE(ctx, coef && r && z, "null pointer");
+    KM_REQUIRE(ctx, NX >= 3 && NY >= 3 && (int64_t)NX * NY < ((int64_t)1 << 31), "system shape %d x %d", NX, NY);
+    if (int rc = kmcoarse::setup(ctx, NX, NY, coef, ctx->stream)) return rc;
+    if (int rc = kmcoarse::restrict_solve<double>(ctx, r, nullptr, ctx->stream)) return rc;
+    return kmcoarse::smooth<double>(ctx, coef, r, z, ctx->stream);
+}
+
+}  // extern "C"

@@ -238,6 +238,54 @@ def test_coarse_correction_matches_oracle(dev):
     assert ez < 1e-10 and er < 1e-12
 
 
+@pytest.mark.parametrize("which,block,nx,ny", [("c1", 0, 0, 0), ("small", 8, 101, 43), ("small", 5, 101, 43),
+                                              ("small", 2, 150, 60)])
+def test_twolevel_precond_matches_oracle(which, block, nx, ny, dev):
+    """M(r) of the fused smoothing kernel (P A_c^-1 P^T r, then three damped
+    Jacobi sweeps on 32 x 32 tiles with a 4-cell halo) against the oracle's
+    dense restatement: ragged systems whose tiles and aggregates do not divide
+    the grid, aggregates of 2 x 2 (a coarse level of 2000+ unknowns, so one
+    workgroup owns several aggregates: 2250 > 2048 workgroups), an aggregate
+    size that is not a power of two."""
+    if which == "c1":
+        coef, r = c1_pcor()
+    else:
+        coef, r = small_system(5, nx=nx, ny=ny)
+    ctx = _lib.context_for(dev)
+    ct, rt_ = T(coef, dev), T(r, dev)
+    z = torch.empty_like(rt_)
+    ctx.call("km_set_coarse", block)
+    try:
+        ctx.call("km_twolevel_precond", r.shape[0], r.shape[1], ptr(ct), ptr(rt_), ptr(z))
+        blk, m = ctypes.c_int(), ctypes.c_int()
+        ctx.call("km_coarse_info", ctypes.byref(blk), ctypes.byref(m))
+    finally:
+        ctx.call("km_set_coarse", 0)
+    B = krylov.coarse_block(r.shape[0] - 2, r.shape[1] - 2, block)
+    assert blk.value == B
+    zo = krylov.twolevel_jacobi_precond(coef, B)(r)
+    e = rel(z.cpu().numpy(), zo)
+    print(f"two-level M(r) {which} B={B} ({m.value} unknowns): rel {e:.2e}")
+    assert e < 1e-10
+
+
+def test_twolevel_jacobi_many_aggregates(dev):
+    """The fused flexible solve with 2 x 2 aggregates on a ragged system
+    (several aggregates per workgroup in the Krylov update kernels)."""
+    coef, b = small_system(9, nx=150, ny=61)
+    ctx = _lib.context_for(dev)
+    ctx.call("km_set_coarse", 2)
+    try:
+        x, info = coarse_flex(dev, coef, b, None, _lib.KM_FLAG_COARSE)
+    finally:
+        ctx.call("km_set_coarse", 0)
+    true = np.linalg.norm(b - krylov.stencil(coef, x)) / np.linalg.norm(b)
+    _, io = krylov.fbicgstab(coef, b, eps=1e-8, precond=krylov.twolevel_jacobi_precond(coef, 2), fallback=False)
+    print(f"two-level 2x2: gpu {info.iterations} it, oracle {io['iterations']} it, true {true:.3e}")
+    assert info.status == _lib.KM_OK and true < 1e-8 and info.fallback == 0
+    assert abs(info.iterations - io["iterations"]) <= 0.25 * io["iterations"] + 5
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_twolevel_jacobi_matches_oracle(mixed, dev):
     """Two-level preconditioner without the network: coarse correction + 3 damped
