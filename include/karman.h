@@ -225,6 +225,11 @@ int km_twolevel_precond(km_ctx* ctx, int NX, int NY, const double* coef, const d
 #define KM_KERNEL_BICG_A 0   /* x, r, p updates + stencil + <r~, v>, <r, r> */
 #define KM_KERNEL_BICG_B 1   /* s update + stencil + <t, s>, <t, t>, <s, r~>, <t, r~> */
 #define KM_KERNEL_STENCIL 3  /* plain y = A x */
+/* the fused two-level flexible iteration (KM_FLAG_COARSE, no network) */
+#define KM_KERNEL_TL_P 4     /* x, r, p updates + P^T p + <r, r> */
+#define KM_KERNEL_TL_V 5     /* p^ = M(p): prolongation + 3 Jacobi sweeps, v = A p^, <r~, v> */
+#define KM_KERNEL_TL_S 6     /* stopping test, s update + P^T s */
+#define KM_KERNEL_TL_T 7     /* s^ = M(s), t = A s^, <t, s>, <t, t>, <s, r~>, <t, r~> */
 #define KM_KERNEL_BACK_TO_BACK 0x100   /* or-ed in: time `reps` identical back-to-back launches between two events */
 int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps,
                    double* avg_us);

@@ -14,7 +14,7 @@ struct km_coarse {
     double* cc = nullptr;     // 5 planes of m coarse coefficients
     double* band = nullptr;   // m x (2 bw + 1) band of A_c, then its LU factors
     double* inv = nullptr;    // m x m A_c^-1, row-major
-    double* rc = nullptr;     // m restricted residual
+    double* rc = nullptr;     // kMaxSlices x m restricted residual (per-slice partial sums)
     double* yc = nullptr;     // m coarse solution
     uint8_t* mask = nullptr;  // NX x NY: 1 on the free interior cells (the support of P)
 };
@@ -24,6 +24,9 @@ namespace kmcoarse {
 // Damped Jacobi sweeps of the two-level smoother and their damping.
 constexpr int kSweeps = 3;
 constexpr double kOmega = 0.8;
+// Row slices per aggregate in the fused Krylov updates: their partial
+// restrictions rc[s m + k] are summed by the coarse solve.
+constexpr int kMaxSlices = 8;
 
 // Aggregate edge length for an nx x ny interior (req > 0: that size; else the
 // smallest of 8, 16, ..., 256 that keeps the coarse level at <= 1024 unknowns).
@@ -36,7 +39,8 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s);
 // rc = P^T r (one workgroup per aggregate) and yc = A_c^-1 rc.
 template <typename T>
 int restrict_solve(km_ctx* ctx, const T* r, const int* skip, hipStream_t s);
-int coarse_solve(km_ctx* ctx, const int* skip, hipStream_t s);
+// yc = A_c^-1 rc with rc given as S slice partials.
+int coarse_solve(km_ctx* ctx, int S, const int* skip, hipStream_t s);
 
 // z1 = P A_c^-1 P^T r and r1 = r - A z1 (A in T for the residual, the FP64
 // coefficients for the free-cell mask); Prr (may be null) receives the
@@ -55,6 +59,10 @@ int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s);
 template <typename T>
 int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
                 bool use_graph, KState* out);
+
+// km_time_kernel for the fused kernels (which 4..7: K_P, K_V, K_S, K_T).
+int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps, bool b2b,
+                double* avg_us);
 
 }  // namespace kmcoarse
 

@@ -94,12 +94,40 @@ k_coarse_band(int m, int mJ, int bw, const double* __restrict__ cc, double* __re
     }
 }
 
-// Banded LU without pivoting (A_c is SPD) in one workgroup: step k updates the
+// Banded LU without pivoting (A_c is SPD) in one wave: step k updates the
 // (bw x bw) window below and right of the pivot; L (unit) overwrites the
 // strict lower band, U the rest.  The bw + 1 rows that steps k .. k + bw touch
 // live in an LDS ring (row r in slot r mod (bw + 1)); row k goes back to the
 // band once step k is done and row k + bw + 1, untouched so far, takes its slot.
-__global__ void __launch_bounds__(1024)
+// The whole band in LDS: no global traffic inside the elimination.
+__global__ void __launch_bounds__(64)
+k_coarse_factor_lds(int m, int bw, double* __restrict__ band) {
+    extern __shared__ double a[];
+    const int W = 2 * bw + 1;
+    for (int e = threadIdx.x; e < m * W; e += blockDim.x) a[e] = band[e];
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        const int rows = (k + bw < m - 1 ? k + bw : m - 1) - k;
+        const double* pk = a + k * W + bw;
+        const double piv = pk[0];
+        for (int t = threadIdx.x; t < rows * rows; t += blockDim.x) {
+            const int i = k + 1 + t / rows, j = k + 1 + t % rows;
+            double* ri = a + i * W + bw;
+            const double l = ri[k - i] / piv;
+            const int o = j - i;
+            if (o >= -bw && o <= bw) ri[o] -= l * pk[j - k];
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < rows; t += blockDim.x) {
+            const int i = k + 1 + t;
+            a[i * W + bw + (k - i)] /= piv;
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < m * W; e += blockDim.x) band[e] = a[e];
+}
+
+__global__ void __launch_bounds__(64)
 k_coarse_factor(int m, int bw, double* __restrict__ band) {
     extern __shared__ double ring[];
     const int W = 2 * bw + 1, R = bw + 1;
@@ -107,6 +135,17 @@ k_coarse_factor(int m, int bw, double* __restrict__ band) {
         const int row = e / W;
         ring[e] = row < m ? band[(int64_t)row * W + (e - row * W)] : 0.0;
     }
+    // the row entering the ring after step k, loaded during step k (thread e
+    // holds entries e, e + 64, ...; W <= 4 * 64 when the ring fits 64 KiB)
+    double inc[4];
+    auto fetch = [&](int row) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = threadIdx.x + u * 64;
+            inc[u] = row < m && e < W ? band[(int64_t)row * W + e] : 0.0;
+        }
+    };
+    fetch(R);
     __syncthreads();
     for (int k = 0; k < m; ++k) {
         const int rows = (k + bw < m - 1 ? k + bw : m - 1) - k;   // rows k+1 .. k+rows
@@ -126,57 +165,125 @@ k_coarse_factor(int m, int bw, double* __restrict__ band) {
         }
         __syncthreads();
         double* slot = ring + (k % R) * W;
-        for (int e = threadIdx.x; e < W; e += blockDim.x) {
-            band[(int64_t)k * W + e] = slot[e];
-            slot[e] = k + R < m ? band[(int64_t)(k + R) * W + e] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = threadIdx.x + u * 64;
+            if (e < W) {
+                band[(int64_t)k * W + e] = slot[e];
+                slot[e] = inc[u];
+            }
         }
+        fetch(k + R + 1);
         __syncthreads();
     }
 }
 
 // A_c^-1, row-major, from the band factors: thread c owns column c and all
-// threads of a workgroup walk the rows together, so the factor entries of row
-// i are the same for every thread and the last BWT values of the column's
-// substitution stay in registers (BWT >= bw).  Forward: L y = e_c for rows
-// from the workgroup's first column on (y = 0 above it); back: U x = y, the
-// rows in reverse.  y waits in inv until the back substitution overwrites it.
+// threads of a workgroup walk the rows together, so the factor entries of a
+// row are the same for every thread (read from an LDS copy of CH rows at a
+// time; the next CH rows are loaded into registers meanwhile) and the last
+// BWT values of the column's substitution stay in registers (BWT >= bw; the
+// LDS rows are padded to 2 BWT + 1 entries with zeros, so every row is read
+// without a branch).
+// Forward: L y = e_c for rows from the workgroup's first column on (y = 0
+// above it); back: U x = y, the rows in reverse, y read back from inv CH rows
+// ahead.  The bw products of a row are summed as a tree.
+template <int BWT>
+__device__ __forceinline__ double tree_dot(const double* __restrict__ f, int stride, const double (&w)[BWT]) {
+    double t[BWT];
+#pragma unroll
+    for (int d = 0; d < BWT; ++d) t[d] = f[d * stride] * w[d];
+#pragma unroll
+    for (int h = BWT / 2; h >= 1; h /= 2)
+#pragma unroll
+        for (int d = 0; d < h; ++d) t[d] += t[d + h];
+    return t[0];
+}
+
 template <int BWT>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_coarse_inverse(int m, int bw, const double* __restrict__ band, double* __restrict__ inv) {
+    constexpr int CH = 32;                        // rows per LDS chunk
+    constexpr int WMAX = 2 * BWT + 1;
+    constexpr int PF = (CH * WMAX + KM_BLOCK - 1) / KM_BLOCK;   // chunk values per thread
+    __shared__ double fb[CH * WMAX];
     const int c0 = blockIdx.x * KM_BLOCK, c = c0 + threadIdx.x;
     const bool own = c < m;
-    const int W = 2 * bw + 1;
+    const int W = 2 * bw + 1;   // the band's row length
+    double pf[PF];
+    // rows [r0, r0 + CH) of the band into pf, padded to WMAX entries per row
+    // (zero past the bandwidth and past m) / from pf into fb
+    auto fetch = [&](int r0) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int e = threadIdx.x + u * KM_BLOCK;
+            const int row = r0 + e / WMAX, o = e % WMAX - BWT;
+            const bool in = e < CH * WMAX && row >= 0 && row < m && o >= -bw && o <= bw;
+            pf[u] = in ? band[(int64_t)row * W + bw + o] : 0.0;
+        }
+    };
+    auto stage = [&]() {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int e = threadIdx.x + u * KM_BLOCK;
+            if (e < CH * WMAX) fb[e] = pf[u];
+        }
+        __syncthreads();
+    };
     double w[BWT];
 #pragma unroll
     for (int d = 0; d < BWT; ++d) w[d] = 0.0;
-    for (int i = c0; i < m; ++i) {   // w[d] = y[i - 1 - d]
-        const double* row = band + (int64_t)i * W + bw;
-        double l[BWT];
+    fetch(c0);
+    for (int i0 = c0; i0 < m; i0 += CH) {   // w[d] = y[i - 1 - d]
+        stage();
+        fetch(i0 + CH);
+        const int n = m - i0 < CH ? m - i0 : CH;
+        for (int u = 0; u < n; ++u) {
+            const int i = i0 + u;
+            const double* row = fb + u * WMAX + BWT;   // row[o] = factor entry (i, i + o), |o| <= BWT
+            const double v = (i == c ? 1.0 : 0.0) - tree_dot<BWT>(row - 1, -1, w);
 #pragma unroll
-        for (int d = 0; d < BWT; ++d) l[d] = row[d < bw ? -1 - d : 0];   // in-row, no branch
-        double v = i == c ? 1.0 : 0.0;
-#pragma unroll
-        for (int d = 0; d < BWT; ++d) v -= (d < bw ? l[d] : 0.0) * w[d];
-#pragma unroll
-        for (int d = BWT - 1; d > 0; --d) w[d] = w[d - 1];
-        w[0] = v;
-        if (own) inv[(int64_t)i * m + c] = v;
+            for (int d = BWT - 1; d > 0; --d) w[d] = w[d - 1];
+            w[0] = v;
+            if (own) inv[(int64_t)i * m + c] = v;
+        }
     }
 #pragma unroll
     for (int d = 0; d < BWT; ++d) w[d] = 0.0;
-    for (int i = m - 1; i >= 0; --i) {   // w[d] = x[i + 1 + d]
-        const double* row = band + (int64_t)i * W + bw;
-        double u[BWT];
+    const int top = ((m - 1) / CH) * CH;     // chunks [i0, i0 + CH) from the last one down
+    double yv[CH];
+    auto fetch_y = [&](int r0) {
 #pragma unroll
-        for (int d = 0; d < BWT; ++d) u[d] = row[d < bw ? 1 + d : 0];
-        double v = (own && i >= c0) ? inv[(int64_t)i * m + c] : 0.0;
+        for (int u = 0; u < CH; ++u) {
+            const int i = r0 + u;
+            yv[u] = own && i >= c0 && i < m ? inv[(int64_t)i * m + c] : 0.0;
+        }
+    };
+    fetch(top);
+    __syncthreads();   // the forward stores of this thread's column precede the reads below
+    fetch_y(top);
+    for (int i0 = top; i0 >= 0; i0 -= CH) {   // w[d] = x[i + 1 + d]
+        stage();
+        double yc_[CH];
 #pragma unroll
-        for (int d = 0; d < BWT; ++d) v -= (d < bw ? u[d] : 0.0) * w[d];
-        v = v / row[0];
+        for (int u = 0; u < CH; ++u) yc_[u] = yv[u];
+        if (i0 >= CH) {
+            fetch(i0 - CH);
+            fetch_y(i0 - CH);
+        }
+        const int n = m - i0 < CH ? m - i0 : CH;
 #pragma unroll
-        for (int d = BWT - 1; d > 0; --d) w[d] = w[d - 1];
-        w[0] = v;
-        if (own) inv[(int64_t)i * m + c] = v;
+        for (int u = CH - 1; u >= 0; --u) {
+            if (u >= n) continue;
+            const int i = i0 + u;
+            const double* row = fb + u * WMAX + BWT;
+            const double v = (yc_[u] - tree_dot<BWT>(row + 1, 1, w)) / row[0];
+#pragma unroll
+            for (int d = BWT - 1; d > 0; --d) w[d] = w[d - 1];
+            w[0] = v;
+            if (own) inv[(int64_t)i * m + c] = v;
+        }
     }
 }
 
@@ -224,16 +331,29 @@ k_coarse_restrict(int NX, int NY, int B, int mJ, const uint8_t* __restrict__ mas
     if (threadIdx.x == 0) rc[agg] = a;
 }
 
-// yc = inv rc: one wave per row
+// yc = inv rc, rc[k] = sum over the S slices of rc[s m + k]: the workgroup
+// first sums the slices into LDS (all loads in flight: slices past S re-read
+// the last one and are masked), then one wave per row.
 __global__ void __launch_bounds__(KM_BLOCK)
-k_coarse_matvec(int m, const double* __restrict__ inv, const double* __restrict__ rc, double* __restrict__ yc,
+k_coarse_matvec(int m, int S, const double* __restrict__ inv, const double* __restrict__ rc, double* __restrict__ yc,
                 const int* __restrict__ skip) {
+    extern __shared__ double rs[];
     if (skip && *skip) return;
+    for (int k = threadIdx.x; k < m; k += KM_BLOCK) {
+        double v[kmcoarse::kMaxSlices];
+#pragma unroll
+        for (int q = 0; q < kmcoarse::kMaxSlices; ++q) v[q] = rc[(int64_t)(q < S ? q : S - 1) * m + k];
+        double a = v[0];
+#pragma unroll
+        for (int q = 1; q < kmcoarse::kMaxSlices; ++q) a += q < S ? v[q] : 0.0;
+        rs[k] = a;
+    }
+    __syncthreads();
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (row >= m) return;
     const double* a = inv + (int64_t)row * m;
     double s = 0.0;
-    for (int k = lane; k < m; k += 64) s += a[k] * rc[k];
+    for (int k = lane; k < m; k += 64) s += a[k] * rs[k];
     s = wave_sum(s);
     if (lane == 0) yc[row] = s;
 }
@@ -320,7 +440,7 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
                "coarse band of width %d is too wide (aggregate %d)", bw, B);
     const int64_t n = (int64_t)NX * NY;
     const size_t need = a256(5 * (size_t)m * 8) + a256((size_t)m * (2 * bw + 1) * 8) + a256((size_t)m * m * 8) +
-                        2 * a256((size_t)m * 8) + a256((size_t)n);
+                        a256((size_t)kMaxSlices * m * 8) + a256((size_t)m * 8) + a256((size_t)n);
     if (!C->arena || C->bytes < need) {
         KM_HIP(ctx, hipStreamSynchronize(s));
         km_drop_graphs(ctx);
@@ -336,7 +456,7 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
     C->cc = reinterpret_cast<double*>(q); q += a256(5 * (size_t)m * 8);
     C->band = reinterpret_cast<double*>(q); q += a256((size_t)m * (2 * bw + 1) * 8);
     C->inv = reinterpret_cast<double*>(q); q += a256((size_t)m * m * 8);
-    C->rc = reinterpret_cast<double*>(q); q += a256((size_t)m * 8);
+    C->rc = reinterpret_cast<double*>(q); q += a256((size_t)kMaxSlices * m * 8);
     C->yc = reinterpret_cast<double*>(q); q += a256((size_t)m * 8);
     C->mask = reinterpret_cast<uint8_t*>(q);
     C->NX = NX; C->NY = NY; C->B = B; C->mI = mI; C->mJ = mJ; C->m = m; C->bw = bw;
@@ -344,8 +464,14 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
     hipLaunchKernelGGL(k_coarse_coef, dim3(m), dim3(KM_BLOCK), 0, s, NX, NY, B, mI, mJ, coef, C->cc);
     hipLaunchKernelGGL(k_coarse_band, dim3(flat_blocks2((int64_t)m * (2 * bw + 1))), dim3(1024), 0, s, m, mJ, bw,
                        (const double*)C->cc, C->band);
-    hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(1024), (size_t)(bw + 1) * (2 * bw + 1) * sizeof(double), s, m,
-                       bw, C->band);
+    // one wave (the ring's register prefetch assumes 64 threads): a step
+    // updates at most bw x bw entries
+    const size_t band_bytes = (size_t)m * (2 * bw + 1) * sizeof(double);
+    if (band_bytes <= 64 * 1024)
+        hipLaunchKernelGGL(k_coarse_factor_lds, dim3(1), dim3(64), band_bytes, s, m, bw, C->band);
+    else
+        hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(64), (size_t)(bw + 1) * (2 * bw + 1) * sizeof(double), s,
+                           m, bw, C->band);
     const dim3 gi((m + KM_BLOCK - 1) / KM_BLOCK);
     const double* band = C->band;
     if (bw <= 8)
@@ -360,11 +486,12 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
     return KM_OK;
 }
 
-int coarse_solve(km_ctx* ctx, const int* skip, hipStream_t s) {
+int coarse_solve(km_ctx* ctx, int S, const int* skip, hipStream_t s) {
     km_coarse* C = ctx->coarse;
     KM_REQUIRE(ctx, C && C->arena && C->m > 0, "coarse level not set up");
-    hipLaunchKernelGGL(k_coarse_matvec, dim3((C->m + 3) / 4), dim3(KM_BLOCK), 0, s, C->m, (const double*)C->inv,
-                       (const double*)C->rc, C->yc, skip);
+    KM_REQUIRE(ctx, S >= 1 && S <= kMaxSlices, "restriction slices %d", S);
+    hipLaunchKernelGGL(k_coarse_matvec, dim3((C->m + 3) / 4), dim3(KM_BLOCK), (size_t)C->m * sizeof(double), s, C->m,
+                       S, (const double*)C->inv, (const double*)C->rc, C->yc, skip);
     KM_CHECK_LAUNCH(ctx);
     return KM_OK;
 }
@@ -376,7 +503,7 @@ int restrict_solve(km_ctx* ctx, const T* r, const int* skip, hipStream_t s) {
     hipLaunchKernelGGL(k_coarse_restrict<T>, dim3(C->m), dim3(KM_BLOCK), 0, s, C->NX, C->NY, C->B, C->mJ,
                        (const uint8_t*)C->mask, r, C->rc, skip);
     KM_CHECK_LAUNCH(ctx);
-    return coarse_solve(ctx, skip, s);
+    return coarse_solve(ctx, 1, skip, s);
 }
 
 template <typename T>

@@ -373,3 +373,42 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ P, int G
         out[q] = block_sum_all(a, sh);
     }
 }
+
+// sum_partials with every plane's loads issued before any is reduced and the
+// NV workgroup sums sharing one pair of barriers: the same totals, bit for
+// bit, for kernels that reduce many planes before short work (it holds 8 NV
+// values per thread in registers, which the stencil kernels cannot spare).
+template <int NV>
+__device__ __forceinline__ void sum_partials_wide(const double* __restrict__ P, int G, double (&out)[NV]) {
+    __shared__ double shp[4 * NV];
+    double acc[NV][8];
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[q][u] = 0.0;
+    for (int k = threadIdx.x; k < G; k += 8 * KM_BLOCK) {
+        double v[NV][8];
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                v[q][u] = (k + u * KM_BLOCK < G) ? P[(size_t)q * KM_MAX_RED + k + u * KM_BLOCK] : 0.0;
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[q][u] += v[q][u];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double a = ((acc[q][0] + acc[q][1]) + (acc[q][2] + acc[q][3])) +
+                   ((acc[q][4] + acc[q][5]) + (acc[q][6] + acc[q][7]));
+        a = wave_sum(a);
+        if (q == 0) __syncthreads();   // earlier readers of shp are done
+        if (lane == 0) shp[q * 4 + w] = a;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = (shp[q * 4] + shp[q * 4 + 1]) + (shp[q * 4 + 2] + shp[q * 4 + 3]);
+    __syncthreads();
+}

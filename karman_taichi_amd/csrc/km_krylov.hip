@@ -21,6 +21,7 @@
 // T = double (plain FP64 path) or float (inner solves of the mixed-precision
 // path); dot products and scalars are always FP64.
 #include "km_common.h"
+#include "km_coarse.h"
 
 extern __shared__ __align__(16) unsigned char km_smem[];
 
@@ -1048,6 +1049,8 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
     KM_REQUIRE(ctx, coef && b && avg_us && reps > 0 && reps <= 4096, "km_time_kernel: invalid arguments");
     const bool b2b = (which & KM_KERNEL_BACK_TO_BACK) != 0;
     which &= ~KM_KERNEL_BACK_TO_BACK;
+    if (which >= KM_KERNEL_TL_P && which <= KM_KERNEL_TL_T)
+        return kmcoarse::time_kernel(ctx, which, NX, NY, coef, b, reps, b2b, avg_us);
     KM_REQUIRE(ctx, which == KM_KERNEL_BICG_A || which == KM_KERNEL_BICG_B || which == KM_KERNEL_STENCIL,
                "unknown kernel id %d", which);
     const int64_t n = (int64_t)NX * NY;

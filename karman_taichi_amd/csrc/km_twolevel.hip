@@ -4,14 +4,14 @@
 // p^ / s^ hooks of bicgsolver.py:48-58.
 //
 // Six launches per inner iteration i (ring slot ph = i mod KM_RING):
-//   K_P  one workgroup per aggregate.  omega = <t,s> / <t,t>,
+//   K_P  S workgroups per aggregate (row slices).  omega = <t,s> / <t,t>,
 //        rho = <s,r~> - omega <t,r~>, beta = (rho / rho') (alpha / omega);
 //        x += alpha p^ + omega s^, r = s - omega t (iteration i-1),
 //        p = r + beta (p - omega v); rc = P^T p; partial <r,r>
 //   K_M  yc = A_c^-1 rc
 //   K_V  2-D tiles with a 4-cell halo: p^ = M(p) (z1 from yc, three sweeps
 //        on shrinking halos in LDS), v = A p^; partial <r~,v>
-//   K_S  one workgroup per aggregate.  Stopping test on <r,r>;
+//   K_S  S workgroups per aggregate.  Stopping test on <r,r>;
 //        alpha = rho / <r~,v>; s = r - alpha v; rc = P^T s
 //   K_M  yc = A_c^-1 rc
 //   K_T  tiles: s^ = M(s), t = A s^; partials <t,s>, <t,t>, <s,r~>, <t,r~>
@@ -21,12 +21,14 @@
 #include "km_coarse.h"
 
 #include <cmath>
+#include <vector>
 
 namespace {
 
-constexpr int TL_PR = 0;                 // [1] <r, r> (aggregate workgroups)
-constexpr int TL_PV = KM_MAX_RED;        // [1] <r~, v> (tile workgroups)
-constexpr int TL_PT = 2 * KM_MAX_RED;    // [4] <t, s>, <t, t>, <s, r~>, <t, r~>
+constexpr int kMaxAggWg = 8192;            // aggregate-slice workgroups
+constexpr int TL_PR = 0;                   // [kMaxAggWg] <r, r> (aggregate-slice workgroups)
+constexpr int TL_PV = kMaxAggWg;           // [1] <r~, v> (tile workgroups)
+constexpr int TL_PT = kMaxAggWg + KM_MAX_RED;   // [4] <t, s>, <t, t>, <s, r~>, <t, r~>
 constexpr int kStallWindow = 300;        // as the unfused flexible solve
 constexpr int kChunk = 8;
 constexpr int kTI = 32, kTJ = 32;        // tile of the smoothing launches
@@ -40,37 +42,41 @@ __device__ __forceinline__ void finish_state(KState* st, int status, int iter, d
     st->done = 1;
 }
 
-// Geometry of the aggregate launches: aggregate agg covers the interior block
-// [i0, i0 + bi) x [j0, j0 + bj); edge aggregates also own the adjacent halo
-// cells, so every cell of the NX x NY system belongs to exactly one aggregate.
+// Geometry of the aggregate launches: aggregate (I, J) covers the interior
+// block [1 + I B, 1 + (I + 1) B) x [1 + J B, 1 + (J + 1) B), clipped; edge
+// aggregates also own the adjacent halo rows and columns, so every cell of the
+// NX x NY system belongs to exactly one aggregate.  Each aggregate's rows are
+// cut into S slices, one workgroup each (G = m S workgroups).
 struct AggGeom {
-    int NX, NY, B, mI, mJ, m, G;
+    int NX, NY, B, mI, mJ, m, S, G;
 };
 
-// f(k, free) over the cells of aggregate agg: the interior block first, in the
-// thread-strided order of the unfused restriction, then the halo cells.
-template <typename F>
-__device__ __forceinline__ void agg_cells(const AggGeom& g, int agg, const uint8_t* __restrict__ mask, F f) {
+// Cells of slice sl of aggregate agg: rows [ra, ra + cnt / ej), columns
+// [je0, je0 + ej); cell t of the slice is k(t).
+struct Slice {
+    int ra, je0, ej, cnt, NY;
+    __device__ __forceinline__ int64_t k(int t) const { return (int64_t)(ra + t / ej) * NY + (je0 + t % ej); }
+};
+
+__device__ __forceinline__ Slice slice_of(const AggGeom& g, int agg, int sl) {
     const int I = agg / g.mJ, J = agg - I * g.mJ;
     const int nx = g.NX - 2, ny = g.NY - 2;
     const int i0 = 1 + I * g.B, j0 = 1 + J * g.B;
-    const int bi = (i0 + g.B - 1 < nx ? i0 + g.B - 1 : nx) - i0 + 1;
-    const int bj = (j0 + g.B - 1 < ny ? j0 + g.B - 1 : ny) - j0 + 1;
-    for (int t = threadIdx.x; t < bi * bj; t += KM_BLOCK) {
-        const int64_t k = (int64_t)(i0 + t / bj) * g.NY + (j0 + t % bj);
-        f(k, mask[k] != 0);
-    }
-    if (I == 0 || I == g.mI - 1 || J == 0 || J == g.mJ - 1) {
-        const int ie0 = I == 0 ? 0 : i0, ie1 = I == g.mI - 1 ? g.NX - 1 : i0 + bi - 1;
-        const int je0 = J == 0 ? 0 : j0, je1 = J == g.mJ - 1 ? g.NY - 1 : j0 + bj - 1;
-        const int ei = ie1 - ie0 + 1, ej = je1 - je0 + 1;
-        for (int t = threadIdx.x; t < ei * ej; t += KM_BLOCK) {
-            const int i = ie0 + t / ej, j = je0 + t % ej;
-            if (i >= i0 && i < i0 + bi && j >= j0 && j < j0 + bj) continue;
-            f((int64_t)i * g.NY + j, false);
-        }
-    }
+    const int ie0 = I == 0 ? 0 : i0, ie1 = I == g.mI - 1 ? g.NX - 1 : (i0 + g.B - 1 < nx ? i0 + g.B - 1 : nx);
+    const int je0 = J == 0 ? 0 : j0, je1 = J == g.mJ - 1 ? g.NY - 1 : (j0 + g.B - 1 < ny ? j0 + g.B - 1 : ny);
+    const int er = ie1 - ie0 + 1;
+    Slice q;
+    q.ej = je1 - je0 + 1;
+    q.je0 = je0;
+    q.ra = ie0 + (er * sl) / g.S;
+    q.cnt = (ie0 + (er * (sl + 1)) / g.S - q.ra) * q.ej;
+    q.NY = g.NY;
+    return q;
 }
+
+// Cells per thread whose loads are issued before the scalars are reduced
+// (slices hold about kStage workgroups' worth; larger ones loop over the rest).
+constexpr int kStage = 4;
 
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
@@ -110,10 +116,31 @@ k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restric
     if (st->done) return;
     const KScal sp = st->ring[(ph + KM_RING - 1) % KM_RING];
     const int i = sp.iter + 1;
+    const int agg = blockIdx.x / g.S, sl = blockIdx.x - agg * g.S;
+    const Slice sq = slice_of(g, agg, sl);
+    int64_t kk[kStage];
+    bool ok[kStage], fr[kStage];
+    T xv[kStage], hv[kStage], s2v[kStage], sv[kStage], tv[kStage], pv[kStage], vv[kStage];
+#pragma unroll
+    for (int c = 0; c < kStage; ++c) {
+        const int tt = threadIdx.x + c * KM_BLOCK;
+        ok[c] = tt < sq.cnt;
+        kk[c] = ok[c] ? sq.k(tt) : 0;
+        const int64_t k = kk[c];
+        fr[c] = ok[c] && mask[k] != 0;
+        const bool ld = ok[c] && i > 0;
+        sv[c] = ok[c] ? (i == 0 ? r[k] : s[k]) : T(0);
+        xv[c] = ld ? x[k] : T(0);
+        hv[c] = ld ? ph_[k] : T(0);
+        s2v[c] = ld ? sh_[k] : T(0);
+        tv[c] = ld ? t[k] : T(0);
+        pv[c] = ld ? p[k] : T(0);
+        vv[c] = ld ? v[k] : T(0);
+    }
     double omega = 0.0, beta = 0.0;
     if (i > 0) {
         double pt[4];
-        sum_partials<4>(P + TL_PT, Gt, pt, sh);
+        sum_partials_wide<4>(P + TL_PT, Gt, pt);
         omega = pt[1] != 0.0 ? pt[0] / pt[1] : 0.0;                  // bicgsolver.py:118
         const double rho = pt[2] - omega * pt[3];                    // <r_i, r~> (bicgsolver.py:95)
         beta = omega != 0.0 ? (rho / sp.rho) * (sp.alpha / omega) : 0.0;   // bicgsolver.py:102
@@ -129,30 +156,38 @@ k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restric
     }
     const double alpha = sp.alpha;
     const T fo = (T)omega, fb = (T)beta;
-    double rr = 0.0;
-    for (int agg = blockIdx.x; agg < g.m; agg += g.G) {
-        double a = 0.0;
-        agg_cells(g, agg, mask, [&](int64_t k, bool free) {
-            T rn;
-            T pk;
-            if (i == 0) {
-                rn = r[k];
-                pk = rn;
-            } else {
-                x[k] = x[k] + (T)(alpha * (double)ph_[k] + omega * (double)sh_[k]);
-                rn = s[k] - fo * t[k];
-                r[k] = rn;
-                pk = rn + fb * (p[k] - fo * v[k]);
-            }
-            p[k] = pk;
-            rr += (double)rn * (double)rn;
-            if (free) a += (double)pk;
-        });
-        a = block_sum(a, sh);
-        if (threadIdx.x == 0) rc[agg] = a;
+    double rr = 0.0, a = 0.0;
+    auto upd = [&](int64_t k, bool free, T xk, T hk, T sk2, T sk, T tk, T pk0, T vk) {
+        T rn, pk;
+        if (i == 0) {
+            rn = sk;                       // r_0 (staged in the s slot)
+            pk = rn;
+        } else {
+            x[k] = xk + (T)(alpha * (double)hk + omega * (double)sk2);
+            rn = sk - fo * tk;
+            r[k] = rn;
+            pk = rn + fb * (pk0 - fo * vk);
+        }
+        p[k] = pk;
+        rr += (double)rn * (double)rn;
+        if (free) a += (double)pk;
+    };
+#pragma unroll
+    for (int c = 0; c < kStage; ++c)
+        if (ok[c]) upd(kk[c], fr[c], xv[c], hv[c], s2v[c], sv[c], tv[c], pv[c], vv[c]);
+    for (int tt = threadIdx.x + kStage * KM_BLOCK; tt < sq.cnt; tt += KM_BLOCK) {
+        const int64_t k = sq.k(tt);
+        if (i == 0)
+            upd(k, mask[k] != 0, T(0), T(0), T(0), r[k], T(0), T(0), T(0));
+        else
+            upd(k, mask[k] != 0, x[k], ph_[k], sh_[k], s[k], t[k], p[k], v[k]);
     }
+    a = block_sum(a, sh);
     rr = block_sum(rr, sh);
-    if (threadIdx.x == 0) P[TL_PR + blockIdx.x] = rr;
+    if (threadIdx.x == 0) {
+        rc[(int64_t)sl * g.m + agg] = a;
+        P[TL_PR + blockIdx.x] = rr;
+    }
 }
 
 // K_S: stopping test, alpha, s = r - alpha v, P^T s
@@ -163,9 +198,23 @@ k_tl_s(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, const T* __r
        double* __restrict__ P) {
     __shared__ double sh[8];
     if (st->done) return;
+    const int agg = blockIdx.x / g.S, sl = blockIdx.x - agg * g.S;
+    const Slice sq = slice_of(g, agg, sl);
+    int64_t kk[kStage];
+    bool ok[kStage], fr[kStage];
+    T rv_[kStage], vv[kStage];
+#pragma unroll
+    for (int c = 0; c < kStage; ++c) {
+        const int tt = threadIdx.x + c * KM_BLOCK;
+        ok[c] = tt < sq.cnt;
+        kk[c] = ok[c] ? sq.k(tt) : 0;
+        fr[c] = ok[c] && mask[kk[c]] != 0;
+        rv_[c] = ok[c] ? r[kk[c]] : T(0);
+        vv[c] = ok[c] ? v[kk[c]] : T(0);
+    }
     double tv[1], tr[1];
-    sum_partials<1>(P + TL_PV, Gt, tv, sh);
-    sum_partials<1>(P + TL_PR, g.G, tr, sh);
+    sum_partials_wide<1>(P + TL_PV, Gt, tv);
+    sum_partials_wide<1>(P + TL_PR, g.G, tr);
     const double rv = tv[0], rr = tr[0];
     const KScal sp = st->ring[ph];
     const KScal spp = st->ring[(ph + KM_RING - 1) % KM_RING];
@@ -200,16 +249,21 @@ k_tl_s(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, const T* __r
         st->iter = i;
     }
     const T fa = (T)alpha;
-    for (int agg = blockIdx.x; agg < g.m; agg += g.G) {
-        double a = 0.0;
-        agg_cells(g, agg, mask, [&](int64_t k, bool free) {
-            const T sk = r[k] - fa * v[k];
-            s[k] = sk;
-            if (free) a += (double)sk;
-        });
-        a = block_sum(a, sh);
-        if (threadIdx.x == 0) rc[agg] = a;
+    double a = 0.0;
+    auto upd = [&](int64_t k, bool free, T rk, T vk) {
+        const T sk = rk - fa * vk;
+        s[k] = sk;
+        if (free) a += (double)sk;
+    };
+#pragma unroll
+    for (int c = 0; c < kStage; ++c)
+        if (ok[c]) upd(kk[c], fr[c], rv_[c], vv[c]);
+    for (int tt = threadIdx.x + kStage * KM_BLOCK; tt < sq.cnt; tt += KM_BLOCK) {
+        const int64_t k = sq.k(tt);
+        upd(k, mask[k] != 0, r[k], v[k]);
     }
+    a = block_sum(a, sh);
+    if (threadIdx.x == 0) rc[(int64_t)sl * g.m + agg] = a;
 }
 
 // K_V / K_T / the standalone smoother: z = M(src) on the tiles, then (MODE 1)
@@ -217,7 +271,7 @@ k_tl_s(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, const T* __r
 // The coefficients and src of the tile's 3-cell halo frame stay in registers
 // (Q cells per thread); z lives in LDS on the 4-cell halo frame, ping-ponged
 // between two buffers over the sweeps.  The arithmetic is that of one
-// damped Jacobi sweep per cell: z + omega (src - A z) / aP, neighbours
+// damped Jacobi sweep per cell: z + (omega / aP) (src - A z), neighbours
 // outside the system read as zero, rows with aP = 0 left unchanged.
 template <typename T, int TI, int TJ, int MODE>
 __global__ void __launch_bounds__(KM_BLOCK)
@@ -240,7 +294,7 @@ k_tl_smooth(Tile tl, int B, int mJ, const T* __restrict__ c, const T* __restrict
     const int t0 = L * tl.tpb, t1 = t0 + tl.tpb < tl.T ? t0 + tl.tpb : tl.T;
     for (int tile = t0; tile < t1; ++tile) {
         const int ti0 = (tile / tl.tiles_j) * TI, tj0 = (tile % tl.tiles_j) * TJ;
-        T cr[Q][5], sr[Q];
+        T cr[Q][5], sr[Q], rq[Q], wd[Q];
         int64_t kq[Q];
         bool in[Q];
 #pragma unroll
@@ -253,15 +307,31 @@ k_tl_smooth(Tile tl, int B, int mJ, const T* __restrict__ c, const T* __restrict
 #pragma unroll
             for (int e = 0; e < 5; ++e) cr[q][e] = in[q] ? c[e * n + kq[q]] : T(0);
             sr[q] = in[q] ? src[kq[q]] : T(0);
+            wd[q] = cr[q][0] != T(0) ? om * (T(1) / cr[q][0]) : T(0);   // omega D^-1
+            const bool centre = in[q] && a >= 3 && a <= TI + 2 && b >= 3 && b <= TJ + 2;
+            rq[q] = MODE > 0 && centre ? rt[kq[q]] : T(0);
         }
-        // z1 = P yc on the 4-cell halo frame (zero off the free cells and outside the system)
-        for (int idx = threadIdx.x; idx < H4 * W4; idx += KM_BLOCK) {
+        // z1 = P yc on the 4-cell halo frame (zero off the free cells and outside
+        // the system); every mask and coarse value load is issued before the
+        // first one is used
+        constexpr int Q4 = (H4 * W4 + KM_BLOCK - 1) / KM_BLOCK;
+        const int nx = NX - 2, ny = NY - 2;
+        double yq[Q4];
+        uint8_t mq[Q4];
+#pragma unroll
+        for (int q = 0; q < Q4; ++q) {
+            const int idx = threadIdx.x + q * KM_BLOCK;
             const int a = idx / W4, b = idx - (idx / W4) * W4;
             const int i = ti0 - 4 + a, j = tj0 - 4 + b;
-            T v = T(0);
-            if (i >= 0 && i < NX && j >= 0 && j < NY && mask[(int64_t)i * NY + j])
-                v = (T)yc[((i - 1) / B) * mJ + (j - 1) / B];
-            za[idx] = v;
+            const bool ing = idx < H4 * W4 && i >= 0 && i < NX && j >= 0 && j < NY;
+            const int ic = i < 1 ? 0 : (i > nx ? nx - 1 : i - 1), jc = j < 1 ? 0 : (j > ny ? ny - 1 : j - 1);
+            yq[q] = yc[(ic / B) * mJ + jc / B];
+            mq[q] = ing ? mask[(int64_t)i * NY + j] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < Q4; ++q) {
+            const int idx = threadIdx.x + q * KM_BLOCK;
+            if (idx < H4 * W4) za[idx] = mq[q] ? (T)yq[q] : T(0);
         }
         __syncthreads();
         // sweeps on the halo-3, -2 and -1 frames: za -> zb -> za -> zb
@@ -283,9 +353,7 @@ k_tl_smooth(Tile tl, int B, int mJ, const T* __restrict__ c, const T* __restrict
                     az = az + cr[q][2] * zi[p4 + W4];
                     az = az + cr[q][3] * zi[p4 + 1];
                     az = az + cr[q][4] * zi[p4 - 1];
-                    const T rk = sr[q] - az;
-                    const T d = cr[q][0];
-                    out = zc + (d != T(0) ? om * rk / d : T(0));
+                    out = zc + wd[q] * (sr[q] - az);
                 }
                 zo[p4] = out;
             }
@@ -308,7 +376,7 @@ k_tl_smooth(Tile tl, int B, int mJ, const T* __restrict__ c, const T* __restrict
                 yv = yv + cr[q][3] * zb[p4 + 1];
                 yv = yv + cr[q][4] * zb[p4 - 1];
                 y[k] = yv;
-                const double rtk = (double)rt[k];
+                const double rtk = (double)rq[q];
                 if (MODE == 1) {
                     a0 += rtk * (double)yv;
                 } else {
@@ -339,10 +407,18 @@ k_tl_smooth(Tile tl, int B, int mJ, const T* __restrict__ c, const T* __restrict
     }
 }
 
+// About kStage x 256 cells per slice workgroup (fewer workgroups also means
+// fewer redundant partial-sum reads), at most kMaxSlices slices and
+// kMaxAggWg workgroups
 AggGeom agg_geom(const km_coarse* C) {
     AggGeom g;
     g.NX = C->NX; g.NY = C->NY; g.B = C->B; g.mI = C->mI; g.mJ = C->mJ; g.m = C->m;
-    g.G = C->m < KM_MAX_RED ? C->m : KM_MAX_RED;
+    int S = (int)(((int64_t)C->B * C->B) / (kStage * KM_BLOCK));
+    if (S > kmcoarse::kMaxSlices) S = kmcoarse::kMaxSlices;
+    if (S < 1) S = 1;
+    while (S > 1 && (int64_t)C->m * S > kMaxAggWg) --S;
+    g.S = S;
+    g.G = C->m * S;
     return g;
 }
 
@@ -367,6 +443,7 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
                 bool use_graph, KState* out) {
     km_coarse* C = ctx->coarse;
     KM_REQUIRE(ctx, C && C->arena && C->NX == NX && C->NY == NY, "coarse level not set up for this system");
+    KM_REQUIRE(ctx, C->m <= kMaxAggWg, "coarse level of %d unknowns", C->m);
     const int64_t n = (int64_t)NX * NY;
     T* r = ws;
     T* rt = ws + n;
@@ -390,12 +467,12 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
     auto enqueue = [=](hipStream_t q, int phs) {
         hipLaunchKernelGGL(k_tl_p<T>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, tl.G, phs, mask, xd, r, p, (const T*)v,
                            (const T*)s, (const T*)t, (const T*)ph, (const T*)sh, rc, st, P);
-        (void)coarse_solve(ctx, skip, q);
+        (void)coarse_solve(ctx, g.S, skip, q);
         hipLaunchKernelGGL((k_tl_smooth<T, kTI, kTJ, 1>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, g.B, g.mJ, coefT,
                            (const T*)p, mask, yc, (const T*)rt, ph, v, (const KState*)st, P);
         hipLaunchKernelGGL(k_tl_s<T>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, tl.G, phs, mask, (const T*)r, (const T*)v, s,
                            rc, st, P);
-        (void)coarse_solve(ctx, skip, q);
+        (void)coarse_solve(ctx, g.S, skip, q);
         hipLaunchKernelGGL((k_tl_smooth<T, kTI, kTJ, 2>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, g.B, g.mJ, coefT,
                            (const T*)s, mask, yc, (const T*)rt, sh, t, (const KState*)st, P);
     };
@@ -403,6 +480,103 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
     key.v = {8, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
              (uintptr_t)ws, (uintptr_t)st, (uintptr_t)C, (uintptr_t)C->arena, (uintptr_t)C->m, (uintptr_t)C->B};
     return run_chunks(ctx, st, key, use_graph, kChunk, max_iter, enqueue, out);
+}
+
+// Average device time of one launch of a fused two-level kernel (which: 4 K_P,
+// 5 K_V, 6 K_S, 7 K_T) inside FP64 iterations on (coef, b) with the stopping
+// test disabled; events around every launch of that kernel, or (b2b) around
+// reps identical launches after eight warm iterations.
+int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps, bool b2b,
+                double* avg_us) {
+    KM_REQUIRE(ctx, which >= 4 && which <= 7, "unknown kernel id %d", which);
+    if (int rc = setup(ctx, NX, NY, coef, ctx->stream)) return rc;
+    const int64_t n = (int64_t)NX * NY;
+    if (int rc = km_ensure_workspace(ctx, (size_t)9 * n * sizeof(double))) return rc;
+    double* ws = static_cast<double*>(km_ws_vectors(ctx));
+    km_coarse* C = ctx->coarse;
+    double *r = ws, *rt = ws + n, *p = ws + 2 * n, *v = ws + 3 * n, *sv = ws + 4 * n, *t = ws + 5 * n;
+    double *ph = ws + 6 * n, *sh = ws + 7 * n, *x = ws + 8 * n;
+    KState* st = &ctx->st[4];
+    double* P = ctx->partials;
+    const Flat fl = make_flat(n);
+    const Tile tl = make_tile(NX, NY, kTI, kTJ);
+    const AggGeom g = agg_geom(C);
+    const uint8_t* mask = C->mask;
+    std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
+    for (auto& e : ev) KM_HIP(ctx, hipEventCreate(&e));
+    auto cleanup = [&]() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    hipLaunchKernelGGL(k_tl_init<double>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, rt, 0.0, 1 << 30,
+                       st);
+    hipStream_t q = ctx->stream;
+    // the kernel alone; one() adds the coarse solve that follows K_P and K_S
+    auto launch = [&](int k, int phs) {
+        switch (k) {
+        case 4:
+            hipLaunchKernelGGL(k_tl_p<double>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, tl.G, phs, mask, x, r, p,
+                               (const double*)v, (const double*)sv, (const double*)t, (const double*)ph,
+                               (const double*)sh, C->rc, st, P);
+            break;
+        case 5:
+            hipLaunchKernelGGL((k_tl_smooth<double, kTI, kTJ, 1>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, g.B, g.mJ,
+                               coef, (const double*)p, mask, (const double*)C->yc, (const double*)rt, ph, v,
+                               (const KState*)st, P);
+            break;
+        case 6:
+            hipLaunchKernelGGL(k_tl_s<double>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, tl.G, phs, mask, (const double*)r,
+                               (const double*)v, sv, C->rc, st, P);
+            break;
+        default:
+            hipLaunchKernelGGL((k_tl_smooth<double, kTI, kTJ, 2>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, g.B, g.mJ,
+                               coef, (const double*)sv, mask, (const double*)C->yc, (const double*)rt, sh, t,
+                               (const KState*)st, P);
+        }
+    };
+    auto one = [&](int k, int phs, hipEvent_t e0, hipEvent_t e1) {
+        if (e0) (void)hipEventRecord(e0, q);
+        launch(k, phs);
+        if (e1) (void)hipEventRecord(e1, q);
+        if (k == 4 || k == 6) (void)coarse_solve(ctx, g.S, nullptr, q);
+    };
+    const int warm = 8;
+    for (int it = 0; it < warm; ++it)
+        for (int k = 4; k <= 7; ++k) one(k, it % KM_RING, nullptr, nullptr);
+    if (b2b) {
+        // K_P re-runs the scalars and updates of the next phase, the others
+        // read what the last warm iteration left
+        const int phs = which == 4 ? warm % KM_RING : (warm - 1) % KM_RING;
+        KM_HIP(ctx, hipEventRecord(ev[0], q));
+        for (int k = 0; k < reps; ++k) launch(which, phs);
+        KM_HIP(ctx, hipEventRecord(ev[1], q));
+    } else {
+        for (int it = 0; it < reps; ++it)
+            for (int k = 4; k <= 7; ++k)
+                one(k, (warm + it) % KM_RING, k == which ? ev[2 * it] : nullptr, k == which ? ev[2 * it + 1] : nullptr);
+    }
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(q);
+    if (le != hipSuccess || se != hipSuccess) {
+        cleanup();
+        return km_fail(ctx, KM_EHIP, "km_time_kernel: %s / %s", hipGetErrorString(le), hipGetErrorString(se));
+    }
+    double total_ms = 0.0;
+    for (int k = 0; k < (b2b ? 1 : reps); ++k) {
+        float ms = 0.f;
+        hipError_t e = hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]);
+        if (e != hipSuccess) {
+            cleanup();
+            return km_fail(ctx, KM_EHIP, "hipEventElapsedTime: %s", hipGetErrorString(e));
+        }
+        total_ms += ms;
+    }
+    cleanup();
+    KM_HIP(ctx, hipMemcpy(&ctx->h_st[3], st, sizeof(KState), hipMemcpyDeviceToHost));
+    KM_REQUIRE(ctx, !ctx->h_st[3].done, "km_time_kernel: the iteration stopped during timing (status %d)",
+               ctx->h_st[3].status);
+    *avg_us = 1e3 * total_ms / reps;
+    return KM_OK;
 }
 
 template int smooth<float>(km_ctx*, const float*, const float*, float*, hipStream_t);
