@@ -169,6 +169,11 @@ def main():
     # which agrees with rocprofv3's per-kernel averages; events around every
     # dispatch of a running iteration add ~3 us of event overhead per launch.
     dom = max(kern, key=kern.get)
+    tile = None
+    if twolevel and not learned:
+        ti, tj = ctypes.c_int(), ctypes.c_int()
+        ctx.call("km_twolevel_tile", ctypes.byref(ti), ctypes.byref(tj))
+        tile = [ti.value, tj.value]
     alg_bytes = BYTES_PER_CELL[dom] * cells_p
     achieved = alg_bytes / (kern[dom] * 1e-6) / 1e9
     traffic = None
@@ -215,7 +220,7 @@ def main():
                                f"{case.obstacle_cells}x{case.obstacle_cells} cylinder, "
                                f"{n_outer} outer steps per time step, inner eps 1e-8",
                    "mode": args.mode, "p_solver": p_solver, "p_twolevel": twolevel, "p_network": learned,
-                   "p_vectors": "f32" if fp32 else "f64"},
+                   "p_vectors": "f32" if fp32 else "f64", "smoothing_tile": tile},
         "p_solve_ms_median": round(statistics.median(p_ms), 3),
         "p_iters_median": statistics.median(p_it),
         "p_rel_resid_true_max": max(r["p_true"] for r in rec),

@@ -212,10 +212,14 @@ int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const dou
 int km_coarse_info(km_ctx* ctx, int* block, int* unknowns);
 /* z = M(r) for one FP64 system: the two-level Jacobi preconditioner of the
  * flexible solve with KM_FLAG_COARSE and no network, z = P A_c^-1 P^T r then
- * three sweeps z += 0.8 D^-1 (r - A z).  It fills the p^ / s^ hooks of the
+ * four sweeps z += 0.8 D^-1 (r - A z).  It fills the p^ / s^ hooks of the
  * reference's BiCGSTAB (bicgsolver.py:48-58, karman_taichi.py:399,419 pass
  * a Jacobi scaling there); sets the coarse level up first. */
 int km_twolevel_precond(km_ctx* ctx, int NX, int NY, const double* coef, const double* r, double* z);
+/* Tile shape (rows, columns) of the fused two-level smoothing kernels, chosen
+ * by timing on the first KM_FLAG_COARSE solve of the coarse level's system.
+ * No counterpart in the reference. */
+int km_twolevel_tile(km_ctx* ctx, int* ti, int* tj);
 
 /* ---- measurement hook (no counterpart in the reference) ----
  * Average device time in microseconds of one launch of a solver kernel on an

@@ -5,6 +5,9 @@
 
 #include "km_common.h"
 
+#include <map>
+#include <tuple>
+
 // Coarse level of one p' system: B x B aggregates of the interior, i-major.
 struct km_coarse {
     int NX = 0, NY = 0, B = 0, mI = 0, mJ = 0, m = 0, bw = 0;
@@ -17,12 +20,16 @@ struct km_coarse {
     double* rc = nullptr;     // kMaxSlices x m restricted residual (per-slice partial sums)
     double* yc = nullptr;     // m coarse solution
     uint8_t* mask = nullptr;  // NX x NY: 1 on the free interior cells (the support of P)
+    // tile shape of the fused smoothing launches per (NX, NY, sizeof T)
+    std::map<std::tuple<int, int, int>, int> tiles;
 };
 
 namespace kmcoarse {
 
-// Damped Jacobi sweeps of the two-level smoother and their damping.
-constexpr int kSweeps = 3;
+// Damped Jacobi sweeps of the two-level smoother and their damping (four
+// sweeps at 0.8 take the 1280 x 256 channel's p' solve from 99 flexible
+// iterations with three to 74 in the oracle; 0.7, 0.75, 0.85, 0.9 do worse).
+constexpr int kSweeps = 4;
 constexpr double kOmega = 0.8;
 // Row slices per aggregate in the fused Krylov updates: their partial
 // restrictions rc[s m + k] are summed by the coarse solve.
@@ -59,6 +66,10 @@ int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s);
 template <typename T>
 int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
                 bool use_graph, KState* out);
+
+// Tile shape (rows, columns) of the fused smoothing kernels chosen for the
+// coarse level's system in FP64 (32 x 32 before the first fused solve).
+int tile_info(km_ctx* ctx, int* ti, int* tj);
 
 // km_time_kernel for the fused kernels (which 4..7: K_P, K_V, K_S, K_T).
 int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps, bool b2b,

@@ -1,5 +1,5 @@
 // Fused flexible BiCGSTAB with the two-level Jacobi preconditioner
-//   M(r) = z1 + kSweeps damped Jacobi sweeps on r - A z,  z1 = P A_c^-1 P^T r
+//   M(r) = z1, then kSweeps damped Jacobi sweeps on r - A z,  z1 = P A_c^-1 P^T r
 // (the aggregation coarse level of km_coarse.hip), the preconditioner of the
 // p^ / s^ hooks of bicgsolver.py:48-58.
 //
@@ -9,13 +9,13 @@
 //        x += alpha p^ + omega s^, r = s - omega t (iteration i-1),
 //        p = r + beta (p - omega v); rc = P^T p; partial <r,r>
 //   K_M  yc = A_c^-1 rc
-//   K_V  2-D tiles with a 4-cell halo: p^ = M(p) (z1 from yc, three sweeps
-//        on shrinking halos in LDS), v = A p^; partial <r~,v>
+//   K_V  2-D tiles with a (kSweeps + 1)-cell halo: p^ = M(p) (z1 from yc,
+//        kSweeps sweeps on shrinking halos in LDS), v = A p^; partial <r~,v>
 //   K_S  S workgroups per aggregate.  Stopping test on <r,r>;
 //        alpha = rho / <r~,v>; s = r - alpha v; rc = P^T s
 //   K_M  yc = A_c^-1 rc
 //   K_T  tiles: s^ = M(s), t = A s^; partials <t,s>, <t,t>, <s,r~>, <t,r~>
-// The unfused sequence (restrict, matvec, prolongation, three sweeps and the
+// The unfused sequence (restrict, matvec, prolongation, the sweeps and the
 // Krylov updates as separate kernels) takes fifteen launches; every vector of
 // the fused form is written once per iteration.
 #include "km_coarse.h"
@@ -31,7 +31,11 @@ constexpr int TL_PV = kMaxAggWg;           // [1] <r~, v> (tile workgroups)
 constexpr int TL_PT = kMaxAggWg + KM_MAX_RED;   // [4] <t, s>, <t, t>, <s, r~>, <t, r~>
 constexpr int kStallWindow = 300;        // as the unfused flexible solve
 constexpr int kChunk = 8;
-constexpr int kTI = 32, kTJ = 32;        // tile of the smoothing launches
+// Tile shapes of the smoothing launches (rows x columns, columns contiguous);
+// the fastest for a system is chosen on its first solve.
+constexpr int kNumTiles = 5;
+constexpr int kTileI[kNumTiles] = {32, 16, 16, 8, 8};
+constexpr int kTileJ[kNumTiles] = {32, 64, 32, 64, 128};
 
 __device__ __forceinline__ bool fin(double v) { return v == v && v - v == 0.0; }
 
@@ -268,9 +272,9 @@ k_tl_s(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, const T* __r
 
 // K_V / K_T / the standalone smoother: z = M(src) on the tiles, then (MODE 1)
 // v = A z with <r~, v> or (MODE 2) t = A z with <t,s>, <t,t>, <s,r~>, <t,r~>.
-// The coefficients and src of the tile's 3-cell halo frame stay in registers
-// (Q cells per thread); z lives in LDS on the 4-cell halo frame, ping-ponged
-// between two buffers over the sweeps.  The arithmetic is that of one
+// The coefficients and src of the tile's kSweeps-cell halo frame stay in
+// registers (Q cells per thread); z lives in LDS on the (kSweeps + 1)-cell
+// halo frame, ping-ponged between two buffers over the sweeps.  The arithmetic is that of one
 // damped Jacobi sweep per cell: z + (omega / aP) (src - A z), neighbours
 // outside the system read as zero, rows with aP = 0 left unchanged.
 template <typename T, int TI, int TJ, int MODE>
@@ -278,12 +282,13 @@ __global__ void __launch_bounds__(KM_BLOCK)
 k_tl_smooth(Tile tl, int B, int mJ, const T* __restrict__ c, const T* __restrict__ src,
             const uint8_t* __restrict__ mask, const double* __restrict__ yc, const T* __restrict__ rt,
             T* __restrict__ z, T* __restrict__ y, const KState* __restrict__ st, double* __restrict__ P) {
-    constexpr int W4 = TJ + 8, H4 = TI + 8;
-    constexpr int W3 = TJ + 6, H3 = TI + 6;
-    constexpr int N3 = W3 * H3;
-    constexpr int Q = (N3 + KM_BLOCK - 1) / KM_BLOCK;
-    __shared__ T za[H4 * W4];
-    __shared__ T zb[H4 * W4];
+    constexpr int NS = kmcoarse::kSweeps;
+    constexpr int WZ = TJ + 2 * (NS + 1), HZ = TI + 2 * (NS + 1);   // z frame: halo NS + 1
+    constexpr int WR = TJ + 2 * NS, HR = TI + 2 * NS;               // register frame: halo NS
+    constexpr int NR = WR * HR;
+    constexpr int Q = (NR + KM_BLOCK - 1) / KM_BLOCK;
+    __shared__ T za[HZ * WZ];
+    __shared__ T zb[HZ * WZ];
     __shared__ double sh[8];
     if (st && st->done) return;
     const int L = xcd_remap(blockIdx.x, tl.G);
@@ -300,81 +305,82 @@ k_tl_smooth(Tile tl, int B, int mJ, const T* __restrict__ c, const T* __restrict
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int idx = threadIdx.x + q * KM_BLOCK;
-            const int a = idx / W3, b = idx - (idx / W3) * W3;
-            const int i = ti0 - 3 + a, j = tj0 - 3 + b;
-            in[q] = idx < N3 && i >= 0 && i < NX && j >= 0 && j < NY;
+            const int a = idx / WR, b = idx - (idx / WR) * WR;
+            const int i = ti0 - NS + a, j = tj0 - NS + b;
+            in[q] = idx < NR && i >= 0 && i < NX && j >= 0 && j < NY;
             kq[q] = in[q] ? (int64_t)i * NY + j : 0;
 #pragma unroll
             for (int e = 0; e < 5; ++e) cr[q][e] = in[q] ? c[e * n + kq[q]] : T(0);
             sr[q] = in[q] ? src[kq[q]] : T(0);
             wd[q] = cr[q][0] != T(0) ? om * (T(1) / cr[q][0]) : T(0);   // omega D^-1
-            const bool centre = in[q] && a >= 3 && a <= TI + 2 && b >= 3 && b <= TJ + 2;
+            const bool centre = in[q] && a >= NS && a < TI + NS && b >= NS && b < TJ + NS;
             rq[q] = MODE > 0 && centre ? rt[kq[q]] : T(0);
         }
-        // z1 = P yc on the 4-cell halo frame (zero off the free cells and outside
-        // the system); every mask and coarse value load is issued before the
-        // first one is used
-        constexpr int Q4 = (H4 * W4 + KM_BLOCK - 1) / KM_BLOCK;
+        // z1 = P yc on the z frame (zero off the free cells and outside the
+        // system); every mask and coarse value load is issued before the first
+        // one is used
+        constexpr int QZ = (HZ * WZ + KM_BLOCK - 1) / KM_BLOCK;
         const int nx = NX - 2, ny = NY - 2;
-        double yq[Q4];
-        uint8_t mq[Q4];
+        double yq[QZ];
+        uint8_t mq[QZ];
 #pragma unroll
-        for (int q = 0; q < Q4; ++q) {
+        for (int q = 0; q < QZ; ++q) {
             const int idx = threadIdx.x + q * KM_BLOCK;
-            const int a = idx / W4, b = idx - (idx / W4) * W4;
-            const int i = ti0 - 4 + a, j = tj0 - 4 + b;
-            const bool ing = idx < H4 * W4 && i >= 0 && i < NX && j >= 0 && j < NY;
+            const int a = idx / WZ, b = idx - (idx / WZ) * WZ;
+            const int i = ti0 - NS - 1 + a, j = tj0 - NS - 1 + b;
+            const bool ing = idx < HZ * WZ && i >= 0 && i < NX && j >= 0 && j < NY;
             const int ic = i < 1 ? 0 : (i > nx ? nx - 1 : i - 1), jc = j < 1 ? 0 : (j > ny ? ny - 1 : j - 1);
             yq[q] = yc[(ic / B) * mJ + jc / B];
             mq[q] = ing ? mask[(int64_t)i * NY + j] : 0;
         }
 #pragma unroll
-        for (int q = 0; q < Q4; ++q) {
+        for (int q = 0; q < QZ; ++q) {
             const int idx = threadIdx.x + q * KM_BLOCK;
-            if (idx < H4 * W4) za[idx] = mq[q] ? (T)yq[q] : T(0);
+            if (idx < HZ * WZ) za[idx] = mq[q] ? (T)yq[q] : T(0);
         }
         __syncthreads();
-        // sweeps on the halo-3, -2 and -1 frames: za -> zb -> za -> zb
+        // sweep h (h = NS .. 1) on the halo-h frame, za and zb alternating
 #pragma unroll
-        for (int h = 3; h >= 1; --h) {
-            const T* __restrict__ zi = (h == 2) ? zb : za;
-            T* __restrict__ zo = (h == 2) ? za : zb;
+        for (int h = NS; h >= 1; --h) {
+            const T* __restrict__ zi = ((NS - h) & 1) ? zb : za;
+            T* __restrict__ zo = ((NS - h) & 1) ? za : zb;
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const int idx = threadIdx.x + q * KM_BLOCK;
-                const int a = idx / W3, b = idx - (idx / W3) * W3;
-                if (idx >= N3 || a < 3 - h || a > TI + 2 + h || b < 3 - h || b > TJ + 2 + h) continue;
-                const int p4 = (a + 1) * W4 + (b + 1);
+                const int a = idx / WR, b = idx - (idx / WR) * WR;
+                if (idx >= NR || a < NS - h || a >= TI + NS + h || b < NS - h || b >= TJ + NS + h) continue;
+                const int pz = (a + 1) * WZ + (b + 1);
                 T out = T(0);
                 if (in[q]) {
-                    const T zc = zi[p4];
+                    const T zc = zi[pz];
                     T az = cr[q][0] * zc;
-                    az = az + cr[q][1] * zi[p4 - W4];
-                    az = az + cr[q][2] * zi[p4 + W4];
-                    az = az + cr[q][3] * zi[p4 + 1];
-                    az = az + cr[q][4] * zi[p4 - 1];
+                    az = az + cr[q][1] * zi[pz - WZ];
+                    az = az + cr[q][2] * zi[pz + WZ];
+                    az = az + cr[q][3] * zi[pz + 1];
+                    az = az + cr[q][4] * zi[pz - 1];
                     out = zc + wd[q] * (sr[q] - az);
                 }
-                zo[p4] = out;
+                zo[pz] = out;
             }
             __syncthreads();
         }
+        const T* __restrict__ zf = (NS & 1) ? zb : za;   // the last sweep's output
         // the tile itself: z out, then y = A z
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int idx = threadIdx.x + q * KM_BLOCK;
-            const int a = idx / W3, b = idx - (idx / W3) * W3;
-            if (idx >= N3 || a < 3 || a > TI + 2 || b < 3 || b > TJ + 2 || !in[q]) continue;
-            const int p4 = (a + 1) * W4 + (b + 1);
+            const int a = idx / WR, b = idx - (idx / WR) * WR;
+            if (idx >= NR || a < NS || a >= TI + NS || b < NS || b >= TJ + NS || !in[q]) continue;
+            const int pz = (a + 1) * WZ + (b + 1);
             const int64_t k = kq[q];
-            const T zc = zb[p4];
+            const T zc = zf[pz];
             z[k] = zc;
             if (MODE > 0) {
                 T yv = cr[q][0] * zc;
-                yv = yv + cr[q][1] * zb[p4 - W4];
-                yv = yv + cr[q][2] * zb[p4 + W4];
-                yv = yv + cr[q][3] * zb[p4 + 1];
-                yv = yv + cr[q][4] * zb[p4 - 1];
+                yv = yv + cr[q][1] * zf[pz - WZ];
+                yv = yv + cr[q][2] * zf[pz + WZ];
+                yv = yv + cr[q][3] * zf[pz + 1];
+                yv = yv + cr[q][4] * zf[pz - 1];
                 y[k] = yv;
                 const double rtk = (double)rq[q];
                 if (MODE == 1) {
@@ -407,6 +413,30 @@ k_tl_smooth(Tile tl, int B, int mJ, const T* __restrict__ c, const T* __restrict
     }
 }
 
+// One smoothing launch with tile shape `which` (an index into kTileI / kTileJ).
+template <typename T, int MODE>
+void launch_smooth(int which, hipStream_t q, int NX, int NY, int B, int mJ, const T* c, const T* src,
+                   const uint8_t* mask, const double* yc, const T* rt, T* z, T* y, const KState* st, double* P) {
+#define KM_TL_CASE(IDX)                                                                                           \
+    case IDX: {                                                                                                   \
+        const Tile tl = make_tile(NX, NY, kTileI[IDX], kTileJ[IDX]);                                              \
+        hipLaunchKernelGGL((k_tl_smooth<T, kTileI[IDX], kTileJ[IDX], MODE>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, \
+                           B, mJ, c, src, mask, yc, rt, z, y, st, P);                                             \
+        break;                                                                                                    \
+    }
+    switch (which) {
+        KM_TL_CASE(0)
+        KM_TL_CASE(1)
+        KM_TL_CASE(2)
+        KM_TL_CASE(3)
+        default:
+        KM_TL_CASE(4)
+    }
+#undef KM_TL_CASE
+}
+
+int smooth_workgroups(int which, int NX, int NY) { return make_tile(NX, NY, kTileI[which], kTileJ[which]).G; }
+
 // About kStage x 256 cells per slice workgroup (fewer workgroups also means
 // fewer redundant partial-sum reads), at most kMaxSlices slices and
 // kMaxAggWg workgroups
@@ -426,14 +456,71 @@ AggGeom agg_geom(const km_coarse* C) {
 
 namespace kmcoarse {
 
+// Tile shape of the smoothing launches for the coarse level's system in T:
+// on the first call for a (shape, precision), each candidate runs 3 + 16
+// back-to-back MODE 1 launches on scratch vectors (ws: its first 4n values of
+// T are overwritten) and the fastest is kept.
+template <typename T>
+int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
+    km_coarse* C = ctx->coarse;
+    const auto key = std::make_tuple(C->NX, C->NY, (int)sizeof(T));
+    auto it = C->tiles.find(key);
+    if (it != C->tiles.end()) {
+        *out = it->second;
+        return KM_OK;
+    }
+    const int64_t n = (int64_t)C->NX * C->NY;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    KM_HIP(ctx, hipEventCreate(&e0));
+    KM_HIP(ctx, hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 0.f;
+    for (int w = 0; w < kNumTiles; ++w) {
+        auto once = [&]() {
+            launch_smooth<T, 1>(w, ctx->stream, C->NX, C->NY, C->B, C->mJ, coefT, src, (const uint8_t*)C->mask,
+                                (const double*)C->yc, (const T*)ws, ws + 2 * n, ws + 3 * n, (const KState*)nullptr,
+                                ctx->partials + TL_PV);
+        };
+        for (int k = 0; k < 3; ++k) once();
+        (void)hipEventRecord(e0, ctx->stream);
+        for (int k = 0; k < 16; ++k) once();
+        (void)hipEventRecord(e1, ctx->stream);
+        hipError_t se = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (se == hipSuccess) se = hipEventElapsedTime(&ms, e0, e1);
+        if (se != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return km_fail(ctx, KM_EHIP, "smoothing tile timing: %s", hipGetErrorString(se));
+        }
+        if (w == 0 || ms < best_ms) {
+            best = w;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    C->tiles[key] = best;
+    *out = best;
+    return KM_OK;
+}
+
+int tile_info(km_ctx* ctx, int* ti, int* tj) {
+    const km_coarse* C = ctx->coarse;
+    KM_REQUIRE(ctx, C && C->arena, "no coarse level");
+    auto it = C->tiles.find(std::make_tuple(C->NX, C->NY, (int)sizeof(double)));
+    const int w = it == C->tiles.end() ? 0 : it->second;
+    *ti = kTileI[w];
+    *tj = kTileJ[w];
+    return KM_OK;
+}
+
 template <typename T>
 int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s) {
     const km_coarse* C = ctx->coarse;
     KM_REQUIRE(ctx, C && C->arena && C->m > 0, "coarse level not set up");
-    const Tile tl = make_tile(C->NX, C->NY, kTI, kTJ);
-    hipLaunchKernelGGL((k_tl_smooth<T, kTI, kTJ, 0>), dim3(tl.G), dim3(KM_BLOCK), 0, s, tl, C->B, C->mJ, coefT, r,
-                       (const uint8_t*)C->mask, (const double*)C->yc, (const T*)nullptr, z, (T*)nullptr,
-                       (const KState*)nullptr, (double*)nullptr);
+    launch_smooth<T, 0>(0, s, C->NX, C->NY, C->B, C->mJ, coefT, r, (const uint8_t*)C->mask, (const double*)C->yc,
+                        (const T*)nullptr, z, (T*)nullptr, (const KState*)nullptr, (double*)nullptr);
     KM_CHECK_LAUNCH(ctx);
     return KM_OK;
 }
@@ -456,29 +543,32 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
     KState* st = &ctx->st[4];
     double* P = ctx->partials;
     const Flat fl = make_flat(n);
-    const Tile tl = make_tile(NX, NY, kTI, kTJ);
     const AggGeom g = agg_geom(C);
     const uint8_t* mask = C->mask;
     double* rc = C->rc;
     const double* yc = C->yc;
+    int tile = 0;
+    if (int rc_ = tuned_tile<T>(ctx, coefT, rhs, ws, &tile)) return rc_;
+    const int Gt = smooth_workgroups(tile, NX, NY);
     hipLaunchKernelGGL(k_tl_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, rhs, xd, r, rt, eps, max_iter, st);
     KM_CHECK_LAUNCH(ctx);
     const int* skip = &st->done;
     auto enqueue = [=](hipStream_t q, int phs) {
-        hipLaunchKernelGGL(k_tl_p<T>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, tl.G, phs, mask, xd, r, p, (const T*)v,
+        hipLaunchKernelGGL(k_tl_p<T>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, Gt, phs, mask, xd, r, p, (const T*)v,
                            (const T*)s, (const T*)t, (const T*)ph, (const T*)sh, rc, st, P);
         (void)coarse_solve(ctx, g.S, skip, q);
-        hipLaunchKernelGGL((k_tl_smooth<T, kTI, kTJ, 1>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, g.B, g.mJ, coefT,
-                           (const T*)p, mask, yc, (const T*)rt, ph, v, (const KState*)st, P);
-        hipLaunchKernelGGL(k_tl_s<T>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, tl.G, phs, mask, (const T*)r, (const T*)v, s,
+        launch_smooth<T, 1>(tile, q, NX, NY, g.B, g.mJ, coefT, (const T*)p, mask, yc, (const T*)rt, ph, v,
+                            (const KState*)st, P);
+        hipLaunchKernelGGL(k_tl_s<T>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, Gt, phs, mask, (const T*)r, (const T*)v, s,
                            rc, st, P);
         (void)coarse_solve(ctx, g.S, skip, q);
-        hipLaunchKernelGGL((k_tl_smooth<T, kTI, kTJ, 2>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, g.B, g.mJ, coefT,
-                           (const T*)s, mask, yc, (const T*)rt, sh, t, (const KState*)st, P);
+        launch_smooth<T, 2>(tile, q, NX, NY, g.B, g.mJ, coefT, (const T*)s, mask, yc, (const T*)rt, sh, t,
+                            (const KState*)st, P);
     };
     GraphKey key;
     key.v = {8, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
-             (uintptr_t)ws, (uintptr_t)st, (uintptr_t)C, (uintptr_t)C->arena, (uintptr_t)C->m, (uintptr_t)C->B};
+             (uintptr_t)ws, (uintptr_t)st, (uintptr_t)C, (uintptr_t)C->arena, (uintptr_t)C->m, (uintptr_t)C->B,
+             (uintptr_t)tile};
     return run_chunks(ctx, st, key, use_graph, kChunk, max_iter, enqueue, out);
 }
 
@@ -499,9 +589,11 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
     KState* st = &ctx->st[4];
     double* P = ctx->partials;
     const Flat fl = make_flat(n);
-    const Tile tl = make_tile(NX, NY, kTI, kTJ);
     const AggGeom g = agg_geom(C);
     const uint8_t* mask = C->mask;
+    int tile = 0;
+    if (int rc = tuned_tile<double>(ctx, coef, b, ws, &tile)) return rc;
+    const int Gt = smooth_workgroups(tile, NX, NY);
     std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
     for (auto& e : ev) KM_HIP(ctx, hipEventCreate(&e));
     auto cleanup = [&]() {
@@ -515,23 +607,21 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
     auto launch = [&](int k, int phs) {
         switch (k) {
         case 4:
-            hipLaunchKernelGGL(k_tl_p<double>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, tl.G, phs, mask, x, r, p,
+            hipLaunchKernelGGL(k_tl_p<double>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, Gt, phs, mask, x, r, p,
                                (const double*)v, (const double*)sv, (const double*)t, (const double*)ph,
                                (const double*)sh, C->rc, st, P);
             break;
         case 5:
-            hipLaunchKernelGGL((k_tl_smooth<double, kTI, kTJ, 1>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, g.B, g.mJ,
-                               coef, (const double*)p, mask, (const double*)C->yc, (const double*)rt, ph, v,
-                               (const KState*)st, P);
+            launch_smooth<double, 1>(tile, q, NX, NY, g.B, g.mJ, coef, (const double*)p, mask, (const double*)C->yc,
+                                     (const double*)rt, ph, v, (const KState*)st, P);
             break;
         case 6:
-            hipLaunchKernelGGL(k_tl_s<double>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, tl.G, phs, mask, (const double*)r,
+            hipLaunchKernelGGL(k_tl_s<double>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, Gt, phs, mask, (const double*)r,
                                (const double*)v, sv, C->rc, st, P);
             break;
         default:
-            hipLaunchKernelGGL((k_tl_smooth<double, kTI, kTJ, 2>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, g.B, g.mJ,
-                               coef, (const double*)sv, mask, (const double*)C->yc, (const double*)rt, sh, t,
-                               (const KState*)st, P);
+            launch_smooth<double, 2>(tile, q, NX, NY, g.B, g.mJ, coef, (const double*)sv, mask, (const double*)C->yc,
+                                     (const double*)rt, sh, t, (const KState*)st, P);
         }
     };
     auto one = [&](int k, int phs, hipEvent_t e0, hipEvent_t e1) {
@@ -589,6 +679,11 @@ template int fused_inner<double>(km_ctx*, int, int, const double*, const double*
 }  // namespace kmcoarse
 
 extern "C" {
+
+int km_twolevel_tile(km_ctx* ctx, int* ti, int* tj) {
+    if (!ctx || !ti || !tj) return KM_EINVAL;
+    return kmcoarse::tile_info(ctx, ti, tj);
+}
 
 int km_twolevel_precond(km_ctx* ctx, int NX, int NY, const double* coef, const double* r, double* z) {
     if (!ctx) return KM_EINVAL;

@@ -341,7 +341,7 @@ def coarse_correct(coef, r, B):
     return z1, r - stencil(coef, z1)
 
 
-def twolevel_jacobi_precond(coef, B, sweeps=3, omega=0.8):
+def twolevel_jacobi_precond(coef, B, sweeps=4, omega=0.8):
     """M(r) = z1 + damped Jacobi sweeps on r - A z: the preconditioner of the
     flexible solve with the coarse level and no network."""
     P, Ac = aggregation(coef, B)

@@ -238,15 +238,16 @@ def test_coarse_correction_matches_oracle(dev):
     assert ez < 1e-10 and er < 1e-12
 
 
-@pytest.mark.parametrize("which,block,nx,ny", [("c1", 0, 0, 0), ("small", 8, 101, 43), ("small", 5, 101, 43),
+@pytest.mark.parametrize("which,block,nx,ny", [("c1", 0, 0, 0), ("c1", 16, 0, 0), ("small", 8, 101, 43),
+                                              ("small", 16, 101, 43), ("small", 5, 101, 43),
                                               ("small", 2, 150, 60)])
 def test_twolevel_precond_matches_oracle(which, block, nx, ny, dev):
-    """M(r) of the fused smoothing kernel (P A_c^-1 P^T r, then three damped
-    Jacobi sweeps on 32 x 32 tiles with a 4-cell halo) against the oracle's
+    """M(r) of the fused smoothing kernel (P A_c^-1 P^T r, then four damped
+    Jacobi sweeps on 32 x 32 tiles with a 5-cell halo) against the oracle's
     dense restatement: ragged systems whose tiles and aggregates do not divide
     the grid, aggregates of 2 x 2 (a coarse level of 2000+ unknowns, so one
     workgroup owns several aggregates: 2250 > 2048 workgroups), an aggregate
-    size that is not a power of two."""
+    size that is not a power of two, and aggregates of 16 x 16."""
     if which == "c1":
         coef, r = c1_pcor()
     else:
@@ -286,9 +287,25 @@ def test_twolevel_jacobi_many_aggregates(dev):
     assert abs(info.iterations - io["iterations"]) <= 0.25 * io["iterations"] + 5
 
 
+def test_twolevel_jacobi_block16(dev):
+    """The fused solve at C1 with 16 x 16 aggregates against the oracle."""
+    coef, b = c1_pcor()
+    ctx = _lib.context_for(dev)
+    ctx.call("km_set_coarse", 16)
+    try:
+        x, info = coarse_flex(dev, coef, b, None, _lib.KM_FLAG_COARSE)
+    finally:
+        ctx.call("km_set_coarse", 0)
+    true = np.linalg.norm(b - krylov.stencil(coef, x)) / np.linalg.norm(b)
+    _, io = krylov.fbicgstab(coef, b, eps=1e-8, precond=krylov.twolevel_jacobi_precond(coef, 16), fallback=False)
+    print(f"two-level 16x16: gpu {info.iterations} it, oracle {io['iterations']} it, true {true:.3e}")
+    assert info.status == _lib.KM_OK and true < 1e-8 and info.fallback == 0
+    assert abs(info.iterations - io["iterations"]) <= 0.25 * io["iterations"] + 5
+
+
 @pytest.mark.parametrize("mixed", [False, True])
 def test_twolevel_jacobi_matches_oracle(mixed, dev):
-    """Two-level preconditioner without the network: coarse correction + 3 damped
+    """Two-level preconditioner without the network: coarse correction + 4 damped
     Jacobi sweeps; FP64 iteration count within 25% of the oracle's flexible solve
     with the same preconditioner; FP32 vectors converge too (the preconditioned
     system is well conditioned)."""
