@@ -202,12 +202,18 @@ int km_batch_time_forward(km_ctx* ctx, int n, const km_case* cases);
 /* The batch's tile shape (chosen on its first step). */
 int km_batch_get_tile(km_ctx* ctx, int* ti, int* tj);
 
-/* Aggregate size of the coarse level (0: automatic, the smallest of 8, 16, ..., 256
- * giving at most 1024 coarse unknowns).  No counterpart in the reference. */
+/* Aggregate size of the coarse level (0: automatic, the smallest of 8, 12, 16, 24,
+ * 32, 48, ..., 256 giving at most 1024 coarse unknowns).  No counterpart in the
+ * reference. */
 int km_set_coarse(km_ctx* ctx, int block);
 /* z1 = P A_c^-1 P^T r and r1 = r - A z1 for one FP64 system (sets the coarse
  * level up first); the training data of the two-level network and tests use it. */
 int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const double* r, double* z1, double* r1);
+/* on != 0 (the default): a coarse level of the same shape as the last one
+ * updates the held A_c^-1 by one Newton-Schulz step X (2 I - A_c X) when
+ * ||I - A_c X||_inf < 0.1, instead of factoring A_c again; 0: always factor.
+ * No counterpart in the reference. */
+int km_set_coarse_update(km_ctx* ctx, int on);
 /* Aggregate size and coarse unknowns of the last coarse level. */
 int km_coarse_info(km_ctx* ctx, int* block, int* unknowns);
 /* z = M(r) for one FP64 system: the two-level Jacobi preconditioner of the

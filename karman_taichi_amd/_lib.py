@@ -95,6 +95,7 @@ SIGNATURES = {
     "km_set_coarse": [_CTX, _I],
     "km_coarse_correct": [_CTX, _I, _I, _P, _P, _P, _P],
     "km_coarse_info": [_CTX, ctypes.POINTER(_I), ctypes.POINTER(_I)],
+    "km_set_coarse_update": [_CTX, _I],
     "km_twolevel_precond": [_CTX, _I, _I, _P, _P, _P],
     "km_twolevel_tile": [_CTX, ctypes.POINTER(_I), ctypes.POINTER(_I)],
 }

@@ -20,6 +20,7 @@ LIB = HERE / "libkarman_hip.so"
 ARCH = os.environ.get("KM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+ROCM_LIB = os.environ.get("ROCM_PATH", "/opt/rocm") + "/lib"
 
 # -ffp-contract=off: no FMA contraction, so the discretisation kernels give
 # the bit pattern of the IEEE expression they mirror.  Packed FP32 math is
@@ -52,7 +53,9 @@ def build(verbose: bool = True) -> Path:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    # rocBLAS: the coarse level's inverse update is a plain DGEMM
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+           f"-L{ROCM_LIB}", "-lrocblas", f"-Wl,-rpath,{ROCM_LIB}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")

@@ -20,6 +20,10 @@ struct km_coarse {
     double* rc = nullptr;     // kMaxSlices x m restricted residual (per-slice partial sums)
     double* yc = nullptr;     // m coarse solution
     uint8_t* mask = nullptr;  // NX x NY: 1 on the free interior cells (the support of P)
+    size_t lds_max = 0;       // LDS per workgroup on this device (the whole-band factorisation)
+    bool held = false;        // inv holds A_c^-1 of a coarse level of the current shape
+    bool update = true;       // km_set_coarse_update: Newton-Schulz updates of the held inverse
+    void* blas = nullptr;     // rocBLAS handle (the update's GEMM)
     // tile shape of the fused smoothing launches per (NX, NY, sizeof T)
     std::map<std::tuple<int, int, int>, int> tiles;
 };
@@ -31,12 +35,17 @@ namespace kmcoarse {
 // iterations with three to 74 in the oracle; 0.7, 0.75, 0.85, 0.9 do worse).
 constexpr int kSweeps = 4;
 constexpr double kOmega = 0.8;
+// The held inverse X of the last coarse operator is updated to
+// X (2 I - A_c X) for the next one when ||I - A_c X||_inf < kNsTol (the
+// update's error is then below kNsTol^2); otherwise A_c is factored anew.
+constexpr double kNsTol = 0.1;
 // Row slices per aggregate in the fused Krylov updates: their partial
 // restrictions rc[s m + k] are summed by the coarse solve.
 constexpr int kMaxSlices = 8;
 
 // Aggregate edge length for an nx x ny interior (req > 0: that size; else the
-// smallest of 8, 16, ..., 256 that keeps the coarse level at <= 1024 unknowns).
+// smallest of 8, 12, 16, 24, ..., 256 that keeps the coarse level at <= 1024
+// unknowns).
 int choose_block(int nx, int ny, int req);
 
 // Galerkin coarse operator P^T A P of the FP64 system, its band LU, its

@@ -11,6 +11,8 @@
 // on r1 and its output is added to z1.
 #include "km_coarse.h"
 
+#include <rocblas/rocblas.h>
+
 #include <cmath>
 
 namespace {
@@ -94,42 +96,56 @@ k_coarse_band(int m, int mJ, int bw, const double* __restrict__ cc, double* __re
     }
 }
 
+// The whole band in LDS, one barrier per elimination step: step k reads
+// column k (the multipliers l = A[i][k] / A[k][k]) and updates columns > k;
+// the multipliers wait in a side buffer and are written over column k during
+// step k + 1, which reads no column below k + 1.
+__global__ void __launch_bounds__(64)
+k_coarse_factor_lds(int m, int bw, double* __restrict__ band, const int* __restrict__ skip) {
+    extern __shared__ double a[];
+    if (*skip) return;
+    const int W = 2 * bw + 1;
+    double* pend = a + (size_t)m * W;   // [2][bw]: multipliers of the last two steps
+    for (int e = threadIdx.x; e < m * W; e += blockDim.x) a[e] = band[e];
+    __syncthreads();
+    int prev_rows = 0;
+    for (int k = 0; k < m; ++k) {
+        const double* pl = pend + ((k + 1) & 1) * bw;   // step k - 1's multipliers, rows k .. k - 1 + prev_rows
+        for (int t = threadIdx.x; t < prev_rows; t += blockDim.x) {
+            const int i = k + t;
+            a[(size_t)i * W + bw + (k - 1 - i)] = pl[t];
+        }
+        const int rows = (k + bw < m - 1 ? k + bw : m - 1) - k;   // rows k+1 .. k+rows
+        const double* pk = a + (size_t)k * W + bw;                  // pk[o] = A[k][k + o]
+        const double rp = 1.0 / pk[0];
+        double* pn = pend + (k & 1) * bw;
+        // thread t updates column j = k + 1 + t of rows k + 1 .. k + rows (bw <= 64)
+        const int t = threadIdx.x;
+        if (t < rows) {
+            const int j = k + 1 + t;
+            const double pkj = pk[j - k];
+            for (int i = k + 1; i <= k + rows; ++i) {
+                double* ri = a + (size_t)i * W + bw;                // ri[o] = A[i][i + o]
+                const double l = ri[k - i] * rp;
+                ri[j - i] -= l * pkj;
+                if (t == 0) pn[i - k - 1] = l;
+            }
+        }
+        __syncthreads();
+        prev_rows = rows;
+    }
+    for (int e = threadIdx.x; e < m * W; e += blockDim.x) band[e] = a[e];
+}
+
 // Banded LU without pivoting (A_c is SPD) in one wave: step k updates the
 // (bw x bw) window below and right of the pivot; L (unit) overwrites the
 // strict lower band, U the rest.  The bw + 1 rows that steps k .. k + bw touch
 // live in an LDS ring (row r in slot r mod (bw + 1)); row k goes back to the
 // band once step k is done and row k + bw + 1, untouched so far, takes its slot.
-// The whole band in LDS: no global traffic inside the elimination.
 __global__ void __launch_bounds__(64)
-k_coarse_factor_lds(int m, int bw, double* __restrict__ band) {
-    extern __shared__ double a[];
-    const int W = 2 * bw + 1;
-    for (int e = threadIdx.x; e < m * W; e += blockDim.x) a[e] = band[e];
-    __syncthreads();
-    for (int k = 0; k < m; ++k) {
-        const int rows = (k + bw < m - 1 ? k + bw : m - 1) - k;
-        const double* pk = a + k * W + bw;
-        const double piv = pk[0];
-        for (int t = threadIdx.x; t < rows * rows; t += blockDim.x) {
-            const int i = k + 1 + t / rows, j = k + 1 + t % rows;
-            double* ri = a + i * W + bw;
-            const double l = ri[k - i] / piv;
-            const int o = j - i;
-            if (o >= -bw && o <= bw) ri[o] -= l * pk[j - k];
-        }
-        __syncthreads();
-        for (int t = threadIdx.x; t < rows; t += blockDim.x) {
-            const int i = k + 1 + t;
-            a[i * W + bw + (k - i)] /= piv;
-        }
-        __syncthreads();
-    }
-    for (int e = threadIdx.x; e < m * W; e += blockDim.x) band[e] = a[e];
-}
-
-__global__ void __launch_bounds__(64)
-k_coarse_factor(int m, int bw, double* __restrict__ band) {
+k_coarse_factor(int m, int bw, double* __restrict__ band, const int* __restrict__ skip) {
     extern __shared__ double ring[];
+    if (*skip) return;
     const int W = 2 * bw + 1, R = bw + 1;
     for (int e = threadIdx.x; e < R * W; e += blockDim.x) {
         const int row = e / W;
@@ -151,12 +167,14 @@ k_coarse_factor(int m, int bw, double* __restrict__ band) {
         const int rows = (k + bw < m - 1 ? k + bw : m - 1) - k;   // rows k+1 .. k+rows
         const double* pk = ring + (k % R) * W + bw;                 // pk[o] = A[k][k + o]
         const double piv = pk[0];
-        for (int t = threadIdx.x; t < rows * rows; t += blockDim.x) {
-            const int i = k + 1 + t / rows, j = k + 1 + t % rows;
-            double* ri = ring + (i % R) * W + bw;                   // ri[o] = A[i][i + o]
-            const double l = ri[k - i] / piv;
-            const int o = j - i;
-            if (o >= -bw && o <= bw) ri[o] -= l * pk[j - k];
+        // thread t updates column j = k + 1 + t of rows k + 1 .. k + rows (bw <= 64)
+        if ((int)threadIdx.x < rows) {
+            const int j = k + 1 + threadIdx.x;
+            const double pkj = pk[j - k];
+            for (int i = k + 1; i <= k + rows; ++i) {
+                double* ri = ring + (i % R) * W + bw;               // ri[o] = A[i][i + o]
+                ri[j - i] -= ri[k - i] / piv * pkj;
+            }
         }
         __syncthreads();
         for (int t = threadIdx.x; t < rows; t += blockDim.x) {
@@ -184,7 +202,7 @@ k_coarse_factor(int m, int bw, double* __restrict__ band) {
 // time; the next CH rows are loaded into registers meanwhile) and the last
 // BWT values of the column's substitution stay in registers (BWT >= bw; the
 // LDS rows are padded to 2 BWT + 1 entries with zeros, so every row is read
-// without a branch).
+// without a branch, and hold the reciprocal of U's diagonal).
 // Forward: L y = e_c for rows from the workgroup's first column on (y = 0
 // above it); back: U x = y, the rows in reverse, y read back from inv CH rows
 // ahead.  The bw products of a row are summed as a tree.
@@ -202,11 +220,13 @@ __device__ __forceinline__ double tree_dot(const double* __restrict__ f, int str
 
 template <int BWT>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_coarse_inverse(int m, int bw, const double* __restrict__ band, double* __restrict__ inv) {
+k_coarse_inverse(int m, int bw, const double* __restrict__ band, double* __restrict__ inv,
+                 const int* __restrict__ skip) {
     constexpr int CH = 32;                        // rows per LDS chunk
     constexpr int WMAX = 2 * BWT + 1;
     constexpr int PF = (CH * WMAX + KM_BLOCK - 1) / KM_BLOCK;   // chunk values per thread
     __shared__ double fb[CH * WMAX];
+    if (*skip) return;
     const int c0 = blockIdx.x * KM_BLOCK, c = c0 + threadIdx.x;
     const bool own = c < m;
     const int W = 2 * bw + 1;   // the band's row length
@@ -219,7 +239,8 @@ k_coarse_inverse(int m, int bw, const double* __restrict__ band, double* __restr
             const int e = threadIdx.x + u * KM_BLOCK;
             const int row = r0 + e / WMAX, o = e % WMAX - BWT;
             const bool in = e < CH * WMAX && row >= 0 && row < m && o >= -bw && o <= bw;
-            pf[u] = in ? band[(int64_t)row * W + bw + o] : 0.0;
+            const double val = in ? band[(int64_t)row * W + bw + o] : 0.0;
+            pf[u] = o == 0 ? (val != 0.0 ? 1.0 / val : 0.0) : val;   // the diagonal as its reciprocal
         }
     };
     auto stage = [&]() {
@@ -278,7 +299,7 @@ k_coarse_inverse(int m, int bw, const double* __restrict__ band, double* __restr
             if (u >= n) continue;
             const int i = i0 + u;
             const double* row = fb + u * WMAX + BWT;
-            const double v = (yc_[u] - tree_dot<BWT>(row + 1, 1, w)) / row[0];
+            const double v = (yc_[u] - tree_dot<BWT>(row + 1, 1, w)) * row[0];
 #pragma unroll
             for (int d = BWT - 1; d > 0; --d) w[d] = w[d - 1];
             w[0] = v;
@@ -290,7 +311,9 @@ k_coarse_inverse(int m, int bw, const double* __restrict__ band, double* __restr
 // The same for any bandwidth: one thread per column with the substitution in
 // the output row, each step reading its own factor entries.
 __global__ void __launch_bounds__(KM_BLOCK)
-k_coarse_inverse_wide(int m, int bw, const double* __restrict__ band, double* __restrict__ x) {
+k_coarse_inverse_wide(int m, int bw, const double* __restrict__ band, double* __restrict__ x,
+                      const int* __restrict__ skip) {
+    if (*skip) return;
     const int c = blockIdx.x * KM_BLOCK + threadIdx.x;
     if (c >= m) return;
     const int W = 2 * bw + 1;
@@ -308,6 +331,57 @@ k_coarse_inverse_wide(int m, int bw, const double* __restrict__ band, double* __
         for (int k = i + 1; k <= k1; ++k) v -= band[(int64_t)i * W + bw + (k - i)] * X(k);
         X(i) = v / band[(int64_t)i * W + bw];
     }
+}
+
+// Newton-Schulz update of the held inverse X for the new A_c: one wave per
+// row i forms (A_c X)[i][:] from the 5-point coarse coefficients, writes
+// M = 2 I - A_c X and the row sum of |I - A_c X| (its infinity norm, row by
+// row); X (2 I - A_c X) then has error at most the square of that norm.
+__global__ void __launch_bounds__(KM_BLOCK)
+k_coarse_ns_residual(int m, int mJ, const double* __restrict__ cc, const double* __restrict__ X,
+                     double* __restrict__ M, double* __restrict__ rowerr) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= m) return;
+    const double cp = cc[i], cw = cc[m + i], ce = cc[2 * m + i], cn = cc[3 * m + i], cs = cc[4 * m + i];
+    const double* x0 = X + (int64_t)i * m;
+    double e = 0.0;
+    for (int j = lane; j < m; j += 64) {
+        double v = cp * x0[j];
+        if (i >= mJ) v += cw * x0[j - (int64_t)mJ * m];
+        if (i + mJ < m) v += ce * x0[j + (int64_t)mJ * m];
+        if (i + 1 < m) v += cn * x0[j + m];
+        if (i > 0) v += cs * x0[j - m];
+        const double d = (i == j ? 1.0 : 0.0) - v;
+        M[(int64_t)i * m + j] = d + (i == j ? 1.0 : 0.0);
+        e += fabs(d);
+    }
+    e = wave_sum(e);
+    if (lane == 0) rowerr[i] = e;
+}
+
+// ok = 1 when max_i rowerr[i] < tol (the update is kept and the
+// factorisation skipped), else 0; NaN rows count as too large
+__global__ void __launch_bounds__(KM_BLOCK)
+k_coarse_ns_check(int m, const double* __restrict__ rowerr, double tol, int* __restrict__ ok) {
+    __shared__ double sh[4];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < m; i += KM_BLOCK) {
+        const double e = rowerr[i];
+        v = fmax(v, e == e ? e : 1e300);
+    }
+    v = block_max(v, sh);
+    if (threadIdx.x == 0) *ok = v < tol ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(KM_BLOCK)
+k_coarse_ns_accept(int64_t n, const double* __restrict__ Xn, double* __restrict__ X, const int* __restrict__ ok) {
+    if (!*ok) return;
+    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK)
+        X[k] = Xn[k];
+}
+
+__global__ void k_coarse_flag(int* __restrict__ f, int v) {
+    if (threadIdx.x == 0) *f = v;
 }
 
 // rc[agg] = sum of r over the aggregate's free cells (one workgroup per aggregate)
@@ -409,6 +483,25 @@ int flat_blocks2(int64_t n) {
 
 size_t a256(size_t v) { return (v + 255) & ~size_t(255); }
 
+// Xn = X M for row-major m x m X, M: the column-major product M^T X^T
+// (a plain library GEMM)
+int ns_gemm(km_ctx* ctx, int m, const double* M, const double* X, double* Xn, hipStream_t s) {
+    km_coarse* C = ctx->coarse;
+    if (!C->blas) {
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success)
+            return km_fail(ctx, KM_EHIP, "rocblas_create_handle failed");
+        C->blas = h;
+    }
+    rocblas_handle h = static_cast<rocblas_handle>(C->blas);
+    if (rocblas_set_stream(h, s) != rocblas_status_success) return km_fail(ctx, KM_EHIP, "rocblas_set_stream failed");
+    const double one = 1.0, zero = 0.0;
+    const rocblas_status st = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, m, m, &one, M, m, X,
+                                            m, &zero, Xn, m);
+    if (st != rocblas_status_success) return km_fail(ctx, KM_EHIP, "rocblas_dgemm: status %d", (int)st);
+    return KM_OK;
+}
+
 km_coarse* get_coarse(km_ctx* ctx) {
     if (!ctx->coarse) ctx->coarse = new km_coarse();
     return ctx->coarse;
@@ -420,7 +513,7 @@ namespace kmcoarse {
 
 int choose_block(int nx, int ny, int req) {
     if (req > 0) return req;
-    for (int B : {8, 16, 32, 64, 128, 256}) {
+    for (int B : {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256}) {
         const int64_t m = (int64_t)((nx + B - 1) / B) * ((ny + B - 1) / B);
         if (m <= 1024) return B;
     }
@@ -439,8 +532,9 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
     KM_REQUIRE(ctx, (size_t)(bw + 1) * (2 * bw + 1) * sizeof(double) <= 64 * 1024,
                "coarse band of width %d is too wide (aggregate %d)", bw, B);
     const int64_t n = (int64_t)NX * NY;
-    const size_t need = a256(5 * (size_t)m * 8) + a256((size_t)m * (2 * bw + 1) * 8) + a256((size_t)m * m * 8) +
-                        a256((size_t)kMaxSlices * m * 8) + a256((size_t)m * 8) + a256((size_t)n);
+    const size_t mm = (size_t)m * m * 8;
+    const size_t need = a256(5 * (size_t)m * 8) + a256((size_t)m * (2 * bw + 1) * 8) + 3 * a256(mm) +
+                        a256((size_t)kMaxSlices * m * 8) + 2 * a256((size_t)m * 8) + a256((size_t)n) + 256;
     if (!C->arena || C->bytes < need) {
         KM_HIP(ctx, hipStreamSynchronize(s));
         km_drop_graphs(ctx);
@@ -451,38 +545,66 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
         if (e != hipSuccess) return km_fail(ctx, KM_ENOMEM, "coarse hipMalloc(%zu): %s", need, hipGetErrorString(e));
         C->arena = static_cast<char*>(p);
         C->bytes = need;
+        C->held = false;
     }
     char* q = C->arena;
     C->cc = reinterpret_cast<double*>(q); q += a256(5 * (size_t)m * 8);
     C->band = reinterpret_cast<double*>(q); q += a256((size_t)m * (2 * bw + 1) * 8);
-    C->inv = reinterpret_cast<double*>(q); q += a256((size_t)m * m * 8);
+    C->inv = reinterpret_cast<double*>(q); q += a256(mm);
+    double* nsM = reinterpret_cast<double*>(q); q += a256(mm);
+    double* nsX = reinterpret_cast<double*>(q); q += a256(mm);
     C->rc = reinterpret_cast<double*>(q); q += a256((size_t)kMaxSlices * m * 8);
     C->yc = reinterpret_cast<double*>(q); q += a256((size_t)m * 8);
+    double* rowerr = reinterpret_cast<double*>(q); q += a256((size_t)m * 8);
+    int* ok = reinterpret_cast<int*>(q); q += 256;
     C->mask = reinterpret_cast<uint8_t*>(q);
+    // the held inverse belongs to a coarse level of the same shape: try the update
+    const bool update = C->held && C->update && C->NX == NX && C->NY == NY && C->B == B;
     C->NX = NX; C->NY = NY; C->B = B; C->mI = mI; C->mJ = mJ; C->m = m; C->bw = bw;
     hipLaunchKernelGGL(k_coarse_mask, dim3(flat_blocks2(n)), dim3(KM_BLOCK), 0, s, NX, NY, coef, C->mask);
     hipLaunchKernelGGL(k_coarse_coef, dim3(m), dim3(KM_BLOCK), 0, s, NX, NY, B, mI, mJ, coef, C->cc);
     hipLaunchKernelGGL(k_coarse_band, dim3(flat_blocks2((int64_t)m * (2 * bw + 1))), dim3(1024), 0, s, m, mJ, bw,
                        (const double*)C->cc, C->band);
+    if (update) {
+        // X <- X (2 I - A_c X) with the previous solve's X, kept when
+        // ||I - A_c X||_inf < kNsTol (then the error is below kNsTol^2)
+        hipLaunchKernelGGL(k_coarse_ns_residual, dim3((m + 3) / 4), dim3(KM_BLOCK), 0, s, m, mJ, (const double*)C->cc,
+                           (const double*)C->inv, nsM, rowerr);
+        hipLaunchKernelGGL(k_coarse_ns_check, dim3(1), dim3(KM_BLOCK), 0, s, m, (const double*)rowerr, kNsTol, ok);
+        if (int rc = ns_gemm(ctx, m, nsM, C->inv, nsX, s)) return rc;
+        hipLaunchKernelGGL(k_coarse_ns_accept, dim3(flat_blocks2((int64_t)m * m)), dim3(KM_BLOCK), 0, s,
+                           (int64_t)m * m, (const double*)nsX, C->inv, (const int*)ok);
+    } else {
+        hipLaunchKernelGGL(k_coarse_flag, dim3(1), dim3(64), 0, s, ok, 0);
+    }
     // one wave (the ring's register prefetch assumes 64 threads): a step
     // updates at most bw x bw entries
-    const size_t band_bytes = (size_t)m * (2 * bw + 1) * sizeof(double);
-    if (band_bytes <= 64 * 1024)
-        hipLaunchKernelGGL(k_coarse_factor_lds, dim3(1), dim3(64), band_bytes, s, m, bw, C->band);
+    const size_t lds_bytes = ((size_t)m * (2 * bw + 1) + 2 * (size_t)bw) * sizeof(double);
+    if (C->lds_max == 0) {
+        int dev = 0, v = 0;
+        KM_HIP(ctx, hipGetDevice(&dev));
+        KM_HIP(ctx, hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        C->lds_max = v > 0 ? (size_t)v : 64 * 1024;
+    }
+    // the factorisation and explicit inverse run unless the update was kept
+    const int* skip = ok;
+    if (lds_bytes <= C->lds_max)
+        hipLaunchKernelGGL(k_coarse_factor_lds, dim3(1), dim3(64), lds_bytes, s, m, bw, C->band, skip);
     else
         hipLaunchKernelGGL(k_coarse_factor, dim3(1), dim3(64), (size_t)(bw + 1) * (2 * bw + 1) * sizeof(double), s,
-                           m, bw, C->band);
+                           m, bw, C->band, skip);
     const dim3 gi((m + KM_BLOCK - 1) / KM_BLOCK);
     const double* band = C->band;
     if (bw <= 8)
-        hipLaunchKernelGGL(k_coarse_inverse<8>, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv);
+        hipLaunchKernelGGL(k_coarse_inverse<8>, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv, skip);
     else if (bw <= 16)
-        hipLaunchKernelGGL(k_coarse_inverse<16>, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv);
+        hipLaunchKernelGGL(k_coarse_inverse<16>, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv, skip);
     else if (bw <= 32)
-        hipLaunchKernelGGL(k_coarse_inverse<32>, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv);
+        hipLaunchKernelGGL(k_coarse_inverse<32>, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv, skip);
     else
-        hipLaunchKernelGGL(k_coarse_inverse_wide, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv);
+        hipLaunchKernelGGL(k_coarse_inverse_wide, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv, skip);
     KM_CHECK_LAUNCH(ctx);
+    C->held = true;
     return KM_OK;
 }
 
@@ -529,6 +651,7 @@ template int apply<double>(km_ctx*, const double*, const double*, const double*,
 
 void km_coarse_free(km_ctx* ctx) {
     if (!ctx || !ctx->coarse) return;
+    if (ctx->coarse->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->coarse->blas));
     if (ctx->coarse->arena) (void)hipFree(ctx->coarse->arena);
     delete ctx->coarse;
     ctx->coarse = nullptr;
@@ -549,6 +672,12 @@ int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const dou
     KM_REQUIRE(ctx, NX >= 3 && NY >= 3 && (int64_t)NX * NY < ((int64_t)1 << 31), "system shape %d x %d", NX, NY);
     if (int rc = kmcoarse::setup(ctx, NX, NY, coef, ctx->stream)) return rc;
     return kmcoarse::apply<double>(ctx, coef, coef, r, z1, r1, nullptr, nullptr, ctx->stream);
+}
+
+int km_set_coarse_update(km_ctx* ctx, int on) {
+    if (!ctx) return KM_EINVAL;
+    get_coarse(ctx)->update = on != 0;
+    return KM_OK;
 }
 
 int km_coarse_info(km_ctx* ctx, int* block, int* unknowns) {

@@ -301,11 +301,11 @@ def fbicgstab(coef, b, eps=1e-8, x0=None, max_iter=None, precond=None, stall_win
 # ---------------------------------------------------------------- two-level
 
 def coarse_block(nx, ny, req=0):
-    """Aggregate edge length: req, else the smallest of 8, 16, ..., 256 giving
-    at most 1024 coarse unknowns."""
+    """Aggregate edge length: req, else the smallest of 8, 12, 16, 24, ..., 256
+    giving at most 1024 coarse unknowns."""
     if req > 0:
         return req
-    for B in (8, 16, 32, 64, 128, 256):
+    for B in (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256):
         if (-(-nx // B)) * (-(-ny // B)) <= 1024:
             return B
     return 512

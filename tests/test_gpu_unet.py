@@ -270,6 +270,39 @@ def test_twolevel_precond_matches_oracle(which, block, nx, ny, dev):
     assert e < 1e-10
 
 
+@pytest.mark.parametrize("change", ["same", "perturbed", "other"])
+def test_twolevel_inverse_update(change, dev):
+    """The held coarse inverse carried to the next coarse operator of the same
+    shape: the same system again (the Newton-Schulz step is kept and exact to
+    rounding), a 1e-7 relative perturbation of the links (kept: error about
+    the square of ||I - A_c X||), an unrelated system (rejected: A_c is factored
+    anew).  M(r) against the oracle's dense restatement for the second system."""
+    coef1, r = small_system(5, nx=101, ny=43)
+    if change == "same":
+        coef2 = coef1.copy()
+    elif change == "perturbed":
+        rng = np.random.default_rng(2)
+        coef2 = coef1.copy()
+        links = coef2[1:] != 0
+        coef2[1:][links] *= 1.0 + 1e-7 * rng.standard_normal(int(links.sum()))
+        coef2[0] = np.where(coef1[0] != 1.0, -coef2[1:].sum(axis=0) + (coef1[0] + coef1[1:].sum(axis=0)), 1.0)
+    else:
+        coef2, _ = small_system(6, nx=101, ny=43)
+    ctx = _lib.context_for(dev)
+    z = torch.empty(r.shape, dtype=torch.float64, device=dev)
+    ctx.call("km_set_coarse", 16)
+    try:
+        for coef in (coef1, coef2):
+            ct, rt_ = T(coef, dev), T(r, dev)
+            ctx.call("km_twolevel_precond", r.shape[0], r.shape[1], ptr(ct), ptr(rt_), ptr(z))
+    finally:
+        ctx.call("km_set_coarse", 0)
+    zo = krylov.twolevel_jacobi_precond(coef2, 16)(r)
+    e = rel(z.cpu().numpy(), zo)
+    print(f"inverse update ({change}): M(r) rel {e:.2e}")
+    assert e < (1e-10 if change != "perturbed" else 1e-8)
+
+
 def test_twolevel_jacobi_many_aggregates(dev):
     """The fused flexible solve with 2 x 2 aggregates on a ragged system
     (several aggregates per workgroup in the Krylov update kernels)."""
