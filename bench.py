@@ -67,7 +67,8 @@ def parse():
                     help="repetitions of the p' solve on the final system, plain BiCGSTAB vs the mode's solver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=24, help="outer steps in the CPU baseline sample")
-    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "k_bicg_traffic_c2_1280x256.json"))
+    ap.add_argument("--traffic-json", default=None,
+                    help="HBM bytes per launch from rocprofv3 counters (default: the mode's file under profiles/)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a device-to-device copy of this many MiB (counter calibration for rocprofv3 --pmc)")
     return ap.parse_args()
@@ -177,8 +178,10 @@ def main():
     alg_bytes = BYTES_PER_CELL[dom] * cells_p
     achieved = alg_bytes / (kern[dom] * 1e-6) / 1e9
     traffic = None
+    tjson = args.traffic_json or os.path.join(
+        ROOT, "profiles", ("k_tl_traffic_" if twolevel and not learned else "k_bicg_traffic_") + "c2_1280x256.json")
     try:
-        with open(args.traffic_json) as f:
+        with open(tjson) as f:
             tj = json.load(f)
         traffic = tj.get(args.case, {}).get(dom)
     except (OSError, ValueError):

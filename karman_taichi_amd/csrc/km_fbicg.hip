@@ -415,7 +415,13 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         return KM_OK;
     }
     double rr = 0.0;
-    if (int rc = residual64(ctx, NX, NY, coef, b, x, r64, &rr)) return rc;
+    if (flags & KM_FLAG_X0) {
+        if (int rc = residual64(ctx, NX, NY, coef, b, x, r64, &rr)) return rc;
+    } else {
+        // x = 0: r = b and r.r = b.b, no second round trip to the host
+        KM_HIP(ctx, hipMemcpyAsync(r64, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        rr = bb;
+    }
     double rel = sqrt(rr) / bnorm;
     int total = 0, sweeps = 0, status = KM_MAXITER;
     // warm start x += ||r|| M(r / ||r||), kept only if it lowers the residual

@@ -120,7 +120,10 @@ k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restric
     if (st->done) return;
     const KScal sp = st->ring[(ph + KM_RING - 1) % KM_RING];
     const int i = sp.iter + 1;
-    const int agg = blockIdx.x / g.S, sl = blockIdx.x - agg * g.S;
+    // each XCD takes a contiguous run of aggregates: the cache lines that
+    // neighbouring aggregates share are fetched into one L2
+    const int wg = xcd_remap(blockIdx.x, g.G);
+    const int agg = wg / g.S, sl = wg - agg * g.S;
     const Slice sq = slice_of(g, agg, sl);
     int64_t kk[kStage];
     bool ok[kStage], fr[kStage];
@@ -190,7 +193,7 @@ k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restric
     rr = block_sum(rr, sh);
     if (threadIdx.x == 0) {
         rc[(int64_t)sl * g.m + agg] = a;
-        P[TL_PR + blockIdx.x] = rr;
+        P[TL_PR + wg] = rr;
     }
 }
 
@@ -202,7 +205,8 @@ k_tl_s(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, const T* __r
        double* __restrict__ P) {
     __shared__ double sh[8];
     if (st->done) return;
-    const int agg = blockIdx.x / g.S, sl = blockIdx.x - agg * g.S;
+    const int wg = xcd_remap(blockIdx.x, g.G);
+    const int agg = wg / g.S, sl = wg - agg * g.S;
     const Slice sq = slice_of(g, agg, sl);
     int64_t kk[kStage];
     bool ok[kStage], fr[kStage];
