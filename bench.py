@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--mode", default="twolevel", choices=list(MODES))
     ap.add_argument("--weights", default=os.path.join(ROOT, "karman_taichi_amd", "weights", "unet_c2.npz"),
                     help="network parameters for the learned modes")
+    ap.add_argument("--coarse-block", type=int, default=0,
+                    help="aggregate edge of the two-level coarse level (0: automatic)")
     ap.add_argument("--compare-reps", type=int, default=5,
                     help="repetitions of the p' solve on the final system, plain BiCGSTAB vs the mode's solver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -98,6 +100,7 @@ def main():
     p_solver, fp32, twolevel, learned = MODES[args.mode]
     s.mixed = fp32
     s.p_coarse = twolevel
+    s.ctx.call("km_set_coarse", int(args.coarse_block))
     if p_solver == "cg":
         s.p_solver = "cg"
     if learned:
@@ -170,11 +173,14 @@ def main():
     # which agrees with rocprofv3's per-kernel averages; events around every
     # dispatch of a running iteration add ~3 us of event overhead per launch.
     dom = max(kern, key=kern.get)
-    tile = None
+    tile = coarse = None
     if twolevel and not learned:
         ti, tj = ctypes.c_int(), ctypes.c_int()
         ctx.call("km_twolevel_tile", ctypes.byref(ti), ctypes.byref(tj))
         tile = [ti.value, tj.value]
+        blk, mc = ctypes.c_int(), ctypes.c_int()
+        ctx.call("km_coarse_info", ctypes.byref(blk), ctypes.byref(mc))
+        coarse = {"aggregate": blk.value, "unknowns": mc.value}
     alg_bytes = BYTES_PER_CELL[dom] * cells_p
     achieved = alg_bytes / (kern[dom] * 1e-6) / 1e9
     traffic = None
@@ -223,7 +229,7 @@ def main():
                                f"{case.obstacle_cells}x{case.obstacle_cells} cylinder, "
                                f"{n_outer} outer steps per time step, inner eps 1e-8",
                    "mode": args.mode, "p_solver": p_solver, "p_twolevel": twolevel, "p_network": learned,
-                   "p_vectors": "f32" if fp32 else "f64", "smoothing_tile": tile},
+                   "p_vectors": "f32" if fp32 else "f64", "smoothing_tile": tile, "coarse": coarse},
         "p_solve_ms_median": round(statistics.median(p_ms), 3),
         "p_iters_median": statistics.median(p_it),
         "p_rel_resid_true_max": max(r["p_true"] for r in rec),

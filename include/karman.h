@@ -203,7 +203,7 @@ int km_batch_time_forward(km_ctx* ctx, int n, const km_case* cases);
 int km_batch_get_tile(km_ctx* ctx, int* ti, int* tj);
 
 /* Aggregate size of the coarse level (0: automatic, the smallest of 8, 12, 16, 24,
- * 32, 48, ..., 256 giving at most 1024 coarse unknowns).  No counterpart in the
+ * 32, 48, ..., 256 giving at most 2048 coarse unknowns).  No counterpart in the
  * reference. */
 int km_set_coarse(km_ctx* ctx, int block);
 /* z1 = P A_c^-1 P^T r and r1 = r - A z1 for one FP64 system (sets the coarse

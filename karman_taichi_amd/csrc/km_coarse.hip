@@ -427,7 +427,15 @@ k_coarse_matvec(int m, int S, const double* __restrict__ inv, const double* __re
     if (row >= m) return;
     const double* a = inv + (int64_t)row * m;
     double s = 0.0;
-    for (int k = lane; k < m; k += 64) s += a[k] * rs[k];
+    // eight row loads in flight per lane, summed in k order
+    for (int k0 = lane; k0 < m; k0 += 8 * 64) {
+        double av[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) av[u] = k0 + 64 * u < m ? a[k0 + 64 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k0 + 64 * u < m) s += av[u] * rs[k0 + 64 * u];
+    }
     s = wave_sum(s);
     if (lane == 0) yc[row] = s;
 }
@@ -515,7 +523,7 @@ int choose_block(int nx, int ny, int req) {
     if (req > 0) return req;
     for (int B : {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256}) {
         const int64_t m = (int64_t)((nx + B - 1) / B) * ((ny + B - 1) / B);
-        if (m <= 1024) return B;
+        if (m <= 2048) return B;
     }
     return 512;
 }

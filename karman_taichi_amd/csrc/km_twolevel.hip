@@ -52,22 +52,37 @@ __device__ __forceinline__ void finish_state(KState* st, int status, int iter, d
 // NX x NY system belongs to exactly one aggregate.  Each aggregate's rows are
 // cut into S slices, one workgroup each (G = m S workgroups).
 struct AggGeom {
-    int NX, NY, B, mI, mJ, m, S, G;
+    int NX, NY, B, mI, mJ, m;
+    int A;      // aggregates per workgroup, adjacent along J (small aggregates)
+    int ngrp;   // groups of A per aggregate row
+    int S;      // row slices per group (large aggregates)
+    int G;      // workgroups: mI ngrp S
 };
+constexpr int kMaxA = 8;
 
-// Cells of slice sl of aggregate agg: rows [ra, ra + cnt / ej), columns
-// [je0, je0 + ej); cell t of the slice is k(t).
+// Cells of one workgroup: rows [ra, ra + cnt / ej) x columns [je0, je0 + ej)
+// of aggregates (I, J0 .. J0 + nA - 1); cell t is k(t), and loc(t) is the
+// index (0 .. nA - 1) of its aggregate within the group.
 struct Slice {
-    int ra, je0, ej, cnt, NY;
+    int ra, je0, ej, cnt, NY, B, mJ, I, J0, nA, sl;
     __device__ __forceinline__ int64_t k(int t) const { return (int64_t)(ra + t / ej) * NY + (je0 + t % ej); }
+    __device__ __forceinline__ int loc(int t) const {
+        const int j = je0 + t % ej;
+        int J = j < 1 ? 0 : (j - 1) / B;
+        if (J > mJ - 1) J = mJ - 1;
+        return J - J0;
+    }
 };
 
-__device__ __forceinline__ Slice slice_of(const AggGeom& g, int agg, int sl) {
-    const int I = agg / g.mJ, J = agg - I * g.mJ;
+__device__ __forceinline__ Slice slice_of(const AggGeom& g, int wg) {
+    const int sl = wg % g.S, rest = wg / g.S;
+    const int grp = rest % g.ngrp, I = rest / g.ngrp;
+    const int J0 = grp * g.A, J1 = (J0 + g.A < g.mJ ? J0 + g.A : g.mJ) - 1;
     const int nx = g.NX - 2, ny = g.NY - 2;
-    const int i0 = 1 + I * g.B, j0 = 1 + J * g.B;
+    const int i0 = 1 + I * g.B;
     const int ie0 = I == 0 ? 0 : i0, ie1 = I == g.mI - 1 ? g.NX - 1 : (i0 + g.B - 1 < nx ? i0 + g.B - 1 : nx);
-    const int je0 = J == 0 ? 0 : j0, je1 = J == g.mJ - 1 ? g.NY - 1 : (j0 + g.B - 1 < ny ? j0 + g.B - 1 : ny);
+    const int je0 = J0 == 0 ? 0 : 1 + J0 * g.B;
+    const int je1 = J1 == g.mJ - 1 ? g.NY - 1 : (J1 * g.B + g.B < ny ? J1 * g.B + g.B : ny);
     const int er = ie1 - ie0 + 1;
     Slice q;
     q.ej = je1 - je0 + 1;
@@ -75,7 +90,44 @@ __device__ __forceinline__ Slice slice_of(const AggGeom& g, int agg, int sl) {
     q.ra = ie0 + (er * sl) / g.S;
     q.cnt = (ie0 + (er * (sl + 1)) / g.S - q.ra) * q.ej;
     q.NY = g.NY;
+    q.B = g.B;
+    q.mJ = g.mJ;
+    q.I = I;
+    q.J0 = J0;
+    q.nA = J1 - J0 + 1;
+    q.sl = sl;
     return q;
+}
+
+// The group's restriction sums: each thread's per-aggregate sums, reduced over
+// the workgroup in a fixed order (waves, then the four waves) and written as
+// slice partials rc[sl m + agg].
+__device__ __forceinline__ void write_restriction(const Slice& q, int m, const double (&ac)[kMaxA],
+                                                  double* __restrict__ rc) {
+    __shared__ double shr[4 * kMaxA];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (q.nA == 1) {   // one aggregate: one sum
+        const double v = wave_sum(ac[0]);
+        if (lane == 0) shr[w] = v;
+    } else {
+#pragma unroll
+        for (int u = 0; u < kMaxA; ++u) {
+            const double v = wave_sum(ac[u]);
+            if (lane == 0) shr[u * 4 + w] = v;
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < q.nA) {
+        const int u = threadIdx.x;
+        rc[(int64_t)q.sl * m + (int64_t)q.I * q.mJ + q.J0 + u] =
+            (shr[u * 4] + shr[u * 4 + 1]) + (shr[u * 4 + 2] + shr[u * 4 + 3]);
+    }
+}
+
+__device__ __forceinline__ void add_to(double (&ac)[kMaxA], int a, double v) {
+#pragma unroll
+    for (int u = 0; u < kMaxA; ++u) ac[u] += u == a ? v : 0.0;
 }
 
 // Cells per thread whose loads are issued before the scalars are reduced
@@ -123,9 +175,9 @@ k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restric
     // each XCD takes a contiguous run of aggregates: the cache lines that
     // neighbouring aggregates share are fetched into one L2
     const int wg = xcd_remap(blockIdx.x, g.G);
-    const int agg = wg / g.S, sl = wg - agg * g.S;
-    const Slice sq = slice_of(g, agg, sl);
+    const Slice sq = slice_of(g, wg);
     int64_t kk[kStage];
+    int la[kStage];
     bool ok[kStage], fr[kStage];
     T xv[kStage], hv[kStage], s2v[kStage], sv[kStage], tv[kStage], pv[kStage], vv[kStage];
 #pragma unroll
@@ -133,6 +185,7 @@ k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restric
         const int tt = threadIdx.x + c * KM_BLOCK;
         ok[c] = tt < sq.cnt;
         kk[c] = ok[c] ? sq.k(tt) : 0;
+        la[c] = ok[c] ? sq.loc(tt) : 0;
         const int64_t k = kk[c];
         fr[c] = ok[c] && mask[k] != 0;
         const bool ld = ok[c] && i > 0;
@@ -163,8 +216,11 @@ k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restric
     }
     const double alpha = sp.alpha;
     const T fo = (T)omega, fb = (T)beta;
-    double rr = 0.0, a = 0.0;
-    auto upd = [&](int64_t k, bool free, T xk, T hk, T sk2, T sk, T tk, T pk0, T vk) {
+    double rr = 0.0;
+    double ac[kMaxA];
+#pragma unroll
+    for (int u = 0; u < kMaxA; ++u) ac[u] = 0.0;
+    auto upd = [&](int64_t k, bool free, int a, T xk, T hk, T sk2, T sk, T tk, T pk0, T vk) {
         T rn, pk;
         if (i == 0) {
             rn = sk;                       // r_0 (staged in the s slot)
@@ -177,38 +233,35 @@ k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restric
         }
         p[k] = pk;
         rr += (double)rn * (double)rn;
-        if (free) a += (double)pk;
+        if (free) add_to(ac, a, (double)pk);
     };
 #pragma unroll
     for (int c = 0; c < kStage; ++c)
-        if (ok[c]) upd(kk[c], fr[c], xv[c], hv[c], s2v[c], sv[c], tv[c], pv[c], vv[c]);
+        if (ok[c]) upd(kk[c], fr[c], la[c], xv[c], hv[c], s2v[c], sv[c], tv[c], pv[c], vv[c]);
     for (int tt = threadIdx.x + kStage * KM_BLOCK; tt < sq.cnt; tt += KM_BLOCK) {
         const int64_t k = sq.k(tt);
         if (i == 0)
-            upd(k, mask[k] != 0, T(0), T(0), T(0), r[k], T(0), T(0), T(0));
+            upd(k, mask[k] != 0, sq.loc(tt), T(0), T(0), T(0), r[k], T(0), T(0), T(0));
         else
-            upd(k, mask[k] != 0, x[k], ph_[k], sh_[k], s[k], t[k], p[k], v[k]);
+            upd(k, mask[k] != 0, sq.loc(tt), x[k], ph_[k], sh_[k], s[k], t[k], p[k], v[k]);
     }
-    a = block_sum(a, sh);
+    write_restriction(sq, g.m, ac, rc);
     rr = block_sum(rr, sh);
-    if (threadIdx.x == 0) {
-        rc[(int64_t)sl * g.m + agg] = a;
-        P[TL_PR + wg] = rr;
-    }
+    if (threadIdx.x == 0) P[TL_PR + wg] = rr;
 }
 
 // K_S: stopping test, alpha, s = r - alpha v, P^T s
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_tl_s(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, const T* __restrict__ r,
+k_tl_s(AggGeom g, int Gt, int Gp, int ph, const uint8_t* __restrict__ mask, const T* __restrict__ r,
        const T* __restrict__ v, T* __restrict__ s, double* __restrict__ rc, KState* __restrict__ st,
        double* __restrict__ P) {
     __shared__ double sh[8];
     if (st->done) return;
     const int wg = xcd_remap(blockIdx.x, g.G);
-    const int agg = wg / g.S, sl = wg - agg * g.S;
-    const Slice sq = slice_of(g, agg, sl);
+    const Slice sq = slice_of(g, wg);
     int64_t kk[kStage];
+    int la[kStage];
     bool ok[kStage], fr[kStage];
     T rv_[kStage], vv[kStage];
 #pragma unroll
@@ -216,13 +269,14 @@ k_tl_s(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, const T* __r
         const int tt = threadIdx.x + c * KM_BLOCK;
         ok[c] = tt < sq.cnt;
         kk[c] = ok[c] ? sq.k(tt) : 0;
+        la[c] = ok[c] ? sq.loc(tt) : 0;
         fr[c] = ok[c] && mask[kk[c]] != 0;
         rv_[c] = ok[c] ? r[kk[c]] : T(0);
         vv[c] = ok[c] ? v[kk[c]] : T(0);
     }
     double tv[1], tr[1];
     sum_partials_wide<1>(P + TL_PV, Gt, tv);
-    sum_partials_wide<1>(P + TL_PR, g.G, tr);
+    sum_partials_wide<1>(P + TL_PR, Gp, tr);   // K_P's workgroups
     const double rv = tv[0], rr = tr[0];
     const KScal sp = st->ring[ph];
     const KScal spp = st->ring[(ph + KM_RING - 1) % KM_RING];
@@ -257,21 +311,22 @@ k_tl_s(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, const T* __r
         st->iter = i;
     }
     const T fa = (T)alpha;
-    double a = 0.0;
-    auto upd = [&](int64_t k, bool free, T rk, T vk) {
+    double ac[kMaxA];
+#pragma unroll
+    for (int u = 0; u < kMaxA; ++u) ac[u] = 0.0;
+    auto upd = [&](int64_t k, bool free, int a, T rk, T vk) {
         const T sk = rk - fa * vk;
         s[k] = sk;
-        if (free) a += (double)sk;
+        if (free) add_to(ac, a, (double)sk);
     };
 #pragma unroll
     for (int c = 0; c < kStage; ++c)
-        if (ok[c]) upd(kk[c], fr[c], rv_[c], vv[c]);
+        if (ok[c]) upd(kk[c], fr[c], la[c], rv_[c], vv[c]);
     for (int tt = threadIdx.x + kStage * KM_BLOCK; tt < sq.cnt; tt += KM_BLOCK) {
         const int64_t k = sq.k(tt);
-        upd(k, mask[k] != 0, r[k], v[k]);
+        upd(k, mask[k] != 0, sq.loc(tt), r[k], v[k]);
     }
-    a = block_sum(a, sh);
-    if (threadIdx.x == 0) rc[(int64_t)sl * g.m + agg] = a;
+    write_restriction(sq, g.m, ac, rc);
 }
 
 // K_V / K_T / the standalone smoother: z = M(src) on the tiles, then (MODE 1)
@@ -441,18 +496,31 @@ void launch_smooth(int which, hipStream_t q, int NX, int NY, int B, int mJ, cons
 
 int smooth_workgroups(int which, int NX, int NY) { return make_tile(NX, NY, kTileI[which], kTileJ[which]).G; }
 
-// About kStage x 256 cells per slice workgroup (fewer workgroups also means
-// fewer redundant partial-sum reads), at most kMaxSlices slices and
-// kMaxAggWg workgroups
-AggGeom agg_geom(const km_coarse* C) {
+// About `target` cells per workgroup: large aggregates are cut into at most
+// kMaxSlices row slices, small ones grouped kMaxA at most along J; at most
+// kMaxAggWg workgroups.  K_P, with seven loads per cell and four partial
+// planes to reduce first, takes kStage x 256 (fewer workgroups, fewer
+// redundant partial-sum reads); K_S, two loads per cell, takes 256.
+constexpr int64_t kTargetP = (int64_t)kStage * KM_BLOCK, kTargetS = KM_BLOCK;
+AggGeom agg_geom(const km_coarse* C, int64_t target) {
     AggGeom g;
     g.NX = C->NX; g.NY = C->NY; g.B = C->B; g.mI = C->mI; g.mJ = C->mJ; g.m = C->m;
-    int S = (int)(((int64_t)C->B * C->B) / (kStage * KM_BLOCK));
-    if (S > kmcoarse::kMaxSlices) S = kmcoarse::kMaxSlices;
-    if (S < 1) S = 1;
+    const int64_t cells = (int64_t)C->B * C->B;
+    int S = 1, A = 1;
+    if (cells >= target) {
+        S = (int)(cells / target);
+        if (S > kmcoarse::kMaxSlices) S = kmcoarse::kMaxSlices;
+    } else {
+        A = (int)(target / cells);
+        if (A > kMaxA) A = kMaxA;
+        if (A > C->mJ) A = C->mJ;
+        if (A < 1) A = 1;
+    }
     while (S > 1 && (int64_t)C->m * S > kMaxAggWg) --S;
+    g.A = A;
+    g.ngrp = (C->mJ + A - 1) / A;
     g.S = S;
-    g.G = C->m * S;
+    g.G = C->mI * g.ngrp * S;
     return g;
 }
 
@@ -547,7 +615,7 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
     KState* st = &ctx->st[4];
     double* P = ctx->partials;
     const Flat fl = make_flat(n);
-    const AggGeom g = agg_geom(C);
+    const AggGeom g = agg_geom(C, kTargetP), gs = agg_geom(C, kTargetS);
     const uint8_t* mask = C->mask;
     double* rc = C->rc;
     const double* yc = C->yc;
@@ -563,9 +631,9 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
         (void)coarse_solve(ctx, g.S, skip, q);
         launch_smooth<T, 1>(tile, q, NX, NY, g.B, g.mJ, coefT, (const T*)p, mask, yc, (const T*)rt, ph, v,
                             (const KState*)st, P);
-        hipLaunchKernelGGL(k_tl_s<T>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, Gt, phs, mask, (const T*)r, (const T*)v, s,
-                           rc, st, P);
-        (void)coarse_solve(ctx, g.S, skip, q);
+        hipLaunchKernelGGL(k_tl_s<T>, dim3(gs.G), dim3(KM_BLOCK), 0, q, gs, Gt, g.G, phs, mask, (const T*)r,
+                           (const T*)v, s, rc, st, P);
+        (void)coarse_solve(ctx, gs.S, skip, q);
         launch_smooth<T, 2>(tile, q, NX, NY, g.B, g.mJ, coefT, (const T*)s, mask, yc, (const T*)rt, sh, t,
                             (const KState*)st, P);
     };
@@ -593,7 +661,7 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
     KState* st = &ctx->st[4];
     double* P = ctx->partials;
     const Flat fl = make_flat(n);
-    const AggGeom g = agg_geom(C);
+    const AggGeom g = agg_geom(C, kTargetP), gs = agg_geom(C, kTargetS);
     const uint8_t* mask = C->mask;
     int tile = 0;
     if (int rc = tuned_tile<double>(ctx, coef, b, ws, &tile)) return rc;
@@ -620,8 +688,8 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
                                      (const double*)rt, ph, v, (const KState*)st, P);
             break;
         case 6:
-            hipLaunchKernelGGL(k_tl_s<double>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, Gt, phs, mask, (const double*)r,
-                               (const double*)v, sv, C->rc, st, P);
+            hipLaunchKernelGGL(k_tl_s<double>, dim3(gs.G), dim3(KM_BLOCK), 0, q, gs, Gt, g.G, phs, mask,
+                               (const double*)r, (const double*)v, sv, C->rc, st, P);
             break;
         default:
             launch_smooth<double, 2>(tile, q, NX, NY, g.B, g.mJ, coef, (const double*)sv, mask, (const double*)C->yc,
@@ -632,7 +700,8 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
         if (e0) (void)hipEventRecord(e0, q);
         launch(k, phs);
         if (e1) (void)hipEventRecord(e1, q);
-        if (k == 4 || k == 6) (void)coarse_solve(ctx, g.S, nullptr, q);
+        if (k == 4) (void)coarse_solve(ctx, g.S, nullptr, q);
+        if (k == 6) (void)coarse_solve(ctx, gs.S, nullptr, q);
     };
     const int warm = 8;
     for (int it = 0; it < warm; ++it)

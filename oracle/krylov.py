@@ -302,11 +302,11 @@ def fbicgstab(coef, b, eps=1e-8, x0=None, max_iter=None, precond=None, stall_win
 
 def coarse_block(nx, ny, req=0):
     """Aggregate edge length: req, else the smallest of 8, 12, 16, 24, ..., 256
-    giving at most 1024 coarse unknowns."""
+    giving at most 2048 coarse unknowns."""
     if req > 0:
         return req
     for B in (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256):
-        if (-(-nx // B)) * (-(-ny // B)) <= 1024:
+        if (-(-nx // B)) * (-(-ny // B)) <= 2048:
             return B
     return 512
 

@@ -196,8 +196,8 @@ def test_twolevel_coarse_correction_invariants():
     np.testing.assert_allclose(Ac[np.ix_(live, live)], (Pd.T @ A @ Pd)[np.ix_(live, live)], rtol=1e-13, atol=1e-13)
     z1, r1 = krylov.coarse_correct(coef, r, B)
     assert np.abs(P.T @ r1.ravel()).max() < 1e-12 * np.abs(r).max()
-    assert krylov.coarse_block(1280, 256) == 24 and krylov.coarse_block(320, 64) == 8
-    assert krylov.coarse_block(4096, 820) == 64
+    assert krylov.coarse_block(1280, 256) == 16 and krylov.coarse_block(320, 64) == 8
+    assert krylov.coarse_block(4096, 820) == 48
 
 
 def test_twolevel_jacobi_beats_plain():
