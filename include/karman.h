@@ -194,7 +194,10 @@ typedef struct km_case {
 } km_case;
 /* One outer iteration of every case (plain FP64 solves to eps_mom / eps_p,
  * x0 = 0 as in SIMPLESolver.step).  info (may be NULL) receives 3 entries per
- * case (u, v, p'); rel_resid_true is not recomputed (-1). */
+ * case (u, v, p'); rel_resid_true is not recomputed (-1) for the batched
+ * solves.  flags KM_FLAG_COARSE: each case's p' system is solved by the
+ * two-level flexible BiCGSTAB of km_fbicgstab instead (case by case; its
+ * info carries the true residual). */
 int km_batch_step(km_ctx* ctx, int n, const km_case* cases, double eps_mom, double eps_p, int max_iter, int flags,
                   km_solve_info* info);
 /* time_forward of every case (karman_taichi.py:617-622). */

@@ -519,8 +519,9 @@ kb_bicg_a(const BSys* __restrict__ sys, const int* __restrict__ starts, int n, i
     const Tile tl = S.tl;
     const int64_t N = (int64_t)tl.NX * tl.NY;
     double* ws = S.ws;
-    bicg_a_body<double, CPT>(tl, blockIdx.x - starts[c], phase, S.coef, ws, ws + N, ws + 6 * N, ws + 7 * N, S.x,
-                             ws + 2 * N, ws + 4 * N, S.st, S.P);
+    // the XCD-aware tile order of the single-system kernel, within the system
+    bicg_a_body<double, CPT>(tl, xcd_remap(blockIdx.x - starts[c], tl.G), phase, S.coef, ws, ws + N, ws + 6 * N,
+                             ws + 7 * N, S.x, ws + 2 * N, ws + 4 * N, S.st, S.P);
 }
 
 template <int CPT>
@@ -532,8 +533,8 @@ kb_bicg_b(const BSys* __restrict__ sys, const int* __restrict__ starts, int n, i
     const Tile tl = S.tl;
     const int64_t N = (int64_t)tl.NX * tl.NY;
     double* ws = S.ws;
-    bicg_b_body<double, CPT>(tl, blockIdx.x - starts[c], phase, S.coef, ws, ws + N, ws + 4 * N, ws + 6 * N,
-                             ws + 7 * N, S.st, S.P);
+    bicg_b_body<double, CPT>(tl, xcd_remap(blockIdx.x - starts[c], tl.G), phase, S.coef, ws, ws + N, ws + 4 * N,
+                             ws + 6 * N, ws + 7 * N, S.st, S.P);
 }
 
 // agg->done = every system done (one workgroup)
