@@ -229,6 +229,13 @@ int km_twolevel_precond(km_ctx* ctx, int NX, int NY, const double* coef, const d
  * by timing on the first KM_FLAG_COARSE solve of the coarse level's system.
  * No counterpart in the reference. */
 int km_twolevel_tile(km_ctx* ctx, int* ti, int* tj);
+/* How the fused two-level kernels combine the previous kernel's per-workgroup
+ * partial sums: every consumer workgroup sums them itself, or one workgroup
+ * sums them first (an extra small launch) when partials x consumer workgroups
+ * is large (C5-sized grids).  mode -1 automatic (default), 0 never, 1 always;
+ * the totals are the same bits either way.  Needs a coarse level (one
+ * KM_FLAG_COARSE solve first).  No counterpart in the reference. */
+int km_set_coarse_totals(km_ctx* ctx, int mode);
 
 /* ---- measurement hook (no counterpart in the reference) ----
  * Average device time in microseconds of one launch of a solver kernel on an

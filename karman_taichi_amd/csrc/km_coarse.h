@@ -23,6 +23,7 @@ struct km_coarse {
     size_t lds_max = 0;       // LDS per workgroup on this device (the whole-band factorisation)
     bool held = false;        // inv holds A_c^-1 of a coarse level of the current shape
     bool update = true;       // km_set_coarse_update: Newton-Schulz updates of the held inverse
+    int totals = -1;          // km_set_coarse_totals: -1 automatic, 0 never, 1 always
     void* blas = nullptr;     // rocBLAS handle (the update's GEMM)
     // tile shape of the fused smoothing launches per (NX, NY, sizeof T)
     std::map<std::tuple<int, int, int>, int> tiles;

@@ -378,8 +378,10 @@ __device__ __forceinline__ void sum_partials(const double* __restrict__ P, int G
 // NV workgroup sums sharing one pair of barriers: the same totals, bit for
 // bit, for kernels that reduce many planes before short work (it holds 8 NV
 // values per thread in registers, which the stencil kernels cannot spare).
+// Plane q starts at P + q * stride.
 template <int NV>
-__device__ __forceinline__ void sum_partials_wide(const double* __restrict__ P, int G, double (&out)[NV]) {
+__device__ __forceinline__ void sum_partials_wide(const double* __restrict__ P, int G, double (&out)[NV],
+                                                  int stride = KM_MAX_RED) {
     __shared__ double shp[4 * NV];
     double acc[NV][8];
 #pragma unroll
@@ -392,7 +394,7 @@ __device__ __forceinline__ void sum_partials_wide(const double* __restrict__ P, 
         for (int q = 0; q < NV; ++q)
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-                v[q][u] = (k + u * KM_BLOCK < G) ? P[(size_t)q * KM_MAX_RED + k + u * KM_BLOCK] : 0.0;
+                v[q][u] = (k + u * KM_BLOCK < G) ? P[(size_t)q * stride + k + u * KM_BLOCK] : 0.0;
 #pragma unroll
         for (int q = 0; q < NV; ++q)
 #pragma unroll

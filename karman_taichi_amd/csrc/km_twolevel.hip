@@ -26,16 +26,26 @@
 namespace {
 
 constexpr int kMaxAggWg = 8192;            // aggregate-slice workgroups
+constexpr int kMaxTileWg = 8192;           // smoothing workgroups (one tile each up to 8192 tiles)
 constexpr int TL_PR = 0;                   // [kMaxAggWg] <r, r> (aggregate-slice workgroups)
-constexpr int TL_PV = kMaxAggWg;           // [1] <r~, v> (tile workgroups)
-constexpr int TL_PT = kMaxAggWg + KM_MAX_RED;   // [4] <t, s>, <t, t>, <s, r~>, <t, r~>
+constexpr int TL_PV = kMaxAggWg;           // [kMaxTileWg] <r~, v> (tile workgroups)
+constexpr int TL_PT = kMaxAggWg + kMaxTileWg;   // [4][kMaxTileWg] <t, s>, <t, t>, <s, r~>, <t, r~>
+constexpr int TL_TOT = kMaxAggWg + 5 * kMaxTileWg;   // totals: 0 <r~,v>, 1..4 the K_T planes, 5 <r,r>
+// Consumers sum a producer's partials in every workgroup; when partials x
+// consumer workgroups exceeds this, one workgroup sums them first (k_tl_total)
+// and the consumers read the totals.
+constexpr int64_t kTotalWork = 2000000;
+constexpr int FIN_V = 1, FIN_T = 2, FIN_R = 4;
 constexpr int kStallWindow = 300;        // as the unfused flexible solve
 constexpr int kChunk = 8;
 // Tile shapes of the smoothing launches (rows x columns, columns contiguous);
 // the fastest for a system is chosen on its first solve.
-constexpr int kNumTiles = 5;
-constexpr int kTileI[kNumTiles] = {32, 16, 16, 8, 8};
-constexpr int kTileJ[kNumTiles] = {32, 64, 32, 64, 128};
+constexpr int kNumTiles = 6;
+constexpr int kTileI[kNumTiles] = {32, 16, 16, 8, 8, 16};
+constexpr int kTileJ[kNumTiles] = {32, 64, 32, 64, 128, 32};
+// minimum waves per SIMD the compiler must allow (a VGPR budget): 16 x 32 also
+// fits 128 VGPRs without spills, so a fourth wave per SIMD can hide latency
+constexpr int kTileW[kNumTiles] = {1, 1, 1, 1, 1, 4};
 
 __device__ __forceinline__ bool fin(double v) { return v == v && v - v == 0.0; }
 
@@ -164,7 +174,7 @@ k_tl_init(Flat fl, const T* __restrict__ rhs, T* __restrict__ x, T* __restrict__
 // K_P: the x / r update of iteration i-1, the p update of iteration i, P^T p
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restrict__ x, T* __restrict__ r,
+k_tl_p(AggGeom g, int Gt, int tot, int ph, const uint8_t* __restrict__ mask, T* __restrict__ x, T* __restrict__ r,
        T* __restrict__ p, const T* __restrict__ v, const T* __restrict__ s, const T* __restrict__ t,
        const T* __restrict__ ph_, const T* __restrict__ sh_, double* __restrict__ rc, KState* __restrict__ st,
        double* __restrict__ P) {
@@ -200,7 +210,12 @@ k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restric
     double omega = 0.0, beta = 0.0;
     if (i > 0) {
         double pt[4];
-        sum_partials_wide<4>(P + TL_PT, Gt, pt);
+        if (tot & FIN_T) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) pt[q] = P[TL_TOT + 1 + q];
+        } else {
+            sum_partials_wide<4>(P + TL_PT, Gt, pt, kMaxTileWg);
+        }
         omega = pt[1] != 0.0 ? pt[0] / pt[1] : 0.0;                  // bicgsolver.py:118
         const double rho = pt[2] - omega * pt[3];                    // <r_i, r~> (bicgsolver.py:95)
         beta = omega != 0.0 ? (rho / sp.rho) * (sp.alpha / omega) : 0.0;   // bicgsolver.py:102
@@ -253,7 +268,7 @@ k_tl_p(AggGeom g, int Gt, int ph, const uint8_t* __restrict__ mask, T* __restric
 // K_S: stopping test, alpha, s = r - alpha v, P^T s
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_tl_s(AggGeom g, int Gt, int Gp, int ph, const uint8_t* __restrict__ mask, const T* __restrict__ r,
+k_tl_s(AggGeom g, int Gt, int Gp, int tot, int ph, const uint8_t* __restrict__ mask, const T* __restrict__ r,
        const T* __restrict__ v, T* __restrict__ s, double* __restrict__ rc, KState* __restrict__ st,
        double* __restrict__ P) {
     __shared__ double sh[8];
@@ -275,8 +290,10 @@ k_tl_s(AggGeom g, int Gt, int Gp, int ph, const uint8_t* __restrict__ mask, cons
         vv[c] = ok[c] ? v[kk[c]] : T(0);
     }
     double tv[1], tr[1];
-    sum_partials_wide<1>(P + TL_PV, Gt, tv);
-    sum_partials_wide<1>(P + TL_PR, Gp, tr);   // K_P's workgroups
+    if (tot & FIN_V) tv[0] = P[TL_TOT];
+    else sum_partials_wide<1>(P + TL_PV, Gt, tv);
+    if (tot & FIN_R) tr[0] = P[TL_TOT + 5];
+    else sum_partials_wide<1>(P + TL_PR, Gp, tr);   // K_P's workgroups
     const double rv = tv[0], rr = tr[0];
     const KScal sp = st->ring[ph];
     const KScal spp = st->ring[(ph + KM_RING - 1) % KM_RING];
@@ -336,8 +353,8 @@ k_tl_s(AggGeom g, int Gt, int Gp, int ph, const uint8_t* __restrict__ mask, cons
 // halo frame, ping-ponged between two buffers over the sweeps.  The arithmetic is that of one
 // damped Jacobi sweep per cell: z + (omega / aP) (src - A z), neighbours
 // outside the system read as zero, rows with aP = 0 left unchanged.
-template <typename T, int TI, int TJ, int MODE>
-__global__ void __launch_bounds__(KM_BLOCK)
+template <typename T, int TI, int TJ, int MODE, int WPE>
+__global__ void __launch_bounds__(KM_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_tl_smooth(Tile tl, int B, int mJ, const T* __restrict__ c, const T* __restrict__ src,
             const uint8_t* __restrict__ mask, const double* __restrict__ yc, const T* __restrict__ rt,
             T* __restrict__ z, T* __restrict__ y, const KState* __restrict__ st, double* __restrict__ P) {
@@ -465,11 +482,22 @@ k_tl_smooth(Tile tl, int B, int mJ, const T* __restrict__ c, const T* __restrict
         a3 = block_sum(a3, sh);
         if (threadIdx.x == 0) {
             P[TL_PT + L] = a0;
-            P[TL_PT + KM_MAX_RED + L] = a1;
-            P[TL_PT + 2 * KM_MAX_RED + L] = a2;
-            P[TL_PT + 3 * KM_MAX_RED + L] = a3;
+            P[TL_PT + kMaxTileWg + L] = a1;
+            P[TL_PT + 2 * kMaxTileWg + L] = a2;
+            P[TL_PT + 3 * kMaxTileWg + L] = a3;
         }
     }
+}
+
+// make_tile with up to kMaxTileWg workgroups (one tile each when it fits)
+Tile tl_tile(int NX, int NY, int TI, int TJ) {
+    Tile t;
+    t.NX = NX; t.NY = NY; t.TI = TI; t.TJ = TJ;
+    t.tiles_j = (NY + TJ - 1) / TJ;
+    t.T = ((NX + TI - 1) / TI) * t.tiles_j;
+    t.tpb = (t.T + kMaxTileWg - 1) / kMaxTileWg;
+    t.G = (t.T + t.tpb - 1) / t.tpb;
+    return t;
 }
 
 // One smoothing launch with tile shape `which` (an index into kTileI / kTileJ).
@@ -478,8 +506,9 @@ void launch_smooth(int which, hipStream_t q, int NX, int NY, int B, int mJ, cons
                    const uint8_t* mask, const double* yc, const T* rt, T* z, T* y, const KState* st, double* P) {
 #define KM_TL_CASE(IDX)                                                                                           \
     case IDX: {                                                                                                   \
-        const Tile tl = make_tile(NX, NY, kTileI[IDX], kTileJ[IDX]);                                              \
-        hipLaunchKernelGGL((k_tl_smooth<T, kTileI[IDX], kTileJ[IDX], MODE>), dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, \
+        const Tile tl = tl_tile(NX, NY, kTileI[IDX], kTileJ[IDX]);                                                \
+        hipLaunchKernelGGL((k_tl_smooth<T, kTileI[IDX], kTileJ[IDX], MODE, kTileW[IDX]>), dim3(tl.G),              \
+                           dim3(KM_BLOCK), 0, q, tl,                                                                \
                            B, mJ, c, src, mask, yc, rt, z, y, st, P);                                             \
         break;                                                                                                    \
     }
@@ -488,13 +517,33 @@ void launch_smooth(int which, hipStream_t q, int NX, int NY, int B, int mJ, cons
         KM_TL_CASE(1)
         KM_TL_CASE(2)
         KM_TL_CASE(3)
-        default:
         KM_TL_CASE(4)
+        default:
+        KM_TL_CASE(5)
     }
 #undef KM_TL_CASE
 }
 
-int smooth_workgroups(int which, int NX, int NY) { return make_tile(NX, NY, kTileI[which], kTileJ[which]).G; }
+int smooth_workgroups(int which, int NX, int NY) { return tl_tile(NX, NY, kTileI[which], kTileJ[which]).G; }
+
+// The producers' partials summed by one workgroup (when kTotalWork says so)
+template <int NV>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_tl_total(const double* __restrict__ P, int G, int stride, double* __restrict__ out, const KState* __restrict__ st) {
+    if (st->done) return;
+    double t[NV];
+    sum_partials_wide<NV>(P, G, t, stride);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = t[q];
+}
+
+int total_flags(const km_coarse* C, int Gt, int Gp, int Gs) {
+    if (C->totals == 0) return 0;
+    if (C->totals > 0) return FIN_V | FIN_T | FIN_R;
+    return ((int64_t)Gt * Gs > kTotalWork ? FIN_V : 0) | ((int64_t)Gt * Gp > kTotalWork ? FIN_T : 0) |
+           ((int64_t)Gp * Gs > kTotalWork ? FIN_R : 0);
+}
 
 // About `target` cells per workgroup: large aggregates are cut into at most
 // kMaxSlices row slices, small ones grouped kMaxA at most along J; at most
@@ -622,25 +671,35 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
     int tile = 0;
     if (int rc_ = tuned_tile<T>(ctx, coefT, rhs, ws, &tile)) return rc_;
     const int Gt = smooth_workgroups(tile, NX, NY);
+    const int fin = total_flags(C, Gt, g.G, gs.G);
     hipLaunchKernelGGL(k_tl_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, rhs, xd, r, rt, eps, max_iter, st);
     KM_CHECK_LAUNCH(ctx);
     const int* skip = &st->done;
     auto enqueue = [=](hipStream_t q, int phs) {
-        hipLaunchKernelGGL(k_tl_p<T>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, Gt, phs, mask, xd, r, p, (const T*)v,
+        hipLaunchKernelGGL(k_tl_p<T>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, Gt, fin, phs, mask, xd, r, p, (const T*)v,
                            (const T*)s, (const T*)t, (const T*)ph, (const T*)sh, rc, st, P);
+        if (fin & FIN_R)
+            hipLaunchKernelGGL(k_tl_total<1>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PR), g.G,
+                               kMaxAggWg, P + TL_TOT + 5, (const KState*)st);
         (void)coarse_solve(ctx, g.S, skip, q);
         launch_smooth<T, 1>(tile, q, NX, NY, g.B, g.mJ, coefT, (const T*)p, mask, yc, (const T*)rt, ph, v,
                             (const KState*)st, P);
-        hipLaunchKernelGGL(k_tl_s<T>, dim3(gs.G), dim3(KM_BLOCK), 0, q, gs, Gt, g.G, phs, mask, (const T*)r,
+        if (fin & FIN_V)
+            hipLaunchKernelGGL(k_tl_total<1>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PV), Gt,
+                               kMaxTileWg, P + TL_TOT, (const KState*)st);
+        hipLaunchKernelGGL(k_tl_s<T>, dim3(gs.G), dim3(KM_BLOCK), 0, q, gs, Gt, g.G, fin, phs, mask, (const T*)r,
                            (const T*)v, s, rc, st, P);
         (void)coarse_solve(ctx, gs.S, skip, q);
         launch_smooth<T, 2>(tile, q, NX, NY, g.B, g.mJ, coefT, (const T*)s, mask, yc, (const T*)rt, sh, t,
                             (const KState*)st, P);
+        if (fin & FIN_T)
+            hipLaunchKernelGGL(k_tl_total<4>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PT), Gt,
+                               kMaxTileWg, P + TL_TOT + 1, (const KState*)st);
     };
     GraphKey key;
     key.v = {8, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
              (uintptr_t)ws, (uintptr_t)st, (uintptr_t)C, (uintptr_t)C->arena, (uintptr_t)C->m, (uintptr_t)C->B,
-             (uintptr_t)tile};
+             (uintptr_t)tile, (uintptr_t)fin};
     return run_chunks(ctx, st, key, use_graph, kChunk, max_iter, enqueue, out);
 }
 
@@ -666,6 +725,7 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
     int tile = 0;
     if (int rc = tuned_tile<double>(ctx, coef, b, ws, &tile)) return rc;
     const int Gt = smooth_workgroups(tile, NX, NY);
+    const int fin = total_flags(C, Gt, g.G, gs.G);
     std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
     for (auto& e : ev) KM_HIP(ctx, hipEventCreate(&e));
     auto cleanup = [&]() {
@@ -675,11 +735,11 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
     hipLaunchKernelGGL(k_tl_init<double>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, rt, 0.0, 1 << 30,
                        st);
     hipStream_t q = ctx->stream;
-    // the kernel alone; one() adds the coarse solve that follows K_P and K_S
+    // the kernel alone
     auto launch = [&](int k, int phs) {
         switch (k) {
         case 4:
-            hipLaunchKernelGGL(k_tl_p<double>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, Gt, phs, mask, x, r, p,
+            hipLaunchKernelGGL(k_tl_p<double>, dim3(g.G), dim3(KM_BLOCK), 0, q, g, Gt, fin, phs, mask, x, r, p,
                                (const double*)v, (const double*)sv, (const double*)t, (const double*)ph,
                                (const double*)sh, C->rc, st, P);
             break;
@@ -688,7 +748,7 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
                                      (const double*)rt, ph, v, (const KState*)st, P);
             break;
         case 6:
-            hipLaunchKernelGGL(k_tl_s<double>, dim3(gs.G), dim3(KM_BLOCK), 0, q, gs, Gt, g.G, phs, mask,
+            hipLaunchKernelGGL(k_tl_s<double>, dim3(gs.G), dim3(KM_BLOCK), 0, q, gs, Gt, g.G, fin, phs, mask,
                                (const double*)r, (const double*)v, sv, C->rc, st, P);
             break;
         default:
@@ -696,10 +756,20 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
                                      (const double*)rt, sh, t, (const KState*)st, P);
         }
     };
+    // one() adds what follows each kernel in the iteration: the totals and the coarse solves
     auto one = [&](int k, int phs, hipEvent_t e0, hipEvent_t e1) {
         if (e0) (void)hipEventRecord(e0, q);
         launch(k, phs);
         if (e1) (void)hipEventRecord(e1, q);
+        if (k == 4 && (fin & FIN_R))
+            hipLaunchKernelGGL(k_tl_total<1>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PR), g.G,
+                               kMaxAggWg, P + TL_TOT + 5, (const KState*)st);
+        if (k == 5 && (fin & FIN_V))
+            hipLaunchKernelGGL(k_tl_total<1>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PV), Gt,
+                               kMaxTileWg, P + TL_TOT, (const KState*)st);
+        if (k == 7 && (fin & FIN_T))
+            hipLaunchKernelGGL(k_tl_total<4>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PT), Gt,
+                               kMaxTileWg, P + TL_TOT + 1, (const KState*)st);
         if (k == 4) (void)coarse_solve(ctx, g.S, nullptr, q);
         if (k == 6) (void)coarse_solve(ctx, gs.S, nullptr, q);
     };
@@ -752,6 +822,13 @@ template int fused_inner<double>(km_ctx*, int, int, const double*, const double*
 }  // namespace kmcoarse
 
 extern "C" {
+
+int km_set_coarse_totals(km_ctx* ctx, int mode) {
+    if (!ctx) return KM_EINVAL;
+    KM_REQUIRE(ctx, ctx->coarse != nullptr, "no coarse level yet (solve once first)");
+    ctx->coarse->totals = mode < 0 ? -1 : (mode > 0 ? 1 : 0);
+    return KM_OK;
+}
 
 int km_twolevel_tile(km_ctx* ctx, int* ti, int* tj) {
     if (!ctx || !ti || !tj) return KM_EINVAL;

@@ -303,6 +303,29 @@ def test_twolevel_inverse_update(change, dev):
     assert e < (1e-10 if change != "perturbed" else 1e-8)
 
 
+def test_twolevel_totals_bit_identical(dev):
+    """The fused solve with the partials pre-summed by one workgroup (the path
+    large grids take) against every workgroup summing them: the same sums in
+    the same order, so bit-identical solutions and iteration counts."""
+    coef, b = c1_pcor()
+    ctx = _lib.context_for(dev)
+    x0, i0 = coarse_flex(dev, coef, b, None, _lib.KM_FLAG_COARSE)
+    out = {}
+    ctx.call("km_set_coarse_update", 0)   # factor afresh each time: the same A_c^-1 bits
+    try:
+        for mode in (0, 1):
+            ctx.call("km_set_coarse_totals", mode)
+            out[mode] = coarse_flex(dev, coef, b, None, _lib.KM_FLAG_COARSE)
+    finally:
+        ctx.call("km_set_coarse_totals", -1)
+        ctx.call("km_set_coarse_update", 1)
+    (xa, ia), (xb, ib) = out[0], out[1]
+    print(f"totals: direct {ia.iterations} it, pre-summed {ib.iterations} it")
+    assert ia.status == ib.status == _lib.KM_OK
+    assert ia.iterations == ib.iterations
+    np.testing.assert_array_equal(xa, xb)
+
+
 def test_twolevel_jacobi_many_aggregates(dev):
     """The fused flexible solve with 2 x 2 aggregates on a ragged system
     (several aggregates per workgroup in the Krylov update kernels)."""
