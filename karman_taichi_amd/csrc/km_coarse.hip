@@ -411,30 +411,36 @@ k_coarse_restrict(int NX, int NY, int B, int mJ, const uint8_t* __restrict__ mas
 __global__ void __launch_bounds__(KM_BLOCK)
 k_coarse_matvec(int m, int S, const double* __restrict__ inv, const double* __restrict__ rc, double* __restrict__ yc,
                 const int* __restrict__ skip) {
-    extern __shared__ double rs[];
+    extern __shared__ double rs_[];
     if (skip && *skip) return;
-    for (int k = threadIdx.x; k < m; k += KM_BLOCK) {
+    // one slice: read rc directly (L2), no staging; else sum the slices in LDS
+    const double* rs = S == 1 ? rc : rs_;
+    for (int k = threadIdx.x; S > 1 && k < m; k += KM_BLOCK) {
         double v[kmcoarse::kMaxSlices];
 #pragma unroll
         for (int q = 0; q < kmcoarse::kMaxSlices; ++q) v[q] = rc[(int64_t)(q < S ? q : S - 1) * m + k];
         double a = v[0];
 #pragma unroll
         for (int q = 1; q < kmcoarse::kMaxSlices; ++q) a += q < S ? v[q] : 0.0;
-        rs[k] = a;
+        rs_[k] = a;
     }
-    __syncthreads();
+    if (S > 1) __syncthreads();
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (row >= m) return;
     const double* a = inv + (int64_t)row * m;
     double s = 0.0;
     // eight row loads in flight per lane, summed in k order
     for (int k0 = lane; k0 < m; k0 += 8 * 64) {
-        double av[8];
+        double av[8], rv[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) av[u] = k0 + 64 * u < m ? a[k0 + 64 * u] : 0.0;
+        for (int u = 0; u < 8; ++u) {
+            const bool in = k0 + 64 * u < m;
+            av[u] = in ? a[k0 + 64 * u] : 0.0;
+            rv[u] = in ? rs[k0 + 64 * u] : 0.0;
+        }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (k0 + 64 * u < m) s += av[u] * rs[k0 + 64 * u];
+            if (k0 + 64 * u < m) s += av[u] * rv[u];
     }
     s = wave_sum(s);
     if (lane == 0) yc[row] = s;
@@ -620,8 +626,8 @@ int coarse_solve(km_ctx* ctx, int S, const int* skip, hipStream_t s) {
     km_coarse* C = ctx->coarse;
     KM_REQUIRE(ctx, C && C->arena && C->m > 0, "coarse level not set up");
     KM_REQUIRE(ctx, S >= 1 && S <= kMaxSlices, "restriction slices %d", S);
-    hipLaunchKernelGGL(k_coarse_matvec, dim3((C->m + 3) / 4), dim3(KM_BLOCK), (size_t)C->m * sizeof(double), s, C->m,
-                       S, (const double*)C->inv, (const double*)C->rc, C->yc, skip);
+    hipLaunchKernelGGL(k_coarse_matvec, dim3((C->m + 3) / 4), dim3(KM_BLOCK), S > 1 ? (size_t)C->m * sizeof(double) : 0,
+                       s, C->m, S, (const double*)C->inv, (const double*)C->rc, C->yc, skip);
     KM_CHECK_LAUNCH(ctx);
     return KM_OK;
 }
