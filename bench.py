@@ -65,6 +65,8 @@ def parse():
                     help="network parameters for the learned modes")
     ap.add_argument("--coarse-block", type=int, default=0,
                     help="aggregate edge of the two-level coarse level (0: automatic)")
+    ap.add_argument("--totals", type=int, default=-1, choices=[-1, 0, 1],
+                    help="two-level partial sums: -1 automatic, 0 every workgroup sums, 1 one workgroup first")
     ap.add_argument("--compare-reps", type=int, default=5,
                     help="repetitions of the p' solve on the final system, plain BiCGSTAB vs the mode's solver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -132,6 +134,8 @@ def main():
     log(f"bench: case {case.name} mode {args.mode}, warmup {args.warmup}, steps {args.steps}")
     for _ in range(args.warmup):
         one_step()
+    if twolevel and args.totals != -1:   # the coarse level exists after the first solve
+        s.ctx.call("km_set_coarse_totals", int(args.totals))
     torch.cuda.synchronize()
     rec = []
     t0 = time.perf_counter()

@@ -20,6 +20,7 @@ import torch
 
 from karman_taichi_amd import _lib, cases, learned
 from karman_taichi_amd._lib import KmSolveInfo, ptr
+from karman_taichi_amd.simple import SIMPLESolver
 from oracle import fvm, krylov, unet
 from unet_systems import small_system
 
@@ -385,3 +386,34 @@ def test_twolevel_network_converges(dev):
     true = np.linalg.norm(b - krylov.stencil(coef, x)) / np.linalg.norm(b)
     print(f"two-level untrained network: {info.iterations} it, fallback {info.fallback}, true {true:.3e}")
     assert info.status == _lib.KM_OK and true < 1e-8
+
+
+def test_twolevel_c2_steps_match_plain(dev):
+    """At the benchmark's size (1280 x 256 channel): three SIMPLE steps with the
+    two-level p' solve against three with plain BiCGSTAB from the same start.
+    Every two-level p' solve ends below 1e-8 in the FP64 true residual
+    (recomputed on the host with the oracle's stencil for the last system), and
+    the fields agree to the solve tolerance's reach (1e-6 relative)."""
+    c = cases.c2()
+    runs = {}
+    for coarse in (False, True):
+        s = SIMPLESolver.from_case(c, device=dev)
+        s.init()
+        s.p_coarse = coarse
+        infos = [s.step()["p"] for _ in range(3)]
+        runs[coarse] = (s, infos)
+    s2, i2 = runs[True]
+    s1, _ = runs[False]
+    for inf in i2:
+        assert inf.status == _lib.KM_OK and inf.rel_resid_true < 1e-8
+    coef = s2._coef_p.cpu().numpy().reshape(5, *c.shape_p)
+    b = s2.b_p.cpu().numpy()
+    x = s2.pcor.cpu().numpy()
+    true = np.linalg.norm(b - krylov.stencil(coef, x)) / np.linalg.norm(b)
+    print(f"C2 two-level p' iterations {[i.iterations for i in i2]}, host true residual {true:.2e}")
+    assert true < 1e-8
+    for f in ("u", "v", "p"):
+        a, bb = getattr(s1, f).cpu().numpy(), getattr(s2, f).cpu().numpy()
+        e = np.linalg.norm(a - bb) / np.linalg.norm(a)
+        print(f"  {f}: two-level vs plain rel {e:.2e}")
+        assert e < 1e-6
