@@ -35,10 +35,19 @@ HBM_PEAK_GBS = 8000.0
 #   B reads r, v', r~ and 5 coefficient planes, writes s, t
 BYTES_PER_CELL = {"bicg_a": 15 * 8, "bicg_b": 10 * 8,
                   # the fused two-level iteration (FP64 vectors, 1-byte free-cell mask):
-                  #   tl_p reads x, p^, s^, s, t, p, v, mask; writes x, r, p
-                  #   tl_v reads p, 5 coefficient planes, r~, mask; writes p^, v (tl_t: s -> s^, t)
-                  #   tl_s reads r, v, mask; writes s
-                  "tl_p": 10 * 8 + 1, "tl_v": 9 * 8 + 1, "tl_s": 3 * 8 + 1, "tl_t": 9 * 8 + 1}
+                  #   tl_pv reads s, t, p, v, x, p^, s^, r~, 5 coefficient planes, mask; writes x, r, p', p^', v'
+                  #   tl_st reads r, v', r~, 5 coefficient planes, mask; writes s, s^, t
+                  "tl_pv": 18 * 8 + 1, "tl_st": 11 * 8 + 1}
+# the coarse launches of the fused iteration read A_c^-1 (m x m) and a few m-vectors
+COARSE_VECTORS = {"tl_cv": 4, "tl_ct": 6}
+
+
+def alg_bytes(name, cells, m):
+    """Algorithmic bytes of one launch of kernel `name`."""
+    if name in COARSE_VECTORS:
+        return 8 * m * m + 8 * COARSE_VECTORS[name] * m
+    return BYTES_PER_CELL[name] * cells
+
 
 # p' solver of each mode: (solver, FP32 vectors, two-level, network)
 #   plain          BiCGSTAB in FP64 (bicgsolver.py)
@@ -128,8 +137,8 @@ def main():
         if step_count[0] % n_outer == 0:
             s.time_forward()
         if rec is not None:
-            rec.append({"p_ms": (t2 - t1) * 1e3, "p_it": ip.iterations, "u_it": iu.iterations,
-                        "v_it": iv.iterations, "p_true": ip.rel_resid_true, "p_status": ip.status})
+            rec.append({"mom_ms": (t1 - t0) * 1e3, "p_ms": (t2 - t1) * 1e3, "p_it": ip.iterations,
+                        "u_it": iu.iterations, "v_it": iv.iterations, "p_true": ip.rel_resid_true, "p_status": ip.status})
 
     log(f"bench: case {case.name} mode {args.mode}, warmup {args.warmup}, steps {args.steps}")
     for _ in range(args.warmup):
@@ -158,8 +167,8 @@ def main():
     ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
     NX, NY = case.shape_p
     if twolevel and not learned:
-        kinds = (("tl_p", _lib.KM_KERNEL_TL_P), ("tl_v", _lib.KM_KERNEL_TL_V), ("tl_s", _lib.KM_KERNEL_TL_S),
-                 ("tl_t", _lib.KM_KERNEL_TL_T))
+        kinds = (("tl_pv", _lib.KM_KERNEL_TL_PV), ("tl_cv", _lib.KM_KERNEL_TL_CV), ("tl_st", _lib.KM_KERNEL_TL_ST),
+                 ("tl_ct", _lib.KM_KERNEL_TL_CT))
         reps = 100       # well inside the stall window of the flexible iteration
     else:
         kinds = (("bicg_a", _lib.KM_KERNEL_BICG_A), ("bicg_b", _lib.KM_KERNEL_BICG_B))
@@ -185,8 +194,8 @@ def main():
         blk, mc = ctypes.c_int(), ctypes.c_int()
         ctx.call("km_coarse_info", ctypes.byref(blk), ctypes.byref(mc))
         coarse = {"aggregate": blk.value, "unknowns": mc.value}
-    alg_bytes = BYTES_PER_CELL[dom] * cells_p
-    achieved = alg_bytes / (kern[dom] * 1e-6) / 1e9
+    launch_bytes = alg_bytes(dom, cells_p, coarse["unknowns"] if coarse else 0)
+    achieved = launch_bytes / (kern[dom] * 1e-6) / 1e9
     traffic = None
     tjson = args.traffic_json or os.path.join(
         ROOT, "profiles", ("k_tl_traffic_" if twolevel and not learned else "k_bicg_traffic_") + "c2_1280x256.json")
@@ -236,6 +245,8 @@ def main():
                    "p_vectors": "f32" if fp32 else "f64", "smoothing_tile": tile, "coarse": coarse},
         "p_solve_ms_median": round(statistics.median(p_ms), 3),
         "p_iters_median": statistics.median(p_it),
+        "momentum_ms_median": round(statistics.median(r["mom_ms"] for r in rec), 3),
+        "uv_iters_median": [statistics.median(r["u_it"] for r in rec), statistics.median(r["v_it"] for r in rec)],
         "p_rel_resid_true_max": max(r["p_true"] for r in rec),
         "p_status_all_ok": all(r["p_status"] == _lib.KM_OK for r in rec),
         "p_compare": compare,
@@ -245,7 +256,7 @@ def main():
         "kernel_us_dispatch_events": {k: round(v, 3) for k, v in kern_ev.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "alg_bytes_per_launch": alg_bytes,
+                     "alg_bytes_per_launch": launch_bytes,
                      "note": "working set of one p' system is ~34 MB and fits the 256 MB MALL"},
         "cpu_baseline": cpu,
     }

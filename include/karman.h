@@ -246,10 +246,11 @@ int km_set_coarse_totals(km_ctx* ctx, int mode);
 #define KM_KERNEL_BICG_B 1   /* s update + stencil + <t, s>, <t, t>, <s, r~>, <t, r~> */
 #define KM_KERNEL_STENCIL 3  /* plain y = A x */
 /* the fused two-level flexible iteration (KM_FLAG_COARSE, no network) */
-#define KM_KERNEL_TL_P 4     /* x, r, p updates + P^T p + <r, r> */
-#define KM_KERNEL_TL_V 5     /* p^ = M(p): prolongation + 3 Jacobi sweeps, v = A p^, <r~, v> */
-#define KM_KERNEL_TL_S 6     /* stopping test, s update + P^T s */
-#define KM_KERNEL_TL_T 7     /* s^ = M(s), t = A s^, <t, s>, <t, t>, <s, r~>, <t, r~> */
+#define KM_KERNEL_TL_PV 4    /* x, r, p updates; p^ = M(p) (prolongation + 4 Jacobi sweeps); v = A p^;
+                                <r~, v>, <r, r>; P^T v */
+#define KM_KERNEL_TL_CV 5    /* stopping test, alpha; coarse Y(v) = A_c^-1 P^T v, Y(s) */
+#define KM_KERNEL_TL_ST 6    /* s = r - alpha v; s^ = M(s); t = A s^; <t, s>, <t, t>, <s, r~>, <t, r~>; P^T t */
+#define KM_KERNEL_TL_CT 7    /* omega, beta; coarse Y(t), Y(r), Y(p) */
 #define KM_KERNEL_BACK_TO_BACK 0x100   /* or-ed in: time `reps` identical back-to-back launches between two events */
 int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps,
                    double* avg_us);

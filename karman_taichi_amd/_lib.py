@@ -18,7 +18,7 @@ KM_F64, KM_F32 = 0, 1
 KM_FLAG_X0, KM_FLAG_MIXED, KM_FLAG_NO_GRAPH, KM_FLAG_COARSE = 1, 2, 4, 8
 KM_EAST_WALL, KM_EAST_OUTFLOW = 0, 1
 KM_KERNEL_BICG_A, KM_KERNEL_BICG_B, KM_KERNEL_STENCIL = 0, 1, 3
-KM_KERNEL_TL_P, KM_KERNEL_TL_V, KM_KERNEL_TL_S, KM_KERNEL_TL_T = 4, 5, 6, 7
+KM_KERNEL_TL_PV, KM_KERNEL_TL_CV, KM_KERNEL_TL_ST, KM_KERNEL_TL_CT = 4, 5, 6, 7
 KM_KERNEL_BACK_TO_BACK = 0x100
 
 STATUS_NAMES = {KM_OK: "ok", KM_BREAKDOWN: "breakdown", KM_MAXITER: "maxiter",

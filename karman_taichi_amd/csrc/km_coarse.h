@@ -19,6 +19,8 @@ struct km_coarse {
     double* inv = nullptr;    // m x m A_c^-1, row-major
     double* rc = nullptr;     // kMaxSlices x m restricted residual (per-slice partial sums)
     double* yc = nullptr;     // m coarse solution
+    double* fy = nullptr;     // fused iteration: Y(r), Y(p), Y(v), Y(s) (4 m)
+    double* frc = nullptr;    // fused iteration: slot partials of P^T v, then of P^T t (2 kMaxVS m)
     uint8_t* mask = nullptr;  // NX x NY: 1 on the free interior cells (the support of P)
     size_t lds_max = 0;       // LDS per workgroup on this device (the whole-band factorisation)
     bool held = false;        // inv holds A_c^-1 of a coarse level of the current shape
@@ -31,18 +33,23 @@ struct km_coarse {
 
 namespace kmcoarse {
 
-// Damped Jacobi sweeps of the two-level smoother and their damping (four
-// sweeps at 0.8 take the 1280 x 256 channel's p' solve from 99 flexible
-// iterations with three to 74 in the oracle; 0.7, 0.75, 0.85, 0.9 do worse).
-constexpr int kSweeps = 4;
-constexpr double kOmega = 0.8;
+// Red-black SOR sweeps of the two-level smoother (each a red then a black
+// half-sweep) and their relaxation factor: on the 1280 x 256 channel's p'
+// systems two sweeps at 1.3 take the flexible solve from 45 iterations with
+// four damped Jacobi sweeps (omega 0.8, the same halo) to 27..34 in the
+// oracle; 1.15 .. 1.35 do about as well, 1.5 and above worse.
+constexpr int kSweeps = 2;
+constexpr double kOmega = 1.3;
 // The held inverse X of the last coarse operator is updated to
 // X (2 I - A_c X) for the next one when ||I - A_c X||_inf < kNsTol (the
 // update's error is then below kNsTol^2); otherwise A_c is factored anew.
 constexpr double kNsTol = 0.1;
-// Row slices per aggregate in the fused Krylov updates: their partial
-// restrictions rc[s m + k] are summed by the coarse solve.
+// Row slices per aggregate of a restriction: the partial restrictions
+// rc[s m + k] are summed by the coarse solve.
 constexpr int kMaxSlices = 8;
+// Restriction slots per aggregate in the fused iteration: the tiles that
+// overlap one aggregate (32 x 32 tiles on 256 x 256 aggregates: 64).
+constexpr int kMaxVS = 64;
 
 // Aggregate edge length for an nx x ny interior (req > 0: that size; else the
 // smallest of 8, 12, 16, 24, ..., 256 that keeps the coarse level at <= 2048
@@ -68,13 +75,13 @@ template <typename T>
 int apply(km_ctx* ctx, const double* coef, const T* coefT, const T* r, T* z1, T* r1, double* Prr, const int* skip,
           hipStream_t s);
 
-// z = M(r): z1 = P yc from the current coarse solution yc, then kSweeps damped
-// Jacobi sweeps on r - A z (one tiled launch).
+// z = M(r): z1 = P yc from the current coarse solution yc, then kSweeps
+// red-black SOR sweeps on A z = r (one tiled launch).
 template <typename T>
 int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s);
 
 // Inner flexible BiCGSTAB for A d = rhs with the two-level Jacobi
-// preconditioner, six launches per iteration; ws holds 8n values of T.
+// preconditioner, four launches per iteration; ws holds 10n values of T.
 template <typename T>
 int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
                 bool use_graph, KState* out);
@@ -83,7 +90,7 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
 // coarse level's system in FP64 (32 x 32 before the first fused solve).
 int tile_info(km_ctx* ctx, int* ti, int* tj);
 
-// km_time_kernel for the fused kernels (which 4..7: K_P, K_V, K_S, K_T).
+// km_time_kernel for the fused kernels (which 4..7: K_PV, K_CV, K_ST, K_CT).
 int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps, bool b2b,
                 double* avg_us);
 

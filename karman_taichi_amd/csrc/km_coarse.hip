@@ -548,7 +548,8 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
     const int64_t n = (int64_t)NX * NY;
     const size_t mm = (size_t)m * m * 8;
     const size_t need = a256(5 * (size_t)m * 8) + a256((size_t)m * (2 * bw + 1) * 8) + 3 * a256(mm) +
-                        a256((size_t)kMaxSlices * m * 8) + 2 * a256((size_t)m * 8) + a256((size_t)n) + 256;
+                        a256((size_t)kMaxSlices * m * 8) + 2 * a256((size_t)m * 8) + a256((size_t)4 * m * 8) +
+                        a256((size_t)2 * kMaxVS * m * 8) + a256((size_t)n) + 256;
     if (!C->arena || C->bytes < need) {
         KM_HIP(ctx, hipStreamSynchronize(s));
         km_drop_graphs(ctx);
@@ -570,6 +571,8 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
     C->rc = reinterpret_cast<double*>(q); q += a256((size_t)kMaxSlices * m * 8);
     C->yc = reinterpret_cast<double*>(q); q += a256((size_t)m * 8);
     double* rowerr = reinterpret_cast<double*>(q); q += a256((size_t)m * 8);
+    C->fy = reinterpret_cast<double*>(q); q += a256((size_t)4 * m * 8);
+    C->frc = reinterpret_cast<double*>(q); q += a256((size_t)2 * kMaxVS * m * 8);
     int* ok = reinterpret_cast<int*>(q); q += 256;
     C->mask = reinterpret_cast<uint8_t*>(q);
     // the held inverse belongs to a coarse level of the same shape: try the update

@@ -341,17 +341,34 @@ def coarse_correct(coef, r, B):
     return z1, r - stencil(coef, z1)
 
 
-def twolevel_jacobi_precond(coef, B, sweeps=4, omega=0.8):
-    """M(r) = z1 + damped Jacobi sweeps on r - A z: the preconditioner of the
-    flexible solve with the coarse level and no network."""
-    P, Ac = aggregation(coef, B)
-    Aci = np.linalg.inv(Ac)
+def red_cells(shape):
+    """Red cells of the red-black ordering: (i + j) even."""
+    ii, jj = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    return (ii + jj) % 2 == 0
+
+
+def rbsor_sweep(coef, r, z, omega):
+    """One red-black SOR sweep on A z = r: the red cells, then the black cells,
+    each z + omega (r - A z) / aP with the latest values of the other colour;
+    rows with aP = 0 are left unchanged."""
     d = coef[0]
     dinv = np.where(d != 0.0, 1.0 / np.where(d != 0.0, d, 1.0), 0.0)
+    red = red_cells(r.shape)
+    for colour in (red, ~red):
+        z = np.where(colour, z + omega * dinv * (r - stencil(coef, z)), z)
+    return z
+
+
+def twolevel_precond(coef, B, sweeps=2, omega=1.3):
+    """M(r) = z1 + red-black SOR sweeps on A z = r from z1 = P A_c^-1 P^T r:
+    the preconditioner of the flexible solve with the coarse level and no
+    network."""
+    P, Ac = aggregation(coef, B)
+    Aci = np.linalg.inv(Ac)
 
     def M(r):
         z = (P @ (Aci @ (P.T @ r.ravel()))).reshape(r.shape)
         for _ in range(sweeps):
-            z = z + omega * dinv * (r - stencil(coef, z))
+            z = rbsor_sweep(coef, r, z, omega)
         return z
     return M

@@ -243,8 +243,8 @@ def test_coarse_correction_matches_oracle(dev):
                                               ("small", 16, 101, 43), ("small", 5, 101, 43),
                                               ("small", 2, 150, 60)])
 def test_twolevel_precond_matches_oracle(which, block, nx, ny, dev):
-    """M(r) of the fused smoothing kernel (P A_c^-1 P^T r, then four damped
-    Jacobi sweeps on 32 x 32 tiles with a 5-cell halo) against the oracle's
+    """M(r) of the fused smoothing kernel (P A_c^-1 P^T r, then two red-black
+    SOR sweeps on 32 x 32 tiles with a 5-cell halo) against the oracle's
     dense restatement: ragged systems whose tiles and aggregates do not divide
     the grid, aggregates of 2 x 2 (a coarse level of 2000+ unknowns, so one
     workgroup owns several aggregates: 2250 > 2048 workgroups), an aggregate
@@ -265,7 +265,7 @@ def test_twolevel_precond_matches_oracle(which, block, nx, ny, dev):
         ctx.call("km_set_coarse", 0)
     B = krylov.coarse_block(r.shape[0] - 2, r.shape[1] - 2, block)
     assert blk.value == B
-    zo = krylov.twolevel_jacobi_precond(coef, B)(r)
+    zo = krylov.twolevel_precond(coef, B)(r)
     e = rel(z.cpu().numpy(), zo)
     print(f"two-level M(r) {which} B={B} ({m.value} unknowns): rel {e:.2e}")
     assert e < 1e-10
@@ -298,7 +298,7 @@ def test_twolevel_inverse_update(change, dev):
             ctx.call("km_twolevel_precond", r.shape[0], r.shape[1], ptr(ct), ptr(rt_), ptr(z))
     finally:
         ctx.call("km_set_coarse", 0)
-    zo = krylov.twolevel_jacobi_precond(coef2, 16)(r)
+    zo = krylov.twolevel_precond(coef2, 16)(r)
     e = rel(z.cpu().numpy(), zo)
     print(f"inverse update ({change}): M(r) rel {e:.2e}")
     assert e < (1e-10 if change != "perturbed" else 1e-8)
@@ -327,7 +327,7 @@ def test_twolevel_totals_bit_identical(dev):
     np.testing.assert_array_equal(xa, xb)
 
 
-def test_twolevel_jacobi_many_aggregates(dev):
+def test_twolevel_many_aggregates(dev):
     """The fused flexible solve with 2 x 2 aggregates on a ragged system
     (several aggregates per workgroup in the Krylov update kernels)."""
     coef, b = small_system(9, nx=150, ny=61)
@@ -338,13 +338,13 @@ def test_twolevel_jacobi_many_aggregates(dev):
     finally:
         ctx.call("km_set_coarse", 0)
     true = np.linalg.norm(b - krylov.stencil(coef, x)) / np.linalg.norm(b)
-    _, io = krylov.fbicgstab(coef, b, eps=1e-8, precond=krylov.twolevel_jacobi_precond(coef, 2), fallback=False)
+    _, io = krylov.fbicgstab(coef, b, eps=1e-8, precond=krylov.twolevel_precond(coef, 2), fallback=False)
     print(f"two-level 2x2: gpu {info.iterations} it, oracle {io['iterations']} it, true {true:.3e}")
     assert info.status == _lib.KM_OK and true < 1e-8 and info.fallback == 0
     assert abs(info.iterations - io["iterations"]) <= 0.25 * io["iterations"] + 5
 
 
-def test_twolevel_jacobi_block16(dev):
+def test_twolevel_block16(dev):
     """The fused solve at C1 with 16 x 16 aggregates against the oracle."""
     coef, b = c1_pcor()
     ctx = _lib.context_for(dev)
@@ -354,25 +354,25 @@ def test_twolevel_jacobi_block16(dev):
     finally:
         ctx.call("km_set_coarse", 0)
     true = np.linalg.norm(b - krylov.stencil(coef, x)) / np.linalg.norm(b)
-    _, io = krylov.fbicgstab(coef, b, eps=1e-8, precond=krylov.twolevel_jacobi_precond(coef, 16), fallback=False)
+    _, io = krylov.fbicgstab(coef, b, eps=1e-8, precond=krylov.twolevel_precond(coef, 16), fallback=False)
     print(f"two-level 16x16: gpu {info.iterations} it, oracle {io['iterations']} it, true {true:.3e}")
     assert info.status == _lib.KM_OK and true < 1e-8 and info.fallback == 0
     assert abs(info.iterations - io["iterations"]) <= 0.25 * io["iterations"] + 5
 
 
 @pytest.mark.parametrize("mixed", [False, True])
-def test_twolevel_jacobi_matches_oracle(mixed, dev):
-    """Two-level preconditioner without the network: coarse correction + 4 damped
-    Jacobi sweeps; FP64 iteration count within 25% of the oracle's flexible solve
+def test_twolevel_matches_oracle(mixed, dev):
+    """Two-level preconditioner without the network: coarse correction + two
+    red-black SOR sweeps; FP64 iteration count within 25% of the oracle's flexible solve
     with the same preconditioner; FP32 vectors converge too (the preconditioned
     system is well conditioned)."""
     coef, b = c1_pcor()
     flags = _lib.KM_FLAG_COARSE | (_lib.KM_FLAG_MIXED if mixed else 0)
     x, info = coarse_flex(dev, coef, b, None, flags)
     B = krylov.coarse_block(b.shape[0] - 2, b.shape[1] - 2)
-    _, io = krylov.fbicgstab(coef, b, eps=1e-8, precond=krylov.twolevel_jacobi_precond(coef, B), fallback=False)
+    _, io = krylov.fbicgstab(coef, b, eps=1e-8, precond=krylov.twolevel_precond(coef, B), fallback=False)
     true = np.linalg.norm(b - krylov.stencil(coef, x)) / np.linalg.norm(b)
-    print(f"two-level jacobi mixed={mixed}: gpu {info.iterations} it ({info.refinements} sweeps, fallback "
+    print(f"two-level mixed={mixed}: gpu {info.iterations} it ({info.refinements} sweeps, fallback "
           f"{info.fallback}), oracle {io['iterations']} it, true {true:.3e}")
     assert info.status == _lib.KM_OK and true < 1e-8 and info.fallback == 0
     assert rel(x, krylov.direct_solve(coef, b)) < 1e-4

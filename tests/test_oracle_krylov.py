@@ -200,7 +200,45 @@ def test_twolevel_coarse_correction_invariants():
     assert krylov.coarse_block(4096, 820) == 48
 
 
-def test_twolevel_jacobi_beats_plain():
+def test_rbsor_sweep_is_textbook_sor():
+    """One red-black sweep of the oracle equals SOR on the red-black ordered
+    matrix (red cells first): z + (D / omega + L)^-1 (r - A z), L the strictly
+    lower part (a triangular solve), at omega 1 (Gauss-Seidel) and 1.3.  A
+    wrong colour order, a stale neighbour or a missing omega fails it."""
+    import scipy.sparse as sp
+    from unet_systems import small_system
+    coef, r = small_system(4, nx=17, ny=12)
+    rng = np.random.default_rng(7)
+    z = rng.standard_normal(r.shape)
+    order = np.argsort(~krylov.red_cells(r.shape).ravel(), kind="stable")   # red first
+    A = krylov.to_csr(coef)[order][:, order]
+    res = (r - krylov.stencil(coef, z)).ravel()[order]
+    d = A.diagonal()
+    assert not res[d == 0].any()   # rows with aP = 0 (the halo) have no residual: their step is zero
+    for omega in (1.0, 1.3):
+        lower = sp.tril(A, k=-1) + sp.diags(np.where(d != 0, d / omega, 1.0))
+        step = spla.spsolve_triangular(lower.tocsr(), res, lower=True)
+        want = z.ravel().copy()
+        want[order] += step
+        got = krylov.rbsor_sweep(coef, r, z, omega)
+        np.testing.assert_allclose(got.ravel(), want, rtol=1e-12, atol=1e-12)
+
+
+def test_rbsor_sweeps_converge_to_the_solution():
+    """Many red-black SOR sweeps solve a symmetric positive definite system
+    (SOR converges for 0 < omega < 2); rows with aP = 0 keep their start value."""
+    from unet_systems import small_system
+    coef, b = small_system(5, nx=16, ny=10)
+    z = np.zeros_like(b)
+    for _ in range(3000):
+        z = krylov.rbsor_sweep(coef, b, z, 1.3)
+    free = coef[0] != 0
+    want = krylov.direct_solve(coef, b)
+    assert np.abs(z - want)[free].max() < 1e-10 * np.abs(want).max()
+    assert not z[~free].any()
+
+
+def test_twolevel_beats_plain():
     """The two-level preconditioner cuts the iteration count of the C1 p' system."""
     from karman_taichi_amd import cases
     from oracle import fvm
@@ -209,7 +247,7 @@ def test_twolevel_jacobi_beats_plain():
     st = fvm.State(c, *cases.initial_fields(c, m), m)
     fvm.simple_step(st, solver="direct")
     coef, b, _ = fvm.assemble_p(c, st.u, st.v, st.coef_u[0], st.coef_v[0], m)
-    M = krylov.twolevel_jacobi_precond(coef, krylov.coarse_block(c.nx, c.ny))
+    M = krylov.twolevel_precond(coef, krylov.coarse_block(c.nx, c.ny))
     x, info = krylov.fbicgstab(coef, b, eps=1e-8, precond=M, fallback=False)
     _, plain = krylov.bicgstab(coef, b, eps=1e-8)
     assert info["status"] == krylov.OK and info["rel_resid_true"] < 1e-8
