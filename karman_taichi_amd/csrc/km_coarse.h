@@ -81,7 +81,7 @@ template <typename T>
 int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s);
 
 // Inner flexible BiCGSTAB for A d = rhs with the two-level Jacobi
-// preconditioner, four launches per iteration; ws holds 10n values of T.
+// preconditioner, four launches per iteration; ws holds 8n values of T.
 template <typename T>
 int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
                 bool use_graph, KState* out);

@@ -414,3 +414,42 @@ __device__ __forceinline__ void sum_partials_wide(const double* __restrict__ P, 
     for (int q = 0; q < NV; ++q) out[q] = (shp[q * 4] + shp[q * 4 + 1]) + (shp[q * 4 + 2] + shp[q * 4 + 3]);
     __syncthreads();
 }
+
+// sum_partials_wide over planes of their own lengths G[q] at P[q]: each total
+// has the bits sum_partials_wide<1>(P[q], G[q]) gives.
+template <int NV>
+__device__ __forceinline__ void sum_partials_each(const double* const (&P)[NV], const int (&G)[NV], double (&out)[NV]) {
+    __shared__ double shp[4 * NV];
+    int Gm = 0;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) Gm = G[q] > Gm ? G[q] : Gm;
+    double acc[NV][8];
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[q][u] = 0.0;
+    for (int k = threadIdx.x; k < Gm; k += 8 * KM_BLOCK) {
+        double v[NV][8];
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[q][u] = (k + u * KM_BLOCK < G[q]) ? P[q][k + u * KM_BLOCK] : 0.0;
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[q][u] += v[q][u];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double a = ((acc[q][0] + acc[q][1]) + (acc[q][2] + acc[q][3])) +
+                   ((acc[q][4] + acc[q][5]) + (acc[q][6] + acc[q][7]));
+        a = wave_sum(a);
+        if (q == 0) __syncthreads();   // earlier readers of shp are done
+        if (lane == 0) shp[q * 4 + w] = a;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) out[q] = (shp[q * 4] + shp[q * 4 + 1]) + (shp[q * 4 + 2] + shp[q * 4 + 3]);
+    __syncthreads();
+}

@@ -10,19 +10,20 @@
 // and only v = A p^ and t = A s^ are restricted, by the tile launches that
 // compute them.  Four launches per inner iteration i (ring slot ph = i mod
 // KM_RING):
-//   K_PV  2-D tiles with a (2 kSweeps + 1)-cell halo.  x += alpha p^ + omega s^,
-//         r = s - omega t (iteration i-1) and p = r + beta (p - omega v) on the
-//         tile and its halo; p^ = M(p) with z1 from Y(p), v = A p^;
-//         partials <r~,v>, <r,r>; slot partials of P^T v
+//   K_PV  2-D tiles with a (2 kSweeps + 1)-cell halo: p^ = M(p) with z1 from
+//         Y(p), v = A p^; partials <r~,v>; slot partials of P^T v
 //   K_CV  stopping test on <r,r>, alpha = rho / <r~,v>;
 //         Y(v) = A_c^-1 P^T v, Y(s) = Y(r) - alpha Y(v)
 //   K_ST  tiles: s = r - alpha v on the tile and its halo; s^ = M(s) with z1
 //         from Y(s), t = A s^; partials <t,s>, <t,t>, <s,r~>, <t,r~>; P^T t
-//   K_CT  omega, rho, beta of the next iteration; Y(t) = A_c^-1 P^T t,
-//         Y(r) = Y(s) - omega Y(t), Y(p) = Y(r) + beta (Y(p) - omega Y(v))
-// p and v alternate between two buffers by iteration parity (K_PV reads the
-// previous p and v on its halo while other tiles write the new ones).
-//
+//   K_CT  omega, rho, beta of the next iteration.  Coarse workgroups:
+//         Y(t) = A_c^-1 P^T t, Y(r) = Y(s) - omega Y(t),
+//         Y(p) = Y(r) + beta (Y(p) - omega Y(v)).  Flat workgroups (no halo):
+//         x += alpha p^ + omega s^, r = s - omega t, p = r + beta (p - omega v);
+//         partials <r,r>
+// The flat updates run beside the latency-bound coarse rows, and K_PV reads
+// only p on its halo.
+
 // Tiles are aligned with the aggregates: tile (a, b) owns the interior rows
 // [1 + a TI, 1 + (a + 1) TI) and columns [1 + b TJ, ...), clipped, and the
 // boundary row or column next to it at the edges of the system (one cell
@@ -39,15 +40,18 @@
 namespace {
 
 constexpr int kMaxTileWg = 8192;                 // smoothing workgroups (one tile each up to 8192 tiles)
-constexpr int TL_PA = 0;                         // [2][kMaxTileWg] <r~,v>, <r,r> (K_PV)
-constexpr int TL_PT = 2 * kMaxTileWg;            // [4][kMaxTileWg] <t,s>, <t,t>, <s,r~>, <t,r~> (K_ST)
-constexpr int TL_TOT = 6 * kMaxTileWg;           // totals: 0..1 the K_PV planes, 2..5 the K_ST planes
-// Every coarse workgroup sums the tile partials; when partials x coarse
-// workgroups exceeds this, one workgroup sums them first (k_tl_total) and the
-// coarse launches read the totals.
+constexpr int kMaxFlatWg = 8192;                 // flat workgroups of K_CT and the start
+constexpr int TL_PV = 0;                         // [kMaxTileWg] <r~,v> (K_PV)
+constexpr int TL_PR = kMaxTileWg;                // [kMaxFlatWg] <r,r> (K_CT's flat workgroups, the start)
+constexpr int TL_PT = kMaxTileWg + kMaxFlatWg;   // [4][kMaxTileWg] <t,s>, <t,t>, <s,r~>, <t,r~> (K_ST)
+constexpr int TL_TOT = TL_PT + 4 * kMaxTileWg;   // totals: 0 <r~,v>, 1..4 the K_ST planes
+// The coarse (and flat) workgroups sum the tile partials; when partials x
+// consumer workgroups exceeds this, one workgroup sums them first
+// (k_tl_total) and the consumers read the totals.
 constexpr int64_t kTotalWork = 2000000;
 constexpr int FIN_A = 1, FIN_T = 2;
 constexpr int kStallWindow = 300;        // as the unfused flexible solve
+constexpr int kFlatPer = 8;              // cells per thread of a flat workgroup, loads in flight
 constexpr int kChunk = 8;
 // Tile shapes of the smoothing launches (rows x columns, columns contiguous);
 // the fastest for a system is chosen on its first solve.
@@ -83,6 +87,23 @@ ATile a_tile(int NX, int NY, int TI, int TJ) {
     return t;
 }
 
+// Flat workgroups: contiguous chunks of kFlatPer KM_BLOCK cells (more when
+// that would need over kMaxFlatWg workgroups).
+struct FGeom {
+    int64_t n;
+    int chunk, G;
+};
+
+FGeom f_geom(int64_t n) {
+    FGeom f;
+    f.n = n;
+    int64_t chunk = (int64_t)kFlatPer * KM_BLOCK;
+    if ((n + chunk - 1) / chunk > kMaxFlatWg) chunk = ((n + kMaxFlatWg - 1) / kMaxFlatWg + KM_BLOCK - 1) / KM_BLOCK * KM_BLOCK;
+    f.chunk = (int)chunk;
+    f.G = (int)((n + chunk - 1) / chunk);
+    return f;
+}
+
 // Slots per aggregate of a tile shape: the most tiles that overlap one
 // aggregate along I (SI) and along J (SJ).
 int slots_along(int n, int B, int Tt) {
@@ -100,25 +121,20 @@ struct CGeom {
     int B, mJ, m, SJ;
 };
 
-// Launch arguments of the tile kernels.  Frame inputs f0..f3 are read on the
+// Launch arguments of the tile kernels.  Frame inputs f0, f1 are read on the
 // tile and its 2 kSweeps-cell halo:
 //   MODE 0  z = M(f0)
-//   MODE 1  f0 = s, f1 = t, f2 = p (previous), f3 = v (previous); r read at i = 0
-//   MODE 2  f0 = r, f1 = v
+//   MODE 1  f0 = p: z = p^, y = v
+//   MODE 2  f0 = r, f1 = v: src = s, z = s^, y = t
 template <typename T>
 struct TlArgs {
     const T* c;              // 5 coefficient planes
     const uint8_t* mask;     // free cells
     const T* f0;
     const T* f1;
-    const T* f2;
-    const T* f3;
     const T* rt;             // r~ (MODE 1, 2)
-    T* x;                    // MODE 1: x += alpha p^ + omega s^ (i > 0)
-    T* r;                    // MODE 1: r out (i > 0), r_0 in (i = 0)
-    const T* sh;             // MODE 1: the previous s^
-    T* src;                  // MODE 1: p out; MODE 2: s out
-    T* z;                    // M(src) out (MODE 1: also the previous p^ in)
+    T* src;                  // MODE 2: s out
+    T* z;                    // M(.) out
     T* y;                    // A z out (MODE 1, 2)
     const double* yc;        // coarse vector prolongated as z1
     double* rcs;             // slot partials of P^T y
@@ -127,18 +143,25 @@ struct TlArgs {
     int ph;
 };
 
+// x = 0, r = r~ = p = rhs, partials of <r,r> on the flat workgroups, the state
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_tl_init(Flat fl, const T* __restrict__ rhs, T* __restrict__ x, T* __restrict__ r, T* __restrict__ rt, double eps,
-          int max_iter, KState* __restrict__ st) {
+k_tl_init(FGeom fl, const T* __restrict__ rhs, T* __restrict__ x, T* __restrict__ r, T* __restrict__ rt,
+          T* __restrict__ p, double eps, int max_iter, KState* __restrict__ st, double* __restrict__ P) {
+    __shared__ double sh[8];
     const int64_t lo = (int64_t)blockIdx.x * fl.chunk;
     const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
+    double rr = 0.0;
     for (int64_t k = lo + threadIdx.x; k < hi; k += KM_BLOCK) {
         const T b = rhs[k];
         x[k] = T(0);
         r[k] = b;
         rt[k] = b;
+        p[k] = b;
+        rr += (double)b * (double)b;
     }
+    rr = block_sum(rr, sh);
+    if (threadIdx.x == 0) P[TL_PR + blockIdx.x] = rr;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->eps = eps;
         st->max_iter = max_iter;
@@ -191,29 +214,17 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
     constexpr int WZ = TJ + 2 * (NS + 1), HZ = TI + 2 * (NS + 1);   // z frame: halo NS + 1
     constexpr int WR = TJ + 2 * NS, HR = TI + 2 * NS;               // register frame: halo NS
     constexpr int NR = WR * HR;
-    constexpr int Q = (NR + KM_BLOCK - 1) / KM_BLOCK;
+    constexpr int Q = (NR + KM_BLOCK - 1) / KM_BLOCK;   // register slots: frame cells t + q KM_BLOCK
     __shared__ T za[HZ * WZ];
     __shared__ double yb[MODE > 0 ? TI * TJ : 1];   // y on the free interior cells (tile-local)
-    __shared__ double sh[8];
+    __shared__ double sh[16];
     const KState* st = g.st;
     if (st && st->done) return;
     const int L = xcd_remap(blockIdx.x, tl.G);
     const int NX = tl.NX, NY = tl.NY, nx = NX - 2, ny = NY - 2;
     const int64_t n = (int64_t)NX * NY;
     const T om = (T)kmcoarse::kOmega;
-    // scalars of the iteration
-    int it = 0;
-    double alpha = 0.0, omega = 0.0, beta = 0.0;
-    if (MODE == 1 && st) {
-        const KScal sp = st->ring[g.ph];
-        it = sp.iter;
-        omega = sp.omega;
-        beta = sp.beta;
-        alpha = st->ring[(g.ph + KM_RING - 1) % KM_RING].alpha;
-    } else if (MODE == 2 && st) {
-        alpha = st->ring[g.ph].alpha;
-    }
-    const T fo = (T)omega, fb = (T)beta, fa = (T)alpha;
+    const T fa = (T)(MODE == 2 && st ? st->ring[g.ph].alpha : 0.0);
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     const int t0 = L * tl.tpb, t1 = t0 + tl.tpb < tl.T ? t0 + tl.tpb : tl.T;
     for (int tile = t0; tile < t1; ++tile) {
@@ -223,10 +234,11 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
         // column next to it at the edges of the system
         const int olo = ta == 0 ? 0 : ti0, ohi = ta == tl.tiles_i - 1 ? NX : ti0 + TI;
         const int jlo = tb == 0 ? 0 : tj0, jhi = tb == tl.tiles_j - 1 ? NY : tj0 + TJ;
+
         // every load of the tile is issued before the first store (the
         // argument pointers may alias as far as the compiler knows): the
         // register frame, the tile's own cells and the z frame
-        constexpr int NF = MODE == 1 ? 4 : (MODE == 2 ? 2 : 1);   // frame inputs per cell
+        constexpr int NF = MODE == 2 ? 2 : 1;   // frame inputs per cell
         constexpr int QZ = (HZ * WZ + KM_BLOCK - 1) / KM_BLOCK;
         T cr[Q][5], sr[Q], rq[Q], wd[Q];
         T fv[Q][NF];
@@ -252,15 +264,9 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
             }
 #pragma unroll
             for (int e = 0; e < 5; ++e) cr[q][e] = in[q] ? g.c[e * n + k] : T(0);
-            if (MODE == 1 && it == 0) {
-                fv[q][0] = in[q] ? g.r[k] : T(0);   // p_0 = r_0
+            const T* fp[2] = {g.f0, g.f1};
 #pragma unroll
-                for (int e = 1; e < NF; ++e) fv[q][e] = T(0);
-            } else {
-                const T* fp[4] = {g.f0, g.f1, g.f2, g.f3};
-#pragma unroll
-                for (int e = 0; e < NF; ++e) fv[q][e] = in[q] ? fp[e][k] : T(0);
-            }
+            for (int e = 0; e < NF; ++e) fv[q][e] = in[q] ? fp[e][k] : T(0);
             rq[q] = MODE > 0 && ou[q] ? g.rt[k] : T(0);
         }
         // z1 = P yc on the z frame (zero off the free cells and outside the system)
@@ -274,29 +280,13 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
             yq[q] = g.yc[(ic / cg.B) * cg.mJ + jc / cg.B];
             mq[q] = ing ? g.mask[(int64_t)i * NY + j] : 0;
         }
-        // the source of M(.) on the frame; the Krylov updates of the tile's cells
+        // the source of M(.) on the frame (MODE 2: s, written on the tile's cells)
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int64_t k = kq[q];
-            T sv = T(0);
-            if (MODE == 0) {
-                sv = fv[q][0];
-            } else if (MODE == 1) {
-                if (it == 0) {
-                    sv = fv[q][0];
-                    if (ou[q]) a1 += (double)sv * (double)sv;
-                } else if (in[q]) {
-                    const T rn = fv[q][0] - fo * fv[q][1];             // bicgsolver.py:134
-                    sv = rn + fb * (fv[q][2] - fo * fv[q][3]);         // bicgsolver.py:103
-                    if (ou[q]) {
-                        g.r[k] = rn;
-                        a1 += (double)rn * (double)rn;
-                    }
-                }
-                if (ou[q]) g.src[k] = sv;
-            } else {
+            T sv = fv[q][0];
+            if (MODE == 2) {
                 sv = in[q] ? fv[q][0] - fa * fv[q][1] : T(0);        // bicgsolver.py:110
-                if (ou[q]) g.src[k] = sv;
+                if (ou[q]) g.src[kq[q]] = sv;
             }
             sr[q] = sv;
             wd[q] = cr[q][0] != T(0) ? om * (T(1) / cr[q][0]) : T(0);   // omega D^-1
@@ -307,16 +297,6 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
             if (idx < HZ * WZ) za[idx] = mq[q] ? (T)yq[q] : T(0);
         }
         __syncthreads();
-        // MODE 1: x, the previous p^ and s^ on the tile's cells, in flight
-        // during the sweeps (x += alpha p^ + omega s^ before p^ is replaced)
-        T ex[Q][3];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const bool xo = MODE == 1 && it > 0 && ou[q];
-            ex[q][0] = xo ? g.x[kq[q]] : T(0);
-            ex[q][1] = xo ? g.z[kq[q]] : T(0);
-            ex[q][2] = xo ? g.sh[kq[q]] : T(0);
-        }
         // half-sweep hs: red cells at even hs, black at odd, on the halo-h frame
 #pragma unroll
         for (int hs = 0; hs < NS; ++hs) {
@@ -350,8 +330,6 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
             const int pz = (a + 1) * WZ + (b + 1);
             const int64_t k = kq[q];
             const T zc = zf[pz];
-            if (MODE == 1 && it > 0)   // bicgsolver.py:128
-                g.x[k] = ex[q][0] + (T)(alpha * (double)ex[q][1] + omega * (double)ex[q][2]);
             if (MODE == 0) {
                 g.z[k] = zc;
             } else {
@@ -402,26 +380,29 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
                 }
             }
         }
-        __syncthreads();
+        if (tile + 1 < t1) __syncthreads();   // the next tile reuses za
     }
-    if (MODE == 1) {
-        a0 = block_sum(a0, sh);
-        a1 = block_sum(a1, sh);
-        if (threadIdx.x == 0) {
-            g.P[L] = a0;
-            g.P[kMaxTileWg + L] = a1;
+    if (MODE > 0) {
+        // the partial sums, each reduced in the fixed order of block_sum
+        constexpr int NV = MODE == 1 ? 1 : 4;
+        double v[NV];
+        v[0] = a0;
+        if (NV > 1) {
+            v[NV > 1 ? 1 : 0] = a1;
+            v[NV > 2 ? 2 : 0] = a2;
+            v[NV > 3 ? 3 : 0] = a3;
         }
-    } else if (MODE == 2) {
-        a0 = block_sum(a0, sh);
-        a1 = block_sum(a1, sh);
-        a2 = block_sum(a2, sh);
-        a3 = block_sum(a3, sh);
-        if (threadIdx.x == 0) {
-            g.P[L] = a0;
-            g.P[kMaxTileWg + L] = a1;
-            g.P[2 * kMaxTileWg + L] = a2;
-            g.P[3 * kMaxTileWg + L] = a3;
-        }
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
+        if (lane == 0)
+#pragma unroll
+            for (int q = 0; q < NV; ++q) sh[q * 4 + w] = v[q];
+        __syncthreads();
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int q = 0; q < NV; ++q)
+                g.P[q * kMaxTileWg + L] = (sh[q * 4] + sh[q * 4 + 1]) + (sh[q * 4 + 2] + sh[q * 4 + 3]);
     }
 }
 
@@ -462,15 +443,6 @@ int tile_slots(const km_coarse* C, int which) {
     return slots_along(C->NX - 2, C->B, kTileI[which]) * slots_along(C->NY - 2, C->B, kTileJ[which]);
 }
 
-// A state for timing K_PV outside a solve: iteration 1, not done.
-__global__ void k_tl_tune_state(KState* st) {
-    st->done = 0;
-    st->ring[0].iter = 1;
-    st->ring[0].omega = 0.5;
-    st->ring[0].beta = 0.5;
-    st->ring[KM_RING - 1].alpha = 0.5;
-}
-
 // The producers' partials summed by one workgroup (when kTotalWork says so)
 template <int NV>
 __global__ void __launch_bounds__(KM_BLOCK)
@@ -483,134 +455,215 @@ k_tl_total(const double* __restrict__ P, int G, int stride, double* __restrict__
         for (int q = 0; q < NV; ++q) out[q] = t[q];
 }
 
-int total_flags(const km_coarse* C, int Gt, int Gc) {
+// FIN_A: K_PV's partials (read by K_CV's Gc workgroups); FIN_T: K_ST's
+// partials (read by K_CT's Gc + Gf workgroups)
+int total_flags(const km_coarse* C, int Gt, int Gc, int Gf) {
     if (C->totals == 0) return 0;
     if (C->totals > 0) return FIN_A | FIN_T;
-    return (int64_t)Gt * Gc > kTotalWork ? FIN_A | FIN_T : 0;
+    return ((int64_t)Gt * Gc > kTotalWork ? FIN_A : 0) | ((int64_t)Gt * (Gc + Gf) > kTotalWork ? FIN_T : 0);
 }
 
-// K_CV (MODE 1) / K_CT (MODE 2): the scalars from the tile launch's partials
-// (every workgroup sums them in the same order), then one wave per coarse row
-// of y = A_c^-1 rc, rc the slot sum (staged in LDS).  A lane's first
-// kCoarseCh row values, rc and the row's coarse values are loaded before the
-// partials are summed, so one memory round trip serves them all (rows of up to
-// 64 kCoarseCh unknowns).  Y holds Y(r), Y(p), Y(v), Y(s).
-constexpr int kCoarseCh = 32;
+// The coarse rows of K_CV / K_CT: one wave per row of y = A_c^-1 rc, rc the
+// slot sum staged in LDS by load_rc.  A lane's first kCoarseCh row values are
+// loaded (first_row) before the launch's partials are summed, so one memory
+// round trip serves them all (rows of up to 64 kCoarseCh unknowns); the rest
+// of longer rows follow.  Summed in k order per lane.
+constexpr int kCoarseCh = 24;
 
-template <int MODE>
+__device__ __forceinline__ void first_row(const double* __restrict__ a, int m, int lane, double (&av)[kCoarseCh]) {
+#pragma unroll
+    for (int u = 0; u < kCoarseCh; ++u) av[u] = lane + 64 * u < m ? a[lane + 64 * u] : 0.0;
+}
+
+__device__ __forceinline__ void load_rc(const double* __restrict__ rcs, int m, int S, double* rs) {
+    for (int k = threadIdx.x; k < m; k += KM_BLOCK) {
+        double acc = rcs[k];
+        for (int q = 1; q < S; ++q) acc += rcs[(int64_t)q * m + k];
+        rs[k] = acc;
+    }
+}
+
+__device__ __forceinline__ double row_dot(const double* __restrict__ a, int m, int lane, const double (&av)[kCoarseCh],
+                                          const double* rs) {
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < kCoarseCh; ++u)
+        if (lane + 64 * u < m) s += av[u] * rs[lane + 64 * u];
+    for (int k0 = lane + 64 * kCoarseCh; k0 < m; k0 += 64 * kCoarseCh) {
+        double bv[kCoarseCh];
+#pragma unroll
+        for (int u = 0; u < kCoarseCh; ++u) bv[u] = k0 + 64 * u < m ? a[k0 + 64 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kCoarseCh; ++u)
+            if (k0 + 64 * u < m) s += bv[u] * rs[k0 + 64 * u];
+    }
+    return wave_sum(s);
+}
+
+// K_CV: the stopping test on <r,r> and alpha (every workgroup sums the
+// partials in the same order), then Y(v) = A_c^-1 P^T v and Y(s) = Y(r) -
+// alpha Y(v) on coarse row blockIdx.x * 4 + wave.  Y holds Y(r), Y(p), Y(v), Y(s).
 __global__ void __launch_bounds__(KM_BLOCK)
-k_tl_coarse(int m, int S, int Gt, int tot, int ph, const double* __restrict__ inv, const double* __restrict__ rcs,
-            double* __restrict__ Y, KState* __restrict__ st, const double* __restrict__ P) {
+k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict__ inv, const double* __restrict__ rcs,
+        double* __restrict__ Y, KState* __restrict__ st, const double* __restrict__ P) {
     extern __shared__ double rs_[];
     if (st->done) return;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     const bool live = row < m;
     const int rw = live ? row : 0;
     const double* a = inv + (int64_t)rw * m;
-    double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
-    if (lane == 0) {
-        y0 = Y[rw];
-        y1 = Y[m + rw];
-        y2 = Y[2 * m + rw];
-        y3 = Y[3 * m + rw];
-    }
+    const double yr = lane == 0 ? Y[rw] : 0.0;
     double av[kCoarseCh];
-#pragma unroll
-    for (int u = 0; u < kCoarseCh; ++u) av[u] = lane + 64 * u < m ? a[lane + 64 * u] : 0.0;
-    for (int k = threadIdx.x; k < m; k += KM_BLOCK) {
-        double acc = rcs[k];
-        for (int q = 1; q < S; ++q) acc += rcs[(int64_t)q * m + k];
-        rs_[k] = acc;
-    }
+    first_row(a, m, lane, av);
+    load_rc(rcs, m, S, rs_);
     const bool lead0 = blockIdx.x == 0 && threadIdx.x == 0;
-    double c1 = 0.0, c2 = 0.0;   // MODE 1: alpha; MODE 2: omega, beta
-    if (MODE == 1) {
-        double pa[2];
-        if (tot & FIN_A) {
-            pa[0] = P[TL_TOT];
-            pa[1] = P[TL_TOT + 1];
-        } else {
-            sum_partials_wide<2>(P + TL_PA, Gt, pa, kMaxTileWg);
-        }
-        const double rvv = pa[0], rr = pa[1];
-        const KScal sp = st->ring[ph];
-        const KScal spp = st->ring[(ph + KM_RING - 1) % KM_RING];
-        const int i = sp.iter;
-        const double rho = i == 0 ? rr : sp.rho;
-        const double r0 = i == 0 ? rr : st->r0;
-        int status = -1;
-        if (!fin(rr)) status = KM_NONFINITE;
-        else if (i == 0 && rr == 0.0) status = KM_OK;
-        else if (i > 0 && sqrt(rr / r0) < st->eps) status = KM_OK;   // bicgsolver.py:130
-        else if (i > 0 && sp.omega == 0.0) status = KM_BREAKDOWN;
-        else if (i >= st->max_iter) status = KM_MAXITER;
-        else if (!fin(rho)) status = KM_NONFINITE;
-        else if (rho == 0.0) status = KM_BREAKDOWN;
-        else if (!fin(rvv)) status = KM_NONFINITE;
-        else if (rvv == 0.0) status = KM_BREAKDOWN;
-        const bool better = rr < 0.81 * spp.best;
-        if (status < 0 && !better && i - spp.best_it >= kStallWindow) status = KM_STALL;
-        if (lead0 && i == 0) st->r0 = r0;
-        if (status >= 0) {
-            if (lead0) finish_state(st, status, i, rr);
-            return;
-        }
-        c1 = rho / rvv;   // alpha, bicgsolver.py:108
-        if (lead0) {
-            st->ring[ph].alpha = c1;
-            if (i == 0) st->ring[ph].rho = rho;
-            st->ring[ph].best = better ? rr : spp.best;
-            st->ring[ph].best_it = better ? i : spp.best_it;
-            st->rTr = rr;
-            st->iter = i;
-        }
+    double rvv, rr;
+    if (tot & FIN_A) {
+        double t1[1];
+        sum_partials_wide<1>(P + TL_PR, Gf, t1);
+        rvv = P[TL_TOT];
+        rr = t1[0];
     } else {
-        double pt[4];
-        if (tot & FIN_T) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) pt[q] = P[TL_TOT + 2 + q];
-        } else {
-            sum_partials_wide<4>(P + TL_PT, Gt, pt, kMaxTileWg);
-        }
-        const KScal sp = st->ring[ph];
-        const double omega = pt[1] != 0.0 ? pt[0] / pt[1] : 0.0;                  // bicgsolver.py:118
-        const double rho = pt[2] - omega * pt[3];                                 // <r_i+1, r~> (bicgsolver.py:95)
-        const double beta = omega != 0.0 ? (rho / sp.rho) * (sp.alpha / omega) : 0.0;   // bicgsolver.py:102
-        c1 = omega;
-        c2 = beta;
-        if (lead0) {
-            KScal& nxt = st->ring[(ph + 1) % KM_RING];
-            nxt.rho = rho;
-            nxt.omega = omega;
-            nxt.beta = beta;
-            nxt.iter = sp.iter + 1;
-        }
+        const double* planes[2] = {P + TL_PV, P + TL_PR};
+        const int counts[2] = {Gt, Gf};
+        double t2[2];
+        sum_partials_each<2>(planes, counts, t2);
+        rvv = t2[0];
+        rr = t2[1];
     }
-    __syncthreads();
+    const KScal sp = st->ring[ph];
+    const KScal spp = st->ring[(ph + KM_RING - 1) % KM_RING];
+    const int i = sp.iter;
+    const double rho = i == 0 ? rr : sp.rho;
+    const double r0 = i == 0 ? rr : st->r0;
+    int status = -1;
+    if (!fin(rr)) status = KM_NONFINITE;
+    else if (i == 0 && rr == 0.0) status = KM_OK;
+    else if (i > 0 && sqrt(rr / r0) < st->eps) status = KM_OK;   // bicgsolver.py:130
+    else if (i > 0 && sp.omega == 0.0) status = KM_BREAKDOWN;
+    else if (i >= st->max_iter) status = KM_MAXITER;
+    else if (!fin(rho)) status = KM_NONFINITE;
+    else if (rho == 0.0) status = KM_BREAKDOWN;
+    else if (!fin(rvv)) status = KM_NONFINITE;
+    else if (rvv == 0.0) status = KM_BREAKDOWN;
+    const bool better = rr < 0.81 * spp.best;
+    if (status < 0 && !better && i - spp.best_it >= kStallWindow) status = KM_STALL;
+    if (lead0 && i == 0) st->r0 = r0;
+    if (status >= 0) {
+        if (lead0) finish_state(st, status, i, rr);
+        return;
+    }
+    const double alpha = rho / rvv;   // bicgsolver.py:108
+    if (lead0) {
+        st->ring[ph].alpha = alpha;
+        if (i == 0) st->ring[ph].rho = rho;
+        st->ring[ph].best = better ? rr : spp.best;
+        st->ring[ph].best_it = better ? i : spp.best_it;
+        st->rTr = rr;
+        st->iter = i;
+    }
+    __syncthreads();   // rs_
     if (!live) return;
-    // summed in k order per lane
-    double s = 0.0;
-#pragma unroll
-    for (int u = 0; u < kCoarseCh; ++u)
-        if (lane + 64 * u < m) s += av[u] * rs_[lane + 64 * u];
-    for (int k0 = lane + 64 * kCoarseCh; k0 < m; k0 += 64 * kCoarseCh) {
-#pragma unroll
-        for (int u = 0; u < kCoarseCh; ++u) av[u] = k0 + 64 * u < m ? a[k0 + 64 * u] : 0.0;
-#pragma unroll
-        for (int u = 0; u < kCoarseCh; ++u)
-            if (k0 + 64 * u < m) s += av[u] * rs_[k0 + 64 * u];
-    }
-    s = wave_sum(s);
+    const double s = row_dot(a, m, lane, av, rs_);
     if (lane == 0) {
-        if (MODE == 1) {
-            Y[2 * m + row] = s;                  // Y(v)
-            Y[3 * m + row] = y0 - c1 * s;        // Y(s)
-        } else {
-            const double yr = y3 - c1 * s;       // Y(r)
+        Y[2 * m + row] = s;                  // Y(v)
+        Y[3 * m + row] = yr - alpha * s;     // Y(s)
+    }
+}
+
+// K_CT: omega, rho, beta of the next iteration from K_ST's partials (every
+// workgroup), then workgroups [0, Gc) take coarse rows (Y(t) = A_c^-1 P^T t,
+// Y(r) = Y(s) - omega Y(t), Y(p) = Y(r) + beta (Y(p) - omega Y(v))) and the
+// rest flat chunks of the fine vectors: x += alpha p^ + omega s^ (bicgsolver.py:128),
+// r = s - omega t (:134), p = r + beta (p - omega v) (:103) of the next
+// iteration, partials of <r,r>.
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_tl_ct(int m, int S, int Gt, int Gc, FGeom fl, int tot, int ph, const double* __restrict__ inv,
+        const double* __restrict__ rcs, double* __restrict__ Y, T* __restrict__ x, T* __restrict__ r,
+        T* __restrict__ p, const T* __restrict__ v, const T* __restrict__ s, const T* __restrict__ t,
+        const T* __restrict__ ph_, const T* __restrict__ sh_, KState* __restrict__ st, double* __restrict__ P) {
+    extern __shared__ double rs_[];
+    __shared__ double shr[8];
+    if (st->done) return;
+    const bool coarse = (int)blockIdx.x < Gc;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const bool live = coarse && row < m;
+    const int rw = live ? row : 0;
+    const double* a = inv + (int64_t)rw * m;
+    double y1 = 0.0, y2 = 0.0, y3 = 0.0;
+    double av[kCoarseCh];
+    if (coarse) {
+        if (lane == 0) {
+            y1 = Y[m + rw];
+            y2 = Y[2 * m + rw];
+            y3 = Y[3 * m + rw];
+        }
+        first_row(a, m, lane, av);
+        load_rc(rcs, m, S, rs_);
+    }
+    double pt[4];
+    if (tot & FIN_T) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pt[q] = P[TL_TOT + 1 + q];
+    } else {
+        sum_partials_wide<4>(P + TL_PT, Gt, pt, kMaxTileWg);
+    }
+    const KScal sp = st->ring[ph];
+    const double omega = pt[1] != 0.0 ? pt[0] / pt[1] : 0.0;                  // bicgsolver.py:118
+    const double rho = pt[2] - omega * pt[3];                                 // <r_i+1, r~> (bicgsolver.py:95)
+    const double beta = omega != 0.0 ? (rho / sp.rho) * (sp.alpha / omega) : 0.0;   // bicgsolver.py:102
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        KScal& nxt = st->ring[(ph + 1) % KM_RING];
+        nxt.rho = rho;
+        nxt.omega = omega;
+        nxt.beta = beta;
+        nxt.iter = sp.iter + 1;
+    }
+    if (coarse) {
+        __syncthreads();   // rs_
+        if (!live) return;
+        const double yt = row_dot(a, m, lane, av, rs_);
+        if (lane == 0) {
+            const double yr = y3 - omega * yt;   // Y(r)
             Y[row] = yr;
-            Y[m + row] = yr + c2 * (y1 - c1 * y2);   // Y(p)
+            Y[m + row] = yr + beta * (y1 - omega * y2);   // Y(p)
+        }
+        return;
+    }
+    const double alpha = sp.alpha;
+    const T fo = (T)omega, fb = (T)beta;
+    const int wf = blockIdx.x - Gc;
+    const int64_t lo = (int64_t)wf * fl.chunk;
+    const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
+    double rr = 0.0;
+    for (int64_t base = lo; base < hi; base += (int64_t)kFlatPer * KM_BLOCK) {
+        T sv[kFlatPer], tv[kFlatPer], pv[kFlatPer], vv[kFlatPer], xv[kFlatPer], hv[kFlatPer], s2v[kFlatPer];
+#pragma unroll
+        for (int c = 0; c < kFlatPer; ++c) {
+            const int64_t k = base + threadIdx.x + c * KM_BLOCK;
+            const bool ok = k < hi;
+            sv[c] = ok ? s[k] : T(0);
+            tv[c] = ok ? t[k] : T(0);
+            pv[c] = ok ? p[k] : T(0);
+            vv[c] = ok ? v[k] : T(0);
+            xv[c] = ok ? x[k] : T(0);
+            hv[c] = ok ? ph_[k] : T(0);
+            s2v[c] = ok ? sh_[k] : T(0);
+        }
+#pragma unroll
+        for (int c = 0; c < kFlatPer; ++c) {
+            const int64_t k = base + threadIdx.x + c * KM_BLOCK;
+            if (k >= hi) continue;
+            x[k] = xv[c] + (T)(alpha * (double)hv[c] + omega * (double)s2v[c]);
+            const T rn = sv[c] - fo * tv[c];
+            r[k] = rn;
+            p[k] = rn + fb * (pv[c] - fo * vv[c]);
+            rr += (double)rn * (double)rn;
         }
     }
+    rr = block_sum(rr, shr);
+    if (threadIdx.x == 0) P[TL_PR + wf] = rr;
 }
 
 }  // namespace
@@ -620,8 +673,8 @@ namespace kmcoarse {
 // Tile shape of the smoothing launches for the coarse level's system in T:
 // among the shapes with the fewest restriction slots, on the first call for a
 // (shape, precision) each runs 3 + 16 back-to-back K_PV + K_ST pairs on
-// scratch vectors (ws: 10n values of T, overwritten; src is read) with a
-// scratch state, and the fastest is kept.
+// scratch vectors (ws: its first 5n values of T are overwritten; src is read)
+// and the fastest is kept.
 template <typename T>
 int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
     km_coarse* C = ctx->coarse;
@@ -635,9 +688,6 @@ int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
     for (int w = 0; w < kNumTiles; ++w) fewest = std::min(fewest, tile_slots(C, w));
     KM_REQUIRE(ctx, fewest <= kMaxVS, "aggregates of %d cells overlap %d tiles", C->B, fewest);
     const int64_t n = (int64_t)C->NX * C->NY;
-    KState* tst = &ctx->st[5];
-    hipLaunchKernelGGL(k_tl_tune_state, dim3(1), dim3(1), 0, ctx->stream, tst);
-    KM_CHECK_LAUNCH(ctx);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     KM_HIP(ctx, hipEventCreate(&e0));
     KM_HIP(ctx, hipEventCreate(&e1));
@@ -645,31 +695,22 @@ int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
     float best_ms = 0.f;
     for (int w = 0; w < kNumTiles; ++w) {
         if (tile_slots(C, w) != fewest) continue;
-        TlArgs<T> a{};
+        TlArgs<T> a{};   // K_PV: p^ = M(src), v
         a.c = coefT;
         a.mask = C->mask;
-        a.f0 = ws;
-        a.f1 = ws + n;
-        a.f2 = ws + 2 * n;
-        a.f3 = ws + 3 * n;
+        a.f0 = src;
         a.rt = src;
-        a.x = ws + 4 * n;
-        a.r = ws + 5 * n;
-        a.sh = ws + 6 * n;
-        a.src = ws + 7 * n;
-        a.z = ws + 8 * n;
-        a.y = ws + 9 * n;
+        a.z = ws;
+        a.y = ws + n;
         a.yc = C->yc;
         a.rcs = C->frc;
-        a.st = tst;
-        a.P = ctx->partials + TL_PA;
-        a.ph = 0;
-        TlArgs<T> b = a;   // K_ST: s = r - alpha v, s^, t
-        b.f0 = ws + 5 * n;
-        b.f1 = ws + 9 * n;
-        b.src = ws;
-        b.z = ws + 6 * n;
-        b.y = ws + n;
+        a.P = ctx->partials + TL_PV;
+        TlArgs<T> b = a;   // K_ST: s = r - alpha v (alpha 0 without a state), s^, t
+        b.f0 = src;
+        b.f1 = ws + n;
+        b.src = ws + 2 * n;
+        b.z = ws + 3 * n;
+        b.y = ws + 4 * n;
         b.rcs = C->frc + (size_t)kMaxVS * C->m;
         b.P = ctx->partials + TL_PT;
         const CGeom cg = c_geom(C, w);
@@ -721,7 +762,6 @@ int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s) {
     a.f0 = r;
     a.z = z;
     a.yc = C->yc;
-    a.st = nullptr;
     launch_tile<T, 0>(0, s, C->NX, C->NY, c_geom(C, 0), a);
     KM_CHECK_LAUNCH(ctx);
     return KM_OK;
@@ -729,12 +769,13 @@ int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s) {
 
 namespace {
 
-// The vectors and launch arguments of one fused solve (ws: 10 n values of T).
+// The vectors and launch arguments of one fused solve (ws: 8 n values of T).
 template <typename T>
 struct Fused {
     int NX, NY, tile, Gt, Gc, S, fin;
+    FGeom fl;
     CGeom cg;
-    T *r, *rt, *p[2], *v[2], *s, *t, *ph, *sh;
+    T *r, *rt, *p, *v, *s, *t, *ph, *sh;
     T* x;
     const T* coefT;
     double *Y, *rcv, *rct;
@@ -747,44 +788,35 @@ struct Fused {
         const int64_t n = (int64_t)NX * NY;
         r = ws;
         rt = ws + n;
-        p[0] = ws + 2 * n;
-        p[1] = ws + 3 * n;
-        v[0] = ws + 4 * n;
-        v[1] = ws + 5 * n;
-        s = ws + 6 * n;
-        t = ws + 7 * n;
-        ph = ws + 8 * n;
-        sh = ws + 9 * n;
+        p = ws + 2 * n;
+        v = ws + 3 * n;
+        s = ws + 4 * n;
+        t = ws + 5 * n;
+        ph = ws + 6 * n;
+        sh = ws + 7 * n;
+        fl = f_geom(n);
         Gt = tile_workgroups(tile, NX, NY);
         Gc = (C->m + 3) / 4;
         S = tile_slots(C, tile);
         cg = c_geom(C, tile);
-        fin = total_flags(C, Gt, Gc);
+        fin = total_flags(C, Gt, Gc, fl.G);
         Y = C->fy;
         rcv = C->frc;
         rct = C->frc + (size_t)kMaxVS * C->m;
     }
 
     void pv(hipStream_t q, int phs) const {
-        const int par = phs & 1;
         TlArgs<T> a{};
         a.c = coefT;
         a.mask = C->mask;
-        a.f0 = s;
-        a.f1 = t;
-        a.f2 = p[par ^ 1];
-        a.f3 = v[par ^ 1];
+        a.f0 = p;
         a.rt = rt;
-        a.x = x;
-        a.r = r;
-        a.sh = sh;
-        a.src = p[par];
         a.z = ph;
-        a.y = v[par];
+        a.y = v;
         a.yc = Y + C->m;   // Y(p)
         a.rcs = rcv;
         a.st = st;
-        a.P = P + TL_PA;
+        a.P = P + TL_PV;
         a.ph = phs;
         launch_tile<T, 1>(tile, q, NX, NY, cg, a);
     }
@@ -793,7 +825,7 @@ struct Fused {
         a.c = coefT;
         a.mask = C->mask;
         a.f0 = r;
-        a.f1 = v[phs & 1];
+        a.f1 = v;
         a.rt = rt;
         a.src = s;
         a.z = sh;
@@ -807,39 +839,40 @@ struct Fused {
     }
     void total_a(hipStream_t q) const {
         if (fin & FIN_A)
-            hipLaunchKernelGGL(k_tl_total<2>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PA), Gt,
+            hipLaunchKernelGGL(k_tl_total<1>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PV), Gt,
                                kMaxTileWg, P + TL_TOT, (const KState*)st);
     }
     void total_t(hipStream_t q) const {
         if (fin & FIN_T)
             hipLaunchKernelGGL(k_tl_total<4>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PT), Gt,
-                               kMaxTileWg, P + TL_TOT + 2, (const KState*)st);
+                               kMaxTileWg, P + TL_TOT + 1, (const KState*)st);
     }
-    template <int MODE>
-    void coarse(hipStream_t q, int phs) const {
-        hipLaunchKernelGGL(k_tl_coarse<MODE>, dim3(Gc), dim3(KM_BLOCK), (size_t)C->m * sizeof(double), q,
-                           C->m, S, Gt, fin, phs, (const double*)C->inv, (const double*)(MODE == 1 ? rcv : rct), Y,
-                           st, (const double*)P);
+    void cv(hipStream_t q, int phs) const {
+        hipLaunchKernelGGL(k_tl_cv, dim3(Gc), dim3(KM_BLOCK), (size_t)C->m * sizeof(double), q, C->m, S, Gt, fl.G,
+                           fin, phs, (const double*)C->inv, (const double*)rcv, Y, st, (const double*)P);
+    }
+    void ct(hipStream_t q, int phs) const {
+        hipLaunchKernelGGL(k_tl_ct<T>, dim3(Gc + fl.G), dim3(KM_BLOCK), (size_t)C->m * sizeof(double), q, C->m, S, Gt,
+                           Gc, fl, fin, phs, (const double*)C->inv, (const double*)rct, Y, x, r, p, (const T*)v,
+                           (const T*)s, (const T*)t, (const T*)ph, (const T*)sh, st, P);
     }
     // one inner iteration
     void iteration(hipStream_t q, int phs) const {
         pv(q, phs);
         total_a(q);
-        coarse<1>(q, phs);
+        cv(q, phs);
         st_(q, phs);
         total_t(q);
-        coarse<2>(q, phs);
+        ct(q, phs);
     }
 };
 
-// x = 0, r = r~ = rhs, the state, Y(r) = Y(p) = A_c^-1 P^T rhs, the slots cleared
+// x = 0, r = r~ = p = rhs, the state, Y(r) = Y(p) = A_c^-1 P^T rhs, the slots cleared
 template <typename T>
 int fused_start(km_ctx* ctx, const Fused<T>& f, const T* rhs, double eps, int max_iter) {
     km_coarse* C = f.C;
-    const int64_t n = (int64_t)f.NX * f.NY;
-    const Flat fl = make_flat(n);
-    hipLaunchKernelGGL(k_tl_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, rhs, f.x, f.r, f.rt, eps,
-                       max_iter, f.st);
+    hipLaunchKernelGGL(k_tl_init<T>, dim3(f.fl.G), dim3(KM_BLOCK), 0, ctx->stream, f.fl, rhs, f.x, f.r, f.rt, f.p, eps,
+                       max_iter, f.st, f.P);
     KM_CHECK_LAUNCH(ctx);
     if (int rc = restrict_solve<T>(ctx, rhs, nullptr, ctx->stream)) return rc;
     const int64_t cells = std::max<int64_t>(C->m, (int64_t)f.S * C->m);
@@ -862,7 +895,7 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
     if (int rc = fused_start<T>(ctx, f, rhs, eps, max_iter)) return rc;
     auto enqueue = [&f](hipStream_t q, int phs) { f.iteration(q, phs); };
     GraphKey key;
-    key.v = {9, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
+    key.v = {10, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
              (uintptr_t)ws, (uintptr_t)f.st, (uintptr_t)C, (uintptr_t)C->arena, (uintptr_t)C->m, (uintptr_t)C->B,
              (uintptr_t)tile, (uintptr_t)f.fin};
     return run_chunks(ctx, f.st, key, use_graph, kChunk, max_iter, enqueue, out);
@@ -877,9 +910,9 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
     KM_REQUIRE(ctx, which >= 4 && which <= 7, "unknown kernel id %d", which);
     if (int rc = setup(ctx, NX, NY, coef, ctx->stream)) return rc;
     const int64_t n = (int64_t)NX * NY;
-    if (int rc = km_ensure_workspace(ctx, (size_t)11 * n * sizeof(double))) return rc;
+    if (int rc = km_ensure_workspace(ctx, (size_t)9 * n * sizeof(double))) return rc;
     double* ws = static_cast<double*>(km_ws_vectors(ctx));
-    double* x = ws + 10 * n;
+    double* x = ws + 8 * n;
     int tile = 0;
     if (int rc = tuned_tile<double>(ctx, coef, b, ws, &tile)) return rc;
     const Fused<double> f(ctx, NX, NY, coef, ws, x, tile);
@@ -894,9 +927,9 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
     auto launch = [&](int k, int phs) {
         switch (k) {
         case 4: f.pv(q, phs); break;
-        case 5: f.coarse<1>(q, phs); break;
+        case 5: f.cv(q, phs); break;
         case 6: f.st_(q, phs); break;
-        default: f.coarse<2>(q, phs);
+        default: f.ct(q, phs);
         }
     };
     // one() adds the totals that follow the tile launches
@@ -911,9 +944,8 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
     for (int it = 0; it < warm; ++it)
         for (int k = 4; k <= 7; ++k) one(k, it % KM_RING, nullptr, nullptr);
     if (b2b) {
-        // K_PV re-runs the scalars and updates of the next phase, the others
-        // read what the last warm iteration left
-        const int phs = which == 4 ? warm % KM_RING : (warm - 1) % KM_RING;
+        // each kernel re-run with the phase of the last warm iteration
+        const int phs = (warm - 1) % KM_RING;
         KM_HIP(ctx, hipEventRecord(ev[0], q));
         for (int k = 0; k < reps; ++k) launch(which, phs);
         KM_HIP(ctx, hipEventRecord(ev[1], q));
