@@ -76,6 +76,11 @@ def parse():
                     help="aggregate edge of the two-level coarse level (0: automatic)")
     ap.add_argument("--totals", type=int, default=-1, choices=[-1, 0, 1],
                     help="two-level partial sums: -1 automatic, 0 every workgroup sums, 1 one workgroup first")
+    ap.add_argument("--coarse-update", type=int, default=8,
+                    help="two-level: update the coarse inverse every k-th p' solve (1 each, 0 factor each; default 8)")
+    ap.add_argument("--chunks", default="",
+                    help="two-level launch schedule big,small,lead (iterations per graph far from / near the expected "
+                         "end of a solve, and where near begins); default 8,2,2")
     ap.add_argument("--compare-reps", type=int, default=5,
                     help="repetitions of the p' solve on the final system, plain BiCGSTAB vs the mode's solver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -112,6 +117,7 @@ def main():
     s.mixed = fp32
     s.p_coarse = twolevel
     s.ctx.call("km_set_coarse", int(args.coarse_block))
+    s.ctx.call("km_set_coarse_update", int(args.coarse_update))
     if p_solver == "cg":
         s.p_solver = "cg"
     if learned:
@@ -145,6 +151,8 @@ def main():
         one_step()
     if twolevel and args.totals != -1:   # the coarse level exists after the first solve
         s.ctx.call("km_set_coarse_totals", int(args.totals))
+    if twolevel and args.chunks:
+        s.ctx.call("km_set_coarse_chunks", *[int(v) for v in args.chunks.split(",")])
     torch.cuda.synchronize()
     rec = []
     t0 = time.perf_counter()
@@ -245,6 +253,8 @@ def main():
                    "p_vectors": "f32" if fp32 else "f64", "smoothing_tile": tile, "coarse": coarse},
         "p_solve_ms_median": round(statistics.median(p_ms), 3),
         "p_iters_median": statistics.median(p_it),
+        "p_iters": p_it,
+        "p_ms": [round(v, 3) for v in p_ms],
         "momentum_ms_median": round(statistics.median(r["mom_ms"] for r in rec), 3),
         "uv_iters_median": [statistics.median(r["u_it"] for r in rec), statistics.median(r["v_it"] for r in rec)],
         "p_rel_resid_true_max": max(r["p_true"] for r in rec),

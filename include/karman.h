@@ -212,11 +212,17 @@ int km_set_coarse(km_ctx* ctx, int block);
 /* z1 = P A_c^-1 P^T r and r1 = r - A z1 for one FP64 system (sets the coarse
  * level up first); the training data of the two-level network and tests use it. */
 int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const double* r, double* z1, double* r1);
-/* on != 0 (the default): a coarse level of the same shape as the last one
- * updates the held A_c^-1 by one Newton-Schulz step X (2 I - A_c X) when
- * ||I - A_c X||_inf < 0.1, instead of factoring A_c again; 0: always factor.
- * No counterpart in the reference. */
-int km_set_coarse_update(km_ctx* ctx, int on);
+/* How the coarse level follows a changing system.  every = 1: a coarse level
+ * of the same shape as the last one updates the held A_c^-1 by one
+ * Newton-Schulz step X (2 I - A_c X) when ||I - A_c X||_inf < 0.5, instead of
+ * factoring A_c again; 0: always factor; k > 1 (default 8): do that on every
+ * k-th set-up and, on the set-ups between, reuse the held level (free-cell
+ * mask, A_c^-1) unchanged when the coefficient array is the same one as last
+ * time (a SIMPLE solver's p' system across outer iterations).  The flexible
+ * solve stays exact with a stale coarse inverse; on the 1280 x 256 channel
+ * its iteration count does not change.  A caller that rewrites obstacle cells
+ * in place should use every <= 1.  No counterpart in the reference. */
+int km_set_coarse_update(km_ctx* ctx, int every);
 /* Aggregate size and coarse unknowns of the last coarse level. */
 int km_coarse_info(km_ctx* ctx, int* block, int* unknowns);
 /* z = M(r) for one FP64 system: the two-level Jacobi preconditioner of the
@@ -236,6 +242,13 @@ int km_twolevel_tile(km_ctx* ctx, int* ti, int* tj);
  * the totals are the same bits either way.  Needs a coarse level (one
  * KM_FLAG_COARSE solve first).  No counterpart in the reference. */
 int km_set_coarse_totals(km_ctx* ctx, int mode);
+/* Launch schedule of the fused two-level iteration: captured graphs of `big`
+ * iterations (a multiple of 4; 0: 8) until `lead` iterations before the
+ * shortest of the last eight solves of this system shape, then graphs of
+ * `small` (1, 2 or 4; 0: 2), so that few launches are left in flight when
+ * the stopping test ends the solve.  Defaults 8, 2, 2.  Needs a coarse level.
+ * No counterpart in the reference. */
+int km_set_coarse_chunks(km_ctx* ctx, int big, int small, int lead);
 
 /* ---- measurement hook (no counterpart in the reference) ----
  * Average device time in microseconds of one launch of a solver kernel on an

@@ -99,6 +99,7 @@ SIGNATURES = {
     "km_twolevel_precond": [_CTX, _I, _I, _P, _P, _P],
     "km_twolevel_tile": [_CTX, ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "km_set_coarse_totals": [_CTX, _I],
+    "km_set_coarse_chunks": [_CTX, _I, _I, _I],
 }
 _RESTYPE = {"km_last_error": ctypes.c_char_p}
 

@@ -24,8 +24,13 @@ struct km_coarse {
     uint8_t* mask = nullptr;  // NX x NY: 1 on the free interior cells (the support of P)
     size_t lds_max = 0;       // LDS per workgroup on this device (the whole-band factorisation)
     bool held = false;        // inv holds A_c^-1 of a coarse level of the current shape
-    bool update = true;       // km_set_coarse_update: Newton-Schulz updates of the held inverse
+    int update = 8;           // km_set_coarse_update: 0 factor, 1 Newton-Schulz update each solve, k every k-th
+    int since = 0;            // solves that reused the held inverse since it was last updated
+    const double* coef = nullptr;   // the coefficient array of the last set-up (reuse needs the same one)
+    bool refresh = false;     // a solve ran long: factor on the next set-up
     int totals = -1;          // km_set_coarse_totals: -1 automatic, 0 never, 1 always
+    int recent[8] = {};       // iteration counts of the last fused solves of this shape (0: none)
+    int chunk_big = 8, chunk_small = 2, chunk_lead = 2;   // km_set_coarse_chunks
     void* blas = nullptr;     // rocBLAS handle (the update's GEMM)
     // tile shape of the fused smoothing launches per (NX, NY, sizeof T)
     std::map<std::tuple<int, int, int>, int> tiles;
@@ -41,9 +46,10 @@ namespace kmcoarse {
 constexpr int kSweeps = 2;
 constexpr double kOmega = 1.3;
 // The held inverse X of the last coarse operator is updated to
-// X (2 I - A_c X) for the next one when ||I - A_c X||_inf < kNsTol (the
-// update's error is then below kNsTol^2); otherwise A_c is factored anew.
-constexpr double kNsTol = 0.1;
+// X (2 I - A_c X) for the next one when ||I - A_c X||_inf < kNsTol (then
+// I - A_c X' = (I - A_c X)^2 and the error is below kNsTol^2); otherwise A_c
+// is factored anew.
+constexpr double kNsTol = 0.5;
 // Row slices per aggregate of a restriction: the partial restrictions
 // rc[s m + k] are summed by the coarse solve.
 constexpr int kMaxSlices = 8;
@@ -59,8 +65,9 @@ constexpr int kMaxVS = 64;
 int choose_block(int nx, int ny, int req);
 
 // Galerkin coarse operator P^T A P of the FP64 system, its band LU, its
-// explicit inverse and the free-cell mask (enqueued on s).
-int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s);
+// explicit inverse and the free-cell mask (enqueued on s).  solver: set up for
+// a flexible solve, which may run with a lagged level (km_set_coarse_update).
+int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s, bool solver = false);
 
 // rc = P^T r (one workgroup per aggregate) and yc = A_c^-1 rc.
 template <typename T>

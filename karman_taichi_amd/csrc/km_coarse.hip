@@ -534,7 +534,7 @@ int choose_block(int nx, int ny, int req) {
     return 512;
 }
 
-int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
+int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s, bool solver) {
     km_coarse* C = get_coarse(ctx);
     const int nx = NX - 2, ny = NY - 2;
     KM_REQUIRE(ctx, nx >= 1 && ny >= 1, "system too small for a coarse level");
@@ -576,7 +576,21 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
     int* ok = reinterpret_cast<int*>(q); q += 256;
     C->mask = reinterpret_cast<uint8_t*>(q);
     // the held inverse belongs to a coarse level of the same shape: try the update
-    const bool update = C->held && C->update && C->NX == NX && C->NY == NY && C->B == B;
+    const bool same = C->held && C->NX == NX && C->NY == NY && C->B == B;
+    if (!(C->NX == NX && C->NY == NY && C->B == B))   // another system shape
+        for (int& c : C->recent) c = 0;
+    // a solve of the system set up last time (same coefficient array) may run
+    // with a lagged coarse inverse: the flexible solve only needs an
+    // approximate one
+    const bool lagged = solver && same && coef == C->coef && C->update > 1 && !C->refresh;
+    if (lagged && C->since + 1 < C->update) {
+        C->since++;   // reuse the held level as it is
+        return KM_OK;
+    }
+    C->since = 0;
+    C->coef = coef;
+    C->refresh = false;
+    const bool update = same && C->update > 0;
     C->NX = NX; C->NY = NY; C->B = B; C->mI = mI; C->mJ = mJ; C->m = m; C->bw = bw;
     hipLaunchKernelGGL(k_coarse_mask, dim3(flat_blocks2(n)), dim3(KM_BLOCK), 0, s, NX, NY, coef, C->mask);
     hipLaunchKernelGGL(k_coarse_coef, dim3(m), dim3(KM_BLOCK), 0, s, NX, NY, B, mI, mJ, coef, C->cc);
@@ -603,8 +617,13 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
         KM_HIP(ctx, hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         C->lds_max = v > 0 ? (size_t)v : 64 * 1024;
     }
-    // the factorisation and explicit inverse run unless the update was kept
+    // the factorisation and explicit inverse run unless the update was kept;
+    // a lagged level keeps its inverse when the update is rejected
     const int* skip = ok;
+    if (lagged) {
+        C->held = true;
+        return KM_OK;
+    }
     if (lds_bytes <= C->lds_max)
         hipLaunchKernelGGL(k_coarse_factor_lds, dim3(1), dim3(64), lds_bytes, s, m, bw, C->band, skip);
     else
@@ -621,6 +640,10 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s) {
     else
         hipLaunchKernelGGL(k_coarse_inverse_wide, gi, dim3(KM_BLOCK), 0, s, m, bw, band, C->inv, skip);
     KM_CHECK_LAUNCH(ctx);
+    // the first factorisation also runs one update product into scratch, so
+    // that rocBLAS loads its kernels here rather than in a later solve
+    if (!update && !C->blas && C->update > 0)
+        if (int rc = ns_gemm(ctx, m, C->inv, C->inv, nsX, s)) return rc;
     C->held = true;
     return KM_OK;
 }
@@ -691,9 +714,12 @@ int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const dou
     return kmcoarse::apply<double>(ctx, coef, coef, r, z1, r1, nullptr, nullptr, ctx->stream);
 }
 
-int km_set_coarse_update(km_ctx* ctx, int on) {
+int km_set_coarse_update(km_ctx* ctx, int every) {
     if (!ctx) return KM_EINVAL;
-    get_coarse(ctx)->update = on != 0;
+    KM_REQUIRE(ctx, every >= 0, "update interval %d", every);
+    km_coarse* C = get_coarse(ctx);
+    C->update = every;
+    C->since = 0;
     return KM_OK;
 }
 

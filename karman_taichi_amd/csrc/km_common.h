@@ -223,6 +223,79 @@ inline int run_chunks(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int
 }
 
 
+// run_chunks with a chunk schedule: chunks of `big` iterations while the
+// iterations enqueued stay `expect` - 2 or more short of `expect` (a guess of
+// the solve's length, 0: none), then chunks of `small` (a divisor of KM_RING),
+// so that when the device sets ``done`` few no-op launches remain in flight.
+// A chunk of `big` starts at phase 0; a chunk of `small` at any multiple of
+// `small`.  Graphs are cached per (key, length, first phase).
+template <typename F>
+inline int run_chunks_sched(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int big, int small, int expect,
+                            int max_iter, F enqueue, KState* final_state) {
+    auto graph_for = [&](int len, int ph0, hipGraphExec_t* out) -> int {
+        GraphKey k = key;
+        k.v.push_back((uintptr_t)len);
+        k.v.push_back((uintptr_t)ph0);
+        k.v.push_back((uintptr_t)ctx->ws_gen);
+        auto it = ctx->graphs.find(k);
+        if (it != ctx->graphs.end()) {
+            *out = it->second;
+            return KM_OK;
+        }
+        hipGraph_t graph = nullptr;
+        KM_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
+        for (int c = 0; c < len; ++c) enqueue(ctx->cap_stream, (ph0 + c) % KM_RING);
+        hipError_t le = hipGetLastError();
+        hipError_t ce = hipStreamEndCapture(ctx->cap_stream, &graph);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return km_fail(ctx, KM_EHIP, "graph capture failed: %s / %s", hipGetErrorString(le), hipGetErrorString(ce));
+        }
+        hipGraphExec_t exec = nullptr;
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return km_fail(ctx, KM_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        ctx->graphs[k] = exec;
+        *out = exec;
+        return KM_OK;
+    };
+    int64_t issued = 0;
+    auto launch_chunk = [&]() -> int {
+        const int ph0 = (int)(issued % KM_RING);
+        const bool near = expect > 0 && issued + big > (int64_t)expect - 2;
+        const int len = (near || ph0 != 0) ? small : big;
+        if (use_graph) {
+            hipGraphExec_t exec = nullptr;
+            if (int rc = graph_for(len, ph0, &exec)) return rc;
+            KM_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+        } else {
+            for (int c = 0; c < len; ++c) enqueue(ctx->stream, (ph0 + c) % KM_RING);
+            KM_CHECK_LAUNCH(ctx);
+        }
+        issued += len;
+        return KM_OK;
+    };
+    // a device-side cap ends every solve; the host cap only guards against a
+    // state that never reports done because a launch failed
+    const int64_t host_cap = (int64_t)max_iter / small + 4;
+    if (int rc = launch_chunk()) return rc;
+    KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[0], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    for (int64_t c = 1;; ++c) {
+        if (c > host_cap) return km_fail(ctx, KM_EHIP, "solver did not report completion after %lld chunks", (long long)c);
+        if (int rc = launch_chunk()) return rc;
+        KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[c & 1], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
+        KM_HIP(ctx, hipEventRecord(ctx->ev[c & 1], ctx->stream));
+        KM_HIP(ctx, hipEventSynchronize(ctx->ev[(c - 1) & 1]));
+        if (ctx->h_st[(c - 1) & 1].done) break;
+    }
+    KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[2], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *final_state = ctx->h_st[2];
+    return KM_OK;
+}
+
+
 // Grid, physics and boundary data of one case on the device (km_grid, flattened).
 struct GridD {
     int nx, ny, east, pin;

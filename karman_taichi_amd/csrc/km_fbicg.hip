@@ -393,7 +393,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         if (int rc = kmnet::solve_stats(ctx, NX, NY, c32, ctx->stream)) return rc;
     }
     if (coarse)
-        if (int rc = kmcoarse::setup(ctx, NX, NY, coef, ctx->stream)) return rc;
+        if (int rc = kmcoarse::setup(ctx, NX, NY, coef, ctx->stream, true)) return rc;
     double bb = 0.0;
     if (int rc = km_dot(ctx, n, KM_F64, b, b, &bb)) return rc < 0 ? rc : KM_EHIP;
     const double bnorm = sqrt(bb);

@@ -52,7 +52,6 @@ constexpr int64_t kTotalWork = 2000000;
 constexpr int FIN_A = 1, FIN_T = 2;
 constexpr int kStallWindow = 300;        // as the unfused flexible solve
 constexpr int kFlatPer = 8;              // cells per thread of a flat workgroup, loads in flight
-constexpr int kChunk = 8;
 // Tile shapes of the smoothing launches (rows x columns, columns contiguous);
 // the fastest for a system is chosen on its first solve.
 constexpr int kNumTiles = 6;
@@ -898,7 +897,25 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
     key.v = {10, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
              (uintptr_t)ws, (uintptr_t)f.st, (uintptr_t)C, (uintptr_t)C->arena, (uintptr_t)C->m, (uintptr_t)C->B,
              (uintptr_t)tile, (uintptr_t)f.fin};
-    return run_chunks(ctx, f.st, key, use_graph, kChunk, max_iter, enqueue, out);
+    // the shortest of the last solves of this system shape (the outer steps
+    // of a time step differ) sets where the short chunks begin
+    int expect = 0;
+    for (int c : C->recent)
+        if (c > 0 && (expect == 0 || c < expect)) expect = c;
+    if (expect > 0) expect = std::max(1, expect - C->chunk_lead + 2);
+    const int rc = run_chunks_sched(ctx, f.st, key, use_graph, C->chunk_big, C->chunk_small, expect, max_iter,
+                                    enqueue, out);
+    if (rc == KM_OK) {
+        for (int k = 7; k > 0; --k) C->recent[k] = C->recent[k - 1];
+        C->recent[0] = out->iter;
+        // a solve over twice as long as the shortest recent one: the lagged
+        // coarse inverse no longer serves, factor on the next set-up
+        int shortest = 0;
+        for (int k = 1; k < 8; ++k)
+            if (C->recent[k] > 0 && (shortest == 0 || C->recent[k] < shortest)) shortest = C->recent[k];
+        if (shortest > 0 && out->iter > 2 * shortest + 4) C->refresh = true;
+    }
+    return rc;
 }
 
 // Average device time of one launch of a fused two-level kernel (which: 4 K_PV,
@@ -993,6 +1010,19 @@ int km_set_coarse_totals(km_ctx* ctx, int mode) {
     if (!ctx) return KM_EINVAL;
     KM_REQUIRE(ctx, ctx->coarse != nullptr, "no coarse level yet (solve once first)");
     ctx->coarse->totals = mode < 0 ? -1 : (mode > 0 ? 1 : 0);
+    return KM_OK;
+}
+
+int km_set_coarse_chunks(km_ctx* ctx, int big, int small, int lead) {
+    if (!ctx) return KM_EINVAL;
+    KM_REQUIRE(ctx, ctx->coarse != nullptr, "no coarse level yet (solve once first)");
+    KM_REQUIRE(ctx, big >= 0 && big % KM_RING == 0 && big <= 64, "chunk of %d iterations", big);
+    KM_REQUIRE(ctx, small >= 0 && (small == 0 || KM_RING % small == 0), "short chunk of %d iterations", small);
+    KM_REQUIRE(ctx, lead >= 0 && lead <= 1000, "lead of %d iterations", lead);
+    km_coarse* C = ctx->coarse;
+    C->chunk_big = big ? big : 8;
+    C->chunk_small = small ? small : 2;
+    C->chunk_lead = lead;
     return KM_OK;
 }
 

@@ -292,12 +292,16 @@ def test_twolevel_inverse_update(change, dev):
     ctx = _lib.context_for(dev)
     z = torch.empty(r.shape, dtype=torch.float64, device=dev)
     ctx.call("km_set_coarse", 16)
+    ctx.call("km_set_coarse_update", 1)   # update on every set-up
     try:
+        ct = torch.empty((5,) + r.shape, dtype=torch.float64, device=dev)   # one coefficient array, rewritten
         for coef in (coef1, coef2):
-            ct, rt_ = T(coef, dev), T(r, dev)
+            ct.copy_(torch.from_numpy(coef))
+            rt_ = T(r, dev)
             ctx.call("km_twolevel_precond", r.shape[0], r.shape[1], ptr(ct), ptr(rt_), ptr(z))
     finally:
         ctx.call("km_set_coarse", 0)
+        ctx.call("km_set_coarse_update", 8)
     zo = krylov.twolevel_precond(coef2, 16)(r)
     e = rel(z.cpu().numpy(), zo)
     print(f"inverse update ({change}): M(r) rel {e:.2e}")
@@ -319,7 +323,7 @@ def test_twolevel_totals_bit_identical(dev):
             out[mode] = coarse_flex(dev, coef, b, None, _lib.KM_FLAG_COARSE)
     finally:
         ctx.call("km_set_coarse_totals", -1)
-        ctx.call("km_set_coarse_update", 1)
+        ctx.call("km_set_coarse_update", 8)
     (xa, ia), (xb, ib) = out[0], out[1]
     print(f"totals: direct {ia.iterations} it, pre-summed {ib.iterations} it")
     assert ia.status == ib.status == _lib.KM_OK
