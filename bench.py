@@ -76,6 +76,9 @@ def parse():
                     help="aggregate edge of the two-level coarse level (0: automatic)")
     ap.add_argument("--totals", type=int, default=-1, choices=[-1, 0, 1],
                     help="two-level partial sums: -1 automatic, 0 every workgroup sums, 1 one workgroup first")
+    ap.add_argument("--momentum", default="auto", choices=["auto", "plain", "rbgs"],
+                    help="u/v solver: plain BiCGSTAB (bicgsolver.py) or flexible BiCGSTAB with one red-black "
+                         "Gauss-Seidel sweep; auto: rbgs with the FP64 two-level modes, plain otherwise")
     ap.add_argument("--coarse-update", type=int, default=8,
                     help="two-level: update the coarse inverse every k-th p' solve (1 each, 0 factor each; default 8)")
     ap.add_argument("--chunks", default="",
@@ -116,6 +119,8 @@ def main():
     p_solver, fp32, twolevel, learned = MODES[args.mode]
     s.mixed = fp32
     s.p_coarse = twolevel
+    momentum = args.momentum if args.momentum != "auto" else ("rbgs" if twolevel and not fp32 else "plain")
+    s.mom_smooth = momentum == "rbgs"
     s.ctx.call("km_set_coarse", int(args.coarse_block))
     s.ctx.call("km_set_coarse_update", int(args.coarse_update))
     if p_solver == "cg":
@@ -250,7 +255,8 @@ def main():
                                f"{case.obstacle_cells}x{case.obstacle_cells} cylinder, "
                                f"{n_outer} outer steps per time step, inner eps 1e-8",
                    "mode": args.mode, "p_solver": p_solver, "p_twolevel": twolevel, "p_network": learned,
-                   "p_vectors": "f32" if fp32 else "f64", "smoothing_tile": tile, "coarse": coarse},
+                   "p_vectors": "f32" if fp32 else "f64", "smoothing_tile": tile, "coarse": coarse,
+                   "momentum": "fbicgstab+rbgs" if momentum == "rbgs" else "bicgstab"},
         "p_solve_ms_median": round(statistics.median(p_ms), 3),
         "p_iters_median": statistics.median(p_it),
         "p_iters": p_it,

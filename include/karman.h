@@ -52,6 +52,7 @@ extern "C" {
 #define KM_FLAG_MIXED 2       /* FP32 matvec/axpy inner solves, FP64 scalars and FP64 true-residual refinement */
 #define KM_FLAG_NO_GRAPH 4    /* launch kernels directly instead of replaying captured graphs */
 #define KM_FLAG_COARSE 8      /* km_fbicgstab: two-level preconditioner (aggregation coarse correction first) */
+#define KM_FLAG_SMOOTH 16     /* km_fbicgstab: one red-black Gauss-Seidel sweep as the preconditioner, no coarse level */
 
 /* east boundary kinds */
 #define KM_EAST_WALL 0        /* simple.py:184-186, :214-216 (with the bicgsimp.py:196 fix) */

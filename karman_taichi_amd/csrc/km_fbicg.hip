@@ -388,42 +388,84 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     hipLaunchKernelGGL(k_scale_to<float>, dim3(blocks(5 * n)), dim3(KM_BLOCK), 0, ctx->stream, 5 * n, coef, c32, 1.0);
     KM_CHECK_LAUNCH(ctx);
     const bool coarse = (flags & KM_FLAG_COARSE) != 0;
+    const bool smooth = (flags & KM_FLAG_SMOOTH) != 0;
+    KM_REQUIRE(ctx, !(smooth && (coarse || params)), "KM_FLAG_SMOOTH takes neither a coarse level nor a network");
     if (params) {
         if (int rc = kmnet::prepare(ctx, 1, NX, NY, params)) return rc;
         if (int rc = kmnet::solve_stats(ctx, NX, NY, c32, ctx->stream)) return rc;
     }
     if (coarse)
         if (int rc = kmcoarse::setup(ctx, NX, NY, coef, ctx->stream, true)) return rc;
-    double bb = 0.0;
-    if (int rc = km_dot(ctx, n, KM_F64, b, b, &bb)) return rc < 0 ? rc : KM_EHIP;
-    const double bnorm = sqrt(bb);
-    if (info) {
-        info->bnorm = bnorm;
-        info->refinements = 0;
-        info->fallback = 0;
-    }
-    if (!(flags & KM_FLAG_X0)) KM_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
-    if (bb == 0.0) {
-        KM_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
-        KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (info) {
-            info->iterations = 0;
-            info->status = KM_OK;
-            info->rel_resid = 0.0;
-            info->rel_resid_true = 0.0;
-        }
-        return KM_OK;
-    }
-    double rr = 0.0;
-    if (flags & KM_FLAG_X0) {
-        if (int rc = residual64(ctx, NX, NY, coef, b, x, r64, &rr)) return rc;
-    } else {
-        // x = 0: r = b and r.r = b.b, no second round trip to the host
-        KM_HIP(ctx, hipMemcpyAsync(r64, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-        rr = bb;
-    }
-    double rel = sqrt(rr) / bnorm;
+    double bb = 0.0, bnorm = 0.0, rr = 0.0, rel = 0.0;
     int total = 0, sweeps = 0, status = KM_MAXITER;
+    // FP64 vectors from x = 0 with a fused inner solve: the first sweep solves
+    // A x = b straight into x (x = 0 is its start); b.b is the solve's own
+    // first <r,r>, so no round trip to the host precedes it
+    const bool direct = f64 && !params && !(flags & KM_FLAG_X0) && (coarse || smooth);
+    if (direct) {
+        KState st;
+        T* xT = reinterpret_cast<T*>(x);
+        const T* bT = reinterpret_cast<const T*>(b);
+        const int rc = smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter, use_graph,
+                                                          &st)
+                              : kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter, use_graph,
+                                                         &st);
+        if (rc) return rc;
+        bb = st.r0;
+        bnorm = sqrt(bb);
+        if (info) {
+            info->bnorm = bnorm;
+            info->refinements = 0;
+            info->fallback = 0;
+        }
+        if (!(bb > 0.0) && bb == bb) {   // b = 0: x stays 0
+            if (info) {
+                info->iterations = 0;
+                info->status = KM_OK;
+                info->rel_resid = 0.0;
+                info->rel_resid_true = 0.0;
+            }
+            return KM_OK;
+        }
+        total = st.iter;
+        sweeps = 1;
+        if (int rc2 = residual64(ctx, NX, NY, coef, b, x, r64, &rr)) return rc2;
+        rel = sqrt(rr) / bnorm;
+        if (!(rel < 0.5)) {
+            // the sweep stalled or diverged: back to x = 0, plain BiCGSTAB finishes
+            KM_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
+            status = KM_STALL;
+            rel = 1.0;
+        }
+    } else {
+        if (int rc = km_dot(ctx, n, KM_F64, b, b, &bb)) return rc < 0 ? rc : KM_EHIP;
+        bnorm = sqrt(bb);
+        if (info) {
+            info->bnorm = bnorm;
+            info->refinements = 0;
+            info->fallback = 0;
+        }
+        if (!(flags & KM_FLAG_X0)) KM_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
+        if (bb == 0.0) {
+            KM_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
+            KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (info) {
+                info->iterations = 0;
+                info->status = KM_OK;
+                info->rel_resid = 0.0;
+                info->rel_resid_true = 0.0;
+            }
+            return KM_OK;
+        }
+        if (flags & KM_FLAG_X0) {
+            if (int rc = residual64(ctx, NX, NY, coef, b, x, r64, &rr)) return rc;
+        } else {
+            // x = 0: r = b and r.r = b.b, no second round trip to the host
+            KM_HIP(ctx, hipMemcpyAsync(r64, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            rr = bb;
+        }
+        rel = sqrt(rr) / bnorm;
+    }
     // warm start x += ||r|| M(r / ||r||), kept only if it lowers the residual
     if (params && rel >= eps && rel == rel) {
         const double rn = sqrt(rr);
@@ -455,7 +497,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     }
     const double inner_floor = f64 ? 0.0 : 1e-5;
     const int inner_cap = f64 ? max_iter : 500;
-    while (true) {
+    while (status != KM_STALL) {
         if (!(rel == rel)) { status = KM_NONFINITE; break; }
         if (rel < eps) { status = KM_OK; break; }
         if (total >= max_iter || sweeps >= 64) { status = KM_MAXITER; break; }
@@ -471,10 +513,12 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         int cap = max_iter - total;
         if (cap > inner_cap) cap = inner_cap;
         KState st;
-        const int rc = coarse && !params
-                           ? kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st)
-                           : fb_inner<T>(ctx, NX, NY, coefT, c32, coef, rhs, d, wT, inner_eps, cap, use_graph, params,
-                                         coarse, &st);
+        const int rc =
+            smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st)
+            : coarse && !params
+                ? kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st)
+                : fb_inner<T>(ctx, NX, NY, coefT, c32, coef, rhs, d, wT, inner_eps, cap, use_graph, params, coarse,
+                              &st);
         if (rc) return rc;
         total += st.iter;
         sweeps++;

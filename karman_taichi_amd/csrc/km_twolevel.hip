@@ -44,12 +44,12 @@ constexpr int kMaxFlatWg = 8192;                 // flat workgroups of K_CT and 
 constexpr int TL_PV = 0;                         // [kMaxTileWg] <r~,v> (K_PV)
 constexpr int TL_PR = kMaxTileWg;                // [kMaxFlatWg] <r,r> (K_CT's flat workgroups, the start)
 constexpr int TL_PT = kMaxTileWg + kMaxFlatWg;   // [4][kMaxTileWg] <t,s>, <t,t>, <s,r~>, <t,r~> (K_ST)
-constexpr int TL_TOT = TL_PT + 4 * kMaxTileWg;   // totals: 0 <r~,v>, 1..4 the K_ST planes
+constexpr int TL_TOT = TL_PT + 4 * kMaxTileWg;   // totals: 0 <r~,v>, 1..4 the K_ST planes, 5 <r,r>
 // The coarse (and flat) workgroups sum the tile partials; when partials x
 // consumer workgroups exceeds this, one workgroup sums them first
 // (k_tl_total) and the consumers read the totals.
 constexpr int64_t kTotalWork = 2000000;
-constexpr int FIN_A = 1, FIN_T = 2;
+constexpr int FIN_A = 1, FIN_T = 2, FIN_R = 4;
 constexpr int kStallWindow = 300;        // as the unfused flexible solve
 constexpr int kFlatPer = 8;              // cells per thread of a flat workgroup, loads in flight
 // Tile shapes of the smoothing launches (rows x columns, columns contiguous);
@@ -140,6 +140,14 @@ struct TlArgs {
     const KState* st;        // null: no state (tuning, MODE 0)
     double* P;               // partial planes of this launch
     int ph;
+    T om;                    // relaxation of the red-black sweeps
+    // K_ST with its own scalars (a solve without a coarse level): the
+    // partials of <r~,v> (Gin[0] workgroups) and <r,r> (Gin[1]) at Pin, or
+    // their totals (tot: FIN_A and FIN_R), and the state it updates
+    const double* Pin;
+    int Gin[2];
+    int tot;
+    KState* stw;
 };
 
 // x = 0, r = r~ = p = rhs, partials of <r,r> on the flat workgroups, the state
@@ -196,20 +204,62 @@ k_tl_cinit(int m, int S, const double* __restrict__ yc, double* __restrict__ Y, 
     }
 }
 
+// The stopping test of iteration ph on <r,r> and alpha = rho / <r~,v>
+// (bicgsolver.py:108,130) with the stall test of the flexible solve; every
+// workgroup derives the same, the first one writes the state.  false: the
+// solve has ended (``done`` set).
+__device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, double rvv, double rr, double* alpha) {
+    const bool lead0 = blockIdx.x == 0 && threadIdx.x == 0;
+    const KScal sp = st->ring[ph];
+    const KScal spp = st->ring[(ph + KM_RING - 1) % KM_RING];
+    const int i = sp.iter;
+    const double rho = i == 0 ? rr : sp.rho;
+    const double r0 = i == 0 ? rr : st->r0;
+    int status = -1;
+    if (!fin(rr)) status = KM_NONFINITE;
+    else if (i == 0 && rr == 0.0) status = KM_OK;
+    else if (i > 0 && sqrt(rr / r0) < st->eps) status = KM_OK;   // bicgsolver.py:130
+    else if (i > 0 && sp.omega == 0.0) status = KM_BREAKDOWN;
+    else if (i >= st->max_iter) status = KM_MAXITER;
+    else if (!fin(rho)) status = KM_NONFINITE;
+    else if (rho == 0.0) status = KM_BREAKDOWN;
+    else if (!fin(rvv)) status = KM_NONFINITE;
+    else if (rvv == 0.0) status = KM_BREAKDOWN;
+    const bool better = rr < 0.81 * spp.best;
+    if (status < 0 && !better && i - spp.best_it >= kStallWindow) status = KM_STALL;
+    if (lead0 && i == 0) st->r0 = r0;
+    if (status >= 0) {
+        if (lead0) finish_state(st, status, i, rr);
+        return false;
+    }
+    *alpha = rho / rvv;   // bicgsolver.py:108
+    if (lead0) {
+        st->ring[ph].alpha = *alpha;
+        if (i == 0) st->ring[ph].rho = rho;
+        st->ring[ph].best = better ? rr : spp.best;
+        st->ring[ph].best_it = better ? i : spp.best_it;
+        st->rTr = rr;
+        st->iter = i;
+    }
+    return true;
+}
+
 // K_PV (MODE 1) / K_ST (MODE 2) / the standalone smoother (MODE 0).  The
 // coefficients and the frame input of the tile's NS-cell halo frame stay in
-// registers (Q cells per thread, NS = 2 kSweeps half-sweeps); z lives in LDS
+// registers (Q cells per thread, NS = 2 SW half-sweeps); z lives in LDS
 // on the (NS + 1)-cell halo frame and is updated in place: a red half-sweep
 // changes only red cells ((i + j) even) and reads only black ones, and the
 // other way round.  The arithmetic per cell is z + (omega / aP) (src - A z),
 // neighbours outside the system read as zero, rows with aP = 0 left
 // unchanged.  Half-sweep s covers the halo-(NS - s) frame, so after the last
 // one z is exact one cell into the halo, which covers the boundary cells the
-// edge tiles own.
-template <typename T, int TI, int TJ, int MODE, int WPE>
+// edge tiles own.  Without a coarse vector (yc null) z starts from zero;
+// without slots (rcs null) nothing is restricted.  SELF (MODE 2): the
+// launch derives alpha and the stopping test itself (cv_scalars).
+template <typename T, int TI, int TJ, int MODE, int WPE, int SW, bool SELF>
 __global__ void __launch_bounds__(KM_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
-    constexpr int NS = 2 * kmcoarse::kSweeps;                       // half-sweeps = halo
+    constexpr int NS = 2 * SW;                                      // half-sweeps = halo
     constexpr int WZ = TJ + 2 * (NS + 1), HZ = TI + 2 * (NS + 1);   // z frame: halo NS + 1
     constexpr int WR = TJ + 2 * NS, HR = TI + 2 * NS;               // register frame: halo NS
     constexpr int NR = WR * HR;
@@ -222,8 +272,23 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
     const int L = xcd_remap(blockIdx.x, tl.G);
     const int NX = tl.NX, NY = tl.NY, nx = NX - 2, ny = NY - 2;
     const int64_t n = (int64_t)NX * NY;
-    const T om = (T)kmcoarse::kOmega;
-    const T fa = (T)(MODE == 2 && st ? st->ring[g.ph].alpha : 0.0);
+    const T om = g.om;
+    double alpha = MODE == 2 && st && !SELF ? st->ring[g.ph].alpha : 0.0;
+    if (SELF) {
+        double rvv, rr;
+        if ((g.tot & (FIN_A | FIN_R)) == (FIN_A | FIN_R)) {
+            rvv = g.Pin[TL_TOT];
+            rr = g.Pin[TL_TOT + 5];
+        } else {
+            const double* planes[2] = {g.Pin + TL_PV, g.Pin + TL_PR};
+            double t2[2];
+            sum_partials_each<2>(planes, g.Gin, t2);
+            rvv = t2[0];
+            rr = t2[1];
+        }
+        if (!cv_scalars(g.stw, g.ph, rvv, rr, &alpha)) return;
+    }
+    const T fa = (T)alpha;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     const int t0 = L * tl.tpb, t1 = t0 + tl.tpb < tl.T ? t0 + tl.tpb : tl.T;
     for (int tile = t0; tile < t1; ++tile) {
@@ -257,7 +322,7 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
             kq[q] = in[q] ? (int64_t)i * NY + j : 0;
             const int64_t k = kq[q];
             if ((i + j) & 1) blk |= 1u << q;
-            if (MODE > 0 && ou[q] && i >= 1 && i <= nx && j >= 1 && j <= ny) {
+            if (MODE > 0 && g.rcs && ou[q] && i >= 1 && i <= nx && j >= 1 && j <= ny) {
                 ib |= 1u << q;
                 if (g.mask[k]) fm |= 1u << q;
             }
@@ -276,8 +341,8 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
             const int i = ti0 - NS - 1 + a, j = tj0 - NS - 1 + b;
             const bool ing = idx < HZ * WZ && i >= 0 && i < NX && j >= 0 && j < NY;
             const int ic = i < 1 ? 0 : (i > nx ? nx - 1 : i - 1), jc = j < 1 ? 0 : (j > ny ? ny - 1 : j - 1);
-            yq[q] = g.yc[(ic / cg.B) * cg.mJ + jc / cg.B];
-            mq[q] = ing ? g.mask[(int64_t)i * NY + j] : 0;
+            yq[q] = g.yc ? g.yc[(ic / cg.B) * cg.mJ + jc / cg.B] : 0.0;
+            mq[q] = ing && g.yc ? g.mask[(int64_t)i * NY + j] : 0;
         }
         // the source of M(.) on the frame (MODE 2: s, written on the tile's cells)
 #pragma unroll
@@ -352,7 +417,7 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
                 }
             }
         }
-        if (MODE > 0) {
+        if (MODE > 0 && g.rcs) {
             // P^T y over the aggregates this tile overlaps: one wave per
             // aggregate piece of yb, summed in a fixed order
             __syncthreads();
@@ -405,14 +470,15 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
     }
 }
 
-// One tile launch with tile shape `which` (an index into kTileI / kTileJ).
+// One tile launch of the two-level smoother (kmcoarse::kSweeps red-black
+// sweeps) with tile shape `which` (an index into kTileI / kTileJ).
 template <typename T, int MODE>
 void launch_tile(int which, hipStream_t q, int NX, int NY, const CGeom& cg, const TlArgs<T>& a) {
 #define KM_TL_CASE(IDX)                                                                                           \
     case IDX: {                                                                                                   \
         const ATile tl = a_tile(NX, NY, kTileI[IDX], kTileJ[IDX]);                                               \
-        hipLaunchKernelGGL((k_tl_tile<T, kTileI[IDX], kTileJ[IDX], MODE, kTileW[IDX]>), dim3(tl.G), dim3(KM_BLOCK), \
-                           0, q, tl, cg, a);                                                                      \
+        hipLaunchKernelGGL((k_tl_tile<T, kTileI[IDX], kTileJ[IDX], MODE, kTileW[IDX], kmcoarse::kSweeps, false>), \
+                           dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, cg, a);                                          \
         break;                                                                                                    \
     }
     switch (which) {
@@ -515,7 +581,6 @@ k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict_
     double av[kCoarseCh];
     first_row(a, m, lane, av);
     load_rc(rcs, m, S, rs_);
-    const bool lead0 = blockIdx.x == 0 && threadIdx.x == 0;
     double rvv, rr;
     if (tot & FIN_A) {
         double t1[1];
@@ -530,37 +595,8 @@ k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict_
         rvv = t2[0];
         rr = t2[1];
     }
-    const KScal sp = st->ring[ph];
-    const KScal spp = st->ring[(ph + KM_RING - 1) % KM_RING];
-    const int i = sp.iter;
-    const double rho = i == 0 ? rr : sp.rho;
-    const double r0 = i == 0 ? rr : st->r0;
-    int status = -1;
-    if (!fin(rr)) status = KM_NONFINITE;
-    else if (i == 0 && rr == 0.0) status = KM_OK;
-    else if (i > 0 && sqrt(rr / r0) < st->eps) status = KM_OK;   // bicgsolver.py:130
-    else if (i > 0 && sp.omega == 0.0) status = KM_BREAKDOWN;
-    else if (i >= st->max_iter) status = KM_MAXITER;
-    else if (!fin(rho)) status = KM_NONFINITE;
-    else if (rho == 0.0) status = KM_BREAKDOWN;
-    else if (!fin(rvv)) status = KM_NONFINITE;
-    else if (rvv == 0.0) status = KM_BREAKDOWN;
-    const bool better = rr < 0.81 * spp.best;
-    if (status < 0 && !better && i - spp.best_it >= kStallWindow) status = KM_STALL;
-    if (lead0 && i == 0) st->r0 = r0;
-    if (status >= 0) {
-        if (lead0) finish_state(st, status, i, rr);
-        return;
-    }
-    const double alpha = rho / rvv;   // bicgsolver.py:108
-    if (lead0) {
-        st->ring[ph].alpha = alpha;
-        if (i == 0) st->ring[ph].rho = rho;
-        st->ring[ph].best = better ? rr : spp.best;
-        st->ring[ph].best_it = better ? i : spp.best_it;
-        st->rTr = rr;
-        st->iter = i;
-    }
+    double alpha;
+    if (!cv_scalars(st, ph, rvv, rr, &alpha)) return;
     __syncthreads();   // rs_
     if (!live) return;
     const double s = row_dot(a, m, lane, av, rs_);
@@ -704,6 +740,7 @@ int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
         a.yc = C->yc;
         a.rcs = C->frc;
         a.P = ctx->partials + TL_PV;
+        a.om = (T)kOmega;
         TlArgs<T> b = a;   // K_ST: s = r - alpha v (alpha 0 without a state), s^, t
         b.f0 = src;
         b.f1 = ws + n;
@@ -761,6 +798,7 @@ int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s) {
     a.f0 = r;
     a.z = z;
     a.yc = C->yc;
+    a.om = (T)kOmega;
     launch_tile<T, 0>(0, s, C->NX, C->NY, c_geom(C, 0), a);
     KM_CHECK_LAUNCH(ctx);
     return KM_OK;
@@ -817,6 +855,7 @@ struct Fused {
         a.st = st;
         a.P = P + TL_PV;
         a.ph = phs;
+        a.om = (T)kOmega;
         launch_tile<T, 1>(tile, q, NX, NY, cg, a);
     }
     void st_(hipStream_t q, int phs) const {
@@ -834,6 +873,7 @@ struct Fused {
         a.st = st;
         a.P = P + TL_PT;
         a.ph = phs;
+        a.om = (T)kOmega;
         launch_tile<T, 2>(tile, q, NX, NY, cg, a);
     }
     void total_a(hipStream_t q) const {
@@ -994,6 +1034,134 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
     *avg_us = 1e3 * total_ms / reps;
     return KM_OK;
 }
+
+namespace {
+
+// The smoothed flexible solve without a coarse level (the momentum systems):
+// M(r) = one red-black Gauss-Seidel sweep on A z = r from z = 0, on 16 x 32
+// tiles; three launches per iteration (K_PV, K_ST deriving alpha and the
+// stopping test itself, K_CT with flat workgroups only).
+constexpr int kMomTile = 2;
+constexpr int kMomSweeps = 1;
+constexpr double kMomOmega = 1.0;   // over-relaxed sweeps diverge on the convective momentum systems
+
+template <typename T, int MODE, bool SELF>
+void launch_mom(hipStream_t q, int NX, int NY, const TlArgs<T>& a) {
+    const ATile tl = a_tile(NX, NY, kTileI[kMomTile], kTileJ[kMomTile]);
+    CGeom cg;
+    cg.B = 1;
+    cg.mJ = 1;
+    cg.m = 0;
+    cg.SJ = 1;
+    hipLaunchKernelGGL((k_tl_tile<T, kTileI[kMomTile], kTileJ[kMomTile], MODE, kTileW[kMomTile], kMomSweeps, SELF>),
+                       dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, cg, a);
+}
+
+template <typename T>
+struct SmoothFused {
+    int NX, NY, Gt, fin;
+    FGeom fl;
+    T *r, *rt, *p, *v, *s, *t, *ph, *sh, *x;
+    const T* coefT;
+    KState* st;
+    double* P;
+
+    SmoothFused(km_ctx* ctx, int NX_, int NY_, const T* coef, T* ws, T* xd)
+        : NX(NX_), NY(NY_), x(xd), coefT(coef), st(&ctx->st[4]), P(ctx->partials) {
+        const int64_t n = (int64_t)NX * NY;
+        r = ws;
+        rt = ws + n;
+        p = ws + 2 * n;
+        v = ws + 3 * n;
+        s = ws + 4 * n;
+        t = ws + 5 * n;
+        ph = ws + 6 * n;
+        sh = ws + 7 * n;
+        fl = f_geom(n);
+        Gt = a_tile(NX, NY, kTileI[kMomTile], kTileJ[kMomTile]).G;
+        // K_ST's Gt workgroups read <r~,v> and <r,r>, K_CT's fl.G read K_ST's planes
+        fin = ((int64_t)Gt * (Gt + fl.G) > kTotalWork ? FIN_A | FIN_R : 0) |
+              ((int64_t)Gt * fl.G > kTotalWork ? FIN_T : 0);
+    }
+    void pv(hipStream_t q, int phs) const {
+        TlArgs<T> a{};
+        a.c = coefT;
+        a.f0 = p;
+        a.rt = rt;
+        a.z = ph;
+        a.y = v;
+        a.st = st;
+        a.P = P + TL_PV;
+        a.ph = phs;
+        a.om = (T)kMomOmega;
+        launch_mom<T, 1, false>(q, NX, NY, a);
+        if (fin & FIN_A)
+            hipLaunchKernelGGL(k_tl_total<1>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PV), Gt,
+                               kMaxTileWg, P + TL_TOT, (const KState*)st);
+    }
+    void st_(hipStream_t q, int phs) const {
+        TlArgs<T> a{};
+        a.c = coefT;
+        a.f0 = r;
+        a.f1 = v;
+        a.rt = rt;
+        a.src = s;
+        a.z = sh;
+        a.y = t;
+        a.st = st;
+        a.P = P + TL_PT;
+        a.ph = phs;
+        a.om = (T)kMomOmega;
+        a.Pin = P;
+        a.Gin[0] = Gt;
+        a.Gin[1] = fl.G;
+        a.tot = fin;
+        a.stw = st;
+        launch_mom<T, 2, true>(q, NX, NY, a);
+        if (fin & FIN_T)
+            hipLaunchKernelGGL(k_tl_total<4>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PT), Gt,
+                               kMaxTileWg, P + TL_TOT + 1, (const KState*)st);
+    }
+    void total_r(hipStream_t q) const {
+        if (fin & FIN_R)
+            hipLaunchKernelGGL(k_tl_total<1>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PR), fl.G,
+                               kMaxFlatWg, P + TL_TOT + 5, (const KState*)st);
+    }
+    void ct(hipStream_t q, int phs) const {
+        hipLaunchKernelGGL(k_tl_ct<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, 0, 1, Gt, 0, fl, fin, phs,
+                           (const double*)nullptr, (const double*)nullptr, (double*)nullptr, x, r, p, (const T*)v,
+                           (const T*)s, (const T*)t, (const T*)ph, (const T*)sh, st, P);
+        total_r(q);
+    }
+    void iteration(hipStream_t q, int phs) const {
+        pv(q, phs);
+        st_(q, phs);
+        ct(q, phs);
+    }
+};
+
+}  // namespace
+
+template <typename T>
+int smooth_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
+                 bool use_graph, KState* out) {
+    const SmoothFused<T> f(ctx, NX, NY, coefT, ws, xd);
+    hipLaunchKernelGGL(k_tl_init<T>, dim3(f.fl.G), dim3(KM_BLOCK), 0, ctx->stream, f.fl, rhs, f.x, f.r, f.rt, f.p, eps,
+                       max_iter, f.st, f.P);
+    KM_CHECK_LAUNCH(ctx);
+    f.total_r(ctx->stream);
+    auto enqueue = [&f](hipStream_t q, int phs) { f.iteration(q, phs); };
+    GraphKey key;
+    key.v = {11, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
+             (uintptr_t)ws, (uintptr_t)f.st, (uintptr_t)f.fin};
+    // these solves take a handful of iterations: short graphs throughout
+    return run_chunks_sched(ctx, f.st, key, use_graph, KM_RING, 2, 1, max_iter, enqueue, out);
+}
+
+template int smooth_inner<float>(km_ctx*, int, int, const float*, const float*, float*, float*, double, int, bool,
+                                 KState*);
+template int smooth_inner<double>(km_ctx*, int, int, const double*, const double*, double*, double*, double, int,
+                                  bool, KState*);
 
 template int smooth<float>(km_ctx*, const float*, const float*, float*, hipStream_t);
 template int smooth<double>(km_ctx*, const double*, const double*, double*, hipStream_t);

@@ -95,7 +95,8 @@ class SIMPLESolver:
         self.mixed = False            # FP32 inner solves + FP64 refinement
         self.warm_start_p = False
         self.p_precond = None         # a learned.UNetPreconditioner: flexible BiCGSTAB for p'
-        self.p_coarse = False         # two-level p' preconditioner (coarse correction + network or Jacobi)
+        self.p_coarse = False         # two-level p' preconditioner (coarse correction + network or SOR sweeps)
+        self.mom_smooth = False       # momentum: flexible BiCGSTAB with a red-black Gauss-Seidel sweep
         self.last = {}
 
     @classmethod
@@ -187,8 +188,8 @@ class SIMPLESolver:
             self.compute_coef_u()
             self.compute_coef_v()
             self.set_bc()
-            iu = self.u_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed)
-            iv = self.v_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed)
+            iu = self.u_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed, smooth=self.mom_smooth)
+            iv = self.v_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed, smooth=self.mom_smooth)
             self.update_velocity()
         return iu, iv
 

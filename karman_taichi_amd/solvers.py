@@ -158,7 +158,7 @@ class BICGSolver(CGSolver):
     _kind = "bicgstab"
 
     def solve(self, eps=1e-8, quiet=True, max_iter=None, mixed=False, x0=False, graph=True,
-              precond=None, coarse=False) -> SolveInfo:
+              precond=None, coarse=False, smooth=False) -> SolveInfo:
         """BiCGSTAB until sqrt(r.r / b.b) < eps (bicgsolver.py:130).
 
         ``mixed``: FP32 matvec/axpy inner solves, FP64 scalars, FP64 refinement
@@ -169,15 +169,17 @@ class BICGSolver(CGSolver):
         FP64 refinement (FP32 vectors with ``mixed``), judged by the true
         residual only, with plain BiCGSTAB as the fallback.  ``coarse``: the
         two-level preconditioner (aggregation coarse correction, then the
-        network -- or damped Jacobi without one -- on the remaining residual).
+        network -- or red-black SOR sweeps without one).  ``smooth``: flexible
+        BiCGSTAB with one red-black Gauss-Seidel sweep as the preconditioner
+        (the momentum systems), judged by the true residual like the others.
         """
         flags = (_lib.KM_FLAG_MIXED if mixed else 0) | (_lib.KM_FLAG_X0 if x0 else 0)
         flags |= 0 if graph else _lib.KM_FLAG_NO_GRAPH
-        if precond is not None or coarse:
+        if precond is not None or coarse or smooth:
             from .learned import fbicgstab
             self._refresh()
             info = fbicgstab(self.coef, self.b, self.x, precond, eps=eps, max_iter=max_iter, x0=x0, graph=graph,
-                             mixed=mixed, coarse=coarse)
+                             mixed=mixed, coarse=coarse, smooth=smooth)
         else:
             info = self._run("km_bicgstab", eps, max_iter, flags)
         if not quiet:
