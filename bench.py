@@ -35,18 +35,18 @@ HBM_PEAK_GBS = 8000.0
 #   B reads r, v', r~ and 5 coefficient planes, writes s, t
 BYTES_PER_CELL = {"bicg_a": 15 * 8, "bicg_b": 10 * 8,
                   # the fused two-level iteration (FP64 vectors, 1-byte free-cell mask):
-                  #   tl_pv reads s, t, p, v, x, p^, s^, r~, 5 coefficient planes, mask; writes x, r, p', p^', v'
-                  #   tl_st reads r, v', r~, 5 coefficient planes, mask; writes s, s^, t
-                  "tl_pv": 18 * 8 + 1, "tl_st": 11 * 8 + 1}
-# the coarse launches of the fused iteration read A_c^-1 (m x m) and a few m-vectors
+                  #   tl_pv reads p, 5 coefficient planes, r~, mask; writes p^, v
+                  #   tl_st reads r, v, 5 coefficient planes, r~, mask; writes s, s^, t
+                  #   tl_ct (flat workgroups) reads s, t, p, v, x, p^, s^; writes x, r, p
+                  "tl_pv": 9 * 8 + 1, "tl_st": 11 * 8 + 1, "tl_ct": 10 * 8}
+# the coarse rows of tl_cv / tl_ct read A_c^-1 (m x m) and a few m-vectors
 COARSE_VECTORS = {"tl_cv": 4, "tl_ct": 6}
 
 
 def alg_bytes(name, cells, m):
     """Algorithmic bytes of one launch of kernel `name`."""
-    if name in COARSE_VECTORS:
-        return 8 * m * m + 8 * COARSE_VECTORS[name] * m
-    return BYTES_PER_CELL[name] * cells
+    coarse = 8 * m * m + 8 * COARSE_VECTORS[name] * m if name in COARSE_VECTORS else 0
+    return coarse + BYTES_PER_CELL.get(name, 0) * cells
 
 
 # p' solver of each mode: (solver, FP32 vectors, two-level, network)
