@@ -43,6 +43,14 @@ BYTES_PER_CELL = {"bicg_a": 15 * 8, "bicg_b": 10 * 8,
 COARSE_VECTORS = {"tl_cv": 4, "tl_ct": 6}
 
 
+def mall_note(cells, m):
+    """Working set of one p' solve: 5 coefficient planes and ~10 vectors in
+    FP64, and A_c^-1, against the 256 MB MALL (Infinity Cache)."""
+    ws = cells * 8 * 15 + 8 * m * m
+    return (f"working set of one p' system ~{ws / 1e6:.0f} MB: "
+            f"{'fits' if ws < 256e6 else 'exceeds'} the 256 MB MALL")
+
+
 def alg_bytes(name, cells, m):
     """Algorithmic bytes of one launch of kernel `name`."""
     coarse = 8 * m * m + 8 * COARSE_VECTORS[name] * m if name in COARSE_VECTORS else 0
@@ -273,7 +281,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": launch_bytes,
-                     "note": "working set of one p' system is ~34 MB and fits the 256 MB MALL"},
+                     "note": mall_note(cells_p, coarse["unknowns"] if coarse else 0)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

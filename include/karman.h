@@ -174,8 +174,13 @@ int km_adam_step(km_ctx* ctx, int64_t n, float* params, const float* grad, float
  * BiCGSTAB finishes (info->fallback = 1).  params == NULL selects the Jacobi
  * preconditioner D^-1 instead (karman_taichi.py:399,419), with no warm start.
  * KM_FLAG_COARSE makes the preconditioner two-level: z1 = P A_c^-1 P^T r on
- * B x B aggregates of free cells (Galerkin A_c, inverted once per solve), then
- * the network -- or 3 sweeps of damped Jacobi -- on r - A z1, added to z1. */
+ * B x B aggregates of free cells (Galerkin A_c, its inverse held across
+ * solves, km_set_coarse_update), then the network on r - A z1, added to z1 --
+ * or, without a network, two red-black SOR sweeps (omega 1.3) on A z = r from
+ * z1, run as a fused four-launch iteration.  KM_FLAG_SMOOTH: one red-black
+ * Gauss-Seidel sweep on A z = r from zero as the preconditioner (no network,
+ * no coarse level; the momentum systems), a fused three-launch iteration.
+ * FP64 solves from x = 0 with either fused iteration solve straight into x. */
 int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
                  int max_iter, int flags, const float* params, km_solve_info* info);
 
@@ -226,11 +231,12 @@ int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const dou
 int km_set_coarse_update(km_ctx* ctx, int every);
 /* Aggregate size and coarse unknowns of the last coarse level. */
 int km_coarse_info(km_ctx* ctx, int* block, int* unknowns);
-/* z = M(r) for one FP64 system: the two-level Jacobi preconditioner of the
- * flexible solve with KM_FLAG_COARSE and no network, z = P A_c^-1 P^T r then
- * four sweeps z += 0.8 D^-1 (r - A z).  It fills the p^ / s^ hooks of the
- * reference's BiCGSTAB (bicgsolver.py:48-58, karman_taichi.py:399,419 pass
- * a Jacobi scaling there); sets the coarse level up first. */
+/* z = M(r) for one FP64 system: the two-level preconditioner of the flexible
+ * solve with KM_FLAG_COARSE and no network, z1 = P A_c^-1 P^T r then two
+ * red-black SOR sweeps (red cells (i + j even) first, omega 1.3) on A z = r.
+ * It fills the p^ / s^ hooks of the reference's BiCGSTAB (bicgsolver.py:48-58,
+ * karman_taichi.py:399,419 pass a Jacobi scaling there); sets the coarse level
+ * up first (always current: no lagged level here). */
 int km_twolevel_precond(km_ctx* ctx, int NX, int NY, const double* coef, const double* r, double* z);
 /* Tile shape (rows, columns) of the fused two-level smoothing kernels, chosen
  * by timing on the first KM_FLAG_COARSE solve of the coarse level's system.
@@ -260,11 +266,10 @@ int km_set_coarse_chunks(km_ctx* ctx, int big, int small, int lead);
 #define KM_KERNEL_BICG_B 1   /* s update + stencil + <t, s>, <t, t>, <s, r~>, <t, r~> */
 #define KM_KERNEL_STENCIL 3  /* plain y = A x */
 /* the fused two-level flexible iteration (KM_FLAG_COARSE, no network) */
-#define KM_KERNEL_TL_PV 4    /* x, r, p updates; p^ = M(p) (prolongation + 4 Jacobi sweeps); v = A p^;
-                                <r~, v>, <r, r>; P^T v */
+#define KM_KERNEL_TL_PV 4    /* p^ = M(p) (prolongation + 2 red-black SOR sweeps); v = A p^; <r~, v>; P^T v */
 #define KM_KERNEL_TL_CV 5    /* stopping test, alpha; coarse Y(v) = A_c^-1 P^T v, Y(s) */
 #define KM_KERNEL_TL_ST 6    /* s = r - alpha v; s^ = M(s); t = A s^; <t, s>, <t, t>, <s, r~>, <t, r~>; P^T t */
-#define KM_KERNEL_TL_CT 7    /* omega, beta; coarse Y(t), Y(r), Y(p) */
+#define KM_KERNEL_TL_CT 7    /* omega, beta; coarse Y(t), Y(r), Y(p); flat x, r, p updates, <r, r> */
 #define KM_KERNEL_BACK_TO_BACK 0x100   /* or-ed in: time `reps` identical back-to-back launches between two events */
 int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps,
                    double* avg_us);

@@ -87,8 +87,8 @@ int apply(km_ctx* ctx, const double* coef, const T* coefT, const T* r, T* z1, T*
 template <typename T>
 int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s);
 
-// Inner flexible BiCGSTAB for A d = rhs with the two-level Jacobi
-// preconditioner, four launches per iteration; ws holds 8n values of T.
+// Inner flexible BiCGSTAB for A d = rhs with the two-level preconditioner
+// (red-black SOR smoothing), four launches per iteration; ws holds 8n values of T.
 template <typename T>
 int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
                 bool use_graph, KState* out);

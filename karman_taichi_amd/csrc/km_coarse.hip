@@ -7,7 +7,8 @@
 // and inverted explicitly once per solve, so one application
 //   z1 = P A_c^-1 P^T r,   r1 = r - A z1
 // is a restriction, a small dense matvec and a prolongation fused with the
-// residual.  The fine-level smoother (the network or damped Jacobi) then acts
+// residual.  The fine-level smoother (the network, or red-black SOR sweeps
+// in the fused iteration of km_twolevel.hip) then acts
 // on r1 and its output is added to z1.
 #include "km_coarse.h"
 

@@ -8,7 +8,9 @@
 // t = A s^ are formed from them.  A sweep that does not halve the true
 // residual is taken back and plain FP64 BiCGSTAB finishes the solve.
 // Without a network (params == null) the preconditioner is Jacobi, D^-1, or
-// with the coarse level the two-level Jacobi preconditioner (km_twolevel.hip).
+// with the coarse level the two-level preconditioner with red-black SOR
+// sweeps, or with KM_FLAG_SMOOTH one red-black Gauss-Seidel sweep (both the
+// fused iterations of km_twolevel.hip).
 //
 // Inner iteration i (ring slot ph = i mod 4, as in the plain solver):
 //   K_p  stopping test on <r, r>; p = r + beta (p - omega v)
@@ -294,8 +296,8 @@ const int kFbChunk = 8;
 // coefT: the coefficients in T (the matvecs); c32: FP32 coefficients (the
 // network's features); coef64: the FP64 system (coarse level).  ws holds 10n
 // values of T.  The preconditioner: the network (params) or Jacobi, after
-// the coarse correction when coarse is set.  The coarse level with Jacobi
-// smoothing runs as the fused iteration of km_twolevel.hip instead.
+// the coarse correction when coarse is set.  Without a network the coarse
+// level (and KM_FLAG_SMOOTH) run as the fused iterations of km_twolevel.hip.
 template <typename T>
 int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, const double* coef64, const T* rhs, T* xd,
              T* ws, double eps, int max_iter, bool use_graph, const float* params, bool coarse, KState* out) {

@@ -80,7 +80,8 @@ def evaluate(case, outer_steps: int, variants: dict, eps: float = 1e-8, device=N
 
 def variants_for(net, untrained, coarse: bool) -> dict:
     """The learned solve and its ablations: the same structure with an untrained
-    network and with damped Jacobi in the network's place."""
+    network and with no network (label "jacobi": the default smoother in its
+    place -- red-black SOR sweeps after the coarse correction, Jacobi without)."""
     v = {"learned": (net, coarse, False), "learned_fp32": (net, coarse, True),
          "untrained": (untrained, coarse, False), "jacobi": (None, coarse, False)}
     if coarse:
