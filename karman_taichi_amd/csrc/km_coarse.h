@@ -91,14 +91,14 @@ int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s);
 // (red-black SOR smoothing), four launches per iteration; ws holds 8n values of T.
 template <typename T>
 int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
-                bool use_graph, KState* out);
+                bool use_graph, KState* out, const std::function<int()>* tail = nullptr);
 
 // Inner flexible BiCGSTAB for A d = rhs with one red-black Gauss-Seidel sweep
 // as the preconditioner and no coarse level (the momentum systems), three
 // launches per iteration; ws holds 8n values of T.
 template <typename T>
 int smooth_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
-                 bool use_graph, KState* out);
+                 bool use_graph, KState* out, const std::function<int()>* tail = nullptr);
 
 // Tile shape (rows, columns) of the fused smoothing kernels chosen for the
 // coarse level's system in FP64 (32 x 32 before the first fused solve).

@@ -6,12 +6,16 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
+#include <atomic>
+#include <chrono>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <set>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -128,6 +132,8 @@ struct km_ctx {
     double* partials = nullptr;    // per-workgroup partial sums
     double* dres = nullptr;        // small device results
     KState* h_st = nullptr;        // pinned host mirrors of st[0] (two polling slots)
+    volatile int* h_flag = nullptr;   // coherent pinned host words the fused solves write: [0] done, [1] iteration
+    int* d_flag = nullptr;         // the same words as the device addresses them
     double* h_res = nullptr;       // pinned host results
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tiles;   // (NX, NY, prec) -> (TI, TJ)
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tile_override;
@@ -162,6 +168,9 @@ int km_ensure_workspace(km_ctx* ctx, size_t vec_bytes);
 void* km_ws_vectors(km_ctx* ctx);
 int km_choose_tile(km_ctx* ctx, int NX, int NY, int prec, int* TI, int* TJ);
 void km_drop_graphs(km_ctx* ctx);
+// r = b - A x and ||r||^2 into ctx->h_res[0] once the stream reaches it (no
+// synchronisation; km_krylov.hip)
+int km_residual_async(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, const double* x, double* r);
 
 // Poll loop: enqueue chunks of iterations (a captured graph, or direct
 // launches), two chunks in flight, until the device sets ``done``.  The chunk
@@ -292,6 +301,96 @@ inline int run_chunks_sched(km_ctx* ctx, KState* st, GraphKey key, bool use_grap
     KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[2], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
     KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *final_state = ctx->h_st[2];
+    return KM_OK;
+}
+
+
+// run_chunks_sched for the fused solves, whose stopping-test launch writes
+// the iteration it has reached and ``done`` into ctx's coherent host words
+// (h_flag): the host keeps at most one chunk queued beyond the iterations the
+// device has reached and stops launching as soon as it reports done, with no
+// copy or event between chunks.  Chunk lengths as run_chunks_sched.  `tail`
+// (may be null) enqueues more work after the last chunk, before the one
+// synchronisation at the end.
+template <typename F>
+inline int run_chunks_flag(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int big, int small, int expect,
+                           int max_iter, F enqueue, KState* final_state, const std::function<int()>* tail = nullptr) {
+    auto graph_for = [&](int len, int ph0, hipGraphExec_t* out) -> int {
+        GraphKey k = key;
+        k.v.push_back((uintptr_t)len);
+        k.v.push_back((uintptr_t)ph0);
+        k.v.push_back((uintptr_t)ctx->ws_gen);
+        auto it = ctx->graphs.find(k);
+        if (it != ctx->graphs.end()) {
+            *out = it->second;
+            return KM_OK;
+        }
+        hipGraph_t graph = nullptr;
+        KM_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
+        for (int c = 0; c < len; ++c) enqueue(ctx->cap_stream, (ph0 + c) % KM_RING);
+        hipError_t le = hipGetLastError();
+        hipError_t ce = hipStreamEndCapture(ctx->cap_stream, &graph);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return km_fail(ctx, KM_EHIP, "graph capture failed: %s / %s", hipGetErrorString(le), hipGetErrorString(ce));
+        }
+        hipGraphExec_t exec = nullptr;
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return km_fail(ctx, KM_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        ctx->graphs[k] = exec;
+        *out = exec;
+        return KM_OK;
+    };
+    volatile int* hf = ctx->h_flag;
+    hf[0] = 0;
+    hf[1] = -1;
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    int64_t issued = 0;
+    const int64_t cap = (int64_t)max_iter + 2 * big;   // the device's own cap ends the solve before this
+    int seen = -1;
+    auto t_seen = std::chrono::steady_clock::now();
+    int spins = 0;
+    while (!hf[0]) {
+        const int reached = hf[1];
+        const int ph0 = (int)(issued % KM_RING);
+        const bool near = expect > 0 && issued + big > (int64_t)expect - 2;
+        const int len = (near || ph0 != 0) ? small : big;
+        if (issued < cap && issued - (reached + 1) <= len) {
+            if (use_graph) {
+                hipGraphExec_t exec = nullptr;
+                if (int rc = graph_for(len, ph0, &exec)) return rc;
+                KM_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+            } else {
+                for (int c = 0; c < len; ++c) enqueue(ctx->stream, (ph0 + c) % KM_RING);
+                KM_CHECK_LAUNCH(ctx);
+            }
+            issued += len;
+            continue;
+        }
+        // waiting: a launch error or a stream that stops making progress ends the solve
+        if (reached != seen) {
+            seen = reached;
+            t_seen = std::chrono::steady_clock::now();
+        } else if (++spins % 4096 == 0) {
+            const hipError_t q = hipStreamQuery(ctx->stream);
+            if (q == hipSuccess && !hf[0]) {
+                // the queue drained without a reported stop: read the state itself
+                break;
+            }
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return km_fail(ctx, KM_EHIP, "solver stream: %s", hipGetErrorString(q));
+            if (std::chrono::steady_clock::now() - t_seen > std::chrono::seconds(120))
+                return km_fail(ctx, KM_EHIP, "solver made no progress for 120 s");
+        }
+        std::this_thread::yield();
+    }
+    KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[2], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
+    if (tail)
+        if (int rc = (*tail)()) return rc;
+    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *final_state = ctx->h_st[2];
+    if (!final_state->done) return km_fail(ctx, KM_EHIP, "solver stopped without reporting completion");
     return KM_OK;
 }
 

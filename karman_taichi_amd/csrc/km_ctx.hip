@@ -84,6 +84,16 @@ int km_create(km_ctx** out, int device, void* hip_stream) {
         delete ctx;
         return KM_ENOMEM;
     }
+    void* flag = nullptr;
+    if (hipHostMalloc(&flag, 64 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->d_flag), flag, 0) != hipSuccess) {
+        if (flag) (void)hipHostFree(flag);
+        (void)hipHostFree(ctx->h_st);
+        (void)hipHostFree(ctx->h_res);
+        delete ctx;
+        return KM_ENOMEM;
+    }
+    ctx->h_flag = static_cast<volatile int*>(flag);
     if (hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev[0], hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev[1], hipEventDisableTiming) != hipSuccess) {
@@ -105,6 +115,7 @@ int km_destroy(km_ctx* ctx) {
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->h_st) (void)hipHostFree(ctx->h_st);
     if (ctx->h_res) (void)hipHostFree(ctx->h_res);
+    if (ctx->h_flag) (void)hipHostFree(const_cast<int*>(ctx->h_flag));
     if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

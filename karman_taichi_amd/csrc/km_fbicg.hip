@@ -408,10 +408,12 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         KState st;
         T* xT = reinterpret_cast<T*>(x);
         const T* bT = reinterpret_cast<const T*>(b);
+        // the true residual is enqueued behind the solve and read with its state
+        const std::function<int()> tail = [&]() { return km_residual_async(ctx, NX, NY, coef, b, x, r64); };
         const int rc = smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter, use_graph,
-                                                          &st)
+                                                          &st, &tail)
                               : kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter, use_graph,
-                                                         &st);
+                                                         &st, &tail);
         if (rc) return rc;
         bb = st.r0;
         bnorm = sqrt(bb);
@@ -431,7 +433,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         }
         total = st.iter;
         sweeps = 1;
-        if (int rc2 = residual64(ctx, NX, NY, coef, b, x, r64, &rr)) return rc2;
+        rr = ctx->h_res[0];
         rel = sqrt(rr) / bnorm;
         if (!(rel < 0.5)) {
             // the sweep stalled or diverged: back to x = 0, plain BiCGSTAB finishes

@@ -1153,6 +1153,19 @@ int km_residual(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     return residual_norm2(ctx, tl, coef, b, x, r, rnorm2 != nullptr, rnorm2 ? rnorm2 : &dummy);
 }
 
+}  // extern "C"
+
+int km_residual_async(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, const double* x, double* r) {
+    Tile tl;
+    if (int rc = get_tile(ctx, NX, NY, KM_F64, &tl)) return rc;
+    double unused = 0.0;
+    if (int rc = residual_norm2(ctx, tl, coef, b, x, r, false, &unused)) return rc;
+    KM_HIP(ctx, hipMemcpyAsync(ctx->h_res, ctx->dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return KM_OK;
+}
+
+extern "C" {
+
 int km_bicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
                 int max_iter, int flags, km_solve_info* info) {
     if (!ctx) return KM_EINVAL;

@@ -148,6 +148,7 @@ struct TlArgs {
     int Gin[2];
     int tot;
     KState* stw;
+    int* flag;               // the host's progress words (cv_scalars)
 };
 
 // x = 0, r = r~ = p = rhs, partials of <r,r> on the flat workgroups, the state
@@ -206,9 +207,11 @@ k_tl_cinit(int m, int S, const double* __restrict__ yc, double* __restrict__ Y, 
 
 // The stopping test of iteration ph on <r,r> and alpha = rho / <r~,v>
 // (bicgsolver.py:108,130) with the stall test of the flexible solve; every
-// workgroup derives the same, the first one writes the state.  false: the
-// solve has ended (``done`` set).
-__device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, double rvv, double rr, double* alpha) {
+// workgroup derives the same, the first one writes the state and, to the
+// host's words `flag` (may be null), the iteration reached and ``done``.
+// false: the solve has ended (``done`` set).
+__device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, double rvv, double rr, double* alpha,
+                                           int* flag) {
     const bool lead0 = blockIdx.x == 0 && threadIdx.x == 0;
     const KScal sp = st->ring[ph];
     const KScal spp = st->ring[(ph + KM_RING - 1) % KM_RING];
@@ -229,9 +232,16 @@ __device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, doub
     if (status < 0 && !better && i - spp.best_it >= kStallWindow) status = KM_STALL;
     if (lead0 && i == 0) st->r0 = r0;
     if (status >= 0) {
-        if (lead0) finish_state(st, status, i, rr);
+        if (lead0) {
+            finish_state(st, status, i, rr);
+            if (flag) {
+                __hip_atomic_store(flag + 1, i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
         return false;
     }
+    if (lead0 && flag) __hip_atomic_store(flag + 1, i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     *alpha = rho / rvv;   // bicgsolver.py:108
     if (lead0) {
         st->ring[ph].alpha = *alpha;
@@ -286,7 +296,7 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
             rvv = t2[0];
             rr = t2[1];
         }
-        if (!cv_scalars(g.stw, g.ph, rvv, rr, &alpha)) return;
+        if (!cv_scalars(g.stw, g.ph, rvv, rr, &alpha, g.flag)) return;
     }
     const T fa = (T)alpha;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -570,7 +580,7 @@ __device__ __forceinline__ double row_dot(const double* __restrict__ a, int m, i
 // alpha Y(v) on coarse row blockIdx.x * 4 + wave.  Y holds Y(r), Y(p), Y(v), Y(s).
 __global__ void __launch_bounds__(KM_BLOCK)
 k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict__ inv, const double* __restrict__ rcs,
-        double* __restrict__ Y, KState* __restrict__ st, const double* __restrict__ P) {
+        double* __restrict__ Y, KState* __restrict__ st, const double* __restrict__ P, int* flag) {
     extern __shared__ double rs_[];
     if (st->done) return;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -596,7 +606,7 @@ k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict_
         rr = t2[1];
     }
     double alpha;
-    if (!cv_scalars(st, ph, rvv, rr, &alpha)) return;
+    if (!cv_scalars(st, ph, rvv, rr, &alpha, flag)) return;
     __syncthreads();   // rs_
     if (!live) return;
     const double s = row_dot(a, m, lane, av, rs_);
@@ -819,9 +829,11 @@ struct Fused {
     km_coarse* C;
     KState* st;
     double* P;
+    int* flag;
 
     Fused(km_ctx* ctx, int NX_, int NY_, const T* coef, T* ws, T* xd, int tile_)
-        : NX(NX_), NY(NY_), tile(tile_), x(xd), coefT(coef), C(ctx->coarse), st(&ctx->st[4]), P(ctx->partials) {
+        : NX(NX_), NY(NY_), tile(tile_), x(xd), coefT(coef), C(ctx->coarse), st(&ctx->st[4]), P(ctx->partials),
+          flag(ctx->d_flag) {
         const int64_t n = (int64_t)NX * NY;
         r = ws;
         rt = ws + n;
@@ -888,7 +900,7 @@ struct Fused {
     }
     void cv(hipStream_t q, int phs) const {
         hipLaunchKernelGGL(k_tl_cv, dim3(Gc), dim3(KM_BLOCK), (size_t)C->m * sizeof(double), q, C->m, S, Gt, fl.G,
-                           fin, phs, (const double*)C->inv, (const double*)rcv, Y, st, (const double*)P);
+                           fin, phs, (const double*)C->inv, (const double*)rcv, Y, st, (const double*)P, flag);
     }
     void ct(hipStream_t q, int phs) const {
         hipLaunchKernelGGL(k_tl_ct<T>, dim3(Gc + fl.G), dim3(KM_BLOCK), (size_t)C->m * sizeof(double), q, C->m, S, Gt,
@@ -925,7 +937,7 @@ int fused_start(km_ctx* ctx, const Fused<T>& f, const T* rhs, double eps, int ma
 
 template <typename T>
 int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
-                bool use_graph, KState* out) {
+                bool use_graph, KState* out, const std::function<int()>* tail) {
     km_coarse* C = ctx->coarse;
     KM_REQUIRE(ctx, C && C->arena && C->NX == NX && C->NY == NY, "coarse level not set up for this system");
     int tile = 0;
@@ -943,8 +955,8 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
     for (int c : C->recent)
         if (c > 0 && (expect == 0 || c < expect)) expect = c;
     if (expect > 0) expect = std::max(1, expect - C->chunk_lead + 2);
-    const int rc = run_chunks_sched(ctx, f.st, key, use_graph, C->chunk_big, C->chunk_small, expect, max_iter,
-                                    enqueue, out);
+    const int rc = run_chunks_flag(ctx, f.st, key, use_graph, C->chunk_big, C->chunk_small, expect, max_iter,
+                                   enqueue, out, tail);
     if (rc == KM_OK) {
         for (int k = 7; k > 0; --k) C->recent[k] = C->recent[k - 1];
         C->recent[0] = out->iter;
@@ -1065,9 +1077,10 @@ struct SmoothFused {
     const T* coefT;
     KState* st;
     double* P;
+    int* flag;
 
     SmoothFused(km_ctx* ctx, int NX_, int NY_, const T* coef, T* ws, T* xd)
-        : NX(NX_), NY(NY_), x(xd), coefT(coef), st(&ctx->st[4]), P(ctx->partials) {
+        : NX(NX_), NY(NY_), x(xd), coefT(coef), st(&ctx->st[4]), P(ctx->partials), flag(ctx->d_flag) {
         const int64_t n = (int64_t)NX * NY;
         r = ws;
         rt = ws + n;
@@ -1117,6 +1130,7 @@ struct SmoothFused {
         a.Gin[1] = fl.G;
         a.tot = fin;
         a.stw = st;
+        a.flag = flag;
         launch_mom<T, 2, true>(q, NX, NY, a);
         if (fin & FIN_T)
             hipLaunchKernelGGL(k_tl_total<4>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PT), Gt,
@@ -1144,7 +1158,7 @@ struct SmoothFused {
 
 template <typename T>
 int smooth_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
-                 bool use_graph, KState* out) {
+                 bool use_graph, KState* out, const std::function<int()>* tail) {
     const SmoothFused<T> f(ctx, NX, NY, coefT, ws, xd);
     hipLaunchKernelGGL(k_tl_init<T>, dim3(f.fl.G), dim3(KM_BLOCK), 0, ctx->stream, f.fl, rhs, f.x, f.r, f.rt, f.p, eps,
                        max_iter, f.st, f.P);
@@ -1155,20 +1169,20 @@ int smooth_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* x
     key.v = {11, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
              (uintptr_t)ws, (uintptr_t)f.st, (uintptr_t)f.fin};
     // these solves take a handful of iterations: short graphs throughout
-    return run_chunks_sched(ctx, f.st, key, use_graph, KM_RING, 2, 1, max_iter, enqueue, out);
+    return run_chunks_flag(ctx, f.st, key, use_graph, KM_RING, 2, 1, max_iter, enqueue, out, tail);
 }
 
 template int smooth_inner<float>(km_ctx*, int, int, const float*, const float*, float*, float*, double, int, bool,
-                                 KState*);
+                                 KState*, const std::function<int()>*);
 template int smooth_inner<double>(km_ctx*, int, int, const double*, const double*, double*, double*, double, int,
-                                  bool, KState*);
+                                  bool, KState*, const std::function<int()>*);
 
 template int smooth<float>(km_ctx*, const float*, const float*, float*, hipStream_t);
 template int smooth<double>(km_ctx*, const double*, const double*, double*, hipStream_t);
 template int fused_inner<float>(km_ctx*, int, int, const float*, const float*, float*, float*, double, int, bool,
-                                KState*);
+                                KState*, const std::function<int()>*);
 template int fused_inner<double>(km_ctx*, int, int, const double*, const double*, double*, double*, double, int,
-                                 bool, KState*);
+                                 bool, KState*, const std::function<int()>*);
 
 }  // namespace kmcoarse
 
