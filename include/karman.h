@@ -183,6 +183,16 @@ int km_adam_step(km_ctx* ctx, int64_t n, float* params, const float* grad, float
  * FP64 solves from x = 0 with either fused iteration solve straight into x. */
 int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
                  int max_iter, int flags, const float* params, km_solve_info* info);
+/* The u and v momentum solves of one SIMPLE step (simple.py:247-248, which
+ * calls BICGSolver.solve on each in turn): two independent km_fbicgstab
+ * solves with the same eps, max_iter and flags, results and info as two
+ * separate calls.  With KM_FLAG_SMOOTH and none of KM_FLAG_MIXED, KM_FLAG_X0,
+ * KM_FLAG_COARSE the two systems' first sweeps iterate in the same launches
+ * (each workgroup serves one system; each system stops on its own test);
+ * otherwise the two solves run one after the other. */
+int km_fbicgstab2(km_ctx* ctx, int NX0, int NY0, const double* coef0, const double* b0, double* x0, int NX1, int NY1,
+                  const double* coef1, const double* b1, double* x1, double eps, int max_iter, int flags,
+                  km_solve_info* info0, km_solve_info* info1);
 
 /* ---- batched cases (no counterpart in the reference, which runs one case) ----
  * Many independent channel cases of uneven size and Reynolds number advance

@@ -99,6 +99,13 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
 template <typename T>
 int smooth_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
                  bool use_graph, KState* out, const std::function<int()>* tail = nullptr);
+// Two independent systems of smooth_inner (arrays of two: shapes, operands,
+// 8n-value workspaces) iterated in the same launches until both stop; out
+// receives both states, tail runs after the last chunk.
+template <typename T>
+int smooth_inner2(km_ctx* ctx, const int* NX, const int* NY, const T* const* coefT, const T* const* rhs, T* const* xd,
+                  T* const* ws, double eps, const int* max_iter, bool use_graph, KState* out,
+                  const std::function<int()>* tail = nullptr);
 
 // Tile shape (rows, columns) of the fused smoothing kernels chosen for the
 // coarse level's system in FP64 (32 x 32 before the first fused solve).

@@ -168,9 +168,10 @@ int km_ensure_workspace(km_ctx* ctx, size_t vec_bytes);
 void* km_ws_vectors(km_ctx* ctx);
 int km_choose_tile(km_ctx* ctx, int NX, int NY, int prec, int* TI, int* TJ);
 void km_drop_graphs(km_ctx* ctx);
-// r = b - A x and ||r||^2 into ctx->h_res[0] once the stream reaches it (no
-// synchronisation; km_krylov.hip)
-int km_residual_async(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, const double* x, double* r);
+// r = b - A x and ||r||^2 into ctx->h_res[slot] once the stream reaches it
+// (no synchronisation; km_krylov.hip)
+int km_residual_async(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, const double* x, double* r,
+                      int slot = 0);
 
 // Poll loop: enqueue chunks of iterations (a captured graph, or direct
 // launches), two chunks in flight, until the device sets ``done``.  The chunk
@@ -311,10 +312,13 @@ inline int run_chunks_sched(km_ctx* ctx, KState* st, GraphKey key, bool use_grap
 // device has reached and stops launching as soon as it reports done, with no
 // copy or event between chunks.  Chunk lengths as run_chunks_sched.  `tail`
 // (may be null) enqueues more work after the last chunk, before the one
-// synchronisation at the end.
+// synchronisation at the end.  nsys independent systems iterate in the same
+// launches (states st[0 .. nsys), host words 2s, 2s + 1): the solve ends when
+// all are done, and the slowest one still running sets the pace.
 template <typename F>
 inline int run_chunks_flag(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int big, int small, int expect,
-                           int max_iter, F enqueue, KState* final_state, const std::function<int()>* tail = nullptr) {
+                           int max_iter, F enqueue, KState* final_state, const std::function<int()>* tail = nullptr,
+                           int nsys = 1) {
     auto graph_for = [&](int len, int ph0, hipGraphExec_t* out) -> int {
         GraphKey k = key;
         k.v.push_back((uintptr_t)len);
@@ -343,16 +347,29 @@ inline int run_chunks_flag(km_ctx* ctx, KState* st, GraphKey key, bool use_graph
         return KM_OK;
     };
     volatile int* hf = ctx->h_flag;
-    hf[0] = 0;
-    hf[1] = -1;
+    for (int q = 0; q < nsys; ++q) {
+        hf[2 * q] = 0;
+        hf[2 * q + 1] = -1;
+    }
     std::atomic_thread_fence(std::memory_order_seq_cst);
+    auto all_done = [&]() {
+        for (int q = 0; q < nsys; ++q)
+            if (!hf[2 * q]) return false;
+        return true;
+    };
+    auto slowest = [&]() {
+        int m = 1 << 30;
+        for (int q = 0; q < nsys; ++q)
+            if (!hf[2 * q] && hf[2 * q + 1] < m) m = hf[2 * q + 1];
+        return m == (1 << 30) ? -1 : m;
+    };
     int64_t issued = 0;
     const int64_t cap = (int64_t)max_iter + 2 * big;   // the device's own cap ends the solve before this
     int seen = -1;
     auto t_seen = std::chrono::steady_clock::now();
     int spins = 0;
-    while (!hf[0]) {
-        const int reached = hf[1];
+    while (!all_done()) {
+        const int reached = slowest();
         const int ph0 = (int)(issued % KM_RING);
         const bool near = expect > 0 && issued + big > (int64_t)expect - 2;
         const int len = (near || ph0 != 0) ? small : big;
@@ -374,7 +391,7 @@ inline int run_chunks_flag(km_ctx* ctx, KState* st, GraphKey key, bool use_graph
             t_seen = std::chrono::steady_clock::now();
         } else if (++spins % 4096 == 0) {
             const hipError_t q = hipStreamQuery(ctx->stream);
-            if (q == hipSuccess && !hf[0]) {
+            if (q == hipSuccess && !all_done()) {
                 // the queue drained without a reported stop: read the state itself
                 break;
             }
@@ -385,12 +402,14 @@ inline int run_chunks_flag(km_ctx* ctx, KState* st, GraphKey key, bool use_graph
         }
         std::this_thread::yield();
     }
-    KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[2], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[2], st, nsys * sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
     if (tail)
         if (int rc = (*tail)()) return rc;
     KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    *final_state = ctx->h_st[2];
-    if (!final_state->done) return km_fail(ctx, KM_EHIP, "solver stopped without reporting completion");
+    for (int q = 0; q < nsys; ++q) {
+        final_state[q] = ctx->h_st[2 + q];
+        if (!final_state[q].done) return km_fail(ctx, KM_EHIP, "solver stopped without reporting completion");
+    }
     return KM_OK;
 }
 

@@ -356,22 +356,33 @@ int residual64(km_ctx* ctx, int NX, int NY, const double* coef, const double* b,
     return rc < 0 ? rc : KM_OK;
 }
 
-// Outer FP64 refinement around inner flexible solves in T.  FP64 inner
-// vectors aim each sweep at the requested tolerance; FP32 inner vectors run
-// sweeps to at most 1e-5 each (what FP32 recurrences reliably reach, at most
-// 500 iterations).  A sweep also ends when its recursive residual stalls.
+// Workspace bytes of fbicg_solve<T> on n cells.
 template <typename T>
-int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
-                int max_iter, int flags, const float* params, km_solve_info* info) {
-    const int64_t n = (int64_t)NX * NY;
-    const bool use_graph = !(flags & KM_FLAG_NO_GRAPH);
+size_t fbicg_bytes(int64_t n) {
     const bool f64 = sizeof(T) == sizeof(double);
     // FP64 r (n); FP32 coefficients (5n); T: coefficients unless FP64 (5n), rhs (n), d (n), inner (8n).
     // At least what the plain fallback needs, so the arena is not regrown mid-solve.
     size_t bytes = (size_t)n * sizeof(double) + (size_t)5 * n * sizeof(float) +
                    (size_t)(f64 ? 12 : 17) * n * sizeof(T) + 2048;
     if (bytes < (size_t)9 * n * sizeof(double)) bytes = (size_t)9 * n * sizeof(double);
-    if (int rc = km_ensure_workspace(ctx, bytes)) return rc;
+    return bytes;
+}
+
+// Outer FP64 refinement around inner flexible solves in T.  FP64 inner
+// vectors aim each sweep at the requested tolerance; FP32 inner vectors run
+// sweeps to at most 1e-5 each (what FP32 recurrences reliably reach, at most
+// 500 iterations).  A sweep also ends when its recursive residual stalls.
+// pre (may be null): the state and true <r,r> of a first sweep already run
+// into x (the direct case below, by the pair solve); the sweep is then not
+// run again.
+template <typename T>
+int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
+                int max_iter, int flags, const float* params, km_solve_info* info, const KState* pre = nullptr,
+                double pre_rr = 0.0) {
+    const int64_t n = (int64_t)NX * NY;
+    const bool use_graph = !(flags & KM_FLAG_NO_GRAPH);
+    const bool f64 = sizeof(T) == sizeof(double);
+    if (int rc = km_ensure_workspace(ctx, fbicg_bytes<T>(n))) return rc;
     double* r64 = static_cast<double*>(km_ws_vectors(ctx));
     float* c32 = reinterpret_cast<float*>(r64 + n);
     // T vectors start at the next 256-byte boundary after the FP32 coefficients
@@ -387,8 +398,12 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     }
     T* d = rhs + n;
     T* wT = d + n;
-    hipLaunchKernelGGL(k_scale_to<float>, dim3(blocks(5 * n)), dim3(KM_BLOCK), 0, ctx->stream, 5 * n, coef, c32, 1.0);
-    KM_CHECK_LAUNCH(ctx);
+    // FP32 coefficients: the network's features and the FP32 vectors' matvecs
+    if (params || !f64) {
+        hipLaunchKernelGGL(k_scale_to<float>, dim3(blocks(5 * n)), dim3(KM_BLOCK), 0, ctx->stream, 5 * n, coef, c32,
+                           1.0);
+        KM_CHECK_LAUNCH(ctx);
+    }
     const bool coarse = (flags & KM_FLAG_COARSE) != 0;
     const bool smooth = (flags & KM_FLAG_SMOOTH) != 0;
     KM_REQUIRE(ctx, !(smooth && (coarse || params)), "KM_FLAG_SMOOTH takes neither a coarse level nor a network");
@@ -404,17 +419,23 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     // A x = b straight into x (x = 0 is its start); b.b is the solve's own
     // first <r,r>, so no round trip to the host precedes it
     const bool direct = f64 && !params && !(flags & KM_FLAG_X0) && (coarse || smooth);
+    KM_REQUIRE(ctx, direct || !pre, "a first sweep was run for a solve that takes none");
     if (direct) {
         KState st;
-        T* xT = reinterpret_cast<T*>(x);
-        const T* bT = reinterpret_cast<const T*>(b);
-        // the true residual is enqueued behind the solve and read with its state
-        const std::function<int()> tail = [&]() { return km_residual_async(ctx, NX, NY, coef, b, x, r64); };
-        const int rc = smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter, use_graph,
-                                                          &st, &tail)
-                              : kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter, use_graph,
-                                                         &st, &tail);
-        if (rc) return rc;
+        if (pre) {
+            st = *pre;
+            ctx->h_res[0] = pre_rr;
+        } else {
+            T* xT = reinterpret_cast<T*>(x);
+            const T* bT = reinterpret_cast<const T*>(b);
+            // the true residual is enqueued behind the solve and read with its state
+            const std::function<int()> tail = [&]() { return km_residual_async(ctx, NX, NY, coef, b, x, r64); };
+            const int rc = smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter,
+                                                              use_graph, &st, &tail)
+                                  : kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter,
+                                                             use_graph, &st, &tail);
+            if (rc) return rc;
+        }
         bb = st.r0;
         bnorm = sqrt(bb);
         if (info) {
@@ -440,6 +461,10 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
             KM_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
             status = KM_STALL;
             rel = 1.0;
+        } else if (pre && !(rel < eps)) {
+            // more sweeps follow: they start from r = b - A x in r64
+            if (int rc = residual64(ctx, NX, NY, coef, b, x, r64, &rr)) return rc;
+            rel = sqrt(rr) / bnorm;
         }
     } else {
         if (int rc = km_dot(ctx, n, KM_F64, b, b, &bb)) return rc < 0 ? rc : KM_EHIP;
@@ -580,6 +605,59 @@ int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* 
     if (max_iter <= 0) max_iter = (int)(n < (1 << 30) ? n : (1 << 30));
     if (flags & KM_FLAG_MIXED) return fbicg_solve<float>(ctx, NX, NY, coef, b, x, eps, max_iter, flags, params, info);
     return fbicg_solve<double>(ctx, NX, NY, coef, b, x, eps, max_iter, flags, params, info);
+}
+
+int km_fbicgstab2(km_ctx* ctx, int NX0, int NY0, const double* coef0, const double* b0, double* x0, int NX1, int NY1,
+                  const double* coef1, const double* b1, double* x1, double eps, int max_iter, int flags,
+                  km_solve_info* info0, km_solve_info* info1) {
+    if (!ctx) return KM_EINVAL;
+    KM_REQUIRE(ctx, coef0 && b0 && x0 && coef1 && b1 && x1, "null pointer");
+    KM_REQUIRE(ctx, x0 != x1, "the two systems share their solution vector");
+    const int NX[2] = {NX0, NX1}, NY[2] = {NY0, NY1};
+    int64_t n[2];
+    int mi[2];
+    for (int q = 0; q < 2; ++q) {
+        KM_REQUIRE(ctx, NX[q] >= 4 && NY[q] >= 4 && (int64_t)NX[q] * NY[q] < ((int64_t)1 << 31),
+                   "system %d shape %d x %d", q, NX[q], NY[q]);
+        n[q] = (int64_t)NX[q] * NY[q];
+        mi[q] = max_iter > 0 ? max_iter : (int)(n[q] < (1 << 30) ? n[q] : (1 << 30));
+    }
+    KM_REQUIRE(ctx, eps > 0.0, "eps must be positive");
+    const bool pair = (flags & KM_FLAG_SMOOTH) && !(flags & (KM_FLAG_MIXED | KM_FLAG_X0 | KM_FLAG_COARSE));
+    if (!pair) {
+        if (int rc = km_fbicgstab(ctx, NX0, NY0, coef0, b0, x0, eps, max_iter, flags, nullptr, info0)) return rc;
+        return km_fbicgstab(ctx, NX1, NY1, coef1, b1, x1, eps, max_iter, flags, nullptr, info1);
+    }
+    // FP64 true residuals r (n0, n1), then the two inner workspaces (8n each)
+    auto up = [](int64_t v) { return (size_t)((v * (int64_t)sizeof(double) + 255) & ~(int64_t)255); };
+    size_t bytes = up(n[0]) + up(n[1]) + up(8 * n[0]) + up(8 * n[1]);
+    bytes = std::max(bytes, std::max(fbicg_bytes<double>(n[0]), fbicg_bytes<double>(n[1])));
+    if (int rc = km_ensure_workspace(ctx, bytes)) return rc;
+    char* V = static_cast<char*>(km_ws_vectors(ctx));
+    double* r[2];
+    double* w[2];
+    r[0] = reinterpret_cast<double*>(V);
+    r[1] = reinterpret_cast<double*>(V + up(n[0]));
+    w[0] = reinterpret_cast<double*>(V + up(n[0]) + up(n[1]));
+    w[1] = reinterpret_cast<double*>(V + up(n[0]) + up(n[1]) + up(8 * n[0]));
+    const double* coef[2] = {coef0, coef1};
+    const double* b[2] = {b0, b1};
+    double* x[2] = {x0, x1};
+    KState st[2];
+    // both true residuals are enqueued behind the solve and read with its states
+    const std::function<int()> tail = [&]() {
+        for (int q = 0; q < 2; ++q)
+            if (int rc = km_residual_async(ctx, NX[q], NY[q], coef[q], b[q], x[q], r[q], q)) return rc;
+        return (int)KM_OK;
+    };
+    if (int rc = kmcoarse::smooth_inner2<double>(ctx, NX, NY, coef, b, x, w, 0.5 * eps, mi,
+                                                 !(flags & KM_FLAG_NO_GRAPH), st, &tail))
+        return rc;
+    const double rr1 = ctx->h_res[1];   // the first system's continuation may reuse h_res
+    if (int rc = fbicg_solve<double>(ctx, NX0, NY0, coef0, b0, x0, eps, mi[0], flags, nullptr, info0, &st[0],
+                                     ctx->h_res[0]))
+        return rc;
+    return fbicg_solve<double>(ctx, NX1, NY1, coef1, b1, x1, eps, mi[1], flags, nullptr, info1, &st[1], rr1);
 }
 
 }  // extern "C"

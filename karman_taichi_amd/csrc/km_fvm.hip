@@ -197,20 +197,34 @@ k_blend(GridD g, double* __restrict__ u, double* __restrict__ v,
 
 // Zero-gradient outlet and flux fix (karman_taichi.py:180-183, :586-614).
 // One workgroup; the column sums run sequentially in one lane so that they
-// are the same sums, in the same order, as the reference's serial loops.
+// are the same sums, in the same order, as the reference's serial loops
+// (the columns are staged in LDS by the whole workgroup first).
 __device__ __forceinline__ void outflow_body(const GridD& g, double* __restrict__ u) {
+    constexpr int kStageCells = 1024;
     const int NYU = g.ny + 2;
     double* uin = u + (int64_t)1 * NYU;
     double* ulast = u + (int64_t)g.nx * NYU;
     double* uout = u + (int64_t)(g.nx + 1) * NYU;
-    for (int j = 1 + threadIdx.x; j <= g.ny; j += KM_BLOCK) uout[j] = ulast[j];
     __shared__ double shift;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double fin = 0.0, fout = 0.0;
-        for (int j = 1; j <= g.ny; ++j) { fin += uin[j]; fout += uout[j]; }
-        shift = (fin - fout) / g.ny;
+    __shared__ double sin_[kStageCells], sout[kStageCells];
+    double fin = 0.0, fout = 0.0;   // thread 0's running sums
+    for (int j0 = 1; j0 <= g.ny; j0 += kStageCells) {
+        const int cnt = g.ny - j0 + 1 < kStageCells ? g.ny - j0 + 1 : kStageCells;
+        __syncthreads();   // thread 0 is done with the previous chunk
+        for (int t = threadIdx.x; t < cnt; t += KM_BLOCK) {
+            const double ul = ulast[j0 + t];
+            uout[j0 + t] = ul;   // zero gradient
+            sin_[t] = uin[j0 + t];
+            sout[t] = ul;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int t = 0; t < cnt; ++t) {
+                fin += sin_[t];
+                fout += sout[t];
+            }
     }
+    if (threadIdx.x == 0) shift = (fin - fout) / g.ny;
     __syncthreads();
     for (int j = 1 + threadIdx.x; j <= g.ny; j += KM_BLOCK) uout[j] += shift;
 }

@@ -1155,12 +1155,16 @@ int km_residual(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
 
 }  // extern "C"
 
-int km_residual_async(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, const double* x, double* r) {
+int km_residual_async(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, const double* x, double* r,
+                      int slot) {
     Tile tl;
     if (int rc = get_tile(ctx, NX, NY, KM_F64, &tl)) return rc;
-    double unused = 0.0;
-    if (int rc = residual_norm2(ctx, tl, coef, b, x, r, false, &unused)) return rc;
-    KM_HIP(ctx, hipMemcpyAsync(ctx->h_res, ctx->dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    double* P = ctx->partials + KM_PX;
+    hipLaunchKernelGGL(k_residual, dim3(tl.G), dim3(KM_BLOCK), tile_lds<double>(tl), ctx->stream, tl, coef, b, x, r, P);
+    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, tl.G, ctx->dres + slot);
+    KM_CHECK_LAUNCH(ctx);
+    KM_HIP(ctx, hipMemcpyAsync(ctx->h_res + slot, ctx->dres + slot, sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
     return KM_OK;
 }
 

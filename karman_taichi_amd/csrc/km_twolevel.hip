@@ -45,6 +45,8 @@ constexpr int TL_PV = 0;                         // [kMaxTileWg] <r~,v> (K_PV)
 constexpr int TL_PR = kMaxTileWg;                // [kMaxFlatWg] <r,r> (K_CT's flat workgroups, the start)
 constexpr int TL_PT = kMaxTileWg + kMaxFlatWg;   // [4][kMaxTileWg] <t,s>, <t,t>, <s,r~>, <t,r~> (K_ST)
 constexpr int TL_TOT = TL_PT + 4 * kMaxTileWg;   // totals: 0 <r~,v>, 1..4 the K_ST planes, 5 <r,r>
+constexpr int kPairPartials = 1 << 16;           // the second system of a pair solve: its partials' offset
+static_assert(kPairPartials >= TL_TOT + 8 && 2 * kPairPartials <= KM_MAX_PARTIALS, "pair partials");
 // The coarse (and flat) workgroups sum the tile partials; when partials x
 // consumer workgroups exceeds this, one workgroup sums them first
 // (k_tl_total) and the consumers read the totals.
@@ -207,12 +209,12 @@ k_tl_cinit(int m, int S, const double* __restrict__ yc, double* __restrict__ Y, 
 
 // The stopping test of iteration ph on <r,r> and alpha = rho / <r~,v>
 // (bicgsolver.py:108,130) with the stall test of the flexible solve; every
-// workgroup derives the same, the first one writes the state and, to the
-// host's words `flag` (may be null), the iteration reached and ``done``.
+// workgroup derives the same, lead0 (one thread of the system's first
+// workgroup) writes the state and, to the host's words `flag` (may be null),
+// the iteration reached and ``done``.
 // false: the solve has ended (``done`` set).
 __device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, double rvv, double rr, double* alpha,
-                                           int* flag) {
-    const bool lead0 = blockIdx.x == 0 && threadIdx.x == 0;
+                                           int* flag, bool lead0) {
     const KScal sp = st->ring[ph];
     const KScal spp = st->ring[(ph + KM_RING - 1) % KM_RING];
     const int i = sp.iter;
@@ -266,9 +268,8 @@ __device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, doub
 // edge tiles own.  Without a coarse vector (yc null) z starts from zero;
 // without slots (rcs null) nothing is restricted.  SELF (MODE 2): the
 // launch derives alpha and the stopping test itself (cv_scalars).
-template <typename T, int TI, int TJ, int MODE, int WPE, int SW, bool SELF>
-__global__ void __launch_bounds__(KM_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
-k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
+template <typename T, int TI, int TJ, int MODE, int SW, bool SELF>
+__device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, const TlArgs<T>& g, int L) {
     constexpr int NS = 2 * SW;                                      // half-sweeps = halo
     constexpr int WZ = TJ + 2 * (NS + 1), HZ = TI + 2 * (NS + 1);   // z frame: halo NS + 1
     constexpr int WR = TJ + 2 * NS, HR = TI + 2 * NS;               // register frame: halo NS
@@ -279,7 +280,6 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
     __shared__ double sh[16];
     const KState* st = g.st;
     if (st && st->done) return;
-    const int L = xcd_remap(blockIdx.x, tl.G);
     const int NX = tl.NX, NY = tl.NY, nx = NX - 2, ny = NY - 2;
     const int64_t n = (int64_t)NX * NY;
     const T om = g.om;
@@ -296,7 +296,7 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
             rvv = t2[0];
             rr = t2[1];
         }
-        if (!cv_scalars(g.stw, g.ph, rvv, rr, &alpha, g.flag)) return;
+        if (!cv_scalars(g.stw, g.ph, rvv, rr, &alpha, g.flag, L == 0 && threadIdx.x == 0)) return;
     }
     const T fa = (T)alpha;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -480,6 +480,23 @@ k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
     }
 }
 
+template <typename T, int TI, int TJ, int MODE, int WPE, int SW, bool SELF>
+__global__ void __launch_bounds__(KM_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
+    tile_body<T, TI, TJ, MODE, SW, SELF>(tl, cg, g, xcd_remap(blockIdx.x, tl.G));
+}
+
+// Two independent systems in one launch: workgroups [0, tl0.G) take the
+// first, the rest the second (each with its own arguments and state).
+template <typename T, int TI, int TJ, int MODE, int WPE, int SW, bool SELF>
+__global__ void __launch_bounds__(KM_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_tl_tile2(ATile tl0, ATile tl1, CGeom cg, TlArgs<T> g0, TlArgs<T> g1) {
+    if ((int)blockIdx.x < tl0.G)
+        tile_body<T, TI, TJ, MODE, SW, SELF>(tl0, cg, g0, xcd_remap(blockIdx.x, tl0.G));
+    else
+        tile_body<T, TI, TJ, MODE, SW, SELF>(tl1, cg, g1, xcd_remap(blockIdx.x - tl0.G, tl1.G));
+}
+
 // One tile launch of the two-level smoother (kmcoarse::kSweeps red-black
 // sweeps) with tile shape `which` (an index into kTileI / kTileJ).
 template <typename T, int MODE>
@@ -606,7 +623,7 @@ k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict_
         rr = t2[1];
     }
     double alpha;
-    if (!cv_scalars(st, ph, rvv, rr, &alpha, flag)) return;
+    if (!cv_scalars(st, ph, rvv, rr, &alpha, flag, blockIdx.x == 0 && threadIdx.x == 0)) return;
     __syncthreads();   // rs_
     if (!live) return;
     const double s = row_dot(a, m, lane, av, rs_);
@@ -614,6 +631,75 @@ k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict_
         Y[2 * m + row] = s;                  // Y(v)
         Y[3 * m + row] = yr - alpha * s;     // Y(s)
     }
+}
+
+// omega, rho, beta of the next iteration from K_ST's partials (bicgsolver.py:
+// 95,102,118; every workgroup derives the same, `lead` writes the next ring
+// slot) and this iteration's alpha.
+__device__ __forceinline__ void ct_scalars(const double* __restrict__ P, int Gt, int tot, KState* __restrict__ st,
+                                           int ph, bool lead, double* omega_out, double* beta_out, double* alpha_out) {
+    double pt[4];
+    if (tot & FIN_T) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pt[q] = P[TL_TOT + 1 + q];
+    } else {
+        sum_partials_wide<4>(P + TL_PT, Gt, pt, kMaxTileWg);
+    }
+    const KScal sp = st->ring[ph];
+    const double omega = pt[1] != 0.0 ? pt[0] / pt[1] : 0.0;                  // bicgsolver.py:118
+    const double rho = pt[2] - omega * pt[3];                                 // <r_i+1, r~> (bicgsolver.py:95)
+    const double beta = omega != 0.0 ? (rho / sp.rho) * (sp.alpha / omega) : 0.0;   // bicgsolver.py:102
+    if (lead) {
+        KScal& nxt = st->ring[(ph + 1) % KM_RING];
+        nxt.rho = rho;
+        nxt.omega = omega;
+        nxt.beta = beta;
+        nxt.iter = sp.iter + 1;
+    }
+    *omega_out = omega;
+    *beta_out = beta;
+    *alpha_out = sp.alpha;
+}
+
+// x += alpha p^ + omega s^ (bicgsolver.py:128), r = s - omega t (:134),
+// p = r + beta (p - omega v) (:103) on flat chunk wf, and its <r,r> partial.
+template <typename T>
+__device__ __forceinline__ void flat_update(const FGeom& fl, int wf, double alpha, double omega, double beta,
+                                            T* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
+                                            const T* __restrict__ v, const T* __restrict__ s, const T* __restrict__ t,
+                                            const T* __restrict__ ph_, const T* __restrict__ sh_,
+                                            double* __restrict__ P, double* shr) {
+    const T fo = (T)omega, fb = (T)beta;
+    const int64_t lo = (int64_t)wf * fl.chunk;
+    const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
+    double rr = 0.0;
+    for (int64_t base = lo; base < hi; base += (int64_t)kFlatPer * KM_BLOCK) {
+        T sv[kFlatPer], tv[kFlatPer], pv[kFlatPer], vv[kFlatPer], xv[kFlatPer], hv[kFlatPer], s2v[kFlatPer];
+#pragma unroll
+        for (int c = 0; c < kFlatPer; ++c) {
+            const int64_t k = base + threadIdx.x + c * KM_BLOCK;
+            const bool ok = k < hi;
+            sv[c] = ok ? s[k] : T(0);
+            tv[c] = ok ? t[k] : T(0);
+            pv[c] = ok ? p[k] : T(0);
+            vv[c] = ok ? v[k] : T(0);
+            xv[c] = ok ? x[k] : T(0);
+            hv[c] = ok ? ph_[k] : T(0);
+            s2v[c] = ok ? sh_[k] : T(0);
+        }
+#pragma unroll
+        for (int c = 0; c < kFlatPer; ++c) {
+            const int64_t k = base + threadIdx.x + c * KM_BLOCK;
+            if (k >= hi) continue;
+            x[k] = xv[c] + (T)(alpha * (double)hv[c] + omega * (double)s2v[c]);
+            const T rn = sv[c] - fo * tv[c];
+            r[k] = rn;
+            p[k] = rn + fb * (pv[c] - fo * vv[c]);
+            rr += (double)rn * (double)rn;
+        }
+    }
+    rr = block_sum(rr, shr);
+    if (threadIdx.x == 0) P[TL_PR + wf] = rr;
 }
 
 // K_CT: omega, rho, beta of the next iteration from K_ST's partials (every
@@ -647,24 +733,8 @@ k_tl_ct(int m, int S, int Gt, int Gc, FGeom fl, int tot, int ph, const double* _
         first_row(a, m, lane, av);
         load_rc(rcs, m, S, rs_);
     }
-    double pt[4];
-    if (tot & FIN_T) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) pt[q] = P[TL_TOT + 1 + q];
-    } else {
-        sum_partials_wide<4>(P + TL_PT, Gt, pt, kMaxTileWg);
-    }
-    const KScal sp = st->ring[ph];
-    const double omega = pt[1] != 0.0 ? pt[0] / pt[1] : 0.0;                  // bicgsolver.py:118
-    const double rho = pt[2] - omega * pt[3];                                 // <r_i+1, r~> (bicgsolver.py:95)
-    const double beta = omega != 0.0 ? (rho / sp.rho) * (sp.alpha / omega) : 0.0;   // bicgsolver.py:102
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        KScal& nxt = st->ring[(ph + 1) % KM_RING];
-        nxt.rho = rho;
-        nxt.omega = omega;
-        nxt.beta = beta;
-        nxt.iter = sp.iter + 1;
-    }
+    double omega, beta, alpha;
+    ct_scalars(P, Gt, tot, st, ph, blockIdx.x == 0 && threadIdx.x == 0, &omega, &beta, &alpha);
     if (coarse) {
         __syncthreads();   // rs_
         if (!live) return;
@@ -676,39 +746,32 @@ k_tl_ct(int m, int S, int Gt, int Gc, FGeom fl, int tot, int ph, const double* _
         }
         return;
     }
-    const double alpha = sp.alpha;
-    const T fo = (T)omega, fb = (T)beta;
-    const int wf = blockIdx.x - Gc;
-    const int64_t lo = (int64_t)wf * fl.chunk;
-    const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
-    double rr = 0.0;
-    for (int64_t base = lo; base < hi; base += (int64_t)kFlatPer * KM_BLOCK) {
-        T sv[kFlatPer], tv[kFlatPer], pv[kFlatPer], vv[kFlatPer], xv[kFlatPer], hv[kFlatPer], s2v[kFlatPer];
-#pragma unroll
-        for (int c = 0; c < kFlatPer; ++c) {
-            const int64_t k = base + threadIdx.x + c * KM_BLOCK;
-            const bool ok = k < hi;
-            sv[c] = ok ? s[k] : T(0);
-            tv[c] = ok ? t[k] : T(0);
-            pv[c] = ok ? p[k] : T(0);
-            vv[c] = ok ? v[k] : T(0);
-            xv[c] = ok ? x[k] : T(0);
-            hv[c] = ok ? ph_[k] : T(0);
-            s2v[c] = ok ? sh_[k] : T(0);
-        }
-#pragma unroll
-        for (int c = 0; c < kFlatPer; ++c) {
-            const int64_t k = base + threadIdx.x + c * KM_BLOCK;
-            if (k >= hi) continue;
-            x[k] = xv[c] + (T)(alpha * (double)hv[c] + omega * (double)s2v[c]);
-            const T rn = sv[c] - fo * tv[c];
-            r[k] = rn;
-            p[k] = rn + fb * (pv[c] - fo * vv[c]);
-            rr += (double)rn * (double)rn;
-        }
-    }
-    rr = block_sum(rr, shr);
-    if (threadIdx.x == 0) P[TL_PR + wf] = rr;
+    flat_update<T>(fl, blockIdx.x - Gc, alpha, omega, beta, x, r, p, v, s, t, ph_, sh_, P, shr);
+}
+
+// The flat workgroups of K_CT for two independent systems in one launch
+// (workgroups [0, fl0.G) the first): each derives its system's scalars and
+// updates its chunk.  S holds both systems' arguments.
+template <typename T>
+struct FlatSys {
+    FGeom fl;
+    int Gt, tot;
+    T *x, *r, *p;
+    const T *v, *s, *t, *ph, *sh;
+    KState* st;
+    double* P;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK) k_tl_flat2(int ph, FlatSys<T> a, FlatSys<T> b) {
+    __shared__ double shr[8];
+    const bool first = (int)blockIdx.x < a.fl.G;
+    const FlatSys<T>& S = first ? a : b;
+    const int wf = first ? (int)blockIdx.x : (int)blockIdx.x - a.fl.G;
+    if (S.st->done) return;
+    double omega, beta, alpha;
+    ct_scalars(S.P, S.Gt, S.tot, S.st, ph, wf == 0 && threadIdx.x == 0, &omega, &beta, &alpha);
+    flat_update<T>(S.fl, wf, alpha, omega, beta, S.x, S.r, S.p, S.v, S.s, S.t, S.ph, S.sh, S.P, shr);
 }
 
 }  // namespace
@@ -1069,6 +1132,20 @@ void launch_mom(hipStream_t q, int NX, int NY, const TlArgs<T>& a) {
                        dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, cg, a);
 }
 
+template <typename T, int MODE, bool SELF>
+void launch_mom2(hipStream_t q, int NX0, int NY0, int NX1, int NY1, const TlArgs<T>& a0, const TlArgs<T>& a1) {
+    const ATile t0 = a_tile(NX0, NY0, kTileI[kMomTile], kTileJ[kMomTile]);
+    const ATile t1 = a_tile(NX1, NY1, kTileI[kMomTile], kTileJ[kMomTile]);
+    CGeom cg;
+    cg.B = 1;
+    cg.mJ = 1;
+    cg.m = 0;
+    cg.SJ = 1;
+    hipLaunchKernelGGL((k_tl_tile2<T, kTileI[kMomTile], kTileJ[kMomTile], MODE, kTileW[kMomTile], kMomSweeps, SELF>),
+                       dim3(t0.G + t1.G), dim3(KM_BLOCK), 0, q, t0, t1, cg, a0, a1);
+}
+
+// One system's vectors (ws: 8n values of T), state, partials and host words.
 template <typename T>
 struct SmoothFused {
     int NX, NY, Gt, fin;
@@ -1079,8 +1156,8 @@ struct SmoothFused {
     double* P;
     int* flag;
 
-    SmoothFused(km_ctx* ctx, int NX_, int NY_, const T* coef, T* ws, T* xd)
-        : NX(NX_), NY(NY_), x(xd), coefT(coef), st(&ctx->st[4]), P(ctx->partials), flag(ctx->d_flag) {
+    SmoothFused(int NX_, int NY_, const T* coef, T* ws, T* xd, KState* st_, double* P_, int* flag_)
+        : NX(NX_), NY(NY_), x(xd), coefT(coef), st(st_), P(P_), flag(flag_) {
         const int64_t n = (int64_t)NX * NY;
         r = ws;
         rt = ws + n;
@@ -1096,7 +1173,12 @@ struct SmoothFused {
         fin = ((int64_t)Gt * (Gt + fl.G) > kTotalWork ? FIN_A | FIN_R : 0) |
               ((int64_t)Gt * fl.G > kTotalWork ? FIN_T : 0);
     }
-    void pv(hipStream_t q, int phs) const {
+    void init(hipStream_t q, const T* rhs, double eps, int max_iter) const {
+        hipLaunchKernelGGL(k_tl_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, fl, rhs, x, r, rt, p, eps, max_iter, st, P);
+        total_r(q);
+    }
+    // K_PV: p^ = M(p), v = A p^, <r~,v> partials
+    TlArgs<T> pv_args(int phs) const {
         TlArgs<T> a{};
         a.c = coefT;
         a.f0 = p;
@@ -1107,12 +1189,15 @@ struct SmoothFused {
         a.P = P + TL_PV;
         a.ph = phs;
         a.om = (T)kMomOmega;
-        launch_mom<T, 1, false>(q, NX, NY, a);
+        return a;
+    }
+    void pv_total(hipStream_t q) const {
         if (fin & FIN_A)
             hipLaunchKernelGGL(k_tl_total<1>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PV), Gt,
                                kMaxTileWg, P + TL_TOT, (const KState*)st);
     }
-    void st_(hipStream_t q, int phs) const {
+    // K_ST: alpha and the stopping test, s = r - alpha v, s^ = M(s), t = A s^
+    TlArgs<T> st_args(int phs) const {
         TlArgs<T> a{};
         a.c = coefT;
         a.f0 = r;
@@ -1131,7 +1216,9 @@ struct SmoothFused {
         a.tot = fin;
         a.stw = st;
         a.flag = flag;
-        launch_mom<T, 2, true>(q, NX, NY, a);
+        return a;
+    }
+    void st_total(hipStream_t q) const {
         if (fin & FIN_T)
             hipLaunchKernelGGL(k_tl_total<4>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PT), Gt,
                                kMaxTileWg, P + TL_TOT + 1, (const KState*)st);
@@ -1141,6 +1228,23 @@ struct SmoothFused {
             hipLaunchKernelGGL(k_tl_total<1>, dim3(1), dim3(KM_BLOCK), 0, q, (const double*)(P + TL_PR), fl.G,
                                kMaxFlatWg, P + TL_TOT + 5, (const KState*)st);
     }
+    FlatSys<T> flat() const {
+        FlatSys<T> f;
+        f.fl = fl;
+        f.Gt = Gt;
+        f.tot = fin;
+        f.x = x;
+        f.r = r;
+        f.p = p;
+        f.v = v;
+        f.s = s;
+        f.t = t;
+        f.ph = ph;
+        f.sh = sh;
+        f.st = st;
+        f.P = P;
+        return f;
+    }
     void ct(hipStream_t q, int phs) const {
         hipLaunchKernelGGL(k_tl_ct<T>, dim3(fl.G), dim3(KM_BLOCK), 0, q, 0, 1, Gt, 0, fl, fin, phs,
                            (const double*)nullptr, (const double*)nullptr, (double*)nullptr, x, r, p, (const T*)v,
@@ -1148,22 +1252,37 @@ struct SmoothFused {
         total_r(q);
     }
     void iteration(hipStream_t q, int phs) const {
-        pv(q, phs);
-        st_(q, phs);
+        launch_mom<T, 1, false>(q, NX, NY, pv_args(phs));
+        pv_total(q);
+        launch_mom<T, 2, true>(q, NX, NY, st_args(phs));
+        st_total(q);
         ct(q, phs);
     }
 };
+
+// One iteration of two independent systems in three launches: each launch
+// covers both systems' workgroups, so the pair costs one system's latency.
+template <typename T>
+void pair_iteration(const SmoothFused<T>& a, const SmoothFused<T>& b, hipStream_t q, int phs) {
+    launch_mom2<T, 1, false>(q, a.NX, a.NY, b.NX, b.NY, a.pv_args(phs), b.pv_args(phs));
+    a.pv_total(q);
+    b.pv_total(q);
+    launch_mom2<T, 2, true>(q, a.NX, a.NY, b.NX, b.NY, a.st_args(phs), b.st_args(phs));
+    a.st_total(q);
+    b.st_total(q);
+    hipLaunchKernelGGL(k_tl_flat2<T>, dim3(a.fl.G + b.fl.G), dim3(KM_BLOCK), 0, q, phs, a.flat(), b.flat());
+    a.total_r(q);
+    b.total_r(q);
+}
 
 }  // namespace
 
 template <typename T>
 int smooth_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
                  bool use_graph, KState* out, const std::function<int()>* tail) {
-    const SmoothFused<T> f(ctx, NX, NY, coefT, ws, xd);
-    hipLaunchKernelGGL(k_tl_init<T>, dim3(f.fl.G), dim3(KM_BLOCK), 0, ctx->stream, f.fl, rhs, f.x, f.r, f.rt, f.p, eps,
-                       max_iter, f.st, f.P);
+    const SmoothFused<T> f(NX, NY, coefT, ws, xd, &ctx->st[4], ctx->partials, ctx->d_flag);
+    f.init(ctx->stream, rhs, eps, max_iter);
     KM_CHECK_LAUNCH(ctx);
-    f.total_r(ctx->stream);
     auto enqueue = [&f](hipStream_t q, int phs) { f.iteration(q, phs); };
     GraphKey key;
     key.v = {11, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
@@ -1172,10 +1291,34 @@ int smooth_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* x
     return run_chunks_flag(ctx, f.st, key, use_graph, KM_RING, 2, 1, max_iter, enqueue, out, tail);
 }
 
+template <typename T>
+int smooth_inner2(km_ctx* ctx, const int* NX, const int* NY, const T* const* coefT, const T* const* rhs, T* const* xd,
+                  T* const* ws, double eps, const int* max_iter, bool use_graph, KState* out,
+                  const std::function<int()>* tail) {
+    // states st[4], st[5] (contiguous: one copy reads both), partials and
+    // host words of the second system beyond the first's
+    const SmoothFused<T> a(NX[0], NY[0], coefT[0], ws[0], xd[0], &ctx->st[4], ctx->partials, ctx->d_flag);
+    const SmoothFused<T> b(NX[1], NY[1], coefT[1], ws[1], xd[1], &ctx->st[5], ctx->partials + kPairPartials,
+                           ctx->d_flag + 2);
+    a.init(ctx->stream, rhs[0], eps, max_iter[0]);
+    b.init(ctx->stream, rhs[1], eps, max_iter[1]);
+    KM_CHECK_LAUNCH(ctx);
+    auto enqueue = [&a, &b](hipStream_t q, int phs) { pair_iteration(a, b, q, phs); };
+    GraphKey key;
+    key.v = {12, sizeof(T), (uintptr_t)NX[0], (uintptr_t)NY[0], (uintptr_t)coefT[0], (uintptr_t)rhs[0],
+             (uintptr_t)xd[0], (uintptr_t)ws[0], (uintptr_t)a.fin, (uintptr_t)NX[1], (uintptr_t)NY[1],
+             (uintptr_t)coefT[1], (uintptr_t)rhs[1], (uintptr_t)xd[1], (uintptr_t)ws[1], (uintptr_t)b.fin};
+    return run_chunks_flag(ctx, a.st, key, use_graph, KM_RING, 2, 1, std::max(max_iter[0], max_iter[1]), enqueue, out,
+                           tail, 2);
+}
+
 template int smooth_inner<float>(km_ctx*, int, int, const float*, const float*, float*, float*, double, int, bool,
                                  KState*, const std::function<int()>*);
 template int smooth_inner<double>(km_ctx*, int, int, const double*, const double*, double*, double*, double, int,
                                   bool, KState*, const std::function<int()>*);
+template int smooth_inner2<double>(km_ctx*, const int*, const int*, const double* const*, const double* const*,
+                                   double* const*, double* const*, double, const int*, bool, KState*,
+                                   const std::function<int()>*);
 
 template int smooth<float>(km_ctx*, const float*, const float*, float*, hipStream_t);
 template int smooth<double>(km_ctx*, const double*, const double*, double*, hipStream_t);

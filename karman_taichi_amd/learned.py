@@ -155,6 +155,23 @@ def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: Optiona
     return SolveInfo.from_c(info)
 
 
+
+def fbicgstab_pair(systems, eps=1e-8, max_iter=None, graph=True, smooth=True):
+    """Two independent flexible BiCGSTAB solves ``systems = ((coef, b, x), (coef, b, x))``
+    with the same settings (km_fbicgstab2): with ``smooth`` their first sweeps
+    iterate in the same launches; results as two :func:`fbicgstab` calls."""
+    from .solvers import SolveInfo
+    (c0, b0, x0), (c1, b1, x1) = systems
+    infos = (KmSolveInfo(), KmSolveInfo())
+    flags = (0 if graph else _lib.KM_FLAG_NO_GRAPH) | (_lib.KM_FLAG_SMOOTH if smooth else 0)
+    ctx = _lib.context_for(b0.device)
+    ctx.set_stream(torch.cuda.current_stream(b0.device).cuda_stream)
+    ctx.call("km_fbicgstab2", b0.shape[0], b0.shape[1], ptr(c0), ptr(b0), ptr(x0), b1.shape[0], b1.shape[1],
+             ptr(c1), ptr(b1), ptr(x1), float(eps), int(max_iter or 0), flags, ctypes.byref(infos[0]),
+             ctypes.byref(infos[1]))
+    return SolveInfo.from_c(infos[0]), SolveInfo.from_c(infos[1])
+
+
 # -------------------------------------------------------------------- data
 
 @dataclasses.dataclass

@@ -188,8 +188,16 @@ class SIMPLESolver:
             self.compute_coef_u()
             self.compute_coef_v()
             self.set_bc()
-            iu = self.u_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed, smooth=self.mom_smooth)
-            iv = self.v_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed, smooth=self.mom_smooth)
+            if self.mom_smooth and not self.mixed:
+                # both systems' sweeps in the same launches (km_fbicgstab2)
+                from .learned import fbicgstab_pair
+                us, vs = self.u_momentum_solver, self.v_momentum_solver
+                us._refresh()
+                vs._refresh()
+                iu, iv = fbicgstab_pair(((us.coef, us.b, us.x), (vs.coef, vs.b, vs.x)), eps=self.eps_mom)
+            else:
+                iu = self.u_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed, smooth=self.mom_smooth)
+                iv = self.v_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed, smooth=self.mom_smooth)
             self.update_velocity()
         return iu, iv
 

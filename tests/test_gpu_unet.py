@@ -456,6 +456,46 @@ def test_smooth_momentum_matches_oracle(which, dev):
     assert abs(info.iterations - io["iterations"]) <= 0.25 * io["iterations"] + 2
 
 
+def pair_solve(dev, systems, flags, eps=1e-8):
+    ctx = _lib.context_for(dev)
+    ts = [(T(c, dev), T(b, dev), torch.full_like(T(b, dev), 7.0)) for c, b in systems]
+    infos = (KmSolveInfo(), KmSolveInfo())
+    (c0, b0, x0), (c1, b1, x1) = ts
+    ctx.call("km_fbicgstab2", b0.shape[0], b0.shape[1], ptr(c0), ptr(b0), ptr(x0), b1.shape[0], b1.shape[1],
+             ptr(c1), ptr(b1), ptr(x1), eps, 0, flags, ctypes.byref(infos[0]), ctypes.byref(infos[1]))
+    return [t[2].cpu().numpy() for t in ts], infos
+
+
+@pytest.mark.parametrize("flags", [_lib.KM_FLAG_SMOOTH, _lib.KM_FLAG_SMOOTH | _lib.KM_FLAG_NO_GRAPH, 0])
+def test_pair_momentum_matches_separate(flags, dev):
+    """km_fbicgstab2 (the u and v solves in the same launches) against two
+    km_fbicgstab calls on the C1 momentum systems: bit-identical solutions
+    and the same iteration counts and statuses; twice, the second time with
+    the systems swapped (another tile split of each launch).  Flags 0: the
+    pair entry runs the two solves in turn."""
+    mom = c1_momentum()
+    for order in (("u", "v"), ("v", "u")):
+        systems = [mom[k] for k in order]
+        xs, infos = pair_solve(dev, systems, flags)
+        for (coef, b), x, info in zip(systems, xs, infos):
+            xr, ir = coarse_flex(dev, coef, b, None, flags)
+            assert np.array_equal(x, xr)
+            assert (info.iterations, info.status, info.fallback) == (ir.iterations, ir.status, ir.fallback)
+            assert info.status == _lib.KM_OK and info.rel_resid_true < 1e-8
+
+
+def test_pair_momentum_one_zero_rhs(dev):
+    """One system with b = 0 stops at once (x = 0) while the other iterates to
+    the tolerance; the pair solve ends when both have stopped."""
+    mom = c1_momentum()
+    cu, bu = mom["u"]
+    cv, bv = mom["v"]
+    xs, infos = pair_solve(dev, [(cu, np.zeros_like(bu)), (cv, bv)], _lib.KM_FLAG_SMOOTH)
+    assert not xs[0].any() and infos[0].iterations == 0 and infos[0].status == _lib.KM_OK
+    xr, ir = coarse_flex(dev, cv, bv, None, _lib.KM_FLAG_SMOOTH)
+    assert np.array_equal(xs[1], xr) and infos[1].iterations == ir.iterations
+
+
 def test_smooth_momentum_steps_match_plain(dev):
     """Three C2 SIMPLE steps with the smoothed momentum solves (and the
     two-level p' solve) against plain BiCGSTAB throughout, from the same start:
