@@ -92,6 +92,10 @@ def parse():
     ap.add_argument("--chunks", default="",
                     help="two-level launch schedule big,small,lead (iterations per graph far from / near the expected "
                          "end of a solve, and where near begins); default 8,2,2")
+    ap.add_argument("--staged", action="store_true",
+                    help="time the step stage by stage from Python instead of one km_simple_step call")
+    ap.add_argument("--stage-steps", type=int, default=8,
+                    help="extra stage-by-stage steps after the timed region (per-solve times)")
     ap.add_argument("--compare-reps", type=int, default=5,
                     help="repetitions of the p' solve on the final system, plain BiCGSTAB vs the mode's solver")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -142,8 +146,20 @@ def main():
     cells_p = case.shape_p[0] * case.shape_p[1]
 
     step_count = [0]
+    native = not args.staged and s._step_flags() is not None
 
-    def one_step(rec=None):
+    def one_step(rec=None, staged=not native):
+        if not staged:
+            # the whole step in one call (km_simple_step)
+            last = s.step()
+            step_count[0] += 1
+            if step_count[0] % n_outer == 0:
+                s.time_forward()
+            if rec is not None:
+                rec.append({"p_it": last["p"].iterations, "u_it": last["u"].iterations,
+                            "v_it": last["v"].iterations, "p_true": last["p"].rel_resid_true,
+                            "p_status": last["p"].status})
+            return
         t0 = time.perf_counter()
         iu, iv = s.solve_momentum_eqn(1)
         s.correct_conserv()
@@ -175,10 +191,16 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     steps_per_s = args.steps / wall
-    p_ms = [r["p_ms"] for r in rec]
     p_it = [r["p_it"] for r in rec]
-    log(f"bench: {steps_per_s:.3f} steps/s, p' median {statistics.median(p_ms):.2f} ms, "
-        f"p' iters median {statistics.median(p_it)}")
+    # per-solve host times from stage-by-stage steps after the timed region
+    srec = rec if not native else []
+    for _ in range(args.stage_steps if native else 0):
+        one_step(srec, staged=True)
+    p_ms = [r["p_ms"] for r in srec]
+    p_ms_it = [r["p_it"] for r in srec]
+    log(f"bench: {steps_per_s:.3f} steps/s ({'one call per step' if native else 'stage by stage'}), "
+        f"p' iters median {statistics.median(p_it)}"
+        + (f", staged p' median {statistics.median(p_ms):.2f} ms" if p_ms else ""))
     compare = p_compare(s, args.compare_reps) if args.compare_reps > 0 else None
 
     # ---- roofline: dominant kernel of the mode's p' iteration, timed with HIP
@@ -244,7 +266,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(case, args.cpu_steps)
 
-    total_p_it = sum(p_it)
+    total_p_it = sum(p_ms_it)
     total_p_s = sum(p_ms) / 1e3
     out = {
         "metric": "SIMPLE steps/s + p' solve time to 1e-8 rel resid: learned+FP32 vs plain, 1 GPU",
@@ -264,18 +286,21 @@ def main():
                                f"{n_outer} outer steps per time step, inner eps 1e-8",
                    "mode": args.mode, "p_solver": p_solver, "p_twolevel": twolevel, "p_network": learned,
                    "p_vectors": "f32" if fp32 else "f64", "smoothing_tile": tile, "coarse": coarse,
-                   "momentum": "fbicgstab+rbgs" if momentum == "rbgs" else "bicgstab"},
-        "p_solve_ms_median": round(statistics.median(p_ms), 3),
+                   "momentum": "fbicgstab+rbgs" if momentum == "rbgs" else "bicgstab",
+                   "step": "km_simple_step" if native else "stage by stage"},
+        "p_solve_ms_median": round(statistics.median(p_ms), 3) if p_ms else None,
         "p_iters_median": statistics.median(p_it),
         "p_iters": p_it,
         "p_ms": [round(v, 3) for v in p_ms],
-        "momentum_ms_median": round(statistics.median(r["mom_ms"] for r in rec), 3),
+        "momentum_ms_median": round(statistics.median(r["mom_ms"] for r in srec), 3) if srec else None,
+        "solve_times_from": (f"{len(srec)} stage-by-stage steps after the timed region" if native
+                             else "the timed steps"),
         "uv_iters_median": [statistics.median(r["u_it"] for r in rec), statistics.median(r["v_it"] for r in rec)],
         "p_rel_resid_true_max": max(r["p_true"] for r in rec),
         "p_status_all_ok": all(r["p_status"] == _lib.KM_OK for r in rec),
         "p_compare": compare,
-        "p_cells_iters_per_s": round(cells_p * total_p_it / total_p_s, 1),
-        "us_per_p_iter": round(total_p_s / total_p_it * 1e6, 3),
+        "p_cells_iters_per_s": round(cells_p * total_p_it / total_p_s, 1) if p_ms else None,
+        "us_per_p_iter": round(total_p_s / total_p_it * 1e6, 3) if p_ms else None,
         "kernel_us_back_to_back": {k: round(v, 3) for k, v in kern.items()},
         "kernel_us_dispatch_events": {k: round(v, 3) for k, v in kern_ev.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

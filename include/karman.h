@@ -218,6 +218,16 @@ int km_batch_step(km_ctx* ctx, int n, const km_case* cases, double eps_mom, doub
                   km_solve_info* info);
 /* time_forward of every case (karman_taichi.py:617-622). */
 int km_batch_time_forward(km_ctx* ctx, int n, const km_case* cases);
+/* SIMPLESolver.step of one case (simple.py:270-281, the outlet fix of
+ * karman_taichi.py:645-646 after the momentum blend) as one call: assembly,
+ * the u and v solves (mom_flags with KM_FLAG_SMOOTH: km_fbicgstab2, else
+ * km_bicgstab on each), blend, outlet fix, p' assembly, the p' solve
+ * (p_flags with KM_FLAG_COARSE or KM_FLAG_SMOOTH: km_fbicgstab without a
+ * network, else km_bicgstab), both corrections and mdiv.  The same kernels
+ * in the same order as those calls made one by one; info receives u, v, p'.
+ * eps and flags as the single solves (max_iter = NX * NY). */
+int km_simple_step(km_ctx* ctx, const km_case* c, double eps_mom, double eps_p, int mom_flags, int p_flags,
+                   km_solve_info* info);
 /* The batch's tile shape (chosen on its first step). */
 int km_batch_get_tile(km_ctx* ctx, int* ti, int* tj);
 

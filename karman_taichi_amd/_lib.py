@@ -91,6 +91,7 @@ SIGNATURES = {
     "km_fbicgstab": [_CTX, _I, _I, _P, _P, _P, _D, _I, _I, _P, _INFO],
     "km_fbicgstab2": [_CTX, _I, _I, _P, _P, _P, _I, _I, _P, _P, _P, _D, _I, _I, _INFO, _INFO],
     "km_batch_step": [_CTX, _I, _P, _D, _D, _I, _I, _P],
+    "km_simple_step": [_CTX, _P, _D, _D, _I, _I, _P],
     "km_batch_time_forward": [_CTX, _I, _P],
     "km_batch_get_tile": [_CTX, ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "km_set_coarse": [_CTX, _I],

@@ -30,7 +30,7 @@ import numpy as np
 import torch
 
 from . import _lib, cases
-from ._lib import KmGrid, ptr
+from ._lib import KmCase, KmGrid, KmSolveInfo, ptr
 from .solvers import BICGSolver, CGSolver, SolveInfo
 
 
@@ -97,6 +97,9 @@ class SIMPLESolver:
         self.p_precond = None         # a learned.UNetPreconditioner: flexible BiCGSTAB for p'
         self.p_coarse = False         # two-level p' preconditioner (coarse correction + network or SOR sweeps)
         self.mom_smooth = False       # momentum: flexible BiCGSTAB with a red-black Gauss-Seidel sweep
+        self.native_step = True       # step() as one km_simple_step call where the configuration allows
+        self._case = KmCase()
+        self._step_info = (KmSolveInfo * 3)()
         self.last = {}
 
     @classmethod
@@ -222,10 +225,39 @@ class SIMPLESolver:
                 t.zero_()
         self.pcor.zero_()
 
+    def _step_flags(self):
+        """(momentum, p') flags of km_simple_step, or None when the step runs
+        stage by stage (CG for p', a network preconditioner, or opted out)."""
+        if not self.native_step or self.p_solver != "bicgstab" or self.p_precond is not None:
+            return None
+        mixed = _lib.KM_FLAG_MIXED if self.mixed else 0
+        mom = mixed | (_lib.KM_FLAG_SMOOTH if self.mom_smooth else 0)
+        pf = mixed | (_lib.KM_FLAG_COARSE if self.p_coarse else 0) | (_lib.KM_FLAG_X0 if self.warm_start_p else 0)
+        return mom, pf
+
+    def _native_step(self, mom_flags, p_flags):
+        """The whole step in one call (km_simple_step): the same kernels in the
+        same order as the stage-by-stage path below."""
+        c = self._case
+        c.g = self._grid()
+        for f in ("u", "v", "p", "u0", "v0", "u_mid", "v_mid", "pcor", "mdiv", "b_u", "b_v", "b_p"):
+            setattr(c, f, getattr(self, f).data_ptr())
+        c.coef_u, c.coef_v, c.coef_p = self._coef_u.data_ptr(), self._coef_v.data_ptr(), self._coef_p.data_ptr()
+        c.mask = self.ct.data_ptr()
+        self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+        info = self._step_info
+        self.ctx.call("km_simple_step", ctypes.byref(c), float(self.eps_mom), float(self.eps_p), mom_flags, p_flags,
+                      info)
+        self.last = {k: SolveInfo.from_c(info[i]) for i, k in enumerate("uvp")}
+        return self.last
+
     def step(self):
         """One SIMPLE outer iteration (simple.py:270-281 order; the outlet fix
         sits between the momentum blend and the p' assembly as in
         karman_taichi.py:645-646)."""
+        flags = self._step_flags()
+        if flags is not None:
+            return self._native_step(*flags)
         iu, iv = self.solve_momentum_eqn(1)
         self.correct_conserv()
         ip = self.solve_pcorrection_eqn()

@@ -349,6 +349,40 @@ def test_simple_multistep_parity_c1(dev):
         assert errs[k] <= 1e-8, (k, errs[k])
 
 
+@pytest.mark.parametrize("config", ["plain", "fast", "fast_warm", "mixed_fast"])
+def test_native_step_matches_staged(config, dev):
+    """SIMPLESolver.step as one km_simple_step call against the same step
+    made stage by stage from Python: bit-identical fields and the same
+    iteration counts and statuses over three outer steps.  The coarse level
+    is factored afresh on every set-up here, so the two runs do not depend
+    on what the shared context held before."""
+    c = cases.c1() if config == "plain" else cases.c2()
+    runs = []
+    ctx = _lib.context_for(dev)
+    ctx.call("km_set_coarse_update", 0)
+    try:
+        for native in (False, True):
+            s = SIMPLESolver.from_case(c, device=dev)
+            s.init()
+            s.native_step = native
+            s.mom_smooth = s.p_coarse = config != "plain"
+            s.warm_start_p = config == "fast_warm"
+            s.mixed = config == "mixed_fast"
+            assert (s._step_flags() is not None) == native
+            infos = [s.step() for _ in range(3)]
+            runs.append((s, infos))
+    finally:
+        ctx.call("km_set_coarse_update", 8)
+    (a, ia), (b, ib) = runs
+    for f in ("u", "v", "p", "pcor", "mdiv", "u_mid", "v_mid"):
+        assert torch.equal(getattr(a, f), getattr(b, f)), f
+    for x, y in zip(ia, ib):
+        for k in "uvp":
+            assert (x[k].iterations, x[k].status) == (y[k].iterations, y[k].status), k
+            assert x[k].status == _lib.KM_OK
+    print(config, [(i["u"].iterations, i["v"].iterations, i["p"].iterations) for i in ib])
+
+
 def test_large_grid_properties(dev):
     """C5-size operator: linearity and symmetry of the assembled p' matrix,
     bitwise run-to-run repeatability of the fused reductions."""
