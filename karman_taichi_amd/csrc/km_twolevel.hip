@@ -779,10 +779,11 @@ __global__ void __launch_bounds__(KM_BLOCK) k_tl_flat2(int ph, FlatSys<T> a, Fla
 namespace kmcoarse {
 
 // Tile shape of the smoothing launches for the coarse level's system in T:
-// among the shapes with the fewest restriction slots, on the first call for a
-// (shape, precision) each runs 3 + 16 back-to-back K_PV + K_ST pairs on
-// scratch vectors (ws: its first 5n values of T are overwritten; src is read)
-// and the fastest is kept.
+// on the first call for a (shape, precision) each shape whose restriction
+// slots fit kMaxVS runs 3 + 16 back-to-back K_PV + K_ST pairs on scratch
+// vectors (ws: its first 5n values of T are overwritten; src is read) and
+// the fastest is kept (more slots only lengthen the coarse launches' slot
+// sums, m S values).
 template <typename T>
 int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
     km_coarse* C = ctx->coarse;
@@ -802,7 +803,7 @@ int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
     int best = -1;
     float best_ms = 0.f;
     for (int w = 0; w < kNumTiles; ++w) {
-        if (tile_slots(C, w) != fewest) continue;
+        if (tile_slots(C, w) > kMaxVS) continue;
         TlArgs<T> a{};   // K_PV: p^ = M(src), v
         a.c = coefT;
         a.mask = C->mask;

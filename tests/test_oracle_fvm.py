@@ -172,6 +172,11 @@ def test_pcor_matrix_invariants():
     coef, b, _ = fvm.assemble_p(c, st.u, st.v, st.coef_u[0], st.coef_v[0], m)
     A = krylov.to_csr(coef)
     assert abs(A - A.T).max() == 0.0
+    # in plane form too: aE(i,j) == aW(i+1,j), aN(i,j) == aS(i,j+1) bit for
+    # bit, zero past the last row and column
+    aw, ae, an, as_ = coef[1], coef[2], coef[3], coef[4]
+    assert np.array_equal(ae[:-1], aw[1:]) and not ae[-1].any()
+    assert np.array_equal(an[:, :-1], as_[:, 1:]) and not an[:, -1].any()
     off = coef[1:].copy()
     assert (off <= 0).all()
     live = coef[0] != 0
