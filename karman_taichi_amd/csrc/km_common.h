@@ -132,7 +132,7 @@ struct km_ctx {
     double* partials = nullptr;    // per-workgroup partial sums
     double* dres = nullptr;        // small device results
     KState* h_st = nullptr;        // pinned host mirrors of st[0] (two polling slots)
-    volatile int* h_flag = nullptr;   // coherent pinned host words the fused solves write: [0] done, [1] iteration
+    volatile int* h_flag = nullptr;   // coherent pinned host words the fused solves write (KM_HW per system)
     int* d_flag = nullptr;         // the same words as the device addresses them
     double* h_res = nullptr;       // pinned host results
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tiles;   // (NX, NY, prec) -> (TI, TJ)
@@ -168,10 +168,23 @@ int km_ensure_workspace(km_ctx* ctx, size_t vec_bytes);
 void* km_ws_vectors(km_ctx* ctx);
 int km_choose_tile(km_ctx* ctx, int NX, int NY, int prec, int* TI, int* TJ);
 void km_drop_graphs(km_ctx* ctx);
-// r = b - A x and ||r||^2 into ctx->h_res[slot] once the stream reaches it
-// (no synchronisation; km_krylov.hip)
+// The coherent host words (km_ctx::h_flag / d_flag): KM_HW ints per system
+// of a fused solve, which its stopping test writes: [0] done, [1] the
+// iteration reached (the last one once done), [2] status, [4..5] r0 and
+// [6..7] <r,r> at the stop (doubles); then, from KM_HW_RES, the doubles of
+// km_residual_async.
+#define KM_HW 16
+#define KM_HW_SYSTEMS 4
+#define KM_HW_RES (KM_HW * KM_HW_SYSTEMS)
+#define KM_HW_WORDS (KM_HW_RES + 2 * 16)
+
+// r = b - A x, with ||r||^2 written by the device into the host words'
+// result slot `slot` (read it with km_mapped_res after synchronising; km_krylov.hip)
 int km_residual_async(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, const double* x, double* r,
                       int slot = 0);
+inline double km_mapped_res(const km_ctx* ctx, int slot) {
+    return reinterpret_cast<const volatile double*>(ctx->h_flag + KM_HW_RES)[slot];
+}
 
 // Poll loop: enqueue chunks of iterations (a captured graph, or direct
 // launches), two chunks in flight, until the device sets ``done``.  The chunk
@@ -347,20 +360,21 @@ inline int run_chunks_flag(km_ctx* ctx, KState* st, GraphKey key, bool use_graph
         return KM_OK;
     };
     volatile int* hf = ctx->h_flag;
+    if (nsys < 1 || nsys > KM_HW_SYSTEMS) return km_fail(ctx, KM_EINVAL, "%d systems in one solve", nsys);
     for (int q = 0; q < nsys; ++q) {
-        hf[2 * q] = 0;
-        hf[2 * q + 1] = -1;
+        hf[KM_HW * q] = 0;
+        hf[KM_HW * q + 1] = -1;
     }
     std::atomic_thread_fence(std::memory_order_seq_cst);
     auto all_done = [&]() {
         for (int q = 0; q < nsys; ++q)
-            if (!hf[2 * q]) return false;
+            if (!hf[KM_HW * q]) return false;
         return true;
     };
     auto slowest = [&]() {
         int m = 1 << 30;
         for (int q = 0; q < nsys; ++q)
-            if (!hf[2 * q] && hf[2 * q + 1] < m) m = hf[2 * q + 1];
+            if (!hf[KM_HW * q] && hf[KM_HW * q + 1] < m) m = hf[KM_HW * q + 1];
         return m == (1 << 30) ? -1 : m;
     };
     int64_t issued = 0;
@@ -402,13 +416,22 @@ inline int run_chunks_flag(km_ctx* ctx, KState* st, GraphKey key, bool use_graph
         }
         std::this_thread::yield();
     }
-    KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[2], st, nsys * sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
+    // the final state comes from the host words the stopping test wrote
+    (void)st;
     if (tail)
         if (int rc = (*tail)()) return rc;
     KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::atomic_thread_fence(std::memory_order_acquire);
     for (int q = 0; q < nsys; ++q) {
-        final_state[q] = ctx->h_st[2 + q];
-        if (!final_state[q].done) return km_fail(ctx, KM_EHIP, "solver stopped without reporting completion");
+        volatile int* w = hf + KM_HW * q;
+        KState& f = final_state[q];
+        f = KState{};
+        f.done = w[0];
+        f.iter = w[1];
+        f.status = w[2];
+        f.r0 = *reinterpret_cast<volatile double*>(w + 4);
+        f.rTr = *reinterpret_cast<volatile double*>(w + 6);
+        if (!f.done) return km_fail(ctx, KM_EHIP, "solver stopped without reporting completion");
     }
     return KM_OK;
 }

@@ -85,7 +85,7 @@ int km_create(km_ctx** out, int device, void* hip_stream) {
         return KM_ENOMEM;
     }
     void* flag = nullptr;
-    if (hipHostMalloc(&flag, 64 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+    if (hipHostMalloc(&flag, KM_HW_WORDS * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->d_flag), flag, 0) != hipSuccess) {
         if (flag) (void)hipHostFree(flag);
         (void)hipHostFree(ctx->h_st);

@@ -422,9 +422,9 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     KM_REQUIRE(ctx, direct || !pre, "a first sweep was run for a solve that takes none");
     if (direct) {
         KState st;
+        double rr_first = pre_rr;
         if (pre) {
             st = *pre;
-            ctx->h_res[0] = pre_rr;
         } else {
             T* xT = reinterpret_cast<T*>(x);
             const T* bT = reinterpret_cast<const T*>(b);
@@ -435,6 +435,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
                                   : kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter,
                                                              use_graph, &st, &tail);
             if (rc) return rc;
+            rr_first = km_mapped_res(ctx, 0);
         }
         bb = st.r0;
         bnorm = sqrt(bb);
@@ -454,7 +455,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         }
         total = st.iter;
         sweeps = 1;
-        rr = ctx->h_res[0];
+        rr = rr_first;
         rel = sqrt(rr) / bnorm;
         if (!(rel < 0.5)) {
             // the sweep stalled or diverged: back to x = 0, plain BiCGSTAB finishes
@@ -653,9 +654,9 @@ int km_fbicgstab2(km_ctx* ctx, int NX0, int NY0, const double* coef0, const doub
     if (int rc = kmcoarse::smooth_inner2<double>(ctx, NX, NY, coef, b, x, w, 0.5 * eps, mi,
                                                  !(flags & KM_FLAG_NO_GRAPH), st, &tail))
         return rc;
-    const double rr1 = ctx->h_res[1];   // the first system's continuation may reuse h_res
+    const double rr1 = km_mapped_res(ctx, 1);   // the first system's continuation may reuse the slots
     if (int rc = fbicg_solve<double>(ctx, NX0, NY0, coef0, b0, x0, eps, mi[0], flags, nullptr, info0, &st[0],
-                                     ctx->h_res[0]))
+                                     km_mapped_res(ctx, 0)))
         return rc;
     return fbicg_solve<double>(ctx, NX1, NY1, coef1, b1, x1, eps, mi[1], flags, nullptr, info1, &st[1], rr1);
 }

@@ -1161,10 +1161,10 @@ int km_residual_async(km_ctx* ctx, int NX, int NY, const double* coef, const dou
     if (int rc = get_tile(ctx, NX, NY, KM_F64, &tl)) return rc;
     double* P = ctx->partials + KM_PX;
     hipLaunchKernelGGL(k_residual, dim3(tl.G), dim3(KM_BLOCK), tile_lds<double>(tl), ctx->stream, tl, coef, b, x, r, P);
-    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, tl.G, ctx->dres + slot);
+    KM_REQUIRE(ctx, slot >= 0 && slot < 16, "result slot %d", slot);
+    double* out = reinterpret_cast<double*>(ctx->d_flag + KM_HW_RES) + slot;   // host memory: no copy back
+    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, tl.G, out);
     KM_CHECK_LAUNCH(ctx);
-    KM_HIP(ctx, hipMemcpyAsync(ctx->h_res + slot, ctx->dres + slot, sizeof(double), hipMemcpyDeviceToHost,
-                               ctx->stream));
     return KM_OK;
 }
 

@@ -237,8 +237,14 @@ __device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, doub
         if (lead0) {
             finish_state(st, status, i, rr);
             if (flag) {
+                // the host's copy of the final state, then done (release: after the rest)
                 __hip_atomic_store(flag + 1, i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(flag + 2, status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(reinterpret_cast<double*>(flag + 4), r0, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(reinterpret_cast<double*>(flag + 6), rr, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+                __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
             }
         }
         return false;
@@ -1300,7 +1306,7 @@ int smooth_inner2(km_ctx* ctx, const int* NX, const int* NY, const T* const* coe
     // host words of the second system beyond the first's
     const SmoothFused<T> a(NX[0], NY[0], coefT[0], ws[0], xd[0], &ctx->st[4], ctx->partials, ctx->d_flag);
     const SmoothFused<T> b(NX[1], NY[1], coefT[1], ws[1], xd[1], &ctx->st[5], ctx->partials + kPairPartials,
-                           ctx->d_flag + 2);
+                           ctx->d_flag + KM_HW);
     a.init(ctx->stream, rhs[0], eps, max_iter[0]);
     b.init(ctx->stream, rhs[1], eps, max_iter[1]);
     KM_CHECK_LAUNCH(ctx);
