@@ -233,7 +233,9 @@ def main():
     if twolevel and not learned:
         ti, tj = ctypes.c_int(), ctypes.c_int()
         ctx.call("km_twolevel_tile", ctypes.byref(ti), ctypes.byref(tj))
-        tile = [ti.value, tj.value]
+        nt = ctypes.c_int()
+        ctx.call("km_twolevel_tile_threads", ctypes.byref(nt))
+        tile = [ti.value, tj.value, nt.value]
         blk, mc = ctypes.c_int(), ctypes.c_int()
         ctx.call("km_coarse_info", ctypes.byref(blk), ctypes.byref(mc))
         coarse = {"aggregate": blk.value, "unknowns": mc.value}

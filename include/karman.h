@@ -262,6 +262,13 @@ int km_twolevel_precond(km_ctx* ctx, int NX, int NY, const double* coef, const d
  * by timing on the first KM_FLAG_COARSE solve of the coarse level's system.
  * No counterpart in the reference. */
 int km_twolevel_tile(km_ctx* ctx, int* ti, int* tj);
+/* Threads per workgroup of that tile shape (256 or 512). */
+int km_twolevel_tile_threads(km_ctx* ctx, int* threads);
+/* Pin the smoothing tile of the coarse level's system to candidate `which`
+ * (0 .. 8; KM_EINVAL past the last), or clear the choice (which < 0) so the
+ * next solve times the candidates again.  Needs a coarse level.  For tests
+ * and tuning studies; no counterpart in the reference. */
+int km_twolevel_set_tile(km_ctx* ctx, int which);
 /* How the fused two-level kernels combine the previous kernel's per-workgroup
  * partial sums: every consumer workgroup sums them itself, or one workgroup
  * sums them first (an extra small launch) when partials x consumer workgroups

@@ -100,6 +100,8 @@ SIGNATURES = {
     "km_set_coarse_update": [_CTX, _I],
     "km_twolevel_precond": [_CTX, _I, _I, _P, _P, _P],
     "km_twolevel_tile": [_CTX, ctypes.POINTER(_I), ctypes.POINTER(_I)],
+    "km_twolevel_tile_threads": [_CTX, ctypes.POINTER(_I)],
+    "km_twolevel_set_tile": [_CTX, _I],
     "km_set_coarse_totals": [_CTX, _I],
     "km_set_coarse_chunks": [_CTX, _I, _I, _I],
 }

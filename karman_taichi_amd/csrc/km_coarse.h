@@ -109,7 +109,8 @@ int smooth_inner2(km_ctx* ctx, const int* NX, const int* NY, const T* const* coe
 
 // Tile shape (rows, columns) of the fused smoothing kernels chosen for the
 // coarse level's system in FP64 (32 x 32 before the first fused solve).
-int tile_info(km_ctx* ctx, int* ti, int* tj);
+int tile_info(km_ctx* ctx, int* ti, int* tj, int* threads);
+int set_tile(km_ctx* ctx, int which);
 
 // km_time_kernel for the fused kernels (which 4..7: K_PV, K_CV, K_ST, K_CT).
 int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps, bool b2b,

@@ -56,12 +56,15 @@ constexpr int kStallWindow = 300;        // as the unfused flexible solve
 constexpr int kFlatPer = 8;              // cells per thread of a flat workgroup, loads in flight
 // Tile shapes of the smoothing launches (rows x columns, columns contiguous);
 // the fastest for a system is chosen on its first solve.
-constexpr int kNumTiles = 6;
-constexpr int kTileI[kNumTiles] = {32, 16, 16, 8, 8, 16};
-constexpr int kTileJ[kNumTiles] = {32, 64, 32, 64, 128, 32};
+constexpr int kNumTiles = 9;
+constexpr int kTileI[kNumTiles] = {32, 16, 16, 8, 8, 16, 16, 32, 16};
+constexpr int kTileJ[kNumTiles] = {32, 64, 32, 64, 128, 32, 32, 32, 64};
 // minimum waves per SIMD the compiler must allow (a VGPR budget): 16 x 32 also
 // fits 128 VGPRs, so a fourth wave per SIMD can hide latency
-constexpr int kTileW[kNumTiles] = {1, 1, 3, 1, 1, 4};
+constexpr int kTileW[kNumTiles] = {1, 1, 3, 1, 1, 4, 4, 2, 2};
+// threads per workgroup: with 512 a thread holds half as many frame cells, so
+// each one's chain of loads and half-sweep updates is shorter
+constexpr int kTileT[kNumTiles] = {256, 256, 256, 256, 256, 256, 512, 512, 512};
 
 __device__ __forceinline__ bool fin(double v) { return v == v && v - v == 0.0; }
 
@@ -274,16 +277,18 @@ __device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, doub
 // edge tiles own.  Without a coarse vector (yc null) z starts from zero;
 // without slots (rcs null) nothing is restricted.  SELF (MODE 2): the
 // launch derives alpha and the stopping test itself (cv_scalars).
-template <typename T, int TI, int TJ, int MODE, int SW, bool SELF>
+template <typename T, int TI, int TJ, int MODE, int SW, bool SELF, int NT>
 __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, const TlArgs<T>& g, int L) {
+    static_assert((NT == 256 || NT == 512) && (!SELF || NT == KM_BLOCK), "4 or 8 waves; SELF sums as KM_BLOCK");
+    constexpr int NW = NT / 64;                                     // waves per workgroup
     constexpr int NS = 2 * SW;                                      // half-sweeps = halo
     constexpr int WZ = TJ + 2 * (NS + 1), HZ = TI + 2 * (NS + 1);   // z frame: halo NS + 1
     constexpr int WR = TJ + 2 * NS, HR = TI + 2 * NS;               // register frame: halo NS
     constexpr int NR = WR * HR;
-    constexpr int Q = (NR + KM_BLOCK - 1) / KM_BLOCK;   // register slots: frame cells t + q KM_BLOCK
+    constexpr int Q = (NR + NT - 1) / NT;   // register slots: frame cells t + q NT
     __shared__ T za[HZ * WZ];
     __shared__ double yb[MODE > 0 ? TI * TJ : 1];   // y on the free interior cells (tile-local)
-    __shared__ double sh[16];
+    __shared__ double sh[4 * NW];
     const KState* st = g.st;
     if (st && st->done) return;
     const int NX = tl.NX, NY = tl.NY, nx = NX - 2, ny = NY - 2;
@@ -319,7 +324,7 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
         // argument pointers may alias as far as the compiler knows): the
         // register frame, the tile's own cells and the z frame
         constexpr int NF = MODE == 2 ? 2 : 1;   // frame inputs per cell
-        constexpr int QZ = (HZ * WZ + KM_BLOCK - 1) / KM_BLOCK;
+        constexpr int QZ = (HZ * WZ + NT - 1) / NT;
         T cr[Q][5], sr[Q], rq[Q], wd[Q];
         T fv[Q][NF];
         int64_t kq[Q];
@@ -330,7 +335,7 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
         uint8_t mq[QZ];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int idx = threadIdx.x + q * KM_BLOCK;
+            const int idx = threadIdx.x + q * NT;
             const int a = idx / WR, b = idx - (idx / WR) * WR;
             const int i = ti0 - NS + a, j = tj0 - NS + b;
             in[q] = idx < NR && i >= 0 && i < NX && j >= 0 && j < NY;
@@ -352,7 +357,7 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
         // z1 = P yc on the z frame (zero off the free cells and outside the system)
 #pragma unroll
         for (int q = 0; q < QZ; ++q) {
-            const int idx = threadIdx.x + q * KM_BLOCK;
+            const int idx = threadIdx.x + q * NT;
             const int a = idx / WZ, b = idx - (idx / WZ) * WZ;
             const int i = ti0 - NS - 1 + a, j = tj0 - NS - 1 + b;
             const bool ing = idx < HZ * WZ && i >= 0 && i < NX && j >= 0 && j < NY;
@@ -373,7 +378,7 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
         }
 #pragma unroll
         for (int q = 0; q < QZ; ++q) {
-            const int idx = threadIdx.x + q * KM_BLOCK;
+            const int idx = threadIdx.x + q * NT;
             if (idx < HZ * WZ) za[idx] = mq[q] ? (T)yq[q] : T(0);
         }
         __syncthreads();
@@ -383,7 +388,7 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
             const int h = NS - hs;
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                const int idx = threadIdx.x + q * KM_BLOCK;
+                const int idx = threadIdx.x + q * NT;
                 const int a = idx / WR, b = idx - (idx / WR) * WR;
                 if (idx >= NR || !in[q] || (int)((blk >> q) & 1u) != (hs & 1) || a < NS - h || a >= TI + NS + h ||
                     b < NS - h || b >= TJ + NS + h)
@@ -405,7 +410,7 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             if (!ou[q]) continue;
-            const int idx = threadIdx.x + q * KM_BLOCK;
+            const int idx = threadIdx.x + q * NT;
             const int a = idx / WR, b = idx - (idx / WR) * WR;
             const int pz = (a + 1) * WZ + (b + 1);
             const int64_t k = kq[q];
@@ -443,7 +448,7 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
             const int B = cg.B;
             const int I0 = r0 / B, J0 = c0 / B;
             const int nJ = (c1 - 1) / B - J0 + 1, np = ((r1 - 1) / B - I0 + 1) * nJ;
-            for (int u = w; u < np; u += 4) {
+            for (int u = w; u < np; u += NW) {
                 const int I = I0 + u / nJ, J = J0 + u % nJ;
                 const int ra = I * B > r0 ? I * B : r0, rb = (I + 1) * B < r1 ? (I + 1) * B : r1;
                 const int ca = J * B > c0 ? J * B : c0, cb = (J + 1) * B < c1 ? (J + 1) * B : c1;
@@ -477,19 +482,23 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
         for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
         if (lane == 0)
 #pragma unroll
-            for (int q = 0; q < NV; ++q) sh[q * 4 + w] = v[q];
+            for (int q = 0; q < NV; ++q) sh[q * NW + w] = v[q];
         __syncthreads();
         if (threadIdx.x == 0)
 #pragma unroll
-            for (int q = 0; q < NV; ++q)
-                g.P[q * kMaxTileWg + L] = (sh[q * 4] + sh[q * 4 + 1]) + (sh[q * 4 + 2] + sh[q * 4 + 3]);
+            for (int q = 0; q < NV; ++q) {
+                const double* hq = sh + q * NW;
+                double t = (hq[0] + hq[1]) + (hq[2] + hq[3]);
+                if (NW == 8) t = t + ((hq[4 % NW] + hq[5 % NW]) + (hq[6 % NW] + hq[7 % NW]));
+                g.P[q * kMaxTileWg + L] = t;
+            }
     }
 }
 
-template <typename T, int TI, int TJ, int MODE, int WPE, int SW, bool SELF>
-__global__ void __launch_bounds__(KM_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+template <typename T, int TI, int TJ, int MODE, int WPE, int SW, bool SELF, int NT = KM_BLOCK>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
-    tile_body<T, TI, TJ, MODE, SW, SELF>(tl, cg, g, xcd_remap(blockIdx.x, tl.G));
+    tile_body<T, TI, TJ, MODE, SW, SELF, NT>(tl, cg, g, xcd_remap(blockIdx.x, tl.G));
 }
 
 // Two independent systems in one launch: workgroups [0, tl0.G) take the
@@ -498,9 +507,9 @@ template <typename T, int TI, int TJ, int MODE, int WPE, int SW, bool SELF>
 __global__ void __launch_bounds__(KM_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_tl_tile2(ATile tl0, ATile tl1, CGeom cg, TlArgs<T> g0, TlArgs<T> g1) {
     if ((int)blockIdx.x < tl0.G)
-        tile_body<T, TI, TJ, MODE, SW, SELF>(tl0, cg, g0, xcd_remap(blockIdx.x, tl0.G));
+        tile_body<T, TI, TJ, MODE, SW, SELF, KM_BLOCK>(tl0, cg, g0, xcd_remap(blockIdx.x, tl0.G));
     else
-        tile_body<T, TI, TJ, MODE, SW, SELF>(tl1, cg, g1, xcd_remap(blockIdx.x - tl0.G, tl1.G));
+        tile_body<T, TI, TJ, MODE, SW, SELF, KM_BLOCK>(tl1, cg, g1, xcd_remap(blockIdx.x - tl0.G, tl1.G));
 }
 
 // One tile launch of the two-level smoother (kmcoarse::kSweeps red-black
@@ -510,8 +519,9 @@ void launch_tile(int which, hipStream_t q, int NX, int NY, const CGeom& cg, cons
 #define KM_TL_CASE(IDX)                                                                                           \
     case IDX: {                                                                                                   \
         const ATile tl = a_tile(NX, NY, kTileI[IDX], kTileJ[IDX]);                                               \
-        hipLaunchKernelGGL((k_tl_tile<T, kTileI[IDX], kTileJ[IDX], MODE, kTileW[IDX], kmcoarse::kSweeps, false>), \
-                           dim3(tl.G), dim3(KM_BLOCK), 0, q, tl, cg, a);                                          \
+        hipLaunchKernelGGL((k_tl_tile<T, kTileI[IDX], kTileJ[IDX], MODE, kTileW[IDX], kmcoarse::kSweeps, false,    \
+                                      kTileT[IDX]>),                                                              \
+                           dim3(tl.G), dim3(kTileT[IDX]), 0, q, tl, cg, a);                                      \
         break;                                                                                                    \
     }
     switch (which) {
@@ -520,8 +530,11 @@ void launch_tile(int which, hipStream_t q, int NX, int NY, const CGeom& cg, cons
         KM_TL_CASE(2)
         KM_TL_CASE(3)
         KM_TL_CASE(4)
-        default:
         KM_TL_CASE(5)
+        KM_TL_CASE(6)
+        KM_TL_CASE(7)
+        default:
+        KM_TL_CASE(8)
     }
 #undef KM_TL_CASE
 }
@@ -858,13 +871,30 @@ int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
     return KM_OK;
 }
 
-int tile_info(km_ctx* ctx, int* ti, int* tj) {
+int tile_info(km_ctx* ctx, int* ti, int* tj, int* threads) {
     const km_coarse* C = ctx->coarse;
     KM_REQUIRE(ctx, C && C->arena, "no coarse level");
     auto it = C->tiles.find(std::make_tuple(C->NX, C->NY, (int)sizeof(double)));
     const int w = it == C->tiles.end() ? 0 : it->second;
     *ti = kTileI[w];
     *tj = kTileJ[w];
+    if (threads) *threads = kTileT[w];
+    return KM_OK;
+}
+
+// Pin the smoothing tile of the coarse level's system (both precisions);
+// which < 0 clears the choice, so the next solve times the shapes again.
+int set_tile(km_ctx* ctx, int which) {
+    km_coarse* C = ctx->coarse;
+    KM_REQUIRE(ctx, C && C->arena, "no coarse level yet (solve once first)");
+    KM_REQUIRE(ctx, which < kNumTiles, "tile %d of %d", which, kNumTiles);
+    if (which >= 0)
+        KM_REQUIRE(ctx, tile_slots(C, which) <= kMaxVS, "tile %d: %d restriction slots", which, tile_slots(C, which));
+    for (int sz : {(int)sizeof(float), (int)sizeof(double)}) {
+        const auto key = std::make_tuple(C->NX, C->NY, sz);
+        if (which < 0) C->tiles.erase(key);
+        else C->tiles[key] = which;
+    }
     return KM_OK;
 }
 
@@ -1360,7 +1390,18 @@ int km_set_coarse_chunks(km_ctx* ctx, int big, int small, int lead) {
 
 int km_twolevel_tile(km_ctx* ctx, int* ti, int* tj) {
     if (!ctx || !ti || !tj) return KM_EINVAL;
-    return kmcoarse::tile_info(ctx, ti, tj);
+    return kmcoarse::tile_info(ctx, ti, tj, nullptr);
+}
+
+int km_twolevel_tile_threads(km_ctx* ctx, int* threads) {
+    if (!ctx || !threads) return KM_EINVAL;
+    int ti = 0, tj = 0;
+    return kmcoarse::tile_info(ctx, &ti, &tj, threads);
+}
+
+int km_twolevel_set_tile(km_ctx* ctx, int which) {
+    if (!ctx) return KM_EINVAL;
+    return kmcoarse::set_tile(ctx, which);
 }
 
 int km_twolevel_precond(km_ctx* ctx, int NX, int NY, const double* coef, const double* r, double* z) {

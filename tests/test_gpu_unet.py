@@ -331,6 +331,36 @@ def test_twolevel_totals_bit_identical(dev):
     np.testing.assert_array_equal(xa, xb)
 
 
+def test_twolevel_every_tile_shape(dev):
+    """Each smoothing tile candidate (4 or 8 waves per workgroup, 8 x 64 to
+    32 x 32 cells) pinned in turn on the C1 p' system: true residual below
+    1e-8 and the iteration count within 2 of the tuned choice (the tiles
+    change only the order of the partial sums)."""
+    coef, b = c1_pcor()
+    ctx = _lib.context_for(dev)
+    _, tuned = coarse_flex(dev, coef, b, None, _lib.KM_FLAG_COARSE)
+    seen = []
+    ctx.call("km_set_coarse_update", 0)
+    try:
+        for which in range(9):
+            ctx.call("km_twolevel_set_tile", which)
+            x, info = coarse_flex(dev, coef, b, None, _lib.KM_FLAG_COARSE)
+            ti, tj, nt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            ctx.call("km_twolevel_tile", ctypes.byref(ti), ctypes.byref(tj))
+            ctx.call("km_twolevel_tile_threads", ctypes.byref(nt))
+            true = np.linalg.norm(b - krylov.stencil(coef, x)) / np.linalg.norm(b)
+            seen.append((ti.value, tj.value, nt.value, info.iterations))
+            assert info.status == _lib.KM_OK and info.fallback == 0 and true < 1e-8, (which, true)
+            assert abs(info.iterations - tuned.iterations) <= 2, (which, info.iterations, tuned.iterations)
+        with pytest.raises(_lib.KarmanError):
+            ctx.call("km_twolevel_set_tile", 9)
+    finally:
+        ctx.call("km_twolevel_set_tile", -1)
+        ctx.call("km_set_coarse_update", 8)
+    print("tiles (ti, tj, threads, iterations):", seen, "tuned", tuned.iterations)
+    assert {t[2] for t in seen} == {256, 512}
+
+
 def test_twolevel_many_aggregates(dev):
     """The fused flexible solve with 2 x 2 aggregates on a ragged system
     (several aggregates per workgroup in the Krylov update kernels)."""
