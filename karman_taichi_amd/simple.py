@@ -288,6 +288,16 @@ class SIMPLESolver:
         for name, val in self.fields_numpy().items():
             np.savetxt(f"{folder}/{step:06}-{name}-{msg}.csv", val, delimiter=",")
 
+    def dump_coef(self, step, msg, folder="log"):
+        """The coefficient planes and right-hand sides of the three systems as
+        CSV, named as simple.py:55-75 (including the space in "bu -")."""
+        for sys_, coef, b in (("u", self._coef_u, self.b_u), ("v", self._coef_v, self.b_v),
+                              ("p", self._coef_p, self.b_p)):
+            planes = coef.detach().cpu().numpy()
+            for k, pre in enumerate(("ap", "aw", "ae", "an", "as")):
+                np.savetxt(f"{folder}/{step:06}-{pre}{sys_}-{msg}.csv", planes[k], delimiter=",")
+            np.savetxt(f"{folder}/{step:06}-b{sys_} -{msg}.csv", b.detach().cpu().numpy(), delimiter=",")
+
     def forces(self):
         """Pressure + viscous force on the obstacle (drag, lift) per unit depth.
 

@@ -383,6 +383,25 @@ def test_native_step_matches_staged(config, dev):
     print(config, [(i["u"].iterations, i["v"].iterations, i["p"].iterations) for i in ib])
 
 
+def test_dumps_named_as_reference(dev, tmp_path):
+    """dump_matrix / dump_coef write the reference's CSV names
+    (simple.py:51-75) with the fields' values."""
+    c = cases.c1()
+    s = SIMPLESolver.from_case(c, device=dev)
+    s.init()
+    s.step()
+    s.dump_matrix(7, "t", folder=str(tmp_path))
+    s.dump_coef(7, "t", folder=str(tmp_path))
+    names = sorted(f.name for f in tmp_path.iterdir())
+    want = [f"000007-{n}-t.csv" for n in ("u", "v", "p", "mdiv", "pcor")]
+    for sys_ in "uvp":
+        want += [f"000007-{pre}{sys_}-t.csv" for pre in ("ap", "aw", "ae", "an", "as")] + [f"000007-b{sys_} -t.csv"]
+    assert names == sorted(want)
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "000007-aep-t.csv", delimiter=","), N(s.coef_p[:, :, 2]))
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "000007-bu -t.csv", delimiter=","), N(s.b_u))
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "000007-pcor-t.csv", delimiter=","), N(s.pcor))
+
+
 def test_large_grid_properties(dev):
     """C5-size operator: linearity and symmetry of the assembled p' matrix,
     bitwise run-to-run repeatability of the fused reductions."""
