@@ -618,7 +618,7 @@ __global__ void __launch_bounds__(KM_BLOCK)
 k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict__ inv, const double* __restrict__ rcs,
         double* __restrict__ Y, KState* __restrict__ st, const double* __restrict__ P, int* flag) {
     extern __shared__ double rs_[];
-    if (st->done) return;
+    const int done = st->done;   // tested once the row loads are in flight
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     const bool live = row < m;
     const int rw = live ? row : 0;
@@ -627,6 +627,7 @@ k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict_
     double av[kCoarseCh];
     first_row(a, m, lane, av);
     load_rc(rcs, m, S, rs_);
+    if (done) return;
     double rvv, rr;
     if (tot & FIN_A) {
         double t1[1];
@@ -682,12 +683,16 @@ __device__ __forceinline__ void ct_scalars(const double* __restrict__ P, int Gt,
 
 // x += alpha p^ + omega s^ (bicgsolver.py:128), r = s - omega t (:134),
 // p = r + beta (p - omega v) (:103) on flat chunk wf, and its <r,r> partial.
-template <typename T>
-__device__ __forceinline__ void flat_update(const FGeom& fl, int wf, double alpha, double omega, double beta,
-                                            T* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
-                                            const T* __restrict__ v, const T* __restrict__ s, const T* __restrict__ t,
+// scalars(alpha, omega, beta) runs first (the whole workgroup calls it);
+// false: the solve has ended, nothing is written.
+template <typename T, typename F>
+__device__ __forceinline__ void flat_update(const FGeom& fl, int wf, F scalars, T* __restrict__ x,
+                                            T* __restrict__ r, T* __restrict__ p, const T* __restrict__ v,
+                                            const T* __restrict__ s, const T* __restrict__ t,
                                             const T* __restrict__ ph_, const T* __restrict__ sh_,
                                             double* __restrict__ P, double* shr) {
+    double alpha = 0.0, omega = 0.0, beta = 0.0;
+    if (!scalars(alpha, omega, beta)) return;
     const T fo = (T)omega, fb = (T)beta;
     const int64_t lo = (int64_t)wf * fl.chunk;
     const int64_t hi = lo + fl.chunk < fl.n ? lo + fl.chunk : fl.n;
@@ -735,7 +740,7 @@ k_tl_ct(int m, int S, int Gt, int Gc, FGeom fl, int tot, int ph, const double* _
         const T* __restrict__ ph_, const T* __restrict__ sh_, KState* __restrict__ st, double* __restrict__ P) {
     extern __shared__ double rs_[];
     __shared__ double shr[8];
-    if (st->done) return;
+    const int done = st->done;   // tested once the first loads are in flight
     const bool coarse = (int)blockIdx.x < Gc;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     const bool live = coarse && row < m;
@@ -752,9 +757,11 @@ k_tl_ct(int m, int S, int Gt, int Gc, FGeom fl, int tot, int ph, const double* _
         first_row(a, m, lane, av);
         load_rc(rcs, m, S, rs_);
     }
-    double omega, beta, alpha;
-    ct_scalars(P, Gt, tot, st, ph, blockIdx.x == 0 && threadIdx.x == 0, &omega, &beta, &alpha);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (coarse) {
+        double omega, beta, alpha;
+        ct_scalars(P, Gt, tot, st, ph, lead && !done, &omega, &beta, &alpha);
+        if (done) return;
         __syncthreads();   // rs_
         if (!live) return;
         const double yt = row_dot(a, m, lane, av, rs_);
@@ -765,7 +772,12 @@ k_tl_ct(int m, int S, int Gt, int Gc, FGeom fl, int tot, int ph, const double* _
         }
         return;
     }
-    flat_update<T>(fl, blockIdx.x - Gc, alpha, omega, beta, x, r, p, v, s, t, ph_, sh_, P, shr);
+    // the partial sums load beside the state; a finished solve writes nothing
+    auto scalars = [&](double& al, double& om, double& be) {
+        ct_scalars(P, Gt, tot, st, ph, lead && !done, &om, &be, &al);
+        return !done;
+    };
+    flat_update<T>(fl, blockIdx.x - Gc, scalars, x, r, p, v, s, t, ph_, sh_, P, shr);
 }
 
 // The flat workgroups of K_CT for two independent systems in one launch
@@ -787,10 +799,12 @@ __global__ void __launch_bounds__(KM_BLOCK) k_tl_flat2(int ph, FlatSys<T> a, Fla
     const bool first = (int)blockIdx.x < a.fl.G;
     const FlatSys<T>& S = first ? a : b;
     const int wf = first ? (int)blockIdx.x : (int)blockIdx.x - a.fl.G;
-    if (S.st->done) return;
-    double omega, beta, alpha;
-    ct_scalars(S.P, S.Gt, S.tot, S.st, ph, wf == 0 && threadIdx.x == 0, &omega, &beta, &alpha);
-    flat_update<T>(S.fl, wf, alpha, omega, beta, S.x, S.r, S.p, S.v, S.s, S.t, S.ph, S.sh, S.P, shr);
+    const int done = S.st->done;
+    auto scalars = [&](double& al, double& om, double& be) {
+        ct_scalars(S.P, S.Gt, S.tot, S.st, ph, wf == 0 && threadIdx.x == 0 && !done, &om, &be, &al);
+        return !done;
+    };
+    flat_update<T>(S.fl, wf, scalars, S.x, S.r, S.p, S.v, S.s, S.t, S.ph, S.sh, S.P, shr);
 }
 
 }  // namespace
