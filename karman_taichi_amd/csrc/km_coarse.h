@@ -31,12 +31,15 @@ struct km_coarse {
     int totals = -1;          // km_set_coarse_totals: -1 automatic, 0 never, 1 always
     int recent[8] = {};       // iteration counts of the last fused solves of this shape (0: none)
     int chunk_big = 8, chunk_small = 2, chunk_lead = 2;   // km_set_coarse_chunks
-    void* blas = nullptr;     // rocBLAS handle (the update's GEMM)
+    uint64_t used = 0;        // coarse_tick of the last solve that used this level
     // tile shape of the fused smoothing launches per (NX, NY, sizeof T)
     std::map<std::tuple<int, int, int>, int> tiles;
 };
 
 namespace kmcoarse {
+
+// f on every coarse level of the context (a setting applies to all)
+void for_each_level(km_ctx* ctx, const std::function<void(km_coarse*)>& f);
 
 // Red-black SOR sweeps of the two-level smoother (each a red then a black
 // half-sweep) and their relaxation factor: on the 1280 x 256 channel's p'

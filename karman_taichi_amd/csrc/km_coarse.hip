@@ -501,14 +501,13 @@ size_t a256(size_t v) { return (v + 255) & ~size_t(255); }
 // Xn = X M for row-major m x m X, M: the column-major product M^T X^T
 // (a plain library GEMM)
 int ns_gemm(km_ctx* ctx, int m, const double* M, const double* X, double* Xn, hipStream_t s) {
-    km_coarse* C = ctx->coarse;
-    if (!C->blas) {
+    if (!ctx->blas) {
         rocblas_handle h = nullptr;
         if (rocblas_create_handle(&h) != rocblas_status_success)
             return km_fail(ctx, KM_EHIP, "rocblas_create_handle failed");
-        C->blas = h;
+        ctx->blas = h;
     }
-    rocblas_handle h = static_cast<rocblas_handle>(C->blas);
+    rocblas_handle h = static_cast<rocblas_handle>(ctx->blas);
     if (rocblas_set_stream(h, s) != rocblas_status_success) return km_fail(ctx, KM_EHIP, "rocblas_set_stream failed");
     const double one = 1.0, zero = 0.0;
     const rocblas_status st = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, m, m, &one, M, m, X,
@@ -521,6 +520,50 @@ km_coarse* get_coarse(km_ctx* ctx) {
     if (!ctx->coarse) ctx->coarse = new km_coarse();
     return ctx->coarse;
 }
+
+// The settings a new level takes over from the current one.
+void copy_settings(const km_coarse* from, km_coarse* to) {
+    to->block_req = from->block_req;
+    to->update = from->update;
+    to->totals = from->totals;
+    to->chunk_big = from->chunk_big;
+    to->chunk_small = from->chunk_small;
+    to->chunk_lead = from->chunk_lead;
+    to->tiles = from->tiles;
+    to->lds_max = from->lds_max;
+}
+
+constexpr size_t kMaxLevels = 128;
+
+// The level of a fused solve of `coef` (NX x NY): each coefficient array
+// keeps its own, so the cases of a batch each keep their held inverse across
+// outer steps; past kMaxLevels the least recently used level is recycled.
+km_coarse* level_for(km_ctx* ctx, int NX, int NY, const double* coef) {
+    km_coarse* cur = get_coarse(ctx);
+    auto& pool = ctx->coarse_pool;
+    if (pool.empty()) pool.push_back(cur);
+    const uint64_t tick = ++ctx->coarse_tick;
+    km_coarse* pick = nullptr;
+    for (km_coarse* C : pool)
+        if (C->arena && C->coef == coef && C->NX == NX && C->NY == NY) pick = C;
+    if (!pick)
+        for (km_coarse* C : pool)
+            if (!C->arena) pick = C;   // a level that never ran
+    if (!pick && pool.size() < kMaxLevels) {
+        pick = new km_coarse();
+        copy_settings(cur, pick);
+        pool.push_back(pick);
+    }
+    if (!pick) {
+        pick = pool[0];
+        for (km_coarse* C : pool)
+            if (C->used < pick->used) pick = C;
+    }
+    pick->used = tick;
+    ctx->coarse = pick;
+    return pick;
+}
+
 
 }  // namespace
 
@@ -535,8 +578,18 @@ int choose_block(int nx, int ny, int req) {
     return 512;
 }
 
+void for_each_level(km_ctx* ctx, const std::function<void(km_coarse*)>& f) {
+    km_coarse* cur = get_coarse(ctx);
+    bool seen = false;
+    for (km_coarse* C : ctx->coarse_pool) {
+        f(C);
+        seen = seen || C == cur;
+    }
+    if (!seen) f(cur);
+}
+
 int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s, bool solver) {
-    km_coarse* C = get_coarse(ctx);
+    km_coarse* C = solver ? level_for(ctx, NX, NY, coef) : get_coarse(ctx);
     const int nx = NX - 2, ny = NY - 2;
     KM_REQUIRE(ctx, nx >= 1 && ny >= 1, "system too small for a coarse level");
     const int B = choose_block(nx, ny, C->block_req);
@@ -643,7 +696,7 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s, bool s
     KM_CHECK_LAUNCH(ctx);
     // the first factorisation also runs one update product into scratch, so
     // that rocBLAS loads its kernels here rather than in a later solve
-    if (!update && !C->blas && C->update > 0)
+    if (!update && !ctx->blas && C->update > 0)
         if (int rc = ns_gemm(ctx, m, C->inv, C->inv, nsX, s)) return rc;
     C->held = true;
     return KM_OK;
@@ -691,10 +744,20 @@ template int apply<double>(km_ctx*, const double*, const double*, const double*,
 }  // namespace kmcoarse
 
 void km_coarse_free(km_ctx* ctx) {
-    if (!ctx || !ctx->coarse) return;
-    if (ctx->coarse->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->coarse->blas));
-    if (ctx->coarse->arena) (void)hipFree(ctx->coarse->arena);
-    delete ctx->coarse;
+    if (!ctx) return;
+    if (ctx->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
+    ctx->blas = nullptr;
+    bool cur_in_pool = false;
+    for (km_coarse* C : ctx->coarse_pool) {
+        cur_in_pool = cur_in_pool || C == ctx->coarse;
+        if (C->arena) (void)hipFree(C->arena);
+        delete C;
+    }
+    ctx->coarse_pool.clear();
+    if (ctx->coarse && !cur_in_pool) {
+        if (ctx->coarse->arena) (void)hipFree(ctx->coarse->arena);
+        delete ctx->coarse;
+    }
     ctx->coarse = nullptr;
 }
 
@@ -703,7 +766,7 @@ extern "C" {
 int km_set_coarse(km_ctx* ctx, int block) {
     if (!ctx) return KM_EINVAL;
     KM_REQUIRE(ctx, block >= 0 && block <= 4096, "aggregate size %d out of range", block);
-    get_coarse(ctx)->block_req = block;
+    kmcoarse::for_each_level(ctx, [&](km_coarse* C) { C->block_req = block; });
     return KM_OK;
 }
 
@@ -718,9 +781,10 @@ int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const dou
 int km_set_coarse_update(km_ctx* ctx, int every) {
     if (!ctx) return KM_EINVAL;
     KM_REQUIRE(ctx, every >= 0, "update interval %d", every);
-    km_coarse* C = get_coarse(ctx);
-    C->update = every;
-    C->since = 0;
+    kmcoarse::for_each_level(ctx, [&](km_coarse* C) {
+        C->update = every;
+        C->since = 0;
+    });
     return KM_OK;
 }
 

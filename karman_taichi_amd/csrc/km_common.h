@@ -142,7 +142,10 @@ struct km_ctx {
     std::set<std::tuple<int, int, int>> tuned;
     km_net* net = nullptr;
     km_batch* batch = nullptr;
-    km_coarse* coarse = nullptr;
+    km_coarse* coarse = nullptr;   // the coarse level in use (one of coarse_pool once a fused solve ran)
+    std::vector<km_coarse*> coarse_pool;   // the fused solves' levels, one per coefficient array
+    uint64_t coarse_tick = 0;
+    void* blas = nullptr;          // rocBLAS handle (the coarse levels' Newton-Schulz GEMM)
 };
 
 void km_batch_free(km_ctx* ctx);

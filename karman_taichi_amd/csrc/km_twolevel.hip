@@ -904,11 +904,14 @@ int set_tile(km_ctx* ctx, int which) {
     KM_REQUIRE(ctx, which < kNumTiles, "tile %d of %d", which, kNumTiles);
     if (which >= 0)
         KM_REQUIRE(ctx, tile_slots(C, which) <= kMaxVS, "tile %d: %d restriction slots", which, tile_slots(C, which));
-    for (int sz : {(int)sizeof(float), (int)sizeof(double)}) {
-        const auto key = std::make_tuple(C->NX, C->NY, sz);
-        if (which < 0) C->tiles.erase(key);
-        else C->tiles[key] = which;
-    }
+    const int NX = C->NX, NY = C->NY;
+    for_each_level(ctx, [&](km_coarse* L) {
+        for (int sz : {(int)sizeof(float), (int)sizeof(double)}) {
+            const auto key = std::make_tuple(NX, NY, sz);
+            if (which < 0) L->tiles.erase(key);
+            else L->tiles[key] = which;
+        }
+    });
     return KM_OK;
 }
 
@@ -1385,7 +1388,7 @@ extern "C" {
 int km_set_coarse_totals(km_ctx* ctx, int mode) {
     if (!ctx) return KM_EINVAL;
     KM_REQUIRE(ctx, ctx->coarse != nullptr, "no coarse level yet (solve once first)");
-    ctx->coarse->totals = mode < 0 ? -1 : (mode > 0 ? 1 : 0);
+    kmcoarse::for_each_level(ctx, [&](km_coarse* C) { C->totals = mode < 0 ? -1 : (mode > 0 ? 1 : 0); });
     return KM_OK;
 }
 
@@ -1395,10 +1398,11 @@ int km_set_coarse_chunks(km_ctx* ctx, int big, int small, int lead) {
     KM_REQUIRE(ctx, big >= 0 && big % KM_RING == 0 && big <= 64, "chunk of %d iterations", big);
     KM_REQUIRE(ctx, small >= 0 && (small == 0 || KM_RING % small == 0), "short chunk of %d iterations", small);
     KM_REQUIRE(ctx, lead >= 0 && lead <= 1000, "lead of %d iterations", lead);
-    km_coarse* C = ctx->coarse;
-    C->chunk_big = big ? big : 8;
-    C->chunk_small = small ? small : 2;
-    C->chunk_lead = lead;
+    kmcoarse::for_each_level(ctx, [&](km_coarse* C) {
+        C->chunk_big = big ? big : 8;
+        C->chunk_small = small ? small : 2;
+        C->chunk_lead = lead;
+    });
     return KM_OK;
 }
 

@@ -361,6 +361,34 @@ def test_twolevel_every_tile_shape(dev):
     assert {t[2] for t in seen} == {256, 512}
 
 
+def test_twolevel_levels_per_system(dev):
+    """Two p' systems solved in turn (A, B, A, as the cases of a batch): each
+    coefficient array keeps its own coarse level, so the second solve of A
+    runs on A's held inverse and repeats the first bit for bit."""
+    ca, ba = c1_pcor()
+    cb, bb = small_system(3, nx=150, ny=60)
+    cb, bb = cb.astype(np.float64), bb.astype(np.float64)
+    ctx = _lib.context_for(dev)
+    sys_a = (T(ca, dev), T(ba, dev))
+    sys_b = (T(cb, dev), T(bb, dev))
+
+    def solve(sys_):
+        ct, bt = sys_
+        xt = torch.empty_like(bt)
+        info = KmSolveInfo()
+        ctx.call("km_fbicgstab", bt.shape[0], bt.shape[1], ptr(ct), ptr(bt), ptr(xt), 1e-8, 0, _lib.KM_FLAG_COARSE,
+                 None, ctypes.byref(info))
+        return xt.cpu().numpy(), info
+
+    x1, i1 = solve(sys_a)
+    xb, ib = solve(sys_b)
+    x2, i2 = solve(sys_a)
+    print(f"levels: A {i1.iterations} it, B {ib.iterations} it, A again {i2.iterations} it")
+    assert i1.status == ib.status == i2.status == _lib.KM_OK
+    assert i1.iterations == i2.iterations
+    np.testing.assert_array_equal(x1, x2)
+
+
 def test_twolevel_many_aggregates(dev):
     """The fused flexible solve with 2 x 2 aggregates on a ragged system
     (several aggregates per workgroup in the Krylov update kernels)."""
