@@ -343,10 +343,10 @@ def p_compare(s, reps):
 
 def bench_batch(args):
     """The 96 uneven channel cases (nx 256..2048, Re 80..300) advanced together
-    with km_batch_step: one launch per SIMPLE stage over all cases, batched
-    momentum BiCGSTAB, p' batched (plain modes) or case by case with the
-    two-level solver (two-level modes).  A step = one outer iteration of every
-    case."""
+    with km_batch_step: one launch per SIMPLE stage over all cases; momentum
+    and p' batched (plain modes) or case by case with the smoothed momentum
+    and two-level p' solvers (two-level modes).  A step = one outer iteration
+    of every case."""
     import torch
     from karman_taichi_amd import cases
     from karman_taichi_amd.batch import BatchedSIMPLE
@@ -356,12 +356,16 @@ def bench_batch(args):
     b = BatchedSIMPLE(case_list, device=dev)
     b.init()
     coarse = MODES[args.mode][2]
+    fp32 = MODES[args.mode][1]
+    # momentum as the single-case bench: the smoothed solve with the two-level p' solve
+    momentum = args.momentum if args.momentum != "auto" else ("rbgs" if coarse and not fp32 else "plain")
+    smooth = momentum == "rbgs"
     log(f"bench c3: {b.n} cases, {b.cells()} p' cells, warmup {args.warmup}, steps {args.steps}, mode {args.mode}")
     for _ in range(args.warmup):
-        b.step(coarse=coarse)
+        b.step(coarse=coarse, smooth=smooth)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    recs = [b.step(coarse=coarse) for _ in range(args.steps)]
+    recs = [b.step(coarse=coarse, smooth=smooth) for _ in range(args.steps)]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     p_it = [inf[2].iterations for rec in recs for inf in rec]
@@ -372,7 +376,8 @@ def bench_batch(args):
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded channel start states)",
            "config": {"workload": f"96 uneven channels nx 256..2048, ny nx/5, D ny/8, Re 80..300, "
                                   f"{b.cells()} p' cells in total", "tile": list(b.tile),
-                      "p_solver": "two-level flexible BiCGSTAB per case" if coarse else "batched BiCGSTAB"},
+                      "p_solver": "two-level flexible BiCGSTAB per case" if coarse else "batched BiCGSTAB",
+                      "momentum": "fbicgstab+rbgs per case (u, v together)" if smooth else "batched BiCGSTAB"},
            "p_iters_min_median_max": [min(p_it), sorted(p_it)[len(p_it) // 2], max(p_it)],
            "p_cells_iters_per_s": round(cell_its / wall, 1),
            "case_steps_per_s": round(args.steps * b.n / wall, 3)}

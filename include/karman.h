@@ -213,7 +213,8 @@ typedef struct km_case {
  * case (u, v, p'); rel_resid_true is not recomputed (-1) for the batched
  * solves.  flags KM_FLAG_COARSE: each case's p' system is solved by the
  * two-level flexible BiCGSTAB of km_fbicgstab instead (case by case; its
- * info carries the true residual). */
+ * info carries the true residual); KM_FLAG_SMOOTH: each case's u and v
+ * systems by km_fbicgstab2 with KM_FLAG_SMOOTH (case by case). */
 int km_batch_step(km_ctx* ctx, int n, const km_case* cases, double eps_mom, double eps_p, int max_iter, int flags,
                   km_solve_info* info);
 /* time_forward of every case (karman_taichi.py:617-622). */

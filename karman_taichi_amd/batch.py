@@ -52,12 +52,14 @@ class BatchedSIMPLE:
             s.init()
 
     def step(self, eps_mom: float = 1e-8, eps_p: float = 1e-8, max_iter: int = 0, graph: bool = True,
-             coarse: bool = False):
+             coarse: bool = False, smooth: bool = False):
         """One outer iteration of every case; returns [(u, v, p') SolveInfo] per case.
         ``coarse``: p' by the two-level preconditioned flexible BiCGSTAB, case by
-        case (the stages and momentum solves stay batched)."""
+        case; ``smooth``: u and v by the smoothed flexible BiCGSTAB, case by case
+        (km_fbicgstab2).  The stages, and the solves not selected, stay batched."""
         self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
         flags = (0 if graph else _lib.KM_FLAG_NO_GRAPH) | (_lib.KM_FLAG_COARSE if coarse else 0)
+        flags |= _lib.KM_FLAG_SMOOTH if smooth else 0
         self.ctx.call("km_batch_step", self.n, self._table, float(eps_mom), float(eps_p), int(max_iter), flags,
                       self._info)
         return [tuple(SolveInfo.from_c(self._info[3 * k + f]) for f in range(3)) for k in range(self.n)]

@@ -299,10 +299,28 @@ int km_batch_step(km_ctx* ctx, int n, const km_case* cases, double eps_mom, doub
     if (int rc = run_stages(ctx, B, 0, use_graph)) return rc;
     if (!B->tuned && ctx->autotune)
         if (int rc = autotune(ctx, B, 0)) return rc;
-    for (int f = 0; f < 2; ++f) {
-        if (int rc = solve(ctx, B, f, eps_mom, max_iter, use_graph)) return rc;
-        if (info)
-            for (int c = 0; c < n; ++c) fill(&info[3 * c + f], B->h_states[c]);
+    if (flags & KM_FLAG_SMOOTH) {
+        // u and v case by case with the smoothed flexible solve, both systems
+        // of a case in the same launches (km_fbicgstab2)
+        for (int c = 0; c < n; ++c) {
+            const km_case& K = B->cases[c];
+            km_solve_info iu{}, iv{};
+            const int nx = K.g.nx, ny = K.g.ny;
+            const int rc = km_fbicgstab2(ctx, nx + 3, ny + 2, K.coef_u, K.b_u, K.u_mid, nx + 2, ny + 3, K.coef_v,
+                                         K.b_v, K.v_mid, eps_mom, max_iter,
+                                         KM_FLAG_SMOOTH | (flags & KM_FLAG_NO_GRAPH), &iu, &iv);
+            if (rc < 0) return rc;
+            if (info) {
+                info[3 * c] = iu;
+                info[3 * c + 1] = iv;
+            }
+        }
+    } else {
+        for (int f = 0; f < 2; ++f) {
+            if (int rc = solve(ctx, B, f, eps_mom, max_iter, use_graph)) return rc;
+            if (info)
+                for (int c = 0; c < n; ++c) fill(&info[3 * c + f], B->h_states[c]);
+        }
     }
     if (int rc = run_stages(ctx, B, 1, use_graph)) return rc;
     if (flags & KM_FLAG_COARSE) {

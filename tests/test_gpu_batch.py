@@ -72,9 +72,11 @@ def test_batch_uneven_c3_sample(dev):
     print("C3 sample iterations:", [(c.nx, tuple(i.iterations for i in inf)) for c, inf in zip(sample, infos)])
 
 
-def test_batch_twolevel_p(dev):
+@pytest.mark.parametrize("smooth", [False, True])
+def test_batch_twolevel_p(smooth, dev):
     """km_batch_step with KM_FLAG_COARSE: the p' systems go case by case through
-    the two-level flexible BiCGSTAB; every p' solve reaches 1e-8 in the true
+    the two-level flexible BiCGSTAB (and with KM_FLAG_SMOOTH the u, v systems
+    through the smoothed pair solve); every such solve reaches 1e-8 in the true
     residual and the fields agree with the plain batch to the solve tolerance."""
     plain = BatchedSIMPLE(small_batch(), device=dev)
     plain.init()
@@ -82,9 +84,12 @@ def test_batch_twolevel_p(dev):
     two.init()
     for _ in range(2):
         ip = plain.step()
-        it = two.step(coarse=True)
+        it = two.step(coarse=True, smooth=smooth)
     for c, a, b, i2 in zip(plain.cases, plain.solvers, two.solvers, it):
         assert i2[2].status == _lib.KM_OK and i2[2].rel_resid_true < 1e-8, c.name
+        if smooth:
+            for f in (0, 1):
+                assert i2[f].status == _lib.KM_OK and i2[f].rel_resid_true < 1e-8, (c.name, f)
         for f in ("u", "v", "p"):
             x, y = getattr(a, f).cpu().numpy(), getattr(b, f).cpu().numpy()
             e = np.linalg.norm(x - y) / max(np.linalg.norm(x), 1e-300)
