@@ -383,6 +383,34 @@ def test_native_step_matches_staged(config, dev):
     print(config, [(i["u"].iterations, i["v"].iterations, i["p"].iterations) for i in ib])
 
 
+def test_step_and_pair_entry_errors(dev):
+    """Argument errors of km_simple_step and km_fbicgstab2 come back as
+    KM_EINVAL with a message, before any launch."""
+    c = cases.c1()
+    s = SIMPLESolver.from_case(c, device=dev)
+    s.init()
+    info = (KmSolveInfo * 3)()
+    case = _lib.KmCase()
+    case.g = s._grid()
+    for f in ("u", "v", "p", "u0", "v0", "u_mid", "v_mid", "pcor", "mdiv", "b_u", "b_v", "b_p"):
+        setattr(case, f, getattr(s, f).data_ptr())
+    case.coef_u, case.coef_v, case.coef_p = s._coef_u.data_ptr(), s._coef_v.data_ptr(), s._coef_p.data_ptr()
+    case.mask = s.ct.data_ptr()
+    for mom in (_lib.KM_FLAG_COARSE, _lib.KM_FLAG_X0):
+        with pytest.raises(_lib.KarmanError):
+            s.ctx.call("km_simple_step", ctypes.byref(case), 1e-8, 1e-8, mom, _lib.KM_FLAG_COARSE, info)
+    us, vs = s.u_momentum_solver, s.v_momentum_solver
+    pair = (KmSolveInfo(), KmSolveInfo())
+    with pytest.raises(_lib.KarmanError):   # both systems write the same x
+        s.ctx.call("km_fbicgstab2", us.NX, us.NY, ptr(us.coef), ptr(us.b), ptr(us.x), vs.NX, vs.NY, ptr(vs.coef),
+                   ptr(vs.b), ptr(us.x), 1e-8, 0, _lib.KM_FLAG_SMOOTH, ctypes.byref(pair[0]), ctypes.byref(pair[1]))
+    with pytest.raises(_lib.KarmanError):   # eps must be positive
+        s.ctx.call("km_fbicgstab2", us.NX, us.NY, ptr(us.coef), ptr(us.b), ptr(us.x), vs.NX, vs.NY, ptr(vs.coef),
+                   ptr(vs.b), ptr(vs.x), 0.0, 0, _lib.KM_FLAG_SMOOTH, ctypes.byref(pair[0]), ctypes.byref(pair[1]))
+    s.step()   # the context still works
+    assert s.last["p"].status == _lib.KM_OK
+
+
 def test_dumps_named_as_reference(dev, tmp_path):
     """dump_matrix / dump_coef write the reference's CSV names
     (simple.py:51-75) with the fields' values."""
