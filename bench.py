@@ -92,6 +92,9 @@ def parse():
     ap.add_argument("--chunks", default="",
                     help="two-level launch schedule big,small,lead (iterations per graph far from / near the expected "
                          "end of a solve, and where near begins); default 8,2,2")
+    ap.add_argument("--case-workers", type=int, default=3,
+                    help="c3 with the two-level mode: step the cases as independent SIMPLE steps, this many at a "
+                         "time on their own contexts and streams (0: km_batch_step)")
     ap.add_argument("--staged", action="store_true",
                     help="time the step stage by stage from Python instead of one km_simple_step call")
     ap.add_argument("--stage-steps", type=int, default=8,
@@ -361,11 +364,16 @@ def bench_batch(args):
     momentum = args.momentum if args.momentum != "auto" else ("rbgs" if coarse and not fp32 else "plain")
     smooth = momentum == "rbgs"
     log(f"bench c3: {b.n} cases, {b.cells()} p' cells, warmup {args.warmup}, steps {args.steps}, mode {args.mode}")
+    workers = args.case_workers if coarse and smooth else 0
+
+    def batch_step():
+        return b.step_cases(workers) if workers > 0 else b.step(coarse=coarse, smooth=smooth)
+
     for _ in range(args.warmup):
-        b.step(coarse=coarse, smooth=smooth)
+        batch_step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    recs = [b.step(coarse=coarse, smooth=smooth) for _ in range(args.steps)]
+    recs = [batch_step() for _ in range(args.steps)]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     p_it = [inf[2].iterations for rec in recs for inf in rec]
@@ -375,9 +383,10 @@ def bench_batch(args):
            "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded channel start states)",
            "config": {"workload": f"96 uneven channels nx 256..2048, ny nx/5, D ny/8, Re 80..300, "
-                                  f"{b.cells()} p' cells in total", "tile": list(b.tile),
+                                  f"{b.cells()} p' cells in total", "tile": None if workers else list(b.tile),
                       "p_solver": "two-level flexible BiCGSTAB per case" if coarse else "batched BiCGSTAB",
-                      "momentum": "fbicgstab+rbgs per case (u, v together)" if smooth else "batched BiCGSTAB"},
+                      "momentum": "fbicgstab+rbgs per case (u, v together)" if smooth else "batched BiCGSTAB",
+                      "step": f"km_simple_step per case, {workers} at a time" if workers else "km_batch_step"},
            "p_iters_min_median_max": [min(p_it), sorted(p_it)[len(p_it) // 2], max(p_it)],
            "p_cells_iters_per_s": round(cell_its / wall, 1),
            "case_steps_per_s": round(args.steps * b.n / wall, 3)}

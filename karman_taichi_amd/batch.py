@@ -64,6 +64,45 @@ class BatchedSIMPLE:
                       self._info)
         return [tuple(SolveInfo.from_c(self._info[3 * k + f]) for f in range(3)) for k in range(self.n)]
 
+    def step_cases(self, workers: int = 4, eps_mom: float = 1e-8, eps_p: float = 1e-8):
+        """One outer iteration of every case as independent ``SIMPLESolver.step``
+        calls (``km_simple_step``: smoothed u/v pair, two-level p'), ``workers``
+        cases at a time, each worker with its own context and stream, so that
+        the latency-bound solves of different cases overlap on the GPU.  Case
+        k always runs on worker k mod ``workers`` (its coarse level stays
+        there).  Returns [(u, v, p') SolveInfo] per case."""
+        import concurrent.futures
+        main = torch.cuda.current_stream(self.device)
+        if getattr(self, "_workers", None) is None or len(self._workers) != workers:
+            idx = self.device.index if self.device.index is not None else torch.cuda.current_device()
+            self._workers = []
+            for _ in range(workers):
+                st = torch.cuda.Stream(device=self.device)
+                self._workers.append((st, _lib.Context(idx, st.cuda_stream)))
+            self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=workers)
+        for st, _ in self._workers:
+            st.wait_stream(main)
+
+        def run(w):
+            st, ctx = self._workers[w]
+            out = {}
+            with torch.cuda.stream(st):
+                for k in range(w, self.n, workers):
+                    s = self.solvers[k]
+                    s.ctx = ctx
+                    s.mom_smooth = s.p_coarse = True
+                    s.eps_mom, s.eps_p = eps_mom, eps_p
+                    last = s.step()
+                    out[k] = (last["u"], last["v"], last["p"])
+            return out
+
+        res = {}
+        for part in self._pool.map(run, range(workers)):
+            res.update(part)
+        for st, _ in self._workers:
+            main.wait_stream(st)
+        return [res[k] for k in range(self.n)]
+
     def time_forward(self):
         self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
         self.ctx.call("km_batch_time_forward", self.n, self._table)
