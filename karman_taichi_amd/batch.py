@@ -7,7 +7,9 @@ cases, the three BiCGSTAB solves run all cases' systems in the same fused
 iteration kernels with per-case scalars and stopping tests, and the stage
 sequences between solves replay as captured hipGraphs.  The reference runs
 one case per process (simple.py:285-303); the per-case arithmetic is that of
-``SIMPLESolver.step``.
+``SIMPLESolver.step``.  :meth:`BatchedSIMPLE.step_cases` instead steps the
+cases as independent ``SIMPLESolver.step`` calls, several at a time on their
+own contexts and streams.
 """
 from __future__ import annotations
 
