@@ -329,10 +329,11 @@ inline int run_chunks_sched(km_ctx* ctx, KState* st, GraphKey key, bool use_grap
 // copy or event between chunks.  Chunk lengths as run_chunks_sched.  `tail`
 // (may be null) enqueues more work after the last chunk, before the one
 // synchronisation at the end.  nsys independent systems iterate in the same
-// launches (states st[0 .. nsys), host words 2s, 2s + 1): the solve ends when
-// all are done, and the slowest one still running sets the pace.
+// launches (system s: host words KM_HW s onwards): the solve ends when all
+// are done, and the slowest one still running sets the pace; final_state[s]
+// is filled from the words the stopping test wrote.
 template <typename F>
-inline int run_chunks_flag(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int big, int small, int expect,
+inline int run_chunks_flag(km_ctx* ctx, GraphKey key, bool use_graph, int big, int small, int expect,
                            int max_iter, F enqueue, KState* final_state, const std::function<int()>* tail = nullptr,
                            int nsys = 1) {
     auto graph_for = [&](int len, int ph0, hipGraphExec_t* out) -> int {
@@ -420,7 +421,6 @@ inline int run_chunks_flag(km_ctx* ctx, KState* st, GraphKey key, bool use_graph
         std::this_thread::yield();
     }
     // the final state comes from the host words the stopping test wrote
-    (void)st;
     if (tail)
         if (int rc = (*tail)()) return rc;
     KM_HIP(ctx, hipStreamSynchronize(ctx->stream));

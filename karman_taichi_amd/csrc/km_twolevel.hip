@@ -1072,7 +1072,7 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
     for (int c : C->recent)
         if (c > 0 && (expect == 0 || c < expect)) expect = c;
     if (expect > 0) expect = std::max(1, expect - C->chunk_lead + 2);
-    const int rc = run_chunks_flag(ctx, f.st, key, use_graph, C->chunk_big, C->chunk_small, expect, max_iter,
+    const int rc = run_chunks_flag(ctx, key, use_graph, C->chunk_big, C->chunk_small, expect, max_iter,
                                    enqueue, out, tail);
     if (rc == KM_OK) {
         for (int k = 7; k > 0; --k) C->recent[k] = C->recent[k - 1];
@@ -1342,7 +1342,7 @@ int smooth_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* x
     key.v = {11, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
              (uintptr_t)ws, (uintptr_t)f.st, (uintptr_t)f.fin};
     // these solves take a handful of iterations: short graphs throughout
-    return run_chunks_flag(ctx, f.st, key, use_graph, KM_RING, 2, 1, max_iter, enqueue, out, tail);
+    return run_chunks_flag(ctx, key, use_graph, KM_RING, 2, 1, max_iter, enqueue, out, tail);
 }
 
 template <typename T>
@@ -1362,7 +1362,7 @@ int smooth_inner2(km_ctx* ctx, const int* NX, const int* NY, const T* const* coe
     key.v = {12, sizeof(T), (uintptr_t)NX[0], (uintptr_t)NY[0], (uintptr_t)coefT[0], (uintptr_t)rhs[0],
              (uintptr_t)xd[0], (uintptr_t)ws[0], (uintptr_t)a.fin, (uintptr_t)NX[1], (uintptr_t)NY[1],
              (uintptr_t)coefT[1], (uintptr_t)rhs[1], (uintptr_t)xd[1], (uintptr_t)ws[1], (uintptr_t)b.fin};
-    return run_chunks_flag(ctx, a.st, key, use_graph, KM_RING, 2, 1, std::max(max_iter[0], max_iter[1]), enqueue, out,
+    return run_chunks_flag(ctx, key, use_graph, KM_RING, 2, 1, std::max(max_iter[0], max_iter[1]), enqueue, out,
                            tail, 2);
 }
 
