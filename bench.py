@@ -87,8 +87,8 @@ def parse():
     ap.add_argument("--momentum", default="auto", choices=["auto", "plain", "rbgs"],
                     help="u/v solver: plain BiCGSTAB (bicgsolver.py) or flexible BiCGSTAB with one red-black "
                          "Gauss-Seidel sweep; auto: rbgs with the FP64 two-level modes, plain otherwise")
-    ap.add_argument("--coarse-update", type=int, default=8,
-                    help="two-level: update the coarse inverse every k-th p' solve (1 each, 0 factor each; default 8)")
+    ap.add_argument("--coarse-update", type=int, default=32,
+                    help="two-level: update the coarse inverse every k-th p' solve (1 each, 0 factor each; default 32)")
     ap.add_argument("--chunks", default="",
                     help="two-level launch schedule big,small,lead (iterations per graph far from / near the expected "
                          "end of a solve, and where near begins); default 8,2,2")

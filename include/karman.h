@@ -242,7 +242,7 @@ int km_coarse_correct(km_ctx* ctx, int NX, int NY, const double* coef, const dou
 /* How the coarse level follows a changing system.  every = 1: a coarse level
  * of the same shape as the last one updates the held A_c^-1 by one
  * Newton-Schulz step X (2 I - A_c X) when ||I - A_c X||_inf < 0.5, instead of
- * factoring A_c again; 0: always factor; k > 1 (default 8): do that on every
+ * factoring A_c again; 0: always factor; k > 1 (default 32): do that on every
  * k-th set-up and, on the set-ups between, reuse the held level (free-cell
  * mask, A_c^-1) unchanged when the coefficient array is the same one as last
  * time (a SIMPLE solver's p' system across outer iterations).  The flexible

@@ -24,7 +24,7 @@ struct km_coarse {
     uint8_t* mask = nullptr;  // NX x NY: 1 on the free interior cells (the support of P)
     size_t lds_max = 0;       // LDS per workgroup on this device (the whole-band factorisation)
     bool held = false;        // inv holds A_c^-1 of a coarse level of the current shape
-    int update = 8;           // km_set_coarse_update: 0 factor, 1 Newton-Schulz update each solve, k every k-th
+    int update = 32;          // km_set_coarse_update: 0 factor, 1 Newton-Schulz update each solve, k every k-th
     int since = 0;            // solves that reused the held inverse since it was last updated
     const double* coef = nullptr;   // the coefficient array of the last set-up (reuse needs the same one)
     bool refresh = false;     // a solve ran long: factor on the next set-up

@@ -372,7 +372,7 @@ def test_native_step_matches_staged(config, dev):
             infos = [s.step() for _ in range(3)]
             runs.append((s, infos))
     finally:
-        ctx.call("km_set_coarse_update", 8)
+        ctx.call("km_set_coarse_update", 32)
     (a, ia), (b, ib) = runs
     for f in ("u", "v", "p", "pcor", "mdiv", "u_mid", "v_mid"):
         assert torch.equal(getattr(a, f), getattr(b, f)), f

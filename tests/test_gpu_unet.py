@@ -301,7 +301,7 @@ def test_twolevel_inverse_update(change, dev):
             ctx.call("km_twolevel_precond", r.shape[0], r.shape[1], ptr(ct), ptr(rt_), ptr(z))
     finally:
         ctx.call("km_set_coarse", 0)
-        ctx.call("km_set_coarse_update", 8)
+        ctx.call("km_set_coarse_update", 32)
     zo = krylov.twolevel_precond(coef2, 16)(r)
     e = rel(z.cpu().numpy(), zo)
     print(f"inverse update ({change}): M(r) rel {e:.2e}")
@@ -323,7 +323,7 @@ def test_twolevel_totals_bit_identical(dev):
             out[mode] = coarse_flex(dev, coef, b, None, _lib.KM_FLAG_COARSE)
     finally:
         ctx.call("km_set_coarse_totals", -1)
-        ctx.call("km_set_coarse_update", 8)
+        ctx.call("km_set_coarse_update", 32)
     (xa, ia), (xb, ib) = out[0], out[1]
     print(f"totals: direct {ia.iterations} it, pre-summed {ib.iterations} it")
     assert ia.status == ib.status == _lib.KM_OK
@@ -356,7 +356,7 @@ def test_twolevel_every_tile_shape(dev):
             ctx.call("km_twolevel_set_tile", 9)
     finally:
         ctx.call("km_twolevel_set_tile", -1)
-        ctx.call("km_set_coarse_update", 8)
+        ctx.call("km_set_coarse_update", 32)
     print("tiles (ti, tj, threads, iterations):", seen, "tuned", tuned.iterations)
     assert {t[2] for t in seen} == {256, 512}
 
