@@ -91,18 +91,24 @@ class BatchedSIMPLE:
             with torch.cuda.stream(st):
                 for k in range(w, self.n, workers):
                     s = self.solvers[k]
-                    s.ctx = ctx
-                    s.mom_smooth = s.p_coarse = True
-                    s.eps_mom, s.eps_p = eps_mom, eps_p
-                    last = s.step()
+                    saved = (s.ctx, s.mom_smooth, s.p_coarse, s.eps_mom, s.eps_p)
+                    try:
+                        s.ctx = ctx
+                        s.mom_smooth = s.p_coarse = True
+                        s.eps_mom, s.eps_p = eps_mom, eps_p
+                        last = s.step()
+                    finally:
+                        s.ctx, s.mom_smooth, s.p_coarse, s.eps_mom, s.eps_p = saved
                     out[k] = (last["u"], last["v"], last["p"])
             return out
 
-        res = {}
-        for part in self._pool.map(run, range(workers)):
-            res.update(part)
+        futures = [self._pool.submit(run, w) for w in range(workers)]
+        concurrent.futures.wait(futures)
         for st, _ in self._workers:
             main.wait_stream(st)
+        res = {}
+        for f in futures:
+            res.update(f.result())   # re-raises a worker's error once every worker has ended
         return [res[k] for k in range(self.n)]
 
     def time_forward(self):

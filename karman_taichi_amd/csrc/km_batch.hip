@@ -39,6 +39,7 @@ bool same_table(const km_batch* B, int n, const km_case* cases) {
 }
 
 void drop_stage_graphs(km_batch* B) {
+    KmHostLock lk(km_host_mutex());
     for (auto& g : B->stage_graph)
         if (g) {
             (void)hipGraphExecDestroy(g);
@@ -69,9 +70,9 @@ int set_tile(km_ctx* ctx, km_batch* B, int TI, int TJ) {
         }
         B->tblocks[f] = ts[n];
         B->fblocks[f] = fs[n];
-        KM_HIP(ctx, hipMemcpy(B->d_sys[f], B->h_sys[f].data(), sizeof(BSys) * n, hipMemcpyHostToDevice));
-        KM_HIP(ctx, hipMemcpy(B->d_tstart[f], ts.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-        KM_HIP(ctx, hipMemcpy(B->d_fstart[f], fs.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+        KM_HIP_SYNC(ctx, hipMemcpy(B->d_sys[f], B->h_sys[f].data(), sizeof(BSys) * n, hipMemcpyHostToDevice));
+        KM_HIP_SYNC(ctx, hipMemcpy(B->d_tstart[f], ts.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+        KM_HIP_SYNC(ctx, hipMemcpy(B->d_fstart[f], fs.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
     }
     B->TI = TI;
     B->TJ = TJ;
@@ -86,12 +87,12 @@ int prepare(km_ctx* ctx, int n, const km_case* cases) {
     km_batch* B = ctx->batch;
     if (same_table(B, n, cases)) return KM_OK;
     KM_HIP(ctx, hipSetDevice(ctx->device));
-    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
     if (!B) B = ctx->batch = new km_batch();
     drop_stage_graphs(B);
     km_drop_graphs(ctx);
     if (B->arena) {
-        KM_HIP(ctx, hipFree(B->arena));
+        KM_HIP_SYNC(ctx, hipFree(B->arena));
         B->arena = nullptr;
     }
     B->n = n;
@@ -110,7 +111,11 @@ int prepare(km_ctx* ctx, int n, const km_case* cases) {
     const size_t sz_vec = a256(sizeof(double) * (size_t)vec);
     const size_t total = sz_cases + sz_stage + 3 * (sz_sys + 2 * sz_starts) + sz_states + sz_P + sz_vec;
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, total);
+    hipError_t e;
+    {
+        KmHostLock lk(km_host_mutex());
+        e = hipMalloc(&p, total);
+    }
     if (e != hipSuccess) return km_fail(ctx, KM_ENOMEM, "batch hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
     B->arena = static_cast<char*>(p);
     B->bytes = total;
@@ -125,7 +130,7 @@ int prepare(km_ctx* ctx, int n, const km_case* cases) {
     B->d_states = reinterpret_cast<KState*>(q); q += sz_states;
     B->P = reinterpret_cast<double*>(q); q += sz_P;
     B->vec = reinterpret_cast<double*>(q);
-    KM_HIP(ctx, hipMemset(B->d_states, 0, sz_states));
+    KM_HIP_SYNC(ctx, hipMemset(B->d_states, 0, sz_states));
     // case table and stage prefix sums
     std::vector<BCase> hc(n);
     std::vector<int> st(BS_COUNT * (n + 1));
@@ -145,8 +150,8 @@ int prepare(km_ctx* ctx, int n, const km_case* cases) {
         }
     }
     for (int s = 0; s < BS_COUNT; ++s) B->stage_blocks[s] = st[s * (n + 1) + n];
-    KM_HIP(ctx, hipMemcpy(B->d_cases, hc.data(), sizeof(BCase) * n, hipMemcpyHostToDevice));
-    KM_HIP(ctx, hipMemcpy(B->d_stage, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice));
+    KM_HIP_SYNC(ctx, hipMemcpy(B->d_cases, hc.data(), sizeof(BCase) * n, hipMemcpyHostToDevice));
+    KM_HIP_SYNC(ctx, hipMemcpy(B->d_stage, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice));
     // solve tables: u (coef_u, b_u -> u_mid), v (-> v_mid), p' (-> pcor)
     double* vp = B->vec;
     for (int f = 0; f < 3; ++f) B->h_sys[f].assign(n, BSys{});
@@ -195,17 +200,7 @@ int run_stages(km_ctx* ctx, km_batch* B, int which, bool use_graph) {
     };
     if (!use_graph) return enqueue(ctx->stream);
     if (!B->stage_graph[which]) {
-        hipGraph_t graph = nullptr;
-        KM_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue(ctx->cap_stream);
-        hipError_t ce = hipStreamEndCapture(ctx->cap_stream, &graph);
-        if (rc != KM_OK || ce != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return rc != KM_OK ? rc : km_fail(ctx, KM_EHIP, "stage capture failed: %s", hipGetErrorString(ce));
-        }
-        hipError_t ie = hipGraphInstantiate(&B->stage_graph[which], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ie != hipSuccess) return km_fail(ctx, KM_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        if (int rc = km_capture(ctx, enqueue, &B->stage_graph[which])) return rc;
     }
     KM_HIP(ctx, hipGraphLaunch(B->stage_graph[which], ctx->stream));
     return KM_OK;
@@ -225,8 +220,8 @@ int autotune(km_ctx* ctx, km_batch* B, int f) {
         for (int ti : {1, 2, 4, 8, 16})
             if (tile_ok(ti, tj) && ti * tj >= 128) cand.push_back({ti, tj});
     hipEvent_t e0, e1;
-    KM_HIP(ctx, hipEventCreate(&e0));
-    KM_HIP(ctx, hipEventCreate(&e1));
+    KM_HIP_SYNC(ctx, hipEventCreate(&e0));
+    KM_HIP_SYNC(ctx, hipEventCreate(&e1));
     double best = 1e300;
     std::pair<int, int> pick = {B->TI, B->TJ};
     for (const auto& c : cand) {
@@ -237,7 +232,7 @@ int autotune(km_ctx* ctx, km_batch* B, int f) {
                                     B->d_states + 3 * B->n + f, B->h_states.data()))
             return rc;
         KM_HIP(ctx, hipEventRecord(e1, ctx->stream));
-        KM_HIP(ctx, hipEventSynchronize(e1));
+        KM_HIP_SYNC(ctx, hipEventSynchronize(e1));
         float ms = 0.f;
         KM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
         if (ms < best) {
@@ -245,8 +240,8 @@ int autotune(km_ctx* ctx, km_batch* B, int f) {
             pick = c;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e0); }
+    { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e1); }
     B->tuned = true;
     return set_tile(ctx, B, pick.first, pick.second);
 }
@@ -280,6 +275,7 @@ int check_cases(km_ctx* ctx, int n, const km_case* cases) {
 void km_batch_free(km_ctx* ctx) {
     if (!ctx || !ctx->batch) return;
     drop_stage_graphs(ctx->batch);
+    KmHostLock lk(km_host_mutex());
     if (ctx->batch->arena) (void)hipFree(ctx->batch->arena);
     delete ctx->batch;
     ctx->batch = nullptr;

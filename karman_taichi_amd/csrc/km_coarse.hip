@@ -501,6 +501,7 @@ size_t a256(size_t v) { return (v + 255) & ~size_t(255); }
 // Xn = X M for row-major m x m X, M: the column-major product M^T X^T
 // (a plain library GEMM)
 int ns_gemm(km_ctx* ctx, int m, const double* M, const double* X, double* Xn, hipStream_t s) {
+    KmHostLock lk(km_host_mutex());   // handle creation and the first GEMM allocate and load code
     if (!ctx->blas) {
         rocblas_handle h = nullptr;
         if (rocblas_create_handle(&h) != rocblas_status_success)
@@ -605,12 +606,16 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s, bool s
                         a256((size_t)kMaxSlices * m * 8) + 2 * a256((size_t)m * 8) + a256((size_t)4 * m * 8) +
                         a256((size_t)2 * kMaxVS * m * 8) + a256((size_t)n) + 256;
     if (!C->arena || C->bytes < need) {
-        KM_HIP(ctx, hipStreamSynchronize(s));
+        KM_HIP_SYNC(ctx, hipStreamSynchronize(s));
         km_drop_graphs(ctx);
-        if (C->arena) KM_HIP(ctx, hipFree(C->arena));
+        if (C->arena) KM_HIP_SYNC(ctx, hipFree(C->arena));
         C->arena = nullptr;
         void* p = nullptr;
-        hipError_t e = hipMalloc(&p, need);
+        hipError_t e;
+        {
+            KmHostLock lk(km_host_mutex());
+            e = hipMalloc(&p, need);
+        }
         if (e != hipSuccess) return km_fail(ctx, KM_ENOMEM, "coarse hipMalloc(%zu): %s", need, hipGetErrorString(e));
         C->arena = static_cast<char*>(p);
         C->bytes = need;
@@ -745,6 +750,7 @@ template int apply<double>(km_ctx*, const double*, const double*, const double*,
 
 void km_coarse_free(km_ctx* ctx) {
     if (!ctx) return;
+    KmHostLock lk(km_host_mutex());
     if (ctx->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
     ctx->blas = nullptr;
     bool cur_in_pool = false;

@@ -13,7 +13,9 @@
 #include <cstdio>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <set>
+#include <shared_mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -161,6 +163,31 @@ int km_fail(km_ctx* ctx, int code, const char* fmt, ...);
 
 #define KM_CHECK_LAUNCH(ctx) KM_HIP(ctx, hipGetLastError())
 
+// Graph captures and the HIP calls that may synchronise the device or
+// allocate (hipMalloc, hipFree, synchronous copies and fills, stream and event
+// waits, graph destruction, rocBLAS set-up and GEMMs) are ordered by one
+// process-wide lock.  A capture holds it exclusively from
+// hipStreamBeginCapture to hipGraphInstantiate; the other calls hold it
+// shared while they run.  hipFree and hipMalloc synchronise the whole device,
+// and a device-wide synchronisation while another host thread's stream is
+// capturing invalidates that capture: hipStreamEndCapture then fails with
+// "operation failed due to a previous error during capture".  Contexts driven
+// from several host threads at once (one per worker, each on its own stream)
+// therefore never overlap a capture with such a call.
+std::shared_mutex& km_host_mutex();
+using KmCaptureLock = std::unique_lock<std::shared_mutex>;
+using KmHostLock = std::shared_lock<std::shared_mutex>;
+
+#define KM_HIP_SYNC(ctx, expr)                     \
+    do {                                           \
+        KmHostLock lk_(km_host_mutex());           \
+        KM_HIP(ctx, expr);                         \
+    } while (0)
+
+// Capture the work `body` enqueues on the stream it is given (the context's
+// private capture stream) and instantiate it, under the exclusive lock.
+int km_capture(km_ctx* ctx, const std::function<int(hipStream_t)>& body, hipGraphExec_t* out);
+
 #define KM_REQUIRE(ctx, cond, ...)                     \
     do {                                               \
         if (!(cond)) return km_fail((ctx), KM_EINVAL, __VA_ARGS__); \
@@ -203,19 +230,14 @@ inline int run_chunks(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int
         if (it != ctx->graphs.end()) {
             exec = it->second;
         } else {
-            hipGraph_t graph = nullptr;
-            KM_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
-            for (int c = 0; c < chunk; ++c) enqueue(ctx->cap_stream, c % KM_RING);
-            hipError_t le = hipGetLastError();
-            hipError_t ce = hipStreamEndCapture(ctx->cap_stream, &graph);
-            if (le != hipSuccess || ce != hipSuccess) {
-                if (graph) (void)hipGraphDestroy(graph);
-                return km_fail(ctx, KM_EHIP, "graph capture failed: %s / %s", hipGetErrorString(le),
-                               hipGetErrorString(ce));
-            }
-            hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (ie != hipSuccess) return km_fail(ctx, KM_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+            const int rc = km_capture(
+                ctx,
+                [&](hipStream_t q) {
+                    for (int c = 0; c < chunk; ++c) enqueue(q, c % KM_RING);
+                    return KM_OK;
+                },
+                &exec);
+            if (rc) return rc;
             ctx->graphs[key] = exec;
         }
     }
@@ -239,11 +261,11 @@ inline int run_chunks(km_ctx* ctx, KState* st, GraphKey key, bool use_graph, int
         if (int rc = launch_chunk()) return rc;
         KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[c & 1], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
         KM_HIP(ctx, hipEventRecord(ctx->ev[c & 1], ctx->stream));
-        KM_HIP(ctx, hipEventSynchronize(ctx->ev[(c - 1) & 1]));
+        KM_HIP_SYNC(ctx, hipEventSynchronize(ctx->ev[(c - 1) & 1]));
         if (ctx->h_st[(c - 1) & 1].done) break;
     }
     KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[2], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
-    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
     *final_state = ctx->h_st[2];
     return KM_OK;
 }
@@ -268,19 +290,15 @@ inline int run_chunks_sched(km_ctx* ctx, KState* st, GraphKey key, bool use_grap
             *out = it->second;
             return KM_OK;
         }
-        hipGraph_t graph = nullptr;
-        KM_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
-        for (int c = 0; c < len; ++c) enqueue(ctx->cap_stream, (ph0 + c) % KM_RING);
-        hipError_t le = hipGetLastError();
-        hipError_t ce = hipStreamEndCapture(ctx->cap_stream, &graph);
-        if (le != hipSuccess || ce != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return km_fail(ctx, KM_EHIP, "graph capture failed: %s / %s", hipGetErrorString(le), hipGetErrorString(ce));
-        }
         hipGraphExec_t exec = nullptr;
-        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ie != hipSuccess) return km_fail(ctx, KM_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        const int rc = km_capture(
+            ctx,
+            [&](hipStream_t q) {
+                for (int c = 0; c < len; ++c) enqueue(q, (ph0 + c) % KM_RING);
+                return KM_OK;
+            },
+            &exec);
+        if (rc) return rc;
         ctx->graphs[k] = exec;
         *out = exec;
         return KM_OK;
@@ -312,11 +330,11 @@ inline int run_chunks_sched(km_ctx* ctx, KState* st, GraphKey key, bool use_grap
         if (int rc = launch_chunk()) return rc;
         KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[c & 1], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
         KM_HIP(ctx, hipEventRecord(ctx->ev[c & 1], ctx->stream));
-        KM_HIP(ctx, hipEventSynchronize(ctx->ev[(c - 1) & 1]));
+        KM_HIP_SYNC(ctx, hipEventSynchronize(ctx->ev[(c - 1) & 1]));
         if (ctx->h_st[(c - 1) & 1].done) break;
     }
     KM_HIP(ctx, hipMemcpyAsync(&ctx->h_st[2], st, sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
-    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
     *final_state = ctx->h_st[2];
     return KM_OK;
 }
@@ -346,19 +364,15 @@ inline int run_chunks_flag(km_ctx* ctx, GraphKey key, bool use_graph, int big, i
             *out = it->second;
             return KM_OK;
         }
-        hipGraph_t graph = nullptr;
-        KM_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
-        for (int c = 0; c < len; ++c) enqueue(ctx->cap_stream, (ph0 + c) % KM_RING);
-        hipError_t le = hipGetLastError();
-        hipError_t ce = hipStreamEndCapture(ctx->cap_stream, &graph);
-        if (le != hipSuccess || ce != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return km_fail(ctx, KM_EHIP, "graph capture failed: %s / %s", hipGetErrorString(le), hipGetErrorString(ce));
-        }
         hipGraphExec_t exec = nullptr;
-        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ie != hipSuccess) return km_fail(ctx, KM_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        const int rc = km_capture(
+            ctx,
+            [&](hipStream_t q) {
+                for (int c = 0; c < len; ++c) enqueue(q, (ph0 + c) % KM_RING);
+                return KM_OK;
+            },
+            &exec);
+        if (rc) return rc;
         ctx->graphs[k] = exec;
         *out = exec;
         return KM_OK;
@@ -423,7 +437,7 @@ inline int run_chunks_flag(km_ctx* ctx, GraphKey key, bool use_graph, int big, i
     // the final state comes from the host words the stopping test wrote
     if (tail)
         if (int rc = (*tail)()) return rc;
-    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
     std::atomic_thread_fence(std::memory_order_acquire);
     for (int q = 0; q < nsys; ++q) {
         volatile int* w = hf + KM_HW * q;

@@ -18,7 +18,31 @@ int km_fail(km_ctx* ctx, int code, const char* fmt, ...) {
     return code;
 }
 
+std::shared_mutex& km_host_mutex() {
+    static std::shared_mutex m;
+    return m;
+}
+
+int km_capture(km_ctx* ctx, const std::function<int(hipStream_t)>& body, hipGraphExec_t* out) {
+    KmCaptureLock lk(km_host_mutex());
+    hipGraph_t graph = nullptr;
+    KM_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = body(ctx->cap_stream);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(ctx->cap_stream, &graph);
+    if (rc != KM_OK || le != hipSuccess || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc != KM_OK) return rc;
+        return km_fail(ctx, KM_EHIP, "graph capture failed: %s / %s", hipGetErrorString(le), hipGetErrorString(ce));
+    }
+    const hipError_t ie = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return km_fail(ctx, KM_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    return KM_OK;
+}
+
 void km_drop_graphs(km_ctx* ctx) {
+    KmHostLock lk(km_host_mutex());
     for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     ctx->graphs.clear();
 }
@@ -39,22 +63,26 @@ int km_ensure_workspace(km_ctx* ctx, size_t vec_bytes) {
     if (ctx->ws && ctx->ws_bytes >= need) return KM_OK;
     KM_HIP(ctx, hipSetDevice(ctx->device));
     if (ctx->ws) {
-        KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
         km_drop_graphs(ctx);
-        KM_HIP(ctx, hipFree(ctx->ws));
+        KM_HIP_SYNC(ctx, hipFree(ctx->ws));
         ctx->ws = nullptr;
         ctx->ws_bytes = 0;
     }
     size_t bytes = need + need / 4;   // headroom so small growth does not thrash
-    hipError_t e = hipMalloc(&ctx->ws, bytes);
+    hipError_t e;
+    {
+        KmHostLock lk(km_host_mutex());
+        e = hipMalloc(&ctx->ws, bytes);
+    }
     if (e != hipSuccess) {
         ctx->ws = nullptr;
         return km_fail(ctx, KM_ENOMEM, "workspace hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
     }
     ctx->ws_bytes = bytes;
     ctx->ws_gen++;
-    // zero once, synchronously, outside any captured graph
-    KM_HIP(ctx, hipMemset(ctx->ws, 0, header_bytes()));
+    // zero the header once, ordered on the context's stream before any use
+    KM_HIP(ctx, hipMemsetAsync(ctx->ws, 0, header_bytes(), ctx->stream));
     char* base = static_cast<char*>(ctx->ws);
     size_t off = 0;
     ctx->st = reinterpret_cast<KState*>(base + off);
@@ -75,6 +103,7 @@ int km_create(km_ctx** out, int device, void* hip_stream) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return KM_EHIP;
     if (device < 0 || device >= n) return KM_EINVAL;
+    KmHostLock lk(km_host_mutex());
     km_ctx* ctx = new km_ctx();
     ctx->device = device;
     ctx->stream = static_cast<hipStream_t>(hip_stream);
@@ -107,11 +136,15 @@ int km_create(km_ctx** out, int device, void* hip_stream) {
 int km_destroy(km_ctx* ctx) {
     if (!ctx) return KM_OK;
     (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) {
+        KmHostLock lk(km_host_mutex());
+        (void)hipStreamSynchronize(ctx->stream);
+    }
     km_drop_graphs(ctx);
     km_net_free(ctx);
     km_batch_free(ctx);
     km_coarse_free(ctx);
+    KmHostLock lk(km_host_mutex());
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->h_st) (void)hipHostFree(ctx->h_st);
     if (ctx->h_res) (void)hipHostFree(ctx->h_res);
@@ -133,7 +166,7 @@ const char* km_last_error(const km_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 
 int km_synchronize(km_ctx* ctx) {
     if (!ctx) return KM_EINVAL;
-    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
     return KM_OK;
 }
 

@@ -478,7 +478,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         if (!(flags & KM_FLAG_X0)) KM_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
         if (bb == 0.0) {
             KM_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
-            KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
             if (info) {
                 info->iterations = 0;
                 info->status = KM_OK;

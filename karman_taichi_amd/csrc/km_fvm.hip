@@ -582,7 +582,7 @@ int km_mdiv(km_ctx* ctx, const km_grid* g, const double* u, const double* v, dou
         hipLaunchKernelGGL(k_mdiv_finish, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, fl.G, ctx->dres);
         KM_CHECK_LAUNCH(ctx);
         KM_HIP(ctx, hipMemcpyAsync(ctx->h_res, ctx->dres, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
         stats[0] = ctx->h_res[0];
         stats[1] = ctx->h_res[1];
     }

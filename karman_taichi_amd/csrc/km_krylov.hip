@@ -774,7 +774,7 @@ int residual_norm2(km_ctx* ctx, const Tile& tl, const double* coef, const double
     KM_CHECK_LAUNCH(ctx);
     if (sync) {
         KM_HIP(ctx, hipMemcpyAsync(ctx->h_res, ctx->dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
         *out = ctx->h_res[0];
     }
     return KM_OK;
@@ -792,7 +792,7 @@ int dot_sync(km_ctx* ctx, int64_t n, int prec, const void* a, const void* b, dou
     hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, fl.G, ctx->dres + 1);
     KM_CHECK_LAUNCH(ctx);
     KM_HIP(ctx, hipMemcpyAsync(ctx->h_res + 1, ctx->dres + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
     *out = ctx->h_res[1];
     return KM_OK;
 }
@@ -833,8 +833,8 @@ int autotune_tiles(km_ctx* ctx, int NX, int NY, int prec, const T* coef, const T
     double* P = ctx->partials;
     const Flat fl = make_flat(n);
     hipEvent_t e0, e1;
-    KM_HIP(ctx, hipEventCreate(&e0));
-    KM_HIP(ctx, hipEventCreate(&e1));
+    KM_HIP_SYNC(ctx, hipEventCreate(&e0));
+    KM_HIP_SYNC(ctx, hipEventCreate(&e1));
     double best = 1e300;
     std::pair<int, int> pick = cand[0];
     for (const auto& c : cand) {
@@ -849,18 +849,18 @@ int autotune_tiles(km_ctx* ctx, int NX, int NY, int prec, const T* coef, const T
             launch_bicg_b<T>(tl, ph, coef, r, rt, vbuf, sv, t, st, P, lds, ctx->stream);
         }
         (void)hipEventRecord(e1, ctx->stream);
-        KM_HIP(ctx, hipEventSynchronize(e1));
+        KM_HIP_SYNC(ctx, hipEventSynchronize(e1));
         KM_CHECK_LAUNCH(ctx);
         float ms = 0.f;
         KM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-        KM_HIP(ctx, hipMemcpy(&ctx->h_st[3], st, sizeof(KState), hipMemcpyDeviceToHost));
+        KM_HIP_SYNC(ctx, hipMemcpy(&ctx->h_st[3], st, sizeof(KState), hipMemcpyDeviceToHost));
         if (!ctx->h_st[3].done && ms < best) {   // a candidate whose run stopped early is not comparable
             best = ms;
             pick = c;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e0); }
+    { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e1); }
     ctx->tiles[std::make_tuple(NX, NY, prec)] = pick;
     return KM_OK;
 }
@@ -996,7 +996,7 @@ int batch_bicgstab(km_ctx* ctx, int n, const BSys* sys, const BSys* d_sys, const
     KState fin;
     if (int rc = run_chunks(ctx, agg, key, use_graph, kChunk, max_iter, enqueue, &fin)) return rc;
     KM_HIP(ctx, hipMemcpyAsync(states, sys[0].st, n * sizeof(KState), hipMemcpyDeviceToHost, ctx->stream));
-    KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
     return KM_OK;
 }
 
@@ -1067,10 +1067,10 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
     double* P = ctx->partials;
     const size_t lds = tile_lds<double>(tl);
     std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
-    for (auto& e : ev) KM_HIP(ctx, hipEventCreate(&e));
+    for (auto& e : ev) KM_HIP_SYNC(ctx, hipEventCreate(&e));
     auto cleanup = [&]() {
         for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
+            if (e) { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e); }
     };
     hipLaunchKernelGGL(k_bicg_init<double>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, rt, pbuf, vbuf,
                        0, 0.0, 1 << 30, 0.0, st);
@@ -1109,7 +1109,11 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
                               which == KM_KERNEL_BICG_B ? e0 : nullptr, e1);
     }
     hipError_t le = hipGetLastError();
-    hipError_t se = hipStreamSynchronize(ctx->stream);
+    hipError_t se;
+    {
+        KmHostLock lk(km_host_mutex());
+        se = hipStreamSynchronize(ctx->stream);
+    }
     if (le != hipSuccess || se != hipSuccess) {
         cleanup();
         return km_fail(ctx, KM_EHIP, "km_time_kernel: %s / %s", hipGetErrorString(le), hipGetErrorString(se));
@@ -1125,7 +1129,7 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
         total_ms += ms;
     }
     cleanup();
-    KM_HIP(ctx, hipMemcpy(&ctx->h_st[3], st, sizeof(KState), hipMemcpyDeviceToHost));
+    KM_HIP_SYNC(ctx, hipMemcpy(&ctx->h_st[3], st, sizeof(KState), hipMemcpyDeviceToHost));
     KM_REQUIRE(ctx, !ctx->h_st[3].done, "km_time_kernel: the iteration stopped during timing (status %d)",
                ctx->h_st[3].status);
     *avg_us = 1e3 * total_ms / reps;

@@ -831,8 +831,8 @@ int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
     KM_REQUIRE(ctx, fewest <= kMaxVS, "aggregates of %d cells overlap %d tiles", C->B, fewest);
     const int64_t n = (int64_t)C->NX * C->NY;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    KM_HIP(ctx, hipEventCreate(&e0));
-    KM_HIP(ctx, hipEventCreate(&e1));
+    KM_HIP_SYNC(ctx, hipEventCreate(&e0));
+    KM_HIP_SYNC(ctx, hipEventCreate(&e1));
     int best = -1;
     float best_ms = 0.f;
     for (int w = 0; w < kNumTiles; ++w) {
@@ -865,12 +865,16 @@ int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
         (void)hipEventRecord(e0, ctx->stream);
         for (int k = 0; k < 16; ++k) once();
         (void)hipEventRecord(e1, ctx->stream);
-        hipError_t se = hipEventSynchronize(e1);
+        hipError_t se;
+        {
+            KmHostLock lk(km_host_mutex());
+            se = hipEventSynchronize(e1);
+        }
         float ms = 0.f;
         if (se == hipSuccess) se = hipEventElapsedTime(&ms, e0, e1);
         if (se != hipSuccess) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
+            { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e0); }
+            { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e1); }
             return km_fail(ctx, KM_EHIP, "smoothing tile timing: %s", hipGetErrorString(se));
         }
         if (best < 0 || ms < best_ms) {
@@ -878,8 +882,8 @@ int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
             best_ms = ms;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e0); }
+    { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e1); }
     C->tiles[key] = best;
     *out = best;
     return KM_OK;
@@ -1104,10 +1108,10 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
     const Fused<double> f(ctx, NX, NY, coef, ws, x, tile);
     if (int rc = fused_start<double>(ctx, f, b, 0.0, 1 << 30)) return rc;
     std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
-    for (auto& e : ev) KM_HIP(ctx, hipEventCreate(&e));
+    for (auto& e : ev) KM_HIP_SYNC(ctx, hipEventCreate(&e));
     auto cleanup = [&]() {
         for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
+            if (e) { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e); }
     };
     hipStream_t q = ctx->stream;
     auto launch = [&](int k, int phs) {
@@ -1141,7 +1145,11 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
                 one(k, (warm + it) % KM_RING, k == which ? ev[2 * it] : nullptr, k == which ? ev[2 * it + 1] : nullptr);
     }
     hipError_t le = hipGetLastError();
-    hipError_t se = hipStreamSynchronize(q);
+    hipError_t se;
+    {
+        KmHostLock lk(km_host_mutex());
+        se = hipStreamSynchronize(q);
+    }
     if (le != hipSuccess || se != hipSuccess) {
         cleanup();
         return km_fail(ctx, KM_EHIP, "km_time_kernel: %s / %s", hipGetErrorString(le), hipGetErrorString(se));
@@ -1157,7 +1165,7 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
         total_ms += ms;
     }
     cleanup();
-    KM_HIP(ctx, hipMemcpy(&ctx->h_st[3], f.st, sizeof(KState), hipMemcpyDeviceToHost));
+    KM_HIP_SYNC(ctx, hipMemcpy(&ctx->h_st[3], f.st, sizeof(KState), hipMemcpyDeviceToHost));
     KM_REQUIRE(ctx, !ctx->h_st[3].done, "km_time_kernel: the iteration stopped during timing (status %d)",
                ctx->h_st[3].status);
     *avg_us = 1e3 * total_ms / reps;

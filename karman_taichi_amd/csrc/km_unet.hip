@@ -854,14 +854,18 @@ int alloc(km_ctx* ctx, km_net* N, int B, int NX, int NY) {
     for (const auto& it : items) total += align256(it.bytes);
     KM_HIP(ctx, hipSetDevice(ctx->device));
     if (N->arena) {
-        KM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
         km_drop_graphs(ctx);     // captured graphs hold pointers into the arena
-        KM_HIP(ctx, hipFree(N->arena));
+        KM_HIP_SYNC(ctx, hipFree(N->arena));
         N->arena = nullptr;
         N->bytes = 0;
     }
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, total);
+    hipError_t e;
+    {
+        KmHostLock lk(km_host_mutex());
+        e = hipMalloc(&p, total);
+    }
     if (e != hipSuccess) return km_fail(ctx, KM_ENOMEM, "network hipMalloc(%zu) failed: %s", total, hipGetErrorString(e));
     N->arena = static_cast<char*>(p);
     N->bytes = total;
@@ -877,8 +881,8 @@ int alloc(km_ctx* ctx, km_net* N, int B, int NX, int NY) {
     N->W = W;
     int64_t gs = n / 4096 + 1;
     N->Gs = (int)(gs > 256 ? 256 : gs);
-    KM_HIP(ctx, hipMemcpy(N->dL, N->lay.L, sizeof(LayerOff) * NL, hipMemcpyHostToDevice));
-    KM_HIP(ctx, hipMemset(N->stats, 0, kMaxBatch * 8 * sizeof(double)));
+    KM_HIP_SYNC(ctx, hipMemcpy(N->dL, N->lay.L, sizeof(LayerOff) * NL, hipMemcpyHostToDevice));
+    KM_HIP_SYNC(ctx, hipMemset(N->stats, 0, kMaxBatch * 8 * sizeof(double)));
     return KM_OK;
 }
 
@@ -1063,6 +1067,7 @@ template int apply<double>(km_ctx*, int, int, const float*, const double*, doubl
 
 void km_net_free(km_ctx* ctx) {
     if (!ctx || !ctx->net) return;
+    KmHostLock lk(km_host_mutex());
     if (ctx->net->arena) (void)hipFree(ctx->net->arena);
     delete ctx->net;
     ctx->net = nullptr;
@@ -1125,7 +1130,7 @@ int km_unet_loss_grad(km_ctx* ctx, const float* params, int B, int NX, int NY, c
     KM_CHECK_LAUNCH(ctx);
     if (loss) {
         KM_HIP(ctx, hipMemcpyAsync(ctx->h_res + 32, N->loss, B * sizeof(double), hipMemcpyDeviceToHost, s));
-        KM_HIP(ctx, hipStreamSynchronize(s));
+        KM_HIP_SYNC(ctx, hipStreamSynchronize(s));
         double m = 0.0;
         for (int b = 0; b < B; ++b) m += ctx->h_res[32 + b];
         *loss = m / B;
