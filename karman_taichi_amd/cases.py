@@ -196,18 +196,6 @@ def cavity(n: int = 32, re: float = 100.0, **kw) -> Case:
 CASES = {"c1": c1, "c2": c2, "c5": c5}
 
 
-def strouhal_from_signal(sig: np.ndarray, dt: float, d: float, u_ref: float = 1.0) -> float:
-    """Dominant-frequency Strouhal number f D / U of a lift signal (zero-mean FFT peak)."""
-    s = np.asarray(sig, dtype=np.float64)
-    s = s - s.mean()
-    if s.size < 8 or not np.any(s):
-        return float("nan")
-    spec = np.abs(np.fft.rfft(s * np.hanning(s.size)))
-    freq = np.fft.rfftfreq(s.size, dt)
-    k = int(np.argmax(spec[1:])) + 1
-    return float(freq[k] * d / u_ref)
-
-
 def _selfcheck():  # pragma: no cover - quick manual check
     c = c1()
     print(c.obstacle_box(), c.mu, math.isclose(c.mu, 0.125 / 150))

@@ -29,7 +29,7 @@ from typing import Optional
 import numpy as np
 import torch
 
-from . import _lib, cases
+from . import _lib, cases, post
 from ._lib import KmCase, KmGrid, KmSolveInfo, ptr
 from .solvers import BICGSolver, CGSolver, SolveInfo
 
@@ -298,38 +298,21 @@ class SIMPLESolver:
                 np.savetxt(f"{folder}/{step:06}-{pre}{sys_}-{msg}.csv", planes[k], delimiter=",")
             np.savetxt(f"{folder}/{step:06}-b{sys_} -{msg}.csv", b.detach().cpu().numpy(), delimiter=",")
 
-    def forces(self):
-        """Pressure + viscous force on the obstacle (drag, lift) per unit depth.
+    def write_fields(self, time_step, folder="."):
+        """The five field files the reference writes every 10th time step
+        (karman_taichi.py:678-683): ``timestep{n:06}_{pcor,p,u,v,udiv}.csv``
+        (udiv is mdiv), each as ``write_matrix`` writes it."""
+        f = self.fields_numpy()
+        src = {"pcor": f["pcor"], "p": f["p"], "u": f["u"], "v": f["v"], "udiv": f["mdiv"]}
+        return [post.write_matrix(src[k], f"{folder}/{name}") for k, name in post.field_dump_names(time_step).items()]
 
-        Pressure acts on the obstacle's outer faces (cell-centre p of the fluid
-        neighbour); wall shear is mu * (u_t - 0) / (d/2) with the tangential
-        velocity of the adjacent fluid face (the same 2 mu / d wall model the
-        momentum equations use).
-        """
-        m = self.ct.cpu().numpy().astype(bool)
-        if not m.any():
-            return 0.0, 0.0
-        p = self.p.cpu().numpy()
-        u = self.u.cpu().numpy()
-        v = self.v.cpu().numpy()
-        ii, jj = np.nonzero(m)
-        i0, i1, j0, j1 = ii.min(), ii.max(), jj.min(), jj.max()
-        dx, dy, mu = self.dx, self.dy, self.mu
-        J = slice(j0, j1 + 1)
-        I = slice(i0, i1 + 1)
-        fx = (p[i0 - 1, J] - p[i1 + 1, J]).sum() * dy
-        fy = (p[I, j0 - 1] - p[I, j1 + 1]).sum() * dx
-        # shear on top/bottom faces from u at faces i0+1..i1 (interior of the span)
-        ut = 0.5 * (u[i0:i1 + 1, j1 + 1] + u[i0 + 1:i1 + 2, j1 + 1])
-        ub = 0.5 * (u[i0:i1 + 1, j0 - 1] + u[i0 + 1:i1 + 2, j0 - 1])
-        fx += mu * (ut.sum() + ub.sum()) * dx / (0.5 * dy)
-        vl = 0.5 * (v[i0 - 1, j0:j1 + 1] + v[i0 - 1, j0 + 1:j1 + 2])
-        vr = 0.5 * (v[i1 + 1, j0:j1 + 1] + v[i1 + 1, j0 + 1:j1 + 2])
-        fy += mu * (vl.sum() + vr.sum()) * dy / (0.5 * dx)
-        return float(fx), float(fy)
+    def forces(self):
+        """Pressure + viscous force (drag, lift) per unit depth on the
+        obstacle (``post.obstacle_forces``)."""
+        return post.obstacle_forces(self.p.cpu().numpy(), self.u.cpu().numpy(), self.v.cpu().numpy(),
+                                    self.ct.cpu().numpy(), self.dx, self.dy, self.mu)
 
     def drag_lift_coefficients(self, u_ref=1.0):
         d = self.case.obstacle_diameter() if self.case else 1.0
         fx, fy = self.forces()
-        q = 0.5 * self.rho * u_ref * u_ref * d
-        return fx / q, fy / q
+        return post.force_coefficients(fx, fy, d, self.rho, u_ref)

@@ -430,6 +430,28 @@ def test_dumps_named_as_reference(dev, tmp_path):
     np.testing.assert_array_equal(np.loadtxt(tmp_path / "000007-pcor-t.csv", delimiter=","), N(s.pcor))
 
 
+def test_field_files_and_display(dev, tmp_path):
+    """write_fields: the reference's per-time-step CSV names
+    (karman_taichi.py:678-683) with the fields' values; Display.display: the
+    stacked image of display.py:28-50 as a PNG."""
+    from karman_taichi_amd import post
+    c = cases.c1()
+    s = SIMPLESolver.from_case(c, device=dev)
+    s.init()
+    s.step()
+    paths = s.write_fields(10, folder=str(tmp_path))
+    names = sorted(p.rsplit("/", 1)[-1] for p in paths)
+    assert names == sorted(f"timestep000010_{f}.csv" for f in ("pcor", "p", "u", "v", "udiv"))
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "timestep000010_udiv.csv", delimiter=","), N(s.mdiv))
+    np.testing.assert_array_equal(np.loadtxt(tmp_path / "timestep000010_u.csv", delimiter=","), N(s.u))
+    png = str(tmp_path / "timestep000010.png")
+    post.Display(s).display(png)
+    f = s.fields_numpy()
+    want = post.image_bytes(post.display_fields(f["u"], f["v"], f["p"], f["pcor"], f["mdiv"]))
+    np.testing.assert_array_equal(post.read_png_gray(png), want)
+    assert want.shape == (5 * (c.ny + 2), c.nx + 2)
+
+
 def test_large_grid_properties(dev):
     """C5-size operator: linearity and symmetry of the assembled p' matrix,
     bitwise run-to-run repeatability of the fused reductions."""
