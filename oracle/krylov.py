@@ -70,7 +70,7 @@ class Info(dict):
     pass
 
 
-def bicgstab(coef, b, eps=1e-8, x0=None, max_iter=None, precond=None, history=None):
+def bicgstab(coef, b, eps=1e-8, x0=None, max_iter=None, precond=None, history=None, rho_form="direct"):
     """BiCGSTAB of bicgsolver.py:88-136 (van der Vorst, right-preconditioned hooks).
 
     x0 = 0 and r~ = r = b (bicgsolver.py:21-35) unless ``x0`` is given (then
@@ -82,6 +82,11 @@ def bicgstab(coef, b, eps=1e-8, x0=None, max_iter=None, precond=None, history=No
     converged, BREAKDOWN (the omega == 0 check of karman_taichi.py:451-454);
     non-finite residual -> NONFINITE.  max_iter defaults to NX*NY
     (cgsolver.py:14).
+
+    ``rho_form``: "direct" forms rho = <r, r~> as bicgsolver.py:95 does;
+    "split" forms it from the previous iteration's vectors as
+    <s, r~> - omega <t, r~>, which equals <r, r~> in exact arithmetic (r = s -
+    omega t) and is what a fused GPU iteration can reduce before r exists.
     """
     NX, NY = b.shape
     steps = NX * NY if max_iter is None else max_iter
@@ -104,8 +109,11 @@ def bicgstab(coef, b, eps=1e-8, x0=None, max_iter=None, precond=None, history=No
     rTr = initial_rTr
     status = MAXITER
     it = 0
+    if rho_form not in ("direct", "split"):
+        raise ValueError(rho_form)
+    s_rt = t_rt = 0.0
     for i in range(steps):
-        rho = dot(r, r_tld)
+        rho = dot(r, r_tld) if (rho_form == "direct" or i == 0) else s_rt - omega * t_rt
         if rho == 0.0:
             status = BREAKDOWN
             break
@@ -126,6 +134,8 @@ def bicgstab(coef, b, eps=1e-8, x0=None, max_iter=None, precond=None, history=No
         t = stencil(coef, s_hat)
         tt = dot(t, t)
         omega = dot(t, s) / tt if tt != 0.0 else 0.0
+        if rho_form == "split":
+            s_rt, t_rt = dot(s, r_tld), dot(t, r_tld)
         x = x + (alpha * p_hat + omega * s_hat)
         r = s - omega * t
         rTr = dot(r, r)

@@ -115,6 +115,31 @@ def test_bicgstab_scalar_sequence_vs_textbook():
     np.testing.assert_allclose(x.ravel(), x_ref, rtol=1e-11, atol=1e-14)
 
 
+def test_split_rho_form_tracks_the_direct_form():
+    """rho = <s, r~> - omega <t, r~> (the fused GPU iteration's form) against
+    the reference's <r, r~> (bicgsolver.py:95): the same scalar sequence while
+    rounding is small (a 8x8 system, first 6 iterations, 1e-10); on the C1
+    p' system the same LU solution, but rounding makes the split form take
+    more iterations (measured 630 vs 596 to 1e-6, 768 vs 687 to 1e-8, 889 vs
+    778 to 1e-10): within 20% here."""
+    c, b = random_5pt(8, 8, seed=5)
+    _, d = krylov.bicgstab(c, b, eps=0.0, max_iter=6)
+    _, s_ = krylov.bicgstab(c, b, eps=0.0, max_iter=6, rho_form="split")
+    for key in ("rho", "alpha", "omega"):
+        for a, bb in zip(d[key], s_[key]):
+            assert math.isclose(a, bb, rel_tol=1e-10), (key, a, bb)
+    coef, rhs = p_system(cases.c1(n_outer=1), steps=1)
+    xd = krylov.direct_solve(coef, rhs)
+    out = {}
+    for form in ("direct", "split"):
+        x, info = krylov.bicgstab(coef, rhs, eps=1e-10, rho_form=form)
+        assert info["status"] == krylov.OK
+        assert np.linalg.norm(x - xd) <= 1e-5 * np.linalg.norm(xd)
+        out[form] = info["iterations"]
+    print("C1 p' iterations to 1e-10:", out)
+    assert out["direct"] <= out["split"] <= 1.2 * out["direct"]
+
+
 def test_bicgstab_iterations_near_scipy():
     c, b = random_5pt(32, 32, seed=7, shift=0.01)
     A = krylov.to_csr(c)
