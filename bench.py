@@ -30,10 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per cell of the two fused BiCGSTAB kernels (FP64):
-#   A reads s, t, p, v, x, r~ and 5 coefficient planes, writes r, p', v', x
-#   B reads r, v', r~ and 5 coefficient planes, writes s, t
-BYTES_PER_CELL = {"bicg_a": 15 * 8, "bicg_b": 10 * 8,
+# algorithmic bytes per cell of the three BiCGSTAB kernels (FP64):
+#   bicg_p reads r, p, v, r~ and 5 coefficient planes, writes p', v'
+#   bicg_s reads r, v' and 5 coefficient planes, writes s, t
+#   bicg_r reads s, t, p', x, r~, writes x, r
+BYTES_PER_CELL = {"bicg_p": 11 * 8, "bicg_s": 9 * 8, "bicg_r": 7 * 8,
                   # the fused two-level iteration (FP64 vectors, 1-byte free-cell mask):
                   #   tl_pv reads p, 5 coefficient planes, r~, mask; writes p^, v
                   #   tl_st reads r, v, 5 coefficient planes, r~, mask; writes s, s^, t
@@ -95,6 +96,11 @@ def parse():
     ap.add_argument("--case-workers", type=int, default=3,
                     help="c3 with the two-level mode: step the cases as independent SIMPLE steps, this many at a "
                          "time on their own contexts and streams (0: km_batch_step)")
+    ap.add_argument("--spinup-time-steps", type=int, default=100,
+                    help="time steps the flow is advanced from the seeded start before the warm-up (untimed): the "
+                         "timed window starts at this time step")
+    ap.add_argument("--learned-reps", type=int, default=3,
+                    help="repetitions of the learned+FP32 vs plain p' comparison on the final system (0: skip)")
     ap.add_argument("--staged", action="store_true",
                     help="time the step stage by stage from Python instead of one km_simple_step call")
     ap.add_argument("--stage-steps", type=int, default=8,
@@ -178,7 +184,14 @@ def main():
             rec.append({"mom_ms": (t1 - t0) * 1e3, "p_ms": (t2 - t1) * 1e3, "p_it": ip.iterations,
                         "u_it": iu.iterations, "v_it": iv.iterations, "p_true": ip.rel_resid_true, "p_status": ip.status})
 
-    log(f"bench: case {case.name} mode {args.mode}, warmup {args.warmup}, steps {args.steps}")
+    log(f"bench: case {case.name} mode {args.mode}, spin-up {args.spinup_time_steps} time steps, "
+        f"warmup {args.warmup}, steps {args.steps}")
+    t_spin = time.perf_counter()
+    for _ in range(args.spinup_time_steps * n_outer):
+        one_step()
+    torch.cuda.synchronize()
+    if args.spinup_time_steps:
+        log(f"bench: spin-up to time step {args.spinup_time_steps} in {time.perf_counter() - t_spin:.1f} s")
     for _ in range(args.warmup):
         one_step()
     if twolevel and args.totals != -1:   # the coarse level exists after the first solve
@@ -205,6 +218,7 @@ def main():
         f"p' iters median {statistics.median(p_it)}"
         + (f", staged p' median {statistics.median(p_ms):.2f} ms" if p_ms else ""))
     compare = p_compare(s, args.compare_reps) if args.compare_reps > 0 else None
+    lcompare = learned_compare(s, args.weights, args.learned_reps) if args.learned_reps > 0 else None
 
     # ---- roofline: dominant kernel of the mode's p' iteration, timed with HIP
     # events on the solver stream (one launch of each kernel per iteration)
@@ -217,7 +231,8 @@ def main():
                  ("tl_ct", _lib.KM_KERNEL_TL_CT))
         reps = 100       # well inside the stall window of the flexible iteration
     else:
-        kinds = (("bicg_a", _lib.KM_KERNEL_BICG_A), ("bicg_b", _lib.KM_KERNEL_BICG_B))
+        kinds = (("bicg_p", _lib.KM_KERNEL_BICG_A), ("bicg_s", _lib.KM_KERNEL_BICG_B),
+                 ("bicg_r", _lib.KM_KERNEL_BICG_R))
         reps = 200
     kern, kern_ev = {}, {}
     for name, kid in kinds:
@@ -285,7 +300,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if fp32 else "f64",
-        "data": "synthetic (seeded channel start state)",
+        "data": f"synthetic (seeded channel start state advanced {args.spinup_time_steps} time steps before the "
+                f"warm-up)",
         "config": {"workload": f"{case.name}: {case.nx}x{case.ny} channel, Re={case.re:g}, dt={case.dt}, "
                                f"{case.obstacle_cells}x{case.obstacle_cells} cylinder, "
                                f"{n_outer} outer steps per time step, inner eps 1e-8",
@@ -304,6 +320,7 @@ def main():
         "p_rel_resid_true_max": max(r["p_true"] for r in rec),
         "p_status_all_ok": all(r["p_status"] == _lib.KM_OK for r in rec),
         "p_compare": compare,
+        "learned_compare": lcompare,
         "p_cells_iters_per_s": round(cells_p * total_p_it / total_p_s, 1) if p_ms else None,
         "us_per_p_iter": round(total_p_s / total_p_it * 1e6, 3) if p_ms else None,
         "kernel_us_back_to_back": {k: round(v, 3) for k, v in kern.items()},
@@ -341,6 +358,45 @@ def p_compare(s, reps):
                      "rel_resid_true": last.rel_resid_true, "status": last.status_name}
     out["speedup"] = round(out["plain"]["ms_median"] / out["mode"]["ms_median"], 3)
     log(f"bench: p' compare {out}")
+    return out
+
+
+def learned_compare(s, weights, reps):
+    """The metric's second leg on the last outer step's p' system, each solve
+    from zero to 1e-8 in the true residual, ``reps`` times: plain FP64
+    BiCGSTAB against the learned flexible BiCGSTAB (two-level preconditioner
+    with the trained U-Net as its smoother, BF16 network on the matrix
+    cores, FP32 Krylov vectors, FP64 scalars and true residual), with the
+    same solve without the network (coarse correction and red-black SOR
+    sweeps, FP32 vectors) beside it.  Median wall ms (each solve returns
+    after a stream synchronisation), iterations, true residuals."""
+    import torch
+    from karman_taichi_amd import learned as kl
+    if not os.path.exists(weights):
+        return {"status": f"unmeasured: no trained network parameters at {os.path.relpath(weights, ROOT)}"}
+    net = kl.UNetPreconditioner.load(weights, device=s.device)
+    x = torch.zeros_like(s.b_p)
+    runs = {
+        "plain": lambda: s.p_correction_solver.solve(eps=1e-8),
+        "learned_fp32": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, net, eps=1e-8, mixed=True, coarse=True),
+        "no_network_fp32": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, mixed=True, coarse=True),
+    }
+    out = {"weights": os.path.relpath(weights, ROOT)}
+    for name, fn in runs.items():
+        fn()
+        torch.cuda.synchronize()
+        ms, last = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            last = fn()
+            torch.cuda.synchronize()
+            ms.append((time.perf_counter() - t0) * 1e3)
+        out[name] = {"ms_median": round(statistics.median(ms), 3), "iterations": last.iterations,
+                     "rel_resid_true": last.rel_resid_true, "status": last.status_name,
+                     "fallback": bool(getattr(last, "fallback", 0))}
+    out["speedup_vs_plain"] = round(out["plain"]["ms_median"] / out["learned_fp32"]["ms_median"], 3)
+    out["speedup_vs_no_network"] = round(out["no_network_fp32"]["ms_median"] / out["learned_fp32"]["ms_median"], 3)
+    log(f"bench: learned compare {out}")
     return out
 
 

@@ -53,6 +53,10 @@ extern "C" {
 #define KM_FLAG_NO_GRAPH 4    /* launch kernels directly instead of replaying captured graphs */
 #define KM_FLAG_COARSE 8      /* km_fbicgstab: two-level preconditioner (aggregation coarse correction first) */
 #define KM_FLAG_SMOOTH 16     /* km_fbicgstab: one red-black Gauss-Seidel sweep as the preconditioner, no coarse level */
+#define KM_FLAG_TRUE 32       /* km_bicgstab: after the reference's stop on the recursive residual, check the true
+                               * residual ||b - A x|| / ||b|| and restart from x (r = b - A x) while it is above eps
+                               * and each restart at least halves it (at most 8 restarts; no counterpart in the
+                               * reference, which stops on the recursive residual, bicgsolver.py:128-131) */
 
 /* east boundary kinds */
 #define KM_EAST_WALL 0        /* simple.py:184-186, :214-216 (with the bicgsimp.py:196 fix) */
@@ -290,8 +294,9 @@ int km_set_coarse_chunks(km_ctx* ctx, int big, int small, int lead);
  * NX x NY system, from HIP events around `reps` launches on the context's
  * stream.  Used by the benchmark to report achieved bandwidth of the kernel
  * that dominates the solve; rocprofv3 --kernel-trace reports the same kernels. */
-#define KM_KERNEL_BICG_A 0   /* x, r, p updates + stencil + <r~, v>, <r, r> */
-#define KM_KERNEL_BICG_B 1   /* s update + stencil + <t, s>, <t, t>, <s, r~>, <t, r~> */
+#define KM_KERNEL_BICG_A 0   /* P: stopping test, beta; p update + stencil (v = A p) + <r~, v> */
+#define KM_KERNEL_BICG_B 1   /* S: alpha; s update + stencil (t = A s) + <t, s>, <t, t> */
+#define KM_KERNEL_BICG_R 2   /* R: omega; x, r updates (flat) + <r, r>, <r, r~> */
 #define KM_KERNEL_STENCIL 3  /* plain y = A x */
 /* the fused two-level flexible iteration (KM_FLAG_COARSE, no network) */
 #define KM_KERNEL_TL_PV 4    /* p^ = M(p) (prolongation + 2 red-black SOR sweeps); v = A p^; <r~, v>; P^T v */

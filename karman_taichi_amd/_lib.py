@@ -15,9 +15,9 @@ LIB_PATH = Path(__file__).resolve().parent / "libkarman_hip.so"
 KM_OK, KM_BREAKDOWN, KM_MAXITER, KM_NONFINITE, KM_STALL = 0, 1, 2, 3, 4
 KM_EINVAL, KM_EHIP, KM_ENOMEM = -1, -2, -3
 KM_F64, KM_F32 = 0, 1
-KM_FLAG_X0, KM_FLAG_MIXED, KM_FLAG_NO_GRAPH, KM_FLAG_COARSE, KM_FLAG_SMOOTH = 1, 2, 4, 8, 16
+KM_FLAG_X0, KM_FLAG_MIXED, KM_FLAG_NO_GRAPH, KM_FLAG_COARSE, KM_FLAG_SMOOTH, KM_FLAG_TRUE = 1, 2, 4, 8, 16, 32
 KM_EAST_WALL, KM_EAST_OUTFLOW = 0, 1
-KM_KERNEL_BICG_A, KM_KERNEL_BICG_B, KM_KERNEL_STENCIL = 0, 1, 3
+KM_KERNEL_BICG_A, KM_KERNEL_BICG_B, KM_KERNEL_BICG_R, KM_KERNEL_STENCIL = 0, 1, 2, 3
 KM_KERNEL_TL_PV, KM_KERNEL_TL_CV, KM_KERNEL_TL_ST, KM_KERNEL_TL_CT = 4, 5, 6, 7
 KM_KERNEL_BACK_TO_BACK = 0x100
 

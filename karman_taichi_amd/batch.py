@@ -55,7 +55,8 @@ class BatchedSIMPLE:
 
     def step(self, eps_mom: float = 1e-8, eps_p: float = 1e-8, max_iter: int = 0, graph: bool = True,
              coarse: bool = False, smooth: bool = False):
-        """One outer iteration of every case; returns [(u, v, p') SolveInfo] per case.
+        """One outer iteration of every case in FP64 (``Case.mixed`` is honoured by
+        :meth:`step_cases`); returns [(u, v, p') SolveInfo] per case.
         ``coarse``: p' by the two-level preconditioned flexible BiCGSTAB, case by
         case; ``smooth``: u and v by the smoothed flexible BiCGSTAB, case by case
         (km_fbicgstab2).  The stages, and the solves not selected, stay batched."""
@@ -70,9 +71,11 @@ class BatchedSIMPLE:
         """One outer iteration of every case as independent ``SIMPLESolver.step``
         calls (``km_simple_step``: smoothed u/v pair, two-level p'), ``workers``
         cases at a time, each worker with its own context and stream, so that
-        the latency-bound solves of different cases overlap on the GPU.  Case
-        k always runs on worker k mod ``workers`` (its coarse level stays
-        there).  Returns [(u, v, p') SolveInfo] per case."""
+        the latency-bound solves of different cases overlap on the GPU.  Each
+        case runs in its own precision (``Case.mixed``: FP32 Krylov vectors
+        with FP64 scalars and refinement).  Case k always runs on worker k mod
+        ``workers`` (its coarse level stays there).  Returns [(u, v, p')
+        SolveInfo] per case."""
         import concurrent.futures
         main = torch.cuda.current_stream(self.device)
         if getattr(self, "_workers", None) is None or len(self._workers) != workers:
@@ -91,14 +94,15 @@ class BatchedSIMPLE:
             with torch.cuda.stream(st):
                 for k in range(w, self.n, workers):
                     s = self.solvers[k]
-                    saved = (s.ctx, s.mom_smooth, s.p_coarse, s.eps_mom, s.eps_p)
+                    saved = (s.ctx, s.mom_smooth, s.p_coarse, s.eps_mom, s.eps_p, s.mixed)
                     try:
                         s.ctx = ctx
                         s.mom_smooth = s.p_coarse = True
                         s.eps_mom, s.eps_p = eps_mom, eps_p
+                        s.mixed = bool(self.cases[k].mixed)
                         last = s.step()
                     finally:
-                        s.ctx, s.mom_smooth, s.p_coarse, s.eps_mom, s.eps_p = saved
+                        s.ctx, s.mom_smooth, s.p_coarse, s.eps_mom, s.eps_p, s.mixed = saved
                     out[k] = (last["u"], last["v"], last["p"])
             return out
 

@@ -64,6 +64,7 @@ class Case:
     eps_p: float = 1e-8
     seed: int = 0
     v_perturb: float = 1e-3
+    mixed: bool = False               # FP32 Krylov vectors (FP64 scalars, true residuals and refinement)
 
     @property
     def dx(self) -> float:
@@ -170,8 +171,11 @@ def c5(**kw) -> Case:
 
 def batched_cases(n: int = 96, seed: int = 0) -> List[Case]:
     """Uneven batch: nx = 256 + 64 k (k ~ U{0..28}), ny = round(nx/5),
-    D = round(ny/8), Re ~ U[80, 300], dt = 0.64 dx (CFL 0.64), seeded."""
+    D = round(ny/8), Re ~ U[80, 300], dt = 0.64 dx (CFL 0.64), seeded; about
+    half the cases run with FP32 Krylov vectors (``mixed``, drawn from a
+    second generator seeded seed + 1 so the geometry draws stay as they are)."""
     rng = np.random.default_rng(seed)
+    prec = np.random.default_rng(seed + 1)
     out = []
     for b in range(n):
         nx = 256 + 64 * int(rng.integers(0, 29))
@@ -179,7 +183,8 @@ def batched_cases(n: int = 96, seed: int = 0) -> List[Case]:
         d = max(2, int(round(ny / 8)))
         re = float(rng.uniform(80.0, 300.0))
         dt = 0.64 * (5.0 / nx)
-        out.append(channel(f"batch{b:02d}_{nx}x{ny}", nx, ny, d, re, dt, 200, seed=seed + b))
+        out.append(channel(f"batch{b:02d}_{nx}x{ny}", nx, ny, d, re, dt, 200, seed=seed + b,
+                           mixed=bool(prec.random() < 0.5)))
     return out
 
 

@@ -103,31 +103,37 @@ int km_create(km_ctx** out, int device, void* hip_stream) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return KM_EHIP;
     if (device < 0 || device >= n) return KM_EINVAL;
-    KmHostLock lk(km_host_mutex());
     km_ctx* ctx = new km_ctx();
     ctx->device = device;
     ctx->stream = static_cast<hipStream_t>(hip_stream);
-    if (hipSetDevice(device) != hipSuccess) { delete ctx; return KM_EHIP; }
-    if (hipHostMalloc(reinterpret_cast<void**>(&ctx->h_st), 4 * sizeof(KState), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&ctx->h_res), 128 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-        delete ctx;
-        return KM_ENOMEM;
+    int rc = KM_OK;
+    {
+        KmHostLock lk(km_host_mutex());
+        void* flag = nullptr;
+        if (hipSetDevice(device) != hipSuccess) {
+            rc = KM_EHIP;
+        } else if (hipHostMalloc(reinterpret_cast<void**>(&ctx->h_st), 4 * sizeof(KState), hipHostMallocDefault) !=
+                       hipSuccess ||
+                   hipHostMalloc(reinterpret_cast<void**>(&ctx->h_res), 128 * sizeof(double),
+                                 hipHostMallocDefault) != hipSuccess) {
+            rc = KM_ENOMEM;
+        } else if (hipHostMalloc(&flag, KM_HW_WORDS * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped) !=
+                       hipSuccess ||
+                   hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->d_flag), flag, 0) != hipSuccess) {
+            if (flag) (void)hipHostFree(flag);
+            rc = KM_ENOMEM;
+        } else {
+            ctx->h_flag = static_cast<volatile int*>(flag);
+            if (hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&ctx->ev[0], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&ctx->ev[1], hipEventDisableTiming) != hipSuccess)
+                rc = KM_EHIP;
+        }
     }
-    void* flag = nullptr;
-    if (hipHostMalloc(&flag, KM_HW_WORDS * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
-        hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->d_flag), flag, 0) != hipSuccess) {
-        if (flag) (void)hipHostFree(flag);
-        (void)hipHostFree(ctx->h_st);
-        (void)hipHostFree(ctx->h_res);
-        delete ctx;
-        return KM_ENOMEM;
-    }
-    ctx->h_flag = static_cast<volatile int*>(flag);
-    if (hipStreamCreateWithFlags(&ctx->cap_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev[1], hipEventDisableTiming) != hipSuccess) {
-        delete ctx;
-        return KM_EHIP;
+    if (rc != KM_OK) {
+        ctx->stream = nullptr;   // nothing was enqueued
+        km_destroy(ctx);
+        return rc;
     }
     *out = ctx;
     return KM_OK;

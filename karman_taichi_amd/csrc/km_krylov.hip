@@ -113,6 +113,24 @@ __device__ __forceinline__ void write_partial(double* P, int slot, double v, dou
     if (threadIdx.x == 0) P[slot] = s;
 }
 
+// Two partial sums of the workgroup with one pair of barriers (sh: 8 doubles):
+// P[slot] = sum a, P[KM_MAX_RED + slot] = sum b, each in block_sum_all's order.
+__device__ __forceinline__ void write_partials2(double* P, int slot, double a, double b, double* sh) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        sh[w] = a;
+        sh[4 + w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        P[slot] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        P[KM_MAX_RED + slot] = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+    }
+}
+
 __device__ __forceinline__ bool finite(double x) { return x == x && x - x == 0.0; }
 
 __device__ __forceinline__ bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
@@ -182,30 +200,37 @@ k_add_scaled(int64_t n, double* __restrict__ x, const float* __restrict__ d, dou
 
 // ------------------------------------------------------------ BiCGSTAB
 //
-// Two kernels per iteration i (reference iteration, bicgsolver.py:94-135):
+// Three kernels per iteration i (bicgsolver.py:94-135), with rho formed as
+// the reference forms it, <r_i, r~> (bicgsolver.py:95):
 //
-//   A(i)  omega_{i-1} = <t,s>/<t,t> and rho_i = <s,r~> - omega_{i-1} <t,r~>
-//         from B(i-1)'s partials (rho_i = <r_i, r~> since r_i = s - omega t);
-//         on the tile + halo: r_i = s - omega t, p_i = r_i + beta (p - omega v);
-//         at the centre: x_i = x + (alpha p + omega s), store r_i and p_i;
-//         v_i = A p_i.  Partials <r~, v_i> and <r_i, r_i>.
-//   B(i)  the stopping test on ||r_i|| (the reference's test at the end of
-//         iteration i-1), the breakdown tests, alpha_i = rho_i / <r~, v_i>;
-//         on the tile + halo: s = r_i - alpha v_i; t = A s.  Partials <t,s>,
-//         <t,t>, <s,r~>, <t,r~>.
+//   P(i)  from R(i-1)'s partials <r_i, r_i> and <r_i, r~>: the stopping test
+//         on ||r_i|| (the reference's test at the end of iteration i-1,
+//         :128-131), the omega == 0 test (karman_taichi.py:451-454), the
+//         max-iteration cap, the rho == 0 breakdown (:96-98);
+//         beta = (rho_i / rho_{i-1}) (alpha / omega) (:102);
+//         on the tile + halo: p_i = r_i + beta (p - omega v) (:43-46);
+//         v_i = A p_i (:72-76).  Partials <r~, v_i> (:107).
+//   S(i)  alpha_i = rho_i / <r~, v_i> (:108); on the tile + halo:
+//         s = r_i - alpha v_i (:60-63); t = A s (:65-69).  Partials <t, s>,
+//         <t, t> (:116-117).
+//   R(i)  omega_i = <t,s> / <t,t> (:118); x += alpha p + omega s (:78-81);
+//         r_{i+1} = s - omega t (:83-86); flat, no halo.  Partials
+//         <r, r> (:127) and <r, r~>.
 //
-// Forming rho from <s,r~> and <t,r~> lets the x and r updates ride along in
-// the next A instead of a third kernel; it is the same quantity in exact
-// arithmetic.  Ring slot of the iteration's phase: rho_i (A), omega_{i-1} (A),
-// beta_i (A), alpha_i (B), iteration index (A).
+// The init kernel writes R's partials for r_0.  Ring slot of the iteration's
+// phase: rho_i, beta_i and the iteration index (P), alpha_i (S), omega_i
+// (R); a launch reads earlier slots and the fields other launches wrote.
 
 // bicgsolver.py:21-35: x = 0, r = r~ = b (or r precomputed from x0), the
-// parity-0 p and v buffers zeroed.
+// parity-0 p and v buffers zeroed; partials of <r, r> and <r, r~> = <r, r>
+// in R's planes.
 template <typename T>
 __device__ __forceinline__ void bicg_init_body(const Flat& fl, int bx, const T* __restrict__ b, T* __restrict__ x,
                                                T* __restrict__ r, T* __restrict__ rt, T* __restrict__ p0,
                                                T* __restrict__ v0, int from_r, double eps, int max_iter,
-                                               double ref_norm2, KState* st) {
+                                               double ref_norm2, KState* st, double* __restrict__ P) {
+    __shared__ double sh[4];
+    double rr = 0.0;
     flat_sweep_b(fl, bx, [&](int64_t k) {
         T rk;
         if (from_r) {
@@ -218,7 +243,10 @@ __device__ __forceinline__ void bicg_init_body(const Flat& fl, int bx, const T* 
         rt[k] = rk;
         p0[k] = T(0);
         v0[k] = T(0);
+        rr += (double)rk * (double)rk;
     });
+    write_partial(P + KM_PC(0), bx, rr, sh);
+    write_partial(P + KM_PC(0) + KM_MAX_RED, bx, rr, sh);
     if (bx == 0 && threadIdx.x == 0) {
         st->eps = eps;
         st->max_iter = max_iter;
@@ -234,8 +262,8 @@ template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_bicg_init(Flat fl, const T* __restrict__ b, T* __restrict__ x, T* __restrict__ r, T* __restrict__ rt,
             T* __restrict__ p0, T* __restrict__ v0, int from_r, double eps, int max_iter, double ref_norm2,
-            KState* st) {
-    bicg_init_body<T>(fl, blockIdx.x, b, x, r, rt, p0, v0, from_r, eps, max_iter, ref_norm2, st);
+            KState* st, double* __restrict__ P) {
+    bicg_init_body<T>(fl, blockIdx.x, b, x, r, rt, p0, v0, from_r, eps, max_iter, ref_norm2, st, P);
 }
 
 // The iteration kernels issue every global load of a tile (the halo inputs of
@@ -292,11 +320,10 @@ struct TileLoads {
 };
 
 // L: the workgroup's logical index within its system (0 .. tl.G-1); the
-// workgroup with L == 0 records the iteration's scalars.
+// workgroup with L == 0 records the iteration's scalars.  Gr: R's workgroups.
 template <typename T, int CPT>
-__device__ __forceinline__ void bicg_a_body(const Tile& tl, int L, int phase, const T* __restrict__ coef, T* r,
-                                            const T* __restrict__ rt, const T* __restrict__ s,
-                                            const T* __restrict__ t, T* __restrict__ x, T* pbuf, T* vbuf,
+__device__ __forceinline__ void bicg_p_body(const Tile& tl, int L, int Gr, int phase, const T* __restrict__ coef,
+                                            const T* __restrict__ r, const T* __restrict__ rt, T* pbuf, T* vbuf,
                                             KState* st, double* __restrict__ P) {
     T* lds = reinterpret_cast<T*>(km_smem);
     __shared__ double sh[4];
@@ -311,65 +338,58 @@ __device__ __forceinline__ void bicg_a_body(const Tile& tl, int L, int phase, co
     const int t0 = L * tl.tpb, t1 = t0 + tl.tpb < tl.T ? t0 + tl.tpb : tl.T;
     const KScal sp = st->ring[(phase + KM_RING - 1) % KM_RING];
     const int i = sp.iter + 1;
-    T bt = T(0), om = T(0), al = T(0);
-    double rv = 0.0, rr = 0.0;
+    T bt = T(0), om = T(0);
+    double rv = 0.0;
     for (int tile = t0; tile < t1; ++tile) {
         TileLoads<T, CPT> ld;
         ld.index(tl, tile_origin(tl, tile));
-        T sa[CPT + 2], ta[CPT + 2], pa[CPT + 2], va[CPT + 2], xa[CPT + 2], rtv[CPT];
+        T ra[CPT + 2], pa[CPT + 2], va[CPT + 2], rtv[CPT];
 #pragma unroll
         for (int q = 0; q < CPT + 2; ++q) {
             const bool v = ld.sval[q];
-            if (i == 0) {
-                sa[q] = v ? r[ld.sk[q]] : T(0);   // r_0 = b
-                ta[q] = pa[q] = va[q] = xa[q] = T(0);
-            } else {
-                sa[q] = v ? s[ld.sk[q]] : T(0);
-                ta[q] = v ? t[ld.sk[q]] : T(0);
-                pa[q] = v ? p_old[ld.sk[q]] : T(0);
-                va[q] = v ? v_old[ld.sk[q]] : T(0);
-                xa[q] = v && ld.scen[q] ? x[ld.sk[q]] : T(0);
-            }
+            ra[q] = v ? r[ld.sk[q]] : T(0);
+            pa[q] = v && i > 0 ? p_old[ld.sk[q]] : T(0);
+            va[q] = v && i > 0 ? v_old[ld.sk[q]] : T(0);
         }
         ld.load_coef(coef, N);
 #pragma unroll
         for (int q = 0; q < CPT; ++q) rtv[q] = ld.cval[q] ? rt[ld.ck[q]] : T(0);
-        if (tile == t0 && i > 0) {
-            double pb[4];
-            sum_partials<4>(P + KM_PB, tl.G, pb, sh);
-            const double omega = pb[1] != 0.0 ? pb[0] / pb[1] : 0.0;   // bicgsolver.py:118
-            const double rho = pb[2] - omega * pb[3];                  // <r_i, r~> (bicgsolver.py:95)
-            const double beta = omega != 0.0 ? (rho / sp.rho) * (sp.alpha / omega) : 0.0;   // :102
+        if (tile == t0) {
+            double pr[2];
+            sum_partials<2>(P + KM_PC(0), Gr, pr, sh);
+            const double rTr = pr[0], rho = pr[1];   // <r_i, r_i>, <r_i, r~> (bicgsolver.py:95)
+            const double r0 = i == 0 ? (st->ref > 0.0 ? st->ref : rTr) : st->r0;
+            int status = -1;
+            if (!finite(rTr)) status = KM_NONFINITE;
+            else if (i == 0 && rTr == 0.0) status = KM_OK;
+            else if (i > 0 && sqrt(rTr / r0) < st->eps) status = KM_OK;   // bicgsolver.py:130
+            else if (i > 0 && sp.omega == 0.0) status = KM_BREAKDOWN;     // karman_taichi.py:451
+            else if (i >= st->max_iter) status = KM_MAXITER;
+            else if (!finite(rho)) status = KM_NONFINITE;
+            else if (rho == 0.0) status = KM_BREAKDOWN;                   // bicgsolver.py:96-98
+            if (lead0 && i == 0) st->r0 = r0;
+            if (status >= 0) {
+                if (lead0) finish(st, status, i, rTr);
+                return;
+            }
+            const double beta = i == 0 ? 0.0 : (rho / sp.rho) * (sp.alpha / sp.omega);   // :102
             if (lead0) {
                 st->ring[phase].rho = rho;
-                st->ring[phase].omega = omega;
                 st->ring[phase].beta = beta;
                 st->ring[phase].iter = i;
+                st->rTr = rTr;
+                st->iter = i;
             }
             bt = (T)beta;
-            om = (T)omega;
-            al = (T)sp.alpha;
-        } else if (tile == t0 && lead0) {
-            st->ring[phase].iter = 0;
-            st->ring[phase].omega = 1.0;
+            om = i == 0 ? T(0) : (T)sp.omega;
         }
-        // update_x, update_r of iteration i-1 (bicgsolver.py:78-86) and
-        // update_p of iteration i (:43-46), on the tile and its halo
+        // update_p (bicgsolver.py:43-46; p = r at i = 0) on the tile and its halo
 #pragma unroll
         for (int q = 0; q < CPT + 2; ++q) {
             const bool v = ld.sval[q];
-            const T ri = i == 0 ? sa[q] : sa[q] - om * ta[q];
-            const T pn = v ? ri + bt * (pa[q] - om * va[q]) : T(0);
+            const T pn = v ? ra[q] + bt * (pa[q] - om * va[q]) : T(0);
             if (ld.slds[q] >= 0) lds[ld.slds[q]] = pn;
-            if (v && ld.scen[q]) {
-                const int64_t k = ld.sk[q];
-                p_new[k] = pn;
-                if (i > 0) {
-                    x[k] = xa[q] + (al * pa[q] + om * sa[q]);
-                    r[k] = ri;
-                }
-                rr += (double)ri * (double)ri;
-            }
+            if (v && ld.scen[q]) p_new[ld.sk[q]] = pn;
         }
         __syncthreads();
         // compute_Ap (bicgsolver.py:72-76) and reduce(r~, v) (:107)
@@ -383,73 +403,45 @@ __device__ __forceinline__ void bicg_a_body(const Tile& tl, int L, int phase, co
         __syncthreads();
     }
     write_partial(P + KM_PA, L, rv, sh);
-    write_partial(P + KM_PA + KM_MAX_RED, L, rr, sh);
 }
 
 template <typename T, int CPT>
-__global__ void __launch_bounds__(KM_BLOCK)
-k_bicg_a(Tile tl, int phase, const T* __restrict__ coef, T* r, const T* __restrict__ rt, const T* __restrict__ s,
-         const T* __restrict__ t, T* __restrict__ x, T* pbuf, T* vbuf, KState* st, double* __restrict__ P) {
-    if (st->done) return;
-    bicg_a_body<T, CPT>(tl, xcd_remap(blockIdx.x, tl.G), phase, coef, r, rt, s, t, x, pbuf, vbuf, st, P);
-}
-
-template <typename T, int CPT>
-__device__ __forceinline__ void bicg_b_body(const Tile& tl, int L, int phase, const T* __restrict__ coef,
-                                            const T* __restrict__ r, const T* __restrict__ rt, const T* vbuf,
-                                            T* __restrict__ s, T* __restrict__ t, KState* st,
-                                            double* __restrict__ P) {
+__device__ __forceinline__ void bicg_s_body(const Tile& tl, int L, int phase, const T* __restrict__ coef,
+                                            const T* __restrict__ r, const T* vbuf, T* __restrict__ s,
+                                            T* __restrict__ t, KState* st, double* __restrict__ P) {
     T* lds = reinterpret_cast<T*>(km_smem);
-    __shared__ double sh[4];
+    __shared__ double sh[8];
     const int64_t N = (int64_t)tl.NX * tl.NY;
     const T* __restrict__ v = vbuf + (1 - (phase & 1)) * N;
     const int W = tl.TJ + 2;
     const bool lead0 = L == 0 && threadIdx.x == 0;
     const int t0 = L * tl.tpb, t1 = t0 + tl.tpb < tl.T ? t0 + tl.tpb : tl.T;
     T al = T(0);
-    double ts = 0.0, tt = 0.0, srt = 0.0, trt = 0.0;
+    double ts = 0.0, tt = 0.0;
     for (int tile = t0; tile < t1; ++tile) {
         TileLoads<T, CPT> ld;
         ld.index(tl, tile_origin(tl, tile));
-        T ra[CPT + 2], va[CPT + 2], rtv[CPT];
+        T ra[CPT + 2], va[CPT + 2];
 #pragma unroll
         for (int q = 0; q < CPT + 2; ++q) {
             ra[q] = ld.sval[q] ? r[ld.sk[q]] : T(0);
             va[q] = ld.sval[q] ? v[ld.sk[q]] : T(0);
         }
         ld.load_coef(coef, N);
-#pragma unroll
-        for (int q = 0; q < CPT; ++q) rtv[q] = ld.cval[q] ? rt[ld.ck[q]] : T(0);
         if (tile == t0) {
-            double pa[2];
-            sum_partials<2>(P + KM_PA, tl.G, pa, sh);
-            const double rv = pa[0], rTr = pa[1];
-            const KScal sp = st->ring[phase];
-            const int i = sp.iter;
-            const double rho = i == 0 ? rTr : sp.rho;
-            const double r0 = i == 0 ? (st->ref > 0.0 ? st->ref : rTr) : st->r0;
+            double pa[1];
+            sum_partials<1>(P + KM_PA, tl.G, pa, sh);
+            const double rv = pa[0];
+            const double rho = st->ring[phase].rho;
             int status = -1;
-            if (!finite(rTr)) status = KM_NONFINITE;
-            else if (i == 0 && rTr == 0.0) status = KM_OK;
-            else if (i > 0 && sqrt(rTr / r0) < st->eps) status = KM_OK;   // bicgsolver.py:130
-            else if (i > 0 && sp.omega == 0.0) status = KM_BREAKDOWN;     // karman_taichi.py:451
-            else if (i >= st->max_iter) status = KM_MAXITER;
-            else if (!finite(rho)) status = KM_NONFINITE;
-            else if (rho == 0.0) status = KM_BREAKDOWN;                   // bicgsolver.py:96-98
-            else if (!finite(rv)) status = KM_NONFINITE;
+            if (!finite(rv)) status = KM_NONFINITE;
             else if (rv == 0.0) status = KM_BREAKDOWN;
-            if (lead0 && i == 0) st->r0 = r0;
             if (status >= 0) {
-                if (lead0) finish(st, status, i, rTr);
+                if (lead0) finish(st, status, st->ring[phase].iter, st->rTr);
                 return;
             }
             const double alpha = rho / rv;   // bicgsolver.py:108
-            if (lead0) {
-                st->ring[phase].alpha = alpha;
-                if (i == 0) st->ring[phase].rho = rho;
-                st->rTr = rTr;
-                st->iter = i;
-            }
+            if (lead0) st->ring[phase].alpha = alpha;
             al = (T)alpha;
         }
         // update_s (bicgsolver.py:60-63) on the tile and its halo
@@ -460,7 +452,7 @@ __device__ __forceinline__ void bicg_b_body(const Tile& tl, int L, int phase, co
             if (ld.sval[q] && ld.scen[q]) s[ld.sk[q]] = sn;
         }
         __syncthreads();
-        // compute_As (:65-69); reduce(t, s), reduce(t, t) (:116-117); <s, r~>, <t, r~>
+        // compute_As (:65-69); reduce(t, s), reduce(t, t) (:116-117)
 #pragma unroll
         for (int q = 0; q < CPT; ++q)
             if (ld.cval[q]) {
@@ -469,23 +461,84 @@ __device__ __forceinline__ void bicg_b_body(const Tile& tl, int L, int phase, co
                 t[ld.ck[q]] = y;
                 ts += (double)y * (double)sk;
                 tt += (double)y * (double)y;
-                srt += (double)sk * (double)rtv[q];
-                trt += (double)y * (double)rtv[q];
             }
         __syncthreads();
     }
-    write_partial(P + KM_PB, L, ts, sh);
-    write_partial(P + KM_PB + KM_MAX_RED, L, tt, sh);
-    write_partial(P + KM_PB + 2 * KM_MAX_RED, L, srt, sh);
-    write_partial(P + KM_PB + 3 * KM_MAX_RED, L, trt, sh);
+    write_partials2(P + KM_PB, L, ts, tt, sh);
+}
+
+// R: bx = the workgroup's index in the flat geometry, Gs = S's workgroups.
+// Each thread keeps kRPer cells' loads in flight.
+constexpr int kRPer = 4;
+template <typename T>
+__device__ __forceinline__ void bicg_r_body(const Flat& fl, int bx, int Gs, int phase, const T* __restrict__ rt,
+                                            const T* __restrict__ s, const T* __restrict__ t, const T* pbuf,
+                                            T* __restrict__ x, T* __restrict__ r, KState* st,
+                                            double* __restrict__ P) {
+    __shared__ double sh[8];
+    const T* __restrict__ p = pbuf + (1 - (phase & 1)) * fl.n;   // p_i, written by P(i)
+    const int64_t k0 = (int64_t)bx * fl.chunk;
+    const int64_t k1 = k0 + fl.chunk < fl.n ? k0 + fl.chunk : fl.n;
+    // the first loads go out before the partial sums are reduced
+    T sv[kRPer], tv[kRPer], pv[kRPer], xv[kRPer], rtv[kRPer];
+    int64_t kk[kRPer];
+    auto load = [&](int64_t base) {
+#pragma unroll
+        for (int u = 0; u < kRPer; ++u) {
+            kk[u] = base + threadIdx.x + (int64_t)u * KM_BLOCK;
+            const bool ok = kk[u] < k1;
+            sv[u] = ok ? s[kk[u]] : T(0);
+            tv[u] = ok ? t[kk[u]] : T(0);
+            pv[u] = ok ? p[kk[u]] : T(0);
+            xv[u] = ok ? x[kk[u]] : T(0);
+            rtv[u] = ok ? rt[kk[u]] : T(0);
+        }
+    };
+    load(k0);
+    double pb[2];
+    sum_partials<2>(P + KM_PB, Gs, pb, sh);
+    const double omega = pb[1] != 0.0 ? pb[0] / pb[1] : 0.0;   // bicgsolver.py:118
+    const double alpha = st->ring[phase].alpha;
+    if (bx == 0 && threadIdx.x == 0) st->ring[phase].omega = omega;
+    const T al = (T)alpha, om = (T)omega;
+    double rr = 0.0, rrt = 0.0;
+    for (int64_t base = k0; base < k1; base += (int64_t)kRPer * KM_BLOCK) {
+        if (base != k0) load(base);
+#pragma unroll
+        for (int u = 0; u < kRPer; ++u) {
+            if (kk[u] >= k1) continue;
+            x[kk[u]] = xv[u] + (al * pv[u] + om * sv[u]);   // update_x (bicgsolver.py:78-81)
+            const T rk = sv[u] - om * tv[u];                 // update_r (:83-86)
+            r[kk[u]] = rk;
+            rr += (double)rk * (double)rk;
+            rrt += (double)rk * (double)rtv[u];
+        }
+    }
+    write_partials2(P + KM_PC(0), bx, rr, rrt, sh);
 }
 
 template <typename T, int CPT>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_bicg_b(Tile tl, int phase, const T* __restrict__ coef, const T* __restrict__ r, const T* __restrict__ rt,
-         const T* vbuf, T* __restrict__ s, T* __restrict__ t, KState* st, double* __restrict__ P) {
+k_bicg_p(Tile tl, int Gr, int phase, const T* __restrict__ coef, const T* __restrict__ r, const T* __restrict__ rt,
+         T* pbuf, T* vbuf, KState* st, double* __restrict__ P) {
     if (st->done) return;
-    bicg_b_body<T, CPT>(tl, xcd_remap(blockIdx.x, tl.G), phase, coef, r, rt, vbuf, s, t, st, P);
+    bicg_p_body<T, CPT>(tl, xcd_remap(blockIdx.x, tl.G), Gr, phase, coef, r, rt, pbuf, vbuf, st, P);
+}
+
+template <typename T, int CPT>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_bicg_s(Tile tl, int phase, const T* __restrict__ coef, const T* __restrict__ r, const T* vbuf, T* __restrict__ s,
+         T* __restrict__ t, KState* st, double* __restrict__ P) {
+    if (st->done) return;
+    bicg_s_body<T, CPT>(tl, xcd_remap(blockIdx.x, tl.G), phase, coef, r, vbuf, s, t, st, P);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_bicg_r(Flat fl, int Gs, int phase, const T* __restrict__ rt, const T* __restrict__ s, const T* __restrict__ t,
+         const T* pbuf, T* __restrict__ x, T* __restrict__ r, KState* st, double* __restrict__ P) {
+    if (st->done) return;
+    bicg_r_body<T>(fl, blockIdx.x, Gs, phase, rt, s, t, pbuf, x, r, st, P);
 }
 
 // ------------------------------------------------------------ batched systems
@@ -501,7 +554,7 @@ kb_bicg_init(const BSys* __restrict__ sys, const int* __restrict__ starts, int n
     const int64_t N = S.fl.n;
     double* ws = S.ws;
     bicg_init_body<double>(S.fl, blockIdx.x - starts[c], S.b, S.x, ws, ws + N, ws + 2 * N, ws + 4 * N, 0, eps,
-                           max_iter, 0.0, S.st);
+                           max_iter, 0.0, S.st, S.P);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         agg->done = 0;
         agg->status = KM_MAXITER;
@@ -510,9 +563,10 @@ kb_bicg_init(const BSys* __restrict__ sys, const int* __restrict__ starts, int n
     }
 }
 
+// tstarts: prefix sums of the systems' tile workgroups; fstarts: of their flat ones
 template <int CPT>
 __global__ void __launch_bounds__(KM_BLOCK)
-kb_bicg_a(const BSys* __restrict__ sys, const int* __restrict__ starts, int n, int phase) {
+kb_bicg_p(const BSys* __restrict__ sys, const int* __restrict__ starts, int n, int phase) {
     const int c = find_case(starts, n, blockIdx.x);
     const BSys& S = sys[c];
     if (S.st->done) return;
@@ -520,21 +574,32 @@ kb_bicg_a(const BSys* __restrict__ sys, const int* __restrict__ starts, int n, i
     const int64_t N = (int64_t)tl.NX * tl.NY;
     double* ws = S.ws;
     // the XCD-aware tile order of the single-system kernel, within the system
-    bicg_a_body<double, CPT>(tl, xcd_remap(blockIdx.x - starts[c], tl.G), phase, S.coef, ws, ws + N, ws + 6 * N,
-                             ws + 7 * N, S.x, ws + 2 * N, ws + 4 * N, S.st, S.P);
+    bicg_p_body<double, CPT>(tl, xcd_remap(blockIdx.x - starts[c], tl.G), S.fl.G, phase, S.coef, ws, ws + N,
+                             ws + 2 * N, ws + 4 * N, S.st, S.P);
 }
 
 template <int CPT>
 __global__ void __launch_bounds__(KM_BLOCK)
-kb_bicg_b(const BSys* __restrict__ sys, const int* __restrict__ starts, int n, int phase) {
+kb_bicg_s(const BSys* __restrict__ sys, const int* __restrict__ starts, int n, int phase) {
     const int c = find_case(starts, n, blockIdx.x);
     const BSys& S = sys[c];
     if (S.st->done) return;
     const Tile tl = S.tl;
     const int64_t N = (int64_t)tl.NX * tl.NY;
     double* ws = S.ws;
-    bicg_b_body<double, CPT>(tl, xcd_remap(blockIdx.x - starts[c], tl.G), phase, S.coef, ws, ws + N, ws + 4 * N,
-                             ws + 6 * N, ws + 7 * N, S.st, S.P);
+    bicg_s_body<double, CPT>(tl, xcd_remap(blockIdx.x - starts[c], tl.G), phase, S.coef, ws, ws + 4 * N, ws + 6 * N,
+                             ws + 7 * N, S.st, S.P);
+}
+
+__global__ void __launch_bounds__(KM_BLOCK)
+kb_bicg_r(const BSys* __restrict__ sys, const int* __restrict__ fstarts, int n, int phase) {
+    const int c = find_case(fstarts, n, blockIdx.x);
+    const BSys& S = sys[c];
+    if (S.st->done) return;
+    const int64_t N = S.fl.n;
+    double* ws = S.ws;
+    bicg_r_body<double>(S.fl, blockIdx.x - fstarts[c], S.tl.G, phase, ws + N, ws + 6 * N, ws + 7 * N, ws + 2 * N,
+                        S.x, ws, S.st, S.P);
 }
 
 // agg->done = every system done (one workgroup)
@@ -563,24 +628,50 @@ inline int tile_cpt(const Tile& tl) {
         else hipLaunchKernelGGL(kern, dim3(g), dim3(KM_BLOCK), l, s, __VA_ARGS__);                    \
     } while (0)
 
+// One BiCGSTAB iteration (P, S, R).  `timed` (a KM_KERNEL_BICG_* id, or -1)
+// selects the launch bracketed by e0 / e1.
 template <typename T>
-void launch_bicg_a(const Tile& tl, int phase, const T* coef, T* r, const T* rt, const T* sv, const T* t, T* x,
-                   T* pbuf, T* vbuf, KState* st, double* P, size_t lds, hipStream_t s, hipEvent_t e0 = nullptr,
-                   hipEvent_t e1 = nullptr) {
+void launch_bicg_iter(const Tile& tl, const Flat& fl, int phase, const T* coef, T* r, const T* rt, T* sv, T* t, T* x,
+                      T* pbuf, T* vbuf, KState* st, double* P, size_t lds, hipStream_t s, int timed = -1,
+                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    hipEvent_t a0 = timed == KM_KERNEL_BICG_A ? e0 : nullptr, a1 = timed == KM_KERNEL_BICG_A ? e1 : nullptr;
     switch (tile_cpt(tl)) {
-        case 1: KM_LAUNCH((k_bicg_a<T, 1>), tl.G, lds, s, e0, e1, tl, phase, coef, r, rt, sv, t, x, pbuf, vbuf, st, P); break;
-        case 2: KM_LAUNCH((k_bicg_a<T, 2>), tl.G, lds, s, e0, e1, tl, phase, coef, r, rt, sv, t, x, pbuf, vbuf, st, P); break;
-        default: KM_LAUNCH((k_bicg_a<T, 4>), tl.G, lds, s, e0, e1, tl, phase, coef, r, rt, sv, t, x, pbuf, vbuf, st, P);
+        case 1: KM_LAUNCH((k_bicg_p<T, 1>), tl.G, lds, s, a0, a1, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P); break;
+        case 2: KM_LAUNCH((k_bicg_p<T, 2>), tl.G, lds, s, a0, a1, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P); break;
+        default: KM_LAUNCH((k_bicg_p<T, 4>), tl.G, lds, s, a0, a1, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P);
     }
+    a0 = timed == KM_KERNEL_BICG_B ? e0 : nullptr;
+    a1 = timed == KM_KERNEL_BICG_B ? e1 : nullptr;
+    switch (tile_cpt(tl)) {
+        case 1: KM_LAUNCH((k_bicg_s<T, 1>), tl.G, lds, s, a0, a1, tl, phase, coef, r, vbuf, sv, t, st, P); break;
+        case 2: KM_LAUNCH((k_bicg_s<T, 2>), tl.G, lds, s, a0, a1, tl, phase, coef, r, vbuf, sv, t, st, P); break;
+        default: KM_LAUNCH((k_bicg_s<T, 4>), tl.G, lds, s, a0, a1, tl, phase, coef, r, vbuf, sv, t, st, P);
+    }
+    a0 = timed == KM_KERNEL_BICG_R ? e0 : nullptr;
+    a1 = timed == KM_KERNEL_BICG_R ? e1 : nullptr;
+    KM_LAUNCH((k_bicg_r<T>), fl.G, 0, s, a0, a1, fl, tl.G, phase, rt, (const T*)sv, (const T*)t, (const T*)pbuf, x, r,
+              st, P);
 }
 
+// A single kernel of an iteration (back-to-back timing).
 template <typename T>
-void launch_bicg_b(const Tile& tl, int phase, const T* coef, const T* r, const T* rt, const T* vbuf, T* sv, T* t,
-                   KState* st, double* P, size_t lds, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    switch (tile_cpt(tl)) {
-        case 1: KM_LAUNCH((k_bicg_b<T, 1>), tl.G, lds, s, e0, e1, tl, phase, coef, r, rt, vbuf, sv, t, st, P); break;
-        case 2: KM_LAUNCH((k_bicg_b<T, 2>), tl.G, lds, s, e0, e1, tl, phase, coef, r, rt, vbuf, sv, t, st, P); break;
-        default: KM_LAUNCH((k_bicg_b<T, 4>), tl.G, lds, s, e0, e1, tl, phase, coef, r, rt, vbuf, sv, t, st, P);
+void launch_bicg_one(int which, const Tile& tl, const Flat& fl, int phase, const T* coef, T* r, const T* rt, T* sv,
+                     T* t, T* x, T* pbuf, T* vbuf, KState* st, double* P, size_t lds, hipStream_t s) {
+    if (which == KM_KERNEL_BICG_A) {
+        switch (tile_cpt(tl)) {
+            case 1: hipLaunchKernelGGL((k_bicg_p<T, 1>), tl.G, KM_BLOCK, lds, s, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P); break;
+            case 2: hipLaunchKernelGGL((k_bicg_p<T, 2>), tl.G, KM_BLOCK, lds, s, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P); break;
+            default: hipLaunchKernelGGL((k_bicg_p<T, 4>), tl.G, KM_BLOCK, lds, s, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P);
+        }
+    } else if (which == KM_KERNEL_BICG_B) {
+        switch (tile_cpt(tl)) {
+            case 1: hipLaunchKernelGGL((k_bicg_s<T, 1>), tl.G, KM_BLOCK, lds, s, tl, phase, coef, r, vbuf, sv, t, st, P); break;
+            case 2: hipLaunchKernelGGL((k_bicg_s<T, 2>), tl.G, KM_BLOCK, lds, s, tl, phase, coef, r, vbuf, sv, t, st, P); break;
+            default: hipLaunchKernelGGL((k_bicg_s<T, 4>), tl.G, KM_BLOCK, lds, s, tl, phase, coef, r, vbuf, sv, t, st, P);
+        }
+    } else {
+        hipLaunchKernelGGL((k_bicg_r<T>), fl.G, KM_BLOCK, 0, s, fl, tl.G, phase, rt, (const T*)sv, (const T*)t,
+                           (const T*)pbuf, x, r, st, P);
     }
 }
 
@@ -719,7 +810,8 @@ const int kChunk = 8;   // iterations per captured chunk (a multiple of KM_RING)
 // Plain BiCGSTAB on typed buffers.  ws holds 8n elements of T.
 template <typename T>
 int bicgstab_typed(km_ctx* ctx, const Tile& tl, const T* coef, const T* b, T* x, T* ws, bool from_r, double eps,
-                   int max_iter, bool use_graph, KState* st, KState* out, double ref_norm2 = 0.0) {
+                   int max_iter, bool use_graph, KState* st, KState* out, double ref_norm2 = 0.0,
+                   double* P = nullptr) {
     const int64_t n = (int64_t)tl.NX * tl.NY;
     T* r = ws;
     T* rt = ws + n;
@@ -728,18 +820,17 @@ int bicgstab_typed(km_ctx* ctx, const Tile& tl, const T* coef, const T* b, T* x,
     T* s = ws + 6 * n;
     T* t = ws + 7 * n;
     const Flat fl = make_flat(n);
-    double* P = ctx->partials;
+    if (!P) P = ctx->partials;
     hipLaunchKernelGGL(k_bicg_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, rt, pbuf, vbuf,
-                       from_r ? 1 : 0, eps, max_iter, ref_norm2, st);
+                       from_r ? 1 : 0, eps, max_iter, ref_norm2, st, P);
     KM_CHECK_LAUNCH(ctx);
     const size_t lds = tile_lds<T>(tl);
     auto enqueue = [=](hipStream_t s_, int phase) {
-        launch_bicg_a<T>(tl, phase, coef, r, rt, s, t, x, pbuf, vbuf, st, P, lds, s_);
-        launch_bicg_b<T>(tl, phase, coef, r, rt, vbuf, s, t, st, P, lds, s_);
+        launch_bicg_iter<T>(tl, fl, phase, coef, r, rt, s, t, x, pbuf, vbuf, st, P, lds, s_);
     };
     GraphKey key;
     key.v = {1, sizeof(T), (uintptr_t)tl.NX, (uintptr_t)tl.NY, (uintptr_t)tl.TI, (uintptr_t)tl.TJ,
-             (uintptr_t)coef, (uintptr_t)x, (uintptr_t)ws, (uintptr_t)st};
+             (uintptr_t)coef, (uintptr_t)x, (uintptr_t)ws, (uintptr_t)st, (uintptr_t)P};
     return run_chunks(ctx, st, key, use_graph, kChunk, max_iter, enqueue, out);
 }
 
@@ -766,41 +857,48 @@ int cg_typed(km_ctx* ctx, const Tile& tl, const T* coef, const T* b, T* x, T* ws
     return run_chunks(ctx, st, key, use_graph, kChunk, max_iter, enqueue, out);
 }
 
+// partials / dres (may be null: the context's workspace) receive the
+// per-workgroup partial sums and the reduced result on the device.
 int residual_norm2(km_ctx* ctx, const Tile& tl, const double* coef, const double* b, const double* x, double* r,
-                   bool sync, double* out) {
-    double* P = ctx->partials + KM_PX;
+                   bool sync, double* out, double* partials = nullptr, double* dres = nullptr) {
+    double* P = (partials ? partials : ctx->partials) + KM_PX;
+    if (!dres) dres = ctx->dres;
     hipLaunchKernelGGL(k_residual, dim3(tl.G), dim3(KM_BLOCK), tile_lds<double>(tl), ctx->stream, tl, coef, b, x, r, P);
-    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, tl.G, ctx->dres);
+    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, tl.G, dres);
     KM_CHECK_LAUNCH(ctx);
     if (sync) {
-        KM_HIP(ctx, hipMemcpyAsync(ctx->h_res, ctx->dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KM_HIP(ctx, hipMemcpyAsync(ctx->h_res, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
         *out = ctx->h_res[0];
     }
     return KM_OK;
 }
 
-int dot_sync(km_ctx* ctx, int64_t n, int prec, const void* a, const void* b, double* out) {
+int dot_sync(km_ctx* ctx, int64_t n, int prec, const void* a, const void* b, double* out, double* partials = nullptr,
+             double* dres = nullptr) {
     const Flat fl = make_flat(n);
-    double* P = ctx->partials + KM_PX;
+    double* P = (partials ? partials : ctx->partials) + KM_PX;
+    if (!dres) dres = ctx->dres;
     if (prec == KM_F64)
         hipLaunchKernelGGL(k_dot<double>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, (const double*)a,
                            (const double*)b, P);
     else
         hipLaunchKernelGGL(k_dot<float>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, (const float*)a,
                            (const float*)b, P);
-    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, fl.G, ctx->dres + 1);
+    hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, (const double*)P, fl.G, dres + 1);
     KM_CHECK_LAUNCH(ctx);
-    KM_HIP(ctx, hipMemcpyAsync(ctx->h_res + 1, ctx->dres + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KM_HIP(ctx, hipMemcpyAsync(ctx->h_res + 1, dres + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
     *out = ctx->h_res[1];
     return KM_OK;
 }
 
 template <int CPT>
-void batch_iter(const BSys* d_sys, const int* tstarts, int n, int64_t tblocks, int phase, size_t lds, hipStream_t q) {
-    hipLaunchKernelGGL(kb_bicg_a<CPT>, dim3((unsigned)tblocks), dim3(KM_BLOCK), lds, q, d_sys, tstarts, n, phase);
-    hipLaunchKernelGGL(kb_bicg_b<CPT>, dim3((unsigned)tblocks), dim3(KM_BLOCK), lds, q, d_sys, tstarts, n, phase);
+void batch_iter(const BSys* d_sys, const int* tstarts, const int* fstarts, int n, int64_t tblocks, int64_t fblocks,
+                int phase, size_t lds, hipStream_t q) {
+    hipLaunchKernelGGL(kb_bicg_p<CPT>, dim3((unsigned)tblocks), dim3(KM_BLOCK), lds, q, d_sys, tstarts, n, phase);
+    hipLaunchKernelGGL(kb_bicg_s<CPT>, dim3((unsigned)tblocks), dim3(KM_BLOCK), lds, q, d_sys, tstarts, n, phase);
+    hipLaunchKernelGGL(kb_bicg_r, dim3((unsigned)fblocks), dim3(KM_BLOCK), 0, q, d_sys, fstarts, n, phase);
 }
 
 void fill_info(km_solve_info* info, const KState& s) {
@@ -841,12 +939,11 @@ int autotune_tiles(km_ctx* ctx, int NX, int NY, int prec, const T* coef, const T
         const Tile tl = make_tile(NX, NY, c.first, c.second);
         const size_t lds = tile_lds<T>(tl);
         hipLaunchKernelGGL(k_bicg_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, rt, pbuf, vbuf, 0,
-                           0.0, 1 << 30, 0.0, st);
+                           0.0, 1 << 30, 0.0, st, P);
         for (int it = 0; it < 20; ++it) {
             if (it == 4) (void)hipEventRecord(e0, ctx->stream);
             const int ph = it % KM_RING;
-            launch_bicg_a<T>(tl, ph, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream);
-            launch_bicg_b<T>(tl, ph, coef, r, rt, vbuf, sv, t, st, P, lds, ctx->stream);
+            launch_bicg_iter<T>(tl, fl, ph, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream);
         }
         (void)hipEventRecord(e1, ctx->stream);
         KM_HIP_SYNC(ctx, hipEventSynchronize(e1));
@@ -984,9 +1081,9 @@ int batch_bicgstab(km_ctx* ctx, int n, const BSys* sys, const BSys* d_sys, const
                        max_iter, agg);
     KM_CHECK_LAUNCH(ctx);
     auto enqueue = [=](hipStream_t q, int phase) {
-        if (cpt == 1) batch_iter<1>(d_sys, tstarts, n, tblocks, phase, lds, q);
-        else if (cpt == 2) batch_iter<2>(d_sys, tstarts, n, tblocks, phase, lds, q);
-        else batch_iter<4>(d_sys, tstarts, n, tblocks, phase, lds, q);
+        if (cpt == 1) batch_iter<1>(d_sys, tstarts, fstarts, n, tblocks, fblocks, phase, lds, q);
+        else if (cpt == 2) batch_iter<2>(d_sys, tstarts, fstarts, n, tblocks, fblocks, phase, lds, q);
+        else batch_iter<4>(d_sys, tstarts, fstarts, n, tblocks, fblocks, phase, lds, q);
         if (phase == KM_RING - 1) hipLaunchKernelGGL(kb_all_done, dim3(1), dim3(KM_BLOCK), 0, q, d_sys, n, agg);
     };
     GraphKey key;
@@ -1052,7 +1149,8 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
     which &= ~KM_KERNEL_BACK_TO_BACK;
     if (which >= KM_KERNEL_TL_PV && which <= KM_KERNEL_TL_CT)
         return kmcoarse::time_kernel(ctx, which, NX, NY, coef, b, reps, b2b, avg_us);
-    KM_REQUIRE(ctx, which == KM_KERNEL_BICG_A || which == KM_KERNEL_BICG_B || which == KM_KERNEL_STENCIL,
+    KM_REQUIRE(ctx, which == KM_KERNEL_BICG_A || which == KM_KERNEL_BICG_B || which == KM_KERNEL_BICG_R ||
+                        which == KM_KERNEL_STENCIL,
                "unknown kernel id %d", which);
     const int64_t n = (int64_t)NX * NY;
     if (int rc = km_ensure_workspace(ctx, (size_t)9 * n * sizeof(double))) return rc;
@@ -1073,24 +1171,22 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
             if (e) { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e); }
     };
     hipLaunchKernelGGL(k_bicg_init<double>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, rt, pbuf, vbuf,
-                       0, 0.0, 1 << 30, 0.0, st);
+                       0, 0.0, 1 << 30, 0.0, st, P);
     const int warm = 8;
     if (b2b) {
         // reach a mid-solve state, then time `reps` identical launches of the
-        // kernel between two events (inputs unchanged from launch to launch)
-        for (int it = 0; it < warm; ++it) {
-            launch_bicg_a<double>(tl, it % KM_RING, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream);
-            launch_bicg_b<double>(tl, it % KM_RING, coef, r, rt, vbuf, sv, t, st, P, lds, ctx->stream);
-        }
+        // kernel between two events: P of the next iteration, or S / R of
+        // the last one (inputs unchanged from launch to launch)
+        for (int it = 0; it < warm; ++it)
+            launch_bicg_iter<double>(tl, fl, it % KM_RING, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream);
         KM_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
         for (int k = 0; k < reps; ++k) {
             if (which == KM_KERNEL_STENCIL)
                 hipLaunchKernelGGL(k_stencil<double>, dim3(tl.G), dim3(KM_BLOCK), lds, ctx->stream, tl, coef,
                                    (const double*)r, t);
-            else if (which == KM_KERNEL_BICG_A)
-                launch_bicg_a<double>(tl, warm % KM_RING, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream);
             else
-                launch_bicg_b<double>(tl, (warm - 1) % KM_RING, coef, r, rt, vbuf, sv, t, st, P, lds, ctx->stream);
+                launch_bicg_one<double>(which, tl, fl, which == KM_KERNEL_BICG_A ? warm % KM_RING : (warm - 1) % KM_RING,
+                                        coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream);
         }
         KM_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
     }
@@ -1103,10 +1199,7 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
             KM_LAUNCH((k_stencil<double>), tl.G, lds, ctx->stream, e0, e1, tl, coef, (const double*)r, t);
             continue;
         }
-        launch_bicg_a<double>(tl, ph, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream,
-                              which == KM_KERNEL_BICG_A ? e0 : nullptr, e1);
-        launch_bicg_b<double>(tl, ph, coef, r, rt, vbuf, sv, t, st, P, lds, ctx->stream,
-                              which == KM_KERNEL_BICG_B ? e0 : nullptr, e1);
+        launch_bicg_iter<double>(tl, fl, ph, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream, which, e0, e1);
     }
     hipError_t le = hipGetLastError();
     hipError_t se;
@@ -1172,6 +1265,71 @@ int km_residual_async(km_ctx* ctx, int NX, int NY, const double* coef, const dou
     return KM_OK;
 }
 
+// Where a plain solve keeps its scalar state, partial sums, reduced results
+// and its eight Krylov vectors (km_bicgstab: the context's workspace).
+struct PlainMem {
+    KState* st;
+    double* P;
+    double* dres;
+    double* ws;
+};
+
+// Plain BiCGSTAB (bicgsolver.py:88-136) on the memory of m, with the true
+// residual after the solve when info is given.  KM_FLAG_TRUE: restarts from
+// x while the true residual stays above eps (see include/karman.h).
+int bicgstab_plain(km_ctx* ctx, const PlainMem& m, const Tile& tl, const double* coef, const double* b, double* x,
+                   double eps, int max_iter, int flags, bool use_graph, km_solve_info* info) {
+    const int64_t n = (int64_t)tl.NX * tl.NY;
+    const bool from_r = (flags & KM_FLAG_X0) != 0;
+    const bool check_true = (flags & KM_FLAG_TRUE) != 0;
+    double bb = 0.0;
+    if (from_r || check_true)
+        if (int rc = dot_sync(ctx, n, KM_F64, b, b, &bb, m.P, m.dres)) return rc;
+    if (from_r) {
+        // r = b - A x0; the stopping test stays relative to ||b||
+        if (int rc = residual_norm2(ctx, tl, coef, b, x, m.ws /* r */, false, nullptr, m.P, m.dres)) return rc;
+    }
+    KState s;
+    if (int rc = bicgstab_typed<double>(ctx, tl, coef, b, x, m.ws, from_r, eps, max_iter, use_graph, m.st, &s,
+                                        from_r ? bb : 0.0, m.P))
+        return rc;
+    int total = s.iter, restarts = 0;
+    double rel_true = -1.0;
+    if (check_true && bb > 0.0) {
+        double rr = 0.0;
+        if (int rc = residual_norm2(ctx, tl, coef, b, x, m.ws /* r */, true, &rr, m.P, m.dres)) return rc;
+        rel_true = sqrt(rr / bb);
+        while (s.status == KM_OK && rel_true >= eps && restarts < 8 && total < max_iter) {
+            // r = b - A x is in place: restart from x, still relative to ||b||
+            const double prev = rel_true;
+            if (int rc = bicgstab_typed<double>(ctx, tl, coef, b, x, m.ws, true, eps, max_iter - total, use_graph,
+                                                m.st, &s, bb, m.P))
+                return rc;
+            total += s.iter;
+            restarts++;
+            if (int rc = residual_norm2(ctx, tl, coef, b, x, m.ws, true, &rr, m.P, m.dres)) return rc;
+            rel_true = sqrt(rr / bb);
+            if (!(rel_true < 0.5 * prev)) break;
+        }
+    }
+    fill_info(info, s);
+    if (info) {
+        if (rel_true < 0.0) {
+            double rr = 0.0;
+            if (!from_r && !check_true)
+                if (int rc = dot_sync(ctx, n, KM_F64, b, b, &bb, m.P, m.dres)) return rc;
+            if (int rc = residual_norm2(ctx, tl, coef, b, x, nullptr, true, &rr, m.P, m.dres)) return rc;
+            rel_true = bb > 0 ? sqrt(rr / bb) : 0.0;
+        }
+        info->iterations = total;
+        info->bnorm = sqrt(bb);
+        info->rel_resid_true = rel_true;
+        info->refinements = restarts;
+        info->fallback = 0;
+    }
+    return KM_OK;
+}
+
 extern "C" {
 
 int km_bicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
@@ -1188,29 +1346,8 @@ int km_bicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     if (int rc = ensure_tuned<double>(ctx, NX, NY, KM_F64, coef, b, ws)) return rc;
     Tile tl;
     if (int rc = get_tile(ctx, NX, NY, KM_F64, &tl)) return rc;
-    const bool from_r = (flags & KM_FLAG_X0) != 0;
-    double bb = 0.0;
-    if (from_r) {
-        // r = b - A x0; the stopping test stays relative to ||b||
-        if (int rc = dot_sync(ctx, n, KM_F64, b, b, &bb)) return rc;
-        if (int rc = residual_norm2(ctx, tl, coef, b, x, ws /* r */, false, nullptr)) return rc;
-    }
-    KState s;
-    if (int rc = bicgstab_typed<double>(ctx, tl, coef, b, x, ws, from_r, eps, max_iter,
-                                        !(flags & KM_FLAG_NO_GRAPH), &ctx->st[0], &s, from_r ? bb : 0.0))
-        return rc;
-    fill_info(info, s);
-    if (info) {
-        double rr = 0.0;
-        if (!from_r)
-            if (int rc = dot_sync(ctx, n, KM_F64, b, b, &bb)) return rc;
-        if (int rc = residual_norm2(ctx, tl, coef, b, x, nullptr, true, &rr)) return rc;
-        info->bnorm = sqrt(bb);
-        info->rel_resid_true = bb > 0 ? sqrt(rr / bb) : 0.0;
-        info->refinements = 0;
-        info->fallback = 0;
-    }
-    return KM_OK;
+    const PlainMem m{&ctx->st[0], ctx->partials, ctx->dres, ws};
+    return bicgstab_plain(ctx, m, tl, coef, b, x, eps, max_iter, flags, !(flags & KM_FLAG_NO_GRAPH), info);
 }
 
 int km_cg(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps, int max_iter,

@@ -98,6 +98,7 @@ class SIMPLESolver:
         self.p_coarse = False         # two-level p' preconditioner (coarse correction + network or SOR sweeps)
         self.mom_smooth = False       # momentum: flexible BiCGSTAB with a red-black Gauss-Seidel sweep
         self.native_step = True       # step() as one km_simple_step call where the configuration allows
+        self.true_residual = False    # plain solves: restart while the true residual is above eps (KM_FLAG_TRUE)
         self._case = KmCase()
         self._step_info = (KmSolveInfo * 3)()
         self.last = {}
@@ -106,6 +107,7 @@ class SIMPLESolver:
     def from_case(cls, case: cases.Case, device=None) -> "SIMPLESolver":
         s = cls(case.lx, case.ly, case.nx, case.ny, device=device, case=case)
         s.eps_mom, s.eps_p = case.eps_mom, case.eps_p
+        s.mixed = bool(case.mixed)
         s.ct.copy_(torch.from_numpy(cases.make_mask(case)))
         return s
 
@@ -199,8 +201,10 @@ class SIMPLESolver:
                 vs._refresh()
                 iu, iv = fbicgstab_pair(((us.coef, us.b, us.x), (vs.coef, vs.b, vs.x)), eps=self.eps_mom)
             else:
-                iu = self.u_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed, smooth=self.mom_smooth)
-                iv = self.v_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed, smooth=self.mom_smooth)
+                iu = self.u_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed, smooth=self.mom_smooth,
+                                                  true_residual=self.true_residual)
+                iv = self.v_momentum_solver.solve(eps=self.eps_mom, mixed=self.mixed, smooth=self.mom_smooth,
+                                                  true_residual=self.true_residual)
             self.update_velocity()
         return iu, iv
 
@@ -212,7 +216,8 @@ class SIMPLESolver:
         if self.p_precond is not None or self.p_coarse:
             return self.p_correction_solver.solve(eps=self.eps_p, x0=self.warm_start_p, precond=self.p_precond,
                                                  mixed=self.mixed, coarse=self.p_coarse)
-        return self.p_correction_solver.solve(eps=self.eps_p, mixed=self.mixed, x0=self.warm_start_p)
+        return self.p_correction_solver.solve(eps=self.eps_p, mixed=self.mixed, x0=self.warm_start_p,
+                                             true_residual=self.true_residual)
 
     def init(self, seed_fields=True):
         """simple.py:257-264; with a Case, the seeded channel start state."""
@@ -231,6 +236,7 @@ class SIMPLESolver:
         if not self.native_step or self.p_solver != "bicgstab" or self.p_precond is not None:
             return None
         mixed = _lib.KM_FLAG_MIXED if self.mixed else 0
+        mixed |= _lib.KM_FLAG_TRUE if self.true_residual else 0
         mom = mixed | (_lib.KM_FLAG_SMOOTH if self.mom_smooth else 0)
         pf = mixed | (_lib.KM_FLAG_COARSE if self.p_coarse else 0) | (_lib.KM_FLAG_X0 if self.warm_start_p else 0)
         return mom, pf

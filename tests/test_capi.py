@@ -45,3 +45,4 @@ def test_case_struct_layout_matches_header():
     # km_case: km_grid (144 bytes) + 16 pointers
     assert ctypes.sizeof(_lib.KmCase) == ctypes.sizeof(_lib.KmGrid) + 16 * 8
     assert _lib.KmCase.mask.offset == ctypes.sizeof(_lib.KmGrid) + 15 * 8
+

@@ -118,3 +118,52 @@ def test_batch_cases_concurrent(dev):
             x, y = getattr(a, f).cpu().numpy(), getattr(b, f).cpu().numpy()
             e = np.linalg.norm(x - y) / max(np.linalg.norm(x), 1e-300)
             assert e < 1e-6, (c.name, f, e)
+
+
+def test_batch_cases_mixed_precision(dev):
+    """step_cases on a batch whose cases alternate FP32 and FP64 Krylov
+    vectors: every case against the same case stepped alone in its own
+    precision (within the solve tolerance), and, with inner eps 1e-12, one
+    FP32 case and one FP64 case against the oracle (exact LU solves) to 1e-9."""
+    from oracle import fvm
+
+    def batch_cases():
+        out = small_batch()[:4]
+        for k, c in enumerate(out):
+            c.mixed = k % 2 == 0
+        return out
+
+    alone = BatchedSIMPLE(batch_cases(), device=dev)
+    alone.init()
+    conc = BatchedSIMPLE(batch_cases(), device=dev)
+    conc.init()
+    for _ in range(2):
+        for c, s in zip(alone.cases, alone.solvers):
+            s.mom_smooth = s.p_coarse = True
+            s.mixed = c.mixed
+            s.step()
+        infos = conc.step_cases(workers=2)
+    torch.cuda.synchronize()
+    for c, a, b, inf in zip(alone.cases, alone.solvers, conc.solvers, infos):
+        assert b.mixed == c.mixed   # the solver keeps its case's precision
+        for i in inf:
+            assert i.status == _lib.KM_OK and i.rel_resid_true < 1e-8, c.name
+        for f in ("u", "v", "p"):
+            x, y = getattr(a, f).cpu().numpy(), getattr(b, f).cpu().numpy()
+            e = np.linalg.norm(x - y) / max(np.linalg.norm(x), 1e-300)
+            assert e < 1e-6, (c.name, f, e)
+    tight = BatchedSIMPLE(batch_cases()[:2], device=dev)
+    tight.init()
+    for _ in range(2):
+        tight.step_cases(workers=2, eps_mom=1e-12, eps_p=1e-12)
+    torch.cuda.synchronize()
+    for c, s in zip(tight.cases, tight.solvers):
+        m = cases.make_mask(c)
+        st = fvm.State(c, *cases.initial_fields(c, m), m)
+        for _ in range(2):
+            fvm.simple_step(st, solver="direct")
+        for f in ("u", "v", "p"):
+            x = getattr(s, f).cpu().numpy()
+            e = np.linalg.norm(x - getattr(st, f)) / np.linalg.norm(getattr(st, f))
+            print(f"{c.name} ({'FP32' if c.mixed else 'FP64'} vectors) {f}: {e:.2e} against the oracle")
+            assert e <= 1e-9, (c.name, f, e)

@@ -206,12 +206,13 @@ def test_bicgstab_first_iterations_track_oracle(dev, c1_pcor):
 
 
 def test_bicgstab_converges_like_oracle(dev, c1_pcor):
-    """Iteration counts within 25%: on this system (kappa ~ 3e5) the count to
-    1e-8 / 1e-12 moves by up to 17% under rounding alone -- the oracle itself
-    gives 687 / 839 with numpy dots, 732 / 893 with exactly rounded dots
-    (math.fsum) and 768 / 981 with rho formed as <s, r~> - omega <t, r~>, the
-    identity the GPU kernels use.  The first iterations are pinned tightly by
-    test_bicgstab_first_iterations_track_oracle and the solution by LU."""
+    """Iteration counts within 12%: the GPU iteration forms rho = <r, r~> as
+    the reference does (bicgsolver.py:95), and on this system (kappa ~ 3e5)
+    the count to 1e-8 / 1e-12 moves by ~7% under the rounding of the dot
+    products alone -- the oracle itself gives 687 / 839 with numpy dots and
+    732 / 893 with exactly rounded dots (math.fsum).  The first iterations are
+    pinned tightly by test_bicgstab_first_iterations_track_oracle and the
+    solution by LU."""
     coef, b = c1_pcor
     xd = krylov.direct_solve(coef, b)
     for eps in (1e-8, 1e-12):
@@ -220,10 +221,34 @@ def test_bicgstab_converges_like_oracle(dev, c1_pcor):
         print(f"eps={eps}: gpu {info.iterations} it, oracle {io['iterations']} it, "
               f"true resid {info.rel_resid_true:.3e}, err vs LU {np.linalg.norm(x - xd) / np.linalg.norm(xd):.3e}")
         assert info.status == _lib.KM_OK and io["status"] == krylov.OK
-        assert abs(info.iterations - io["iterations"]) <= 0.25 * io["iterations"] + 5
+        assert abs(info.iterations - io["iterations"]) <= 0.12 * io["iterations"] + 5
         assert info.rel_resid < eps
         assert info.rel_resid_true < 10 * eps
         assert np.linalg.norm(x - xd) <= 1e4 * eps * np.linalg.norm(xd)
+
+
+def test_bicgstab_true_residual_restarts(dev):
+    """KM_FLAG_TRUE on the C2 p' system: the reference's stop on the
+    recursive residual (bicgsolver.py:128-131) leaves the true residual above
+    a tight eps (measured 5.7e-10 at eps 1e-10); the restarts from x bring it
+    below eps.  (The FP64 floor of ||b - A x|| / ||b|| on this system is
+    ~3e-11: restarts stall there.)"""
+    c = cases.c2()
+    s = SIMPLESolver.from_case(c, device=dev)
+    s.init()
+    s.solve_momentum_eqn(1)
+    s.correct_conserv()
+    s.compute_coef_p()
+    eps = 1e-10
+    plain = s.p_correction_solver.solve(eps=eps)
+    x_plain = s.pcor.clone()
+    checked = s.p_correction_solver.solve(eps=eps, true_residual=True)
+    print(f"C2 p' to {eps}: reference stop {plain.iterations} it, true {plain.rel_resid_true:.2e}; "
+          f"with restarts {checked.iterations} it, {checked.refinements} restarts, true {checked.rel_resid_true:.2e}")
+    assert plain.status == checked.status == _lib.KM_OK
+    assert checked.rel_resid_true < eps <= plain.rel_resid_true or checked.refinements == 0
+    assert checked.rel_resid_true <= plain.rel_resid_true
+    assert torch.isfinite(x_plain).all()
 
 
 def test_bicgstab_edge_cases(dev, c1_pcor):
