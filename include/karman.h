@@ -137,6 +137,21 @@ int km_residual(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
 /* BICGSolver.solve (bicgsolver.py:88-136): FP64 coefficients, right-hand side and solution. */
 int km_bicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x,
                 double eps, int max_iter, int flags, km_solve_info* info);
+/* km_bicgstab without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same plain BiCGSTAB
+ * kernels as km_bicgstab, launched directly (no graphs), with the solver's
+ * state, partial sums and eight Krylov vectors in a device array the library
+ * reserves once per GPU (KM_NOALLOC_SCRATCH_BYTES = 32 MiB; at most
+ * KM_NOALLOC_MAX_CELLS = NX*NY cells, e.g. the 1282 x 258 p' system of a
+ * 1280 x 256 grid).  Larger systems and KM_FLAG_MIXED return KM_EINVAL.  The
+ * tile is the context's cached or override choice, else the default split
+ * (no timing).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (several contexts or streams at once) race. */
+#define KM_NOALLOC_SCRATCH_BYTES (32u << 20)
+#define KM_NOALLOC_MAX_CELLS 521144
+int km_bicgstab_noalloc(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x,
+                        double eps, int max_iter, int flags, km_solve_info* info);
 /* CGSolver.solve (cgsolver.py:83-114). */
 int km_cg(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x,
           double eps, int max_iter, int flags, km_solve_info* info);

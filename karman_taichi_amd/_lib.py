@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libkarman_hip.so"
 KM_OK, KM_BREAKDOWN, KM_MAXITER, KM_NONFINITE, KM_STALL = 0, 1, 2, 3, 4
 KM_EINVAL, KM_EHIP, KM_ENOMEM = -1, -2, -3
 KM_F64, KM_F32 = 0, 1
+NOALLOC_MAX_CELLS = 521144    # KM_NOALLOC_MAX_CELLS: largest system km_bicgstab_noalloc takes
 KM_FLAG_X0, KM_FLAG_MIXED, KM_FLAG_NO_GRAPH, KM_FLAG_COARSE, KM_FLAG_SMOOTH, KM_FLAG_TRUE = 1, 2, 4, 8, 16, 32
 KM_EAST_WALL, KM_EAST_OUTFLOW = 0, 1
 KM_KERNEL_BICG_A, KM_KERNEL_BICG_B, KM_KERNEL_BICG_R, KM_KERNEL_STENCIL = 0, 1, 2, 3
@@ -82,6 +83,7 @@ SIGNATURES = {
     "km_dot": [_CTX, ctypes.c_int64, _I, _P, _P, ctypes.POINTER(_D)],
     "km_residual": [_CTX, _I, _I, _P, _P, _P, _P, ctypes.POINTER(_D)],
     "km_bicgstab": [_CTX, _I, _I, _P, _P, _P, _D, _I, _I, _INFO],
+    "km_bicgstab_noalloc": [_CTX, _I, _I, _P, _P, _P, _D, _I, _I, _INFO],
     "km_cg": [_CTX, _I, _I, _P, _P, _P, _D, _I, _I, _INFO],
     "km_time_kernel": [_CTX, _I, _I, _I, _P, _P, _I, ctypes.POINTER(_D)],
     "km_unet_param_count": [ctypes.POINTER(ctypes.c_int64)],

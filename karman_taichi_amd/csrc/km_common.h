@@ -148,6 +148,7 @@ struct km_ctx {
     std::vector<km_coarse*> coarse_pool;   // the fused solves' levels, one per coefficient array
     uint64_t coarse_tick = 0;
     void* blas = nullptr;          // rocBLAS handle (the coarse levels' Newton-Schulz GEMM)
+    void* scratch = nullptr;       // device address of this device's no-allocation scratch array
 };
 
 void km_batch_free(km_ctx* ctx);
@@ -192,6 +193,10 @@ int km_capture(km_ctx* ctx, const std::function<int(hipStream_t)>& body, hipGrap
     do {                                               \
         if (!(cond)) return km_fail((ctx), KM_EINVAL, __VA_ARGS__); \
     } while (0)
+
+// Device address of the no-allocation scratch array on ctx's device, into
+// ctx->scratch (km_krylov.hip; called once per context at creation).
+int km_resolve_scratch(km_ctx* ctx);
 
 // Workspace management (km_ctx.hip)
 int km_ensure_workspace(km_ctx* ctx, size_t vec_bytes);

@@ -130,6 +130,8 @@ int km_create(km_ctx** out, int device, void* hip_stream) {
                 rc = KM_EHIP;
         }
     }
+    // the no-allocation entry point's scratch address on this device
+    if (rc == KM_OK && km_resolve_scratch(ctx) != KM_OK) rc = KM_EHIP;
     if (rc != KM_OK) {
         ctx->stream = nullptr;   // nothing was enqueued
         km_destroy(ctx);

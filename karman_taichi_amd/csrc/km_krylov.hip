@@ -1266,7 +1266,8 @@ int km_residual_async(km_ctx* ctx, int NX, int NY, const double* coef, const dou
 }
 
 // Where a plain solve keeps its scalar state, partial sums, reduced results
-// and its eight Krylov vectors (km_bicgstab: the context's workspace).
+// and its eight Krylov vectors (km_bicgstab: the context's workspace;
+// km_bicgstab_noalloc: the library's scratch array).
 struct PlainMem {
     KState* st;
     double* P;
@@ -1330,6 +1331,64 @@ int bicgstab_plain(km_ctx* ctx, const PlainMem& m, const Tile& tl, const double*
     return KM_OK;
 }
 
+// The no-allocation entry point's scratch: one file-scope device array per
+// GPU (each device holds its own copy from the library's load until the
+// process ends).  Layout: KState slots | partial sums (the plain solve's
+// regions, up to KM_PX + 2 KM_MAX_RED doubles) | 64 reduced results | the
+// eight Krylov vectors (8 n doubles).  In host code `scratch` only names the
+// symbol; its device address comes from hipGetSymbolAddress (ctx->scratch).
+constexpr size_t kScratchStateBytes = 4096;
+constexpr size_t kScratchPartialBytes = (size_t)(KM_PX + 2 * KM_MAX_RED) * sizeof(double);
+constexpr size_t kScratchHeaderBytes = kScratchStateBytes + kScratchPartialBytes + 64 * sizeof(double);
+static_assert(8 * sizeof(KState) <= kScratchStateBytes, "KState slots exceed their scratch region");
+static_assert(kScratchHeaderBytes % 256 == 0, "the scratch vectors must stay 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[KM_NOALLOC_SCRATCH_BYTES];
+
+static_assert((KM_NOALLOC_SCRATCH_BYTES - kScratchHeaderBytes) / (8 * sizeof(double)) == KM_NOALLOC_MAX_CELLS,
+              "KM_NOALLOC_MAX_CELLS does not match the scratch layout");
+
+size_t scratch_need(int64_t n) { return kScratchHeaderBytes + (size_t)n * 8 * sizeof(double); }
+int64_t scratch_max_cells() { return (int64_t)((KM_NOALLOC_SCRATCH_BYTES - kScratchHeaderBytes) / (8 * sizeof(double))); }
+
+// The tile of an n-cell system without timing candidates or caching a choice
+// (the autotuner's timing needs workspace): an override, else the cached
+// choice, else the default split of km_choose_tile.
+int peek_tile(km_ctx* ctx, int NX, int NY, Tile* out) {
+    auto key = std::make_tuple(NX, NY, (int)KM_F64);
+    int TI, TJ;
+    auto ov = ctx->tile_override.find(key);
+    auto it = ctx->tiles.find(key);
+    if (ov != ctx->tile_override.end()) {
+        TI = ov->second.first;
+        TJ = ov->second.second;
+    } else if (it != ctx->tiles.end()) {
+        TI = it->second.first;
+        TJ = it->second.second;
+    } else {
+        const int segs = (NY + 255) / 256;
+        TJ = (NY + segs - 1) / segs;
+        TI = (256 + TJ - 1) / TJ;
+        if (TI > NX) TI = NX;
+    }
+    KM_REQUIRE(ctx, tile_ok(TI, TJ), "tile %dx%d is not valid for the iteration kernels", TI, TJ);
+    *out = make_tile(NX, NY, TI, TJ);
+    return KM_OK;
+}
+
+int km_resolve_scratch(km_ctx* ctx) {
+    static std::mutex mu;
+    static void* addr[64] = {};
+    std::lock_guard<std::mutex> lk(mu);
+    KM_REQUIRE(ctx, ctx->device >= 0 && ctx->device < 64, "device %d", ctx->device);
+    if (!addr[ctx->device]) {
+        void* p = nullptr;
+        KM_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        addr[ctx->device] = p;
+    }
+    ctx->scratch = addr[ctx->device];
+    return KM_OK;
+}
+
 extern "C" {
 
 int km_bicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
@@ -1348,6 +1407,32 @@ int km_bicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     if (int rc = get_tile(ctx, NX, NY, KM_F64, &tl)) return rc;
     const PlainMem m{&ctx->st[0], ctx->partials, ctx->dres, ws};
     return bicgstab_plain(ctx, m, tl, coef, b, x, eps, max_iter, flags, !(flags & KM_FLAG_NO_GRAPH), info);
+}
+
+int km_bicgstab_noalloc(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
+                        int max_iter, int flags, km_solve_info* info) {
+    if (!ctx) return KM_EINVAL;
+    if (int rc = check_system(ctx, NX, NY)) return rc;
+    KM_REQUIRE(ctx, coef && b && x, "km_bicgstab_noalloc: NULL pointer");
+    KM_REQUIRE(ctx, eps >= 0.0, "eps must be non-negative");
+    KM_REQUIRE(ctx, !(flags & KM_FLAG_MIXED), "km_bicgstab_noalloc has no mixed-precision path");
+    const int64_t n = (int64_t)NX * NY;
+    const size_t need = scratch_need(n);
+    KM_REQUIRE(ctx, need <= KM_NOALLOC_SCRATCH_BYTES,
+               "km_bicgstab_noalloc: a %d x %d system needs %zu bytes of scratch; the library reserves %zu "
+               "(at most %lld cells)",
+               NX, NY, need, (size_t)KM_NOALLOC_SCRATCH_BYTES, (long long)scratch_max_cells());
+    KM_REQUIRE(ctx, ctx->scratch, "km_bicgstab_noalloc: the scratch address of device %d was not resolved",
+               ctx->device);
+    if (max_iter <= 0) max_iter = (int)n;   // cgsolver.py:14
+    Tile tl;
+    if (int rc = peek_tile(ctx, NX, NY, &tl)) return rc;
+    char* base = static_cast<char*>(ctx->scratch);
+    const PlainMem m{reinterpret_cast<KState*>(base), reinterpret_cast<double*>(base + kScratchStateBytes),
+                     reinterpret_cast<double*>(base + kScratchStateBytes + kScratchPartialBytes),
+                     reinterpret_cast<double*>(base + kScratchHeaderBytes)};
+    // direct launches only: instantiating a graph allocates
+    return bicgstab_plain(ctx, m, tl, coef, b, x, eps, max_iter, flags, false, info);
 }
 
 int km_cg(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps, int max_iter,

@@ -46,3 +46,8 @@ def test_case_struct_layout_matches_header():
     assert ctypes.sizeof(_lib.KmCase) == ctypes.sizeof(_lib.KmGrid) + 16 * 8
     assert _lib.KmCase.mask.offset == ctypes.sizeof(_lib.KmGrid) + 15 * 8
 
+
+def test_noalloc_limit_matches_header():
+    text = (ROOT / "include" / "karman.h").read_text()
+    m = re.search(r"#define KM_NOALLOC_MAX_CELLS (\d+)", text)
+    assert m and int(m.group(1)) == _lib.NOALLOC_MAX_CELLS
