@@ -53,6 +53,12 @@ extern "C" {
 #define KM_FLAG_NO_GRAPH 4    /* launch kernels directly instead of replaying captured graphs */
 #define KM_FLAG_COARSE 8      /* km_fbicgstab: two-level preconditioner (aggregation coarse correction first) */
 #define KM_FLAG_SMOOTH 16     /* km_fbicgstab: one red-black Gauss-Seidel sweep as the preconditioner, no coarse level */
+#define KM_FLAG_GCR 64        /* km_fbicgstab: flexible GCR (restarted every 16 directions) as the outer method instead
+                               * of flexible BiCGSTAB: ||r|| never increases whatever the preconditioner does, so a
+                               * non-linear network preconditioner cannot break its recurrences (no counterpart in the
+                               * reference) */
+#define KM_FLAG_JACOBI 128    /* km_fbicgstab with KM_FLAG_GCR and KM_FLAG_COARSE, no network: one Jacobi step after the
+                               * coarse correction instead of the red-black SOR sweeps */
 #define KM_FLAG_TRUE 32       /* km_bicgstab: after the reference's stop on the recursive residual, check the true
                                * residual ||b - A x|| / ||b|| and restart from x (r = b - A x) while it is above eps
                                * and each restart at least halves it (at most 8 restarts; no counterpart in the

@@ -284,6 +284,13 @@ k_axpy_to(int64_t n, double* __restrict__ x, const T* __restrict__ d, double sca
         x[i] = x[i] + scale * (double)d[i];
 }
 
+template <typename F>
+__device__ __forceinline__ void flat_sweep_fg(const Flat& fl, F f) {
+    const int64_t k0 = (int64_t)blockIdx.x * fl.chunk;
+    const int64_t k1 = k0 + fl.chunk < fl.n ? k0 + fl.chunk : fl.n;
+    for (int64_t k = k0 + threadIdx.x; k < k1; k += KM_BLOCK) f(k);
+}
+
 int blocks(int64_t n) {
     int64_t g = (n + KM_BLOCK - 1) / KM_BLOCK;
     if (g > 2048) g = 2048;
@@ -592,6 +599,297 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     return KM_OK;
 }
 
+// ------------------------------------------------------------ flexible GCR
+//
+// The outer method for a preconditioner that changes from one application
+// to the next (KM_FLAG_GCR): flexible GCR, restarted every kFgMax
+// directions.  Iteration k: z = M(r / ||r||), q = A z; q and z orthogonalised
+// against the stored q_j twice (classical Gram-Schmidt), scaled to ||q|| = 1
+// and stored; alpha = <r, q>, x += alpha z, r -= alpha q.  ||r|| never
+// increases, whatever M does, so a non-linear network cannot break the
+// recurrences as it breaks flexible BiCGSTAB's.  z, q and the stored
+// directions are in T (FP32 with KM_FLAG_MIXED); x, r and every dot product
+// in FP64.  At each restart, and when the recursive residual reaches eps, r
+// is replaced by the true residual b - A x (FP64), which alone decides.  A
+// restart cycle that does not lower the true residual by 10% ends the solve
+// (stall), and plain BiCGSTAB finishes from x.
+constexpr int kFgMax = 16;
+constexpr int FG_PD = 16 * KM_MAX_RED;   // [kFgMax][KM_MAX_RED] partials of <q, q_j>
+constexpr int FG_PN = 32 * KM_MAX_RED;   // [2] <q, q>, <r, q>; [1] <r, r> at +2
+static_assert(FG_PN + 3 * KM_MAX_RED <= KM_MAX_PARTIALS, "FGCR partials");
+// device scalars in ctx->dres: beta[kFgMax], then <q,q>, <r,q>, <r,r>
+constexpr int FG_SB = 0, FG_SN = kFgMax, FG_SR = kFgMax + 2;
+
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_fg_stencil(int64_t n, int NX, int NY, const T* __restrict__ c, const T* __restrict__ z, T* __restrict__ q) {
+    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK)
+        q[k] = sten5(c, z, k, n, NX, NY);
+}
+
+// z += D^-1 r1 (a Jacobi step after the coarse correction; aP = 0 rows: none)
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_fg_jacobi_add(int64_t n, const T* __restrict__ c, const T* __restrict__ r1, T* __restrict__ z) {
+    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK)
+        if (c[k] != T(0)) z[k] = z[k] + r1[k] / c[k];
+}
+
+// partials of <q, Q_j>, j < m
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_fg_dots(Flat fl, int m, const T* __restrict__ q, const T* __restrict__ Q, double* __restrict__ P) {
+    __shared__ double sh[4];
+    double acc[kFgMax];
+#pragma unroll
+    for (int j = 0; j < kFgMax; ++j) acc[j] = 0.0;
+    flat_sweep_fg(fl, [&](int64_t k) {
+        const double qk = (double)q[k];
+#pragma unroll
+        for (int j = 0; j < kFgMax; ++j)
+            if (j < m) acc[j] += qk * (double)Q[(int64_t)j * fl.n + k];
+    });
+    for (int j = 0; j < m; ++j) {
+        const double v = block_sum(acc[j], sh);
+        if (threadIdx.x == 0) P[(int64_t)j * KM_MAX_RED + blockIdx.x] = v;
+        __syncthreads();
+    }
+}
+
+// out[j] = the sum of the partials of plane j (one workgroup per plane, fixed order)
+__global__ void __launch_bounds__(KM_BLOCK)
+k_fg_reduce(int G, const double* __restrict__ P, double* __restrict__ out) {
+    __shared__ double sh[4];
+    double t[1];
+    sum_partials<1>(P + (int64_t)blockIdx.x * KM_MAX_RED, G, t, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = t[0];
+}
+
+// q -= sum_j beta_j Q_j, z -= sum_j beta_j Z_j
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_fg_sub(Flat fl, int m, const double* __restrict__ beta, T* __restrict__ q, T* __restrict__ z,
+         const T* __restrict__ Q, const T* __restrict__ Z) {
+    __shared__ double bs[kFgMax];
+    if (threadIdx.x < kFgMax) bs[threadIdx.x] = threadIdx.x < (unsigned)m ? beta[threadIdx.x] : 0.0;
+    __syncthreads();
+    flat_sweep_fg(fl, [&](int64_t k) {
+        T qk = q[k], zk = z[k];
+        for (int j = 0; j < m; ++j) {
+            const T bj = (T)bs[j];
+            qk = qk - bj * Q[(int64_t)j * fl.n + k];
+            zk = zk - bj * Z[(int64_t)j * fl.n + k];
+        }
+        q[k] = qk;
+        z[k] = zk;
+    });
+}
+
+// partials of <q, q> and <r, q>
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_fg_norms(Flat fl, const T* __restrict__ q, const double* __restrict__ r, double* __restrict__ P) {
+    __shared__ double sh[4];
+    double a = 0.0, b = 0.0;
+    flat_sweep_fg(fl, [&](int64_t k) {
+        const double qk = (double)q[k];
+        a += qk * qk;
+        b += r[k] * qk;
+    });
+    a = block_sum(a, sh);
+    __syncthreads();
+    b = block_sum(b, sh);
+    if (threadIdx.x == 0) {
+        P[blockIdx.x] = a;
+        P[KM_MAX_RED + blockIdx.x] = b;
+    }
+}
+
+// Q_k = q / ||q||, Z_k = z / ||q||; x += alpha Z_k, r -= alpha Q_k with
+// alpha = <r, q> / ||q||; partials of <r, r>
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_fg_step(Flat fl, const double* __restrict__ s, const T* __restrict__ q, const T* __restrict__ z, T* __restrict__ Qk,
+          T* __restrict__ Zk, double* __restrict__ x, double* __restrict__ r, double* __restrict__ P) {
+    __shared__ double sh[4];
+    const double nq = sqrt(s[0]);
+    const double inv = nq > 0.0 ? 1.0 / nq : 0.0;
+    const double alpha = s[1] * inv;
+    double a = 0.0;
+    flat_sweep_fg(fl, [&](int64_t k) {
+        const T qn = (T)((double)q[k] * inv), zn = (T)((double)z[k] * inv);
+        Qk[k] = qn;
+        Zk[k] = zn;
+        x[k] = x[k] + alpha * (double)zn;
+        const double rk = r[k] - alpha * (double)qn;
+        r[k] = rk;
+        a += rk * rk;
+    });
+    a = block_sum(a, sh);
+    if (threadIdx.x == 0) P[blockIdx.x] = a;
+}
+
+enum FgPrecond { FG_NET, FG_TWOLEVEL, FG_COARSE_JACOBI, FG_JACOBI };
+
+template <typename T>
+int fgcr_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
+               int max_iter, int flags, const float* params, km_solve_info* info) {
+    const int64_t n = (int64_t)NX * NY;
+    const bool f64 = sizeof(T) == sizeof(double);
+    const bool coarse = (flags & KM_FLAG_COARSE) != 0;
+    KM_REQUIRE(ctx, !(flags & KM_FLAG_SMOOTH), "KM_FLAG_GCR takes no KM_FLAG_SMOOTH");
+    const FgPrecond kind = params ? FG_NET
+                                  : (coarse ? ((flags & KM_FLAG_JACOBI) ? FG_COARSE_JACOBI : FG_TWOLEVEL) : FG_JACOBI);
+    // r64 | c32 | rhs, z, q, r1 | Q[kFgMax] | Z[kFgMax]
+    const size_t bytes = (size_t)n * sizeof(double) + (size_t)5 * n * sizeof(float) + 512 +
+                         (size_t)(4 + 2 * kFgMax) * n * sizeof(T);
+    if (int rc = km_ensure_workspace(ctx, bytes)) return rc;
+    double* r64 = static_cast<double*>(km_ws_vectors(ctx));
+    float* c32 = reinterpret_cast<float*>(r64 + n);
+    T* base = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(c32 + 5 * n) + 255) & ~uintptr_t(255));
+    T* rhs = base;
+    T* z = rhs + n;
+    T* q = z + n;
+    T* r1 = q + n;
+    T* Q = r1 + n;
+    T* Z = Q + (size_t)kFgMax * n;
+    if (params || !f64) {
+        hipLaunchKernelGGL(k_scale_to<float>, dim3(blocks(5 * n)), dim3(KM_BLOCK), 0, ctx->stream, 5 * n, coef, c32,
+                           1.0);
+        KM_CHECK_LAUNCH(ctx);
+    }
+    const T* coefT = f64 ? reinterpret_cast<const T*>(coef) : reinterpret_cast<const T*>(c32);
+    if (params) {
+        if (int rc = kmnet::prepare(ctx, 1, NX, NY, params)) return rc;
+        if (int rc = kmnet::solve_stats(ctx, NX, NY, c32, ctx->stream)) return rc;
+    }
+    if (coarse)
+        if (int rc = kmcoarse::setup(ctx, NX, NY, coef, ctx->stream, true)) return rc;
+    double bb = 0.0;
+    if (int rc = km_dot(ctx, n, KM_F64, b, b, &bb)) return rc < 0 ? rc : KM_EHIP;
+    if (info) {
+        info->bnorm = sqrt(bb);
+        info->refinements = 0;
+        info->fallback = 0;
+    }
+    if (!(flags & KM_FLAG_X0)) KM_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
+    if (bb == 0.0) {
+        KM_HIP(ctx, hipMemsetAsync(x, 0, n * sizeof(double), ctx->stream));
+        KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
+        if (info) {
+            info->iterations = 0;
+            info->status = KM_OK;
+            info->rel_resid = info->rel_resid_true = 0.0;
+        }
+        return KM_OK;
+    }
+    const double bn = sqrt(bb);
+    auto true_residual = [&](double* rr) -> int {
+        if (int rc = km_residual_async(ctx, NX, NY, coef, b, x, r64, 1)) return rc;
+        KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
+        *rr = km_mapped_res(ctx, 1);
+        return KM_OK;
+    };
+    double rr = 0.0;
+    if (flags & KM_FLAG_X0) {
+        if (int rc = true_residual(&rr)) return rc;
+    } else {
+        KM_HIP(ctx, hipMemcpyAsync(r64, b, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        rr = bb;
+    }
+    const Flat fl = make_flat(n);
+    double* P = ctx->partials;
+    double* S = ctx->dres;
+    const int jb = blocks(n);
+    int status = KM_MAXITER, total = 0, k = 0, restarts = 0;
+    double rel = sqrt(rr) / bn, cycle_start = rel;
+    while (true) {
+        if (!(rel == rel)) { status = KM_NONFINITE; break; }
+        if (total >= max_iter) { status = KM_MAXITER; break; }
+        // z = M(r / ||r||)
+        hipLaunchKernelGGL(k_scale_to<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, (const double*)r64, rhs,
+                           1.0 / sqrt(rr));
+        KM_CHECK_LAUNCH(ctx);
+        if (kind == FG_NET && coarse) {
+            if (int rc = kmcoarse::apply<T>(ctx, coef, coefT, rhs, z, r1, P + FB_PR, nullptr, ctx->stream)) return rc;
+            if (int rc = kmnet::apply_pre<T>(ctx, NX, NY, c32, r1, z, nullptr, ctx->stream, params, P + FB_PR, fl.G,
+                                             true))
+                return rc;
+        } else if (kind == FG_NET) {
+            if (int rc = kmnet::apply<T>(ctx, NX, NY, c32, rhs, z, nullptr, ctx->stream, params)) return rc;
+        } else if (kind == FG_TWOLEVEL) {
+            if (int rc = kmcoarse::restrict_solve<T>(ctx, rhs, nullptr, ctx->stream)) return rc;
+            if (int rc = kmcoarse::smooth<T>(ctx, coefT, rhs, z, ctx->stream)) return rc;
+        } else if (kind == FG_COARSE_JACOBI) {
+            if (int rc = kmcoarse::apply<T>(ctx, coef, coefT, rhs, z, r1, nullptr, nullptr, ctx->stream)) return rc;
+            hipLaunchKernelGGL(k_fg_jacobi_add<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, coefT, (const T*)r1, z);
+        } else {
+            hipLaunchKernelGGL(k_jacobi<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, coefT, (const T*)rhs, z,
+                               (const int*)nullptr);
+        }
+        // q = A z, orthogonalised twice against the stored directions
+        hipLaunchKernelGGL(k_fg_stencil<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, NX, NY, coefT, (const T*)z, q);
+        for (int pass = 0; pass < 2 && k > 0; ++pass) {
+            hipLaunchKernelGGL(k_fg_dots<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, k, (const T*)q,
+                               (const T*)Q, P + FG_PD);
+            hipLaunchKernelGGL(k_fg_reduce, dim3(k), dim3(KM_BLOCK), 0, ctx->stream, fl.G, (const double*)(P + FG_PD),
+                               S + FG_SB);
+            hipLaunchKernelGGL(k_fg_sub<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, k, (const double*)(S + FG_SB),
+                               q, z, (const T*)Q, (const T*)Z);
+        }
+        hipLaunchKernelGGL(k_fg_norms<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, (const T*)q,
+                           (const double*)r64, P + FG_PN);
+        hipLaunchKernelGGL(k_fg_reduce, dim3(2), dim3(KM_BLOCK), 0, ctx->stream, fl.G, (const double*)(P + FG_PN),
+                           S + FG_SN);
+        hipLaunchKernelGGL(k_fg_step<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, (const double*)(S + FG_SN),
+                           (const T*)q, (const T*)z, Q + (size_t)k * n, Z + (size_t)k * n, x, r64,
+                           P + FG_PN + 2 * KM_MAX_RED);
+        hipLaunchKernelGGL(k_fg_reduce, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, fl.G,
+                           (const double*)(P + FG_PN + 2 * KM_MAX_RED), S + FG_SR);
+        KM_CHECK_LAUNCH(ctx);
+        KM_HIP(ctx, hipMemcpyAsync(ctx->h_res + 40, S + FG_SN, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
+        const double qq = ctx->h_res[40];
+        if (!(qq > 0.0) || !(qq == qq)) {   // no new direction: the preconditioner gave nothing usable
+            status = qq == qq ? KM_BREAKDOWN : KM_NONFINITE;
+            break;
+        }
+        rr = ctx->h_res[42];
+        total++;
+        k++;
+        rel = sqrt(rr) / bn;
+        if (rel < eps || k == kFgMax) {
+            if (int rc = true_residual(&rr)) return rc;
+            rel = sqrt(rr) / bn;
+            if (rel < eps) { status = KM_OK; break; }
+            if (k == kFgMax) {
+                if (!(rel < 0.9 * cycle_start)) { status = KM_STALL; break; }
+                cycle_start = rel;
+                k = 0;
+                restarts++;
+            }
+        }
+    }
+    if (status != KM_OK) {
+        km_solve_info sub;
+        const int rc = km_bicgstab(ctx, NX, NY, coef, b, x, eps, max_iter > total ? max_iter - total : 1,
+                                   KM_FLAG_X0 | KM_FLAG_TRUE, &sub);
+        if (rc < 0) return rc;
+        total += sub.iterations;
+        status = sub.status;
+        rel = sub.rel_resid_true;
+        if (info) info->fallback = 1;
+    }
+    if (info) {
+        info->iterations = total;
+        info->status = status;
+        info->refinements = restarts;
+        info->rel_resid = rel;
+        info->rel_resid_true = rel;
+    }
+    return KM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -604,6 +902,9 @@ int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* 
     KM_REQUIRE(ctx, eps > 0.0, "eps must be positive");
     const int64_t n = (int64_t)NX * NY;
     if (max_iter <= 0) max_iter = (int)(n < (1 << 30) ? n : (1 << 30));
+    if (flags & KM_FLAG_GCR)
+        return (flags & KM_FLAG_MIXED) ? fgcr_solve<float>(ctx, NX, NY, coef, b, x, eps, max_iter, flags, params, info)
+                                       : fgcr_solve<double>(ctx, NX, NY, coef, b, x, eps, max_iter, flags, params, info);
     if (flags & KM_FLAG_MIXED) return fbicg_solve<float>(ctx, NX, NY, coef, b, x, eps, max_iter, flags, params, info);
     return fbicg_solve<double>(ctx, NX, NY, coef, b, x, eps, max_iter, flags, params, info);
 }

@@ -134,20 +134,25 @@ class UNetPreconditioner:
 
 
 def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: Optional[UNetPreconditioner], eps=1e-8,
-              max_iter=None, x0=False, graph=True, mixed=False, coarse=False, smooth=False):
+              max_iter=None, x0=False, graph=True, mixed=False, coarse=False, smooth=False, gcr=False,
+              jacobi=False):
     """Flexible BiCGSTAB with the network as right preconditioner
     (km_fbicgstab); ``net=None`` uses the Jacobi preconditioner instead.
     ``mixed``: FP32 Krylov vectors (FP64 scalars and refinement).
     ``coarse``: two-level preconditioner (aggregation coarse correction, then
     the network, or red-black SOR sweeps without one).  ``smooth``: one
     red-black Gauss-Seidel sweep as the preconditioner (no network, no coarse
-    level; the momentum systems)."""
+    level; the momentum systems).  ``gcr``: flexible GCR as the outer method
+    (KM_FLAG_GCR), which a non-linear preconditioner cannot break;
+    ``jacobi`` (with ``gcr`` and ``coarse``, no network): one Jacobi step
+    after the coarse correction instead of the SOR sweeps."""
     from .solvers import SolveInfo
     NX, NY = b.shape
     info = KmSolveInfo()
     flags = (_lib.KM_FLAG_X0 if x0 else 0) | (0 if graph else _lib.KM_FLAG_NO_GRAPH)
     flags |= (_lib.KM_FLAG_MIXED if mixed else 0) | (_lib.KM_FLAG_COARSE if coarse else 0)
     flags |= _lib.KM_FLAG_SMOOTH if smooth else 0
+    flags |= (_lib.KM_FLAG_GCR if gcr else 0) | (_lib.KM_FLAG_JACOBI if jacobi else 0)
     ctx = _lib.context_for(b.device) if net is None else net.ctx
     ctx.set_stream(torch.cuda.current_stream(b.device).cuda_stream)
     ctx.call("km_fbicgstab", NX, NY, ptr(coef), ptr(b), ptr(x), float(eps), int(max_iter or 0), flags,

@@ -99,6 +99,7 @@ class SIMPLESolver:
         self.mom_smooth = False       # momentum: flexible BiCGSTAB with a red-black Gauss-Seidel sweep
         self.native_step = True       # step() as one km_simple_step call where the configuration allows
         self.true_residual = False    # plain solves: restart while the true residual is above eps (KM_FLAG_TRUE)
+        self.p_gcr = False            # p' with a network: flexible GCR as the outer method (KM_FLAG_GCR)
         self._case = KmCase()
         self._step_info = (KmSolveInfo * 3)()
         self.last = {}
@@ -215,7 +216,7 @@ class SIMPLESolver:
             return self.p_correction_cg.solve(eps=self.eps_p)
         if self.p_precond is not None or self.p_coarse:
             return self.p_correction_solver.solve(eps=self.eps_p, x0=self.warm_start_p, precond=self.p_precond,
-                                                 mixed=self.mixed, coarse=self.p_coarse)
+                                                 mixed=self.mixed, coarse=self.p_coarse, gcr=self.p_gcr)
         return self.p_correction_solver.solve(eps=self.eps_p, mixed=self.mixed, x0=self.warm_start_p,
                                              true_residual=self.true_residual)
 

@@ -158,7 +158,7 @@ class BICGSolver(CGSolver):
     _kind = "bicgstab"
 
     def solve(self, eps=1e-8, quiet=True, max_iter=None, mixed=False, x0=False, graph=True,
-              precond=None, coarse=False, smooth=False, true_residual=False) -> SolveInfo:
+              precond=None, coarse=False, smooth=False, true_residual=False, gcr=False) -> SolveInfo:
         """BiCGSTAB until sqrt(r.r / b.b) < eps (bicgsolver.py:130).
 
         ``mixed``: FP32 matvec/axpy inner solves, FP64 scalars, FP64 refinement
@@ -172,6 +172,9 @@ class BICGSolver(CGSolver):
         network -- or red-black SOR sweeps without one).  ``smooth``: flexible
         BiCGSTAB with one red-black Gauss-Seidel sweep as the preconditioner
         (the momentum systems), judged by the true residual like the others.
+        ``gcr``: flexible GCR instead of flexible BiCGSTAB as the outer
+        method of the preconditioned solve (the network's: a non-linear
+        preconditioner cannot break it).
         ``true_residual`` (plain solve): after the reference's stop on the
         recursive residual, restart from x while the true residual is above
         eps (KM_FLAG_TRUE; the reference has no such check).
@@ -179,11 +182,11 @@ class BICGSolver(CGSolver):
         flags = (_lib.KM_FLAG_MIXED if mixed else 0) | (_lib.KM_FLAG_X0 if x0 else 0)
         flags |= _lib.KM_FLAG_TRUE if true_residual else 0
         flags |= 0 if graph else _lib.KM_FLAG_NO_GRAPH
-        if precond is not None or coarse or smooth:
+        if precond is not None or coarse or smooth or gcr:
             from .learned import fbicgstab
             self._refresh()
             info = fbicgstab(self.coef, self.b, self.x, precond, eps=eps, max_iter=max_iter, x0=x0, graph=graph,
-                             mixed=mixed, coarse=coarse, smooth=smooth)
+                             mixed=mixed, coarse=coarse, smooth=smooth, gcr=gcr)
         else:
             info = self._run("km_bicgstab", eps, max_iter, flags)
         if not quiet:

@@ -42,8 +42,9 @@ def log(msg):
 def evaluate(case, outer_steps: int, variants: dict, eps: float = 1e-8, device=None):
     """Per outer step of ``case``: plain FP64 BiCGSTAB and every variant's
     flexible solve on the same p' system (the SIMPLE run continues with the
-    plain solution).  variants: label -> (network or None for Jacobi, coarse,
-    mixed).  Returns {label: {"it": [...], "ms": [...], "fallback": n, "true": max}}."""
+    plain solution).  variants: label -> keyword arguments of
+    ``learned.fbicgstab`` (``net``: the network or None).  Returns
+    {label: {"it": [...], "ms": [...], "fallback": n, "true": max}}."""
     s = SIMPLESolver.from_case(case, device=device)
     s.init()
     res = {k: {"it": [], "ms": [], "fallback": 0, "true": 0.0} for k in ["plain", *variants]}
@@ -52,11 +53,13 @@ def evaluate(case, outer_steps: int, variants: dict, eps: float = 1e-8, device=N
         s.solve_momentum_eqn(1)
         s.correct_conserv()
         s.compute_coef_p()
-        for label, (m, coarse, mixed) in variants.items():
+        for label, kw in variants.items():
+            kw = dict(kw)
+            net = kw.pop("net", None)
             x.zero_()
             torch.cuda.synchronize(device)
             t0 = time.perf_counter()
-            info = learned.fbicgstab(s._coef_p, s.b_p, x, m, eps=eps, mixed=mixed, coarse=coarse)
+            info = learned.fbicgstab(s._coef_p, s.b_p, x, net, eps=eps, **kw)
             torch.cuda.synchronize(device)
             r = res[label]
             r["ms"].append((time.perf_counter() - t0) * 1e3)
@@ -79,13 +82,22 @@ def evaluate(case, outer_steps: int, variants: dict, eps: float = 1e-8, device=N
 
 
 def variants_for(net, untrained, coarse: bool) -> dict:
-    """The learned solve and its ablations: the same structure with an untrained
-    network and with no network (label "jacobi": the default smoother in its
-    place -- red-black SOR sweeps after the coarse correction, Jacobi without)."""
-    v = {"learned": (net, coarse, False), "learned_fp32": (net, coarse, True),
-         "untrained": (untrained, coarse, False), "jacobi": (None, coarse, False)}
+    """The learned solve and its ablations (keyword arguments of
+    learned.fbicgstab): the network as the right preconditioner of flexible
+    BiCGSTAB and of flexible GCR (``gcr``, which a non-linear preconditioner
+    cannot break), in FP32 directions; the same with an untrained network
+    (what training gains); with no network -- a Jacobi step after the coarse
+    correction in GCR (the cheapest smoother in the network's place), and the
+    default two-level solve (red-black SOR sweeps, flexible BiCGSTAB)."""
+    v = {"learned_fp32": dict(net=net, coarse=coarse, mixed=True),
+         "learned_gcr_fp32": dict(net=net, coarse=coarse, mixed=True, gcr=True),
+         "untrained_gcr_fp32": dict(net=untrained, coarse=coarse, mixed=True, gcr=True)}
     if coarse:
-        v["jacobi_fp32"] = (None, True, True)
+        v["coarse_jacobi_gcr_fp32"] = dict(net=None, coarse=True, mixed=True, gcr=True, jacobi=True)
+        v["twolevel_sor"] = dict(net=None, coarse=True)
+        v["twolevel_sor_fp32"] = dict(net=None, coarse=True, mixed=True)
+    else:
+        v["jacobi_gcr_fp32"] = dict(net=None, mixed=True, gcr=True)
     return v
 
 

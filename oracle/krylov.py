@@ -310,6 +310,84 @@ def fbicgstab(coef, b, eps=1e-8, x0=None, max_iter=None, precond=None, stall_win
 
 # ---------------------------------------------------------------- two-level
 
+def fgcr(coef, b, eps=1e-8, max_iter=None, precond=None, restart=16, history=None):
+    """Flexible GCR (the generalised conjugate residual method with a
+    preconditioner that may change from one application to the next, e.g. a
+    neural network) judged by the residual relative to ||b||, x0 = 0.
+
+    Iteration k: z = M(r), q = A z; q and z are orthogonalised against the
+    stored q_j (classical Gram-Schmidt, twice) and scaled so that ||q|| = 1;
+    alpha = <r, q>; x += alpha z; r -= alpha q.  Each step minimises ||r||
+    over the stored directions, so ||r|| never increases.  After ``restart``
+    directions the basis is cleared (x and r are kept).  On the recursive
+    residual reaching eps the true residual b - A x replaces r and decides.
+    With M = identity and no restart the residual norms are those of GMRES.
+    """
+    NX, NY = b.shape
+    steps = NX * NY if max_iter is None else max_iter
+    M = (lambda z: z) if precond is None else precond
+    bb = dot(b, b)
+    info = Info(iterations=0, status=OK)
+    x = np.zeros_like(b)
+    if bb == 0.0:
+        info["rel_resid_true"] = 0.0
+        return x, info
+    bn = math.sqrt(bb)
+    r = b.copy()
+    Q, Z = [], []
+    status = MAXITER
+    it = 0
+    while it < steps:
+        z = M(r)
+        q = stencil(coef, z)
+        for _ in range(2):
+            beta = [dot(q, qj) for qj in Q]
+            for bj, qj, zj in zip(beta, Q, Z):
+                q = q - bj * qj
+                z = z - bj * zj
+        nq = math.sqrt(dot(q, q))
+        if not (nq > 0.0) or not math.isfinite(nq):
+            status = BREAKDOWN if nq == 0.0 else NONFINITE
+            break
+        q, z = q / nq, z / nq
+        alpha = dot(r, q)
+        x = x + alpha * z
+        r = r - alpha * q
+        it += 1
+        rel = math.sqrt(dot(r, r)) / bn
+        if history is not None:
+            history.append(rel)
+        if rel < eps:
+            r = b - stencil(coef, x)
+            if math.sqrt(dot(r, r)) / bn < eps:
+                status = OK
+                break
+        Q.append(q)
+        Z.append(z)
+        if len(Q) >= restart:
+            Q, Z = [], []
+    info["iterations"] = it
+    info["status"] = status
+    rt = b - stencil(coef, x)
+    info["rel_resid_true"] = math.sqrt(dot(rt, rt)) / bn
+    return x, info
+
+
+def coarse_jacobi_precond(coef, B):
+    """M(r) = z1 + D^-1 (r - A z1), z1 = P A_c^-1 P^T r: the coarse correction
+    followed by one Jacobi step (the coarse correction alone maps every
+    residual into the aggregates' piecewise-constant space, so it cannot
+    precondition by itself)."""
+    P, Ac = aggregation(coef, B)
+    Aci = np.linalg.inv(Ac)
+    jac = jacobi_precond(coef)
+
+    def M(r):
+        z1 = (P @ (Aci @ (P.T @ r.ravel()))).reshape(r.shape)
+        return z1 + jac(r - stencil(coef, z1))
+    return M
+
+
 def coarse_block(nx, ny, req=0):
     """Aggregate edge length: req, else the smallest of 8, 12, 16, 24, ..., 256
     giving at most 2048 coarse unknowns."""
