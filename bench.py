@@ -364,12 +364,15 @@ def p_compare(s, reps):
 def learned_compare(s, weights, reps):
     """The metric's second leg on the last outer step's p' system, each solve
     from zero to 1e-8 in the true residual, ``reps`` times: plain FP64
-    BiCGSTAB against the learned flexible BiCGSTAB (two-level preconditioner
-    with the trained U-Net as its smoother, BF16 network on the matrix
-    cores, FP32 Krylov vectors, FP64 scalars and true residual), with the
-    same solve without the network (coarse correction and red-black SOR
-    sweeps, FP32 vectors) beside it.  Median wall ms (each solve returns
-    after a stream synchronisation), iterations, true residuals."""
+    BiCGSTAB against the learned solve -- flexible GCR whose every
+    preconditioner application gives two directions after the coarse
+    correction, a Jacobi step and the trained U-Net (BF16 on the matrix
+    cores), FP32 residual-space vectors, FP64 solution-space vectors,
+    scalars and true residual, plain BiCGSTAB as the fallback when a restart
+    cycle stalls -- beside the same solve without the network (coarse
+    correction + Jacobi step) and the default two-level flexible BiCGSTAB
+    (red-black SOR sweeps).  Median wall ms (each solve returns after a
+    stream synchronisation), iterations, true residuals, fallbacks."""
     import torch
     from karman_taichi_amd import learned as kl
     if not os.path.exists(weights):
@@ -378,8 +381,10 @@ def learned_compare(s, weights, reps):
     x = torch.zeros_like(s.b_p)
     runs = {
         "plain": lambda: s.p_correction_solver.solve(eps=1e-8),
-        "learned_fp32": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, net, eps=1e-8, mixed=True, coarse=True),
-        "no_network_fp32": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, mixed=True, coarse=True),
+        "learned_fp32": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, net, eps=1e-8, mixed=True, coarse=True, gcr=True),
+        "no_network_fp32": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, mixed=True, coarse=True,
+                                                gcr=True, jacobi=True),
+        "twolevel_sor": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, coarse=True),
     }
     out = {"weights": os.path.relpath(weights, ROOT)}
     for name, fn in runs.items():

@@ -57,8 +57,10 @@ extern "C" {
                                * of flexible BiCGSTAB: ||r|| never increases whatever the preconditioner does, so a
                                * non-linear network preconditioner cannot break its recurrences (no counterpart in the
                                * reference) */
-#define KM_FLAG_JACOBI 128    /* km_fbicgstab with KM_FLAG_GCR and KM_FLAG_COARSE, no network: one Jacobi step after the
-                               * coarse correction instead of the red-black SOR sweeps */
+#define KM_FLAG_JACOBI 128    /* km_fbicgstab with KM_FLAG_GCR and KM_FLAG_COARSE: without a network, one Jacobi step
+                               * after the coarse correction instead of the red-black SOR sweeps; with a network, one
+                               * direction z1 + net(r1) followed by a Jacobi step instead of the two directions
+                               * z1 + D^-1 r1 and z1 + net(r1) */
 #define KM_FLAG_TRUE 32       /* km_bicgstab: after the reference's stop on the recursive residual, check the true
                                * residual ||b - A x|| / ||b|| and restart from x (r = b - A x) while it is above eps
                                * and each restart at least halves it (at most 8 restarts; no counterpart in the

@@ -607,9 +607,11 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
 // against the stored q_j twice (classical Gram-Schmidt), scaled to ||q|| = 1
 // and stored; alpha = <r, q>, x += alpha z, r -= alpha q.  ||r|| never
 // increases, whatever M does, so a non-linear network cannot break the
-// recurrences as it breaks flexible BiCGSTAB's.  z, q and the stored
-// directions are in T (FP32 with KM_FLAG_MIXED); x, r and every dot product
-// in FP64.  At each restart, and when the recursive residual reaches eps, r
+// recurrences as it breaks flexible BiCGSTAB's.  The residual-space vectors
+// (q and the stored q_j) are in T (FP32 with KM_FLAG_MIXED); the
+// solution-space ones (z, the stored z_j), x, r, A z and every dot product in
+// FP64: A amplifies the rounding noise of an FP32 z by up to ~1e5 on smooth
+// directions, which would leave q = A z and z inconsistent.  At each restart, and when the recursive residual reaches eps, r
 // is replaced by the true residual b - A x (FP64), which alone decides.  A
 // restart cycle that does not lower the true residual by 10% ends the solve
 // (stall), and plain BiCGSTAB finishes from x.
@@ -620,19 +622,62 @@ static_assert(FG_PN + 3 * KM_MAX_RED <= KM_MAX_PARTIALS, "FGCR partials");
 // device scalars in ctx->dres: beta[kFgMax], then <q,q>, <r,q>, <r,r>
 constexpr int FG_SB = 0, FG_SN = kFgMax, FG_SR = kFgMax + 2;
 
+// q = A z with the FP64 coefficients and FP64 products and sums, rounded to
+// T once: on a smooth z (the coarse correction is piecewise constant) the
+// five terms of a p' row cancel to a small fraction of aP z, which FP32
+// arithmetic would leave as rounding noise.
 template <typename T>
-__global__ void __launch_bounds__(KM_BLOCK)
-k_fg_stencil(int64_t n, int NX, int NY, const T* __restrict__ c, const T* __restrict__ z, T* __restrict__ q) {
-    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK)
-        q[k] = sten5(c, z, k, n, NX, NY);
+__device__ __forceinline__ double sten5_64(const double* __restrict__ c, const T* __restrict__ x, int64_t k, int64_t n,
+                                           int NX, int NY) {
+    const int i = (int)(k / NY), j = (int)(k % NY);
+    double y = c[k] * (double)x[k];
+    y = y + c[n + k] * (i > 0 ? (double)x[k - NY] : 0.0);
+    y = y + c[2 * n + k] * (i < NX - 1 ? (double)x[k + NY] : 0.0);
+    y = y + c[3 * n + k] * (j < NY - 1 ? (double)x[k + 1] : 0.0);
+    y = y + c[4 * n + k] * (j > 0 ? (double)x[k - 1] : 0.0);
+    return y;
 }
 
-// z += D^-1 r1 (a Jacobi step after the coarse correction; aP = 0 rows: none)
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_fg_jacobi_add(int64_t n, const T* __restrict__ c, const T* __restrict__ r1, T* __restrict__ z) {
+k_fg_stencil(int64_t n, int NX, int NY, const double* __restrict__ c, const double* __restrict__ z,
+             T* __restrict__ q) {
     for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK)
-        if (c[k] != T(0)) z[k] = z[k] + r1[k] / c[k];
+        q[k] = (T)sten5_64(c, z, k, n, NX, NY);
+}
+
+// zd = z (widened to FP64)
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK) k_fg_widen(int64_t n, const T* __restrict__ z, double* __restrict__ zd) {
+    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK)
+        zd[k] = (double)z[k];
+}
+
+// r1 = r - A z1 in FP64 arithmetic (see k_fg_stencil), with the partials of
+// r1.r1 in the flat geometry fl (the network's input scale)
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_fg_resid1(Flat fl, int NX, int NY, const double* __restrict__ c, const T* __restrict__ r, const T* __restrict__ z1,
+            T* __restrict__ r1, double* __restrict__ P) {
+    __shared__ double sh[4];
+    double a = 0.0;
+    flat_sweep_fg(fl, [&](int64_t k) {
+        const double v = (double)r[k] - sten5_64(c, z1, k, fl.n, NX, NY);
+        const T vt = (T)v;
+        r1[k] = vt;
+        a += (double)vt * (double)vt;
+    });
+    a = block_sum(a, sh);
+    if (threadIdx.x == 0 && P) P[blockIdx.x] = a;
+}
+
+// zd = z1 + D^-1 r1 in FP64 (a Jacobi step after the coarse correction; aP = 0 rows: z1)
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_fg_jacobi_add(int64_t n, const double* __restrict__ c, const T* __restrict__ z1, const T* __restrict__ r1,
+                double* __restrict__ zd) {
+    for (int64_t k = (int64_t)blockIdx.x * KM_BLOCK + threadIdx.x; k < n; k += (int64_t)gridDim.x * KM_BLOCK)
+        zd[k] = (double)z1[k] + (c[k] != 0.0 ? (double)r1[k] / c[k] : 0.0);
 }
 
 // partials of <q, Q_j>, j < m
@@ -665,22 +710,22 @@ k_fg_reduce(int G, const double* __restrict__ P, double* __restrict__ out) {
     if (threadIdx.x == 0) out[blockIdx.x] = t[0];
 }
 
-// q -= sum_j beta_j Q_j, z -= sum_j beta_j Z_j
+// q -= sum_j beta_j Q_j, z -= sum_j beta_j Z_j (FP64 arithmetic)
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_fg_sub(Flat fl, int m, const double* __restrict__ beta, T* __restrict__ q, T* __restrict__ z,
-         const T* __restrict__ Q, const T* __restrict__ Z) {
+k_fg_sub(Flat fl, int m, const double* __restrict__ beta, T* __restrict__ q, double* __restrict__ z,
+         const T* __restrict__ Q, const double* __restrict__ Z) {
     __shared__ double bs[kFgMax];
     if (threadIdx.x < kFgMax) bs[threadIdx.x] = threadIdx.x < (unsigned)m ? beta[threadIdx.x] : 0.0;
     __syncthreads();
     flat_sweep_fg(fl, [&](int64_t k) {
-        T qk = q[k], zk = z[k];
+        double qk = (double)q[k], zk = z[k];
         for (int j = 0; j < m; ++j) {
-            const T bj = (T)bs[j];
-            qk = qk - bj * Q[(int64_t)j * fl.n + k];
+            const double bj = bs[j];
+            qk = qk - bj * (double)Q[(int64_t)j * fl.n + k];
             zk = zk - bj * Z[(int64_t)j * fl.n + k];
         }
-        q[k] = qk;
+        q[k] = (T)qk;
         z[k] = zk;
     });
 }
@@ -709,18 +754,20 @@ k_fg_norms(Flat fl, const T* __restrict__ q, const double* __restrict__ r, doubl
 // alpha = <r, q> / ||q||; partials of <r, r>
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
-k_fg_step(Flat fl, const double* __restrict__ s, const T* __restrict__ q, const T* __restrict__ z, T* __restrict__ Qk,
-          T* __restrict__ Zk, double* __restrict__ x, double* __restrict__ r, double* __restrict__ P) {
+k_fg_step(Flat fl, const double* __restrict__ s, const T* __restrict__ q, const double* __restrict__ z,
+          T* __restrict__ Qk, double* __restrict__ Zk, double* __restrict__ x, double* __restrict__ r,
+          double* __restrict__ P) {
     __shared__ double sh[4];
     const double nq = sqrt(s[0]);
     const double inv = nq > 0.0 ? 1.0 / nq : 0.0;
     const double alpha = s[1] * inv;
     double a = 0.0;
     flat_sweep_fg(fl, [&](int64_t k) {
-        const T qn = (T)((double)q[k] * inv), zn = (T)((double)z[k] * inv);
+        const T qn = (T)((double)q[k] * inv);
+        const double zn = z[k] * inv;
         Qk[k] = qn;
         Zk[k] = zn;
-        x[k] = x[k] + alpha * (double)zn;
+        x[k] = x[k] + alpha * zn;
         const double rk = r[k] - alpha * (double)qn;
         r[k] = rk;
         a += rk * rk;
@@ -740,19 +787,22 @@ int fgcr_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b,
     KM_REQUIRE(ctx, !(flags & KM_FLAG_SMOOTH), "KM_FLAG_GCR takes no KM_FLAG_SMOOTH");
     const FgPrecond kind = params ? FG_NET
                                   : (coarse ? ((flags & KM_FLAG_JACOBI) ? FG_COARSE_JACOBI : FG_TWOLEVEL) : FG_JACOBI);
-    // r64 | c32 | rhs, z, q, r1 | Q[kFgMax] | Z[kFgMax]
-    const size_t bytes = (size_t)n * sizeof(double) + (size_t)5 * n * sizeof(float) + 512 +
-                         (size_t)(4 + 2 * kFgMax) * n * sizeof(T);
+    // r64 | zd, z2d, Z[kFgMax] (FP64) | c32 | rhs, z, q, r1, z2 | Q[kFgMax] (T)
+    const size_t bytes = (size_t)(3 + kFgMax) * n * sizeof(double) + (size_t)5 * n * sizeof(float) + 512 +
+                         (size_t)(5 + kFgMax) * n * sizeof(T);
     if (int rc = km_ensure_workspace(ctx, bytes)) return rc;
     double* r64 = static_cast<double*>(km_ws_vectors(ctx));
-    float* c32 = reinterpret_cast<float*>(r64 + n);
+    double* zd = r64 + n;
+    double* z2d = zd + n;
+    double* Z = z2d + n;
+    float* c32 = reinterpret_cast<float*>(Z + (size_t)kFgMax * n);
     T* base = reinterpret_cast<T*>((reinterpret_cast<uintptr_t>(c32 + 5 * n) + 255) & ~uintptr_t(255));
     T* rhs = base;
     T* z = rhs + n;
     T* q = z + n;
     T* r1 = q + n;
-    T* Q = r1 + n;
-    T* Z = Q + (size_t)kFgMax * n;
+    T* z2 = r1 + n;
+    T* Q = z2 + n;
     if (params || !f64) {
         hipLaunchKernelGGL(k_scale_to<float>, dim3(blocks(5 * n)), dim3(KM_BLOCK), 0, ctx->stream, 5 * n, coef, c32,
                            1.0);
@@ -806,69 +856,113 @@ int fgcr_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b,
     while (true) {
         if (!(rel == rel)) { status = KM_NONFINITE; break; }
         if (total >= max_iter) { status = KM_MAXITER; break; }
-        // z = M(r / ||r||)
+        // the directions of this application of M to r / ||r||: one, or with
+        // the network after the coarse correction two -- z = z1 + D^-1 r1 and
+        // z2 = z1 + net(r1) -- so that GCR minimises over both and the
+        // network's direction can only add to the classical one
         hipLaunchKernelGGL(k_scale_to<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, (const double*)r64, rhs,
                            1.0 / sqrt(rr));
         KM_CHECK_LAUNCH(ctx);
+        int ndir = 1;
         if (kind == FG_NET && coarse) {
-            if (int rc = kmcoarse::apply<T>(ctx, coef, coefT, rhs, z, r1, P + FB_PR, nullptr, ctx->stream)) return rc;
-            if (int rc = kmnet::apply_pre<T>(ctx, NX, NY, c32, r1, z, nullptr, ctx->stream, params, P + FB_PR, fl.G,
+            if (int rc = kmcoarse::apply<T>(ctx, coef, coefT, rhs, z, r1, nullptr, nullptr, ctx->stream)) return rc;
+            hipLaunchKernelGGL(k_fg_resid1<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, NX, NY, coef,
+                               (const T*)rhs, (const T*)z, r1, P + FB_PR);
+            KM_HIP(ctx, hipMemcpyAsync(z2, z, n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+            if (int rc = kmnet::apply_pre<T>(ctx, NX, NY, c32, r1, z2, nullptr, ctx->stream, params, P + FB_PR, fl.G,
                                              true))
                 return rc;
+            if (flags & KM_FLAG_JACOBI) {
+                // one direction: z1 + net(r1), then a Jacobi step on what is left
+                hipLaunchKernelGGL(k_fg_resid1<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, NX, NY, coef,
+                                   (const T*)rhs, (const T*)z2, r1, (double*)nullptr);
+                hipLaunchKernelGGL(k_fg_jacobi_add<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, coef,
+                                   (const T*)z2, (const T*)r1, zd);
+            } else {
+                hipLaunchKernelGGL(k_fg_widen<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, (const T*)z2, z2d);
+                hipLaunchKernelGGL(k_fg_jacobi_add<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, coef, (const T*)z,
+                                   (const T*)r1, zd);
+                ndir = 2;
+            }
         } else if (kind == FG_NET) {
             if (int rc = kmnet::apply<T>(ctx, NX, NY, c32, rhs, z, nullptr, ctx->stream, params)) return rc;
+            hipLaunchKernelGGL(k_fg_widen<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, (const T*)z, zd);
         } else if (kind == FG_TWOLEVEL) {
             if (int rc = kmcoarse::restrict_solve<T>(ctx, rhs, nullptr, ctx->stream)) return rc;
             if (int rc = kmcoarse::smooth<T>(ctx, coefT, rhs, z, ctx->stream)) return rc;
+            hipLaunchKernelGGL(k_fg_widen<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, (const T*)z, zd);
         } else if (kind == FG_COARSE_JACOBI) {
             if (int rc = kmcoarse::apply<T>(ctx, coef, coefT, rhs, z, r1, nullptr, nullptr, ctx->stream)) return rc;
-            hipLaunchKernelGGL(k_fg_jacobi_add<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, coefT, (const T*)r1, z);
+            hipLaunchKernelGGL(k_fg_resid1<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, NX, NY, coef,
+                               (const T*)rhs, (const T*)z, r1, (double*)nullptr);
+            hipLaunchKernelGGL(k_fg_jacobi_add<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, coef, (const T*)z,
+                               (const T*)r1, zd);
         } else {
             hipLaunchKernelGGL(k_jacobi<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, coefT, (const T*)rhs, z,
                                (const int*)nullptr);
+            hipLaunchKernelGGL(k_fg_widen<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, (const T*)z, zd);
         }
-        // q = A z, orthogonalised twice against the stored directions
-        hipLaunchKernelGGL(k_fg_stencil<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, NX, NY, coefT, (const T*)z, q);
-        for (int pass = 0; pass < 2 && k > 0; ++pass) {
-            hipLaunchKernelGGL(k_fg_dots<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, k, (const T*)q,
-                               (const T*)Q, P + FG_PD);
-            hipLaunchKernelGGL(k_fg_reduce, dim3(k), dim3(KM_BLOCK), 0, ctx->stream, fl.G, (const double*)(P + FG_PD),
-                               S + FG_SB);
-            hipLaunchKernelGGL(k_fg_sub<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, k, (const double*)(S + FG_SB),
-                               q, z, (const T*)Q, (const T*)Z);
-        }
-        hipLaunchKernelGGL(k_fg_norms<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, (const T*)q,
-                           (const double*)r64, P + FG_PN);
-        hipLaunchKernelGGL(k_fg_reduce, dim3(2), dim3(KM_BLOCK), 0, ctx->stream, fl.G, (const double*)(P + FG_PN),
-                           S + FG_SN);
-        hipLaunchKernelGGL(k_fg_step<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, (const double*)(S + FG_SN),
-                           (const T*)q, (const T*)z, Q + (size_t)k * n, Z + (size_t)k * n, x, r64,
-                           P + FG_PN + 2 * KM_MAX_RED);
-        hipLaunchKernelGGL(k_fg_reduce, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, fl.G,
-                           (const double*)(P + FG_PN + 2 * KM_MAX_RED), S + FG_SR);
-        KM_CHECK_LAUNCH(ctx);
-        KM_HIP(ctx, hipMemcpyAsync(ctx->h_res + 40, S + FG_SN, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
-        const double qq = ctx->h_res[40];
-        if (!(qq > 0.0) || !(qq == qq)) {   // no new direction: the preconditioner gave nothing usable
-            status = qq == qq ? KM_BREAKDOWN : KM_NONFINITE;
-            break;
-        }
-        rr = ctx->h_res[42];
-        total++;
-        k++;
-        rel = sqrt(rr) / bn;
-        if (rel < eps || k == kFgMax) {
-            if (int rc = true_residual(&rr)) return rc;
+        bool stop = false;
+        for (int d = 0; d < ndir && !stop; ++d) {
+            double* zdir = d == 0 ? zd : z2d;
+            // q = A zd, orthogonalised twice against the stored directions
+            hipLaunchKernelGGL(k_fg_stencil<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, NX, NY, coef,
+                               (const double*)zdir, q);
+            for (int pass = 0; pass < 2 && k > 0; ++pass) {
+                hipLaunchKernelGGL(k_fg_dots<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, k, (const T*)q,
+                                   (const T*)Q, P + FG_PD);
+                hipLaunchKernelGGL(k_fg_reduce, dim3(k), dim3(KM_BLOCK), 0, ctx->stream, fl.G,
+                                   (const double*)(P + FG_PD), S + FG_SB);
+                hipLaunchKernelGGL(k_fg_sub<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, k,
+                                   (const double*)(S + FG_SB), q, zdir, (const T*)Q, (const double*)Z);
+            }
+            hipLaunchKernelGGL(k_fg_norms<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, (const T*)q,
+                               (const double*)r64, P + FG_PN);
+            hipLaunchKernelGGL(k_fg_reduce, dim3(2), dim3(KM_BLOCK), 0, ctx->stream, fl.G, (const double*)(P + FG_PN),
+                               S + FG_SN);
+            hipLaunchKernelGGL(k_fg_step<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl,
+                               (const double*)(S + FG_SN), (const T*)q, (const double*)zdir, Q + (size_t)k * n,
+                               Z + (size_t)k * n, x, r64, P + FG_PN + 2 * KM_MAX_RED);
+            hipLaunchKernelGGL(k_fg_reduce, dim3(1), dim3(KM_BLOCK), 0, ctx->stream, fl.G,
+                               (const double*)(P + FG_PN + 2 * KM_MAX_RED), S + FG_SR);
+            KM_CHECK_LAUNCH(ctx);
+            KM_HIP(ctx, hipMemcpyAsync(ctx->h_res + 40, S + FG_SN, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            KM_HIP_SYNC(ctx, hipStreamSynchronize(ctx->stream));
+            const double qq = ctx->h_res[40];
+            if (!(qq > 0.0) || !(qq == qq)) {
+                // no new direction (zd lies in the stored ones); another direction may still add one
+                if (!(qq == qq)) {
+                    status = KM_NONFINITE;
+                    stop = true;
+                } else if (d + 1 == ndir) {
+                    status = KM_BREAKDOWN;
+                    stop = true;
+                }
+                continue;
+            }
+            rr = ctx->h_res[42];
+            total++;
+            k++;
             rel = sqrt(rr) / bn;
-            if (rel < eps) { status = KM_OK; break; }
-            if (k == kFgMax) {
-                if (!(rel < 0.9 * cycle_start)) { status = KM_STALL; break; }
-                cycle_start = rel;
-                k = 0;
-                restarts++;
+            if (rel < eps || k == kFgMax) {
+                if (int rc = true_residual(&rr)) return rc;
+                rel = sqrt(rr) / bn;
+                if (rel < eps) {
+                    status = KM_OK;
+                    stop = true;
+                } else if (k == kFgMax) {
+                    if (!(rel < 0.9 * cycle_start)) {
+                        status = KM_STALL;
+                        stop = true;
+                    }
+                    cycle_start = rel;
+                    k = 0;
+                    restarts++;
+                }
             }
         }
+        if (stop) break;
     }
     if (status != KM_OK) {
         km_solve_info sub;

@@ -332,3 +332,61 @@ def train(net: UNetPreconditioner, pairs: PairSet, steps: int, batch: int = 8, l
             if progress is not None:
                 progress(f"train: step {it}/{steps}, loss {m:.4f}, {time.perf_counter() - t0:.1f} s")
     return hist
+
+
+# -------------------------------------------------------------------- cost
+
+def inference_flops(NX: int, NY: int) -> int:
+    """Multiply-add FLOPs (2 per MAC) of one inference on an NX x NY padded
+    system: the network runs on the interior rounded up to a multiple of 8,
+    H x W, with levels at H, H/2, H/4, H/8 (3x3 convolutions) and the 1x1
+    output layer at full resolution."""
+    H, W = (NX - 2 + 7) // 8 * 8, (NY - 2 + 7) // 8 * 8
+    level = {"e0": 0, "e1": 1, "e2": 2, "e3": 3, "d2": 2, "d1": 1, "d0": 0}
+    f = 0
+    for name, ci, co in LAYERS:
+        s = 1 << level[name]
+        f += 2 * 9 * ci * co * (H // s) * (W // s)
+    return f + 2 * OUT_IN * H * W
+
+
+def main(argv=None):
+    """Apply the network ``--reps`` times to the p' system of the first
+    outer step of the 1280 x 256 channel (a kernel-profiling workload) and
+    print the mean wall time per inference and the FLOP rate."""
+    import argparse
+    import json
+    import sys
+    from . import cases
+    from .simple import SIMPLESolver
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--weights", default=str(WEIGHTS_DIR / "unet_c2.npz"))
+    ap.add_argument("--reps", type=int, default=20)
+    args = ap.parse_args(argv)
+    dev = torch.device("cuda", 0)
+    s = SIMPLESolver.from_case(cases.c2(), device=dev)
+    s.init()
+    s.solve_momentum_eqn(1)
+    s.correct_conserv()
+    s.compute_coef_p()
+    net = UNetPreconditioner.load(args.weights, device=dev)
+    r = s.b_p.clone()
+    z = torch.empty_like(r)
+    for _ in range(3):
+        net.apply(s._coef_p, r, z)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(args.reps):
+        net.apply(s._coef_p, r, z)
+    torch.cuda.synchronize(dev)
+    ms = (time.perf_counter() - t0) * 1e3 / args.reps
+    NX, NY = r.shape
+    fl = inference_flops(NX, NY)
+    out = {"system": [NX, NY], "reps": args.reps, "ms_per_inference": round(ms, 4), "gflop": round(fl / 1e9, 3),
+           "tflops_wall": round(fl / (ms * 1e-3) / 1e12, 2), "z_finite": bool(torch.isfinite(z).all())}
+    print(json.dumps(out), flush=True)
+    print(f"network inference on {NX}x{NY}: {ms:.3f} ms, {fl / 1e9:.2f} GFLOP", file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()

@@ -91,7 +91,9 @@ def variants_for(net, untrained, coarse: bool) -> dict:
     default two-level solve (red-black SOR sweeps, flexible BiCGSTAB)."""
     v = {"learned_fp32": dict(net=net, coarse=coarse, mixed=True),
          "learned_gcr_fp32": dict(net=net, coarse=coarse, mixed=True, gcr=True),
-         "untrained_gcr_fp32": dict(net=untrained, coarse=coarse, mixed=True, gcr=True)}
+         "learned_gcr_jacobi_fp32": dict(net=net, coarse=coarse, mixed=True, gcr=True, jacobi=True),
+         "untrained_gcr_fp32": dict(net=untrained, coarse=coarse, mixed=True, gcr=True),
+         "untrained_gcr_jacobi_fp32": dict(net=untrained, coarse=coarse, mixed=True, gcr=True, jacobi=True)}
     if coarse:
         v["coarse_jacobi_gcr_fp32"] = dict(net=None, coarse=True, mixed=True, gcr=True, jacobi=True)
         v["twolevel_sor"] = dict(net=None, coarse=True)

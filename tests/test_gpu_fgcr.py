@@ -95,3 +95,24 @@ def test_fgcr_zero_rhs(c1_pcor, dev):
     coef, b, _ = c1_pcor
     x, info = gcr(dev, coef, np.zeros_like(b), _lib.KM_FLAG_COARSE)
     assert info.status == _lib.KM_OK and info.iterations == 0 and not x.any()
+
+
+def test_fgcr_fp32_directions_c2(dev):
+    """FP32 residual-space directions on the C2 p' system (the coarse
+    correction + one Jacobi step): the true residual reaches 1e-8 without the
+    plain fallback (the solution-space directions stay FP64)."""
+    from karman_taichi_amd.simple import SIMPLESolver
+    s = SIMPLESolver.from_case(cases.c2(), device=dev)
+    s.init()
+    s.solve_momentum_eqn(1)
+    s.correct_conserv()
+    s.compute_coef_p()
+    coef, b = s._coef_p.cpu().numpy(), s.b_p.cpu().numpy()
+    out = {}
+    for mixed in (False, True):
+        flags = _lib.KM_FLAG_COARSE | _lib.KM_FLAG_JACOBI | (_lib.KM_FLAG_MIXED if mixed else 0)
+        x, info = gcr(dev, coef, b, flags, eps=1e-8)
+        true = resid(coef, x, b)
+        out["fp32" if mixed else "fp64"] = (info.iterations, info.fallback, true)
+        assert info.status == _lib.KM_OK and info.fallback == 0 and true < 1e-8, (mixed, info.iterations)
+    print("C2 FGCR coarse + Jacobi (iterations, fallback, true residual):", out)
