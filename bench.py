@@ -101,6 +101,9 @@ def parse():
                          "timed window starts at this time step")
     ap.add_argument("--learned-reps", type=int, default=3,
                     help="repetitions of the learned+FP32 vs plain p' comparison on the final system (0: skip)")
+    ap.add_argument("--smoothing-tile", type=int, default=-1,
+                    help="two-level modes: pin the smoothing-tile shape (an index of the library's candidates) after "
+                         "the first outer step instead of the autotuner's choice (-1)")
     ap.add_argument("--staged", action="store_true",
                     help="time the step stage by stage from Python instead of one km_simple_step call")
     ap.add_argument("--stage-steps", type=int, default=8,
@@ -192,8 +195,10 @@ def main():
     torch.cuda.synchronize()
     if args.spinup_time_steps:
         log(f"bench: spin-up to time step {args.spinup_time_steps} in {time.perf_counter() - t_spin:.1f} s")
-    for _ in range(args.warmup):
+    for w in range(args.warmup):
         one_step()
+        if w == 0 and twolevel and args.smoothing_tile >= 0:
+            s.ctx.call("km_twolevel_set_tile", int(args.smoothing_tile))
     if twolevel and args.totals != -1:   # the coarse level exists after the first solve
         s.ctx.call("km_set_coarse_totals", int(args.totals))
     if twolevel and args.chunks:
@@ -261,11 +266,13 @@ def main():
     achieved = launch_bytes / (kern[dom] * 1e-6) / 1e9
     traffic = None
     tjson = args.traffic_json or os.path.join(
-        ROOT, "profiles", ("k_tl_traffic_" if twolevel and not learned else "k_bicg_traffic_") + "c2_1280x256.json")
+        ROOT, "profiles", "k_tl_traffic.json" if twolevel and not learned else "k_bicg_traffic.json")
     try:
         with open(tjson) as f:
             tj = json.load(f)
-        traffic = tj.get(args.case, {}).get(dom)
+        per_case = tj.get(args.case, {})
+        # the smoothing kernels' traffic depends on the tile shape the run uses
+        traffic = per_case.get(f"{dom}@{tile[0]}x{tile[1]}x{tile[2]}" if tile else dom, per_case.get(dom))
     except (OSError, ValueError):
         pass
     log(f"bench: kernel us {kern} (dispatch events {kern_ev}), dominant {dom}: {achieved:.0f} GB/s")
