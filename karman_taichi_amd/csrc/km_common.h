@@ -428,8 +428,12 @@ inline int run_chunks_flag(km_ctx* ctx, GraphKey key, bool use_graph, int big, i
             t_seen = std::chrono::steady_clock::now();
         } else if (++spins % 4096 == 0) {
             const hipError_t q = hipStreamQuery(ctx->stream);
-            if (q == hipSuccess && !all_done()) {
-                // the queue drained without a reported stop: read the state itself
+            if (q == hipSuccess) {
+                // `reached` was read before the query: the device may have run
+                // every queued iteration since and be waiting for the next chunk
+                std::atomic_thread_fence(std::memory_order_acquire);
+                if (all_done() || slowest() != reached) continue;
+                // the queue drained with no progress and no reported stop
                 break;
             }
             if (q != hipSuccess && q != hipErrorNotReady)
