@@ -19,6 +19,7 @@ Prints ONE JSON line (rank 0).  Extra keys:
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import math
 import os
@@ -96,6 +97,9 @@ def parse():
     ap.add_argument("--case-workers", type=int, default=3,
                     help="c3 with the two-level mode: step the cases as independent SIMPLE steps, this many at a "
                          "time on their own contexts and streams (0: km_batch_step)")
+    ap.add_argument("--case-precision", choices=["per-case", "f64", "f32"], default="per-case",
+                    help="c3 with case workers: each case's own precision (half the cases FP32 Krylov vectors), or "
+                         "every case FP64 / FP32 vectors")
     ap.add_argument("--spinup-time-steps", type=int, default=100,
                     help="time steps the flow is advanced from the seeded start before the warm-up (untimed): the "
                          "timed window starts at this time step")
@@ -377,8 +381,9 @@ def learned_compare(s, weights, reps):
     cores), FP32 residual-space vectors, FP64 solution-space vectors,
     scalars and true residual, plain BiCGSTAB as the fallback when a restart
     cycle stalls -- beside the same solve without the network (coarse
-    correction + Jacobi step) and the default two-level flexible BiCGSTAB
-    (red-black SOR sweeps).  Median wall ms (each solve returns after a
+    correction + Jacobi step), the default two-level flexible BiCGSTAB
+    (red-black SOR sweeps) and flexible BiCGSTAB / GCR with one multigrid
+    V-cycle as the preconditioner.  Median wall ms (each solve returns after a
     stream synchronisation), iterations, true residuals, fallbacks."""
     import torch
     from karman_taichi_amd import learned as kl
@@ -392,6 +397,8 @@ def learned_compare(s, weights, reps):
         "no_network_fp32": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, mixed=True, coarse=True,
                                                 gcr=True, jacobi=True),
         "twolevel_sor": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, coarse=True),
+        "multigrid_fbicgstab": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, mg=True),
+        "multigrid_gcr": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, mg=True, gcr=True),
     }
     out = {"weights": os.path.relpath(weights, ROOT)}
     for name, fn in runs.items():
@@ -424,6 +431,8 @@ def bench_batch(args):
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     case_list = cases.batched_cases(96, seed=0)
+    if args.case_precision != "per-case":
+        case_list = [dataclasses.replace(c, mixed=args.case_precision == "f32") for c in case_list]
     b = BatchedSIMPLE(case_list, device=dev)
     b.init()
     coarse = MODES[args.mode][2]
@@ -445,16 +454,19 @@ def bench_batch(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     p_it = [inf[2].iterations for rec in recs for inf in rec]
+    n_fp32 = sum(bool(c.mixed) for c in case_list) if workers else 0
     cell_its = sum(c.shape_p[0] * c.shape_p[1] * inf[2].iterations for rec in recs for c, inf in zip(case_list, rec))
     out = {"metric": "batched SIMPLE outer steps/s (every case advanced one outer iteration)",
            "value": round(args.steps / wall, 4), "unit": "batch steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded channel start states)",
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if n_fp32 else "f64",
+           "data": "synthetic (seeded channel start states)",
            "config": {"workload": f"96 uneven channels nx 256..2048, ny nx/5, D ny/8, Re 80..300, "
                                   f"{b.cells()} p' cells in total", "tile": None if workers else list(b.tile),
                       "p_solver": "two-level flexible BiCGSTAB per case" if coarse else "batched BiCGSTAB",
                       "momentum": "fbicgstab+rbgs per case (u, v together)" if smooth else "batched BiCGSTAB",
-                      "step": f"km_simple_step per case, {workers} at a time" if workers else "km_batch_step"},
+                      "step": f"km_simple_step per case, {workers} at a time" if workers else "km_batch_step",
+                      "fp32_vector_cases": n_fp32},
            "p_iters_min_median_max": [min(p_it), sorted(p_it)[len(p_it) // 2], max(p_it)],
            "p_cells_iters_per_s": round(cell_its / wall, 1),
            "case_steps_per_s": round(args.steps * b.n / wall, 3)}

@@ -61,6 +61,10 @@ extern "C" {
                                * after the coarse correction instead of the red-black SOR sweeps; with a network, one
                                * direction z1 + net(r1) followed by a Jacobi step instead of the two directions
                                * z1 + D^-1 r1 and z1 + net(r1) */
+#define KM_FLAG_MG 256       /* km_fbicgstab: the preconditioner is one multigrid V-cycle (km_mg_vcycle) in place of
+                               * Jacobi; FP64 vectors, no coarse level, smoothing or network.  A comparator for the
+                               * two-level and learned preconditioners in the spirit of benchmark/mgpcg.py:91-135
+                               * (the reference's V-cycle is for a constant-coefficient Poisson operator) */
 #define KM_FLAG_TRUE 32       /* km_bicgstab: after the reference's stop on the recursive residual, check the true
                                * residual ||b - A x|| / ||b|| and restart from x (r = b - A x) while it is above eps
                                * and each restart at least halves it (at most 8 restarts; no counterpart in the
@@ -160,6 +164,18 @@ int km_bicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
 #define KM_NOALLOC_MAX_CELLS 521144
 int km_bicgstab_noalloc(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x,
                         double eps, int max_iter, int flags, km_solve_info* info);
+/* z = M(r): one multigrid V-cycle from z = 0 on the hierarchy of the NX x NY
+ * system coef (built on the call; levels, if not null, receives its depth).
+ * Levels: 2 x 2 aggregates of interior cells with the Galerkin operator
+ * P^T A P, down to at most 400 interior cells; per level one red-black SOR
+ * sweep (red first), the residual summed into the aggregates, the next level,
+ * the correction prolonged by injection with weight 1.5, one sweep black
+ * first; 50 sweeps on the coarsest level.  The V-cycle of
+ * benchmark/mgpcg.py:112-135 (red-black smoothing :118-123, summed
+ * restriction :103-110, injection :112-115, 50 bottom sweeps :127-128),
+ * restated for the variable-coefficient p' operator; the preconditioner
+ * KM_FLAG_MG applies.  Synchronises the context's stream. */
+int km_mg_vcycle(km_ctx* ctx, int NX, int NY, const double* coef, const double* r, double* z, int* levels);
 /* CGSolver.solve (cgsolver.py:83-114). */
 int km_cg(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x,
           double eps, int max_iter, int flags, km_solve_info* info);

@@ -17,7 +17,7 @@ KM_EINVAL, KM_EHIP, KM_ENOMEM = -1, -2, -3
 KM_F64, KM_F32 = 0, 1
 NOALLOC_MAX_CELLS = 521144    # KM_NOALLOC_MAX_CELLS: largest system km_bicgstab_noalloc takes
 KM_FLAG_X0, KM_FLAG_MIXED, KM_FLAG_NO_GRAPH, KM_FLAG_COARSE, KM_FLAG_SMOOTH, KM_FLAG_TRUE = 1, 2, 4, 8, 16, 32
-KM_FLAG_GCR, KM_FLAG_JACOBI = 64, 128
+KM_FLAG_GCR, KM_FLAG_JACOBI, KM_FLAG_MG = 64, 128, 256
 KM_EAST_WALL, KM_EAST_OUTFLOW = 0, 1
 KM_KERNEL_BICG_A, KM_KERNEL_BICG_B, KM_KERNEL_BICG_R, KM_KERNEL_STENCIL = 0, 1, 2, 3
 KM_KERNEL_TL_PV, KM_KERNEL_TL_CV, KM_KERNEL_TL_ST, KM_KERNEL_TL_CT = 4, 5, 6, 7
@@ -92,6 +92,7 @@ SIGNATURES = {
     "km_unet_loss_grad": [_CTX, _P, _I, _I, _I, _P, _P, _P, _P, ctypes.POINTER(_D)],
     "km_adam_step": [_CTX, ctypes.c_int64, _P, _P, _P, _P, _I, _D, _D, _D, _D],
     "km_fbicgstab": [_CTX, _I, _I, _P, _P, _P, _D, _I, _I, _P, _INFO],
+    "km_mg_vcycle": [_CTX, _I, _I, _P, _P, _P, _P],
     "km_fbicgstab2": [_CTX, _I, _I, _P, _P, _P, _I, _I, _P, _P, _P, _D, _I, _I, _INFO, _INFO],
     "km_batch_step": [_CTX, _I, _P, _D, _D, _I, _I, _P],
     "km_simple_step": [_CTX, _P, _D, _D, _I, _I, _P],

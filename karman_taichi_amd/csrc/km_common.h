@@ -119,6 +119,7 @@ struct GraphKey {
 struct km_net;   // learned preconditioner buffers (km_unet.hip)
 struct km_batch; // batched cases (km_batch.hip)
 struct km_coarse; // coarse level of the two-level preconditioner (km_coarse.hip)
+struct km_mg;     // multigrid hierarchy of the V-cycle preconditioner (km_mg.hip)
 
 struct km_ctx {
     int device = 0;
@@ -149,6 +150,7 @@ struct km_ctx {
     uint64_t coarse_tick = 0;
     void* blas = nullptr;          // rocBLAS handle (the coarse levels' Newton-Schulz GEMM)
     void* scratch = nullptr;       // device address of this device's no-allocation scratch array
+    km_mg* mg = nullptr;           // V-cycle hierarchy (KM_FLAG_MG)
 };
 
 void km_batch_free(km_ctx* ctx);

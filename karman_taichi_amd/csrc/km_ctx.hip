@@ -2,6 +2,7 @@
 #include "km_common.h"
 #include "km_unet.h"
 #include "km_coarse.h"
+#include "km_mg.h"
 
 #include <cstdlib>
 #include <cstring>
@@ -152,6 +153,7 @@ int km_destroy(km_ctx* ctx) {
     km_net_free(ctx);
     km_batch_free(ctx);
     km_coarse_free(ctx);
+    km_mg_free(ctx);
     KmHostLock lk(km_host_mutex());
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->h_st) (void)hipHostFree(ctx->h_st);

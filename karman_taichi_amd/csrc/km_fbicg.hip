@@ -22,6 +22,7 @@
 //   K_x  omega = <t, s> / <t, t>; x += alpha p^ + omega s^; r = s - omega t; partials <r~, r>, <r, r>
 #include "km_unet.h"
 #include "km_coarse.h"
+#include "km_mg.h"
 
 #include <cmath>
 
@@ -307,7 +308,8 @@ const int kFbChunk = 8;
 // level (and KM_FLAG_SMOOTH) run as the fused iterations of km_twolevel.hip.
 template <typename T>
 int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, const double* coef64, const T* rhs, T* xd,
-             T* ws, double eps, int max_iter, bool use_graph, const float* params, bool coarse, KState* out) {
+             T* ws, double eps, int max_iter, bool use_graph, const float* params, bool coarse, KState* out,
+             bool mg = false) {
     const int64_t n = (int64_t)NX * NY;
     T* r = ws;
     T* rt = ws + n;
@@ -332,6 +334,9 @@ int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, cons
             (void)kmnet::apply_pre<T>(ctx, NX, NY, c32, r1, dst, skip, q, params, P + FB_PR, fl.G, true);
         } else if (params) {
             (void)kmnet::apply_pre<T>(ctx, NX, NY, c32, src, dst, skip, q, params, norm_partials, fl.G, false);
+        } else if (mg) {
+            // FP64 vectors only (fbicg_solve rejects KM_FLAG_MG with KM_FLAG_MIXED)
+            (void)kmmg::vcycle(ctx, reinterpret_cast<const double*>(src), reinterpret_cast<double*>(dst), q, skip);
         } else {
             hipLaunchKernelGGL(k_jacobi<T>, dim3(jb), dim3(KM_BLOCK), 0, q, n, coefT, src, dst, skip);
         }
@@ -353,7 +358,8 @@ int fb_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const float* c32, cons
     GraphKey key;
     key.v = {7, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)c32, (uintptr_t)rhs,
              (uintptr_t)xd, (uintptr_t)ws, (uintptr_t)params, (uintptr_t)ctx->net, (uintptr_t)st,
-             (uintptr_t)coarse, (uintptr_t)ctx->coarse};
+             (uintptr_t)coarse, (uintptr_t)ctx->coarse, (uintptr_t)mg,
+             (uintptr_t)(mg && ctx->mg ? kmmg::arena(ctx) : nullptr)};
     return run_chunks(ctx, st, key, use_graph, kFbChunk, max_iter, enqueue, out);
 }
 
@@ -413,7 +419,12 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     }
     const bool coarse = (flags & KM_FLAG_COARSE) != 0;
     const bool smooth = (flags & KM_FLAG_SMOOTH) != 0;
+    const bool mg = (flags & KM_FLAG_MG) != 0;
     KM_REQUIRE(ctx, !(smooth && (coarse || params)), "KM_FLAG_SMOOTH takes neither a coarse level nor a network");
+    KM_REQUIRE(ctx, !(mg && (coarse || smooth || params || !f64)),
+               "KM_FLAG_MG takes FP64 vectors and no other preconditioner");
+    if (mg)
+        if (int rc = kmmg::setup(ctx, NX, NY, coef, ctx->stream)) return rc;
     if (params) {
         if (int rc = kmnet::prepare(ctx, 1, NX, NY, params)) return rc;
         if (int rc = kmnet::solve_stats(ctx, NX, NY, c32, ctx->stream)) return rc;
@@ -555,7 +566,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
             : coarse && !params
                 ? kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st)
                 : fb_inner<T>(ctx, NX, NY, coefT, c32, coef, rhs, d, wT, inner_eps, cap, use_graph, params, coarse,
-                              &st);
+                              &st, mg);
         if (rc) return rc;
         total += st.iter;
         sweeps++;
@@ -776,7 +787,7 @@ k_fg_step(Flat fl, const double* __restrict__ s, const T* __restrict__ q, const 
     if (threadIdx.x == 0) P[blockIdx.x] = a;
 }
 
-enum FgPrecond { FG_NET, FG_TWOLEVEL, FG_COARSE_JACOBI, FG_JACOBI };
+enum FgPrecond { FG_NET, FG_TWOLEVEL, FG_COARSE_JACOBI, FG_JACOBI, FG_MG };
 
 template <typename T>
 int fgcr_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
@@ -785,7 +796,10 @@ int fgcr_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b,
     const bool f64 = sizeof(T) == sizeof(double);
     const bool coarse = (flags & KM_FLAG_COARSE) != 0;
     KM_REQUIRE(ctx, !(flags & KM_FLAG_SMOOTH), "KM_FLAG_GCR takes no KM_FLAG_SMOOTH");
+    const bool mg = (flags & KM_FLAG_MG) != 0;
+    KM_REQUIRE(ctx, !(mg && (coarse || params || !f64)), "KM_FLAG_MG takes FP64 vectors and no other preconditioner");
     const FgPrecond kind = params ? FG_NET
+                           : mg   ? FG_MG
                                   : (coarse ? ((flags & KM_FLAG_JACOBI) ? FG_COARSE_JACOBI : FG_TWOLEVEL) : FG_JACOBI);
     // r64 | zd, z2d, Z[kFgMax] (FP64) | c32 | rhs, z, q, r1, z2 | Q[kFgMax] (T)
     const size_t bytes = (size_t)(3 + kFgMax) * n * sizeof(double) + (size_t)5 * n * sizeof(float) + 512 +
@@ -815,6 +829,8 @@ int fgcr_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b,
     }
     if (coarse)
         if (int rc = kmcoarse::setup(ctx, NX, NY, coef, ctx->stream, true)) return rc;
+    if (mg)
+        if (int rc = kmmg::setup(ctx, NX, NY, coef, ctx->stream)) return rc;
     double bb = 0.0;
     if (int rc = km_dot(ctx, n, KM_F64, b, b, &bb)) return rc < 0 ? rc : KM_EHIP;
     if (info) {
@@ -891,6 +907,9 @@ int fgcr_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b,
             if (int rc = kmcoarse::restrict_solve<T>(ctx, rhs, nullptr, ctx->stream)) return rc;
             if (int rc = kmcoarse::smooth<T>(ctx, coefT, rhs, z, ctx->stream)) return rc;
             hipLaunchKernelGGL(k_fg_widen<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, (const T*)z, zd);
+        } else if (kind == FG_MG) {
+            // FP64 vectors (T = double): one V-cycle straight into the direction
+            if (int rc = kmmg::vcycle(ctx, reinterpret_cast<const double*>(rhs), zd, ctx->stream, nullptr)) return rc;
         } else if (kind == FG_COARSE_JACOBI) {
             if (int rc = kmcoarse::apply<T>(ctx, coef, coefT, rhs, z, r1, nullptr, nullptr, ctx->stream)) return rc;
             hipLaunchKernelGGL(k_fg_resid1<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, NX, NY, coef,

@@ -135,7 +135,7 @@ class UNetPreconditioner:
 
 def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: Optional[UNetPreconditioner], eps=1e-8,
               max_iter=None, x0=False, graph=True, mixed=False, coarse=False, smooth=False, gcr=False,
-              jacobi=False):
+              jacobi=False, mg=False):
     """Flexible BiCGSTAB with the network as right preconditioner
     (km_fbicgstab); ``net=None`` uses the Jacobi preconditioner instead.
     ``mixed``: FP32 Krylov vectors (FP64 scalars and refinement).
@@ -145,7 +145,9 @@ def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: Optiona
     level; the momentum systems).  ``gcr``: flexible GCR as the outer method
     (KM_FLAG_GCR), which a non-linear preconditioner cannot break;
     ``jacobi`` (with ``gcr`` and ``coarse``, no network): one Jacobi step
-    after the coarse correction instead of the SOR sweeps."""
+    after the coarse correction instead of the SOR sweeps.  ``mg`` (no
+    network, FP64): one multigrid V-cycle as the preconditioner
+    (KM_FLAG_MG)."""
     from .solvers import SolveInfo
     NX, NY = b.shape
     info = KmSolveInfo()
@@ -153,6 +155,7 @@ def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: Optiona
     flags |= (_lib.KM_FLAG_MIXED if mixed else 0) | (_lib.KM_FLAG_COARSE if coarse else 0)
     flags |= _lib.KM_FLAG_SMOOTH if smooth else 0
     flags |= (_lib.KM_FLAG_GCR if gcr else 0) | (_lib.KM_FLAG_JACOBI if jacobi else 0)
+    flags |= _lib.KM_FLAG_MG if mg else 0
     ctx = _lib.context_for(b.device) if net is None else net.ctx
     ctx.set_stream(torch.cuda.current_stream(b.device).cuda_stream)
     ctx.call("km_fbicgstab", NX, NY, ptr(coef), ptr(b), ptr(x), float(eps), int(max_iter or 0), flags,
