@@ -267,16 +267,25 @@ __device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, doub
 
 // K_PV (MODE 1) / K_ST (MODE 2) / the standalone smoother (MODE 0).  The
 // coefficients and the frame input of the tile's NS-cell halo frame stay in
-// registers (Q cells per thread, NS = 2 SW half-sweeps); z lives in LDS
-// on the (NS + 1)-cell halo frame and is updated in place: a red half-sweep
-// changes only red cells ((i + j) even) and reads only black ones, and the
-// other way round.  The arithmetic per cell is z + (omega / aP) (src - A z),
-// neighbours outside the system read as zero, rows with aP = 0 left
-// unchanged.  Half-sweep s covers the halo-(NS - s) frame, so after the last
-// one z is exact one cell into the halo, which covers the boundary cells the
-// edge tiles own.  Without a coarse vector (yc null) z starts from zero;
-// without slots (rcs null) nothing is restricted.  SELF (MODE 2): the
-// launch derives alpha and the stopping test itself (cv_scalars).
+// registers (NS = 2 SW half-sweeps); z lives in LDS on the (NS + 1)-cell halo
+// frame and is updated in place: a red half-sweep changes only red cells
+// ((i + j) even) and reads only black ones, and the other way round.  The
+// arithmetic per cell is z + (omega / aP) (src - A z), neighbours outside the
+// system read as zero, rows with aP = 0 left unchanged.  Half-sweep s covers
+// the halo-(NS - s) frame, so after the last one z is exact one cell into the
+// halo, which covers the boundary cells the edge tiles own.  Without a coarse
+// vector (yc null) z starts from zero; without slots (rcs null) nothing is
+// restricted.  SELF (MODE 2): the launch derives alpha and the stopping test
+// itself (cv_scalars).
+//
+// Each thread holds pairs of neighbouring cells along j, (a, b0) and
+// (a, b0 + 1) with b0 even: one red and one black cell, kept in registers by
+// colour, so every lane has a cell to update in every half-sweep.  Pair slot
+// q holds pair t + q NT of the frame, the tile's own block first (NPI
+// pairs), then the ring of halo cells (the top and bottom bands, then the
+// side bands row by row); the output pass (z, A z, partial sums) covers the
+// slots holding the block, and the others only in the tiles at the system's
+// edges (whose boundary row or column lies in the ring).
 template <typename T, int TI, int TJ, int MODE, int SW, bool SELF, int NT>
 __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, const TlArgs<T>& g, int L) {
     static_assert((NT == 256 || NT == 512) && (!SELF || NT == KM_BLOCK), "4 or 8 waves; SELF sums as KM_BLOCK");
@@ -284,8 +293,12 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
     constexpr int NS = 2 * SW;                                      // half-sweeps = halo
     constexpr int WZ = TJ + 2 * (NS + 1), HZ = TI + 2 * (NS + 1);   // z frame: halo NS + 1
     constexpr int WR = TJ + 2 * NS, HR = TI + 2 * NS;               // register frame: halo NS
-    constexpr int NR = WR * HR;
-    constexpr int Q = (NR + NT - 1) / NT;   // register slots: frame cells t + q NT
+    static_assert(TJ % 2 == 0 && NS % 2 == 0, "cell pairs along j");
+    constexpr int NPI = TI * TJ / 2;          // pairs of the tile's own block
+    constexpr int NP = HR * WR / 2;           // pairs of the register frame
+    constexpr int QI = (NPI + NT - 1) / NT;   // slots holding the block's pairs
+    constexpr int Q = (NP + NT - 1) / NT;     // pair slots
+    constexpr int BAND = NS * (WR / 2);       // pairs of the top (and of the bottom) band
     __shared__ T za[HZ * WZ];
     __shared__ double yb[MODE > 0 ? TI * TJ : 1];   // y on the free interior cells (tile-local)
     __shared__ double sh[4 * NW];
@@ -317,126 +330,198 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
         const int ti0 = 1 + ta * TI, tj0 = 1 + tb * TJ;
         // cells this tile writes: its interior block plus the boundary row or
         // column next to it at the edges of the system
+        const bool edge = ta == 0 || ta == tl.tiles_i - 1 || tb == 0 || tb == tl.tiles_j - 1;
         const int olo = ta == 0 ? 0 : ti0, ohi = ta == tl.tiles_i - 1 ? NX : ti0 + TI;
         const int jlo = tb == 0 ? 0 : tj0, jhi = tb == tl.tiles_j - 1 ? NY : tj0 + TJ;
+        const int par0 = (ti0 + tj0) & 1;   // (i + j) parity of frame cell (0, 0)
 
         // every load of the tile is issued before the first store (the
         // argument pointers may alias as far as the compiler knows): the
-        // register frame, the tile's own cells and the z frame
+        // register frame, the tile's own cells and the z frame.  Index c:
+        // the pair's red (0) and black (1) cell.
         constexpr int NF = MODE == 2 ? 2 : 1;   // frame inputs per cell
         constexpr int QZ = (HZ * WZ + NT - 1) / NT;
-        T cr[Q][5], sr[Q], rq[Q], wd[Q];
-        T fv[Q][NF];
-        int64_t kq[Q];
-        bool in[Q], ou[Q];
-        unsigned ib = 0, fm = 0;   // bit q: interior cell of the tile; free cell
-        unsigned blk = 0;          // bit q: black cell ((i + j) odd)
+        // Every load reads a valid address (cells outside the system read
+        // cell 0) and sits in one basic block with the others; what it
+        // contributes is selected only after all of them are issued, so the
+        // tile's loads are in flight together instead of one round trip per
+        // conditional load.
+        T cr[Q][2][5], sr[Q][2], rq[Q][2], wd[Q][2];
+        T fv[Q][2][NF];
+        uint8_t mk[Q][2];
+        int64_t kq[Q][2];
+        int pa[Q], pb[Q][2];       // frame row of the pair, frame column of each colour's cell
+        uint32_t inb = 0, oub = 0, ib = 0, fm = 0;   // bit 2 q + c: in the system; written; interior; free
         double yq[QZ];
         uint8_t mq[QZ];
+        // the free-cell mask where there are slots to restrict into (ib is
+        // set only then), else the coefficient array (in bounds, unused)
+        const uint8_t* mkp = g.rcs ? g.mask : reinterpret_cast<const uint8_t*>(g.c);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            const int idx = threadIdx.x + q * NT;
-            const int a = idx / WR, b = idx - (idx / WR) * WR;
-            const int i = ti0 - NS + a, j = tj0 - NS + b;
-            in[q] = idx < NR && i >= 0 && i < NX && j >= 0 && j < NY;
-            ou[q] = in[q] && i >= olo && i < ohi && j >= jlo && j < jhi;
-            kq[q] = in[q] ? (int64_t)i * NY + j : 0;
-            const int64_t k = kq[q];
-            if ((i + j) & 1) blk |= 1u << q;
-            if (MODE > 0 && g.rcs && ou[q] && i >= 1 && i <= nx && j >= 1 && j <= ny) {
-                ib |= 1u << q;
-                if (g.mask[k]) fm |= 1u << q;
+            const int P = threadIdx.x + q * NT;
+            int a, b0;
+            if (P < NPI) {
+                a = NS + P / (TJ / 2);
+                b0 = NS + 2 * (P - (P / (TJ / 2)) * (TJ / 2));
+            } else {
+                const int r = P - NPI;
+                if (r < BAND) {
+                    a = r / (WR / 2);
+                    b0 = 2 * (r - a * (WR / 2));
+                } else if (r < 2 * BAND) {
+                    const int u = r - BAND;
+                    a = NS + TI + u / (WR / 2);
+                    b0 = 2 * (u - (u / (WR / 2)) * (WR / 2));
+                } else {
+                    const int u = r - 2 * BAND, e = u - (u / NS) * NS;
+                    a = NS + u / NS;
+                    b0 = e < NS / 2 ? 2 * e : TJ + NS + 2 * (e - NS / 2);
+                }
             }
+            const bool valid = P < NP;
+            const int par = (par0 + a) & 1;   // parity of (a, b0); colour c sits at b0 + (par ^ c)
+            pa[q] = a;
 #pragma unroll
-            for (int e = 0; e < 5; ++e) cr[q][e] = in[q] ? g.c[e * n + k] : T(0);
-            const T* fp[2] = {g.f0, g.f1};
+            for (int c = 0; c < 2; ++c) {
+                const int b = b0 + (par ^ c);
+                pb[q][c] = b;
+                const int i = ti0 - NS + a, j = tj0 - NS + b;
+                const bool inc = valid && i >= 0 && i < NX && j >= 0 && j < NY;
+                const bool ouc = inc && i >= olo && i < ohi && j >= jlo && j < jhi;
+                const int64_t k = inc ? (int64_t)i * NY + j : 0;
+                kq[q][c] = k;
+                const uint32_t bit = 1u << (2 * q + c);
+                if (inc) inb |= bit;
+                if (ouc) oub |= bit;
+                if (MODE > 0 && g.rcs && ouc && i >= 1 && i <= nx && j >= 1 && j <= ny) ib |= bit;
 #pragma unroll
-            for (int e = 0; e < NF; ++e) fv[q][e] = in[q] ? fp[e][k] : T(0);
-            rq[q] = MODE > 0 && ou[q] ? g.rt[k] : T(0);
+                for (int e = 0; e < 5; ++e) cr[q][c][e] = g.c[e * n + k];
+                const T* fp[2] = {g.f0, g.f1};
+#pragma unroll
+                for (int e = 0; e < NF; ++e) fv[q][c][e] = fp[e][k];
+                if (MODE > 0) rq[q][c] = g.rt[k];
+                if (MODE > 0) mk[q][c] = mkp[k];
+            }
         }
-        // z1 = P yc on the z frame (zero off the free cells and outside the system)
+        // z1 = P yc on the z frame (zero off the free cells and outside the
+        // system).  Without a coarse vector the loads read the coefficient
+        // array instead (in bounds: index < NY doubles, < NX NY bytes) and
+        // nothing they read is used.
+        uint32_t zin = 0;   // bit q: z-frame slot q lies in the system
+        {
+            const double* ycp = g.yc ? g.yc : reinterpret_cast<const double*>(g.c);
+            const uint8_t* mzp = g.yc ? g.mask : reinterpret_cast<const uint8_t*>(g.c);
+            const int B = g.yc ? cg.B : 1, mJ = g.yc ? cg.mJ : 0;
 #pragma unroll
-        for (int q = 0; q < QZ; ++q) {
-            const int idx = threadIdx.x + q * NT;
-            const int a = idx / WZ, b = idx - (idx / WZ) * WZ;
-            const int i = ti0 - NS - 1 + a, j = tj0 - NS - 1 + b;
-            const bool ing = idx < HZ * WZ && i >= 0 && i < NX && j >= 0 && j < NY;
-            const int ic = i < 1 ? 0 : (i > nx ? nx - 1 : i - 1), jc = j < 1 ? 0 : (j > ny ? ny - 1 : j - 1);
-            yq[q] = g.yc ? g.yc[(ic / cg.B) * cg.mJ + jc / cg.B] : 0.0;
-            mq[q] = ing && g.yc ? g.mask[(int64_t)i * NY + j] : 0;
+            for (int q = 0; q < QZ; ++q) {
+                const int idx = threadIdx.x + q * NT;
+                const int a = idx / WZ, b = idx - (idx / WZ) * WZ;
+                const int i = ti0 - NS - 1 + a, j = tj0 - NS - 1 + b;
+                const bool ing = idx < HZ * WZ && i >= 0 && i < NX && j >= 0 && j < NY;
+                const int ic = i < 1 ? 0 : (i > nx ? nx - 1 : i - 1), jc = j < 1 ? 0 : (j > ny ? ny - 1 : j - 1);
+                if (ing) zin |= 1u << q;
+                yq[q] = ycp[(ic / B) * mJ + jc / B];
+                mq[q] = mzp[ing ? (int64_t)i * NY + j : 0];
+            }
         }
+        // what the loads contribute, now that all of them are issued
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const uint32_t bit = 1u << (2 * q + c);
+                const bool inc = inb & bit;
+#pragma unroll
+                for (int e = 0; e < 5; ++e) cr[q][c][e] = inc ? cr[q][c][e] : T(0);
+#pragma unroll
+                for (int e = 0; e < NF; ++e) fv[q][c][e] = inc ? fv[q][c][e] : T(0);
+                rq[q][c] = MODE > 0 && (oub & bit) ? rq[q][c] : T(0);
+                if (MODE > 0 && (ib & bit) && mk[q][c]) fm |= bit;
+            }
         // the source of M(.) on the frame (MODE 2: s, written on the tile's cells)
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            T sv = fv[q][0];
-            if (MODE == 2) {
-                sv = in[q] ? fv[q][0] - fa * fv[q][1] : T(0);        // bicgsolver.py:110
-                if (ou[q]) g.src[kq[q]] = sv;
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const bool inc = (inb >> (2 * q + c)) & 1u;
+                T sv = fv[q][c][0];
+                if (MODE == 2) {
+                    sv = inc ? fv[q][c][0] - fa * fv[q][c][1] : T(0);        // bicgsolver.py:110
+                    if ((oub >> (2 * q + c)) & 1u) g.src[kq[q][c]] = sv;
+                }
+                sr[q][c] = sv;
+                wd[q][c] = cr[q][c][0] != T(0) ? om * (T(1) / cr[q][c][0]) : T(0);   // omega D^-1
             }
-            sr[q] = sv;
-            wd[q] = cr[q][0] != T(0) ? om * (T(1) / cr[q][0]) : T(0);   // omega D^-1
-        }
 #pragma unroll
         for (int q = 0; q < QZ; ++q) {
             const int idx = threadIdx.x + q * NT;
-            if (idx < HZ * WZ) za[idx] = mq[q] ? (T)yq[q] : T(0);
+            if (idx < HZ * WZ) za[idx] = g.yc && ((zin >> q) & 1u) && mq[q] ? (T)yq[q] : T(0);
         }
         __syncthreads();
         // half-sweep hs: red cells at even hs, black at odd, on the halo-h frame
 #pragma unroll
         for (int hs = 0; hs < NS; ++hs) {
-            const int h = NS - hs;
+            const int h = NS - hs, c = hs & 1;
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                const int idx = threadIdx.x + q * NT;
-                const int a = idx / WR, b = idx - (idx / WR) * WR;
-                if (idx >= NR || !in[q] || (int)((blk >> q) & 1u) != (hs & 1) || a < NS - h || a >= TI + NS + h ||
-                    b < NS - h || b >= TJ + NS + h)
+                const int a = pa[q], b = pb[q][c];
+                if (!((inb >> (2 * q + c)) & 1u) || a < NS - h || a >= TI + NS + h || b < NS - h ||
+                    b >= TJ + NS + h)
                     continue;
                 const int pz = (a + 1) * WZ + (b + 1);
                 const T zc = za[pz];
-                T az = cr[q][0] * zc;
-                az = az + cr[q][1] * za[pz - WZ];
-                az = az + cr[q][2] * za[pz + WZ];
-                az = az + cr[q][3] * za[pz + 1];
-                az = az + cr[q][4] * za[pz - 1];
-                za[pz] = zc + wd[q] * (sr[q] - az);
+                T az = cr[q][c][0] * zc;
+                az = az + cr[q][c][1] * za[pz - WZ];
+                az = az + cr[q][c][2] * za[pz + WZ];
+                az = az + cr[q][c][3] * za[pz + 1];
+                az = az + cr[q][c][4] * za[pz - 1];
+                za[pz] = zc + wd[q][c] * (sr[q][c] - az);
             }
             __syncthreads();
         }
         const T* __restrict__ zf = za;
         // the tile's cells: z out, then y = A z (and y on the free interior
         // cells into yb for the restriction)
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            if (!ou[q]) continue;
-            const int idx = threadIdx.x + q * NT;
-            const int a = idx / WR, b = idx - (idx / WR) * WR;
+        auto emit = [&](int q, int c) {
+            const int a = pa[q], b = pb[q][c];
             const int pz = (a + 1) * WZ + (b + 1);
-            const int64_t k = kq[q];
+            const int64_t k = kq[q][c];
             const T zc = zf[pz];
-            if (MODE == 0) {
-                g.z[k] = zc;
-            } else {
-                g.z[k] = zc;
-                T yv = cr[q][0] * zc;
-                yv = yv + cr[q][1] * zf[pz - WZ];
-                yv = yv + cr[q][2] * zf[pz + WZ];
-                yv = yv + cr[q][3] * zf[pz + 1];
-                yv = yv + cr[q][4] * zf[pz - 1];
+            g.z[k] = zc;
+            if (MODE > 0) {
+                T yv = cr[q][c][0] * zc;
+                yv = yv + cr[q][c][1] * zf[pz - WZ];
+                yv = yv + cr[q][c][2] * zf[pz + WZ];
+                yv = yv + cr[q][c][3] * zf[pz + 1];
+                yv = yv + cr[q][c][4] * zf[pz - 1];
                 g.y[k] = yv;
-                if ((ib >> q) & 1u) yb[(a - NS) * TJ + (b - NS)] = ((fm >> q) & 1u) ? (double)yv : 0.0;
-                const double rtk = (double)rq[q];
+                const uint32_t bit = 1u << (2 * q + c);
+                if (ib & bit) yb[(a - NS) * TJ + (b - NS)] = (fm & bit) ? (double)yv : 0.0;
+                const double rtk = (double)rq[q][c];
                 if (MODE == 1) {
                     a0 += rtk * (double)yv;
                 } else {
-                    const double sk = (double)sr[q];
+                    const double sk = (double)sr[q][c];
                     a0 += (double)yv * sk;
                     a1 += (double)yv * (double)yv;
                     a2 += sk * rtk;
                     a3 += (double)yv * rtk;
                 }
             }
+        };
+#pragma unroll
+        for (int q = 0; q < QI; ++q)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                if ((oub >> (2 * q + c)) & 1u) emit(q, c);
+        if (edge) {
+            // the boundary row or column of the system next to an edge tile
+#pragma unroll
+            for (int q = QI; q < Q; ++q)
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+                    if ((oub >> (2 * q + c)) & 1u) emit(q, c);
         }
         if (MODE > 0 && g.rcs) {
             // P^T y over the aggregates this tile overlaps: one wave per
