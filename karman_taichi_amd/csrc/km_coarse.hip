@@ -593,6 +593,9 @@ int setup(km_ctx* ctx, int NX, int NY, const double* coef, hipStream_t s, bool s
     km_coarse* C = solver ? level_for(ctx, NX, NY, coef) : get_coarse(ctx);
     const int nx = NX - 2, ny = NY - 2;
     KM_REQUIRE(ctx, nx >= 1 && ny >= 1, "system too small for a coarse level");
+    // the smoothing tiles address vectors with 32-bit byte offsets
+    KM_REQUIRE(ctx, (int64_t)NX * NY < ((int64_t)1 << 29), "system of %lld cells is too large for the tile kernels",
+               (long long)NX * NY);
     const int B = choose_block(nx, ny, C->block_req);
     const int mI = (nx + B - 1) / B, mJ = (ny + B - 1) / B, m = mI * mJ;
     KM_REQUIRE(ctx, m <= 8192, "coarse level of %d unknowns is too large (aggregate %d)", m, B);

@@ -53,6 +53,7 @@ static_assert(kPairPartials >= TL_TOT + 8 && 2 * kPairPartials <= KM_MAX_PARTIAL
 constexpr int64_t kTotalWork = 2000000;
 constexpr int FIN_A = 1, FIN_T = 2, FIN_R = 4;
 constexpr int kStallWindow = 300;        // as the unfused flexible solve
+constexpr int64_t kTileCellsMax = (int64_t)1 << 29;   // the tile kernels' 32-bit byte offsets
 constexpr int kFlatPer = 8;              // cells per thread of a flat workgroup, loads in flight
 // Tile shapes of the smoothing launches (rows x columns, columns contiguous);
 // the fastest for a system is chosen on its first solve.
@@ -67,6 +68,24 @@ constexpr int kTileW[kNumTiles] = {1, 1, 3, 1, 1, 4, 4, 2, 2};
 constexpr int kTileT[kNumTiles] = {256, 256, 256, 256, 256, 256, 512, 512, 512};
 
 __device__ __forceinline__ bool fin(double v) { return v == v && v - v == 0.0; }
+
+// 1 / d for finite non-zero d: the hardware reciprocal estimate refined by two
+// Newton steps x' = x + x (1 - d x) (each squares the relative error; the
+// result is within an ulp of the rounded quotient)
+__device__ __forceinline__ double recip(double d) {
+    double x = __builtin_amdgcn_rcp(d);
+    x = __builtin_fma(x, __builtin_fma(-d, x, 1.0), x);
+    x = __builtin_fma(x, __builtin_fma(-d, x, 1.0), x);
+    return x;
+}
+__device__ __forceinline__ float recip(float d) { return 1.0f / d; }
+
+// base[byte offset]: a uniform base and a 32-bit per-lane offset (the
+// scalar-base addressing form of the vector loads)
+template <typename T>
+__device__ __forceinline__ T ld_at(const T* base, uint32_t off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
 
 __device__ __forceinline__ void finish_state(KState* st, int status, int iter, double rr) {
     st->status = status;
@@ -288,6 +307,10 @@ __device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, doub
 // edges (whose boundary row or column lies in the ring).
 template <typename T, int TI, int TJ, int MODE, int SW, bool SELF, int NT>
 __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, const TlArgs<T>& g, int L) {
+    // The smoother and the matvec are issue-bound: fused multiply-adds (one
+    // rounding instead of two per term) and the reciprocal of aP from the
+    // hardware estimate with two Newton steps instead of a full division.
+#pragma clang fp contract(fast)
     static_assert((NT == 256 || NT == 512) && (!SELF || NT == KM_BLOCK), "4 or 8 waves; SELF sums as KM_BLOCK");
     constexpr int NW = NT / 64;                                     // waves per workgroup
     constexpr int NS = 2 * SW;                                      // half-sweeps = halo
@@ -395,13 +418,17 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
                 if (inc) inb |= bit;
                 if (ouc) oub |= bit;
                 if (MODE > 0 && g.rcs && ouc && i >= 1 && i <= nx && j >= 1 && j <= ny) ib |= bit;
+                // 32-bit byte offsets from uniform bases (systems below 2^29
+                // cells, kTileCellsMax: checked by the coarse set-up and the
+                // smoothed solves)
+                const uint32_t ko = (uint32_t)k * (uint32_t)sizeof(T);
 #pragma unroll
-                for (int e = 0; e < 5; ++e) cr[q][c][e] = g.c[e * n + k];
+                for (int e = 0; e < 5; ++e) cr[q][c][e] = ld_at(g.c + e * n, ko);
                 const T* fp[2] = {g.f0, g.f1};
 #pragma unroll
-                for (int e = 0; e < NF; ++e) fv[q][c][e] = fp[e][k];
-                if (MODE > 0) rq[q][c] = g.rt[k];
-                if (MODE > 0) mk[q][c] = mkp[k];
+                for (int e = 0; e < NF; ++e) fv[q][c][e] = ld_at(fp[e], ko);
+                if (MODE > 0) rq[q][c] = ld_at(g.rt, ko);
+                if (MODE > 0) mk[q][c] = mkp[(uint32_t)k];
             }
         }
         // z1 = P yc on the z frame (zero off the free cells and outside the
@@ -425,18 +452,15 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
                 mq[q] = mzp[ing ? (int64_t)i * NY + j : 0];
             }
         }
-        // what the loads contribute, now that all of them are issued
+        // what the loads contribute, now that all of them are issued.  A cell
+        // outside the system read cell 0: its values are never used (the
+        // half-sweeps skip cells without their inb bit, the output pass
+        // covers oub cells, a subset), and its z stays the zero of za.
 #pragma unroll
         for (int q = 0; q < Q; ++q)
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 const uint32_t bit = 1u << (2 * q + c);
-                const bool inc = inb & bit;
-#pragma unroll
-                for (int e = 0; e < 5; ++e) cr[q][c][e] = inc ? cr[q][c][e] : T(0);
-#pragma unroll
-                for (int e = 0; e < NF; ++e) fv[q][c][e] = inc ? fv[q][c][e] : T(0);
-                rq[q][c] = MODE > 0 && (oub & bit) ? rq[q][c] : T(0);
                 if (MODE > 0 && (ib & bit) && mk[q][c]) fm |= bit;
             }
         // the source of M(.) on the frame (MODE 2: s, written on the tile's cells)
@@ -444,14 +468,13 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
         for (int q = 0; q < Q; ++q)
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                const bool inc = (inb >> (2 * q + c)) & 1u;
                 T sv = fv[q][c][0];
                 if (MODE == 2) {
-                    sv = inc ? fv[q][c][0] - fa * fv[q][c][1] : T(0);        // bicgsolver.py:110
+                    sv = fv[q][c][0] - fa * fv[q][c][1];        // bicgsolver.py:110
                     if ((oub >> (2 * q + c)) & 1u) g.src[kq[q][c]] = sv;
                 }
                 sr[q][c] = sv;
-                wd[q][c] = cr[q][c][0] != T(0) ? om * (T(1) / cr[q][c][0]) : T(0);   // omega D^-1
+                wd[q][c] = cr[q][c][0] != T(0) ? om * recip(cr[q][c][0]) : T(0);   // omega D^-1
             }
 #pragma unroll
         for (int q = 0; q < QZ; ++q) {
@@ -1427,6 +1450,8 @@ void pair_iteration(const SmoothFused<T>& a, const SmoothFused<T>& b, hipStream_
 template <typename T>
 int smooth_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
                  bool use_graph, KState* out, const std::function<int()>* tail) {
+    KM_REQUIRE(ctx, (int64_t)NX * NY < kTileCellsMax, "system of %lld cells is too large for the tile kernels",
+               (long long)NX * NY);
     const SmoothFused<T> f(NX, NY, coefT, ws, xd, &ctx->st[4], ctx->partials, ctx->d_flag);
     f.init(ctx->stream, rhs, eps, max_iter);
     KM_CHECK_LAUNCH(ctx);
@@ -1442,6 +1467,8 @@ template <typename T>
 int smooth_inner2(km_ctx* ctx, const int* NX, const int* NY, const T* const* coefT, const T* const* rhs, T* const* xd,
                   T* const* ws, double eps, const int* max_iter, bool use_graph, KState* out,
                   const std::function<int()>* tail) {
+    KM_REQUIRE(ctx, (int64_t)NX[0] * NY[0] < kTileCellsMax && (int64_t)NX[1] * NY[1] < kTileCellsMax,
+               "system too large for the tile kernels");
     // states st[4], st[5] (contiguous: one copy reads both), partials and
     // host words of the second system beyond the first's
     const SmoothFused<T> a(NX[0], NY[0], coefT[0], ws[0], xd[0], &ctx->st[4], ctx->partials, ctx->d_flag);
