@@ -40,7 +40,10 @@ BYTES_PER_CELL = {"bicg_p": 11 * 8, "bicg_s": 9 * 8, "bicg_r": 7 * 8,
                   #   tl_pv reads p, 5 coefficient planes, r~, mask; writes p^, v
                   #   tl_st reads r, v, 5 coefficient planes, r~, mask; writes s, s^, t
                   #   tl_ct (flat workgroups) reads s, t, p, v, x, p^, s^; writes x, r, p
-                  "tl_pv": 9 * 8 + 1, "tl_st": 11 * 8 + 1, "tl_ct": 10 * 8}
+                  "tl_pv": 9 * 8 + 1, "tl_st": 11 * 8 + 1, "tl_ct": 10 * 8,
+                  # the same for a symmetric p' system (KM_FLAG_SYMMETRIC, as the
+                  # step's p' solve runs them): aP, aE, aN planes only
+                  "tl_pv+sym": 7 * 8 + 1, "tl_st+sym": 9 * 8 + 1}
 # the coarse rows of tl_cv / tl_ct read A_c^-1 (m x m) and a few m-vectors
 COARSE_VECTORS = {"tl_cv": 4, "tl_ct": 6}
 
@@ -53,10 +56,12 @@ def mall_note(cells, m):
             f"{'fits' if ws < 256e6 else 'exceeds'} the 256 MB MALL")
 
 
-def alg_bytes(name, cells, m):
-    """Algorithmic bytes of one launch of kernel `name`."""
+def alg_bytes(name, cells, m, sym=False):
+    """Algorithmic bytes of one launch of kernel `name` (sym: the three-plane
+    smoothing kernels of a symmetric system)."""
     coarse = 8 * m * m + 8 * COARSE_VECTORS[name] * m if name in COARSE_VECTORS else 0
-    return coarse + BYTES_PER_CELL.get(name, 0) * cells
+    key = name + "+sym" if sym and name + "+sym" in BYTES_PER_CELL else name
+    return coarse + BYTES_PER_CELL.get(key, 0) * cells
 
 
 # p' solver of each mode: (solver, FP32 vectors, two-level, network)
@@ -235,9 +240,15 @@ def main():
     ctx = s.ctx
     ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
     NX, NY = case.shape_p
+    # the step's p' solve (km_simple_step) runs the smoothing tiles of a
+    # symmetric system (three coefficient planes); the stage-by-stage path
+    # the five-plane ones
+    symk = bool(twolevel and not learned and native)
     if twolevel and not learned:
         kinds = (("tl_pv", _lib.KM_KERNEL_TL_PV), ("tl_cv", _lib.KM_KERNEL_TL_CV), ("tl_st", _lib.KM_KERNEL_TL_ST),
                  ("tl_ct", _lib.KM_KERNEL_TL_CT))
+        if symk:
+            kinds = tuple((nm, kid | _lib.KM_KERNEL_SYMMETRIC) for nm, kid in kinds)
         reps = 100       # well inside the stall window of the flexible iteration
     else:
         kinds = (("bicg_p", _lib.KM_KERNEL_BICG_A), ("bicg_s", _lib.KM_KERNEL_BICG_B),
@@ -266,7 +277,7 @@ def main():
         blk, mc = ctypes.c_int(), ctypes.c_int()
         ctx.call("km_coarse_info", ctypes.byref(blk), ctypes.byref(mc))
         coarse = {"aggregate": blk.value, "unknowns": mc.value}
-    launch_bytes = alg_bytes(dom, cells_p, coarse["unknowns"] if coarse else 0)
+    launch_bytes = alg_bytes(dom, cells_p, coarse["unknowns"] if coarse else 0, sym=symk)
     achieved = launch_bytes / (kern[dom] * 1e-6) / 1e9
     traffic = None
     tjson = args.traffic_json or os.path.join(
@@ -276,7 +287,8 @@ def main():
             tj = json.load(f)
         per_case = tj.get(args.case, {})
         # the smoothing kernels' traffic depends on the tile shape the run uses
-        traffic = per_case.get(f"{dom}@{tile[0]}x{tile[1]}x{tile[2]}" if tile else dom, per_case.get(dom))
+        sfx = "+sym" if symk else ""
+        traffic = per_case.get(f"{dom}{sfx}@{tile[0]}x{tile[1]}x{tile[2]}" if tile else dom + sfx)
     except (OSError, ValueError):
         pass
     log(f"bench: kernel us {kern} (dispatch events {kern_ev}), dominant {dom}: {achieved:.0f} GB/s")
@@ -339,6 +351,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": launch_bytes,
+                     **({"coefficient_planes": 3 if symk else 5} if twolevel and not learned else {}),
                      "note": mall_note(cells_p, coarse["unknowns"] if coarse else 0)},
         "cpu_baseline": cpu,
     }

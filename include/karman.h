@@ -65,6 +65,11 @@ extern "C" {
                                * Jacobi; FP64 vectors, no coarse level, smoothing or network.  A comparator for the
                                * two-level and learned preconditioners in the spirit of benchmark/mgpcg.py:91-135
                                * (the reference's V-cycle is for a constant-coefficient Poisson operator) */
+#define KM_FLAG_SYMMETRIC 512 /* km_fbicgstab with KM_FLAG_COARSE: the caller asserts that the system is symmetric bit
+                               * for bit, aW(i, j) = aE(i - 1, j) and aS(i, j) = aN(i, j - 1), as the assembled p'
+                               * systems are (compute_coef_p, simple.py:143-170); the two-level smoothing kernels then
+                               * read the aP, aE and aN planes only, with results identical to the five-plane kernels.
+                               * Results are undefined for a system that is not (no counterpart in the reference) */
 #define KM_FLAG_TRUE 32       /* km_bicgstab: after the reference's stop on the recursive residual, check the true
                                * residual ||b - A x|| / ||b|| and restart from x (r = b - A x) while it is above eps
                                * and each restart at least halves it (at most 8 restarts; no counterpart in the
@@ -343,6 +348,8 @@ int km_set_coarse_chunks(km_ctx* ctx, int big, int small, int lead);
 #define KM_KERNEL_TL_ST 6    /* s = r - alpha v; s^ = M(s); t = A s^; <t, s>, <t, t>, <s, r~>, <t, r~>; P^T t */
 #define KM_KERNEL_TL_CT 7    /* omega, beta; coarse Y(t), Y(r), Y(p); flat x, r, p updates, <r, r> */
 #define KM_KERNEL_BACK_TO_BACK 0x100   /* or-ed in: time `reps` identical back-to-back launches between two events */
+#define KM_KERNEL_SYMMETRIC 0x200      /* or-ed in with KM_KERNEL_TL_*: the three-plane tile kernels of a symmetric
+                                        * system (KM_FLAG_SYMMETRIC), as the step's p' solve launches them */
 int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps,
                    double* avg_us);
 

@@ -17,11 +17,11 @@ KM_EINVAL, KM_EHIP, KM_ENOMEM = -1, -2, -3
 KM_F64, KM_F32 = 0, 1
 NOALLOC_MAX_CELLS = 521144    # KM_NOALLOC_MAX_CELLS: largest system km_bicgstab_noalloc takes
 KM_FLAG_X0, KM_FLAG_MIXED, KM_FLAG_NO_GRAPH, KM_FLAG_COARSE, KM_FLAG_SMOOTH, KM_FLAG_TRUE = 1, 2, 4, 8, 16, 32
-KM_FLAG_GCR, KM_FLAG_JACOBI, KM_FLAG_MG = 64, 128, 256
+KM_FLAG_GCR, KM_FLAG_JACOBI, KM_FLAG_MG, KM_FLAG_SYMMETRIC = 64, 128, 256, 512
 KM_EAST_WALL, KM_EAST_OUTFLOW = 0, 1
 KM_KERNEL_BICG_A, KM_KERNEL_BICG_B, KM_KERNEL_BICG_R, KM_KERNEL_STENCIL = 0, 1, 2, 3
 KM_KERNEL_TL_PV, KM_KERNEL_TL_CV, KM_KERNEL_TL_ST, KM_KERNEL_TL_CT = 4, 5, 6, 7
-KM_KERNEL_BACK_TO_BACK = 0x100
+KM_KERNEL_BACK_TO_BACK, KM_KERNEL_SYMMETRIC = 0x100, 0x200
 
 STATUS_NAMES = {KM_OK: "ok", KM_BREAKDOWN: "breakdown", KM_MAXITER: "maxiter",
                 KM_NONFINITE: "nonfinite", KM_STALL: "stall"}

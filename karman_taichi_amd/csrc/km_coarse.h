@@ -92,9 +92,11 @@ int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s);
 
 // Inner flexible BiCGSTAB for A d = rhs with the two-level preconditioner
 // (red-black SOR smoothing), four launches per iteration; ws holds 8n values of T.
+// sym: the caller asserts aW(i, j) = aE(i - 1, j) and aS(i, j) = aN(i, j - 1)
+// bit for bit (the smoothing tiles then read three coefficient planes).
 template <typename T>
 int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
-                bool use_graph, KState* out, const std::function<int()>* tail = nullptr);
+                bool use_graph, KState* out, const std::function<int()>* tail = nullptr, bool sym = false);
 
 // Inner flexible BiCGSTAB for A d = rhs with one red-black Gauss-Seidel sweep
 // as the preconditioner and no coarse level (the momentum systems), three
@@ -115,9 +117,10 @@ int smooth_inner2(km_ctx* ctx, const int* NX, const int* NY, const T* const* coe
 int tile_info(km_ctx* ctx, int* ti, int* tj, int* threads);
 int set_tile(km_ctx* ctx, int which);
 
-// km_time_kernel for the fused kernels (which 4..7: K_PV, K_CV, K_ST, K_CT).
+// km_time_kernel for the fused kernels (which 4..7: K_PV, K_CV, K_ST, K_CT;
+// sym: the three-plane smoothing kernels of a symmetric system).
 int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps, bool b2b,
-                double* avg_us);
+                double* avg_us, bool sym = false);
 
 }  // namespace kmcoarse
 

@@ -420,6 +420,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     const bool coarse = (flags & KM_FLAG_COARSE) != 0;
     const bool smooth = (flags & KM_FLAG_SMOOTH) != 0;
     const bool mg = (flags & KM_FLAG_MG) != 0;
+    const bool sym = (flags & KM_FLAG_SYMMETRIC) != 0;
     KM_REQUIRE(ctx, !(smooth && (coarse || params)), "KM_FLAG_SMOOTH takes neither a coarse level nor a network");
     KM_REQUIRE(ctx, !(mg && (coarse || smooth || params || !f64)),
                "KM_FLAG_MG takes FP64 vectors and no other preconditioner");
@@ -451,7 +452,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
             const int rc = smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter,
                                                               use_graph, &st, &tail)
                                   : kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter,
-                                                             use_graph, &st, &tail);
+                                                             use_graph, &st, &tail, sym);
             if (rc) return rc;
             rr_first = km_mapped_res(ctx, 0);
         }
@@ -564,7 +565,8 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         const int rc =
             smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st)
             : coarse && !params
-                ? kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st)
+                ? kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st, nullptr,
+                                           sym)
                 : fb_inner<T>(ctx, NX, NY, coefT, c32, coef, rhs, d, wT, inner_eps, cap, use_graph, params, coarse,
                               &st, mg);
         if (rc) return rc;

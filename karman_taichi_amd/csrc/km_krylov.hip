@@ -1146,9 +1146,11 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
     if (int rc = check_system(ctx, NX, NY)) return rc;
     KM_REQUIRE(ctx, coef && b && avg_us && reps > 0 && reps <= 4096, "km_time_kernel: invalid arguments");
     const bool b2b = (which & KM_KERNEL_BACK_TO_BACK) != 0;
-    which &= ~KM_KERNEL_BACK_TO_BACK;
+    const bool sym = (which & KM_KERNEL_SYMMETRIC) != 0;
+    which &= ~(KM_KERNEL_BACK_TO_BACK | KM_KERNEL_SYMMETRIC);
     if (which >= KM_KERNEL_TL_PV && which <= KM_KERNEL_TL_CT)
-        return kmcoarse::time_kernel(ctx, which, NX, NY, coef, b, reps, b2b, avg_us);
+        return kmcoarse::time_kernel(ctx, which, NX, NY, coef, b, reps, b2b, avg_us, sym);
+    KM_REQUIRE(ctx, !sym, "KM_KERNEL_SYMMETRIC applies to the two-level kernels");
     KM_REQUIRE(ctx, which == KM_KERNEL_BICG_A || which == KM_KERNEL_BICG_B || which == KM_KERNEL_BICG_R ||
                         which == KM_KERNEL_STENCIL,
                "unknown kernel id %d", which);

@@ -43,9 +43,11 @@ int km_simple_step(km_ctx* ctx, const km_case* c, double eps_mom, double eps_p, 
     // solve_pcorrection_eqn (simple.py:252-255)
     if (int rc = km_assemble_p(ctx, g, c->u, c->v, c->coef_u, c->coef_v, c->mask, c->coef_p, c->b_p, c->mdiv))
         return rc;
+    // the assembled p' system is symmetric bit for bit (aW(i, j) and
+    // aE(i - 1, j) are the same expression; a Dirichlet cell zeroes both sides)
     const int rc = p_flags & (KM_FLAG_COARSE | KM_FLAG_SMOOTH)
-                       ? km_fbicgstab(ctx, nx + 2, ny + 2, c->coef_p, c->b_p, c->pcor, eps_p, 0, p_flags, nullptr,
-                                      &info[2])
+                       ? km_fbicgstab(ctx, nx + 2, ny + 2, c->coef_p, c->b_p, c->pcor, eps_p, 0,
+                                      p_flags | ((p_flags & KM_FLAG_COARSE) ? KM_FLAG_SYMMETRIC : 0), nullptr, &info[2])
                        : km_bicgstab(ctx, nx + 2, ny + 2, c->coef_p, c->b_p, c->pcor, eps_p, 0, p_flags, &info[2]);
     if (rc) return rc;
     // correct_pressure + correct_velocity (simple.py:218-230), then mdiv

@@ -305,7 +305,14 @@ __device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, doub
 // side bands row by row); the output pass (z, A z, partial sums) covers the
 // slots holding the block, and the others only in the tiles at the system's
 // edges (whose boundary row or column lies in the ring).
-template <typename T, int TI, int TJ, int MODE, int SW, bool SELF, int NT>
+//
+// SYM (a symmetric system, aW(i, j) = aE(i - 1, j) and aS(i, j) = aN(i, j - 1)
+// bit for bit, as the assembled p' systems are): only the aP, aE and aN
+// planes are read; each cell's aE and aN go to LDS (zero outside the system,
+// where the halo's links are zero) with one more row above the frame and
+// one more column left of it, and aW, aS are read back from there, so the
+// arithmetic and its result are those of the five-plane kernel.
+template <typename T, int TI, int TJ, int MODE, int SW, bool SELF, int NT, bool SYM = false>
 __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, const TlArgs<T>& g, int L) {
     // The smoother and the matvec are issue-bound: fused multiply-adds (one
     // rounding instead of two per term) and the reciprocal of aP from the
@@ -324,6 +331,11 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
     constexpr int BAND = NS * (WR / 2);       // pairs of the top (and of the bottom) band
     __shared__ T za[HZ * WZ];
     __shared__ double yb[MODE > 0 ? TI * TJ : 1];   // y on the free interior cells (tile-local)
+    // SYM: aE of frame rows -1 .. HR - 1 (row a at a + 1), aN of frame columns
+    // -1 .. WR - 1 (column b at b + 1)
+    __shared__ T aes[SYM ? (HR + 1) * WR : 1];
+    __shared__ T ans[SYM ? HR * (WR + 1) : 1];
+    static_assert(!SYM || WR + HR <= NT, "one extra aE / aN cell per thread");
     __shared__ double sh[4 * NW];
     const KState* st = g.st;
     if (st && st->done) return;
@@ -423,13 +435,33 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
                 // smoothed solves)
                 const uint32_t ko = (uint32_t)k * (uint32_t)sizeof(T);
 #pragma unroll
-                for (int e = 0; e < 5; ++e) cr[q][c][e] = ld_at(g.c + e * n, ko);
+                for (int e = 0; e < 5; ++e)
+                    if (!SYM || e == 0 || e == 2 || e == 3) cr[q][c][e] = ld_at(g.c + e * n, ko);
                 const T* fp[2] = {g.f0, g.f1};
 #pragma unroll
                 for (int e = 0; e < NF; ++e) fv[q][c][e] = ld_at(fp[e], ko);
                 if (MODE > 0) rq[q][c] = ld_at(g.rt, ko);
                 if (MODE > 0) mk[q][c] = mkp[(uint32_t)k];
             }
+        }
+        // SYM: aE of the row above the frame (threads 0 .. WR - 1) and aN of
+        // the column left of it (threads WR .. WR + HR - 1)
+        T xtra = T(0);
+        bool xin = false;
+        if (SYM) {
+            const int t = threadIdx.x;
+            int i = -1, j = -1, e = 2;
+            if (t < WR) {
+                i = ti0 - NS - 1;
+                j = tj0 - NS + t;
+            } else if (t < WR + HR) {
+                i = ti0 - NS + (t - WR);
+                j = tj0 - NS - 1;
+                e = 3;
+            }
+            xin = i >= 0 && i < NX && j >= 0 && j < NY;
+            const uint32_t ko = xin ? ((uint32_t)i * (uint32_t)NY + (uint32_t)j) * (uint32_t)sizeof(T) : 0u;
+            xtra = ld_at(g.c + e * n, ko);
         }
         // z1 = P yc on the z frame (zero off the free cells and outside the
         // system).  Without a coarse vector the loads read the coefficient
@@ -462,7 +494,17 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
             for (int c = 0; c < 2; ++c) {
                 const uint32_t bit = 1u << (2 * q + c);
                 if (MODE > 0 && (ib & bit) && mk[q][c]) fm |= bit;
+                if (SYM && threadIdx.x + q * NT < NP) {
+                    const bool inc = inb & bit;
+                    aes[(pa[q] + 1) * WR + pb[q][c]] = inc ? cr[q][c][2] : T(0);
+                    ans[pa[q] * (WR + 1) + pb[q][c] + 1] = inc ? cr[q][c][3] : T(0);
+                }
             }
+        if (SYM) {
+            const int t = threadIdx.x;
+            if (t < WR) aes[t] = xin ? xtra : T(0);
+            else if (t < WR + HR) ans[(t - WR) * (WR + 1)] = xin ? xtra : T(0);
+        }
         // the source of M(.) on the frame (MODE 2: s, written on the tile's cells)
 #pragma unroll
         for (int q = 0; q < Q; ++q)
@@ -482,6 +524,17 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
             if (idx < HZ * WZ) za[idx] = g.yc && ((zin >> q) & 1u) && mq[q] ? (T)yq[q] : T(0);
         }
         __syncthreads();
+        if (SYM) {
+            // aW = aE of the cell above, aS = aN of the cell to the left
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (threadIdx.x + q * NT < NP)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        cr[q][c][1] = aes[pa[q] * WR + pb[q][c]];
+                        cr[q][c][4] = ans[pa[q] * (WR + 1) + pb[q][c]];
+                    }
+        }
         // half-sweep hs: red cells at even hs, black at odd, on the halo-h frame
 #pragma unroll
         for (int hs = 0; hs < NS; ++hs) {
@@ -603,10 +656,10 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
     }
 }
 
-template <typename T, int TI, int TJ, int MODE, int WPE, int SW, bool SELF, int NT = KM_BLOCK>
+template <typename T, int TI, int TJ, int MODE, int WPE, int SW, bool SELF, int NT = KM_BLOCK, bool SYM = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
-    tile_body<T, TI, TJ, MODE, SW, SELF, NT>(tl, cg, g, xcd_remap(blockIdx.x, tl.G));
+    tile_body<T, TI, TJ, MODE, SW, SELF, NT, SYM>(tl, cg, g, xcd_remap(blockIdx.x, tl.G));
 }
 
 // Two independent systems in one launch: workgroups [0, tl0.G) take the
@@ -621,14 +674,15 @@ k_tl_tile2(ATile tl0, ATile tl1, CGeom cg, TlArgs<T> g0, TlArgs<T> g1) {
 }
 
 // One tile launch of the two-level smoother (kmcoarse::kSweeps red-black
-// sweeps) with tile shape `which` (an index into kTileI / kTileJ).
-template <typename T, int MODE>
+// sweeps) with tile shape `which` (an index into kTileI / kTileJ); SYM: the
+// three-plane kernel of a symmetric system.
+template <typename T, int MODE, bool SYM = false>
 void launch_tile(int which, hipStream_t q, int NX, int NY, const CGeom& cg, const TlArgs<T>& a) {
 #define KM_TL_CASE(IDX)                                                                                           \
     case IDX: {                                                                                                   \
         const ATile tl = a_tile(NX, NY, kTileI[IDX], kTileJ[IDX]);                                               \
         hipLaunchKernelGGL((k_tl_tile<T, kTileI[IDX], kTileJ[IDX], MODE, kTileW[IDX], kmcoarse::kSweeps, false,    \
-                                      kTileT[IDX]>),                                                              \
+                                      kTileT[IDX], SYM>),                                                         \
                            dim3(tl.G), dim3(kTileT[IDX]), 0, q, tl, cg, a);                                      \
         break;                                                                                                    \
     }
@@ -926,7 +980,7 @@ namespace kmcoarse {
 // the fastest is kept (more slots only lengthen the coarse launches' slot
 // sums, m S values).
 template <typename T>
-int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
+int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out, bool sym = false) {
     km_coarse* C = ctx->coarse;
     const auto key = std::make_tuple(C->NX, C->NY, (int)sizeof(T));
     auto it = C->tiles.find(key);
@@ -966,8 +1020,13 @@ int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out) {
         b.P = ctx->partials + TL_PT;
         const CGeom cg = c_geom(C, w);
         auto once = [&]() {
-            launch_tile<T, 1>(w, ctx->stream, C->NX, C->NY, cg, a);
-            launch_tile<T, 2>(w, ctx->stream, C->NX, C->NY, cg, b);
+            if (sym) {
+                launch_tile<T, 1, true>(w, ctx->stream, C->NX, C->NY, cg, a);
+                launch_tile<T, 2, true>(w, ctx->stream, C->NX, C->NY, cg, b);
+            } else {
+                launch_tile<T, 1>(w, ctx->stream, C->NX, C->NY, cg, a);
+                launch_tile<T, 2>(w, ctx->stream, C->NX, C->NY, cg, b);
+            }
         };
         for (int k = 0; k < 3; ++k) once();
         (void)hipEventRecord(e0, ctx->stream);
@@ -1060,9 +1119,11 @@ struct Fused {
     double* P;
     int* flag;
 
-    Fused(km_ctx* ctx, int NX_, int NY_, const T* coef, T* ws, T* xd, int tile_)
+    bool sym;   // symmetric system: the three-plane tile kernels
+
+    Fused(km_ctx* ctx, int NX_, int NY_, const T* coef, T* ws, T* xd, int tile_, bool sym_ = false)
         : NX(NX_), NY(NY_), tile(tile_), x(xd), coefT(coef), C(ctx->coarse), st(&ctx->st[4]), P(ctx->partials),
-          flag(ctx->d_flag) {
+          flag(ctx->d_flag), sym(sym_) {
         const int64_t n = (int64_t)NX * NY;
         r = ws;
         rt = ws + n;
@@ -1097,7 +1158,8 @@ struct Fused {
         a.P = P + TL_PV;
         a.ph = phs;
         a.om = (T)kOmega;
-        launch_tile<T, 1>(tile, q, NX, NY, cg, a);
+        if (sym) launch_tile<T, 1, true>(tile, q, NX, NY, cg, a);
+        else launch_tile<T, 1>(tile, q, NX, NY, cg, a);
     }
     void st_(hipStream_t q, int phs) const {
         TlArgs<T> a{};
@@ -1115,7 +1177,8 @@ struct Fused {
         a.P = P + TL_PT;
         a.ph = phs;
         a.om = (T)kOmega;
-        launch_tile<T, 2>(tile, q, NX, NY, cg, a);
+        if (sym) launch_tile<T, 2, true>(tile, q, NX, NY, cg, a);
+        else launch_tile<T, 2>(tile, q, NX, NY, cg, a);
     }
     void total_a(hipStream_t q) const {
         if (fin & FIN_A)
@@ -1166,18 +1229,18 @@ int fused_start(km_ctx* ctx, const Fused<T>& f, const T* rhs, double eps, int ma
 
 template <typename T>
 int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
-                bool use_graph, KState* out, const std::function<int()>* tail) {
+                bool use_graph, KState* out, const std::function<int()>* tail, bool sym) {
     km_coarse* C = ctx->coarse;
     KM_REQUIRE(ctx, C && C->arena && C->NX == NX && C->NY == NY, "coarse level not set up for this system");
     int tile = 0;
-    if (int rc = tuned_tile<T>(ctx, coefT, rhs, ws, &tile)) return rc;
-    const Fused<T> f(ctx, NX, NY, coefT, ws, xd, tile);
+    if (int rc = tuned_tile<T>(ctx, coefT, rhs, ws, &tile, sym)) return rc;
+    const Fused<T> f(ctx, NX, NY, coefT, ws, xd, tile, sym);
     if (int rc = fused_start<T>(ctx, f, rhs, eps, max_iter)) return rc;
     auto enqueue = [&f](hipStream_t q, int phs) { f.iteration(q, phs); };
     GraphKey key;
     key.v = {10, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
              (uintptr_t)ws, (uintptr_t)f.st, (uintptr_t)C, (uintptr_t)C->arena, (uintptr_t)C->m, (uintptr_t)C->B,
-             (uintptr_t)tile, (uintptr_t)f.fin};
+             (uintptr_t)tile, (uintptr_t)f.fin, (uintptr_t)sym};
     // the shortest of the last solves of this system shape (the outer steps
     // of a time step differ) sets where the short chunks begin
     int expect = 0;
@@ -1204,7 +1267,7 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
 // test disabled; events around every launch of that kernel, or (b2b) around
 // reps identical launches after eight warm iterations.
 int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, const double* b, int reps, bool b2b,
-                double* avg_us) {
+                double* avg_us, bool sym) {
     KM_REQUIRE(ctx, which >= 4 && which <= 7, "unknown kernel id %d", which);
     if (int rc = setup(ctx, NX, NY, coef, ctx->stream)) return rc;
     const int64_t n = (int64_t)NX * NY;
@@ -1212,8 +1275,8 @@ int time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, cons
     double* ws = static_cast<double*>(km_ws_vectors(ctx));
     double* x = ws + 8 * n;
     int tile = 0;
-    if (int rc = tuned_tile<double>(ctx, coef, b, ws, &tile)) return rc;
-    const Fused<double> f(ctx, NX, NY, coef, ws, x, tile);
+    if (int rc = tuned_tile<double>(ctx, coef, b, ws, &tile, sym)) return rc;
+    const Fused<double> f(ctx, NX, NY, coef, ws, x, tile, sym);
     if (int rc = fused_start<double>(ctx, f, b, 0.0, 1 << 30)) return rc;
     std::vector<hipEvent_t> ev(2 * (size_t)reps, nullptr);
     for (auto& e : ev) KM_HIP_SYNC(ctx, hipEventCreate(&e));
@@ -1497,9 +1560,9 @@ template int smooth_inner2<double>(km_ctx*, const int*, const int*, const double
 template int smooth<float>(km_ctx*, const float*, const float*, float*, hipStream_t);
 template int smooth<double>(km_ctx*, const double*, const double*, double*, hipStream_t);
 template int fused_inner<float>(km_ctx*, int, int, const float*, const float*, float*, float*, double, int, bool,
-                                KState*, const std::function<int()>*);
+                                KState*, const std::function<int()>*, bool);
 template int fused_inner<double>(km_ctx*, int, int, const double*, const double*, double*, double*, double, int,
-                                 bool, KState*, const std::function<int()>*);
+                                 bool, KState*, const std::function<int()>*, bool);
 
 }  // namespace kmcoarse
 
