@@ -321,12 +321,14 @@ int km_batch_step(km_ctx* ctx, int n, const km_case* cases, double eps_mom, doub
     if (int rc = run_stages(ctx, B, 1, use_graph)) return rc;
     if (flags & KM_FLAG_COARSE) {
         // p' case by case with the two-level preconditioned flexible solve
-        // (its coarse level is rebuilt per case shape)
+        // (its coarse level is rebuilt per case shape); the assembled p'
+        // systems are symmetric bit for bit (three-plane smoothing tiles)
         for (int c = 0; c < n; ++c) {
             const km_case& K = B->cases[c];
             km_solve_info si{};
             const int rc = km_fbicgstab(ctx, K.g.nx + 2, K.g.ny + 2, K.coef_p, K.b_p, K.pcor, eps_p, max_iter,
-                                        KM_FLAG_COARSE | (flags & KM_FLAG_NO_GRAPH), nullptr, &si);
+                                        KM_FLAG_COARSE | KM_FLAG_SYMMETRIC | (flags & KM_FLAG_NO_GRAPH), nullptr,
+                                        &si);
             if (rc < 0) return rc;
             if (info) info[3 * c + 2] = si;
         }

@@ -135,7 +135,7 @@ class UNetPreconditioner:
 
 def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: Optional[UNetPreconditioner], eps=1e-8,
               max_iter=None, x0=False, graph=True, mixed=False, coarse=False, smooth=False, gcr=False,
-              jacobi=False, mg=False):
+              jacobi=False, mg=False, symmetric=False):
     """Flexible BiCGSTAB with the network as right preconditioner
     (km_fbicgstab); ``net=None`` uses the Jacobi preconditioner instead.
     ``mixed``: FP32 Krylov vectors (FP64 scalars and refinement).
@@ -147,7 +147,8 @@ def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: Optiona
     ``jacobi`` (with ``gcr`` and ``coarse``, no network): one Jacobi step
     after the coarse correction instead of the SOR sweeps.  ``mg`` (no
     network, FP64): one multigrid V-cycle as the preconditioner
-    (KM_FLAG_MG)."""
+    (KM_FLAG_MG).  ``symmetric`` (with ``coarse``): the caller asserts the
+    system is symmetric bit for bit (KM_FLAG_SYMMETRIC)."""
     from .solvers import SolveInfo
     NX, NY = b.shape
     info = KmSolveInfo()
@@ -156,6 +157,7 @@ def fbicgstab(coef: torch.Tensor, b: torch.Tensor, x: torch.Tensor, net: Optiona
     flags |= _lib.KM_FLAG_SMOOTH if smooth else 0
     flags |= (_lib.KM_FLAG_GCR if gcr else 0) | (_lib.KM_FLAG_JACOBI if jacobi else 0)
     flags |= _lib.KM_FLAG_MG if mg else 0
+    flags |= _lib.KM_FLAG_SYMMETRIC if symmetric else 0
     ctx = _lib.context_for(b.device) if net is None else net.ctx
     ctx.set_stream(torch.cuda.current_stream(b.device).cuda_stream)
     ctx.call("km_fbicgstab", NX, NY, ptr(coef), ptr(b), ptr(x), float(eps), int(max_iter or 0), flags,

@@ -215,8 +215,10 @@ class SIMPLESolver:
         if self.p_solver == "cg":
             return self.p_correction_cg.solve(eps=self.eps_p)
         if self.p_precond is not None or self.p_coarse:
+            # the assembled p' system is symmetric bit for bit (compute_coef_p)
             return self.p_correction_solver.solve(eps=self.eps_p, x0=self.warm_start_p, precond=self.p_precond,
-                                                 mixed=self.mixed, coarse=self.p_coarse, gcr=self.p_gcr)
+                                                 mixed=self.mixed, coarse=self.p_coarse, gcr=self.p_gcr,
+                                                 symmetric=True)
         return self.p_correction_solver.solve(eps=self.eps_p, mixed=self.mixed, x0=self.warm_start_p,
                                              true_residual=self.true_residual)
 

@@ -158,7 +158,8 @@ class BICGSolver(CGSolver):
     _kind = "bicgstab"
 
     def solve(self, eps=1e-8, quiet=True, max_iter=None, mixed=False, x0=False, graph=True,
-              precond=None, coarse=False, smooth=False, true_residual=False, gcr=False) -> SolveInfo:
+              precond=None, coarse=False, smooth=False, true_residual=False, gcr=False,
+              symmetric=False) -> SolveInfo:
         """BiCGSTAB until sqrt(r.r / b.b) < eps (bicgsolver.py:130).
 
         ``mixed``: FP32 matvec/axpy inner solves, FP64 scalars, FP64 refinement
@@ -178,6 +179,10 @@ class BICGSolver(CGSolver):
         ``true_residual`` (plain solve): after the reference's stop on the
         recursive residual, restart from x while the true residual is above
         eps (KM_FLAG_TRUE; the reference has no such check).
+        ``symmetric`` (with ``coarse``): the caller asserts the system is
+        symmetric bit for bit, as the assembled p' systems are
+        (KM_FLAG_SYMMETRIC: three coefficient planes in the smoothing tiles,
+        identical results).
         """
         flags = (_lib.KM_FLAG_MIXED if mixed else 0) | (_lib.KM_FLAG_X0 if x0 else 0)
         flags |= _lib.KM_FLAG_TRUE if true_residual else 0
@@ -186,7 +191,7 @@ class BICGSolver(CGSolver):
             from .learned import fbicgstab
             self._refresh()
             info = fbicgstab(self.coef, self.b, self.x, precond, eps=eps, max_iter=max_iter, x0=x0, graph=graph,
-                             mixed=mixed, coarse=coarse, smooth=smooth, gcr=gcr)
+                             mixed=mixed, coarse=coarse, smooth=smooth, gcr=gcr, symmetric=symmetric)
         else:
             info = self._run("km_bicgstab", eps, max_iter, flags)
         if not quiet:
