@@ -594,10 +594,19 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
     }
     if (status == KM_STALL || status == KM_NONFINITE || (status == KM_MAXITER && total < max_iter)) {
         km_solve_info sub;
-        const int rc = km_bicgstab(ctx, NX, NY, coef, b, x, eps, max_iter > total ? max_iter - total : 1,
-                                   (flags & ~KM_FLAG_MIXED) | KM_FLAG_X0, &sub);
+        const int left = max_iter > total ? max_iter - total : 1;
+        // FP32 vectors with a classical preconditioner: the same solve with
+        // FP64 vectors from x (a few dozen iterations where the FP32 sweep
+        // stalled), which itself falls back to plain BiCGSTAB; otherwise
+        // plain FP64 BiCGSTAB from x
+        const int rc = !f64 && !params && (coarse || smooth)
+                           ? fbicg_solve<double>(ctx, NX, NY, coef, b, x, eps, left,
+                                                 (flags & ~KM_FLAG_MIXED) | KM_FLAG_X0, nullptr, &sub)
+                           : km_bicgstab(ctx, NX, NY, coef, b, x, eps, left, (flags & ~KM_FLAG_MIXED) | KM_FLAG_X0,
+                                         &sub);
         if (rc < 0) return rc;
         total += sub.iterations;
+        sweeps += sub.refinements;
         status = sub.status;
         rel = sub.rel_resid_true;
         if (info) info->fallback = 1;
