@@ -94,9 +94,12 @@ int smooth(km_ctx* ctx, const T* coefT, const T* r, T* z, hipStream_t s);
 // (red-black SOR smoothing), four launches per iteration; ws holds 8n values of T.
 // sym: the caller asserts aW(i, j) = aE(i - 1, j) and aS(i, j) = aN(i, j - 1)
 // bit for bit (the smoothing tiles then read three coefficient planes).
+// coef64: the FP64 coefficients (required with T = float: the FP32-vector
+// smoothing tiles compute with them).
 template <typename T>
 int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
-                bool use_graph, KState* out, const std::function<int()>* tail = nullptr, bool sym = false);
+                bool use_graph, KState* out, const std::function<int()>* tail = nullptr, bool sym = false,
+                const double* coef64 = nullptr);
 
 // Inner flexible BiCGSTAB for A d = rhs with one red-black Gauss-Seidel sweep
 // as the preconditioner and no coarse level (the momentum systems), three

@@ -452,7 +452,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
             const int rc = smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter,
                                                               use_graph, &st, &tail)
                                   : kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter,
-                                                             use_graph, &st, &tail, sym);
+                                                             use_graph, &st, &tail, sym, coef);
             if (rc) return rc;
             rr_first = km_mapped_res(ctx, 0);
         }
@@ -566,7 +566,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
             smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st)
             : coarse && !params
                 ? kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st, nullptr,
-                                           sym)
+                                           sym, coef)
                 : fb_inner<T>(ctx, NX, NY, coefT, c32, coef, rhs, d, wT, inner_eps, cap, use_graph, params, coarse,
                               &st, mg);
         if (rc) return rc;

@@ -34,6 +34,7 @@
 #include "km_coarse.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <vector>
 
@@ -152,6 +153,7 @@ struct CGeom {
 template <typename T>
 struct TlArgs {
     const T* c;              // 5 coefficient planes
+    const double* cd;        // the same in FP64 (the C64 kernels of FP32 vectors)
     const uint8_t* mask;     // free cells
     const T* f0;
     const T* f1;
@@ -312,12 +314,25 @@ __device__ __forceinline__ bool cv_scalars(KState* __restrict__ st, int ph, doub
 // where the halo's links are zero) with one more row above the frame and
 // one more column left of it, and aW, aS are read back from there, so the
 // arithmetic and its result are those of the five-plane kernel.
-template <typename T, int TI, int TJ, int MODE, int SW, bool SELF, int NT, bool SYM = false>
+//
+// C64 (FP32 vectors, T = float): the coefficients come from the FP64 planes
+// (g.cd) and the smoother and y = A z run in FP64, z rounded to T in LDS.
+// A p' row's five terms cancel to about lambda_min / aP of aP z on smooth z,
+// so FP32 coefficients or FP32 sums err there by ~6e-8 times the condition
+// number (order nx^2): on the largest channels that stalls the FP64
+// refinement around the FP32 solve.  With FP64 coefficients and sums, y is A
+// applied to the stored (rounded) z to FP64 accuracy, and the FP32 iteration
+// solves the FP64 system.
+template <typename T, int TI, int TJ, int MODE, int SW, bool SELF, int NT, bool SYM = false, bool C64 = false>
 __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, const TlArgs<T>& g, int L) {
     // The smoother and the matvec are issue-bound: fused multiply-adds (one
     // rounding instead of two per term) and the reciprocal of aP from the
     // hardware estimate with two Newton steps instead of a full division.
 #pragma clang fp contract(fast)
+    using CT = typename std::conditional<C64, double, T>::type;   // coefficient and arithmetic type
+    const CT* __restrict__ cbase;
+    if constexpr (C64) cbase = g.cd;
+    else cbase = g.c;
     static_assert((NT == 256 || NT == 512) && (!SELF || NT == KM_BLOCK), "4 or 8 waves; SELF sums as KM_BLOCK");
     constexpr int NW = NT / 64;                                     // waves per workgroup
     constexpr int NS = 2 * SW;                                      // half-sweeps = halo
@@ -333,8 +348,8 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
     __shared__ double yb[MODE > 0 ? TI * TJ : 1];   // y on the free interior cells (tile-local)
     // SYM: aE of frame rows -1 .. HR - 1 (row a at a + 1), aN of frame columns
     // -1 .. WR - 1 (column b at b + 1)
-    __shared__ T aes[SYM ? (HR + 1) * WR : 1];
-    __shared__ T ans[SYM ? HR * (WR + 1) : 1];
+    __shared__ CT aes[SYM ? (HR + 1) * WR : 1];
+    __shared__ CT ans[SYM ? HR * (WR + 1) : 1];
     static_assert(!SYM || WR + HR <= NT, "one extra aE / aN cell per thread");
     __shared__ double sh[4 * NW];
     const KState* st = g.st;
@@ -381,7 +396,8 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
         // contributes is selected only after all of them are issued, so the
         // tile's loads are in flight together instead of one round trip per
         // conditional load.
-        T cr[Q][2][5], sr[Q][2], rq[Q][2], wd[Q][2];
+        CT cr[Q][2][5], wd[Q][2];
+        T sr[Q][2], rq[Q][2];
         T fv[Q][2][NF];
         uint8_t mk[Q][2];
         int64_t kq[Q][2];
@@ -434,9 +450,10 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
                 // cells, kTileCellsMax: checked by the coarse set-up and the
                 // smoothed solves)
                 const uint32_t ko = (uint32_t)k * (uint32_t)sizeof(T);
+                const uint32_t kc = (uint32_t)k * (uint32_t)sizeof(CT);
 #pragma unroll
                 for (int e = 0; e < 5; ++e)
-                    if (!SYM || e == 0 || e == 2 || e == 3) cr[q][c][e] = ld_at(g.c + e * n, ko);
+                    if (!SYM || e == 0 || e == 2 || e == 3) cr[q][c][e] = ld_at(cbase + e * n, kc);
                 const T* fp[2] = {g.f0, g.f1};
 #pragma unroll
                 for (int e = 0; e < NF; ++e) fv[q][c][e] = ld_at(fp[e], ko);
@@ -446,7 +463,7 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
         }
         // SYM: aE of the row above the frame (threads 0 .. WR - 1) and aN of
         // the column left of it (threads WR .. WR + HR - 1)
-        T xtra = T(0);
+        CT xtra = CT(0);
         bool xin = false;
         if (SYM) {
             const int t = threadIdx.x;
@@ -460,8 +477,8 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
                 e = 3;
             }
             xin = i >= 0 && i < NX && j >= 0 && j < NY;
-            const uint32_t ko = xin ? ((uint32_t)i * (uint32_t)NY + (uint32_t)j) * (uint32_t)sizeof(T) : 0u;
-            xtra = ld_at(g.c + e * n, ko);
+            const uint32_t kc = xin ? ((uint32_t)i * (uint32_t)NY + (uint32_t)j) * (uint32_t)sizeof(CT) : 0u;
+            xtra = ld_at(cbase + e * n, kc);
         }
         // z1 = P yc on the z frame (zero off the free cells and outside the
         // system).  Without a coarse vector the loads read the coefficient
@@ -496,14 +513,14 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
                 if (MODE > 0 && (ib & bit) && mk[q][c]) fm |= bit;
                 if (SYM && threadIdx.x + q * NT < NP) {
                     const bool inc = inb & bit;
-                    aes[(pa[q] + 1) * WR + pb[q][c]] = inc ? cr[q][c][2] : T(0);
-                    ans[pa[q] * (WR + 1) + pb[q][c] + 1] = inc ? cr[q][c][3] : T(0);
+                    aes[(pa[q] + 1) * WR + pb[q][c]] = inc ? cr[q][c][2] : CT(0);
+                    ans[pa[q] * (WR + 1) + pb[q][c] + 1] = inc ? cr[q][c][3] : CT(0);
                 }
             }
         if (SYM) {
             const int t = threadIdx.x;
-            if (t < WR) aes[t] = xin ? xtra : T(0);
-            else if (t < WR + HR) ans[(t - WR) * (WR + 1)] = xin ? xtra : T(0);
+            if (t < WR) aes[t] = xin ? xtra : CT(0);
+            else if (t < WR + HR) ans[(t - WR) * (WR + 1)] = xin ? xtra : CT(0);
         }
         // the source of M(.) on the frame (MODE 2: s, written on the tile's cells)
 #pragma unroll
@@ -516,7 +533,7 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
                     if ((oub >> (2 * q + c)) & 1u) g.src[kq[q][c]] = sv;
                 }
                 sr[q][c] = sv;
-                wd[q][c] = cr[q][c][0] != T(0) ? om * recip(cr[q][c][0]) : T(0);   // omega D^-1
+                wd[q][c] = cr[q][c][0] != CT(0) ? (CT)om * recip(cr[q][c][0]) : CT(0);   // omega D^-1
             }
 #pragma unroll
         for (int q = 0; q < QZ; ++q) {
@@ -546,13 +563,13 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
                     b >= TJ + NS + h)
                     continue;
                 const int pz = (a + 1) * WZ + (b + 1);
-                const T zc = za[pz];
-                T az = cr[q][c][0] * zc;
-                az = az + cr[q][c][1] * za[pz - WZ];
-                az = az + cr[q][c][2] * za[pz + WZ];
-                az = az + cr[q][c][3] * za[pz + 1];
-                az = az + cr[q][c][4] * za[pz - 1];
-                za[pz] = zc + wd[q][c] * (sr[q][c] - az);
+                const CT zc = za[pz];
+                CT az = cr[q][c][0] * zc;
+                az = az + cr[q][c][1] * (CT)za[pz - WZ];
+                az = az + cr[q][c][2] * (CT)za[pz + WZ];
+                az = az + cr[q][c][3] * (CT)za[pz + 1];
+                az = az + cr[q][c][4] * (CT)za[pz - 1];
+                za[pz] = (T)(zc + wd[q][c] * ((CT)sr[q][c] - az));
             }
             __syncthreads();
         }
@@ -566,12 +583,12 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
             const T zc = zf[pz];
             g.z[k] = zc;
             if (MODE > 0) {
-                T yv = cr[q][c][0] * zc;
-                yv = yv + cr[q][c][1] * zf[pz - WZ];
-                yv = yv + cr[q][c][2] * zf[pz + WZ];
-                yv = yv + cr[q][c][3] * zf[pz + 1];
-                yv = yv + cr[q][c][4] * zf[pz - 1];
-                g.y[k] = yv;
+                CT yv = cr[q][c][0] * (CT)zc;
+                yv = yv + cr[q][c][1] * (CT)zf[pz - WZ];
+                yv = yv + cr[q][c][2] * (CT)zf[pz + WZ];
+                yv = yv + cr[q][c][3] * (CT)zf[pz + 1];
+                yv = yv + cr[q][c][4] * (CT)zf[pz - 1];
+                g.y[k] = (T)yv;
                 const uint32_t bit = 1u << (2 * q + c);
                 if (ib & bit) yb[(a - NS) * TJ + (b - NS)] = (fm & bit) ? (double)yv : 0.0;
                 const double rtk = (double)rq[q][c];
@@ -656,10 +673,11 @@ __device__ __forceinline__ void tile_body(const ATile& tl, const CGeom& cg, cons
     }
 }
 
-template <typename T, int TI, int TJ, int MODE, int WPE, int SW, bool SELF, int NT = KM_BLOCK, bool SYM = false>
+template <typename T, int TI, int TJ, int MODE, int WPE, int SW, bool SELF, int NT = KM_BLOCK, bool SYM = false,
+          bool C64 = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
 k_tl_tile(ATile tl, CGeom cg, TlArgs<T> g) {
-    tile_body<T, TI, TJ, MODE, SW, SELF, NT, SYM>(tl, cg, g, xcd_remap(blockIdx.x, tl.G));
+    tile_body<T, TI, TJ, MODE, SW, SELF, NT, SYM, C64>(tl, cg, g, xcd_remap(blockIdx.x, tl.G));
 }
 
 // Two independent systems in one launch: workgroups [0, tl0.G) take the
@@ -675,14 +693,15 @@ k_tl_tile2(ATile tl0, ATile tl1, CGeom cg, TlArgs<T> g0, TlArgs<T> g1) {
 
 // One tile launch of the two-level smoother (kmcoarse::kSweeps red-black
 // sweeps) with tile shape `which` (an index into kTileI / kTileJ); SYM: the
-// three-plane kernel of a symmetric system.
-template <typename T, int MODE, bool SYM = false>
+// three-plane kernel of a symmetric system; C64: FP64 coefficients and
+// arithmetic for FP32 vectors (a.cd).
+template <typename T, int MODE, bool SYM = false, bool C64 = false>
 void launch_tile(int which, hipStream_t q, int NX, int NY, const CGeom& cg, const TlArgs<T>& a) {
 #define KM_TL_CASE(IDX)                                                                                           \
     case IDX: {                                                                                                   \
         const ATile tl = a_tile(NX, NY, kTileI[IDX], kTileJ[IDX]);                                               \
         hipLaunchKernelGGL((k_tl_tile<T, kTileI[IDX], kTileJ[IDX], MODE, kTileW[IDX], kmcoarse::kSweeps, false,    \
-                                      kTileT[IDX], SYM>),                                                         \
+                                      kTileT[IDX], SYM, C64>),                                                    \
                            dim3(tl.G), dim3(kTileT[IDX]), 0, q, tl, cg, a);                                      \
         break;                                                                                                    \
     }
@@ -980,7 +999,10 @@ namespace kmcoarse {
 // the fastest is kept (more slots only lengthen the coarse launches' slot
 // sums, m S values).
 template <typename T>
-int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out, bool sym = false) {
+int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out, bool sym = false,
+               const double* cd = nullptr) {
+    constexpr bool c64 = sizeof(T) == sizeof(float);   // FP32 vectors: FP64 coefficients (cd)
+    KM_REQUIRE(ctx, !c64 || cd, "FP32 tiles need the FP64 coefficients");
     km_coarse* C = ctx->coarse;
     const auto key = std::make_tuple(C->NX, C->NY, (int)sizeof(T));
     auto it = C->tiles.find(key);
@@ -1001,6 +1023,7 @@ int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out, bool 
         if (tile_slots(C, w) > kMaxVS) continue;
         TlArgs<T> a{};   // K_PV: p^ = M(src), v
         a.c = coefT;
+        a.cd = cd;
         a.mask = C->mask;
         a.f0 = src;
         a.rt = src;
@@ -1021,11 +1044,11 @@ int tuned_tile(km_ctx* ctx, const T* coefT, const T* src, T* ws, int* out, bool 
         const CGeom cg = c_geom(C, w);
         auto once = [&]() {
             if (sym) {
-                launch_tile<T, 1, true>(w, ctx->stream, C->NX, C->NY, cg, a);
-                launch_tile<T, 2, true>(w, ctx->stream, C->NX, C->NY, cg, b);
+                launch_tile<T, 1, true, c64>(w, ctx->stream, C->NX, C->NY, cg, a);
+                launch_tile<T, 2, true, c64>(w, ctx->stream, C->NX, C->NY, cg, b);
             } else {
-                launch_tile<T, 1>(w, ctx->stream, C->NX, C->NY, cg, a);
-                launch_tile<T, 2>(w, ctx->stream, C->NX, C->NY, cg, b);
+                launch_tile<T, 1, false, c64>(w, ctx->stream, C->NX, C->NY, cg, a);
+                launch_tile<T, 2, false, c64>(w, ctx->stream, C->NX, C->NY, cg, b);
             }
         };
         for (int k = 0; k < 3; ++k) once();
@@ -1119,11 +1142,13 @@ struct Fused {
     double* P;
     int* flag;
 
-    bool sym;   // symmetric system: the three-plane tile kernels
+    bool sym;            // symmetric system: the three-plane tile kernels
+    const double* cd;    // FP64 coefficients (the FP32-vector tiles compute with them)
 
-    Fused(km_ctx* ctx, int NX_, int NY_, const T* coef, T* ws, T* xd, int tile_, bool sym_ = false)
+    Fused(km_ctx* ctx, int NX_, int NY_, const T* coef, T* ws, T* xd, int tile_, bool sym_ = false,
+          const double* cd_ = nullptr)
         : NX(NX_), NY(NY_), tile(tile_), x(xd), coefT(coef), C(ctx->coarse), st(&ctx->st[4]), P(ctx->partials),
-          flag(ctx->d_flag), sym(sym_) {
+          flag(ctx->d_flag), sym(sym_), cd(cd_) {
         const int64_t n = (int64_t)NX * NY;
         r = ws;
         rt = ws + n;
@@ -1158,8 +1183,15 @@ struct Fused {
         a.P = P + TL_PV;
         a.ph = phs;
         a.om = (T)kOmega;
-        if (sym) launch_tile<T, 1, true>(tile, q, NX, NY, cg, a);
-        else launch_tile<T, 1>(tile, q, NX, NY, cg, a);
+        tiles<1>(q, a);
+    }
+    // the tile launch of this solve's kernel variant
+    template <int MODE>
+    void tiles(hipStream_t q, TlArgs<T> a) const {
+        constexpr bool c64 = sizeof(T) == sizeof(float);
+        a.cd = cd;
+        if (sym) launch_tile<T, MODE, true, c64>(tile, q, NX, NY, cg, a);
+        else launch_tile<T, MODE, false, c64>(tile, q, NX, NY, cg, a);
     }
     void st_(hipStream_t q, int phs) const {
         TlArgs<T> a{};
@@ -1177,8 +1209,7 @@ struct Fused {
         a.P = P + TL_PT;
         a.ph = phs;
         a.om = (T)kOmega;
-        if (sym) launch_tile<T, 2, true>(tile, q, NX, NY, cg, a);
-        else launch_tile<T, 2>(tile, q, NX, NY, cg, a);
+        tiles<2>(q, a);
     }
     void total_a(hipStream_t q) const {
         if (fin & FIN_A)
@@ -1229,18 +1260,20 @@ int fused_start(km_ctx* ctx, const Fused<T>& f, const T* rhs, double eps, int ma
 
 template <typename T>
 int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd, T* ws, double eps, int max_iter,
-                bool use_graph, KState* out, const std::function<int()>* tail, bool sym) {
+                bool use_graph, KState* out, const std::function<int()>* tail, bool sym, const double* coef64) {
     km_coarse* C = ctx->coarse;
     KM_REQUIRE(ctx, C && C->arena && C->NX == NX && C->NY == NY, "coarse level not set up for this system");
+    if (sizeof(T) == sizeof(double)) coef64 = reinterpret_cast<const double*>(coefT);
+    KM_REQUIRE(ctx, coef64, "the FP32-vector two-level solve needs the FP64 coefficients");
     int tile = 0;
-    if (int rc = tuned_tile<T>(ctx, coefT, rhs, ws, &tile, sym)) return rc;
-    const Fused<T> f(ctx, NX, NY, coefT, ws, xd, tile, sym);
+    if (int rc = tuned_tile<T>(ctx, coefT, rhs, ws, &tile, sym, coef64)) return rc;
+    const Fused<T> f(ctx, NX, NY, coefT, ws, xd, tile, sym, coef64);
     if (int rc = fused_start<T>(ctx, f, rhs, eps, max_iter)) return rc;
     auto enqueue = [&f](hipStream_t q, int phs) { f.iteration(q, phs); };
     GraphKey key;
     key.v = {10, sizeof(T), (uintptr_t)NX, (uintptr_t)NY, (uintptr_t)coefT, (uintptr_t)rhs, (uintptr_t)xd,
              (uintptr_t)ws, (uintptr_t)f.st, (uintptr_t)C, (uintptr_t)C->arena, (uintptr_t)C->m, (uintptr_t)C->B,
-             (uintptr_t)tile, (uintptr_t)f.fin, (uintptr_t)sym};
+             (uintptr_t)tile, (uintptr_t)f.fin, (uintptr_t)sym, (uintptr_t)coef64};
     // the shortest of the last solves of this system shape (the outer steps
     // of a time step differ) sets where the short chunks begin
     int expect = 0;
@@ -1560,9 +1593,9 @@ template int smooth_inner2<double>(km_ctx*, const int*, const int*, const double
 template int smooth<float>(km_ctx*, const float*, const float*, float*, hipStream_t);
 template int smooth<double>(km_ctx*, const double*, const double*, double*, hipStream_t);
 template int fused_inner<float>(km_ctx*, int, int, const float*, const float*, float*, float*, double, int, bool,
-                                KState*, const std::function<int()>*, bool);
+                                KState*, const std::function<int()>*, bool, const double*);
 template int fused_inner<double>(km_ctx*, int, int, const double*, const double*, double*, double*, double, int,
-                                 bool, KState*, const std::function<int()>*, bool);
+                                 bool, KState*, const std::function<int()>*, bool, const double*);
 
 }  // namespace kmcoarse
 
