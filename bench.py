@@ -389,14 +389,14 @@ def learned_compare(s, weights, reps):
     """The metric's second leg on the last outer step's p' system, each solve
     from zero to 1e-8 in the true residual, ``reps`` times: plain FP64
     BiCGSTAB against the learned solve -- flexible GCR whose every
-    preconditioner application gives two directions after the coarse
-    correction, a Jacobi step and the trained U-Net (BF16 on the matrix
-    cores), FP32 residual-space vectors, FP64 solution-space vectors,
-    scalars and true residual, plain BiCGSTAB as the fallback when a restart
-    cycle stalls -- beside the same solve without the network (coarse
-    correction + Jacobi step), the default two-level flexible BiCGSTAB
-    (red-black SOR sweeps) and flexible BiCGSTAB / GCR with one multigrid
-    V-cycle as the preconditioner.  Median wall ms (each solve returns after a
+    preconditioner application gives two directions, the two-level M(r)
+    (coarse correction + red-black SOR sweeps) and the coarse correction
+    plus the trained U-Net (BF16 on the matrix cores), FP32 residual-space
+    vectors, FP64 solution-space vectors, scalars and true residual, plain
+    BiCGSTAB as the fallback when a restart cycle stalls -- beside the same
+    solve without the network (the two-level direction alone), the default
+    two-level flexible BiCGSTAB and flexible BiCGSTAB / GCR with one
+    multigrid V-cycle as the preconditioner.  Median wall ms (each solve returns after a
     stream synchronisation), iterations, true residuals, fallbacks."""
     import torch
     from karman_taichi_amd import learned as kl
@@ -408,7 +408,7 @@ def learned_compare(s, weights, reps):
         "plain": lambda: s.p_correction_solver.solve(eps=1e-8),
         "learned_fp32": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, net, eps=1e-8, mixed=True, coarse=True, gcr=True),
         "no_network_fp32": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, mixed=True, coarse=True,
-                                                gcr=True, jacobi=True),
+                                                gcr=True),
         "twolevel_sor": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, coarse=True),
         "multigrid_fbicgstab": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, mg=True),
         "multigrid_gcr": lambda: kl.fbicgstab(s._coef_p, s.b_p, x, None, eps=1e-8, mg=True, gcr=True),

@@ -60,7 +60,7 @@ extern "C" {
 #define KM_FLAG_JACOBI 128    /* km_fbicgstab with KM_FLAG_GCR and KM_FLAG_COARSE: without a network, one Jacobi step
                                * after the coarse correction instead of the red-black SOR sweeps; with a network, one
                                * direction z1 + net(r1) followed by a Jacobi step instead of the two directions
-                               * z1 + D^-1 r1 and z1 + net(r1) */
+                               * M(r) (the two-level preconditioner: SOR sweeps from z1) and z1 + net(r1) */
 #define KM_FLAG_MG 256       /* km_fbicgstab: the preconditioner is one multigrid V-cycle (km_mg_vcycle) in place of
                                * Jacobi; FP64 vectors, no coarse level, smoothing or network.  A comparator for the
                                * two-level and learned preconditioners in the spirit of benchmark/mgpcg.py:91-135

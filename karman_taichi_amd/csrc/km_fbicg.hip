@@ -884,9 +884,10 @@ int fgcr_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b,
         if (!(rel == rel)) { status = KM_NONFINITE; break; }
         if (total >= max_iter) { status = KM_MAXITER; break; }
         // the directions of this application of M to r / ||r||: one, or with
-        // the network after the coarse correction two -- z = z1 + D^-1 r1 and
-        // z2 = z1 + net(r1) -- so that GCR minimises over both and the
-        // network's direction can only add to the classical one
+        // the network after the coarse correction two -- the two-level M(r)
+        // (SOR sweeps from z1) and z2 = z1 + net(r1) -- so that GCR minimises
+        // over both and the network's direction can only add to the
+        // classical one
         hipLaunchKernelGGL(k_scale_to<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, (const double*)r64, rhs,
                            1.0 / sqrt(rr));
         KM_CHECK_LAUNCH(ctx);
@@ -906,9 +907,11 @@ int fgcr_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b,
                 hipLaunchKernelGGL(k_fg_jacobi_add<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, coef,
                                    (const T*)z2, (const T*)r1, zd);
             } else {
+                // two directions: the two-level M(r) (red-black SOR sweeps
+                // from z1, yc still holding r's coarse solution) and z1 + net(r1)
                 hipLaunchKernelGGL(k_fg_widen<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, (const T*)z2, z2d);
-                hipLaunchKernelGGL(k_fg_jacobi_add<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, coef, (const T*)z,
-                                   (const T*)r1, zd);
+                if (int rc = kmcoarse::smooth<T>(ctx, coefT, rhs, z, ctx->stream)) return rc;
+                hipLaunchKernelGGL(k_fg_widen<T>, dim3(jb), dim3(KM_BLOCK), 0, ctx->stream, n, (const T*)z, zd);
                 ndir = 2;
             }
         } else if (kind == FG_NET) {

@@ -57,8 +57,9 @@ def test_c4_iterations_with_and_without_the_network(c2_systems, dev):
                 "learned": lambda: learned.fbicgstab(coef, b, x, trained, eps=eps, mixed=True, coarse=True, gcr=True),
                 "untrained": lambda: learned.fbicgstab(coef, b, x, untrained, eps=eps, mixed=True, coarse=True,
                                                        gcr=True),
-                "no_network": lambda: learned.fbicgstab(coef, b, x, None, eps=eps, mixed=True, coarse=True, gcr=True,
-                                                        jacobi=True),
+                "no_network": lambda: learned.fbicgstab(coef, b, x, None, eps=eps, mixed=True, coarse=True, gcr=True),
+                "coarse_jacobi": lambda: learned.fbicgstab(coef, b, x, None, eps=eps, mixed=True, coarse=True,
+                                                           gcr=True, jacobi=True),
                 "twolevel_sor": lambda: learned.fbicgstab(coef, b, x, None, eps=eps, coarse=True),
             }
             for name, fn in runs.items():
