@@ -91,6 +91,8 @@ def parse():
                     help="aggregate edge of the two-level coarse level (0: automatic)")
     ap.add_argument("--totals", type=int, default=-1, choices=[-1, 0, 1],
                     help="two-level partial sums: -1 automatic, 0 every workgroup sums, 1 one workgroup first")
+    ap.add_argument("--plain-totals", type=int, default=-1, choices=[-1, 0, 1],
+                    help="plain BiCGSTAB partial sums: -1 automatic, 0 every workgroup sums, 1 one workgroup first")
     ap.add_argument("--momentum", default="auto", choices=["auto", "plain", "rbgs"],
                     help="u/v solver: plain BiCGSTAB (bicgsolver.py) or flexible BiCGSTAB with one red-black "
                          "Gauss-Seidel sweep; auto: rbgs with the FP64 two-level modes, plain otherwise")
@@ -102,6 +104,9 @@ def parse():
     ap.add_argument("--case-workers", type=int, default=3,
                     help="c3 with the two-level mode: step the cases as independent SIMPLE steps, this many at a "
                          "time on their own contexts and streams (0: km_batch_step)")
+    ap.add_argument("--case-group", type=int, default=1,
+                    help="c3 with case workers: a worker steps this many cases per call (km_simple_step_n), their "
+                         "FP64 p' solves in shared launches")
     ap.add_argument("--case-precision", choices=["per-case", "f64", "f32"], default="per-case",
                     help="c3 with case workers: each case's own precision (half the cases FP32 Krylov vectors), or "
                          "every case FP64 / FP32 vectors")
@@ -156,6 +161,7 @@ def main():
     s.mom_smooth = momentum == "rbgs"
     s.ctx.call("km_set_coarse", int(args.coarse_block))
     s.ctx.call("km_set_coarse_update", int(args.coarse_update))
+    s.ctx.call("km_set_plain_totals", int(args.plain_totals))
     if p_solver == "cg":
         s.p_solver = "cg"
     if learned:
@@ -457,7 +463,7 @@ def bench_batch(args):
     workers = args.case_workers if coarse and smooth else 0
 
     def batch_step():
-        return b.step_cases(workers) if workers > 0 else b.step(coarse=coarse, smooth=smooth)
+        return b.step_cases(workers, group=args.case_group) if workers > 0 else b.step(coarse=coarse, smooth=smooth)
 
     for _ in range(args.warmup):
         batch_step()
@@ -478,7 +484,9 @@ def bench_batch(args):
                                   f"{b.cells()} p' cells in total", "tile": None if workers else list(b.tile),
                       "p_solver": "two-level flexible BiCGSTAB per case" if coarse else "batched BiCGSTAB",
                       "momentum": "fbicgstab+rbgs per case (u, v together)" if smooth else "batched BiCGSTAB",
-                      "step": f"km_simple_step per case, {workers} at a time" if workers else "km_batch_step",
+                      "step": (f"km_simple_step_n, {args.case_group} cases per call, {workers} workers"
+                               if workers and args.case_group > 1 else
+                               f"km_simple_step per case, {workers} at a time" if workers else "km_batch_step"),
                       "fp32_vector_cases": n_fp32},
            "p_iters_min_median_max": [min(p_it), sorted(p_it)[len(p_it) // 2], max(p_it)],
            "p_cells_iters_per_s": round(cell_its / wall, 1),

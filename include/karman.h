@@ -241,6 +241,20 @@ int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* 
 int km_fbicgstab2(km_ctx* ctx, int NX0, int NY0, const double* coef0, const double* b0, double* x0, int NX1, int NY1,
                   const double* coef1, const double* b1, double* x1, double eps, int max_iter, int flags,
                   km_solve_info* info0, km_solve_info* info1);
+/* n independent km_fbicgstab solves (no network) with the same eps, max_iter
+ * and flags: results and info[k] as n separate calls in order.  With
+ * KM_FLAG_COARSE | KM_FLAG_SYMMETRIC and none of KM_FLAG_X0, KM_FLAG_MIXED,
+ * KM_FLAG_SMOOTH, KM_FLAG_GCR, KM_FLAG_JACOBI, KM_FLAG_MG (the p' systems of
+ * several cases) every system keeps its own coarse level and smoothing tile,
+ * and the first sweeps of systems of the same tile shape, up to four at a
+ * time, iterate in the same launches, each workgroup serving one system and
+ * each system stopping on its own test; otherwise the solves run one after
+ * the other.  n <= KM_MAX_GROUP_SOLVES; the systems must not share x or
+ * coefficients.  No counterpart in the reference (one case per process). */
+#define KM_MAX_GROUP_SOLVES 256
+int km_fbicgstab_n(km_ctx* ctx, int n, const int* NX, const int* NY, const double* const* coef,
+                   const double* const* b, double* const* x, double eps, int max_iter, int flags,
+                   km_solve_info* info);
 
 /* ---- batched cases (no counterpart in the reference, which runs one case) ----
  * Many independent channel cases of uneven size and Reynolds number advance
@@ -277,6 +291,15 @@ int km_batch_time_forward(km_ctx* ctx, int n, const km_case* cases);
  * eps and flags as the single solves (max_iter = NX * NY). */
 int km_simple_step(km_ctx* ctx, const km_case* c, double eps_mom, double eps_p, int mom_flags, int p_flags,
                    km_solve_info* info);
+/* km_simple_step of n cases (each case's own flags, info 3 entries per case):
+ * every case's stages up to its p' system, then the p' solves -- the cases
+ * with the same p_flags including KM_FLAG_COARSE together through
+ * km_fbicgstab_n (their first sweeps in shared launches), the others one by
+ * one -- then every case's corrections.  Each case's results are those of its
+ * own km_simple_step.  n <= KM_MAX_GROUP_SOLVES.  No counterpart in the
+ * reference (one case per process). */
+int km_simple_step_n(km_ctx* ctx, int n, const km_case* cases, double eps_mom, double eps_p, const int* mom_flags,
+                     const int* p_flags, km_solve_info* info);
 /* The batch's tile shape (chosen on its first step). */
 int km_batch_get_tile(km_ctx* ctx, int* ti, int* tj);
 
@@ -325,6 +348,13 @@ int km_twolevel_set_tile(km_ctx* ctx, int which);
  * the totals are the same bits either way.  Needs a coarse level (one
  * KM_FLAG_COARSE solve first).  No counterpart in the reference. */
 int km_set_coarse_totals(km_ctx* ctx, int mode);
+/* The same choice for the plain BiCGSTAB iteration (km_bicgstab and the
+ * mixed path's FP32 inner solves): P, S and R read the previous kernel's
+ * totals from a one-workgroup launch instead of each summing its partials.
+ * mode -1 automatic (default: R's totals for P when P's workgroups x R's
+ * partials exceed 4e6, e.g. at 4096 x 820), 0 never, 1 all three; the same
+ * bits either way.  No counterpart in the reference. */
+int km_set_plain_totals(km_ctx* ctx, int mode);
 /* Launch schedule of the fused two-level iteration: captured graphs of `big`
  * iterations (a multiple of 4; 0: 8) until `lead` iterations before the
  * shortest of the last eight solves of this system shape, then graphs of

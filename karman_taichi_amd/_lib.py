@@ -96,6 +96,8 @@ SIGNATURES = {
     "km_fbicgstab2": [_CTX, _I, _I, _P, _P, _P, _I, _I, _P, _P, _P, _D, _I, _I, _INFO, _INFO],
     "km_batch_step": [_CTX, _I, _P, _D, _D, _I, _I, _P],
     "km_simple_step": [_CTX, _P, _D, _D, _I, _I, _P],
+    "km_simple_step_n": [_CTX, _I, _P, _D, _D, _P, _P, _P],
+    "km_fbicgstab_n": [_CTX, _I, _P, _P, _P, _P, _P, _D, _I, _I, _P],
     "km_batch_time_forward": [_CTX, _I, _P],
     "km_batch_get_tile": [_CTX, ctypes.POINTER(_I), ctypes.POINTER(_I)],
     "km_set_coarse": [_CTX, _I],
@@ -107,6 +109,7 @@ SIGNATURES = {
     "km_twolevel_tile_threads": [_CTX, ctypes.POINTER(_I)],
     "km_twolevel_set_tile": [_CTX, _I],
     "km_set_coarse_totals": [_CTX, _I],
+    "km_set_plain_totals": [_CTX, _I],
     "km_set_coarse_chunks": [_CTX, _I, _I, _I],
 }
 _RESTYPE = {"km_last_error": ctypes.c_char_p}

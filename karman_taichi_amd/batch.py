@@ -67,15 +67,18 @@ class BatchedSIMPLE:
                       self._info)
         return [tuple(SolveInfo.from_c(self._info[3 * k + f]) for f in range(3)) for k in range(self.n)]
 
-    def step_cases(self, workers: int = 4, eps_mom: float = 1e-8, eps_p: float = 1e-8):
+    def step_cases(self, workers: int = 4, eps_mom: float = 1e-8, eps_p: float = 1e-8, group: int = 1):
         """One outer iteration of every case as independent ``SIMPLESolver.step``
         calls (``km_simple_step``: smoothed u/v pair, two-level p'), ``workers``
         cases at a time, each worker with its own context and stream, so that
         the latency-bound solves of different cases overlap on the GPU.  Each
         case runs in its own precision (``Case.mixed``: FP32 Krylov vectors
         with FP64 scalars and refinement).  Case k always runs on worker k mod
-        ``workers`` (its coarse level stays there).  Returns [(u, v, p')
-        SolveInfo] per case."""
+        ``workers`` (its coarse level stays there).  group > 1: a worker steps
+        its cases ``group`` at a time with ``SIMPLESolver.step_group``
+        (km_simple_step_n), whose FP64 p' solves share launches; each case's
+        result is that of its own step.  Returns [(u, v, p') SolveInfo] per
+        case."""
         import concurrent.futures
         main = torch.cuda.current_stream(self.device)
         if getattr(self, "_workers", None) is None or len(self._workers) != workers:
@@ -91,19 +94,24 @@ class BatchedSIMPLE:
         def run(w):
             st, ctx = self._workers[w]
             out = {}
+            mine = list(range(w, self.n, workers))
             with torch.cuda.stream(st):
-                for k in range(w, self.n, workers):
-                    s = self.solvers[k]
-                    saved = (s.ctx, s.mom_smooth, s.p_coarse, s.eps_mom, s.eps_p, s.mixed)
+                for g0 in range(0, len(mine), max(1, group)):
+                    ks = mine[g0:g0 + max(1, group)]
+                    ss = [self.solvers[k] for k in ks]
+                    saved = [(s.ctx, s.mom_smooth, s.p_coarse, s.eps_mom, s.eps_p, s.mixed) for s in ss]
                     try:
-                        s.ctx = ctx
-                        s.mom_smooth = s.p_coarse = True
-                        s.eps_mom, s.eps_p = eps_mom, eps_p
-                        s.mixed = bool(self.cases[k].mixed)
-                        last = s.step()
+                        for k, s in zip(ks, ss):
+                            s.ctx = ctx
+                            s.mom_smooth = s.p_coarse = True
+                            s.eps_mom, s.eps_p = eps_mom, eps_p
+                            s.mixed = bool(self.cases[k].mixed)
+                        lasts = SIMPLESolver.step_group(ss, ctx) if group > 1 else [ss[0].step()]
                     finally:
-                        s.ctx, s.mom_smooth, s.p_coarse, s.eps_mom, s.eps_p, s.mixed = saved
-                    out[k] = (last["u"], last["v"], last["p"])
+                        for s, sv in zip(ss, saved):
+                            s.ctx, s.mom_smooth, s.p_coarse, s.eps_mom, s.eps_p, s.mixed = sv
+                    for k, last in zip(ks, lasts):
+                        out[k] = (last["u"], last["v"], last["p"])
             return out
 
         futures = [self._pool.submit(run, w) for w in range(workers)]

@@ -101,6 +101,20 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
                 bool use_graph, KState* out, const std::function<int()>* tail = nullptr, bool sym = false,
                 const double* coef64 = nullptr);
 
+// Group solves: up to kGroupSystems independent symmetric FP64 systems of
+// fused_inner (from x = 0), each on its own coarse level C[k] (set up by the
+// caller; ctx->coarse is left on the last), all with smoothing tile `tile`,
+// iterated in the same launches until every one stops; system k keeps its
+// state in ctx->st[4 + k] and out[k] receives it.  Each system's arithmetic
+// is that of its own fused_inner with the same tile.
+constexpr int kGroupSystems = 4;
+int fused_inner_n(km_ctx* ctx, int n, km_coarse* const* C, const int* NX, const int* NY, const double* const* coef,
+                  const double* const* rhs, double* const* xd, double* const* ws, double eps, const int* max_iter,
+                  bool use_graph, KState* out, const std::function<int()>* tail, int tile);
+// The smoothing tile of level C's symmetric FP64 system (timed on its first
+// use; ws: 5n scratch values); leaves ctx->coarse = C.
+int group_tile(km_ctx* ctx, km_coarse* C, const double* coef, const double* rhs, double* ws, int* tile);
+
 // Inner flexible BiCGSTAB for A d = rhs with one red-black Gauss-Seidel sweep
 // as the preconditioner and no coarse level (the momentum systems), three
 // launches per iteration; ws holds 8n values of T.

@@ -61,6 +61,10 @@ struct KState {
 #define KM_PB (2 * KM_MAX_RED)           // [4][KM_MAX_RED]
 #define KM_PC(par) (6 * KM_MAX_RED + (par) * 2 * KM_MAX_RED)   // [2][KM_MAX_RED], double-buffered
 #define KM_PX (10 * KM_MAX_RED)          // helpers (dot, residual, mdiv): [2][KM_MAX_RED]
+// The plain iteration's totals when a one-workgroup kernel sums a launch's
+// partials once for all consumer workgroups (KM_PA's second plane):
+// <r~,v> | <t,s>, <t,t> | <r,r>, <r,r~>
+#define KM_PT (KM_PA + KM_MAX_RED)
 
 // Geometry of a 2-D tiled stencil launch: TI rows x TJ columns per tile,
 // tpb consecutive tiles per workgroup, G workgroups (<= KM_MAX_RED).
@@ -142,6 +146,7 @@ struct km_ctx {
     std::map<std::tuple<int, int, int>, std::pair<int, int>> tile_override;
     std::map<GraphKey, hipGraphExec_t> graphs;
     int autotune = 1;              // time tile candidates on the first solve of a shape
+    int plain_totals = -1;         // km_set_plain_totals: -1 by size, 0 never, 1 always
     std::set<std::tuple<int, int, int>> tuned;
     km_net* net = nullptr;
     km_batch* batch = nullptr;

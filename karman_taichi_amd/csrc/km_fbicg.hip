@@ -24,6 +24,7 @@
 #include "km_coarse.h"
 #include "km_mg.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace {
@@ -430,7 +431,9 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         if (int rc = kmnet::prepare(ctx, 1, NX, NY, params)) return rc;
         if (int rc = kmnet::solve_stats(ctx, NX, NY, c32, ctx->stream)) return rc;
     }
-    if (coarse)
+    // a first sweep run by a group solve (pre) already set the level up and
+    // left it in ctx->coarse
+    if (coarse && !pre)
         if (int rc = kmcoarse::setup(ctx, NX, NY, coef, ctx->stream, true)) return rc;
     double bb = 0.0, bnorm = 0.0, rr = 0.0, rel = 0.0;
     int total = 0, sweeps = 0, status = KM_MAXITER;
@@ -1017,9 +1020,125 @@ int fgcr_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b,
     return KM_OK;
 }
 
+// n symmetric FP64 two-level solves from x = 0 (the p' systems of several
+// cases): each system's coarse level set up and its smoothing tile chosen as
+// its own solve would, then the systems of one tile shape, in groups of up to
+// kmcoarse::kGroupSystems, run their first sweeps in shared launches
+// (fused_inner_n), and each system's own continuation (further sweeps, the
+// stall fallback) runs as in fbicg_solve.  mi: each system's iteration cap.
+int fbicg_group(km_ctx* ctx, int n, const int* NX, const int* NY, const double* const* coef, const double* const* b,
+                double* const* x, double eps, const int* mi, int flags, km_solve_info* info) {
+    auto up = [](int64_t v) { return (size_t)((v * (int64_t)sizeof(double) + 255) & ~(int64_t)255); };
+    // the first kGroupSystems systems' FP64 true residuals and 8n inner
+    // workspaces, reused by each group in turn (a group takes at most that many)
+    std::vector<int64_t> cells(n);
+    for (int k = 0; k < n; ++k) cells[k] = (int64_t)NX[k] * NY[k];
+    std::vector<int64_t> big(cells);
+    std::sort(big.begin(), big.end(), std::greater<int64_t>());
+    size_t bytes = 0;
+    for (int k = 0; k < n && k < kmcoarse::kGroupSystems; ++k) bytes += up(big[k]) + up(8 * big[k]);
+    for (int k = 0; k < n; ++k) bytes = std::max(bytes, fbicg_bytes<double>(cells[k]));
+    if (int rc = km_ensure_workspace(ctx, bytes)) return rc;
+    char* V = static_cast<char*>(km_ws_vectors(ctx));
+    // every level set up (in system order, as separate solves would) and its tile chosen
+    std::vector<km_coarse*> C(n);
+    std::vector<int> tile(n);
+    for (int k = 0; k < n; ++k) {
+        if (int rc = kmcoarse::setup(ctx, NX[k], NY[k], coef[k], ctx->stream, true)) return rc;
+        C[k] = ctx->coarse;
+        double* scratch = reinterpret_cast<double*>(V + up(cells[k]));
+        if (int rc = kmcoarse::group_tile(ctx, C[k], coef[k], b[k], scratch, &tile[k])) return rc;
+    }
+    const bool use_graph = !(flags & KM_FLAG_NO_GRAPH);
+    std::vector<char> taken(n, 0);
+    for (int a = 0; a < n; ++a) {
+        if (taken[a]) continue;
+        int g[kmcoarse::kGroupSystems];
+        int m = 0;
+        for (int c = a; c < n && m < kmcoarse::kGroupSystems; ++c)
+            if (!taken[c] && tile[c] == tile[a]) {
+                taken[c] = 1;
+                g[m++] = c;
+            }
+        km_coarse* gC[kmcoarse::kGroupSystems];
+        int gNX[kmcoarse::kGroupSystems], gNY[kmcoarse::kGroupSystems], gmi[kmcoarse::kGroupSystems];
+        const double* gcoef[kmcoarse::kGroupSystems];
+        const double* gb[kmcoarse::kGroupSystems];
+        double* gx[kmcoarse::kGroupSystems];
+        double* gr[kmcoarse::kGroupSystems];
+        double* gw[kmcoarse::kGroupSystems];
+        char* q = V;
+        for (int j = 0; j < m; ++j) {
+            const int k = g[j];
+            gC[j] = C[k];
+            gNX[j] = NX[k];
+            gNY[j] = NY[k];
+            gmi[j] = mi[k];
+            gcoef[j] = coef[k];
+            gb[j] = b[k];
+            gx[j] = x[k];
+            gr[j] = reinterpret_cast<double*>(q);
+            q += up(cells[k]);
+            gw[j] = reinterpret_cast<double*>(q);
+            q += up(8 * cells[k]);
+        }
+        KState st[kmcoarse::kGroupSystems];
+        // the true residuals are enqueued behind the solves and read with their states
+        const std::function<int()> tail = [&]() {
+            for (int j = 0; j < m; ++j)
+                if (int rc = km_residual_async(ctx, gNX[j], gNY[j], gcoef[j], gb[j], gx[j], gr[j], j)) return rc;
+            return (int)KM_OK;
+        };
+        if (int rc = kmcoarse::fused_inner_n(ctx, m, gC, gNX, gNY, gcoef, gb, gx, gw, 0.5 * eps, gmi, use_graph, st,
+                                             &tail, tile[a]))
+            return rc;
+        double rr[kmcoarse::kGroupSystems];
+        for (int j = 0; j < m; ++j) rr[j] = km_mapped_res(ctx, j);   // a continuation may reuse the slots
+        for (int j = 0; j < m; ++j) {
+            const int k = g[j];
+            ctx->coarse = C[k];
+            if (int rc = fbicg_solve<double>(ctx, NX[k], NY[k], coef[k], b[k], x[k], eps, mi[k], flags, nullptr,
+                                             info ? &info[k] : nullptr, &st[j], rr[j]))
+                return rc;
+        }
+    }
+    return KM_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int km_fbicgstab_n(km_ctx* ctx, int n, const int* NX, const int* NY, const double* const* coef,
+                   const double* const* b, double* const* x, double eps, int max_iter, int flags,
+                   km_solve_info* info) {
+    if (!ctx) return KM_EINVAL;
+    KM_REQUIRE(ctx, n >= 0 && n <= KM_MAX_GROUP_SOLVES, "%d systems (at most %d)", n, KM_MAX_GROUP_SOLVES);
+    if (n == 0) return KM_OK;
+    KM_REQUIRE(ctx, NX && NY && coef && b && x, "null pointer");
+    KM_REQUIRE(ctx, eps > 0.0, "eps must be positive");
+    std::vector<int> mi(n);
+    for (int k = 0; k < n; ++k) {
+        KM_REQUIRE(ctx, coef[k] && b[k] && x[k], "system %d: null pointer", k);
+        KM_REQUIRE(ctx, NX[k] >= 4 && NY[k] >= 4 && (int64_t)NX[k] * NY[k] < ((int64_t)1 << 31),
+                   "system %d shape %d x %d", k, NX[k], NY[k]);
+        for (int j = 0; j < k; ++j)
+            KM_REQUIRE(ctx, x[j] != x[k] && coef[j] != coef[k], "systems %d and %d share x or coefficients", j, k);
+        const int64_t cells = (int64_t)NX[k] * NY[k];
+        mi[k] = max_iter > 0 ? max_iter : (int)(cells < (1 << 30) ? cells : (1 << 30));
+    }
+    const bool group = (flags & KM_FLAG_COARSE) && (flags & KM_FLAG_SYMMETRIC) &&
+                       !(flags & (KM_FLAG_X0 | KM_FLAG_MIXED | KM_FLAG_SMOOTH | KM_FLAG_GCR | KM_FLAG_JACOBI |
+                                  KM_FLAG_MG));
+    if (!group || n == 1) {
+        for (int k = 0; k < n; ++k)
+            if (int rc = km_fbicgstab(ctx, NX[k], NY[k], coef[k], b[k], x[k], eps, max_iter, flags, nullptr,
+                                      info ? &info[k] : nullptr))
+                return rc;
+        return KM_OK;
+    }
+    return fbicg_group(ctx, n, NX, NY, coef, b, x, eps, mi.data(), flags, info);
+}
 
 int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
                  int max_iter, int flags, const float* params, km_solve_info* info) {

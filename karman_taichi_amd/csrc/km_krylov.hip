@@ -321,10 +321,12 @@ struct TileLoads {
 
 // L: the workgroup's logical index within its system (0 .. tl.G-1); the
 // workgroup with L == 0 records the iteration's scalars.  Gr: R's workgroups.
+// tot: R's totals summed once (k_bicg_total at P + KM_PT), or null: every
+// workgroup sums R's partials itself (the same bits either way).
 template <typename T, int CPT>
 __device__ __forceinline__ void bicg_p_body(const Tile& tl, int L, int Gr, int phase, const T* __restrict__ coef,
                                             const T* __restrict__ r, const T* __restrict__ rt, T* pbuf, T* vbuf,
-                                            KState* st, double* __restrict__ P) {
+                                            KState* st, double* __restrict__ P, const double* tot = nullptr) {
     T* lds = reinterpret_cast<T*>(km_smem);
     __shared__ double sh[4];
     const int64_t N = (int64_t)tl.NX * tl.NY;
@@ -356,7 +358,12 @@ __device__ __forceinline__ void bicg_p_body(const Tile& tl, int L, int Gr, int p
         for (int q = 0; q < CPT; ++q) rtv[q] = ld.cval[q] ? rt[ld.ck[q]] : T(0);
         if (tile == t0) {
             double pr[2];
-            sum_partials<2>(P + KM_PC(0), Gr, pr, sh);
+            if (tot) {
+                pr[0] = tot[3];
+                pr[1] = tot[4];
+            } else {
+                sum_partials<2>(P + KM_PC(0), Gr, pr, sh);
+            }
             const double rTr = pr[0], rho = pr[1];   // <r_i, r_i>, <r_i, r~> (bicgsolver.py:95)
             const double r0 = i == 0 ? (st->ref > 0.0 ? st->ref : rTr) : st->r0;
             int status = -1;
@@ -408,7 +415,8 @@ __device__ __forceinline__ void bicg_p_body(const Tile& tl, int L, int Gr, int p
 template <typename T, int CPT>
 __device__ __forceinline__ void bicg_s_body(const Tile& tl, int L, int phase, const T* __restrict__ coef,
                                             const T* __restrict__ r, const T* vbuf, T* __restrict__ s,
-                                            T* __restrict__ t, KState* st, double* __restrict__ P) {
+                                            T* __restrict__ t, KState* st, double* __restrict__ P,
+                                            const double* tot = nullptr) {
     T* lds = reinterpret_cast<T*>(km_smem);
     __shared__ double sh[8];
     const int64_t N = (int64_t)tl.NX * tl.NY;
@@ -430,7 +438,8 @@ __device__ __forceinline__ void bicg_s_body(const Tile& tl, int L, int phase, co
         ld.load_coef(coef, N);
         if (tile == t0) {
             double pa[1];
-            sum_partials<1>(P + KM_PA, tl.G, pa, sh);
+            if (tot) pa[0] = tot[0];
+            else sum_partials<1>(P + KM_PA, tl.G, pa, sh);
             const double rv = pa[0];
             const double rho = st->ring[phase].rho;
             int status = -1;
@@ -474,7 +483,7 @@ template <typename T>
 __device__ __forceinline__ void bicg_r_body(const Flat& fl, int bx, int Gs, int phase, const T* __restrict__ rt,
                                             const T* __restrict__ s, const T* __restrict__ t, const T* pbuf,
                                             T* __restrict__ x, T* __restrict__ r, KState* st,
-                                            double* __restrict__ P) {
+                                            double* __restrict__ P, const double* tot = nullptr) {
     __shared__ double sh[8];
     const T* __restrict__ p = pbuf + (1 - (phase & 1)) * fl.n;   // p_i, written by P(i)
     const int64_t k0 = (int64_t)bx * fl.chunk;
@@ -496,7 +505,12 @@ __device__ __forceinline__ void bicg_r_body(const Flat& fl, int bx, int Gs, int 
     };
     load(k0);
     double pb[2];
-    sum_partials<2>(P + KM_PB, Gs, pb, sh);
+    if (tot) {
+        pb[0] = tot[1];
+        pb[1] = tot[2];
+    } else {
+        sum_partials<2>(P + KM_PB, Gs, pb, sh);
+    }
     const double omega = pb[1] != 0.0 ? pb[0] / pb[1] : 0.0;   // bicgsolver.py:118
     const double alpha = st->ring[phase].alpha;
     if (bx == 0 && threadIdx.x == 0) st->ring[phase].omega = omega;
@@ -520,25 +534,40 @@ __device__ __forceinline__ void bicg_r_body(const Flat& fl, int bx, int Gs, int 
 template <typename T, int CPT>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_bicg_p(Tile tl, int Gr, int phase, const T* __restrict__ coef, const T* __restrict__ r, const T* __restrict__ rt,
-         T* pbuf, T* vbuf, KState* st, double* __restrict__ P) {
+         T* pbuf, T* vbuf, KState* st, double* __restrict__ P, const double* tot) {
     if (st->done) return;
-    bicg_p_body<T, CPT>(tl, xcd_remap(blockIdx.x, tl.G), Gr, phase, coef, r, rt, pbuf, vbuf, st, P);
+    bicg_p_body<T, CPT>(tl, xcd_remap(blockIdx.x, tl.G), Gr, phase, coef, r, rt, pbuf, vbuf, st, P, tot);
 }
 
 template <typename T, int CPT>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_bicg_s(Tile tl, int phase, const T* __restrict__ coef, const T* __restrict__ r, const T* vbuf, T* __restrict__ s,
-         T* __restrict__ t, KState* st, double* __restrict__ P) {
+         T* __restrict__ t, KState* st, double* __restrict__ P, const double* tot) {
     if (st->done) return;
-    bicg_s_body<T, CPT>(tl, xcd_remap(blockIdx.x, tl.G), phase, coef, r, vbuf, s, t, st, P);
+    bicg_s_body<T, CPT>(tl, xcd_remap(blockIdx.x, tl.G), phase, coef, r, vbuf, s, t, st, P, tot);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(KM_BLOCK)
 k_bicg_r(Flat fl, int Gs, int phase, const T* __restrict__ rt, const T* __restrict__ s, const T* __restrict__ t,
-         const T* pbuf, T* __restrict__ x, T* __restrict__ r, KState* st, double* __restrict__ P) {
+         const T* pbuf, T* __restrict__ x, T* __restrict__ r, KState* st, double* __restrict__ P, const double* tot) {
     if (st->done) return;
-    bicg_r_body<T>(fl, blockIdx.x, Gs, phase, rt, s, t, pbuf, x, r, st, P);
+    bicg_r_body<T>(fl, blockIdx.x, Gs, phase, rt, s, t, pbuf, x, r, st, P, tot);
+}
+
+// One launch's NV partial planes (G workgroups each, plane stride KM_MAX_RED)
+// summed once, in sum_partials' order, for the next launch's workgroups
+// (which would each sum them otherwise).  A finished solve skips it.
+template <int NV>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_bicg_total(const double* __restrict__ P, int G, double* __restrict__ out, const KState* __restrict__ st) {
+    if (st->done) return;
+    __shared__ double sh[4];
+    double t[NV];
+    sum_partials<NV>(P, G, t, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = t[q];
 }
 
 // ------------------------------------------------------------ batched systems
@@ -628,50 +657,86 @@ inline int tile_cpt(const Tile& tl) {
         else hipLaunchKernelGGL(kern, dim3(g), dim3(KM_BLOCK), l, s, __VA_ARGS__);                    \
     } while (0)
 
-// One BiCGSTAB iteration (P, S, R).  `timed` (a KM_KERNEL_BICG_* id, or -1)
-// selects the launch bracketed by e0 / e1.
+// Partial sets of the plain iteration summed once by k_bicg_total (P's for
+// S, S's for R, R's and the init kernel's for P) instead of by every consumer
+// workgroup.  mode: km_set_plain_totals (-1 by size, 0 never, 1 all three).
+// By size only R's for P, where P's workgroups x 2 R planes exceed
+// kPlainTotalWork: at 4096 x 820 (1645 P workgroups, 2 x 2048 R partials
+// each) P takes 56.6 instead of 62.3 us and the extra launch ~2.8 us; S and R
+// each gain ~1.3 us, less than a launch.
+constexpr int64_t kPlainTotalWork = 4000000;
+constexpr int PT_P = 1, PT_S = 2, PT_R = 4;
+
+int plain_fin(int mode, const Tile& tl, const Flat& fl) {
+    if (mode == 0) return 0;
+    if (mode > 0) return PT_P | PT_S | PT_R;
+    return (int64_t)tl.G * 2 * fl.G > kPlainTotalWork ? PT_R : 0;
+}
+
+// R's (or the init kernel's) partials totalled for the next P
+void launch_total_r(const Flat& fl, int fin, const KState* st, double* P, hipStream_t s) {
+    if (fin & PT_R)
+        hipLaunchKernelGGL(k_bicg_total<2>, dim3(1), dim3(KM_BLOCK), 0, s, (const double*)(P + KM_PC(0)), fl.G,
+                           P + KM_PT + 3, st);
+}
+
+// One BiCGSTAB iteration (P, S, R, and the totals fin asks for).  `timed` (a
+// KM_KERNEL_BICG_* id, or -1) selects the launch bracketed by e0 / e1.
 template <typename T>
 void launch_bicg_iter(const Tile& tl, const Flat& fl, int phase, const T* coef, T* r, const T* rt, T* sv, T* t, T* x,
-                      T* pbuf, T* vbuf, KState* st, double* P, size_t lds, hipStream_t s, int timed = -1,
+                      T* pbuf, T* vbuf, KState* st, double* P, size_t lds, hipStream_t s, int fin, int timed = -1,
                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    const double* tp = fin & PT_R ? P + KM_PT : nullptr;
+    const double* ts = fin & PT_P ? P + KM_PT : nullptr;
+    const double* tr = fin & PT_S ? P + KM_PT : nullptr;
     hipEvent_t a0 = timed == KM_KERNEL_BICG_A ? e0 : nullptr, a1 = timed == KM_KERNEL_BICG_A ? e1 : nullptr;
     switch (tile_cpt(tl)) {
-        case 1: KM_LAUNCH((k_bicg_p<T, 1>), tl.G, lds, s, a0, a1, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P); break;
-        case 2: KM_LAUNCH((k_bicg_p<T, 2>), tl.G, lds, s, a0, a1, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P); break;
-        default: KM_LAUNCH((k_bicg_p<T, 4>), tl.G, lds, s, a0, a1, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P);
+        case 1: KM_LAUNCH((k_bicg_p<T, 1>), tl.G, lds, s, a0, a1, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P, tp); break;
+        case 2: KM_LAUNCH((k_bicg_p<T, 2>), tl.G, lds, s, a0, a1, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P, tp); break;
+        default: KM_LAUNCH((k_bicg_p<T, 4>), tl.G, lds, s, a0, a1, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P, tp);
     }
+    if (fin & PT_P)
+        hipLaunchKernelGGL(k_bicg_total<1>, dim3(1), dim3(KM_BLOCK), 0, s, (const double*)(P + KM_PA), tl.G,
+                           P + KM_PT, (const KState*)st);
     a0 = timed == KM_KERNEL_BICG_B ? e0 : nullptr;
     a1 = timed == KM_KERNEL_BICG_B ? e1 : nullptr;
     switch (tile_cpt(tl)) {
-        case 1: KM_LAUNCH((k_bicg_s<T, 1>), tl.G, lds, s, a0, a1, tl, phase, coef, r, vbuf, sv, t, st, P); break;
-        case 2: KM_LAUNCH((k_bicg_s<T, 2>), tl.G, lds, s, a0, a1, tl, phase, coef, r, vbuf, sv, t, st, P); break;
-        default: KM_LAUNCH((k_bicg_s<T, 4>), tl.G, lds, s, a0, a1, tl, phase, coef, r, vbuf, sv, t, st, P);
+        case 1: KM_LAUNCH((k_bicg_s<T, 1>), tl.G, lds, s, a0, a1, tl, phase, coef, r, vbuf, sv, t, st, P, ts); break;
+        case 2: KM_LAUNCH((k_bicg_s<T, 2>), tl.G, lds, s, a0, a1, tl, phase, coef, r, vbuf, sv, t, st, P, ts); break;
+        default: KM_LAUNCH((k_bicg_s<T, 4>), tl.G, lds, s, a0, a1, tl, phase, coef, r, vbuf, sv, t, st, P, ts);
     }
+    if (fin & PT_S)
+        hipLaunchKernelGGL(k_bicg_total<2>, dim3(1), dim3(KM_BLOCK), 0, s, (const double*)(P + KM_PB), tl.G,
+                           P + KM_PT + 1, (const KState*)st);
     a0 = timed == KM_KERNEL_BICG_R ? e0 : nullptr;
     a1 = timed == KM_KERNEL_BICG_R ? e1 : nullptr;
     KM_LAUNCH((k_bicg_r<T>), fl.G, 0, s, a0, a1, fl, tl.G, phase, rt, (const T*)sv, (const T*)t, (const T*)pbuf, x, r,
-              st, P);
+              st, P, tr);
+    launch_total_r(fl, fin, st, P, s);
 }
 
 // A single kernel of an iteration (back-to-back timing).
 template <typename T>
 void launch_bicg_one(int which, const Tile& tl, const Flat& fl, int phase, const T* coef, T* r, const T* rt, T* sv,
-                     T* t, T* x, T* pbuf, T* vbuf, KState* st, double* P, size_t lds, hipStream_t s) {
+                     T* t, T* x, T* pbuf, T* vbuf, KState* st, double* P, size_t lds, hipStream_t s, int fin) {
+    const double* tp = fin & PT_R ? P + KM_PT : nullptr;
+    const double* ts = fin & PT_P ? P + KM_PT : nullptr;
+    const double* tr = fin & PT_S ? P + KM_PT : nullptr;
     if (which == KM_KERNEL_BICG_A) {
         switch (tile_cpt(tl)) {
-            case 1: hipLaunchKernelGGL((k_bicg_p<T, 1>), tl.G, KM_BLOCK, lds, s, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P); break;
-            case 2: hipLaunchKernelGGL((k_bicg_p<T, 2>), tl.G, KM_BLOCK, lds, s, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P); break;
-            default: hipLaunchKernelGGL((k_bicg_p<T, 4>), tl.G, KM_BLOCK, lds, s, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P);
+            case 1: hipLaunchKernelGGL((k_bicg_p<T, 1>), tl.G, KM_BLOCK, lds, s, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P, tp); break;
+            case 2: hipLaunchKernelGGL((k_bicg_p<T, 2>), tl.G, KM_BLOCK, lds, s, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P, tp); break;
+            default: hipLaunchKernelGGL((k_bicg_p<T, 4>), tl.G, KM_BLOCK, lds, s, tl, fl.G, phase, coef, r, rt, pbuf, vbuf, st, P, tp);
         }
     } else if (which == KM_KERNEL_BICG_B) {
         switch (tile_cpt(tl)) {
-            case 1: hipLaunchKernelGGL((k_bicg_s<T, 1>), tl.G, KM_BLOCK, lds, s, tl, phase, coef, r, vbuf, sv, t, st, P); break;
-            case 2: hipLaunchKernelGGL((k_bicg_s<T, 2>), tl.G, KM_BLOCK, lds, s, tl, phase, coef, r, vbuf, sv, t, st, P); break;
-            default: hipLaunchKernelGGL((k_bicg_s<T, 4>), tl.G, KM_BLOCK, lds, s, tl, phase, coef, r, vbuf, sv, t, st, P);
+            case 1: hipLaunchKernelGGL((k_bicg_s<T, 1>), tl.G, KM_BLOCK, lds, s, tl, phase, coef, r, vbuf, sv, t, st, P, ts); break;
+            case 2: hipLaunchKernelGGL((k_bicg_s<T, 2>), tl.G, KM_BLOCK, lds, s, tl, phase, coef, r, vbuf, sv, t, st, P, ts); break;
+            default: hipLaunchKernelGGL((k_bicg_s<T, 4>), tl.G, KM_BLOCK, lds, s, tl, phase, coef, r, vbuf, sv, t, st, P, ts);
         }
     } else {
         hipLaunchKernelGGL((k_bicg_r<T>), fl.G, KM_BLOCK, 0, s, fl, tl.G, phase, rt, (const T*)sv, (const T*)t,
-                           (const T*)pbuf, x, r, st, P);
+                           (const T*)pbuf, x, r, st, P, tr);
     }
 }
 
@@ -823,14 +888,16 @@ int bicgstab_typed(km_ctx* ctx, const Tile& tl, const T* coef, const T* b, T* x,
     if (!P) P = ctx->partials;
     hipLaunchKernelGGL(k_bicg_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, rt, pbuf, vbuf,
                        from_r ? 1 : 0, eps, max_iter, ref_norm2, st, P);
+    const int fin = plain_fin(ctx->plain_totals, tl, fl);
+    launch_total_r(fl, fin, st, P, ctx->stream);
     KM_CHECK_LAUNCH(ctx);
     const size_t lds = tile_lds<T>(tl);
     auto enqueue = [=](hipStream_t s_, int phase) {
-        launch_bicg_iter<T>(tl, fl, phase, coef, r, rt, s, t, x, pbuf, vbuf, st, P, lds, s_);
+        launch_bicg_iter<T>(tl, fl, phase, coef, r, rt, s, t, x, pbuf, vbuf, st, P, lds, s_, fin);
     };
     GraphKey key;
     key.v = {1, sizeof(T), (uintptr_t)tl.NX, (uintptr_t)tl.NY, (uintptr_t)tl.TI, (uintptr_t)tl.TJ,
-             (uintptr_t)coef, (uintptr_t)x, (uintptr_t)ws, (uintptr_t)st, (uintptr_t)P};
+             (uintptr_t)coef, (uintptr_t)x, (uintptr_t)ws, (uintptr_t)st, (uintptr_t)P, (uintptr_t)fin};
     return run_chunks(ctx, st, key, use_graph, kChunk, max_iter, enqueue, out);
 }
 
@@ -938,12 +1005,14 @@ int autotune_tiles(km_ctx* ctx, int NX, int NY, int prec, const T* coef, const T
     for (const auto& c : cand) {
         const Tile tl = make_tile(NX, NY, c.first, c.second);
         const size_t lds = tile_lds<T>(tl);
+        const int fin = plain_fin(ctx->plain_totals, tl, fl);
         hipLaunchKernelGGL(k_bicg_init<T>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, rt, pbuf, vbuf, 0,
                            0.0, 1 << 30, 0.0, st, P);
+        launch_total_r(fl, fin, st, P, ctx->stream);
         for (int it = 0; it < 20; ++it) {
             if (it == 4) (void)hipEventRecord(e0, ctx->stream);
             const int ph = it % KM_RING;
-            launch_bicg_iter<T>(tl, fl, ph, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream);
+            launch_bicg_iter<T>(tl, fl, ph, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream, fin);
         }
         (void)hipEventRecord(e1, ctx->stream);
         KM_HIP_SYNC(ctx, hipEventSynchronize(e1));
@@ -1097,6 +1166,12 @@ int batch_bicgstab(km_ctx* ctx, int n, const BSys* sys, const BSys* d_sys, const
     return KM_OK;
 }
 
+int km_set_plain_totals(km_ctx* ctx, int mode) {
+    if (!ctx) return KM_EINVAL;
+    ctx->plain_totals = mode < 0 ? -1 : (mode > 0 ? 1 : 0);
+    return KM_OK;
+}
+
 // Tile choice: an explicit override, else the cached choice, else a default
 // that splits each row into near-equal segments of at most 256 columns and
 // stacks rows until a tile covers about 256 cells.
@@ -1172,15 +1247,18 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
         for (auto& e : ev)
             if (e) { KmHostLock lk_(km_host_mutex()); (void)hipEventDestroy(e); }
     };
+    const int fin = plain_fin(ctx->plain_totals, tl, fl);
     hipLaunchKernelGGL(k_bicg_init<double>, dim3(fl.G), dim3(KM_BLOCK), 0, ctx->stream, fl, b, x, r, rt, pbuf, vbuf,
                        0, 0.0, 1 << 30, 0.0, st, P);
+    launch_total_r(fl, fin, st, P, ctx->stream);
     const int warm = 8;
     if (b2b) {
         // reach a mid-solve state, then time `reps` identical launches of the
         // kernel between two events: P of the next iteration, or S / R of
         // the last one (inputs unchanged from launch to launch)
         for (int it = 0; it < warm; ++it)
-            launch_bicg_iter<double>(tl, fl, it % KM_RING, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream);
+            launch_bicg_iter<double>(tl, fl, it % KM_RING, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream,
+                                     fin);
         KM_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
         for (int k = 0; k < reps; ++k) {
             if (which == KM_KERNEL_STENCIL)
@@ -1188,7 +1266,7 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
                                    (const double*)r, t);
             else
                 launch_bicg_one<double>(which, tl, fl, which == KM_KERNEL_BICG_A ? warm % KM_RING : (warm - 1) % KM_RING,
-                                        coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream);
+                                        coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream, fin);
         }
         KM_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
     }
@@ -1201,7 +1279,8 @@ int km_time_kernel(km_ctx* ctx, int which, int NX, int NY, const double* coef, c
             KM_LAUNCH((k_stencil<double>), tl.G, lds, ctx->stream, e0, e1, tl, coef, (const double*)r, t);
             continue;
         }
-        launch_bicg_iter<double>(tl, fl, ph, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream, which, e0, e1);
+        launch_bicg_iter<double>(tl, fl, ph, coef, r, rt, sv, t, x, pbuf, vbuf, st, P, lds, ctx->stream, fin, which, e0,
+                                 e1);
     }
     hipError_t le = hipGetLastError();
     hipError_t se;

@@ -794,13 +794,15 @@ __device__ __forceinline__ double row_dot(const double* __restrict__ a, int m, i
 
 // K_CV: the stopping test on <r,r> and alpha (every workgroup sums the
 // partials in the same order), then Y(v) = A_c^-1 P^T v and Y(s) = Y(r) -
-// alpha Y(v) on coarse row blockIdx.x * 4 + wave.  Y holds Y(r), Y(p), Y(v), Y(s).
-__global__ void __launch_bounds__(KM_BLOCK)
-k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict__ inv, const double* __restrict__ rcs,
-        double* __restrict__ Y, KState* __restrict__ st, const double* __restrict__ P, int* flag) {
+// alpha Y(v) on coarse row bx * 4 + wave (bx: the workgroup's index within
+// its system).  Y holds Y(r), Y(p), Y(v), Y(s).
+__device__ __forceinline__ void cv_body(int bx, int m, int S, int Gt, int Gf, int tot, int ph,
+                                        const double* __restrict__ inv, const double* __restrict__ rcs,
+                                        double* __restrict__ Y, KState* __restrict__ st,
+                                        const double* __restrict__ P, int* flag) {
     extern __shared__ double rs_[];
     const int done = st->done;   // tested once the row loads are in flight
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int row = bx * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     const bool live = row < m;
     const int rw = live ? row : 0;
     const double* a = inv + (int64_t)rw * m;
@@ -824,7 +826,7 @@ k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict_
         rr = t2[1];
     }
     double alpha;
-    if (!cv_scalars(st, ph, rvv, rr, &alpha, flag, blockIdx.x == 0 && threadIdx.x == 0)) return;
+    if (!cv_scalars(st, ph, rvv, rr, &alpha, flag, bx == 0 && threadIdx.x == 0)) return;
     __syncthreads();   // rs_
     if (!live) return;
     const double s = row_dot(a, m, lane, av, rs_);
@@ -832,6 +834,12 @@ k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict_
         Y[2 * m + row] = s;                  // Y(v)
         Y[3 * m + row] = yr - alpha * s;     // Y(s)
     }
+}
+
+__global__ void __launch_bounds__(KM_BLOCK)
+k_tl_cv(int m, int S, int Gt, int Gf, int tot, int ph, const double* __restrict__ inv, const double* __restrict__ rcs,
+        double* __restrict__ Y, KState* __restrict__ st, const double* __restrict__ P, int* flag) {
+    cv_body(blockIdx.x, m, S, Gt, Gf, tot, ph, inv, rcs, Y, st, P, flag);
 }
 
 // omega, rho, beta of the next iteration from K_ST's partials (bicgsolver.py:
@@ -913,17 +921,19 @@ __device__ __forceinline__ void flat_update(const FGeom& fl, int wf, F scalars, 
 // rest flat chunks of the fine vectors: x += alpha p^ + omega s^ (bicgsolver.py:128),
 // r = s - omega t (:134), p = r + beta (p - omega v) (:103) of the next
 // iteration, partials of <r,r>.
+// bx: the workgroup's index within its system.
 template <typename T>
-__global__ void __launch_bounds__(KM_BLOCK)
-k_tl_ct(int m, int S, int Gt, int Gc, FGeom fl, int tot, int ph, const double* __restrict__ inv,
-        const double* __restrict__ rcs, double* __restrict__ Y, T* __restrict__ x, T* __restrict__ r,
-        T* __restrict__ p, const T* __restrict__ v, const T* __restrict__ s, const T* __restrict__ t,
-        const T* __restrict__ ph_, const T* __restrict__ sh_, KState* __restrict__ st, double* __restrict__ P) {
+__device__ __forceinline__ void ct_body(int bx, int m, int S, int Gt, int Gc, const FGeom& fl, int tot, int ph,
+                                        const double* __restrict__ inv, const double* __restrict__ rcs,
+                                        double* __restrict__ Y, T* __restrict__ x, T* __restrict__ r,
+                                        T* __restrict__ p, const T* __restrict__ v, const T* __restrict__ s,
+                                        const T* __restrict__ t, const T* __restrict__ ph_,
+                                        const T* __restrict__ sh_, KState* __restrict__ st, double* __restrict__ P) {
     extern __shared__ double rs_[];
     __shared__ double shr[8];
     const int done = st->done;   // tested once the first loads are in flight
-    const bool coarse = (int)blockIdx.x < Gc;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const bool coarse = bx < Gc;
+    const int row = bx * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     const bool live = coarse && row < m;
     const int rw = live ? row : 0;
     const double* a = inv + (int64_t)rw * m;
@@ -938,7 +948,7 @@ k_tl_ct(int m, int S, int Gt, int Gc, FGeom fl, int tot, int ph, const double* _
         first_row(a, m, lane, av);
         load_rc(rcs, m, S, rs_);
     }
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = bx == 0 && threadIdx.x == 0;
     if (coarse) {
         double omega, beta, alpha;
         ct_scalars(P, Gt, tot, st, ph, lead && !done, &omega, &beta, &alpha);
@@ -958,7 +968,16 @@ k_tl_ct(int m, int S, int Gt, int Gc, FGeom fl, int tot, int ph, const double* _
         ct_scalars(P, Gt, tot, st, ph, lead && !done, &om, &be, &al);
         return !done;
     };
-    flat_update<T>(fl, blockIdx.x - Gc, scalars, x, r, p, v, s, t, ph_, sh_, P, shr);
+    flat_update<T>(fl, bx - Gc, scalars, x, r, p, v, s, t, ph_, sh_, P, shr);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK)
+k_tl_ct(int m, int S, int Gt, int Gc, FGeom fl, int tot, int ph, const double* __restrict__ inv,
+        const double* __restrict__ rcs, double* __restrict__ Y, T* __restrict__ x, T* __restrict__ r,
+        T* __restrict__ p, const T* __restrict__ v, const T* __restrict__ s, const T* __restrict__ t,
+        const T* __restrict__ ph_, const T* __restrict__ sh_, KState* __restrict__ st, double* __restrict__ P) {
+    ct_body<T>(blockIdx.x, m, S, Gt, Gc, fl, tot, ph, inv, rcs, Y, x, r, p, v, s, t, ph_, sh_, st, P);
 }
 
 // The flat workgroups of K_CT for two independent systems in one launch
@@ -986,6 +1005,104 @@ __global__ void __launch_bounds__(KM_BLOCK) k_tl_flat2(int ph, FlatSys<T> a, Fla
         return !done;
     };
     flat_update<T>(S.fl, wf, scalars, S.x, S.r, S.p, S.v, S.s, S.t, S.ph, S.sh, S.P, shr);
+}
+
+// ------------------------------------------------------------ system groups
+// Up to kGroupMax independent systems (the p' systems of several cases) in
+// the same launches: workgroups [start[q], start[q + 1]) run system q's
+// single-system code on its own geometry, arguments, state and partials,
+// with bx = blockIdx.x - start[q] as its workgroup index (the tile kernels'
+// XCD-aware order within each system), so each system's arithmetic is that
+// of its own solve.
+constexpr int kGroupMax = kmcoarse::kGroupSystems;
+static_assert(kGroupMax <= KM_HW_SYSTEMS && (int64_t)kGroupMax * kPairPartials <= KM_MAX_PARTIALS,
+              "each system of a group needs its host words and partial region");
+
+__device__ __forceinline__ int group_of(const int (&start)[kGroupMax + 1], int n, int bx) {
+    int q = 0;
+#pragma unroll
+    for (int k = 1; k < kGroupMax; ++k)
+        if (k < n && bx >= start[k]) q = k;
+    return q;
+}
+
+template <typename T>
+struct TileGroup {
+    int n;
+    int start[kGroupMax + 1];
+    ATile tl[kGroupMax];
+    CGeom cg[kGroupMax];
+    TlArgs<T> a[kGroupMax];
+};
+
+template <typename T, int TI, int TJ, int MODE, int WPE, int SW, int NT, bool SYM>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+k_tl_tile_n(TileGroup<T> g) {
+    const int q = group_of(g.start, g.n, blockIdx.x);
+    tile_body<T, TI, TJ, MODE, SW, false, NT, SYM>(g.tl[q], g.cg[q], g.a[q],
+                                                   xcd_remap(blockIdx.x - g.start[q], g.tl[q].G));
+}
+
+// One grouped tile launch of tile shape `which` (g.start filled by the caller).
+template <typename T, int MODE, bool SYM>
+void launch_tile_n(int which, hipStream_t q, const TileGroup<T>& g) {
+#define KM_TL_CASE(IDX)                                                                                           \
+    case IDX:                                                                                                     \
+        hipLaunchKernelGGL((k_tl_tile_n<T, kTileI[IDX], kTileJ[IDX], MODE, kTileW[IDX], kmcoarse::kSweeps,        \
+                                        kTileT[IDX], SYM>),                                                       \
+                           dim3(g.start[g.n]), dim3(kTileT[IDX]), 0, q, g);                                       \
+        break;
+    switch (which) {
+        KM_TL_CASE(0)
+        KM_TL_CASE(1)
+        KM_TL_CASE(2)
+        KM_TL_CASE(3)
+        KM_TL_CASE(4)
+        KM_TL_CASE(5)
+        KM_TL_CASE(6)
+        KM_TL_CASE(7)
+        default:
+        KM_TL_CASE(8)
+    }
+#undef KM_TL_CASE
+}
+
+// K_CV / K_CT arguments of one system of a group
+template <typename T>
+struct CoarseSys {
+    int m, S, Gt, Gc, Gf, tot;
+    FGeom fl;
+    const double* inv;
+    const double* rcv;   // K_CV's slot partials (P^T v)
+    const double* rct;   // K_CT's (P^T t)
+    double* Y;
+    T *x, *r, *p;
+    const T *v, *s, *t, *ph, *sh;
+    KState* st;
+    double* P;
+    int* flag;
+};
+
+template <typename T>
+struct CoarseGroup {
+    int n;
+    int start[kGroupMax + 1];
+    CoarseSys<T> a[kGroupMax];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK) k_tl_cv_n(CoarseGroup<T> g, int ph) {
+    const int q = group_of(g.start, g.n, blockIdx.x);
+    const CoarseSys<T>& a = g.a[q];
+    cv_body(blockIdx.x - g.start[q], a.m, a.S, a.Gt, a.Gf, a.tot, ph, a.inv, a.rcv, a.Y, a.st, a.P, a.flag);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(KM_BLOCK) k_tl_ct_n(CoarseGroup<T> g, int ph) {
+    const int q = group_of(g.start, g.n, blockIdx.x);
+    const CoarseSys<T>& a = g.a[q];
+    ct_body<T>(blockIdx.x - g.start[q], a.m, a.S, a.Gt, a.Gc, a.fl, a.tot, ph, a.inv, a.rct, a.Y, a.x, a.r, a.p,
+               a.v, a.s, a.t, a.ph, a.sh, a.st, a.P);
 }
 
 }  // namespace
@@ -1145,10 +1262,12 @@ struct Fused {
     bool sym;            // symmetric system: the three-plane tile kernels
     const double* cd;    // FP64 coefficients (the FP32-vector tiles compute with them)
 
+    // slot: the system's place in a group solve (its state st[4 + slot], its
+    // partials and host words beyond the earlier systems'); C: ctx->coarse
     Fused(km_ctx* ctx, int NX_, int NY_, const T* coef, T* ws, T* xd, int tile_, bool sym_ = false,
-          const double* cd_ = nullptr)
-        : NX(NX_), NY(NY_), tile(tile_), x(xd), coefT(coef), C(ctx->coarse), st(&ctx->st[4]), P(ctx->partials),
-          flag(ctx->d_flag), sym(sym_), cd(cd_) {
+          const double* cd_ = nullptr, int slot = 0)
+        : NX(NX_), NY(NY_), tile(tile_), x(xd), coefT(coef), C(ctx->coarse), st(&ctx->st[4 + slot]),
+          P(ctx->partials + (size_t)slot * kPairPartials), flag(ctx->d_flag + KM_HW * slot), sym(sym_), cd(cd_) {
         const int64_t n = (int64_t)NX * NY;
         r = ws;
         rt = ws + n;
@@ -1169,9 +1288,10 @@ struct Fused {
         rct = C->frc + (size_t)kMaxVS * C->m;
     }
 
-    void pv(hipStream_t q, int phs) const {
+    TlArgs<T> pv_args(int phs) const {
         TlArgs<T> a{};
         a.c = coefT;
+        a.cd = cd;
         a.mask = C->mask;
         a.f0 = p;
         a.rt = rt;
@@ -1183,8 +1303,9 @@ struct Fused {
         a.P = P + TL_PV;
         a.ph = phs;
         a.om = (T)kOmega;
-        tiles<1>(q, a);
+        return a;
     }
+    void pv(hipStream_t q, int phs) const { tiles<1>(q, pv_args(phs)); }
     // the tile launch of this solve's kernel variant
     template <int MODE>
     void tiles(hipStream_t q, TlArgs<T> a) const {
@@ -1193,9 +1314,10 @@ struct Fused {
         if (sym) launch_tile<T, MODE, true, c64>(tile, q, NX, NY, cg, a);
         else launch_tile<T, MODE, false, c64>(tile, q, NX, NY, cg, a);
     }
-    void st_(hipStream_t q, int phs) const {
+    TlArgs<T> st_args(int phs) const {
         TlArgs<T> a{};
         a.c = coefT;
+        a.cd = cd;
         a.mask = C->mask;
         a.f0 = r;
         a.f1 = v;
@@ -1209,7 +1331,34 @@ struct Fused {
         a.P = P + TL_PT;
         a.ph = phs;
         a.om = (T)kOmega;
-        tiles<2>(q, a);
+        return a;
+    }
+    void st_(hipStream_t q, int phs) const { tiles<2>(q, st_args(phs)); }
+    CoarseSys<T> coarse_sys() const {
+        CoarseSys<T> a;
+        a.m = C->m;
+        a.S = S;
+        a.Gt = Gt;
+        a.Gc = Gc;
+        a.Gf = fl.G;
+        a.tot = fin;
+        a.fl = fl;
+        a.inv = C->inv;
+        a.rcv = rcv;
+        a.rct = rct;
+        a.Y = Y;
+        a.x = x;
+        a.r = r;
+        a.p = p;
+        a.v = v;
+        a.s = s;
+        a.t = t;
+        a.ph = ph;
+        a.sh = sh;
+        a.st = st;
+        a.P = P;
+        a.flag = flag;
+        return a;
     }
     void total_a(hipStream_t q) const {
         if (fin & FIN_A)
@@ -1256,6 +1405,62 @@ int fused_start(km_ctx* ctx, const Fused<T>& f, const T* rhs, double eps, int ma
     return KM_OK;
 }
 
+// One iteration of a group of fused solves (symmetric FP64 systems, one tile
+// shape): K_PV, K_CV, K_ST and K_CT each one launch over every system, and
+// each system's totals where its sizes ask for them.
+struct FusedGroup {
+    int n = 0, tile = 0;
+    const Fused<double>* f[kGroupMax] = {};
+
+    void iteration(hipStream_t q, int phs) const {
+        TileGroup<double> g{};
+        g.n = n;
+        CoarseGroup<double> c{};
+        c.n = n;
+        int mmax = 1;
+        for (int k = 0; k < n; ++k) {
+            g.tl[k] = a_tile(f[k]->NX, f[k]->NY, kTileI[tile], kTileJ[tile]);
+            g.cg[k] = f[k]->cg;
+            g.a[k] = f[k]->pv_args(phs);
+            g.start[k + 1] = g.start[k] + g.tl[k].G;
+            c.a[k] = f[k]->coarse_sys();
+            c.start[k + 1] = c.start[k] + f[k]->Gc;
+            mmax = std::max(mmax, f[k]->C->m);
+        }
+        const size_t lds = (size_t)mmax * sizeof(double);
+        launch_tile_n<double, 1, true>(tile, q, g);
+        for (int k = 0; k < n; ++k) f[k]->total_a(q);
+        hipLaunchKernelGGL(k_tl_cv_n<double>, dim3(c.start[n]), dim3(KM_BLOCK), lds, q, c, phs);
+        for (int k = 0; k < n; ++k) g.a[k] = f[k]->st_args(phs);
+        launch_tile_n<double, 2, true>(tile, q, g);
+        for (int k = 0; k < n; ++k) f[k]->total_t(q);
+        for (int k = 0; k < n; ++k) c.start[k + 1] = c.start[k] + f[k]->Gc + f[k]->fl.G;
+        hipLaunchKernelGGL(k_tl_ct_n<double>, dim3(c.start[n]), dim3(KM_BLOCK), lds, q, c, phs);
+    }
+};
+
+// The first chunk length's hint of a level: where its short chunks begin
+// (the shortest of its last solves, less the lead), 0 without history.
+int expected_iterations(const km_coarse* C) {
+    int expect = 0;
+    for (int c : C->recent)
+        if (c > 0 && (expect == 0 || c < expect)) expect = c;
+    if (expect > 0) expect = std::max(1, expect - C->chunk_lead + 2);
+    return expect;
+}
+
+// The level's record of a finished solve: its iteration count, and a solve
+// over twice as long as the shortest recent one asks for a refactorisation
+// (the lagged coarse inverse no longer serves).
+void record_solve(km_coarse* C, int iters) {
+    for (int k = 7; k > 0; --k) C->recent[k] = C->recent[k - 1];
+    C->recent[0] = iters;
+    int shortest = 0;
+    for (int k = 1; k < 8; ++k)
+        if (C->recent[k] > 0 && (shortest == 0 || C->recent[k] < shortest)) shortest = C->recent[k];
+    if (shortest > 0 && iters > 2 * shortest + 4) C->refresh = true;
+}
+
 }  // namespace
 
 template <typename T>
@@ -1276,22 +1481,60 @@ int fused_inner(km_ctx* ctx, int NX, int NY, const T* coefT, const T* rhs, T* xd
              (uintptr_t)tile, (uintptr_t)f.fin, (uintptr_t)sym, (uintptr_t)coef64};
     // the shortest of the last solves of this system shape (the outer steps
     // of a time step differ) sets where the short chunks begin
-    int expect = 0;
-    for (int c : C->recent)
-        if (c > 0 && (expect == 0 || c < expect)) expect = c;
-    if (expect > 0) expect = std::max(1, expect - C->chunk_lead + 2);
+    const int expect = expected_iterations(C);
     const int rc = run_chunks_flag(ctx, key, use_graph, C->chunk_big, C->chunk_small, expect, max_iter,
                                    enqueue, out, tail);
-    if (rc == KM_OK) {
-        for (int k = 7; k > 0; --k) C->recent[k] = C->recent[k - 1];
-        C->recent[0] = out->iter;
-        // a solve over twice as long as the shortest recent one: the lagged
-        // coarse inverse no longer serves, factor on the next set-up
-        int shortest = 0;
-        for (int k = 1; k < 8; ++k)
-            if (C->recent[k] > 0 && (shortest == 0 || C->recent[k] < shortest)) shortest = C->recent[k];
-        if (shortest > 0 && out->iter > 2 * shortest + 4) C->refresh = true;
+    if (rc == KM_OK) record_solve(C, out->iter);
+    return rc;
+}
+
+int group_tile(km_ctx* ctx, km_coarse* C, const double* coef, const double* rhs, double* ws, int* tile) {
+    ctx->coarse = C;
+    return tuned_tile<double>(ctx, coef, rhs, ws, tile, true, coef);
+}
+
+int fused_inner_n(km_ctx* ctx, int n, km_coarse* const* C, const int* NX, const int* NY, const double* const* coef,
+                  const double* const* rhs, double* const* xd, double* const* ws, double eps, const int* max_iter,
+                  bool use_graph, KState* out, const std::function<int()>* tail, int tile) {
+    KM_REQUIRE(ctx, n >= 1 && n <= kGroupMax, "%d systems in one group", n);
+    KM_REQUIRE(ctx, tile >= 0 && tile < kNumTiles, "tile %d", tile);
+    std::vector<Fused<double>> f;
+    f.reserve(n);
+    int mi = 0, expect = 0;
+    bool history = true;
+    for (int k = 0; k < n; ++k) {
+        KM_REQUIRE(ctx, C[k] && C[k]->arena && C[k]->NX == NX[k] && C[k]->NY == NY[k],
+                   "coarse level of system %d not set up", k);
+        KM_REQUIRE(ctx, (int64_t)NX[k] * NY[k] < kTileCellsMax, "system %d too large for the tile kernels", k);
+        KM_REQUIRE(ctx, tile_slots(C[k], tile) <= kMaxVS, "tile %d: %d restriction slots on system %d", tile,
+                   tile_slots(C[k], tile), k);
+        for (int j = 0; j < k; ++j) KM_REQUIRE(ctx, C[j] != C[k] && xd[j] != xd[k], "systems %d and %d overlap", j, k);
+        ctx->coarse = C[k];
+        f.emplace_back(ctx, NX[k], NY[k], coef[k], ws[k], xd[k], tile, true, coef[k], k);
+        if (int rc = fused_start<double>(ctx, f[k], rhs[k], eps, max_iter[k])) return rc;
+        mi = std::max(mi, max_iter[k]);
+        const int e = expected_iterations(C[k]);
+        history = history && e > 0;
+        expect = std::max(expect, e);
     }
+    FusedGroup g;
+    g.n = n;
+    g.tile = tile;
+    GraphKey key;
+    key.v = {13, (uintptr_t)n, (uintptr_t)tile};
+    for (int k = 0; k < n; ++k) {
+        g.f[k] = &f[k];
+        for (uintptr_t v : {(uintptr_t)NX[k], (uintptr_t)NY[k], (uintptr_t)coef[k], (uintptr_t)rhs[k], (uintptr_t)xd[k],
+                            (uintptr_t)ws[k], (uintptr_t)C[k], (uintptr_t)C[k]->arena, (uintptr_t)C[k]->m,
+                            (uintptr_t)C[k]->B, (uintptr_t)f[k].fin})
+            key.v.push_back(v);
+    }
+    auto enqueue = [&g](hipStream_t q, int phs) { g.iteration(q, phs); };
+    // the slowest system sets the pace: short chunks from the latest expected end
+    const int rc = run_chunks_flag(ctx, key, use_graph, C[0]->chunk_big, C[0]->chunk_small, history ? expect : 0, mi,
+                                   enqueue, out, tail, n);
+    if (rc == KM_OK)
+        for (int k = 0; k < n; ++k) record_solve(C[k], out[k].iter);
     return rc;
 }
 

@@ -244,15 +244,50 @@ class SIMPLESolver:
         pf = mixed | (_lib.KM_FLAG_COARSE if self.p_coarse else 0) | (_lib.KM_FLAG_X0 if self.warm_start_p else 0)
         return mom, pf
 
-    def _native_step(self, mom_flags, p_flags):
-        """The whole step in one call (km_simple_step): the same kernels in the
-        same order as the stage-by-stage path below."""
+    def _fill_case(self):
+        """This solver's fields as a km_case (the layout contract of the C-ABI)."""
         c = self._case
         c.g = self._grid()
         for f in ("u", "v", "p", "u0", "v0", "u_mid", "v_mid", "pcor", "mdiv", "b_u", "b_v", "b_p"):
             setattr(c, f, getattr(self, f).data_ptr())
         c.coef_u, c.coef_v, c.coef_p = self._coef_u.data_ptr(), self._coef_v.data_ptr(), self._coef_p.data_ptr()
         c.mask = self.ct.data_ptr()
+        return c
+
+    @staticmethod
+    def step_group(solvers, ctx=None):
+        """One outer iteration of several solvers in one call
+        (km_simple_step_n, on ``ctx`` or the first solver's context and the
+        current stream): every case's stages, then the p' solves -- the
+        two-level ones of equal flags with their first sweeps in shared
+        launches -- then every case's corrections.  Each solver's result is
+        that of its own ``step()``; returns each one's ``last``."""
+        if not solvers:
+            return []
+        flags = [s._step_flags() for s in solvers]
+        if any(f is None for f in flags):
+            raise ValueError("step_group needs solvers whose step runs natively (km_simple_step)")
+        n = len(solvers)
+        ctx = ctx if ctx is not None else solvers[0].ctx
+        arr = (KmCase * n)()
+        for k, s in enumerate(solvers):
+            arr[k] = s._fill_case()
+        mom = (ctypes.c_int * n)(*[f[0] for f in flags])
+        pf = (ctypes.c_int * n)(*[f[1] for f in flags])
+        info = (KmSolveInfo * (3 * n))()
+        eps_mom, eps_p = float(solvers[0].eps_mom), float(solvers[0].eps_p)
+        if any(float(s.eps_mom) != eps_mom or float(s.eps_p) != eps_p for s in solvers):
+            raise ValueError("step_group: the solvers differ in eps_mom / eps_p")
+        ctx.set_stream(torch.cuda.current_stream(solvers[0].device).cuda_stream)
+        ctx.call("km_simple_step_n", n, arr, eps_mom, eps_p, mom, pf, info)
+        for k, s in enumerate(solvers):
+            s.last = {key: SolveInfo.from_c(info[3 * k + i]) for i, key in enumerate("uvp")}
+        return [s.last for s in solvers]
+
+    def _native_step(self, mom_flags, p_flags):
+        """The whole step in one call (km_simple_step): the same kernels in the
+        same order as the stage-by-stage path below."""
+        c = self._fill_case()
         self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
         info = self._step_info
         self.ctx.call("km_simple_step", ctypes.byref(c), float(self.eps_mom), float(self.eps_p), mom_flags, p_flags,

@@ -277,6 +277,28 @@ def test_bicgstab_deterministic_graph_and_direct(dev, c1_pcor):
     np.testing.assert_array_equal(x1, x3)
 
 
+def test_bicgstab_pre_summed_totals_same_bits(dev, c1_pcor):
+    """km_set_plain_totals: P, S and R reading the previous launch's totals
+    from a one-workgroup kernel give the same bits as every workgroup
+    summing the partials itself (FP64, and the FP32 inner solves of the
+    mixed path), with and without captured graphs."""
+    coef, b = c1_pcor
+    ctx = _lib.context_for(dev)
+    out = {}
+    try:
+        for mode in (0, 1):
+            ctx.call("km_set_plain_totals", mode)
+            out[mode] = [bicg(dev, coef, b, eps=1e-10), bicg(dev, coef, b, eps=1e-10, flags=_lib.KM_FLAG_NO_GRAPH),
+                         bicg(dev, coef, b, eps=1e-8, flags=_lib.KM_FLAG_MIXED)]
+    finally:
+        ctx.call("km_set_plain_totals", -1)
+    for (xa, ia), (xb, ib) in zip(out[0], out[1]):
+        print(f"totals off {ia.iterations} it, on {ib.iterations} it")
+        assert ia.status == ib.status == _lib.KM_OK
+        assert ia.iterations == ib.iterations
+        np.testing.assert_array_equal(xa, xb)
+
+
 def test_bicgstab_warm_start(dev, c1_pcor):
     coef, b = c1_pcor
     xd = krylov.direct_solve(coef, b)
