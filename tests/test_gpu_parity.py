@@ -299,6 +299,35 @@ def test_bicgstab_pre_summed_totals_same_bits(dev, c1_pcor):
         np.testing.assert_array_equal(xa, xb)
 
 
+def test_bicgstab_totals_by_size_same_bits(dev):
+    """At 2050 x 700 cells (P's workgroups x R's partials above the automatic
+    bound) the automatic choice sums R's partials once for P: the same bits
+    as every workgroup summing them, over 40 iterations of a random
+    diagonally dominant 5-point system."""
+    rng = np.random.default_rng(5)
+    NX, NY = 2050, 700
+    coef = np.zeros((5, NX, NY))
+    coef[1:, 1:-1, 1:-1] = -rng.random((4, NX - 2, NY - 2))
+    coef[1, 1, :] = 0
+    coef[2, -2, :] = 0
+    coef[4, :, 1] = 0
+    coef[3, :, -2] = 0
+    coef[0, 1:-1, 1:-1] = -coef[1:, 1:-1, 1:-1].sum(axis=0) + 0.05
+    b = np.zeros((NX, NY))
+    b[1:-1, 1:-1] = rng.standard_normal((NX - 2, NY - 2))
+    ctx = _lib.context_for(dev)
+    out = {}
+    try:
+        for mode in (0, -1):
+            ctx.call("km_set_plain_totals", mode)
+            out[mode] = bicg(dev, coef, b, eps=1e-30, max_iter=40)
+    finally:
+        ctx.call("km_set_plain_totals", -1)
+    (xa, ia), (xb, ib) = out[0], out[-1]
+    assert ia.status == ib.status == _lib.KM_MAXITER and ia.iterations == ib.iterations == 40
+    np.testing.assert_array_equal(xa, xb)
+
+
 def test_bicgstab_warm_start(dev, c1_pcor):
     coef, b = c1_pcor
     xd = krylov.direct_solve(coef, b)
