@@ -438,6 +438,17 @@ def learned_compare(s, weights, reps):
     return out
 
 
+def coarse_unknowns(nx, ny):
+    """Coarse unknowns of the two-level level of an nx x ny interior: the
+    smallest aggregate of 8, 12, 16, 24, ... 256 that keeps them <= 2048
+    (kmcoarse::choose_block)."""
+    for B in (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256):
+        m = -(-nx // B) * -(-ny // B)
+        if m <= 2048:
+            return m
+    return -(-nx // 512) * -(-ny // 512)
+
+
 def bench_batch(args):
     """The 96 uneven channel cases (nx 256..2048, Re 80..300) advanced together
     with km_batch_step: one launch per SIMPLE stage over all cases; momentum
@@ -475,6 +486,19 @@ def bench_batch(args):
     p_it = [inf[2].iterations for rec in recs for inf in rec]
     n_fp32 = sum(bool(c.mixed) for c in case_list) if workers else 0
     cell_its = sum(c.shape_p[0] * c.shape_p[1] * inf[2].iterations for rec in recs for c, inf in zip(case_list, rec))
+    # algorithmic bytes of the p' iterations alone (the two-level path's
+    # K_PV, K_CV, K_ST, K_CT: per cell 57 + 73 + 80 B with FP64 vectors, 41 +
+    # 49 + 40 B with FP32 vectors and FP64 coefficient planes, plus two reads
+    # of the m x m FP64 coarse inverse and the coarse vectors); the momentum
+    # solves, stages and refinement residuals are not counted
+    p_bytes = 0.0
+    if coarse:
+        for rec in recs:
+            for c, inf in zip(case_list, rec):
+                m = coarse_unknowns(c.nx, c.ny)
+                per_cell = (41 + 49 + 40) if (workers and c.mixed) else (57 + 73 + 80)
+                cells = c.shape_p[0] * c.shape_p[1]
+                p_bytes += inf[2].iterations * (per_cell * cells + 16.0 * m * m + 80.0 * m)
     out = {"metric": "batched SIMPLE outer steps/s (every case advanced one outer iteration)",
            "value": round(args.steps / wall, 4), "unit": "batch steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True,
@@ -490,6 +514,8 @@ def bench_batch(args):
                       "fp32_vector_cases": n_fp32},
            "p_iters_min_median_max": [min(p_it), sorted(p_it)[len(p_it) // 2], max(p_it)],
            "p_cells_iters_per_s": round(cell_its / wall, 1),
+           "p_iter_bytes_per_s": round(p_bytes / wall, 1) if coarse else None,
+           "p_iter_hbm_fraction": round(p_bytes / wall / (HBM_PEAK_GBS * 1e9), 4) if coarse else None,
            "case_steps_per_s": round(args.steps * b.n / wall, 3)}
     print(json.dumps(out), flush=True)
 
