@@ -565,20 +565,32 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         int cap = max_iter - total;
         if (cap > inner_cap) cap = inner_cap;
         KState st;
+        // the fused inner solves take x += rn d and the FP64 true residual as
+        // their tail, enqueued behind the last iterations and read with the
+        // state: one host round trip per sweep instead of two
+        const bool tailed = smooth || (coarse && !params);
+        const std::function<int()> tail = [&, rn]() {
+            hipLaunchKernelGGL(k_axpy_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const T*)d, rn);
+            return km_residual_async(ctx, NX, NY, coef, b, x, r64, 0);
+        };
         const int rc =
-            smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st)
+            smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st, &tail)
             : coarse && !params
-                ? kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st, nullptr,
+                ? kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, rhs, d, wT, inner_eps, cap, use_graph, &st, &tail,
                                            sym, coef)
                 : fb_inner<T>(ctx, NX, NY, coefT, c32, coef, rhs, d, wT, inner_eps, cap, use_graph, params, coarse,
                               &st, mg);
         if (rc) return rc;
         total += st.iter;
         sweeps++;
-        hipLaunchKernelGGL(k_axpy_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const T*)d, rn);
-        KM_CHECK_LAUNCH(ctx);
         const double prev = rel;
-        if (int rc = residual64(ctx, NX, NY, coef, b, x, r64, &rr)) return rc;
+        if (tailed) {
+            rr = km_mapped_res(ctx, 0);
+        } else {
+            hipLaunchKernelGGL(k_axpy_to<T>, dim3(blocks(n)), dim3(KM_BLOCK), 0, ctx->stream, n, x, (const T*)d, rn);
+            KM_CHECK_LAUNCH(ctx);
+            if (int rc = residual64(ctx, NX, NY, coef, b, x, r64, &rr)) return rc;
+        }
         rel = sqrt(rr) / bnorm;
         if (!(rel < 0.5 * prev)) {
             // the preconditioned sweep stalled or diverged: take it back
