@@ -52,7 +52,7 @@ extern "C" {
 #define KM_FLAG_MIXED 2       /* FP32 matvec/axpy inner solves, FP64 scalars and FP64 true-residual refinement */
 #define KM_FLAG_NO_GRAPH 4    /* launch kernels directly instead of replaying captured graphs */
 #define KM_FLAG_COARSE 8      /* km_fbicgstab: two-level preconditioner (aggregation coarse correction first) */
-#define KM_FLAG_SMOOTH 16     /* km_fbicgstab: one red-black Gauss-Seidel sweep as the preconditioner, no coarse level */
+#define KM_FLAG_SMOOTH 16     /* km_fbicgstab: red-black Gauss-Seidel sweeps as the preconditioner (km_set_momentum_sweeps), no coarse level */
 #define KM_FLAG_GCR 64        /* km_fbicgstab: flexible GCR (restarted every 16 directions) as the outer method instead
                                * of flexible BiCGSTAB: ||r|| never increases whatever the preconditioner does, so a
                                * non-linear network preconditioner cannot break its recurrences (no counterpart in the
@@ -225,8 +225,9 @@ int km_adam_step(km_ctx* ctx, int64_t n, float* params, const float* grad, float
  * B x B aggregates of free cells (Galerkin A_c, its inverse held across
  * solves, km_set_coarse_update), then the network on r - A z1, added to z1 --
  * or, without a network, two red-black SOR sweeps (omega 1.3) on A z = r from
- * z1, run as a fused four-launch iteration.  KM_FLAG_SMOOTH: one red-black
- * Gauss-Seidel sweep on A z = r from zero as the preconditioner (no network,
+ * z1, run as a fused four-launch iteration.  KM_FLAG_SMOOTH: one or two
+ * red-black Gauss-Seidel sweeps (km_set_momentum_sweeps) on A z = r from
+ * zero as the preconditioner (no network,
  * no coarse level; the momentum systems), a fused three-launch iteration.
  * FP64 solves from x = 0 with either fused iteration solve straight into x. */
 int km_fbicgstab(km_ctx* ctx, int NX, int NY, const double* coef, const double* b, double* x, double eps,
@@ -355,6 +356,12 @@ int km_set_coarse_totals(km_ctx* ctx, int mode);
  * partials exceed 4e6, e.g. at 4096 x 820), 0 never, 1 all three; the same
  * bits either way.  No counterpart in the reference. */
 int km_set_plain_totals(km_ctx* ctx, int mode);
+/* Red-black Gauss-Seidel sweeps of KM_FLAG_SMOOTH's preconditioner: 0
+ * automatic (two for FP64 systems of at least 200,000 cells, else one;
+ * default), 1 or 2.  Two sweeps save an iteration on the 1280 x 256
+ * channel's momentum systems; where several contexts' streams share the GPU
+ * one sweep is faster.  No counterpart in the reference. */
+int km_set_momentum_sweeps(km_ctx* ctx, int sweeps);
 /* Launch schedule of the fused two-level iteration: captured graphs of `big`
  * iterations (a multiple of 4; 0: 8) until `lead` iterations before the
  * shortest of the last eight solves of this system shape, then graphs of

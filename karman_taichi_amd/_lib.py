@@ -110,6 +110,7 @@ SIGNATURES = {
     "km_twolevel_set_tile": [_CTX, _I],
     "km_set_coarse_totals": [_CTX, _I],
     "km_set_plain_totals": [_CTX, _I],
+    "km_set_momentum_sweeps": [_CTX, _I],
     "km_set_coarse_chunks": [_CTX, _I, _I, _I],
 }
 _RESTYPE = {"km_last_error": ctypes.c_char_p}

@@ -86,7 +86,11 @@ class BatchedSIMPLE:
             self._workers = []
             for _ in range(workers):
                 st = torch.cuda.Stream(device=self.device)
-                self._workers.append((st, _lib.Context(idx, st.cuda_stream)))
+                ctx = _lib.Context(idx, st.cuda_stream)
+                # the workers' streams share the GPU: one momentum sweep (a
+                # second saves an iteration but its work is not hidden here)
+                ctx.call("km_set_momentum_sweeps", 1)
+                self._workers.append((st, ctx))
             self._pool = concurrent.futures.ThreadPoolExecutor(max_workers=workers)
         for st, _ in self._workers:
             st.wait_stream(main)

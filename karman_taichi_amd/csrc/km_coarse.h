@@ -115,7 +115,7 @@ int fused_inner_n(km_ctx* ctx, int n, km_coarse* const* C, const int* NX, const 
 // use; ws: 5n scratch values); leaves ctx->coarse = C.
 int group_tile(km_ctx* ctx, km_coarse* C, const double* coef, const double* rhs, double* ws, int* tile);
 
-// Inner flexible BiCGSTAB for A d = rhs with one red-black Gauss-Seidel sweep
+// Inner flexible BiCGSTAB for A d = rhs with one or two red-black Gauss-Seidel sweeps
 // as the preconditioner and no coarse level (the momentum systems), three
 // launches per iteration; ws holds 8n values of T.
 template <typename T>

@@ -147,6 +147,7 @@ struct km_ctx {
     std::map<GraphKey, hipGraphExec_t> graphs;
     int autotune = 1;              // time tile candidates on the first solve of a shape
     int plain_totals = -1;         // km_set_plain_totals: -1 by size, 0 never, 1 always
+    int mom_sweeps = 0;            // km_set_momentum_sweeps: 0 by size and precision, else 1 or 2
     std::set<std::tuple<int, int, int>> tuned;
     km_net* net = nullptr;
     km_batch* batch = nullptr;

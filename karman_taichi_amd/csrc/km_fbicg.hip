@@ -9,7 +9,7 @@
 // residual is taken back and plain FP64 BiCGSTAB finishes the solve.
 // Without a network (params == null) the preconditioner is Jacobi, D^-1, or
 // with the coarse level the two-level preconditioner with red-black SOR
-// sweeps, or with KM_FLAG_SMOOTH one red-black Gauss-Seidel sweep (both the
+// sweeps, or with KM_FLAG_SMOOTH one or two red-black Gauss-Seidel sweeps (both the
 // fused iterations of km_twolevel.hip).
 //
 // Inner iteration i (ring slot ph = i mod 4, as in the plain solver):

@@ -171,7 +171,7 @@ class BICGSolver(CGSolver):
         residual only, with plain BiCGSTAB as the fallback.  ``coarse``: the
         two-level preconditioner (aggregation coarse correction, then the
         network -- or red-black SOR sweeps without one).  ``smooth``: flexible
-        BiCGSTAB with one red-black Gauss-Seidel sweep as the preconditioner
+        BiCGSTAB with one or two red-black Gauss-Seidel sweeps as the preconditioner
         (the momentum systems), judged by the true residual like the others.
         ``gcr``: flexible GCR instead of flexible BiCGSTAB as the outer
         method of the preconditioned solve (the network's: a non-linear

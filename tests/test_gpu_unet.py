@@ -492,23 +492,33 @@ def c1_momentum():
     return {"u": (cu, bu), "v": (cv, bv)}
 
 
+@pytest.mark.parametrize("sweeps", [1, 2])
 @pytest.mark.parametrize("which", ["u", "v"])
-def test_smooth_momentum_matches_oracle(which, dev):
-    """KM_FLAG_SMOOTH: flexible BiCGSTAB with one red-black Gauss-Seidel sweep
-    (from zero) as the preconditioner, on the C1 momentum systems: below 1e-8
-    in the FP64 true residual (recomputed on the host), the direct solution to
-    1e-7, and the iteration count within 25% (+2) of the oracle's flexible
-    solve with the same preconditioner -- about half of plain BiCGSTAB's."""
+def test_smooth_momentum_matches_oracle(which, sweeps, dev):
+    """KM_FLAG_SMOOTH: flexible BiCGSTAB with one or two red-black
+    Gauss-Seidel sweeps (from zero; km_set_momentum_sweeps) as the
+    preconditioner, on the C1 momentum systems: below 1e-8 in the FP64 true
+    residual (recomputed on the host), the direct solution to 1e-7, and the
+    iteration count within 25% (+2) of the oracle's flexible solve with the
+    same preconditioner -- about half of plain BiCGSTAB's."""
     coef, b = c1_momentum()[which]
-    x, info = coarse_flex(dev, coef, b, None, _lib.KM_FLAG_SMOOTH)
+    ctx = _lib.context_for(dev)
+    ctx.call("km_set_momentum_sweeps", sweeps)
+    try:
+        x, info = coarse_flex(dev, coef, b, None, _lib.KM_FLAG_SMOOTH)
+    finally:
+        ctx.call("km_set_momentum_sweeps", 0)
     true = np.linalg.norm(b - krylov.stencil(coef, x)) / np.linalg.norm(b)
 
     def M(r):
-        return krylov.rbsor_sweep(coef, r, np.zeros_like(r), 1.0)
+        z = np.zeros_like(r)
+        for _ in range(sweeps):
+            z = krylov.rbsor_sweep(coef, r, z, 1.0)
+        return z
     _, io = krylov.fbicgstab(coef, b, eps=1e-8, precond=M, fallback=False)
     _, plain = krylov.bicgstab(coef, b, eps=1e-8)
-    print(f"smooth momentum {which}: gpu {info.iterations} it, oracle {io['iterations']} it, plain {plain['iterations']} "
-          f"it, true {true:.2e}")
+    print(f"smooth momentum {which}, {sweeps} sweeps: gpu {info.iterations} it, oracle {io['iterations']} it, "
+          f"plain {plain['iterations']} it, true {true:.2e}")
     assert info.status == _lib.KM_OK and true < 1e-8 and info.fallback == 0
     assert rel(x, krylov.direct_solve(coef, b)) < 1e-7
     assert abs(info.iterations - io["iterations"]) <= 0.25 * io["iterations"] + 2
@@ -524,22 +534,28 @@ def pair_solve(dev, systems, flags, eps=1e-8):
     return [t[2].cpu().numpy() for t in ts], infos
 
 
-@pytest.mark.parametrize("flags", [_lib.KM_FLAG_SMOOTH, _lib.KM_FLAG_SMOOTH | _lib.KM_FLAG_NO_GRAPH, 0])
-def test_pair_momentum_matches_separate(flags, dev):
+@pytest.mark.parametrize("flags,sweeps", [(_lib.KM_FLAG_SMOOTH, 0), (_lib.KM_FLAG_SMOOTH | _lib.KM_FLAG_NO_GRAPH, 0),
+                                          (0, 0), (_lib.KM_FLAG_SMOOTH, 2)])
+def test_pair_momentum_matches_separate(flags, sweeps, dev):
     """km_fbicgstab2 (the u and v solves in the same launches) against two
     km_fbicgstab calls on the C1 momentum systems: bit-identical solutions
     and the same iteration counts and statuses; twice, the second time with
     the systems swapped (another tile split of each launch).  Flags 0: the
-    pair entry runs the two solves in turn."""
+    pair entry runs the two solves in turn.  sweeps: km_set_momentum_sweeps."""
     mom = c1_momentum()
-    for order in (("u", "v"), ("v", "u")):
-        systems = [mom[k] for k in order]
-        xs, infos = pair_solve(dev, systems, flags)
-        for (coef, b), x, info in zip(systems, xs, infos):
-            xr, ir = coarse_flex(dev, coef, b, None, flags)
-            assert np.array_equal(x, xr)
-            assert (info.iterations, info.status, info.fallback) == (ir.iterations, ir.status, ir.fallback)
-            assert info.status == _lib.KM_OK and info.rel_resid_true < 1e-8
+    ctx = _lib.context_for(dev)
+    ctx.call("km_set_momentum_sweeps", sweeps)
+    try:
+        for order in (("u", "v"), ("v", "u")):
+            systems = [mom[k] for k in order]
+            xs, infos = pair_solve(dev, systems, flags)
+            for (coef, b), x, info in zip(systems, xs, infos):
+                xr, ir = coarse_flex(dev, coef, b, None, flags)
+                assert np.array_equal(x, xr)
+                assert (info.iterations, info.status, info.fallback) == (ir.iterations, ir.status, ir.fallback)
+                assert info.status == _lib.KM_OK and info.rel_resid_true < 1e-8
+    finally:
+        ctx.call("km_set_momentum_sweeps", 0)
 
 
 def test_pair_momentum_one_zero_rhs(dev):
