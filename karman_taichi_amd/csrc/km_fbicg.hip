@@ -370,6 +370,11 @@ int residual64(km_ctx* ctx, int NX, int NY, const double* coef, const double* b,
     return rc < 0 ? rc : KM_OK;
 }
 
+// The first FP64 sweep of a fused solve aims at kInnerMargin eps in its
+// recursive residual, so that the true residual, which decides, is below
+// eps without a second sweep.
+constexpr double kInnerMargin = 0.75;
+
 // Workspace bytes of fbicg_solve<T> on n cells.
 template <typename T>
 size_t fbicg_bytes(int64_t n) {
@@ -452,9 +457,9 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
             const T* bT = reinterpret_cast<const T*>(b);
             // the true residual is enqueued behind the solve and read with its state
             const std::function<int()> tail = [&]() { return km_residual_async(ctx, NX, NY, coef, b, x, r64); };
-            const int rc = smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter,
+            const int rc = smooth ? kmcoarse::smooth_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, kInnerMargin * eps, max_iter,
                                                               use_graph, &st, &tail)
-                                  : kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, 0.5 * eps, max_iter,
+                                  : kmcoarse::fused_inner<T>(ctx, NX, NY, coefT, bT, xT, wT, kInnerMargin * eps, max_iter,
                                                              use_graph, &st, &tail, sym, coef);
             if (rc) return rc;
             rr_first = km_mapped_res(ctx, 0);
@@ -559,7 +564,7 @@ int fbicg_solve(km_ctx* ctx, int NX, int NY, const double* coef, const double* b
         KM_CHECK_LAUNCH(ctx);
         // FP64: aim a little below eps so rounding between the recursive and
         // the true residual does not cost a second sweep
-        double inner_eps = f64 ? 0.5 * eps / rel : eps / rel;
+        double inner_eps = f64 ? kInnerMargin * eps / rel : eps / rel;
         if (inner_eps < inner_floor) inner_eps = inner_floor;
         if (inner_eps > 1e-1) inner_eps = 1e-1;
         int cap = max_iter - total;
@@ -1101,7 +1106,7 @@ int fbicg_group(km_ctx* ctx, int n, const int* NX, const int* NY, const double* 
                 if (int rc = km_residual_async(ctx, gNX[j], gNY[j], gcoef[j], gb[j], gx[j], gr[j], j)) return rc;
             return (int)KM_OK;
         };
-        if (int rc = kmcoarse::fused_inner_n(ctx, m, gC, gNX, gNY, gcoef, gb, gx, gw, 0.5 * eps, gmi, use_graph, st,
+        if (int rc = kmcoarse::fused_inner_n(ctx, m, gC, gNX, gNY, gcoef, gb, gx, gw, kInnerMargin * eps, gmi, use_graph, st,
                                              &tail, tile[a]))
             return rc;
         double rr[kmcoarse::kGroupSystems];
@@ -1210,7 +1215,7 @@ int km_fbicgstab2(km_ctx* ctx, int NX0, int NY0, const double* coef0, const doub
             if (int rc = km_residual_async(ctx, NX[q], NY[q], coef[q], b[q], x[q], r[q], q)) return rc;
         return (int)KM_OK;
     };
-    if (int rc = kmcoarse::smooth_inner2<double>(ctx, NX, NY, coef, b, x, w, 0.5 * eps, mi,
+    if (int rc = kmcoarse::smooth_inner2<double>(ctx, NX, NY, coef, b, x, w, kInnerMargin * eps, mi,
                                                  !(flags & KM_FLAG_NO_GRAPH), st, &tail))
         return rc;
     const double rr1 = km_mapped_res(ctx, 1);   // the first system's continuation may reuse the slots
