@@ -440,11 +440,11 @@ def learned_compare(s, weights, reps):
 
 def coarse_unknowns(nx, ny):
     """Coarse unknowns of the two-level level of an nx x ny interior: the
-    smallest aggregate of 8, 12, 16, 24, ... 256 that keeps them <= 2048
-    (kmcoarse::choose_block)."""
+    smallest aggregate of 8, 12, 16, 24, ... 256 that keeps them <= 2048, or
+    on larger grids within 4 m^2 <= 21 nx ny (kmcoarse::choose_block)."""
     for B in (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256):
         m = -(-nx // B) * -(-ny // B)
-        if m <= 2048:
+        if m <= 2048 or (m <= 8192 and 4 * m * m <= 21 * nx * ny):
             return m
     return -(-nx // 512) * -(-ny // 512)
 

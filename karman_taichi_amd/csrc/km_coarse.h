@@ -62,9 +62,12 @@ constexpr int kMaxVS = 64;
 
 // Aggregate edge length for an nx x ny interior (req > 0: that size; else the
 // smallest of 8, 12, 16, 24, ..., 256 that keeps the coarse level at <= 2048
-// unknowns: on the 1280 x 256 channel 16 x 16 aggregates (1280 unknowns) take
-// the p' solve to 44 iterations against 65 with 24 x 24, and the larger dense
-// coarse matvec and inverse update cost less than the iterations saved).
+// unknowns, or on larger grids at as many as keep the two reads of the dense
+// coarse inverse per iteration (16 m^2 bytes) within ~40% of the fine grid's
+// ~210 bytes per cell, 4 m^2 <= 21 nx ny, at most 8192).  On the 1280 x 256
+// channel 16 x 16 aggregates (1280 unknowns) beat 12, 14 and 20 (436 against
+// 329-372 SIMPLE steps/s); on the 4096 x 820 channel 32 x 32 (3328) beat the
+// 48 x 48 of the 2048 bound, 24 x 24 and 40 x 40 (45.4 against 38.0-39.0).
 int choose_block(int nx, int ny, int req);
 
 // Galerkin coarse operator P^T A P of the FP64 system, its band LU, its

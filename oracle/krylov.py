@@ -520,11 +520,13 @@ def mg_vcycle_precond(coef, pre=1, post=1, coarse_sweeps=50, omega=1.0, min_cell
 
 def coarse_block(nx, ny, req=0):
     """Aggregate edge length: req, else the smallest of 8, 12, 16, 24, ..., 256
-    giving at most 2048 coarse unknowns."""
+    giving at most 2048 coarse unknowns, or on larger grids at most as many as
+    4 m^2 <= 21 nx ny allows (at most 8192; the fused solver's choice)."""
     if req > 0:
         return req
     for B in (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256):
-        if (-(-nx // B)) * (-(-ny // B)) <= 2048:
+        m = (-(-nx // B)) * (-(-ny // B))
+        if m <= 2048 or (m <= 8192 and 4 * m * m <= 21 * nx * ny):
             return B
     return 512
 

@@ -222,7 +222,7 @@ def test_twolevel_coarse_correction_invariants():
     z1, r1 = krylov.coarse_correct(coef, r, B)
     assert np.abs(P.T @ r1.ravel()).max() < 1e-12 * np.abs(r).max()
     assert krylov.coarse_block(1280, 256) == 16 and krylov.coarse_block(320, 64) == 8
-    assert krylov.coarse_block(4096, 820) == 48
+    assert krylov.coarse_block(4096, 820) == 32
 
 
 def test_rbsor_sweep_is_textbook_sor():
