@@ -444,6 +444,8 @@ def coarse_unknowns(nx, ny):
     on larger grids within 4 m^2 <= 21 nx ny (kmcoarse::choose_block)."""
     for B in (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256):
         m = -(-nx // B) * -(-ny // B)
+        if -(-ny // B) > 63:
+            continue
         if m <= 2048 or (m <= 8192 and 4 * m * m <= 21 * nx * ny):
             return m
     return -(-nx // 512) * -(-ny // 512)

@@ -64,11 +64,16 @@ constexpr int kMaxVS = 64;
 // smallest of 8, 12, 16, 24, ..., 256 that keeps the coarse level at <= 2048
 // unknowns, or on larger grids at as many as keep the two reads of the dense
 // coarse inverse per iteration (16 m^2 bytes) within ~40% of the fine grid's
-// ~210 bytes per cell, 4 m^2 <= 21 nx ny, at most 8192).  On the 1280 x 256
+// ~210 bytes per cell, 4 m^2 <= 21 nx ny, at most 8192), skipping sizes with
+// more than kMaxBandwidth aggregate columns.  On the 1280 x 256
 // channel 16 x 16 aggregates (1280 unknowns) beat 12, 14 and 20 (436 against
 // 329-372 SIMPLE steps/s); on the 4096 x 820 channel 32 x 32 (3328) beat the
 // 48 x 48 of the 2048 bound, 24 x 24 and 40 x 40 (45.4 against 38.0-39.0).
 int choose_block(int nx, int ny, int req);
+// The widest coarse band (aggregate columns) the factorisation's LDS ring of
+// (bw + 1) x (2 bw + 1) doubles holds in 64 KB; choose_block skips sizes
+// with more.
+constexpr int kMaxBandwidth = 63;
 
 // Galerkin coarse operator P^T A P of the FP64 system, its band LU, its
 // explicit inverse and the free-cell mask (enqueued on s).  solver: set up for

@@ -573,7 +573,8 @@ namespace kmcoarse {
 int choose_block(int nx, int ny, int req) {
     if (req > 0) return req;
     for (int B : {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256}) {
-        const int64_t m = (int64_t)((nx + B - 1) / B) * ((ny + B - 1) / B);
+        const int64_t mJ = (ny + B - 1) / B, m = (int64_t)((nx + B - 1) / B) * mJ;
+        if (mJ > kMaxBandwidth) continue;   // the factorisation's LDS ring
         if (m <= 2048 || (m <= 8192 && 4 * m * m <= 21 * (int64_t)nx * ny)) return B;
     }
     return 512;

@@ -525,7 +525,10 @@ def coarse_block(nx, ny, req=0):
     if req > 0:
         return req
     for B in (8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256):
-        m = (-(-nx // B)) * (-(-ny // B))
+        mJ = -(-ny // B)
+        m = (-(-nx // B)) * mJ
+        if mJ > 63:   # the fused solver's band limit
+            continue
         if m <= 2048 or (m <= 8192 and 4 * m * m <= 21 * nx * ny):
             return B
     return 512
