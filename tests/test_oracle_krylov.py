@@ -223,6 +223,7 @@ def test_twolevel_coarse_correction_invariants():
     assert np.abs(P.T @ r1.ravel()).max() < 1e-12 * np.abs(r).max()
     assert krylov.coarse_block(1280, 256) == 16 and krylov.coarse_block(320, 64) == 8
     assert krylov.coarse_block(4096, 820) == 32
+    assert krylov.coarse_block(1000, 4000) == 64   # 63 aggregate columns: the widest band the solver factors
 
 
 def test_rbsor_sweep_is_textbook_sor():
